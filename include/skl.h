@@ -1,0 +1,156 @@
+/*
+ * skl.h -- C ABI of the MI355X sketch-and-precondition least-squares library.
+ *
+ * Plain pointers and sizes only; every matrix is column-major with an explicit
+ * leading dimension; indices are `long` (64-bit), like the reference's ILP64
+ * build (include/config_sk.h:5).  All entry points return an int status
+ * (SKL_OK == 0); skl_last_error() gives the message of the last failure on
+ * the calling thread.
+ *
+ * The three solver entry points replace the reference's public API
+ * (include/ski-lls.h:15-28); the C++ drop-in header ski-lls.h forwards to
+ * them.  The component entry points expose each hot-path stage (pattern,
+ * transform, sketch, QR, triangular inverse, LSQR) so that each can be
+ * checked on its own.
+ */
+#ifndef SKL_H
+#define SKL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SKL_OK 0
+#define SKL_E_SHAPE 1       /* "Matrix is not overdetermined." (n < d) */
+#define SKL_E_RANK0 2       /* "SPQR Factorization fails" (rank 0) */
+#define SKL_E_NOMEM 3       /* device allocation failed */
+#define SKL_E_HIP 4         /* HIP runtime error */
+#define SKL_E_ARG 5         /* invalid argument */
+#define SKL_E_UNSUPPORTED 6 /* size outside the implemented range */
+#define SKL_E_NODEVICE 7    /* no usable GPU */
+
+/* precision modes */
+#define SKL_FP64 0  /* sketch, transform, QR and LSQR all in FP64 */
+#define SKL_MIXED 1 /* sketch, transform and QR in FP32 (FP64 QR fallback when
+                       the condition estimate of R exceeds escalate_cond);
+                       LSQR in FP64 */
+
+typedef struct skl_opts {
+    double gamma;           /* over-sampling ratio, m = ceil(gamma d)          */
+    long k;                 /* nonzeros per column of the hashing matrix       */
+    double abs_tol;         /* early exit when ||b - A x0|| < abs_tol          */
+    double rcond;           /* dense: rank = #leading |R_ii| > rcond           */
+    double it_tol;          /* LSQR relative tolerance                         */
+    long max_it;            /* LSQR iteration limit                            */
+    double rcond_threshold; /* sparse: R11 "well conditioned" threshold        */
+    double perturb;         /* sparse: pivot perturbation when ill conditioned */
+    int ordering;           /* accepted for compatibility, ignored             */
+    int debug;              /* accepted for compatibility, ignored             */
+    int wisdom;             /* accepted for compatibility, ignored             */
+    unsigned seed;          /* seed of the libc-compatible generator           */
+    int precision;          /* SKL_FP64 or SKL_MIXED                           */
+    double escalate_cond;   /* MIXED: condition estimate that forces FP64 QR   */
+    int use_graph;          /* capture the LSQR loop in a hipGraph (1)         */
+    int graph_chunk;        /* LSQR iterations per graph launch                */
+    int reserved[6];
+} skl_opts;
+
+typedef struct skl_info {
+    long rank;
+    int flag;           /* 0 converged / early exit, 1 max_it reached   */
+    long it;            /* LSQR iterations                              */
+    int success;        /* sparse: rcond(R11) > rcond_threshold         */
+    int escalated;      /* MIXED: FP64 QR fallback taken                */
+    double rcond;       /* reciprocal inf-norm condition number of R11  */
+    double cond_est;    /* condition estimate used for escalation       */
+    double x0_residual; /* ||b - A x0|| of the sketch-and-solve start   */
+    double lsqr_nrm_ar, lsqr_nrm_a, lsqr_nrm_r; /* LSQR final estimates */
+    double t_pattern, t_sketch, t_qr, t_x0, t_lsqr, t_total; /* seconds */
+    long m;             /* sketch rows                                  */
+} skl_info;
+
+/* LSQR per-iteration scalars, for step-by-step comparison */
+typedef struct skl_lsqr_step {
+    double alpha, beta, rhobar, phibar, nrm_a;
+} skl_lsqr_step;
+
+void skl_default_opts(skl_opts *o);
+int skl_init(int device);
+void skl_finalize(void);
+const char *skl_error_string(int code);
+const char *skl_last_error(void);
+int skl_device_count(void);
+
+/* ---- solvers: host pointers ------------------------------------------------
+ * ls_dense_hashing_blendenpik (src/ls_dense_hashing_blendenpik.cpp:23-149).
+ * A (n x d, lda) is read-only here: the column permutation the reference
+ * applies in place (dlapmt, :123,147) is folded into the kernels' indexing. */
+int skl_dense_solve(long n, long d, const double *A, long lda, const double *b, double *x,
+                    const skl_opts *opts, skl_info *info);
+/* ls_dense_hashing_blendenpik_noCPQR (src/ls_dense_hashing_blendenpik_noCPQR.cpp:23-60) */
+int skl_dense_solve_nocpqr(long n, long d, const double *A, long lda, const double *b,
+                           double *x, const skl_opts *opts, skl_info *info);
+/* ls_sparse_spqr (src/ls_sparse_spqr.cpp:79-214); A in CSC (cs_dl layout:
+ * colptr d+1, rowind/val nnz) */
+int skl_sparse_solve(long n, long d, long nnz, const long *colptr, const long *rowind,
+                     const double *val, const double *b, double *x, const skl_opts *opts,
+                     skl_info *info);
+
+/* ---- solvers: device-resident inputs (pointers into the current device) ---- */
+int skl_dense_solve_dev(long n, long d, const double *dA, long lda, const double *db,
+                        double *dx, const skl_opts *opts, skl_info *info);
+int skl_dense_solve_nocpqr_dev(long n, long d, const double *dA, long lda, const double *db,
+                               double *dx, const skl_opts *opts, skl_info *info);
+int skl_sparse_solve_dev(long n, long d, long nnz, const long *dcolptr, const long *drowind,
+                         const double *dval, const double *db, double *dx,
+                         const skl_opts *opts, skl_info *info);
+
+/* LSQR per-iteration history of the next solve on this thread (optional):
+ * steps must hold max_steps entries; pass NULL to switch it off. */
+void skl_set_history(skl_lsqr_step *steps, long max_steps);
+
+/* ---- component entry points (host pointers; each runs one stage) -----------
+ * Hashing pattern in the reference's draw order: when dense != 0 the first n
+ * draws give the signs D (src/dense_helpers.cpp:7-16, D may be NULL to skip
+ * them), then n*k row draws and n*k sign draws (src/sparse_helpers.cpp:90-116). */
+int skl_hash_pattern(unsigned seed, int dense, long m, long n, long k, double *D, long *rows,
+                     double *vals);
+/* Y = DHT(X) per column, unnormalised FFTW_DHT (src/fftw_r2r.cpp:17-54);
+ * precision SKL_FP64 or SKL_MIXED (FP32 arithmetic, FP64 in/out) */
+int skl_dht(long n, long ncols, const double *X, long ldx, double *Y, long ldy, int precision);
+/* Y (m x ncols) = alpha S X with S the hashing matrix (rows/vals, k per
+ * column), summed per bucket in increasing source index (SpMat.cpp:53-55) */
+int skl_hash_apply(long m, long n, long k, const long *rows, const double *vals, double alpha,
+                   long ncols, const double *X, long ldx, double *Y, long ldy, int precision);
+/* SA (m x (d + has_b)) = (1/sqrt k) S H D [A, b]; the randomized transform and
+ * hashing of rand_hashing_dct (src/dense_helpers.cpp:83-112) */
+int skl_dense_sketch(long n, long d, const double *A, long lda, const double *b, long m,
+                     long k, const double *D, const long *rows, const double *vals,
+                     double *SA, long ldsa, int precision);
+/* SA (m x d, dense) = S A_csc, and bs = S b when b != NULL; CXSparse
+ * cs_multiply / cs_gaxpy summation order (src/sketching.cpp:90-111) */
+int skl_sparse_sketch(long m, long n, long d, long k, const long *rows, const double *vals,
+                      const long *colptr, const long *rowind, const double *val,
+                      const double *b, double *SA, long ldsa, double *bs);
+/* Householder QR (dgeqrf semantics, src/dense_qr.cpp:8-33) of A (m x n) in
+ * place; the first nfac columns are factored and the rest receive Q^T. */
+int skl_qr(long m, long n, long nfac, double *A, long lda, double *tau, int precision);
+/* column-pivoted QR of the first nfac columns (dgeqp4_INT dispatch,
+ * hqrrp/.../NoFLA_HQRRP_WY_blk_var4_skint.c:131-310: greedy dgeqp3 pivoting
+ * when min(m,nfac) < 1000, else HQRRP); remaining columns receive Q^T.
+ * jpvt is 1-based on output. */
+int skl_cpqr(long m, long n, long nfac, double *A, long lda, long *jpvt, double *tau,
+             int precision);
+/* Minv = (R + perturb I)^{-1} for upper triangular R (r x r, ldr) */
+int skl_trinv(long r, const double *R, long ldr, double perturb, double *Minv, long ldm);
+/* LSQR on op = A(:, E[0:r]) M (M = R^{-1} upper, r x r), from y = 0
+ * (src/IterSolver.cpp:160-284); E may be NULL (identity). */
+int skl_lsqr_dense(long n, long d, const double *A, long lda, long r, const long *E,
+                   const double *M, long ldm, const double *b, double tol, long max_it,
+                   double *y, int *flag, long *it, skl_lsqr_step *hist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SKL_H */

@@ -1,0 +1,340 @@
+// k_blas.hip -- dense building blocks for gfx950: MFMA GEMM (FP64 / FP32),
+// blocked upper-triangular inverse, norms and small copy kernels.
+#include <algorithm>
+#include <cmath>
+
+#include "k_common.h"
+
+namespace skl {
+
+// ---------------------------------------------------------------------------
+// GEMM on the matrix cores.  64x64 output tile per 256-thread workgroup; each
+// wave owns a 32x32 quarter as 2x2 MFMA 16x16 blocks (v_mfma_f64_16x16x4f64 or
+// v_mfma_f32_16x16x4f32, K = 4 per instruction).  Operand layout of the
+// 16x16x4 instructions: A(i,k) in lane i + 16k, B(k,j) in lane j + 16k.
+// Accumulator element r of lane l holds D(row(l, r), l % 16) with
+// row = 4 (l/16) + r for f32 and row = l/16 + 4 r for f64 (measured on gfx950).
+// ---------------------------------------------------------------------------
+template <class T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef double acc __attribute__((ext_vector_type(4)));
+    static __device__ inline acc fma(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct Mfma<float> {
+    typedef float acc __attribute__((ext_vector_type(4)));
+    static __device__ inline acc fma(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+constexpr int GBM = 64, GBN = 64, GBK = 16;
+
+template <class T, bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T *__restrict__ A,
+                                              long lda, const T *__restrict__ B, long ldb, T beta,
+                                              T *__restrict__ C, long ldc, long sA, long sB,
+                                              long sC) {
+    typedef typename Mfma<T>::acc acc_t;
+    __shared__ T As[2][GBK][GBM + 1];
+    __shared__ T Bs[2][GBK][GBN + 1];
+    const long bz = blockIdx.z;
+    A += bz * sA;
+    B += bz * sB;
+    C += bz * sC;
+    const int i0 = blockIdx.x * GBM, j0 = blockIdx.y * GBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    acc_t acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+
+    T ra[4], rb[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int e = tid + 256 * q;
+            int ii, kk;
+            if (!TA) { ii = e % GBM; kk = e / GBM; }
+            else { kk = e % GBK; ii = e / GBK; }
+            int gi = i0 + ii, gk = k0 + kk;
+            T v = 0;
+            if (gi < M && gk < K) v = TA ? A[gk + (long)gi * lda] : A[gi + (long)gk * lda];
+            ra[q] = v;
+            int jj;
+            if (!TB) { kk = e % GBK; jj = e / GBK; }
+            else { jj = e % GBN; kk = e / GBN; }
+            int gj = j0 + jj;
+            gk = k0 + kk;
+            T w = 0;
+            if (gk < K && gj < N) w = TB ? B[gj + (long)gk * ldb] : B[gk + (long)gj * ldb];
+            rb[q] = w;
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int e = tid + 256 * q;
+            int ii, kk, jj;
+            if (!TA) { ii = e % GBM; kk = e / GBM; }
+            else { kk = e % GBK; ii = e / GBK; }
+            As[buf][kk][ii] = ra[q];
+            if (!TB) { kk = e % GBK; jj = e / GBK; }
+            else { jj = e % GBN; kk = e / GBN; }
+            Bs[buf][kk][jj] = rb[q];
+        }
+    };
+    int nk = (K + GBK - 1) / GBK;
+    if (nk > 0) {
+        load(0);
+        store(0);
+    }
+    __syncthreads();
+    for (int t = 0; t < nk; ++t) {
+        int buf = t & 1;
+        if (t + 1 < nk) load((t + 1) * GBK);
+#pragma unroll
+        for (int ks = 0; ks < GBK; ks += 4) {
+            T a0 = As[buf][ks + (lane >> 4)][wm + (lane & 15)];
+            T a1 = As[buf][ks + (lane >> 4)][wm + 16 + (lane & 15)];
+            T b0 = Bs[buf][ks + (lane >> 4)][wn + (lane & 15)];
+            T b1 = Bs[buf][ks + (lane >> 4)][wn + 16 + (lane & 15)];
+            acc[0][0] = Mfma<T>::fma(a0, b0, acc[0][0]);
+            acc[0][1] = Mfma<T>::fma(a0, b1, acc[0][1]);
+            acc[1][0] = Mfma<T>::fma(a1, b0, acc[1][0]);
+            acc[1][1] = Mfma<T>::fma(a1, b1, acc[1][1]);
+        }
+        if (t + 1 < nk) store(buf ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int gi = i0 + wm + mi * 16 + Mfma<T>::row(lane, r);
+                int gj = j0 + wn + ni * 16 + (lane & 15);
+                if (gi < M && gj < N) {
+                    T *c = C + gi + (long)gj * ldc;
+                    T v = alpha * acc[mi][ni][r];
+                    *c = (beta == T(0)) ? v : v + beta * *c;
+                }
+            }
+}
+
+template <class T>
+void gemm(char ta, char tb, long M, long N, long K, T alpha, const T *A, long lda, const T *B,
+          long ldb, T beta, T *C, long ldc, long batch, long sA, long sB, long sC) {
+    if (M <= 0 || N <= 0 || batch <= 0) return;
+    dim3 grid((unsigned)((M + GBM - 1) / GBM), (unsigned)((N + GBN - 1) / GBN), (unsigned)batch);
+    hipStream_t s = ctx().stream;
+    bool TA = (ta == 'T' || ta == 't'), TB = (tb == 'T' || tb == 't');
+    if (!TA && !TB)
+        k_gemm<T, false, false><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, alpha, A, lda, B, ldb,
+                                                     beta, C, ldc, sA, sB, sC);
+    else if (TA && !TB)
+        k_gemm<T, true, false><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, alpha, A, lda, B, ldb,
+                                                    beta, C, ldc, sA, sB, sC);
+    else if (!TA && TB)
+        k_gemm<T, false, true><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, alpha, A, lda, B, ldb,
+                                                    beta, C, ldc, sA, sB, sC);
+    else
+        k_gemm<T, true, true><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, alpha, A, lda, B, ldb,
+                                                   beta, C, ldc, sA, sB, sC);
+    SKL_CHECK_LAUNCH();
+}
+
+template void gemm<double>(char, char, long, long, long, double, const double *, long,
+                           const double *, long, double, double *, long, long, long, long, long);
+template void gemm<float>(char, char, long, long, long, float, const float *, long,
+                          const float *, long, float, float *, long, long, long, long, long);
+
+// ---------------------------------------------------------------------------
+// Small helpers
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_fill2d(long m, long n, T *A, long lda, T v) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long j = blockIdx.y;
+    for (; j < n; j += gridDim.y)
+        if (i < m) A[i + j * lda] = v;
+}
+
+template <class Ts, class Td>
+__global__ void k_copy2d(long m, long n, const Ts *A, long lda, Td *B, long ldb) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (long j = blockIdx.y; j < n; j += gridDim.y)
+        if (i < m) B[i + j * ldb] = (Td)A[i + j * lda];
+}
+
+template <class T> void fill2d(long m, long n, T *A, long lda, T v) {
+    if (m <= 0 || n <= 0) return;
+    dim3 grid((unsigned)((m + 255) / 256), (unsigned)std::min<long>(n, 65535));
+    k_fill2d<T><<<grid, 256, 0, ctx().stream>>>(m, n, A, lda, v);
+    SKL_CHECK_LAUNCH();
+}
+template void fill2d<double>(long, long, double *, long, double);
+template void fill2d<float>(long, long, float *, long, float);
+template void fill2d<int32_t>(long, long, int32_t *, long, int32_t);
+
+template <class Ts, class Td> void copy2d(long m, long n, const Ts *A, long lda, Td *B, long ldb) {
+    if (m <= 0 || n <= 0) return;
+    dim3 grid((unsigned)((m + 255) / 256), (unsigned)std::min<long>(n, 65535));
+    k_copy2d<Ts, Td><<<grid, 256, 0, ctx().stream>>>(m, n, A, lda, B, ldb);
+    SKL_CHECK_LAUNCH();
+}
+template void copy2d<double, double>(long, long, const double *, long, double *, long);
+template void copy2d<double, float>(long, long, const double *, long, float *, long);
+template void copy2d<float, double>(long, long, const float *, long, double *, long);
+template void copy2d<float, float>(long, long, const float *, long, float *, long);
+
+// upper triangle of R (T) into Rd (double), zeros below the diagonal
+template <class T>
+__global__ void k_upcast_upper(long r, const T *R, long ldr, double *Rd, long ldd) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (long j = blockIdx.y; j < r; j += gridDim.y)
+        if (i < r) Rd[i + j * ldd] = (i <= j) ? (double)R[i + j * ldr] : 0.0;
+}
+template <class T> void upcast_upper(long r, const T *R, long ldr, double *Rd, long ldd) {
+    if (r <= 0) return;
+    dim3 grid((unsigned)((r + 255) / 256), (unsigned)std::min<long>(r, 65535));
+    k_upcast_upper<T><<<grid, 256, 0, ctx().stream>>>(r, R, ldr, Rd, ldd);
+    SKL_CHECK_LAUNCH();
+}
+template void upcast_upper<double>(long, const double *, long, double *, long);
+template void upcast_upper<float>(long, const float *, long, double *, long);
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions: per-block partials then one block for the total.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sumsq_partial(long n, const double *x, double *part) {
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        s += x[i] * x[i];
+    s = block_sum<256>(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ __launch_bounds__(256) void k_sum_final(int np, const double *part, double *out,
+                                                   int take_sqrt) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) s += part[i];
+    s = block_sum<256>(s);
+    if (threadIdx.x == 0) *out = take_sqrt ? sqrt(s) : s;
+}
+
+double dev_nrm2(long n, const double *x, Arena &ar) {
+    int np = (int)std::min<long>(1024, std::max<long>(1, (n + 255) / 256));
+    double *part = ar.alloc_n<double>((size_t)np + 1);
+    hipStream_t s = ctx().stream;
+    k_sumsq_partial<<<np, 256, 0, s>>>(n, x, part);
+    k_sum_final<<<1, 256, 0, s>>>(np, part, part + np, 1);
+    SKL_CHECK_LAUNCH();
+    double h = 0;
+    SKL_HIP(hipMemcpyAsync(&h, part + np, sizeof(double), hipMemcpyDeviceToHost, s));
+    SKL_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+// inf-norm of an upper triangular matrix: max_i sum_{j>=i} |R_ij|
+__global__ __launch_bounds__(256) void k_rowabs_upper(long r, const double *R, long ldr,
+                                                      double *rs) {
+    long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= r) return;
+    double s = 0.0;
+    for (long j = i; j < r; ++j) s += fabs(R[i + j * ldr]);
+    rs[i] = s;
+}
+__global__ __launch_bounds__(256) void k_max_final(long n, const double *x, double *out) {
+    double s = 0.0;
+    for (long i = threadIdx.x; i < n; i += 256) s = fmax(s, x[i]);
+    s = block_max<256>(s);
+    if (threadIdx.x == 0) *out = s;
+}
+double norm_inf_upper(long r, const double *R, long ldr, Arena &ar) {
+    if (r <= 0) return 0.0;
+    double *rs = ar.alloc_n<double>((size_t)r + 1);
+    hipStream_t s = ctx().stream;
+    k_rowabs_upper<<<(unsigned)((r + 255) / 256), 256, 0, s>>>(r, R, ldr, rs);
+    k_max_final<<<1, 256, 0, s>>>(r, rs, rs + r);
+    SKL_CHECK_LAUNCH();
+    double h = 0;
+    SKL_HIP(hipMemcpyAsync(&h, rs + r, sizeof(double), hipMemcpyDeviceToHost, s));
+    SKL_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+// ---------------------------------------------------------------------------
+// Upper triangular inverse M = (R + p I)^{-1}.  Diagonal 32x32 blocks are
+// inverted in LDS, one workgroup each, walking the superdiagonals; the
+// off-diagonal blocks follow by doubling: for [M11 M12; 0 M22],
+// M12 = -M11 R12 M22, two batched GEMMs per level.
+// ---------------------------------------------------------------------------
+constexpr int TIB = 32;
+
+__global__ __launch_bounds__(1024) void k_trinv_diag(long r, const double *R, long ldr,
+                                                     double p, double *M, long ldm) {
+    __shared__ double Rs[TIB][TIB + 1];
+    __shared__ double Xs[TIB][TIB + 1];
+    long b0 = (long)blockIdx.x * TIB;
+    int nb = (int)min((long)TIB, r - b0);
+    int i = threadIdx.x % TIB, j = threadIdx.x / TIB;
+    Rs[i][j] = (i < nb && j < nb && i <= j) ? R[(b0 + i) + (b0 + j) * ldr] : 0.0;
+    if (i == j && i < nb) Rs[i][j] += p;
+    Xs[i][j] = 0.0;
+    __syncthreads();
+    if (i == j && i < nb) Xs[i][i] = 1.0 / Rs[i][i];
+    __syncthreads();
+    for (int dist = 1; dist < nb; ++dist) {
+        if (j - i == dist && j < nb) {
+            double s = 0.0;
+            for (int l = i + 1; l <= j; ++l) s += Rs[i][l] * Xs[l][j];
+            Xs[i][j] = -s / Rs[i][i];
+        }
+        __syncthreads();
+    }
+    if (i < nb && j < nb) M[(b0 + i) + (b0 + j) * ldm] = Xs[i][j];
+}
+
+void trinv_upper(long r, const double *R, long ldr, double perturb, double *M, long ldm,
+                 Arena &ar) {
+    if (r <= 0) return;
+    fill2d<double>(r, r, M, ldm, 0.0);
+    long nblk = (r + TIB - 1) / TIB;
+    k_trinv_diag<<<(unsigned)nblk, 1024, 0, ctx().stream>>>(r, R, ldr, perturb, M, ldm);
+    SKL_CHECK_LAUNCH();
+    // doubling over super-blocks [b0, b0+2s): M12 (rows b0.., cols b0+s..)
+    double *tmp = ar.alloc_n<double>((size_t)r * std::min<long>(r, 2 * TIB) + 64);
+    size_t tmp_cap = (size_t)r * std::min<long>(r, 2 * TIB) + 64;
+    for (long s = TIB; s < r; s *= 2) {
+        long nfull = r / (2 * s);         // super-blocks with a full M22
+        long rem = r - nfull * 2 * s;     // tail: [b0, r) with b0 = nfull*2s
+        size_t need = (size_t)s * s * (size_t)std::max<long>(nfull, 1);
+        if (need > tmp_cap) {
+            tmp = ar.alloc_n<double>(need);
+            tmp_cap = need;
+        }
+        if (nfull > 0) {
+            // tmp = R12 * M22 ; M12 = -M11 * tmp
+            long str = 2 * s * (ldr + 1), stm = 2 * s * (ldm + 1);
+            gemm<double>('N', 'N', s, s, s, 1.0, R + s * ldr, ldr, M + s + s * ldm, ldm, 0.0, tmp,
+                         s, nfull, str, stm, s * s);
+            gemm<double>('N', 'N', s, s, s, -1.0, M, ldm, tmp, s, 0.0, M + s * ldm, ldm, nfull,
+                         stm, s * s, stm);
+        }
+        if (rem > s) {
+            long b0 = nfull * 2 * s, s2 = rem - s;
+            const double *R12 = R + b0 + (b0 + s) * ldr;
+            const double *M22 = M + (b0 + s) + (b0 + s) * ldm;
+            const double *M11 = M + b0 + b0 * ldm;
+            double *M12 = M + b0 + (b0 + s) * ldm;
+            gemm<double>('N', 'N', s, s2, s2, 1.0, R12, ldr, M22, ldm, 0.0, tmp, s);
+            gemm<double>('N', 'N', s, s2, s, -1.0, M11, ldm, tmp, s, 0.0, M12, ldm);
+        }
+    }
+}
+
+} // namespace skl

@@ -1,0 +1,541 @@
+// k_dht.hip -- randomized discrete Hartley transform and hashing sketch.
+//
+// The reference applies y = (1/sqrt n) D .* A(:, c), an unnormalised FFTW
+// DHT, then the hashing matrix S (src/fast_unitary_transform.cpp:88-100,
+// src/fftw_r2r.cpp:45-50, src/sketching.cpp:39-62).  Here two real columns
+// share one complex FFT (z = x1 + i x2; with Z_k = a+ib, Z_{n-k} = c+id:
+// H1_k = ((a+c) - (b-d))/2, H2_k = ((b+d) + (a-c))/2).  The FFT is a
+// mixed-radix Stockham DIF held in LDS; each stage keeps its inputs in
+// registers between two barriers so one LDS buffer suffices.
+//
+//  * n <= LDS capacity: one workgroup per column pair does load -> FFT ->
+//    DHT -> bucket sums straight from LDS -> SA, reading A once.
+//  * larger smooth n = n1 n2: four-step FFT (pass 1: length-n1 FFTs on
+//    strided rows + twiddles; pass 2: length-n2 FFTs on pairs of mirrored
+//    rows, which also yields the DHT), then the bucket gather.
+//
+// Bucket sums run with contraction off, in increasing source row, matching
+// the reference's scatter order (src/SpMat.cpp:53-55).
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+
+#include "fft_roots.h"
+#include "k_common.h"
+
+namespace skl {
+
+constexpr int FFT_MAXE = 16;   // complex elements per thread held per stage
+constexpr int MAX_STAGES = 24;
+
+struct FftDims {
+    int L;                  // transform length
+    int nst;                // stages
+    int radix[MAX_STAGES];
+};
+
+template <class T> struct FftPlanDev {
+    long n = 0;
+    int mode = 0;           // 1 single pass, 2 four-step
+    int n1 = 0, n2 = 0;
+    FftDims d1{}, d2{};     // single: d1 only
+    cplx<T> *tw1 = nullptr; // W_{n1}^e
+    cplx<T> *tw2 = nullptr; // W_{n2}^e
+    cplx<T> *twn = nullptr; // W_n^e (four-step twiddles)
+    int J2 = 0, K1 = 0;     // sequences per workgroup in the two passes
+};
+
+// ---------------------------------------------------------------------------
+// radix-R DIF stage on S interleaved sequences in LDS (element j of sequence s
+// at buf[j*S + s]); len = current length, stride = product of earlier radices.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void dft4(cplx<T> p0, cplx<T> p1, cplx<T> p2, cplx<T> p3, cplx<T> &o0,
+                                     cplx<T> &o1, cplx<T> &o2, cplx<T> &o3) {
+    cplx<T> t0 = cadd(p0, p2), t1 = csub(p0, p2), t2 = cadd(p1, p3), t3 = csub(p1, p3);
+    o0 = cadd(t0, t2);
+    o2 = csub(t0, t2);
+    o1 = {t1.x + t3.y, t1.y - t3.x}; // t1 - i t3
+    o3 = {t1.x - t3.y, t1.y + t3.x}; // t1 + i t3
+}
+
+template <class T, int R, int NT>
+__device__ __forceinline__ void fft_stage(cplx<T> *buf, int L, int S, int len, int stride,
+                                          const cplx<T> *__restrict__ tw) {
+    constexpr int NB = (FFT_MAXE + R - 1) / R;
+    const int m = len / R;
+    const int nbfly = (L / R) * S;
+    const int tws = L / len;
+    cplx<T> v[NB][R];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        int id = threadIdx.x + b * NT;
+        if (id < nbfly) {
+            int s = id % S, t = id / S;
+            int q = t % stride, p = t / stride;
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[b][j] = buf[(q + stride * (p + j * m)) * S + s];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        int id = threadIdx.x + b * NT;
+        if (id < nbfly) {
+            int s = id % S, t = id / S;
+            int q = t % stride, p = t / stride;
+            cplx<T> y[R];
+            if constexpr (R == 2) {
+                y[0] = cadd(v[b][0], v[b][1]);
+                y[1] = csub(v[b][0], v[b][1]);
+            } else if constexpr (R == 4) {
+                dft4(v[b][0], v[b][1], v[b][2], v[b][3], y[0], y[1], y[2], y[3]);
+            } else if constexpr (R == 8) {
+                // even/odd split: y_k = a_k + W8^k b_k, y_{k+4} = a_k - W8^k b_k
+                cplx<T> a[4], c[4];
+                dft4(v[b][0], v[b][2], v[b][4], v[b][6], a[0], a[1], a[2], a[3]);
+                dft4(v[b][1], v[b][3], v[b][5], v[b][7], c[0], c[1], c[2], c[3]);
+                const T h = (T)0.70710678118654752440;
+                cplx<T> w1 = {h * (c[1].x + c[1].y), h * (c[1].y - c[1].x)};   // (1-i)/sqrt2
+                cplx<T> w2 = {c[2].y, -c[2].x};                                 // -i
+                cplx<T> w3 = {h * (c[3].y - c[3].x), -h * (c[3].x + c[3].y)};  // -(1+i)/sqrt2
+                y[0] = cadd(a[0], c[0]);
+                y[4] = csub(a[0], c[0]);
+                y[1] = cadd(a[1], w1);
+                y[5] = csub(a[1], w1);
+                y[2] = cadd(a[2], w2);
+                y[6] = csub(a[2], w2);
+                y[3] = cadd(a[3], w3);
+                y[7] = csub(a[3], w3);
+            } else {
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    T sr = 0, si = 0;
+#pragma unroll
+                    for (int j = 0; j < R; ++j) {
+                        const int e = (j * k) % R;
+                        const T cr = (T)RootTab<R>::c[e], ci = (T)RootTab<R>::s[e];
+                        sr += v[b][j].x * cr - v[b][j].y * ci;
+                        si += v[b][j].x * ci + v[b][j].y * cr;
+                    }
+                    y[k] = {sr, si};
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                cplx<T> o = y[k];
+                if (k > 0) {
+                    int e = (int)(((long)p * k) % len) * tws;
+                    o = cmul(o, tw[e]);
+                }
+                buf[(q + stride * (R * p + k)) * S + s] = o;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <class T, int NT>
+__device__ void lds_fft(cplx<T> *buf, int S, const FftDims &D, const cplx<T> *__restrict__ tw) {
+    int len = D.L, stride = 1;
+    for (int st = 0; st < D.nst; ++st) {
+        int r = D.radix[st];
+        switch (r) {
+        case 2: fft_stage<T, 2, NT>(buf, D.L, S, len, stride, tw); break;
+        case 3: fft_stage<T, 3, NT>(buf, D.L, S, len, stride, tw); break;
+        case 4: fft_stage<T, 4, NT>(buf, D.L, S, len, stride, tw); break;
+        case 5: fft_stage<T, 5, NT>(buf, D.L, S, len, stride, tw); break;
+        case 7: fft_stage<T, 7, NT>(buf, D.L, S, len, stride, tw); break;
+        case 8: fft_stage<T, 8, NT>(buf, D.L, S, len, stride, tw); break;
+        case 11: fft_stage<T, 11, NT>(buf, D.L, S, len, stride, tw); break;
+        case 13: fft_stage<T, 13, NT>(buf, D.L, S, len, stride, tw); break;
+        default: break;
+        }
+        len /= r;
+        stride *= r;
+    }
+}
+
+template <class T> __device__ inline T load_scaled(const double *X, long idx, const double *D,
+                                                   long j, double scale) {
+    // (scale * D[j]) * A(j, c) as in fast_unitary_transform.cpp:91
+    double f = D ? scale * D[j] : scale;
+    return (T)(f * X[idx]);
+}
+template <> __device__ inline float load_scaled<float>(const double *X, long idx, const double *D,
+                                                       long j, double scale) {
+    double f = D ? scale * D[j] : scale;
+    return (float)f * (float)X[idx];
+}
+
+// ---------------------------------------------------------------------------
+// Single pass: one workgroup per column pair (c0 = 2 blockIdx.x, c1 = c0 + 1).
+// out_mode 0: hash into SA (m x ncols); 1: write the DHT into Y (n x ncols).
+// ---------------------------------------------------------------------------
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void k_dht_single(long n, long ncols, const double *__restrict__ X,
+                                                   long ldx, const double *__restrict__ D,
+                                                   double scale, FftDims dims,
+                                                   const cplx<T> *__restrict__ tw, int out_mode,
+                                                   long m, const int32_t *__restrict__ bptr,
+                                                   const int32_t *__restrict__ bsrc,
+                                                   const T *__restrict__ bcoef, T *__restrict__ out,
+                                                   long ldo) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const long c0 = 2L * blockIdx.x, c1 = c0 + 1;
+    const bool has1 = c1 < ncols;
+    for (long j = threadIdx.x; j < n; j += NT) {
+        T re = load_scaled<T>(X, j + c0 * ldx, D, j, scale);
+        T im = has1 ? load_scaled<T>(X, j + c1 * ldx, D, j, scale) : T(0);
+        buf[j] = {re, im};
+    }
+    __syncthreads();
+    lds_fft<T, NT>(buf, 1, dims, tw);
+    // DHT extraction through registers
+    T h1[FFT_MAXE], h2[FFT_MAXE];
+#pragma unroll
+    for (int e = 0; e < FFT_MAXE; ++e) {
+        long k = threadIdx.x + (long)e * NT;
+        if (k < n) {
+            long nk = (k == 0) ? 0 : n - k;
+            cplx<T> a = buf[k], c = buf[nk];
+            h1[e] = T(0.5) * ((a.x + c.x) - (a.y - c.y));
+            h2[e] = T(0.5) * ((a.y + c.y) + (a.x - c.x));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < FFT_MAXE; ++e) {
+        long k = threadIdx.x + (long)e * NT;
+        if (k < n) buf[k] = {h1[e], h2[e]};
+    }
+    __syncthreads();
+    if (out_mode == 1) {
+        for (long k = threadIdx.x; k < n; k += NT) {
+            out[k + c0 * ldo] = buf[k].x;
+            if (has1) out[k + c1 * ldo] = buf[k].y;
+        }
+        return;
+    }
+    {
+#pragma clang fp contract(off)
+        for (long h = threadIdx.x; h < m; h += NT) {
+            T s1 = 0, s2 = 0;
+            for (int q = bptr[h]; q < bptr[h + 1]; ++q) {
+                cplx<T> hv = buf[bsrc[q]];
+                T cf = bcoef[q];
+                s1 = s1 + cf * hv.x;
+                s2 = s2 + cf * hv.y;
+            }
+            out[h + c0 * ldo] = s1;
+            if (has1) out[h + c1 * ldo] = s2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Four-step, pass 1: workgroup (pair, block of J2 consecutive j2):
+// Y[k1*n2 + j2] = W_n^{j2 k1} * FFT_{n1}( x[j1*n2 + j2] )[k1]
+// ---------------------------------------------------------------------------
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void k_fft4_pass1(long n, int n1, int n2, int J2, long ncols,
+                                                   const double *__restrict__ X, long ldx,
+                                                   const double *__restrict__ D, double scale,
+                                                   FftDims dims, const cplx<T> *__restrict__ tw1,
+                                                   const cplx<T> *__restrict__ twn,
+                                                   cplx<T> *__restrict__ Y) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const long pair = blockIdx.y;
+    const long c0 = 2 * pair, c1 = c0 + 1;
+    const bool has1 = c1 < ncols;
+    const int j2b = blockIdx.x * J2;
+    const int nJ = min(J2, n2 - j2b);
+    const int tot = n1 * J2;
+    for (int e = threadIdx.x; e < tot; e += NT) {
+        int jj = e % J2, j1 = e / J2;
+        cplx<T> v = {T(0), T(0)};
+        if (jj < nJ) {
+            long j = (long)j1 * n2 + j2b + jj;
+            v.x = load_scaled<T>(X, j + c0 * ldx, D, j, scale);
+            v.y = has1 ? load_scaled<T>(X, j + c1 * ldx, D, j, scale) : T(0);
+        }
+        buf[j1 * J2 + jj] = v;
+    }
+    __syncthreads();
+    lds_fft<T, NT>(buf, J2, dims, tw1);
+    cplx<T> *Yp = Y + pair * n;
+    for (int e = threadIdx.x; e < tot; e += NT) {
+        int jj = e % J2, k1 = e / J2;
+        if (jj < nJ) {
+            int j2 = j2b + jj;
+            long ex = ((long)j2 * k1) % n;
+            Yp[(long)k1 * n2 + j2] = cmul(buf[k1 * J2 + jj], twn[ex]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Four-step, pass 2: workgroup (pair, block of K1 primary rows k1 in
+// [0, n1/2] plus their mirrors n1 - k1): X[k1 + n1 k2] = FFT_{n2}(Y[k1, :])[k2],
+// then H at k and n - k from the primary/mirror rows, written to H (n x ncols).
+// ---------------------------------------------------------------------------
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K1, long ncols,
+                                                   FftDims dims, const cplx<T> *__restrict__ tw2,
+                                                   const cplx<T> *__restrict__ Y,
+                                                   T *__restrict__ H, long ldh) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const long pair = blockIdx.y;
+    const long c0 = 2 * pair, c1 = c0 + 1;
+    const bool has1 = c1 < ncols;
+    const int half = n1 / 2;                 // primary rows 0..half
+    const int kb = blockIdx.x * K1;
+    const int nK = min(K1, half + 1 - kb);   // primary rows in this block
+    const int S = 2 * K1;                    // sequence slots: [0,K1) primary, [K1,2K1) mirror
+    const cplx<T> *Yp = Y + pair * n;
+    const int tot = n2 * S;
+    for (int e = threadIdx.x; e < tot; e += NT) {
+        int sl = e % S, j2 = e / S;
+        int kk = sl % K1;
+        cplx<T> v = {T(0), T(0)};
+        if (kk < nK) {
+            int k1 = kb + kk;
+            int row = (sl < K1) ? k1 : (n1 - k1) % n1;
+            v = Yp[(long)row * n2 + j2];
+        }
+        buf[j2 * S + sl] = v;
+    }
+    __syncthreads();
+    lds_fft<T, NT>(buf, S, dims, tw2);
+    // outputs: for primary row k1 and column k2: k = k1 + n1 k2,
+    //   partner n-k = (n1-k1) + n1 (n2-1-k2) (mirror slot) for k1 > 0,
+    //               = n1 ((n2-k2) % n2)      (same slot)   for k1 = 0.
+    // mirror row k1' = n1 - k1 (k1 > 0, k1' != k1): k' = k1' + n1 k2,
+    //   partner = k1 + n1 (n2-1-k2) (primary slot).
+    for (int e = threadIdx.x; e < tot; e += NT) {
+        int sl = e % S, k2 = e / S;
+        int kk = sl % K1;
+        if (kk >= nK) continue;
+        int k1 = kb + kk;
+        bool mirror = sl >= K1;
+        if (mirror && (k1 == 0 || 2 * k1 == n1)) continue; // self-mirrored rows done once
+        int row = mirror ? n1 - k1 : k1;
+        cplx<T> a = buf[k2 * S + sl];
+        cplx<T> c;
+        if (k1 == 0) {
+            c = buf[((n2 - k2) % n2) * S + sl];
+        } else if (2 * k1 == n1) {
+            c = buf[(n2 - 1 - k2) * S + sl];
+        } else {
+            int osl = mirror ? kk : kk + K1;
+            c = buf[(n2 - 1 - k2) * S + osl];
+        }
+        long k = row + (long)n1 * k2;
+        T h1 = T(0.5) * ((a.x + c.x) - (a.y - c.y));
+        T h2 = T(0.5) * ((a.y + c.y) + (a.x - c.x));
+        H[k + c0 * ldh] = h1;
+        if (has1) H[k + c1 * ldh] = h2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Bucket gather: Y[h, c] = sum_{q in bucket h} coef[q] X[bsrc[q], c]
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_hash_gather(long m, long ncols,
+                                                     const int32_t *__restrict__ bptr,
+                                                     const int32_t *__restrict__ bsrc,
+                                                     const T *__restrict__ bcoef,
+                                                     const T *__restrict__ X, long ldx,
+                                                     T *__restrict__ Y, long ldy) {
+#pragma clang fp contract(off)
+    long h = (long)blockIdx.x * 256 + threadIdx.x;
+    if (h >= m) return;
+    const int q0 = bptr[h], q1 = bptr[h + 1];
+    for (long c = blockIdx.y; c < ncols; c += gridDim.y) {
+        const T *x = X + c * ldx;
+        T s = 0;
+        for (int q = q0; q < q1; ++q) s = s + bcoef[q] * x[bsrc[q]];
+        Y[h + c * ldy] = s;
+    }
+}
+
+template <class T>
+void hash_gather(long m, long ncols, const int32_t *bptr, const int32_t *bsrc, const T *bcoef,
+                 const T *X, long ldx, T *Y, long ldy) {
+    if (m <= 0 || ncols <= 0) return;
+    dim3 g((unsigned)((m + 255) / 256), (unsigned)std::min<long>(ncols, 4096));
+    k_hash_gather<T><<<g, 256, 0, ctx().stream>>>(m, ncols, bptr, bsrc, bcoef, X, ldx, Y, ldy);
+    SKL_CHECK_LAUNCH();
+}
+template void hash_gather<double>(long, long, const int32_t *, const int32_t *, const double *,
+                                  const double *, long, double *, long);
+template void hash_gather<float>(long, long, const int32_t *, const int32_t *, const float *,
+                                 const float *, long, float *, long);
+
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
+static bool factor_dims(long L, FftDims &d) {
+    d.L = (int)L;
+    d.nst = 0;
+    long x = L;
+    const int pref[] = {8, 4, 2, 3, 5, 7, 11, 13};
+    for (int r : pref) {
+        while (x % r == 0) {
+            if (d.nst >= MAX_STAGES) return false;
+            d.radix[d.nst++] = r;
+            x /= r;
+        }
+    }
+    return x == 1;
+}
+
+template <class T> static cplx<T> *make_twiddles(long L) {
+    std::vector<cplx<T>> h((size_t)L);
+    for (long e = 0; e < L; ++e) {
+        long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)e /
+                          (long double)L;
+        h[(size_t)e] = {(T)cosl(ang), (T)sinl(ang)};
+    }
+    cplx<T> *dptr = nullptr;
+    SKL_HIP(hipMalloc(&dptr, sizeof(cplx<T>) * (size_t)std::max<long>(L, 1)));
+    SKL_HIP(hipMemcpy(dptr, h.data(), sizeof(cplx<T>) * (size_t)L, hipMemcpyHostToDevice));
+    return dptr;
+}
+
+template <class T> constexpr int fft_nt() { return sizeof(T) == 8 ? 256 : 512; }
+template <class T> constexpr long lds_elems() { return (long)fft_nt<T>() * FFT_MAXE; } // 4096 / 8192
+
+static bool smooth(long x) {
+    for (int r : {2, 3, 5, 7, 11, 13})
+        while (x % r == 0) x /= r;
+    return x == 1;
+}
+
+template <class T> static const FftPlanDev<T> &get_plan(long n) {
+    static std::map<long, FftPlanDev<T>> cache;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(n);
+    if (it != cache.end()) return it->second;
+    FftPlanDev<T> p;
+    p.n = n;
+    const long cap = lds_elems<T>();
+    if (!smooth(n))
+        throw Error(SKL_E_UNSUPPORTED,
+                    "DHT length " + std::to_string(n) +
+                        " has a prime factor above 13 (not implemented)");
+    if (n <= cap) {
+        p.mode = 1;
+        factor_dims(n, p.d1);
+        p.tw1 = make_twiddles<T>(n);
+    } else {
+        // n = n1 n2, both smooth, n1 <= cap/4 (J2 >= 4), n2 <= cap/8 (K1 >= 4)
+        long best1 = -1;
+        double bestscore = 1e300;
+        for (long a = 2; a * a <= n * 64 && a <= n; ++a) {
+            if (n % a) continue;
+            long b = n / a;
+            if (!smooth(a) || !smooth(b)) continue;
+            if (a > cap / 4 || b > cap / 8) continue;
+            double score = std::fabs(std::log((double)a) - std::log((double)b));
+            if (score < bestscore) {
+                bestscore = score;
+                best1 = a;
+            }
+        }
+        if (best1 < 0)
+            throw Error(SKL_E_UNSUPPORTED, "DHT length " + std::to_string(n) + " too large");
+        p.mode = 2;
+        p.n1 = (int)best1;
+        p.n2 = (int)(n / best1);
+        factor_dims(p.n1, p.d1);
+        factor_dims(p.n2, p.d2);
+        p.tw1 = make_twiddles<T>(p.n1);
+        p.tw2 = make_twiddles<T>(p.n2);
+        p.twn = make_twiddles<T>(n);
+        p.J2 = (int)std::min<long>(32, cap / p.n1);
+        p.K1 = (int)std::min<long>(16, cap / (2L * p.n2));
+    }
+    auto res = cache.emplace(n, p);
+    return res.first->second;
+}
+
+// four-step into H (n x ncols, T)
+template <class T>
+static void dht_fourstep(const FftPlanDev<T> &P, long ncols, const double *X, long ldx,
+                         const double *D, double scale, T *H, long ldh, Arena &ar) {
+    constexpr int NT = fft_nt<T>();
+    long npairs = (ncols + 1) / 2;
+    cplx<T> *Y = ar.alloc_n<cplx<T>>((size_t)npairs * P.n);
+    hipStream_t s = ctx().stream;
+    {
+        dim3 g((unsigned)((P.n2 + P.J2 - 1) / P.J2), (unsigned)npairs);
+        size_t sh = sizeof(cplx<T>) * (size_t)P.n1 * P.J2;
+        k_fft4_pass1<T, NT><<<g, NT, sh, s>>>(P.n, P.n1, P.n2, P.J2, ncols, X, ldx, D, scale, P.d1,
+                                              P.tw1, P.twn, Y);
+        SKL_CHECK_LAUNCH();
+    }
+    {
+        int half = P.n1 / 2;
+        dim3 g((unsigned)((half + 1 + P.K1 - 1) / P.K1), (unsigned)npairs);
+        size_t sh = sizeof(cplx<T>) * (size_t)P.n2 * 2 * P.K1;
+        k_fft4_pass2<T, NT><<<g, NT, sh, s>>>(P.n, P.n1, P.n2, P.K1, ncols, P.d2, P.tw2, Y, H, ldh);
+        SKL_CHECK_LAUNCH();
+    }
+}
+
+template <class T>
+void dht_sketch(long n, long ncols, const double *X, long ldx, const double *D, double scale,
+                long m, const int32_t *bptr, const int32_t *bsrc, const T *bcoef, T *SA,
+                long ldsa, Arena &ar) {
+    const FftPlanDev<T> &P = get_plan<T>(n);
+    constexpr int NT = fft_nt<T>();
+    hipStream_t s = ctx().stream;
+    long npairs = (ncols + 1) / 2;
+    if (P.mode == 1) {
+        size_t sh = sizeof(cplx<T>) * (size_t)n;
+        k_dht_single<T, NT><<<(unsigned)npairs, NT, sh, s>>>(n, ncols, X, ldx, D, scale, P.d1,
+                                                             P.tw1, 0, m, bptr, bsrc, bcoef, SA,
+                                                             ldsa);
+        SKL_CHECK_LAUNCH();
+    } else {
+        T *H = ar.alloc_n<T>((size_t)n * ncols);
+        dht_fourstep<T>(P, ncols, X, ldx, D, scale, H, n, ar);
+        hash_gather<T>(m, ncols, bptr, bsrc, bcoef, H, n, SA, ldsa);
+    }
+}
+template void dht_sketch<double>(long, long, const double *, long, const double *, double, long,
+                                 const int32_t *, const int32_t *, const double *, double *, long,
+                                 Arena &);
+template void dht_sketch<float>(long, long, const double *, long, const double *, double, long,
+                                const int32_t *, const int32_t *, const float *, float *, long,
+                                Arena &);
+
+// plain DHT of columns: X given in double, output in T
+template <class T>
+void dht_cols_from_double(long n, long ncols, const double *X, long ldx, T *Y, long ldy,
+                          Arena &ar) {
+    const FftPlanDev<T> &P = get_plan<T>(n);
+    constexpr int NT = fft_nt<T>();
+    long npairs = (ncols + 1) / 2;
+    if (P.mode == 1) {
+        size_t sh = sizeof(cplx<T>) * (size_t)n;
+        k_dht_single<T, NT><<<(unsigned)npairs, NT, sh, ctx().stream>>>(
+            n, ncols, X, ldx, nullptr, 1.0, P.d1, P.tw1, 1, 0, nullptr, nullptr, nullptr, Y, ldy);
+        SKL_CHECK_LAUNCH();
+    } else {
+        dht_fourstep<T>(P, ncols, X, ldx, nullptr, 1.0, Y, ldy, ar);
+    }
+}
+template void dht_cols_from_double<double>(long, long, const double *, long, double *, long,
+                                           Arena &);
+template void dht_cols_from_double<float>(long, long, const double *, long, float *, long,
+                                          Arena &);
+
+} // namespace skl

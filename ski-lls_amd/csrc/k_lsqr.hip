@@ -1,0 +1,618 @@
+// k_lsqr.hip -- LSQR (src/IterSolver.cpp:160-284) on the preconditioned
+// operator A(:, E[0:r]) M with M = R^{-1}, in FP64 on the device.
+//
+// Each iteration is seven kernels that read A once for A z and once for
+// A^T u (column-major A is streamed with coalesced 16-byte loads), apply M
+// and M^T as triangular GEMVs, and keep every scalar of the Golub-Kahan /
+// Givens recurrences in device memory.  Partial sums are reduced in a fixed
+// order, so a solve is bitwise reproducible run to run.  The iteration
+// sequence is captured once in a hipGraph and replayed until the device-side
+// convergence flag is set; kernels of iterations past convergence exit at
+// entry.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "k_common.h"
+
+namespace skl {
+
+struct LsqrState {
+    double alpha, beta, rhobar, phibar, nrm_a, nrm_r, nrm_ar, coef, tol;
+    long it, max_it;
+    int flag, done;
+};
+
+// ---- A z: column-chunked partials, 2 rows per thread ----------------------
+constexpr int GN_THREADS = 256, GN_ROWS = 2 * GN_THREADS, GN_MAXCW = 512;
+
+// P[c * n + i] = sum_{j in chunk c} A[i, j] z[j]; with upper != 0 only the
+// entries j >= i contribute (A upper triangular, zeros not read).
+__global__ __launch_bounds__(GN_THREADS) void k_gemv_n_part(
+    long n, long d, const double *__restrict__ A, long lda, const double *__restrict__ z, int cw,
+    double *__restrict__ P, int upper, const LsqrState *__restrict__ st) {
+    if (st && st->done) return;
+    __shared__ double zs[GN_MAXCW];
+    const long j0 = (long)blockIdx.y * cw;
+    const long j1 = min(d, j0 + cw);
+    for (long t = threadIdx.x; t < j1 - j0; t += GN_THREADS) zs[t] = z[j0 + t];
+    __syncthreads();
+    const long i = (long)blockIdx.x * GN_ROWS + 2 * threadIdx.x;
+    if (i >= n) return;
+    double a0 = 0.0, a1 = 0.0;
+    long jb = j0;
+    if (upper && jb < i) jb = i; // rows below the diagonal of this chunk see zeros
+    const double *a = A + i + jb * lda;
+    if (i + 1 < n && ((lda & 1) == 0)) {
+        long j = jb;
+        for (; j + 4 <= j1; j += 4) {
+            double2 v0 = *(const double2 *)(a);
+            double2 v1 = *(const double2 *)(a + lda);
+            double2 v2 = *(const double2 *)(a + 2 * lda);
+            double2 v3 = *(const double2 *)(a + 3 * lda);
+            double z0 = zs[j - j0], z1 = zs[j - j0 + 1], z2 = zs[j - j0 + 2], z3 = zs[j - j0 + 3];
+            a0 += v0.x * z0;
+            a1 += v0.y * z0;
+            a0 += v1.x * z1;
+            a1 += v1.y * z1;
+            a0 += v2.x * z2;
+            a1 += v2.y * z2;
+            a0 += v3.x * z3;
+            a1 += v3.y * z3;
+            a += 4 * lda;
+        }
+        for (; j < j1; ++j) {
+            double2 v0 = *(const double2 *)(a);
+            double z0 = zs[j - j0];
+            a0 += v0.x * z0;
+            a1 += v0.y * z0;
+            a += lda;
+        }
+    } else {
+        for (long j = jb; j < j1; ++j) {
+            double zz = zs[j - j0];
+            a0 += a[0] * zz;
+            if (i + 1 < n) a1 += a[1] * zz;
+            a += lda;
+        }
+    }
+    P[(long)blockIdx.y * n + i] = a0;
+    if (i + 1 < n) P[(long)blockIdx.y * n + i + 1] = a1;
+}
+
+// ---- A^T u: row-blocked partials, one wave per column -------------------------
+constexpr int GT_THREADS = 256, GT_RB = 1024;
+
+// P[rb * d + j] = sum_{i in row block rb} A[i, j] u[i]; with upper != 0 only
+// rows i <= j contribute.
+__global__ __launch_bounds__(GT_THREADS) void k_gemv_t_part(
+    long n, long d, const double *__restrict__ A, long lda, const double *__restrict__ u, int cpw,
+    double *__restrict__ P, int upper, const LsqrState *__restrict__ st) {
+    if (st && st->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long i0 = (long)blockIdx.x * GT_RB;
+    double2 uu[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        long ii = i0 + 128 * t + 2 * lane;
+        uu[t].x = (ii < n) ? u[ii] : 0.0;
+        uu[t].y = (ii + 1 < n) ? u[ii + 1] : 0.0;
+    }
+    const bool vec = ((lda & 1) == 0) && (i0 + GT_RB <= n);
+    const long jbeg = (long)blockIdx.y * cpw, jend = min(d, jbeg + cpw);
+    for (long j = jbeg + wave; j < jend; j += 4) {
+        double s = 0.0;
+        if (!(upper && i0 > j)) {
+            const double *a = A + i0 + j * lda;
+            if (vec) {
+                double2 v[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) v[t] = *(const double2 *)(a + 128 * t + 2 * lane);
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    s += v[t].x * uu[t].x;
+                    s += v[t].y * uu[t].y;
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    long ii = i0 + 128 * t + 2 * lane;
+                    if (ii < n) s += a[128 * t + 2 * lane] * uu[t].x;
+                    if (ii + 1 < n) s += a[128 * t + 2 * lane + 1] * uu[t].y;
+                }
+            }
+        }
+        s = wave_sum(s);
+        if (lane == 0) P[(long)blockIdx.x * d + j] = s;
+    }
+}
+
+// ---- reductions / scatter ----------------------------------------------------
+// z[j] = sum_c Pz[c * r + Einv[j]]  (0 when column j is outside E[0:r])
+__global__ void k_z_reduce(long d, long r, int nck, const double *__restrict__ Pz,
+                           const int32_t *__restrict__ Einv, double *__restrict__ z,
+                           const LsqrState *__restrict__ st) {
+    if (st && st->done) return;
+    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d) return;
+    long i = Einv ? (long)Einv[j] : j;
+    double s = 0.0;
+    if (i >= 0 && i < r)
+        for (int c = 0; c < nck; ++c) s += Pz[(long)c * r + i];
+    z[j] = s;
+}
+
+// ut[i] = sum_c Pn[c * n + i] - coef * ut[i];  Sb[block] = sum of squares
+constexpr int RN_PER = 1024;
+__global__ __launch_bounds__(256) void k_reduce_n(long n, int nck, const double *__restrict__ Pn,
+                                                  double *__restrict__ ut, double *__restrict__ Sb,
+                                                  LsqrState *__restrict__ st, int init) {
+    if (st->done) return;
+    const double coef = init ? 0.0 : st->coef;
+    double sq = 0.0;
+    for (int q = 0; q < RN_PER / 256; ++q) {
+        long i = (long)blockIdx.x * RN_PER + q * 256 + threadIdx.x;
+        if (i < n) {
+            double s = 0.0;
+            for (int c = 0; c < nck; ++c) s += Pn[(long)c * n + i];
+            double v = s - coef * ut[i];
+            ut[i] = v;
+            sq += v * v;
+        }
+    }
+    sq = block_sum<256>(sq);
+    if (threadIdx.x == 0) Sb[blockIdx.x] = sq;
+}
+
+// squares of b into Sb (init), ut = b
+__global__ __launch_bounds__(256) void k_init_u(long n, const double *__restrict__ b,
+                                                double *__restrict__ ut, double *__restrict__ Sb) {
+    double sq = 0.0;
+    for (int q = 0; q < RN_PER / 256; ++q) {
+        long i = (long)blockIdx.x * RN_PER + q * 256 + threadIdx.x;
+        if (i < n) {
+            double v = b[i];
+            ut[i] = v;
+            sq += v * v;
+        }
+    }
+    sq = block_sum<256>(sq);
+    if (threadIdx.x == 0) Sb[blockIdx.x] = sq;
+}
+
+__device__ inline double beta_from(const double *Sb, int nsb) {
+    double s = 0.0;
+    for (int q = 0; q < nsb; ++q) s += Sb[q];
+    return sqrt(s);
+}
+
+// t[i] = (sum_rb Pt[rb * d + E[i]]) / beta
+__global__ void k_reduce_t(long d, long r, int nrb, const double *__restrict__ Pt,
+                           const int32_t *__restrict__ E, const double *__restrict__ Sb, int nsb,
+                           double *__restrict__ t, const LsqrState *__restrict__ st) {
+    if (st->done) return;
+    __shared__ double sbeta;
+    if (threadIdx.x == 0) sbeta = beta_from(Sb, nsb);
+    __syncthreads();
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r) return;
+    long j = E ? (long)E[i] : i;
+    double s = 0.0;
+    for (int q = 0; q < nrb; ++q) s += Pt[(long)q * d + j];
+    t[i] = s * (1.0 / sbeta);
+}
+
+// ---- scalar recurrences (one workgroup) ---------------------------------------
+constexpr int UPD_THREADS = 1024;
+
+__global__ __launch_bounds__(UPD_THREADS) void k_update(
+    long r, int nrbm, const double *__restrict__ Pm, const double *__restrict__ Sb, int nsb,
+    double *__restrict__ v, double *__restrict__ w, double *__restrict__ x,
+    LsqrState *__restrict__ st, double *__restrict__ hist, long hist_cap, int init) {
+    if (st->done) return;
+    __shared__ double sh[4];
+    if (threadIdx.x == 0) sh[0] = beta_from(Sb, nsb);
+    __syncthreads();
+    const double beta = sh[0];
+    double alpha_old = init ? 0.0 : st->alpha;
+    double alpha = alpha_old;
+    double nrm_a = init ? 0.0 : st->nrm_a;
+    bool upd = beta > 0;
+    if (upd) {
+        // v = A^T u - beta v ;  alpha = ||v|| ; v /= alpha
+        double sq = 0.0;
+        for (long j = threadIdx.x; j < r; j += UPD_THREADS) {
+            double s = 0.0;
+            for (int q = 0; q < nrbm; ++q) s += Pm[(long)q * r + j];
+            double vv = init ? s : ((-beta) * v[j] + s);
+            v[j] = vv;
+            sq += vv * vv;
+        }
+        sq = block_sum<UPD_THREADS>(sq);
+        alpha = sqrt(sq);
+        if (!init) nrm_a = sqrt(nrm_a * nrm_a + alpha_old * alpha_old + beta * beta);
+        if (alpha > 0) {
+            double sc = 1.0 / alpha;
+            for (long j = threadIdx.x; j < r; j += UPD_THREADS) v[j] = sc * v[j];
+        }
+    } else if (init) {
+        alpha = 0.0;
+        for (long j = threadIdx.x; j < r; j += UPD_THREADS) v[j] = 0.0;
+    }
+    __syncthreads();
+    if (init) {
+        for (long j = threadIdx.x; j < r; j += UPD_THREADS) {
+            w[j] = v[j];
+            x[j] = 0.0;
+        }
+        if (threadIdx.x == 0) {
+            st->alpha = alpha;
+            st->beta = beta;
+            st->rhobar = alpha;
+            st->phibar = beta;
+            st->nrm_a = 0.0;
+            st->nrm_r = beta;
+            st->nrm_ar = alpha * beta;
+            st->coef = (beta > 0) ? alpha / beta : 0.0;
+            st->it = 0;
+            if (alpha * beta == 0) {
+                st->flag = 0;
+                st->done = 1;
+            } else {
+                st->flag = 1;
+                st->done = (st->max_it <= 0) ? 1 : 0;
+            }
+        }
+        return;
+    }
+    double rhobar = st->rhobar, phibar = st->phibar;
+    double rho = sqrt(rhobar * rhobar + beta * beta);
+    double cs = rhobar / rho;
+    double sn = beta / rho;
+    double theta = sn * alpha;
+    rhobar = -cs * alpha;
+    double phi = cs * phibar;
+    phibar = sn * phibar;
+    double c1 = phi / rho, c2 = -theta / rho;
+    for (long j = threadIdx.x; j < r; j += UPD_THREADS) {
+        double wj = w[j];
+        x[j] = x[j] + c1 * wj;
+        w[j] = c2 * wj + 1.0 * v[j];
+    }
+    if (threadIdx.x == 0) {
+        long it = st->it + 1;
+        double nrm_r = phibar;
+        double nrm_ar = phibar * alpha * fabs(cs);
+        st->alpha = alpha;
+        st->beta = beta;
+        st->rhobar = rhobar;
+        st->phibar = phibar;
+        st->nrm_a = nrm_a;
+        st->nrm_r = nrm_r;
+        st->nrm_ar = nrm_ar;
+        st->coef = (beta > 0) ? alpha / beta : 0.0;
+        st->it = it;
+        if (hist && it <= hist_cap) {
+            double *h = hist + 5 * (it - 1);
+            h[0] = alpha;
+            h[1] = beta;
+            h[2] = rhobar;
+            h[3] = phibar;
+            h[4] = nrm_a;
+        }
+        if (nrm_ar < st->tol * nrm_a * nrm_r) {
+            st->flag = 0;
+            st->done = 1;
+        } else if (it >= st->max_it) {
+            st->flag = 1;
+            st->done = 1;
+        }
+    }
+}
+
+// ---- sparse operator kernels ----------------------------------------------------
+// ut[i] = sum_p val[p] z[col[p]] - coef ut[i] over CSR rows; Sb per 1024 rows.
+template <int LPR>
+__global__ __launch_bounds__(256) void k_spmv_n(long n, const int32_t *__restrict__ ptr,
+                                                const int32_t *__restrict__ col,
+                                                const double *__restrict__ val,
+                                                const double *__restrict__ z,
+                                                double *__restrict__ ut, double *__restrict__ Sb,
+                                                LsqrState *__restrict__ st, int init) {
+    if (st->done) return;
+    const double coef = init ? 0.0 : st->coef;
+    const int sub = threadIdx.x % LPR, grp = threadIdx.x / LPR;
+    constexpr int RPI = 256 / LPR; // rows per pass
+    double sq = 0.0;
+    for (int q = 0; q < RN_PER / RPI; ++q) {
+        long i = (long)blockIdx.x * RN_PER + q * RPI + grp;
+        double s = 0.0;
+        if (i < n) {
+            int p0 = ptr[i], p1 = ptr[i + 1];
+            for (int p = p0 + sub; p < p1; p += LPR) s += val[p] * z[col[p]];
+        }
+#pragma unroll
+        for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (sub == 0 && i < n) {
+            double vv = s - coef * ut[i];
+            ut[i] = vv;
+            sq += vv * vv;
+        }
+    }
+    sq = block_sum<256>(sq);
+    if (threadIdx.x == 0) Sb[blockIdx.x] = sq;
+}
+
+// Pt[j] = sum_p val[p] ut[row[p]] over CSC column j (one wave per column)
+__global__ __launch_bounds__(256) void k_csc_t(long d, const int32_t *__restrict__ ptr,
+                                               const int32_t *__restrict__ row,
+                                               const double *__restrict__ val,
+                                               const double *__restrict__ ut,
+                                               double *__restrict__ Pt,
+                                               const LsqrState *__restrict__ st) {
+    if (st && st->done) return;
+    const int lane = threadIdx.x & 63;
+    long j = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= d) return;
+    int p0 = ptr[j], p1 = ptr[j + 1];
+    double s = 0.0;
+    for (int p = p0 + lane; p < p1; p += 64) s += val[p] * ut[row[p]];
+    s = wave_sum(s);
+    if (lane == 0) Pt[j] = s;
+}
+
+// ---- host driver ----------------------------------------------------------------
+struct LsqrPlan {
+    long n, d, r;
+    int cw_a, nck_a;   // A z chunks
+    int cw_m, nck_m;   // M v chunks
+    int nrb_a, cpw_a;  // A^T u row blocks / columns per workgroup
+    int nrb_m, cpw_m;  // M^T t
+    int nsb;           // squares partials
+    bool sparse;
+};
+
+static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
+    LsqrPlan q;
+    q.n = p.n;
+    q.d = p.d;
+    q.r = p.r;
+    q.sparse = (p.csr_ptr != nullptr);
+    long rowblocks = (p.n + GN_ROWS - 1) / GN_ROWS;
+    // enough workgroups to fill the chip several times over
+    long target = 8L * num_cu;
+    long nck = std::max<long>(1, std::min<long>((target + rowblocks - 1) / rowblocks,
+                                                (p.d + 63) / 64));
+    q.cw_a = (int)std::min<long>(GN_MAXCW, (p.d + nck - 1) / nck);
+    q.cw_a = std::max(q.cw_a, 1);
+    q.nck_a = (int)((p.d + q.cw_a - 1) / q.cw_a);
+    long rbm = (p.r + GN_ROWS - 1) / GN_ROWS;
+    long nckm = std::max<long>(1, std::min<long>((target + rbm - 1) / rbm, (p.r + 63) / 64));
+    q.cw_m = (int)std::min<long>(GN_MAXCW, (p.r + nckm - 1) / nckm);
+    q.cw_m = std::max(q.cw_m, 1);
+    q.nck_m = (int)((p.r + q.cw_m - 1) / q.cw_m);
+    q.nrb_a = (int)((p.n + GT_RB - 1) / GT_RB);
+    long cchunks = std::max<long>(1, (target + q.nrb_a - 1) / q.nrb_a);
+    q.cpw_a = (int)std::max<long>(4, (p.d + cchunks - 1) / cchunks);
+    q.nrb_m = (int)((p.r + GT_RB - 1) / GT_RB);
+    long cchm = std::max<long>(1, (target + q.nrb_m - 1) / q.nrb_m);
+    q.cpw_m = (int)std::max<long>(4, (p.r + cchm - 1) / cchm);
+    q.nsb = (int)((p.n + RN_PER - 1) / RN_PER);
+    if (q.sparse) q.nrb_a = 1;
+    return q;
+}
+
+struct LsqrBufs {
+    double *ut, *v, *w, *x, *z, *Pz, *Pn, *Sb, *Pt, *t, *Pm, *hist;
+    long hist_cap;
+    int32_t *Einv;
+    LsqrState *st;
+};
+
+static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs &B,
+                        hipStream_t s, int init) {
+    // Pt = A^T ut
+    if (!q.sparse) {
+        dim3 g((unsigned)q.nrb_a, (unsigned)((q.d + q.cpw_a - 1) / q.cpw_a));
+        k_gemv_t_part<<<g, GT_THREADS, 0, s>>>(q.n, q.d, p.A, p.lda, B.ut, q.cpw_a, B.Pt, 0, B.st);
+    } else {
+        k_csc_t<<<(unsigned)((q.d + 3) / 4), 256, 0, s>>>(q.d, p.csc_ptr, p.csc_row, p.csc_val,
+                                                          B.ut, B.Pt, B.st);
+    }
+    // t = Pt(E) / beta
+    k_reduce_t<<<(unsigned)((q.r + 255) / 256), 256, 0, s>>>(q.d, q.r, q.nrb_a, B.Pt, p.E, B.Sb,
+                                                             q.nsb, B.t, B.st);
+    // Pm = M^T t
+    {
+        dim3 g((unsigned)q.nrb_m, (unsigned)((q.r + q.cpw_m - 1) / q.cpw_m));
+        k_gemv_t_part<<<g, GT_THREADS, 0, s>>>(q.r, q.r, p.M, p.ldm, B.t, q.cpw_m, B.Pm, 1, B.st);
+    }
+    k_update<<<1, UPD_THREADS, 0, s>>>(q.r, q.nrb_m, B.Pm, B.Sb, q.nsb, B.v, B.w, B.x, B.st,
+                                       B.hist, B.hist_cap, init);
+}
+
+static void launch_iteration(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs &B,
+                             hipStream_t s) {
+    // z = scatter_E(M v)
+    {
+        dim3 g((unsigned)((q.r + GN_ROWS - 1) / GN_ROWS), (unsigned)q.nck_m);
+        k_gemv_n_part<<<g, GN_THREADS, 0, s>>>(q.r, q.r, p.M, p.ldm, B.v, q.cw_m, B.Pz, 1, B.st);
+        k_z_reduce<<<(unsigned)((q.d + 255) / 256), 256, 0, s>>>(q.d, q.r, q.nck_m, B.Pz, B.Einv,
+                                                                 B.z, B.st);
+    }
+    // ut = A z - coef ut ; Sb
+    if (!q.sparse) {
+        dim3 g((unsigned)((q.n + GN_ROWS - 1) / GN_ROWS), (unsigned)q.nck_a);
+        k_gemv_n_part<<<g, GN_THREADS, 0, s>>>(q.n, q.d, p.A, p.lda, B.z, q.cw_a, B.Pn, 0, B.st);
+        k_reduce_n<<<(unsigned)q.nsb, 256, 0, s>>>(q.n, q.nck_a, B.Pn, B.ut, B.Sb, B.st, 0);
+    } else {
+        k_spmv_n<8><<<(unsigned)q.nsb, 256, 0, s>>>(q.n, p.csr_ptr, p.csr_col, p.csr_val, B.z, B.ut,
+                                                    B.Sb, B.st, 0);
+    }
+    launch_op_t(p, q, B, s, 0);
+}
+
+__global__ void k_build_einv(long d, long r, const int32_t *E, int32_t *Einv) {
+    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < d) Einv[j] = -1;
+}
+__global__ void k_build_einv2(long r, const int32_t *E, int32_t *Einv) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < r) Einv[E[i]] = (int32_t)i;
+}
+__global__ void k_set_state(LsqrState *st, double tol, long max_it) {
+    st->alpha = st->beta = st->rhobar = st->phibar = 0.0;
+    st->nrm_a = st->nrm_r = st->nrm_ar = st->coef = 0.0;
+    st->tol = tol;
+    st->it = 0;
+    st->max_it = max_it;
+    st->flag = 1;
+    st->done = 0;
+}
+
+// Cached executable graphs, keyed by every pointer and size they bake in.
+struct GraphKey {
+    std::vector<long> k;
+    bool operator<(const GraphKey &o) const { return k < o.k; }
+};
+static thread_local std::map<GraphKey, hipGraphExec_t> g_graphs;
+
+LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long max_it,
+                      double *y, bool use_graph, int chunk, skl_lsqr_step *hist, long hist_len,
+                      Arena &ar) {
+    Context &C = ctx();
+    hipStream_t s = C.stream;
+    LsqrPlan q = make_plan(p, C.num_cu);
+    LsqrBufs B;
+    B.ut = ar.alloc_n<double>((size_t)q.n);
+    B.v = ar.alloc_n<double>((size_t)q.r);
+    B.w = ar.alloc_n<double>((size_t)q.r);
+    B.x = y;
+    B.z = ar.alloc_n<double>((size_t)q.d);
+    B.Pz = ar.alloc_n<double>((size_t)q.nck_m * q.r);
+    B.Pn = q.sparse ? nullptr : ar.alloc_n<double>((size_t)q.nck_a * q.n);
+    B.Sb = ar.alloc_n<double>((size_t)q.nsb);
+    B.Pt = ar.alloc_n<double>((size_t)q.nrb_a * q.d);
+    B.t = ar.alloc_n<double>((size_t)q.r);
+    B.Pm = ar.alloc_n<double>((size_t)q.nrb_m * q.r);
+    B.st = ar.alloc_n<LsqrState>(1);
+    long hl = hist ? std::min(hist_len, max_it) : 0;
+    B.hist = hl > 0 ? ar.alloc_n<double>((size_t)5 * hl) : nullptr;
+    B.hist_cap = hl;
+    B.Einv = nullptr;
+    if (p.E) {
+        B.Einv = ar.alloc_n<int32_t>((size_t)q.d);
+        k_build_einv<<<(unsigned)((q.d + 255) / 256), 256, 0, s>>>(q.d, q.r, p.E, B.Einv);
+        k_build_einv2<<<(unsigned)((q.r + 255) / 256), 256, 0, s>>>(q.r, p.E, B.Einv);
+    }
+    k_set_state<<<1, 1, 0, s>>>(B.st, tol, max_it);
+    SKL_CHECK_LAUNCH();
+
+    // init: ut = b, beta = ||b||, v = M^T A^T u / alpha ...
+    k_init_u<<<(unsigned)q.nsb, 256, 0, s>>>(q.n, b, B.ut, B.Sb);
+    launch_op_t(p, q, B, s, 1);
+    SKL_CHECK_LAUNCH();
+
+    struct HostFlag {
+        int done;
+        long it;
+    };
+    HostFlag *hf = (HostFlag *)C.pinned;
+    if (chunk <= 0) chunk = 16;
+    bool graph_ok = use_graph;
+    hipGraphExec_t exec = nullptr;
+    if (graph_ok) {
+        GraphKey key;
+        key.k = {(long)q.n, (long)q.d, (long)q.r, (long)(intptr_t)p.A, p.lda,
+                 (long)(intptr_t)p.csr_ptr, (long)(intptr_t)p.csc_ptr, (long)(intptr_t)p.E,
+                 (long)(intptr_t)p.M, p.ldm, (long)(intptr_t)B.ut, (long)(intptr_t)B.v,
+                 (long)(intptr_t)B.x, (long)(intptr_t)B.st, (long)(intptr_t)B.hist,
+                 (long)(intptr_t)B.Pn, (long)(intptr_t)B.Pt, (long)(intptr_t)B.Pm,
+                 (long)(intptr_t)B.Einv, chunk, (long)(intptr_t)s};
+        auto it = g_graphs.find(key);
+        if (it != g_graphs.end()) {
+            exec = it->second;
+        } else {
+            hipGraph_t graph;
+            SKL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int c = 0; c < chunk; ++c) launch_iteration(p, q, B, s);
+            hipError_t e1 = hipMemcpyAsync(hf, &B.st->done, sizeof(int), hipMemcpyDeviceToHost, s);
+            hipError_t e2 = hipStreamEndCapture(s, &graph);
+            if (e1 != hipSuccess || e2 != hipSuccess) {
+                graph_ok = false;
+                (void)hipGetLastError();
+            } else {
+                SKL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                SKL_HIP(hipGraphDestroy(graph));
+                g_graphs[key] = exec;
+            }
+        }
+    }
+    // device-side done flag read back once per chunk
+    long launched = 0;
+    while (true) {
+        if (graph_ok) {
+            SKL_HIP(hipGraphLaunch(exec, s));
+        } else {
+            for (int c = 0; c < chunk; ++c) launch_iteration(p, q, B, s);
+            SKL_CHECK_LAUNCH();
+            SKL_HIP(hipMemcpyAsync(hf, &B.st->done, sizeof(int), hipMemcpyDeviceToHost, s));
+        }
+        launched += chunk;
+        SKL_HIP(hipStreamSynchronize(s));
+        if (hf->done) break;
+        if (launched > max_it + chunk) break; // defensive: state always sets done by max_it
+    }
+    LsqrState hs;
+    SKL_HIP(hipMemcpyAsync(&hs, B.st, sizeof hs, hipMemcpyDeviceToHost, s));
+    SKL_HIP(hipStreamSynchronize(s));
+    LsqrResult res;
+    res.flag = hs.flag;
+    res.it = hs.it;
+    res.nrm_ar = hs.nrm_ar;
+    res.nrm_a = hs.nrm_a;
+    res.nrm_r = hs.nrm_r;
+    if (hist && B.hist && hs.it > 0) {
+        long cnt = std::min(hs.it, hl);
+        std::vector<double> h((size_t)(5 * cnt));
+        SKL_HIP(hipMemcpy(h.data(), B.hist, sizeof(double) * 5 * cnt, hipMemcpyDeviceToHost));
+        for (long i = 0; i < cnt; ++i) {
+            hist[i].alpha = h[5 * i];
+            hist[i].beta = h[5 * i + 1];
+            hist[i].rhobar = h[5 * i + 2];
+            hist[i].phibar = h[5 * i + 3];
+            hist[i].nrm_a = h[5 * i + 4];
+        }
+    }
+    return res;
+}
+
+// ---- dense / sparse y = alpha A x + beta y (used for x0 residuals) --------------
+__global__ __launch_bounds__(256) void k_axpby_part(long n, int nck, const double *__restrict__ P,
+                                                    double alpha, double beta,
+                                                    double *__restrict__ y) {
+    long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < nck; ++c) s += P[(long)c * n + i];
+    y[i] = (beta == 0.0 ? 0.0 : beta * y[i]) + alpha * s;
+}
+
+void dev_gemv_n_dense(long n, long d, const double *A, long lda, const double *x, double alpha,
+                      double beta, double *y, Arena &ar) {
+    hipStream_t s = ctx().stream;
+    long rowblocks = (n + GN_ROWS - 1) / GN_ROWS;
+    long target = 8L * ctx().num_cu;
+    long nck = std::max<long>(1, std::min<long>((target + rowblocks - 1) / rowblocks, (d + 63) / 64));
+    int cw = (int)std::max<long>(1, std::min<long>(GN_MAXCW, (d + nck - 1) / nck));
+    int nc = (int)((d + cw - 1) / cw);
+    double *P = ar.alloc_n<double>((size_t)nc * n);
+    dim3 g((unsigned)rowblocks, (unsigned)nc);
+    k_gemv_n_part<<<g, GN_THREADS, 0, s>>>(n, d, A, lda, x, cw, P, 0, nullptr);
+    k_axpby_part<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, nc, P, alpha, beta, y);
+    SKL_CHECK_LAUNCH();
+}
+
+} // namespace skl

@@ -1,0 +1,779 @@
+// k_qr.hip -- Householder QR family of the small sketched matrix SA on gfx950.
+//
+//  * qr_householder: blocked right-looking QR (dgeqrf semantics,
+//    src/dense_qr.cpp:8-33).  Panels are factored by a cooperative kernel in
+//    which each workgroup holds a slice of rows in LDS; the column reductions
+//    of one Householder step (||x||^2 and the products with every other panel
+//    column) travel in a single grid-wide all-reduce, so a column costs one
+//    grid barrier.  Trailing updates are GEMMs on the matrix cores.
+//  * cpqr_greedy: dgeqp3 pivoting (LAPACK 3.9 dlaqp2 downdating) with the
+//    same kernel over the whole matrix; every workgroup keeps a replica of the
+//    partial column norms and makes the identical pivot decision.
+//  * cpqr_hqrrp: HQRRP (hqrrp/lapack_compatible_sources/
+//    NoFLA_HQRRP_WY_blk_var4_skint.c:313-581): block pivots from a Gaussian
+//    sketch Y = G A (column-sliced cooperative kernel), panel QR with pivoting,
+//    GEMM updates and the Y downdate.
+#include <hip/hip_cooperative_groups.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "k_common.h"
+
+namespace cg = cooperative_groups;
+
+namespace skl {
+
+// ---------------------------------------------------------------------------
+// Row-sliced cooperative Householder kernel
+// ---------------------------------------------------------------------------
+template <class T> struct RsArgs {
+    long m;        // rows
+    int nc;        // columns held (all receive the reflectors)
+    int npiv;      // leading columns that are pivot candidates (0 = no pivoting)
+    int nsteps;    // reflectors to compute
+    T *W;
+    long ldw;
+    T *tau;
+    int32_t *jpvt;  // swapped along with pivoted columns (may be null)
+    int32_t *swaps; // pivot column of each step (may be null)
+    int hq_norms;   // 1: HQRRP downdate formula (1+r)(1-r); 0: LAPACK 1 - r^2
+    int rpw;        // rows per workgroup
+    int ldl;        // leading dimension of the LDS slice
+    int two_phase;  // reduce-scatter + all-gather instead of all-to-all
+    T *part;        // [2][P][nc]
+    T *red;         // [2][nc]
+    T *rowv;        // [2][nc]
+    T *rcp;         // [P][nc]
+};
+
+template <class T> __device__ inline T dlapy2_d(T x, T y) {
+    T xa = fabs(x), ya = fabs(y);
+    T w = xa > ya ? xa : ya, z = xa < ya ? xa : ya;
+    if (z == T(0)) return w;
+    T q = z / w;
+    return w * sqrt(T(1) + q * q);
+}
+
+template <class T> __device__ inline T eps_half();
+template <> __device__ inline double eps_half<double>() { return 1.1102230246251565e-16; }
+template <> __device__ inline float eps_half<float>() { return 5.9604644775390625e-08f; }
+
+constexpr int RS_T = 256;
+
+template <class T>
+__global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
+    cg::grid_group grid = cg::this_grid();
+    extern __shared__ unsigned char smem_raw[];
+    const int P = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+    const int nc = a.nc, ldl = a.ldl;
+    T *S = reinterpret_cast<T *>(smem_raw);      // ldl x nc
+    T *Gs = S + (size_t)ldl * nc;                 // nc
+    T *Rs = Gs + nc;                              // nc
+    T *vn1 = Rs + nc;                             // npiv
+    T *vn2 = vn1 + (a.npiv > 0 ? a.npiv : 1);     // npiv
+    T *redbuf = vn2 + (a.npiv > 0 ? a.npiv : 1);  // RS_T
+    int *flags = reinterpret_cast<int *>(redbuf + RS_T); // npiv + 2
+    const long r0 = (long)w * a.rpw;
+    const long r1 = min(a.m, r0 + a.rpw);
+    const int nr = (int)max(0L, r1 - r0);
+    const int npiv = a.npiv;
+    const T tol3z = sqrt(eps_half<T>());
+
+    for (long e = tid; e < (long)nr * nc; e += RS_T) {
+        int i = (int)(e % nr), q = (int)(e / nr);
+        S[i + (size_t)q * ldl] = a.W[(r0 + i) + q * a.ldw];
+    }
+    __syncthreads();
+
+    // all-reduce of vals[q] for q in [q0, nc) (q0 relative), result into Gs
+    auto allreduce = [&](int buf, int q0) {
+        T *pw = a.part + ((size_t)buf * P + w) * nc;
+        // values already in pw[q] for q >= q0
+        __threadfence();
+        grid.sync();
+        if (!a.two_phase) {
+            for (int q = q0 + tid; q < nc; q += RS_T) {
+                T s = 0;
+                for (int u = 0; u < P; ++u) s += a.part[((size_t)buf * P + u) * nc + q];
+                Gs[q] = s;
+            }
+        } else {
+            int ncol = nc - q0;
+            int per = (ncol + P - 1) / P;
+            int qb = q0 + w * per, qe = min(nc, qb + per);
+            for (int q = qb + tid; q < qe; q += RS_T) {
+                T s = 0;
+                for (int u = 0; u < P; ++u) s += a.part[((size_t)buf * P + u) * nc + q];
+                a.red[(size_t)buf * nc + q] = s;
+            }
+            __threadfence();
+            grid.sync();
+            for (int q = q0 + tid; q < nc; q += RS_T) Gs[q] = a.red[(size_t)buf * nc + q];
+        }
+        (void)pw;
+    };
+
+    // initial column norms for pivoting (over all rows)
+    if (npiv > 0) {
+        T *pw = a.part + ((size_t)1 * P + w) * nc;
+        for (int q = tid; q < nc; q += RS_T) {
+            T s = 0;
+            for (int i = 0; i < nr; ++i) {
+                T v = S[i + (size_t)q * ldl];
+                s += v * v;
+            }
+            pw[q] = s;
+        }
+        allreduce(1, 0);
+        __syncthreads();
+        for (int q = tid; q < npiv; q += RS_T) {
+            vn1[q] = sqrt(Gs[q]);
+            vn2[q] = vn1[q];
+        }
+        __syncthreads();
+    }
+
+    for (int jj = 0; jj < a.nsteps; ++jj) {
+        const int buf = jj & 1;
+        if (npiv > 0) {
+            // first index of max |vn1[jj..npiv)| (idamax), identical in all workgroups
+            if (tid == 0) {
+                int best = jj;
+                T bv = fabs(vn1[jj]);
+                for (int q = jj + 1; q < npiv; ++q)
+                    if (fabs(vn1[q]) > bv) {
+                        bv = fabs(vn1[q]);
+                        best = q;
+                    }
+                flags[npiv] = best;
+            }
+            __syncthreads();
+            const int pvt = flags[npiv];
+            if (pvt != jj) {
+                for (int i = tid; i < nr; i += RS_T) {
+                    T t0 = S[i + (size_t)jj * ldl];
+                    S[i + (size_t)jj * ldl] = S[i + (size_t)pvt * ldl];
+                    S[i + (size_t)pvt * ldl] = t0;
+                }
+                if (tid == 0) {
+                    vn1[pvt] = vn1[jj];
+                    vn2[pvt] = vn2[jj];
+                    if (w == 0 && a.jpvt) {
+                        int32_t t1 = a.jpvt[pvt];
+                        a.jpvt[pvt] = a.jpvt[jj];
+                        a.jpvt[jj] = t1;
+                    }
+                }
+            }
+            if (w == 0 && tid == 0 && a.swaps) a.swaps[jj] = pvt;
+            __syncthreads();
+        }
+        // partial Gram row: g[q] = sum_{rows > jj} S(:, jj) S(:, q), q >= jj
+        T *pw = a.part + ((size_t)buf * P + w) * nc;
+        const int ib = (int)max(0L, (long)jj + 1 - r0); // first local row below the diagonal
+        for (int q = jj + tid; q < nc; q += RS_T) {
+            T s = 0;
+            for (int i = ib; i < nr; ++i) s += S[i + (size_t)jj * ldl] * S[i + (size_t)q * ldl];
+            pw[q] = s;
+        }
+        const bool owner = (jj >= r0 && jj < r1);
+        if (owner)
+            for (int q = jj + tid; q < nc; q += RS_T)
+                a.rowv[(size_t)buf * nc + q] = S[(jj - r0) + (size_t)q * ldl];
+        allreduce(buf, jj);
+        for (int q = jj + tid; q < nc; q += RS_T) Rs[q] = a.rowv[(size_t)buf * nc + q];
+        __syncthreads();
+        // Householder scalars (dlarfg), computed redundantly
+        const T alpha = Rs[jj];
+        const T xnorm = sqrt(Gs[jj]);
+        T tau = 0, beta = alpha, sc = 0;
+        if (xnorm != T(0)) {
+            beta = -copysign(dlapy2_d(alpha, xnorm), alpha);
+            tau = (beta - alpha) / beta;
+            sc = T(1) / (alpha - beta);
+        }
+        if (tau != T(0)) {
+            // w_q = v^T a_q = a(jj, q) + sc * g_q ; C -= tau v w^T
+            for (int q = jj + 1 + tid; q < nc; q += RS_T) Gs[q] = -tau * (Rs[q] + sc * Gs[q]);
+            __syncthreads();
+            const int nq = nc - jj - 1;
+            const int nrows = nr - ib;
+            for (long e = tid; e < (long)nrows * nq; e += RS_T) {
+                int i = ib + (int)(e % nrows), q = jj + 1 + (int)(e / nrows);
+                T vi = sc * S[i + (size_t)jj * ldl];
+                S[i + (size_t)q * ldl] = S[i + (size_t)q * ldl] + vi * Gs[q];
+            }
+            if (owner)
+                for (int q = jj + 1 + tid; q < nc; q += RS_T)
+                    S[(jj - r0) + (size_t)q * ldl] = Rs[q] + Gs[q];
+            __syncthreads();
+            for (int i = ib + tid; i < nr; i += RS_T) S[i + (size_t)jj * ldl] *= sc;
+            if (owner && tid == 0) S[(jj - r0) + (size_t)jj * ldl] = beta;
+        } else {
+            for (int q = jj + 1 + tid; q < nc; q += RS_T) Gs[q] = T(0);
+        }
+        if (w == 0 && tid == 0) a.tau[jj] = tau;
+        __syncthreads();
+        if (npiv > 0) {
+            // partial norm downdating with the new row jj of R
+            int any = 0;
+            for (int q = jj + 1 + tid; q < npiv; q += RS_T) {
+                int fl = 0;
+                if (vn1[q] != T(0)) {
+                    T rn = Rs[q] + Gs[q];
+                    T temp = fabs(rn) / vn1[q];
+                    if (a.hq_norms)
+                        temp = (T(1) + temp) * (T(1) - temp);
+                    else
+                        temp = T(1) - temp * temp;
+                    if (temp < T(0)) temp = T(0);
+                    T q5 = vn1[q] / vn2[q];
+                    T temp2 = temp * q5 * q5;
+                    if (temp2 <= tol3z)
+                        fl = 1;
+                    else
+                        vn1[q] = vn1[q] * sqrt(temp);
+                }
+                flags[q] = fl;
+                any |= fl;
+            }
+            if (tid == 0) flags[npiv + 1] = 0;
+            __syncthreads();
+            if (any) atomicOr(&flags[npiv + 1], 1);
+            __syncthreads();
+            if (flags[npiv + 1]) {
+                // exact norms of the flagged columns over the rows below jj
+                T *rw = a.rcp + (size_t)w * nc;
+                const int ib2 = (int)max(0L, (long)jj + 1 - r0);
+                for (int q = jj + 1 + tid; q < npiv; q += RS_T) {
+                    T s = 0;
+                    if (flags[q])
+                        for (int i = ib2; i < nr; ++i) {
+                            T v = S[i + (size_t)q * ldl];
+                            s += v * v;
+                        }
+                    rw[q] = s;
+                }
+                __threadfence();
+                grid.sync();
+                for (int q = jj + 1 + tid; q < npiv; q += RS_T) {
+                    if (!flags[q]) continue;
+                    if (jj < a.m - 1) {
+                        T s = 0;
+                        for (int u = 0; u < P; ++u) s += a.rcp[(size_t)u * nc + q];
+                        vn1[q] = sqrt(s);
+                        vn2[q] = vn1[q];
+                    } else {
+                        vn1[q] = T(0);
+                        vn2[q] = T(0);
+                    }
+                }
+                __syncthreads();
+                // rcp is rewritten only after another grid barrier
+            }
+        }
+        __syncthreads();
+    }
+    for (long e = tid; e < (long)nr * nc; e += RS_T) {
+        int i = (int)(e % nr), q = (int)(e / nr);
+        a.W[(r0 + i) + q * a.ldw] = S[i + (size_t)q * ldl];
+    }
+}
+
+template <class T>
+static void launch_coop(void (*fn)(RsArgs<T>), int P, size_t sh, RsArgs<T> &args) {
+    void *kargs[] = {(void *)&args};
+    if (sh > 65536)
+        SKL_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)sh));
+    SKL_HIP(hipLaunchCooperativeKernel((const void *)fn, dim3((unsigned)P), dim3(RS_T), kargs,
+                                       (unsigned)sh, ctx().stream));
+}
+
+static constexpr size_t RS_LDS_BUDGET = 120 * 1024;
+
+// Factor (or pivot-factor) the m x nc block at W: nsteps reflectors, pivoting
+// over the first npiv columns when npiv > 0.
+template <class T>
+static void rowslice_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T *tau,
+                        int32_t *jpvt, int32_t *swaps, int hq_norms, Arena &ar) {
+    if (nsteps <= 0) return;
+    Context &C = ctx();
+    int P_max = std::max(1, C.num_cu);
+    size_t fixed = sizeof(T) * (2 * (size_t)nc + 2 * (size_t)std::max(npiv, 1) + RS_T) +
+                   sizeof(int) * ((size_t)npiv + 4) + 64;
+    // smallest P whose slice fits, and no more rows per slice than needed
+    long rpw = -1;
+    int P = 0;
+    for (int p = 1; p <= P_max; ++p) {
+        long r = (m + p - 1) / p;
+        long ld = r | 1;
+        size_t need = sizeof(T) * (size_t)ld * nc + fixed;
+        if (need <= RS_LDS_BUDGET) {
+            rpw = r;
+            P = p;
+            break;
+        }
+    }
+    SKL_REQUIRE(rpw > 0, SKL_E_UNSUPPORTED, "QR panel too large for the LDS-resident kernel");
+    // more workgroups shorten each column step when the slices are tall
+    long target_rows = 128;
+    if (rpw > target_rows) {
+        int p2 = (int)std::min<long>(P_max, (m + target_rows - 1) / target_rows);
+        if (p2 > P) {
+            P = p2;
+            rpw = (m + P - 1) / P;
+        }
+    }
+    P = (int)((m + rpw - 1) / rpw);
+    RsArgs<T> a;
+    a.m = m;
+    a.nc = nc;
+    a.npiv = npiv;
+    a.nsteps = nsteps;
+    a.W = W;
+    a.ldw = ldw;
+    a.tau = tau;
+    a.jpvt = jpvt;
+    a.swaps = swaps;
+    a.hq_norms = hq_norms;
+    a.rpw = (int)rpw;
+    a.ldl = (int)(rpw | 1);
+    a.two_phase = ((long)P * nc > 32768) ? 1 : 0;
+    a.part = ar.alloc_n<T>((size_t)2 * P * nc);
+    a.red = ar.alloc_n<T>((size_t)2 * nc);
+    a.rowv = ar.alloc_n<T>((size_t)2 * nc);
+    a.rcp = ar.alloc_n<T>((size_t)P * nc);
+    size_t sh = sizeof(T) * (size_t)a.ldl * nc + fixed;
+    launch_coop<T>(k_qr_rowslice<T>, P, sh, a);
+}
+
+// ---------------------------------------------------------------------------
+// Block reflector helpers
+// ---------------------------------------------------------------------------
+// Vf (mp x b) = unit lower trapezoid of the factored panel
+template <class T>
+__global__ void k_vfull(long mp, int b, const T *W, long ldw, T *Vf) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int q = blockIdx.y;
+    if (i >= mp) return;
+    T v = (i > q) ? W[i + (long)q * ldw] : (i == q ? T(1) : T(0));
+    Vf[i + (long)q * mp] = v;
+}
+
+// T (b x b, ld b) from VtV and tau (dlarft forward columnwise)
+template <class T>
+__global__ __launch_bounds__(256) void k_build_t(int b, const T *VtV, const T *tau, T *Tm) {
+    __shared__ T tmp[256];
+    for (int e = threadIdx.x; e < b * b; e += 256) Tm[e] = T(0);
+    __syncthreads();
+    for (int i = 0; i < b; ++i) {
+        T ti = tau[i];
+        for (int r = threadIdx.x; r < i; r += 256) tmp[r] = -ti * VtV[r + i * b];
+        __syncthreads();
+        for (int r = threadIdx.x; r < i; r += 256) {
+            T s = 0;
+            for (int q = r; q < i; ++q) s += Tm[r + q * b] * tmp[q];
+            Tm[r + i * b] = s;
+        }
+        if (threadIdx.x == 0) Tm[i + i * b] = ti;
+        __syncthreads();
+    }
+}
+
+// C (mp x nc2) := (I - V T V^T)^T C
+template <class T>
+static void apply_block_left_t(long mp, int b, const T *Vf, const T *Tm, long nc2, T *Cm, long ldc,
+                               Arena &ar) {
+    if (nc2 <= 0 || b <= 0) return;
+    T *Wt = ar.alloc_n<T>((size_t)b * nc2);
+    T *Wt2 = ar.alloc_n<T>((size_t)b * nc2);
+    gemm<T>('T', 'N', b, nc2, mp, T(1), Vf, mp, Cm, ldc, T(0), Wt, b);
+    gemm<T>('T', 'N', b, nc2, b, T(1), Tm, b, Wt, b, T(0), Wt2, b);
+    gemm<T>('N', 'N', mp, nc2, b, T(-1), Vf, mp, Wt2, b, T(1), Cm, ldc);
+}
+
+template <class T>
+static void panel_t(long mp, int b, const T *Wp, long ldw, const T *tau, T *Vf, T *Tm, Arena &ar) {
+    dim3 g((unsigned)((mp + 255) / 256), (unsigned)b);
+    k_vfull<T><<<g, 256, 0, ctx().stream>>>(mp, b, Wp, ldw, Vf);
+    T *VtV = ar.alloc_n<T>((size_t)b * b);
+    gemm<T>('T', 'N', b, b, mp, T(1), Vf, mp, Vf, mp, T(0), VtV, b);
+    k_build_t<T><<<1, 256, 0, ctx().stream>>>(b, VtV, tau, Tm);
+    SKL_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------
+// Blocked Householder QR: factor the first nfac columns, apply Q^T to the rest
+// ---------------------------------------------------------------------------
+template <class T>
+void qr_householder(long m, long ncol, long nfac, T *W, long ldw, T *tau, Arena &ar) {
+    long kmax = std::min(m, nfac);
+    const int nb = (nfac >= 256) ? 64 : 32;
+    size_t mark = ar.used();
+    (void)mark;
+    T *Vf = ar.alloc_n<T>((size_t)m * nb);
+    T *Tm = ar.alloc_n<T>((size_t)nb * nb);
+    for (long j = 0; j < kmax; j += nb) {
+        int b = (int)std::min<long>(nb, kmax - j);
+        long mp = m - j;
+        rowslice_qr<T>(mp, b, 0, b, W + j + j * ldw, ldw, tau + j, nullptr, nullptr, 0, ar);
+        long nc2 = ncol - j - b;
+        if (nc2 > 0) {
+            panel_t<T>(mp, b, W + j + j * ldw, ldw, tau + j, Vf, Tm, ar);
+            apply_block_left_t<T>(mp, b, Vf, Tm, nc2, W + j + (j + b) * ldw, ldw, ar);
+        }
+    }
+}
+template void qr_householder<double>(long, long, long, double *, long, double *, Arena &);
+template void qr_householder<float>(long, long, long, float *, long, float *, Arena &);
+
+// ---------------------------------------------------------------------------
+// Greedy CPQR (dgeqp3) over the first nfac columns; columns past nfac are
+// carried along (they receive Q^T, never pivot).
+// ---------------------------------------------------------------------------
+template <class T>
+void cpqr_greedy(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *tau, Arena &ar) {
+    long kmax = std::min(m, nfac);
+    rowslice_qr<T>(m, (int)ncol, (int)nfac, (int)kmax, W, ldw, tau, jpvt, nullptr, 0, ar);
+}
+template void cpqr_greedy<double>(long, long, long, double *, long, int32_t *, double *, Arena &);
+template void cpqr_greedy<float>(long, long, long, float *, long, int32_t *, float *, Arena &);
+
+// ---------------------------------------------------------------------------
+// HQRRP pieces
+// ---------------------------------------------------------------------------
+constexpr int YQ_T = 256;
+
+// Column-sliced cooperative QRP of V (mY x nv, ld mY) for nsteps pivots; the
+// pivot order is returned as a logical -> physical permutation plus the list
+// of positions that changed.  Uses the HQRRP norm downdating
+// (skint.c:920-990); recomputed norms are local to a column's owner.
+template <class T> struct YqArgs {
+    int mY, nv, nsteps, cpw;
+    T *V;
+    int32_t *perm_out, *changed, *nchanged;
+    T *cand;      // [2][P]
+    int32_t *candi; // [2][P]
+};
+
+template <class T>
+__global__ __launch_bounds__(YQ_T) void k_yqrp(YqArgs<T> a) {
+    cg::grid_group grid = cg::this_grid();
+    extern __shared__ unsigned char smem_raw[];
+    const int P = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int nv = a.nv, mY = a.mY;
+    int32_t *perm = reinterpret_cast<int32_t *>(smem_raw); // nv
+    int32_t *pos = perm + nv;                              // nv
+    T *dn = reinterpret_cast<T *>(pos + nv + (nv & 1));    // cpw
+    T *en = dn + a.cpw;                                    // cpw
+    T *vv = en + a.cpw;                                    // mY
+    __shared__ T s_best;
+    __shared__ int s_bpos;
+    __shared__ T s_hh[3];
+    const int c0 = w * a.cpw, c1 = min(nv, c0 + a.cpw);
+    const T tol3z = sqrt(eps_half<T>());
+    for (int t = tid; t < nv; t += YQ_T) {
+        perm[t] = t;
+        pos[t] = t;
+    }
+    // initial norms of own columns
+    for (int c = c0 + wave; c < c1; c += YQ_T / 64) {
+        T s = 0;
+        for (int i = lane; i < mY; i += 64) {
+            T v = a.V[i + (long)c * mY];
+            s += v * v;
+        }
+        s = wave_sum(s);
+        if (lane == 0) {
+            dn[c - c0] = sqrt(s);
+            en[c - c0] = dn[c - c0];
+        }
+    }
+    __syncthreads();
+    for (int jj = 0; jj < a.nsteps; ++jj) {
+        const int buf = jj & 1;
+        if (tid == 0) {
+            T bv = T(-1);
+            int bp = 0x7fffffff;
+            for (int c = c0; c < c1; ++c) {
+                int p = pos[c];
+                if (p < jj) continue;
+                T v = fabs(dn[c - c0]);
+                if (v > bv || (v == bv && p < bp)) {
+                    bv = v;
+                    bp = p;
+                }
+            }
+            a.cand[buf * P + w] = bv;
+            a.candi[buf * P + w] = bp;
+        }
+        __threadfence();
+        grid.sync();
+        if (tid == 0) {
+            T bv = T(-1);
+            int bp = 0x7fffffff;
+            for (int u = 0; u < P; ++u) {
+                T v = a.cand[buf * P + u];
+                int p = a.candi[buf * P + u];
+                if (v > bv || (v == bv && p < bp)) {
+                    bv = v;
+                    bp = p;
+                }
+            }
+            s_bpos = bp;
+        }
+        __syncthreads();
+        const int pl = s_bpos;
+        if (tid == 0) {
+            int a0 = perm[jj], a1 = perm[pl];
+            perm[jj] = a1;
+            perm[pl] = a0;
+            pos[a1] = jj;
+            pos[a0] = pl;
+        }
+        __syncthreads();
+        const int pc = perm[jj];
+        // Householder vector from the pivot column (rows jj..mY-1)
+        if (wave == 0) {
+            T xs = 0;
+            for (int i = jj + 1 + lane; i < mY; i += 64) {
+                T v = a.V[i + (long)pc * mY];
+                xs += v * v;
+            }
+            xs = wave_sum(xs);
+            if (lane == 0) {
+                T alpha = a.V[jj + (long)pc * mY];
+                T xnorm = sqrt(xs);
+                T tau = 0, sc = 0;
+                if (xnorm != T(0)) {
+                    T beta = -copysign(dlapy2_d(alpha, xnorm), alpha);
+                    tau = (beta - alpha) / beta;
+                    sc = T(1) / (alpha - beta);
+                }
+                s_hh[0] = tau;
+                s_hh[1] = sc;
+            }
+        }
+        __syncthreads();
+        const T tau = s_hh[0], sc = s_hh[1];
+        for (int i = jj + 1 + tid; i < mY; i += YQ_T) vv[i] = sc * a.V[i + (long)pc * mY];
+        if (tid == 0) vv[jj] = T(1);
+        __syncthreads();
+        // apply to own columns with logical position > jj; downdate their norms
+        for (int c = c0 + wave; c < c1; c += YQ_T / 64) {
+            if (pos[c] <= jj) continue;
+            T *col = a.V + (long)c * mY;
+            if (tau != T(0)) {
+                T s = 0;
+                for (int i = jj + lane; i < mY; i += 64) s += vv[i] * col[i];
+                s = wave_sum(s);
+                T f = -tau * s;
+                for (int i = jj + lane; i < mY; i += 64) col[i] = col[i] + vv[i] * f;
+            }
+            // norm downdate (all lanes read the same values)
+            T d0 = dn[c - c0];
+            T nd = d0, ne = en[c - c0];
+            bool recompute = false;
+            if (d0 != T(0)) {
+                T temp = fabs(col[jj]) / d0;
+                temp = (T(1) + temp) * (T(1) - temp);
+                if (temp < T(0)) temp = T(0);
+                T t5 = d0 / ne;
+                T temp2 = temp * t5 * t5;
+                if (temp2 <= tol3z)
+                    recompute = true;
+                else
+                    nd = d0 * sqrt(temp);
+            }
+            if (recompute) {
+                T s = 0;
+                for (int i = jj + 1 + lane; i < mY; i += 64) s += col[i] * col[i];
+                s = wave_sum(s);
+                nd = (mY - jj - 1 > 0) ? sqrt(s) : T(0);
+                ne = nd;
+            }
+            if (lane == 0) {
+                dn[c - c0] = nd;
+                en[c - c0] = ne;
+            }
+        }
+        __syncthreads();
+    }
+    if (w == 0) {
+        for (int t = tid; t < nv; t += YQ_T) a.perm_out[t] = perm[t];
+        if (tid == 0) {
+            int cnt = 0;
+            for (int t = 0; t < nv; ++t)
+                if (perm[t] != t) a.changed[cnt++] = t;
+            *a.nchanged = cnt;
+        }
+    }
+}
+
+// gather/scatter of the changed columns: Xout[:, t] = Xin[:, perm[t]]
+template <class T>
+__global__ void k_perm_gather(long rows, const T *X, long ldx, const int32_t *perm,
+                              const int32_t *changed, const int32_t *nch, T *tmp) {
+    int k = blockIdx.y;
+    if (k >= *nch) return;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    tmp[i + (long)k * rows] = X[i + (long)perm[changed[k]] * ldx];
+}
+template <class T>
+__global__ void k_perm_scatter(long rows, T *X, long ldx, const int32_t *changed,
+                               const int32_t *nch, const T *tmp) {
+    int k = blockIdx.y;
+    if (k >= *nch) return;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    X[i + (long)changed[k] * ldx] = tmp[i + (long)k * rows];
+}
+__global__ void k_perm_int(int nv, int32_t *jp, const int32_t *perm, const int32_t *changed,
+                           const int32_t *nch) {
+    // single block: gather then scatter through shared memory
+    extern __shared__ int32_t sj[];
+    int n = *nch;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) sj[k] = jp[perm[changed[k]]];
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) jp[changed[k]] = sj[k];
+}
+
+// apply the panel's column swaps (in order) to extra row blocks
+template <class T>
+__global__ __launch_bounds__(256) void k_apply_swaps(int b, const int32_t *swaps, long r1, T *X1,
+                                                     long ld1, long r2, T *X2, long ld2) {
+    for (int s = 0; s < b; ++s) {
+        int p = swaps[s];
+        if (p != s) {
+            for (long i = threadIdx.x; i < r1; i += 256) {
+                T t = X1[i + (long)s * ld1];
+                X1[i + (long)s * ld1] = X1[i + (long)p * ld1];
+                X1[i + (long)p * ld1] = t;
+            }
+            for (long i = threadIdx.x; i < r2; i += 256) {
+                T t = X2[i + (long)s * ld2];
+                X2[i + (long)s * ld2] = X2[i + (long)p * ld2];
+                X2[i + (long)p * ld2] = t;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// B = G1 - BT U11^T  (b x b unit lower U11 = Vf top rows); mY x b
+template <class T>
+__global__ void k_g1_minus(int mY, int b, const T *G1, long ldg, const T *BT, const T *Vf,
+                           long mp, T *B) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = blockIdx.y;
+    if (i >= mY) return;
+    // (BT U11^T)(i, r) = sum_{q <= r} BT(i, q) U11(r, q), U11(r, r) = 1
+    T s = BT[i + (long)r * mY];
+    for (int q = 0; q < r; ++q) s += BT[i + (long)q * mY] * Vf[r + (long)q * mp];
+    B[i + (long)r * mY] = G1[i + (long)r * ldg] - s;
+}
+
+template <class T>
+void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *tau,
+                const double *G_host, Arena &ar) {
+    const int nb = 64, pp = 10, mY = nb + pp;
+    hipStream_t s = ctx().stream;
+    long n = nfac;
+    long mn = std::min(m, n);
+    // G (mY x m) and Y = G W(:, 0:n)
+    T *G = ar.alloc_n<T>((size_t)mY * m);
+    {
+        std::vector<T> gh((size_t)mY * m);
+        for (size_t i = 0; i < gh.size(); ++i) gh[i] = (T)G_host[i];
+        SKL_HIP(hipMemcpyAsync(G, gh.data(), sizeof(T) * gh.size(), hipMemcpyHostToDevice, s));
+        SKL_HIP(hipStreamSynchronize(s));
+    }
+    T *Y = ar.alloc_n<T>((size_t)mY * n);
+    gemm<T>('N', 'N', mY, n, m, T(1), G, mY, W, ldw, T(0), Y, mY);
+    T *V = ar.alloc_n<T>((size_t)mY * n);
+    T *Vf = ar.alloc_n<T>((size_t)m * nb);
+    T *Tm = ar.alloc_n<T>((size_t)nb * nb);
+    int32_t *perm = ar.alloc_n<int32_t>((size_t)n);
+    int32_t *changed = ar.alloc_n<int32_t>((size_t)n);
+    int32_t *nch = ar.alloc_n<int32_t>(4);
+    int32_t *swaps = ar.alloc_n<int32_t>(nb);
+    T *tmpc = ar.alloc_n<T>((size_t)std::max<long>(m, mY) * 2 * nb);
+    T *B0 = ar.alloc_n<T>((size_t)mY * nb);
+    T *BT = ar.alloc_n<T>((size_t)mY * nb);
+    T *B = ar.alloc_n<T>((size_t)mY * nb);
+    const int P_max = std::max(1, ctx().num_cu);
+    T *cand = ar.alloc_n<T>((size_t)2 * P_max);
+    int32_t *candi = ar.alloc_n<int32_t>((size_t)2 * P_max);
+    for (long j = 0; j < mn; j += nb) {
+        int b = (int)std::min<long>(nb, std::min(n - j, m - j));
+        bool last = (j + nb >= m) || (j + nb >= n);
+        long nv = n - j;
+        if (!last) {
+            // pivot block from Y(:, j:n)
+            SKL_HIP(hipMemcpyAsync(V, Y + j * mY, sizeof(T) * (size_t)mY * nv,
+                                   hipMemcpyDeviceToDevice, s));
+            YqArgs<T> ya;
+            ya.mY = mY;
+            ya.nv = (int)nv;
+            ya.nsteps = b;
+            int cpw = (int)std::max<long>(4, (nv + P_max - 1) / P_max);
+            ya.cpw = cpw;
+            ya.V = V;
+            ya.perm_out = perm;
+            ya.changed = changed;
+            ya.nchanged = nch;
+            ya.cand = cand;
+            ya.candi = candi;
+            int P = (int)((nv + cpw - 1) / cpw);
+            size_t sh = sizeof(int32_t) * (2 * (size_t)nv + 2) + sizeof(T) * (2 * (size_t)cpw + mY) + 64;
+            void *kargs[] = {(void *)&ya};
+            if (sh > 65536)
+                SKL_HIP(hipFuncSetAttribute((const void *)k_yqrp<T>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+            SKL_HIP(hipLaunchCooperativeKernel((const void *)k_yqrp<T>, dim3((unsigned)P),
+                                               dim3(YQ_T), kargs, (unsigned)sh, s));
+            // permute W(:, j:n), Y(:, j:n), jpvt(j:n) by the new column order
+            dim3 gw((unsigned)((m + 255) / 256), (unsigned)(2 * b));
+            k_perm_gather<T><<<gw, 256, 0, s>>>(m, W + j * ldw, ldw, perm, changed, nch, tmpc);
+            k_perm_scatter<T><<<gw, 256, 0, s>>>(m, W + j * ldw, ldw, changed, nch, tmpc);
+            dim3 gy((unsigned)((mY + 255) / 256), (unsigned)(2 * b));
+            k_perm_gather<T><<<gy, 256, 0, s>>>(mY, Y + j * mY, mY, perm, changed, nch, tmpc);
+            k_perm_scatter<T><<<gy, 256, 0, s>>>(mY, Y + j * mY, mY, changed, nch, tmpc);
+            k_perm_int<<<1, 256, sizeof(int32_t) * 2 * b, s>>>((int)nv, jpvt + j, perm, changed, nch);
+            SKL_CHECK_LAUNCH();
+        }
+        // panel QRP of W(j:m, j:j+b), pivoting W(0:j, j:j+b) and Y(:, j:j+b)
+        long mp = m - j;
+        rowslice_qr<T>(mp, b, b, b, W + j + j * ldw, ldw, tau + j, jpvt + j, swaps, 1, ar);
+        k_apply_swaps<T><<<1, 256, 0, s>>>(b, swaps, j, W + j * ldw, ldw, mY, Y + j * mY, mY);
+        SKL_CHECK_LAUNCH();
+        panel_t<T>(mp, b, W + j + j * ldw, ldw, tau + j, Vf, Tm, ar);
+        long nc2 = ncol - j - b;
+        if (nc2 > 0) apply_block_left_t<T>(mp, b, Vf, Tm, nc2, W + j + (j + b) * ldw, ldw, ar);
+        if (!last) {
+            // Y2 -= (G1 - (G V) T U11^T) R12 ; G(:, j:m) -= (G V) T V^T
+            T *G1 = G + j * mY;
+            gemm<T>('N', 'N', mY, b, mp, T(1), G1, mY, Vf, mp, T(0), B0, mY);
+            gemm<T>('N', 'N', mY, b, b, T(1), B0, mY, Tm, nb, T(0), BT, mY);
+            dim3 gb((unsigned)((mY + 255) / 256), (unsigned)b);
+            k_g1_minus<T><<<gb, 256, 0, s>>>(mY, b, G1, mY, BT, Vf, mp, B);
+            SKL_CHECK_LAUNCH();
+            long n2 = n - j - b;
+            if (n2 > 0)
+                gemm<T>('N', 'N', mY, n2, b, T(-1), B, mY, W + j + (j + b) * ldw, ldw, T(1),
+                        Y + (j + b) * mY, mY);
+            gemm<T>('N', 'T', mY, mp, b, T(-1), BT, mY, Vf, mp, T(1), G1, mY);
+        }
+    }
+}
+template void cpqr_hqrrp<double>(long, long, long, double *, long, int32_t *, double *,
+                                 const double *, Arena &);
+template void cpqr_hqrrp<float>(long, long, long, float *, long, int32_t *, float *,
+                                const double *, Arena &);
+
+} // namespace skl

@@ -1,0 +1,214 @@
+// runtime.cpp -- context, device arena, errors and the host-side hashing
+// pattern generator of the MI355X least-squares library.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "skl_internal.h"
+
+namespace skl {
+
+[[noreturn]] void throw_hip(hipError_t e, const char *what, const char *file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e,
+                  hipGetErrorString(e), what, file, line);
+    int code = (e == hipErrorOutOfMemory) ? SKL_E_NOMEM : SKL_E_HIP;
+    throw Error(code, buf);
+}
+
+double wall() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------------------
+Arena::~Arena() { release(); }
+
+void Arena::release() {
+    if (base_) (void)hipFree(base_);
+    for (void *p : overflow_) (void)hipFree(p);
+    overflow_.clear();
+    base_ = nullptr;
+    cap_ = used_ = 0;
+}
+
+void Arena::reserve(size_t bytes) {
+    if (bytes <= cap_) return;
+    if (used_ != 0) return; // cannot move live buffers; alloc() falls back
+    if (base_) SKL_HIP(hipFree(base_));
+    base_ = nullptr;
+    cap_ = 0;
+    for (void *p : overflow_) (void)hipFree(p);
+    overflow_.clear();
+    size_t want = bytes + (bytes >> 3);
+    SKL_HIP(hipMalloc(&base_, want));
+    cap_ = want;
+}
+
+void *Arena::alloc(size_t bytes) {
+    size_t sz = (bytes + 255) & ~size_t(255);
+    if (sz == 0) sz = 256;
+    if (used_ + sz <= cap_) {
+        void *p = base_ + used_;
+        used_ += sz;
+        return p;
+    }
+    void *p = nullptr;
+    SKL_HIP(hipMalloc(&p, sz));
+    overflow_.push_back(p);
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+static thread_local Context *t_ctx = nullptr;
+static std::mutex g_init_mu;
+static int g_device = -1;
+
+void ensure_init() {
+    if (t_ctx) return;
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) throw Error(SKL_E_NODEVICE, "no HIP device available");
+    int dev = g_device >= 0 ? g_device : 0;
+    SKL_HIP(hipSetDevice(dev));
+    Context *c = new Context();
+    c->device = dev;
+    SKL_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    SKL_HIP(hipGetDeviceProperties(&prop, dev));
+    c->num_cu = prop.multiProcessorCount;
+    c->coop_ok = prop.cooperativeLaunch;
+    c->lds_max = prop.sharedMemPerBlock;
+    SKL_HIP(hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault));
+    t_ctx = c;
+}
+
+Context &ctx() {
+    ensure_init();
+    return *t_ctx;
+}
+
+void set_device(int dev) { g_device = dev; }
+
+// ---------------------------------------------------------------------------
+// glibc srandom_r / random_r (TYPE_3: degree 31, separation 3): the generator
+// behind libc rand() that the reference draws its hashing pattern and signs
+// from (src/dense_helpers.cpp:12-14, src/sparse_helpers.cpp:74-116).
+// ---------------------------------------------------------------------------
+void Glibc::seed(unsigned sd) {
+    if (sd == 0) sd = 1;
+    s[0] = (int32_t)sd;
+    int32_t word = (int32_t)sd;
+    for (int i = 1; i < 31; ++i) {
+        long hi = word / 127773;
+        long lo = word % 127773;
+        word = (int32_t)(16807 * lo - 2836 * hi);
+        if (word < 0) word += 2147483647;
+        s[i] = word;
+    }
+    f = 3;
+    r = 0;
+    for (int i = 0; i < 310; ++i) (void)next();
+    toggle = 0;
+    b2 = 0.0;
+}
+
+// Marsaglia polar method on rand()/RAND_MAX with an alternate-call cache
+// (hqrrp/lapack_compatible_sources/NoFLA_HQRRP_WY_blk_var4_skint.c:603-624).
+#pragma clang fp contract(off)
+double Glibc::normal() {
+    if (toggle == 1) {
+        toggle = 0;
+        return 0.0 + 1.0 * b2;
+    }
+    double c1, c2, a;
+    do {
+        c1 = -1.0 + 2.0 * ((double)next() / 2147483647);
+        c2 = -1.0 + 2.0 * ((double)next() / 2147483647);
+        a = c1 * c1 + c2 * c2;
+    } while (a == 0 || a >= 1);
+    double factor = std::sqrt((-2 * std::log(a)) / a);
+    b2 = c2 * factor;
+    toggle = 1;
+    return 0.0 + 1.0 * (c1 * factor);
+}
+
+void gen_signs(Glibc &g, long n, std::vector<double> &D) {
+    D.resize((size_t)n);
+    for (long i = 0; i < n; ++i) D[(size_t)i] = 1.0 * (double)((g.next() % 2) * 2 - 1);
+}
+
+// Division by a runtime constant without the hardware divider: the generator
+// output is < 2^31, so q = (x * magic) >> 64 with a 64-bit magic is exact.
+struct FastMod {
+    uint64_t d, magic;
+    explicit FastMod(uint64_t dd) : d(dd), magic(dd > 1 ? ~uint64_t(0) / dd + 1 : 0) {}
+    inline uint64_t mod(uint64_t x) const {
+        if (d == 1) return 0;
+        uint64_t q = (uint64_t)(((unsigned __int128)x * magic) >> 64);
+        uint64_t r = x - q * d;
+        if (r >= d) r -= d; // guard the rare off-by-one of the approximation
+        return r;
+    }
+};
+
+// select_k_from_n on an index array that persists across columns, then the
+// k*n sign draws (src/sparse_helpers.cpp:74-116).
+void gen_hashing(Glibc &g, long m, long n, long k, Pattern &p) {
+    p.m = m;
+    p.n = n;
+    p.k = k;
+    p.rows.resize((size_t)(n * k));
+    p.vals.resize((size_t)(n * k));
+    std::vector<int32_t> arr((size_t)m);
+    for (long i = 0; i < m; ++i) arr[(size_t)i] = (int32_t)i;
+    std::vector<FastMod> mods;
+    for (long t = 0; t < k; ++t) mods.emplace_back((uint64_t)(m - t));
+    int32_t *rows = p.rows.data();
+    int32_t *a = arr.data();
+    for (long i = 0; i < n; ++i) {
+        for (long t = 0; t < k; ++t) {
+            long j = (long)mods[(size_t)t].mod((uint64_t)g.next()) + t;
+            int32_t tmp = a[j];
+            a[j] = a[t];
+            a[t] = tmp;
+        }
+        for (long t = 0; t < k; ++t) rows[k * i + t] = a[t];
+    }
+    double scaling = std::sqrt((double)k);
+    double pos = 1 / scaling * 1.0, neg = 1 / scaling * -1.0;
+    double *v = p.vals.data();
+    for (long q = 0; q < n * k; ++q) v[q] = (g.next() % 2) ? pos : neg;
+}
+
+// Bucket lists: a stable counting sort of the pattern by target row, so the
+// members of each bucket appear in increasing source index -- the order in
+// which the reference's scatter (SpMat.cpp:53-55, cs_gaxpy) accumulates.
+void build_buckets(Pattern &p) {
+    long m = p.m, nk = p.n * p.k;
+    p.bptr.assign((size_t)m + 1, 0);
+    for (long q = 0; q < nk; ++q) p.bptr[(size_t)p.rows[(size_t)q] + 1]++;
+    for (long h = 0; h < m; ++h) p.bptr[(size_t)h + 1] += p.bptr[(size_t)h];
+    p.bsrc.resize((size_t)nk);
+    p.bslot.resize((size_t)nk);
+    std::vector<int32_t> cur(p.bptr.begin(), p.bptr.end() - 1);
+    for (long q = 0; q < nk; ++q) {
+        int32_t h = p.rows[(size_t)q];
+        int32_t pos = cur[(size_t)h]++;
+        p.bsrc[(size_t)pos] = (int32_t)(q / p.k);
+        p.bslot[(size_t)pos] = (int32_t)q;
+    }
+}
+
+void gen_gaussian(unsigned seed, long rows, long cols, std::vector<double> &G) {
+    Glibc g;
+    g.seed(seed);
+    G.resize((size_t)(rows * cols));
+    for (long j = 0; j < cols; ++j)
+        for (long i = 0; i < rows; ++i) G[(size_t)(i + j * rows)] = g.normal();
+}
+
+} // namespace skl

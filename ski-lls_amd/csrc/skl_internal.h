@@ -1,0 +1,183 @@
+// skl_internal.h -- shared declarations of the MI355X least-squares library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/skl.h"
+
+namespace skl {
+
+// ---------------------------------------------------------------------------
+// Errors: internal code throws skl::Error; the C ABI catches and maps it.
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void throw_hip(hipError_t e, const char *what, const char *file, int line);
+
+#define SKL_HIP(call)                                                   \
+    do {                                                                \
+        hipError_t e_ = (call);                                         \
+        if (e_ != hipSuccess) ::skl::throw_hip(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define SKL_CHECK_LAUNCH() SKL_HIP(hipGetLastError())
+
+#define SKL_REQUIRE(cond, code, msg)                 \
+    do {                                             \
+        if (!(cond)) throw ::skl::Error((code), (msg)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Device memory: a grow-only arena per context; buffers are carved with a
+// bump pointer and released all at once by reset().
+// ---------------------------------------------------------------------------
+class Arena {
+  public:
+    ~Arena();
+    void reserve(size_t bytes);        // ensure capacity (reallocates when empty)
+    void *alloc(size_t bytes);         // 256-byte aligned
+    template <class T> T *alloc_n(size_t n) { return static_cast<T *>(alloc(n * sizeof(T))); }
+    void reset() { used_ = 0; }
+    size_t used() const { return used_; }
+    size_t capacity() const { return cap_; }
+    void release();
+
+  private:
+    char *base_ = nullptr;
+    size_t cap_ = 0, used_ = 0;
+    std::vector<void *> overflow_; // extra blocks when the reservation was too small
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Arena arena;
+    int num_cu = 0;
+    int coop_ok = 0;
+    size_t lds_max = 0;
+    void *pinned = nullptr; // small pinned host buffer for scalars
+};
+
+Context &ctx();                       // default per-thread context
+void ensure_init();
+
+double wall();
+
+// ---------------------------------------------------------------------------
+// Hashing pattern (host): libc-compatible generator and bucket lists.
+// ---------------------------------------------------------------------------
+struct Glibc {
+    int32_t s[31];
+    int f, r;
+    int toggle;
+    double b2;
+    void seed(unsigned sd);
+    inline int next() {
+        uint32_t v = (uint32_t)s[f] + (uint32_t)s[r];
+        s[f] = (int32_t)v;
+        if (++f >= 31) f = 0;
+        if (++r >= 31) r = 0;
+        return (int)(v >> 1);
+    }
+    double normal();
+};
+
+struct Pattern {
+    long m = 0, n = 0, k = 0;
+    std::vector<double> D;      // signs (dense path), n
+    std::vector<int32_t> rows;  // n*k target rows
+    std::vector<double> vals;   // n*k values +-1/sqrt(k)
+    // bucket lists (transpose of the pattern): members of bucket h are
+    // bsrc[bptr[h] .. bptr[h+1]) in increasing source index, bslot = k*i+t
+    std::vector<int32_t> bptr, bsrc, bslot;
+};
+
+void gen_signs(Glibc &g, long n, std::vector<double> &D);
+void gen_hashing(Glibc &g, long m, long n, long k, Pattern &p);
+void build_buckets(Pattern &p);
+void gen_gaussian(unsigned seed, long rows, long cols, std::vector<double> &G);
+
+// ---------------------------------------------------------------------------
+// Kernel-side launchers (defined in the .hip files).  All run on ctx().stream.
+// ---------------------------------------------------------------------------
+// DHT sketch: SA[:, c] = alpha * S * H * (scale D .* X[:, c]) for c < ncols
+template <class T>
+void dht_sketch(long n, long ncols, const double *X, long ldx, const double *D, double scale,
+                long m, const int32_t *bptr, const int32_t *bsrc, const T *bcoef, T *SA,
+                long ldsa, Arena &ar);
+// Plain DHT of columns (unnormalised) into Y (real, same precision as T)
+template <class T>
+void dht_cols(long n, long ncols, const T *X, long ldx, T *Y, long ldy, Arena &ar);
+// bucket-gather hashing of dense columns (already transformed)
+template <class T>
+void hash_gather(long m, long ncols, const int32_t *bptr, const int32_t *bsrc,
+                 const T *bcoef, const T *X, long ldx, T *Y, long ldy);
+// sparse sketch: SA = S A (CSC, int32 indices on device)
+void sparse_sketch_dev(long m, long n, long d, long k, const int32_t *rows, const double *vals,
+                       const int64_t *colptr, const int32_t *rowind, const double *val,
+                       double *SA, long ldsa, Arena &ar);
+
+// QR family (column-major, W is m x ncol with ld; nfac leading columns factored)
+template <class T>
+void qr_householder(long m, long ncol, long nfac, T *W, long ldw, T *tau, Arena &ar);
+template <class T>
+void cpqr_greedy(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *tau,
+                 Arena &ar);
+template <class T>
+void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *tau,
+                const double *G_host, Arena &ar);
+
+// triangular inverse and norms
+void trinv_upper(long r, const double *R, long ldr, double perturb, double *M, long ldm,
+                 Arena &ar);
+template <class T>
+void upcast_upper(long r, const T *R, long ldr, double *Rd, long ldd);
+double norm_inf_upper(long r, const double *R, long ldr, Arena &ar);
+
+// GEMM: C = alpha op(A) op(B) + beta C (column-major)
+template <class T>
+void gemm(char ta, char tb, long M, long N, long K, T alpha, const T *A, long lda, const T *B,
+          long ldb, T beta, T *C, long ldc, long batch = 1, long sA = 0, long sB = 0,
+          long sC = 0);
+
+// LSQR on the preconditioned operator A(:, E) M
+struct LsqrProblem {
+    long n = 0, d = 0, r = 0;
+    // dense A (n x d, lda) or sparse CSR + CSC (int32 indices)
+    const double *A = nullptr;
+    long lda = 0;
+    const int32_t *csr_ptr = nullptr, *csr_col = nullptr;
+    const double *csr_val = nullptr;
+    const int32_t *csc_ptr = nullptr, *csc_row = nullptr;
+    const double *csc_val = nullptr;
+    const int32_t *E = nullptr; // r entries or null (identity)
+    const double *M = nullptr;  // r x r upper, ldm
+    long ldm = 0;
+};
+struct LsqrResult {
+    int flag = 0;
+    long it = 0;
+    double nrm_ar = 0, nrm_a = 0, nrm_r = 0;
+};
+LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long max_it,
+                      double *y, bool use_graph, int chunk, skl_lsqr_step *hist,
+                      long hist_len, Arena &ar);
+
+// small vector helpers
+void dev_gemv_n_dense(long n, long d, const double *A, long lda, const double *x, double alpha,
+                      double beta, double *y, Arena &ar);
+void dev_csc_gemv_n(long n, long d, const int64_t *colptr, const int32_t *rowind,
+                    const double *val, const double *x, double alpha, double beta,
+                    double *y, Arena &ar);
+double dev_nrm2(long n, const double *x, Arena &ar);
+
+} // namespace skl

@@ -1,0 +1,1024 @@
+// solver.hip -- orchestration of the three Ski-LLS solvers on one MI355X and
+// the C ABI of include/skl.h.
+//
+// Dense (src/ls_dense_hashing_blendenpik.cpp:23-149):
+//   signs + hashing pattern (host, libc-compatible draws) -> fused DHT +
+//   bucket-sum sketch of [A | b] -> CPQR of [SA | c] (c receives Q^T c) ->
+//   rank -> M = R^{-1} -> x0 = P M c -> r = b - A x0 -> LSQR on A(:,E) M ->
+//   x = x0 + P M y.  A is never permuted in memory: E is folded into the
+//   LSQR kernels' indexing.
+// Dense without CPQR (src/ls_dense_hashing_blendenpik_noCPQR.cpp:23-60).
+// Sparse (src/ls_sparse_spqr.cpp:79-214): hashing sketch of the CSC matrix,
+//   Householder QR of [SA | S b] standing in for SuiteSparseQR, rank check
+//   against SPQR's default tolerance (column-pivoted refactorisation when it
+//   fails), exact inf-norm condition number of R11, LSQR with the pivot
+//   perturbation when R11 is ill conditioned.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <map>
+#include <mutex>
+
+#include "k_common.h"
+
+namespace skl {
+
+template <class T>
+void dht_cols_from_double(long n, long ncols, const double *X, long ldx, T *Y, long ldy, Arena &ar);
+template <class T>
+void sparse_sketch_t(long m, long d, long k, const int32_t *srows, const double *svals,
+                     const int32_t *colptr, const int32_t *rowind, const double *val, long nnz,
+                     T *SA, long ldsa, Arena &ar);
+void narrow_indices(long n, const long *in, int32_t *out);
+void csc_to_csr(long n, long d, long nnz, const int32_t *colptr, const int32_t *rowind,
+                const double *val, int32_t *rptr, int32_t *ccol, double *cval, Arena &ar);
+void csr_gemv(long n, const int32_t *ptr, const int32_t *col, const double *val, const double *x,
+              double alpha, double beta, double *y);
+void set_device(int dev);
+
+static thread_local std::string t_err;
+static thread_local skl_lsqr_step *t_hist = nullptr;
+static thread_local long t_hist_len = 0;
+
+// ---------------------------------------------------------------------------
+// glue kernels
+// ---------------------------------------------------------------------------
+__global__ void k_scatter_E(long d, long r, const int32_t *E, const double *v, double *x) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < r) x[E ? E[i] : i] = v[i];
+}
+__global__ void k_fill_d(long n, double *x, double v) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+__global__ void k_axpy_d(long n, const double *a, double *y) { // y = a + y
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a[i] + y[i];
+}
+__global__ void k_iota(long n, int32_t *x) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = (int32_t)i;
+}
+template <class T> __global__ void k_diag(long r, const T *W, long ldw, double *dg) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < r) dg[i] = (double)W[i + i * ldw];
+}
+template <class T> __global__ void k_col_to_double(long r, const T *c, double *o) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < r) o[i] = (double)c[i];
+}
+__global__ void k_add_diag(long r, double *R, long ldr, double p) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < r) R[i + i * ldr] += p;
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_colnorm2(long m, long d, const T *W, long ldw, double *out) {
+    long j = blockIdx.x;
+    double s = 0.0;
+    for (long i = threadIdx.x; i < m; i += 256) {
+        double v = (double)W[i + j * ldw];
+        s += v * v;
+    }
+    s = block_sum<256>(s);
+    if (threadIdx.x == 0) out[j] = s;
+}
+
+static inline unsigned gblocks(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// M v (upper r x r) into out (double): out = M v
+static void trmv_n(long r, const double *M, long ldm, const double *v, double *out, Arena &ar) {
+    dev_gemv_n_dense(r, r, M, ldm, v, 1.0, 0.0, out, ar);
+}
+
+struct DevPattern {
+    double *D = nullptr;
+    int32_t *bptr = nullptr, *bsrc = nullptr, *rows = nullptr;
+    double *coef_d = nullptr; // alpha * val in bucket order (double)
+    float *coef_f = nullptr;
+    double *vals = nullptr;   // n*k values (source order)
+};
+
+static DevPattern upload_pattern(const Pattern &P, double alpha, bool want_D, bool want_rows,
+                                 bool want_float, Arena &ar) {
+    hipStream_t s = ctx().stream;
+    DevPattern dp;
+    long nk = P.n * P.k;
+    if (want_D) {
+        dp.D = ar.alloc_n<double>((size_t)P.n);
+        SKL_HIP(hipMemcpyAsync(dp.D, P.D.data(), sizeof(double) * P.n, hipMemcpyHostToDevice, s));
+    }
+    dp.bptr = ar.alloc_n<int32_t>((size_t)P.m + 1);
+    dp.bsrc = ar.alloc_n<int32_t>((size_t)std::max<long>(nk, 1));
+    SKL_HIP(hipMemcpyAsync(dp.bptr, P.bptr.data(), sizeof(int32_t) * (P.m + 1), hipMemcpyHostToDevice, s));
+    SKL_HIP(hipMemcpyAsync(dp.bsrc, P.bsrc.data(), sizeof(int32_t) * nk, hipMemcpyHostToDevice, s));
+    // alpha * vals[idx] as the reference multiplies (SpMat.cpp:55)
+    std::vector<double> cd((size_t)nk);
+    for (long q = 0; q < nk; ++q) cd[(size_t)q] = alpha * P.vals[(size_t)P.bslot[(size_t)q]];
+    dp.coef_d = ar.alloc_n<double>((size_t)std::max<long>(nk, 1));
+    SKL_HIP(hipMemcpyAsync(dp.coef_d, cd.data(), sizeof(double) * nk, hipMemcpyHostToDevice, s));
+    if (want_float) {
+        std::vector<float> cf((size_t)nk);
+        for (long q = 0; q < nk; ++q) cf[(size_t)q] = (float)cd[(size_t)q];
+        dp.coef_f = ar.alloc_n<float>((size_t)std::max<long>(nk, 1));
+        SKL_HIP(hipMemcpyAsync(dp.coef_f, cf.data(), sizeof(float) * nk, hipMemcpyHostToDevice, s));
+        SKL_HIP(hipStreamSynchronize(s)); // cf goes out of scope
+    }
+    if (want_rows) {
+        dp.rows = ar.alloc_n<int32_t>((size_t)std::max<long>(nk, 1));
+        SKL_HIP(hipMemcpyAsync(dp.rows, P.rows.data(), sizeof(int32_t) * nk, hipMemcpyHostToDevice, s));
+        dp.vals = ar.alloc_n<double>((size_t)std::max<long>(nk, 1));
+        SKL_HIP(hipMemcpyAsync(dp.vals, P.vals.data(), sizeof(double) * nk, hipMemcpyHostToDevice, s));
+    }
+    SKL_HIP(hipStreamSynchronize(s));
+    return dp;
+}
+
+static const std::vector<double> &gaussian_for(long m) {
+    static std::mutex mu;
+    static std::map<long, std::vector<double>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(m);
+    if (it != cache.end()) return it->second;
+    std::vector<double> G;
+    gen_gaussian(12, 74, m, G); // srand(12) in HQRRP (skint.c:390)
+    return cache.emplace(m, std::move(G)).first->second;
+}
+
+static void sync() { SKL_HIP(hipStreamSynchronize(ctx().stream)); }
+
+// ---------------------------------------------------------------------------
+// Dense solve (both variants), device pointers
+// ---------------------------------------------------------------------------
+template <class T>
+static void dense_sketch_qr(long n, long d, long m, const double *dA, long lda, const double *db,
+                            const DevPattern &dp, bool cpqr, T *W, int32_t *jpvt, T *tau,
+                            double *t_sketch, double *t_qr, Arena &ar) {
+    double t0 = wall();
+    const T *coef = std::is_same<T, float>::value ? (const T *)dp.coef_f : (const T *)dp.coef_d;
+    double scale = 1 / std::sqrt((double)n);
+    // SA = S H D A (m x d) into W(:, 0:d)
+    dht_sketch<T>(n, d, dA, lda, dp.D, scale, m, dp.bptr, dp.bsrc, coef, W, m, ar);
+    long ncol = d;
+    if (db) {
+        dht_sketch<T>(n, 1, db, n, dp.D, scale, m, dp.bptr, dp.bsrc, coef, W + d * m, m, ar);
+        ncol = d + 1;
+    }
+    sync();
+    double t1 = wall();
+    if (cpqr) {
+        hipLaunchKernelGGL(k_iota, dim3(gblocks(d)), dim3(256), 0, ctx().stream, d, jpvt);
+        if (std::min(m, d) < 1000)
+            cpqr_greedy<T>(m, ncol, d, W, m, jpvt, tau, ar);
+        else
+            cpqr_hqrrp<T>(m, ncol, d, W, m, jpvt, tau, gaussian_for(m).data(), ar);
+    } else {
+        qr_householder<T>(m, ncol, d, W, m, tau, ar);
+    }
+    sync();
+    double t2 = wall();
+    *t_sketch += t1 - t0;
+    *t_qr += t2 - t1;
+}
+
+// reciprocal inf-norm condition number from R (double, upper) and M = R^{-1}
+static double rcond_from(long r, const double *Rd, long ldr, const double *M, long ldm, Arena &ar) {
+    double a = norm_inf_upper(r, Rd, ldr, ar);
+    double b = norm_inf_upper(r, M, ldm, ar);
+    if (!(a > 0) || !(b > 0) || !std::isfinite(b)) return 0.0;
+    return (1.0 / a) / b;
+}
+
+static int dense_impl(long n, long d, const double *dA, long lda, const double *db, double *dx,
+                      const skl_opts &o, skl_info &info, bool cpqr) {
+    Context &C = ctx();
+    Arena &ar = C.arena;
+    hipStream_t s = C.stream;
+    std::memset(&info, 0, sizeof info);
+    if (n < d) throw Error(SKL_E_SHAPE, "Matrix is not overdetermined.");
+    SKL_REQUIRE(d > 0 && n > 0 && lda >= n, SKL_E_ARG, "invalid dense shape");
+    SKL_REQUIRE(o.k >= 1 && o.gamma > 0, SKL_E_ARG, "invalid sketch parameters");
+    long m = (long)std::ceil((double)d * o.gamma);
+    SKL_REQUIRE(m >= o.k && m >= d, SKL_E_ARG, "sketch must have at least d rows");
+    info.m = m;
+    ar.reset();
+    double T0 = wall();
+    // ---- pattern (libc draw order of dense_helpers.cpp:92 / 12-14) ----
+    Glibc g;
+    g.seed(o.seed);
+    Pattern P;
+    gen_signs(g, n, P.D);
+    gen_hashing(g, m, n, o.k, P);
+    build_buckets(P);
+    bool mixed = (o.precision == SKL_MIXED);
+    DevPattern dp = upload_pattern(P, 1 / std::sqrt((double)o.k), true, false, mixed, ar);
+    double T1 = wall();
+    info.t_pattern = T1 - T0;
+    // ---- sketch + QR ----
+    long ncol = cpqr ? d + 1 : d;
+    int32_t *jpvt = ar.alloc_n<int32_t>((size_t)d);
+    double *Rd = nullptr;
+    long rank = d;
+    double *cvec = ar.alloc_n<double>((size_t)d + 1); // Q^T c, first rank entries
+    bool escalated = false;
+    double cond_est = 0.0;
+    double *Mi = ar.alloc_n<double>((size_t)d * d);
+    auto finish_qr = [&](auto *W) {
+        // diagonal -> rank (first |R_ii| <= rcond), R to double, c
+        std::vector<double> dg((size_t)d);
+        double *ddg = ar.alloc_n<double>((size_t)d);
+        hipLaunchKernelGGL(k_diag, dim3(gblocks(d)), dim3(256), 0, s, d, W, m, ddg);
+        SKL_HIP(hipMemcpyAsync(dg.data(), ddg, sizeof(double) * d, hipMemcpyDeviceToHost, s));
+        sync();
+        rank = d;
+        if (cpqr) {
+            rank = 0;
+            for (long i = 0; i < d; ++i) {
+                if (std::fabs(dg[(size_t)i]) > o.rcond)
+                    rank++;
+                else
+                    break;
+            }
+            hipLaunchKernelGGL(k_col_to_double, dim3(gblocks(d)), dim3(256), 0, s, d, W + d * m, cvec);
+        }
+        Rd = ar.alloc_n<double>((size_t)std::max<long>(rank, 1) * std::max<long>(rank, 1));
+        upcast_upper(rank, W, m, Rd, rank);
+        trinv_upper(rank, Rd, rank, 0.0, Mi, rank, ar);
+    };
+    if (!mixed) {
+        double *W = ar.alloc_n<double>((size_t)m * ncol);
+        double *tau = ar.alloc_n<double>((size_t)d);
+        dense_sketch_qr<double>(n, d, m, dA, lda, cpqr ? db : nullptr, dp, cpqr, W, jpvt, tau,
+                                &info.t_sketch, &info.t_qr, ar);
+        finish_qr(W);
+    } else {
+        float *W = ar.alloc_n<float>((size_t)m * ncol);
+        float *tau = ar.alloc_n<float>((size_t)d);
+        dense_sketch_qr<float>(n, d, m, dA, lda, cpqr ? db : nullptr, dp, cpqr, W, jpvt, tau,
+                               &info.t_sketch, &info.t_qr, ar);
+        finish_qr(W);
+        double rc = rcond_from(rank, Rd, rank, Mi, rank, ar);
+        cond_est = rc > 0 ? 1.0 / rc : INFINITY;
+        if (!(cond_est <= o.escalate_cond)) {
+            escalated = true;
+            double *Wd = ar.alloc_n<double>((size_t)m * ncol);
+            double *taud = ar.alloc_n<double>((size_t)d);
+            DevPattern dp2 = dp;
+            dense_sketch_qr<double>(n, d, m, dA, lda, cpqr ? db : nullptr, dp2, cpqr, Wd, jpvt,
+                                    taud, &info.t_sketch, &info.t_qr, ar);
+            finish_qr(Wd);
+        }
+    }
+    info.escalated = escalated ? 1 : 0;
+    info.cond_est = cond_est;
+    info.rank = rank;
+    sync();
+    double T2 = wall();
+    if (rank == 0) throw Error(SKL_E_RANK0, "rank 0: all |R_ii| <= rcond");
+    LsqrProblem p;
+    p.n = n;
+    p.d = d;
+    p.r = rank;
+    p.A = dA;
+    p.lda = lda;
+    p.E = cpqr ? jpvt : nullptr;
+    p.M = Mi;
+    p.ldm = rank;
+    double *y = ar.alloc_n<double>((size_t)rank);
+    if (!cpqr) {
+        // LSQR on b directly, x = M y
+        LsqrResult res = lsqr_solve(p, db, o.it_tol, o.max_it, y, o.use_graph != 0, o.graph_chunk,
+                                    t_hist, t_hist_len, ar);
+        double *my = ar.alloc_n<double>((size_t)rank);
+        trmv_n(rank, Mi, rank, y, my, ar);
+        hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank,
+                           (const int32_t *)nullptr, my, dx);
+        sync();
+        info.flag = res.flag;
+        info.it = res.it;
+        info.lsqr_nrm_ar = res.nrm_ar;
+        info.lsqr_nrm_a = res.nrm_a;
+        info.lsqr_nrm_r = res.nrm_r;
+        double T3 = wall();
+        info.t_lsqr = T3 - T2;
+        info.t_total = T3 - T0;
+        return SKL_OK;
+    }
+    // x0 = P M c ; rs = b - A x0
+    double *c1 = ar.alloc_n<double>((size_t)rank);
+    trmv_n(rank, Mi, rank, cvec, c1, ar);
+    double *x0 = ar.alloc_n<double>((size_t)d);
+    hipLaunchKernelGGL(k_fill_d, dim3(gblocks(d)), dim3(256), 0, s, d, x0, 0.0);
+    hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank, jpvt, c1, x0);
+    double *rs = ar.alloc_n<double>((size_t)n);
+    SKL_HIP(hipMemcpyAsync(rs, db, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    dev_gemv_n_dense(n, d, dA, lda, x0, -1.0, 1.0, rs, ar);
+    double r0 = dev_nrm2(n, rs, ar);
+    info.x0_residual = r0;
+    double T3 = wall();
+    info.t_x0 = T3 - T2;
+    if (r0 < o.abs_tol) {
+        SKL_HIP(hipMemcpyAsync(dx, x0, sizeof(double) * d, hipMemcpyDeviceToDevice, s));
+        sync();
+        info.it = 0;
+        info.flag = 0;
+        info.t_total = wall() - T0;
+        return SKL_OK;
+    }
+    LsqrResult res = lsqr_solve(p, rs, o.it_tol, o.max_it, y, o.use_graph != 0, o.graph_chunk,
+                                t_hist, t_hist_len, ar);
+    double *my = ar.alloc_n<double>((size_t)rank);
+    trmv_n(rank, Mi, rank, y, my, ar);
+    hipLaunchKernelGGL(k_fill_d, dim3(gblocks(d)), dim3(256), 0, s, d, dx, 0.0);
+    hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank, jpvt, my, dx);
+    hipLaunchKernelGGL(k_axpy_d, dim3(gblocks(d)), dim3(256), 0, s, d, x0, dx);
+    sync();
+    info.flag = res.flag;
+    info.it = res.it;
+    info.lsqr_nrm_ar = res.nrm_ar;
+    info.lsqr_nrm_a = res.nrm_a;
+    info.lsqr_nrm_r = res.nrm_r;
+    double T4 = wall();
+    info.t_lsqr = T4 - T3;
+    info.t_total = T4 - T0;
+    return SKL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Sparse solve, device CSC with int64 indices
+// ---------------------------------------------------------------------------
+template <class T>
+static long sparse_factor(long n, long d, long m, long k, const DevPattern &dp, const int32_t *cp,
+                          const int32_t *ri, const double *val, long nnz, const double *db, T *W,
+                          T *tau, int32_t *jpvt, double *tol_out, Arena &ar) {
+    hipStream_t s = ctx().stream;
+    sparse_sketch_t<T>(m, d, k, dp.rows, dp.vals, cp, ri, val, nnz, W, m, ar);
+    // b_s = S b in cs_gaxpy order, as the last column
+    double *bs = ar.alloc_n<double>((size_t)m);
+    hash_gather<double>(m, 1, dp.bptr, dp.bsrc, dp.coef_d, db, n, bs, m);
+    copy2d<double, T>(m, 1, bs, m, W + d * m, m);
+    // SPQR's default tolerance 20 (m + d) eps max_j ||SA_j||
+    double *cn = ar.alloc_n<double>((size_t)d);
+    hipLaunchKernelGGL(k_colnorm2<T>, dim3((unsigned)d), dim3(256), 0, s, m, d, W, m, cn);
+    std::vector<double> hc((size_t)d);
+    SKL_HIP(hipMemcpyAsync(hc.data(), cn, sizeof(double) * d, hipMemcpyDeviceToHost, s));
+    sync();
+    double maxn = 0;
+    for (double v : hc) maxn = std::max(maxn, std::sqrt(v));
+    double eps = std::is_same<T, float>::value ? (double)FLT_EPSILON : DBL_EPSILON;
+    double tol = 20.0 * (double)(m + d) * eps * maxn;
+    *tol_out = tol;
+    qr_householder<T>(m, d + 1, d, W, m, tau, ar);
+    long kmax = std::min(m, d);
+    std::vector<double> dg((size_t)kmax);
+    double *ddg = ar.alloc_n<double>((size_t)kmax);
+    hipLaunchKernelGGL(k_diag<T>, dim3(gblocks(kmax)), dim3(256), 0, s, kmax, W, m, ddg);
+    SKL_HIP(hipMemcpyAsync(dg.data(), ddg, sizeof(double) * kmax, hipMemcpyDeviceToHost, s));
+    sync();
+    bool full = true;
+    for (long i = 0; i < kmax; ++i)
+        if (!(std::fabs(dg[(size_t)i]) > tol)) {
+            full = false;
+            break;
+        }
+    hipLaunchKernelGGL(k_iota, dim3(gblocks(d)), dim3(256), 0, s, d, jpvt);
+    if (full) return kmax;
+    // rank deficient: column-pivoted refactorisation of a fresh sketch
+    sparse_sketch_t<T>(m, d, k, dp.rows, dp.vals, cp, ri, val, nnz, W, m, ar);
+    copy2d<double, T>(m, 1, bs, m, W + d * m, m);
+    cpqr_greedy<T>(m, d + 1, d, W, m, jpvt, tau, ar);
+    hipLaunchKernelGGL(k_diag<T>, dim3(gblocks(kmax)), dim3(256), 0, s, kmax, W, m, ddg);
+    SKL_HIP(hipMemcpyAsync(dg.data(), ddg, sizeof(double) * kmax, hipMemcpyDeviceToHost, s));
+    sync();
+    long rank = 0;
+    for (long i = 0; i < kmax; ++i) {
+        if (std::fabs(dg[(size_t)i]) > tol)
+            rank++;
+        else
+            break;
+    }
+    return rank;
+}
+
+static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dri,
+                       const double *dval, const double *db, double *dx, const skl_opts &o,
+                       skl_info &info) {
+    Context &C = ctx();
+    Arena &ar = C.arena;
+    hipStream_t s = C.stream;
+    std::memset(&info, 0, sizeof info);
+    if (n < d) throw Error(SKL_E_SHAPE, "Matrix is not overdetermined.");
+    SKL_REQUIRE(d > 0 && n > 0 && nnz >= 0, SKL_E_ARG, "invalid sparse shape");
+    SKL_REQUIRE(nnz < 2147483647L && n < 2147483647L, SKL_E_UNSUPPORTED, "nnz must fit in int32");
+    long m = (long)std::ceil((double)d * o.gamma);
+    SKL_REQUIRE(m >= o.k && m >= 1, SKL_E_ARG, "invalid sketch parameters");
+    info.m = m;
+    ar.reset();
+    double T0 = wall();
+    int32_t *cp = ar.alloc_n<int32_t>((size_t)d + 1);
+    int32_t *ri = ar.alloc_n<int32_t>((size_t)std::max<long>(nnz, 1));
+    narrow_indices(d + 1, dcp, cp);
+    narrow_indices(nnz, dri, ri);
+    int32_t *rp = ar.alloc_n<int32_t>((size_t)n + 1);
+    int32_t *cc = ar.alloc_n<int32_t>((size_t)std::max<long>(nnz, 1));
+    double *cv = ar.alloc_n<double>((size_t)std::max<long>(nnz, 1));
+    csc_to_csr(n, d, nnz, cp, ri, dval, rp, cc, cv, ar);
+    // pattern: no srand on the sparse path (sketching.cpp:100); seed = the
+    // state of the process-wide generator, 1 for a fresh process
+    Glibc g;
+    g.seed(o.seed);
+    Pattern P;
+    gen_hashing(g, m, n, o.k, P);
+    build_buckets(P);
+    bool mixed = (o.precision == SKL_MIXED);
+    DevPattern dp = upload_pattern(P, 1.0, false, true, false, ar);
+    double T1 = wall();
+    info.t_pattern = T1 - T0;
+    int32_t *jpvt = ar.alloc_n<int32_t>((size_t)d);
+    double *cvec = ar.alloc_n<double>((size_t)d + 1);
+    double *Rd = nullptr;
+    long rank = 0;
+    double tol = 0;
+    auto extract = [&](auto *W) {
+        Rd = ar.alloc_n<double>((size_t)std::max<long>(rank, 1) * std::max<long>(rank, 1));
+        upcast_upper(rank, W, m, Rd, rank);
+        hipLaunchKernelGGL(k_col_to_double, dim3(gblocks(rank)), dim3(256), 0, s, rank, W + d * m, cvec);
+    };
+    bool escalated = false;
+    double cond_est = 0;
+    double *Mi = ar.alloc_n<double>((size_t)d * d);
+    double tq0 = wall();
+    if (!mixed) {
+        double *W = ar.alloc_n<double>((size_t)m * (d + 1));
+        double *tau = ar.alloc_n<double>((size_t)d + 1);
+        rank = sparse_factor<double>(n, d, m, o.k, dp, cp, ri, dval, nnz, db, W, tau, jpvt, &tol, ar);
+        if (rank == 0) throw Error(SKL_E_RANK0, "SPQR Factorization fails");
+        extract(W);
+        trinv_upper(rank, Rd, rank, 0.0, Mi, rank, ar);
+    } else {
+        float *W = ar.alloc_n<float>((size_t)m * (d + 1));
+        float *tau = ar.alloc_n<float>((size_t)d + 1);
+        rank = sparse_factor<float>(n, d, m, o.k, dp, cp, ri, dval, nnz, db, W, tau, jpvt, &tol, ar);
+        if (rank == 0) throw Error(SKL_E_RANK0, "SPQR Factorization fails");
+        extract(W);
+        trinv_upper(rank, Rd, rank, 0.0, Mi, rank, ar);
+        double rc = rcond_from(rank, Rd, rank, Mi, rank, ar);
+        cond_est = rc > 0 ? 1.0 / rc : INFINITY;
+        if (!(cond_est <= o.escalate_cond)) {
+            escalated = true;
+            double *Wd = ar.alloc_n<double>((size_t)m * (d + 1));
+            double *taud = ar.alloc_n<double>((size_t)d + 1);
+            rank = sparse_factor<double>(n, d, m, o.k, dp, cp, ri, dval, nnz, db, Wd, taud, jpvt,
+                                         &tol, ar);
+            if (rank == 0) throw Error(SKL_E_RANK0, "SPQR Factorization fails");
+            extract(Wd);
+            trinv_upper(rank, Rd, rank, 0.0, Mi, rank, ar);
+        }
+    }
+    info.escalated = escalated;
+    info.cond_est = cond_est;
+    info.rank = rank;
+    double rc = rcond_from(rank, Rd, rank, Mi, rank, ar);
+    info.rcond = rc;
+    info.success = rc > o.rcond_threshold ? 1 : 0;
+    double *Mp = Mi;
+    if (!info.success) {
+        // LSQR operator uses (R11 + perturb I)^{-1} (sparse_helpers.cpp:138-147)
+        Mp = ar.alloc_n<double>((size_t)rank * rank);
+        trinv_upper(rank, Rd, rank, o.perturb, Mp, rank, ar);
+    }
+    sync();
+    double T2 = wall();
+    info.t_sketch = 0;
+    info.t_qr = T2 - tq0;
+    // x0 = P M c (no perturbation, ls_sparse_spqr.cpp:144)
+    double *c1 = ar.alloc_n<double>((size_t)rank);
+    trmv_n(rank, Mi, rank, cvec, c1, ar);
+    double *x0 = ar.alloc_n<double>((size_t)d);
+    hipLaunchKernelGGL(k_fill_d, dim3(gblocks(d)), dim3(256), 0, s, d, x0, 0.0);
+    hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank, jpvt, c1, x0);
+    double *rs = ar.alloc_n<double>((size_t)n);
+    SKL_HIP(hipMemcpyAsync(rs, db, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    csr_gemv(n, rp, cc, cv, x0, -1.0, 1.0, rs);
+    double r0 = dev_nrm2(n, rs, ar);
+    info.x0_residual = r0;
+    double T3 = wall();
+    info.t_x0 = T3 - T2;
+    if (r0 < o.abs_tol) {
+        SKL_HIP(hipMemcpyAsync(dx, x0, sizeof(double) * d, hipMemcpyDeviceToDevice, s));
+        sync();
+        info.it = 0;
+        info.flag = 0;
+        info.t_total = wall() - T0;
+        return SKL_OK;
+    }
+    LsqrProblem p;
+    p.n = n;
+    p.d = d;
+    p.r = rank;
+    p.csr_ptr = rp;
+    p.csr_col = cc;
+    p.csr_val = cv;
+    p.csc_ptr = cp;
+    p.csc_row = ri;
+    p.csc_val = dval;
+    p.E = jpvt;
+    p.M = Mp;
+    p.ldm = rank;
+    double *y = ar.alloc_n<double>((size_t)rank);
+    LsqrResult res = lsqr_solve(p, rs, o.it_tol, o.max_it, y, o.use_graph != 0, o.graph_chunk,
+                                t_hist, t_hist_len, ar);
+    // y <- R11^{-1} y (no perturbation, ls_sparse_spqr.cpp:188); y += c; x = P y
+    double *my = ar.alloc_n<double>((size_t)rank);
+    trmv_n(rank, Mi, rank, y, my, ar);
+    hipLaunchKernelGGL(k_axpy_d, dim3(gblocks(rank)), dim3(256), 0, s, rank, c1, my);
+    hipLaunchKernelGGL(k_fill_d, dim3(gblocks(d)), dim3(256), 0, s, d, dx, 0.0);
+    hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank, jpvt, my, dx);
+    sync();
+    info.flag = res.flag;
+    info.it = res.it;
+    info.lsqr_nrm_ar = res.nrm_ar;
+    info.lsqr_nrm_a = res.nrm_a;
+    info.lsqr_nrm_r = res.nrm_r;
+    double T4 = wall();
+    info.t_lsqr = T4 - T3;
+    info.t_total = T4 - T0;
+    return SKL_OK;
+}
+
+} // namespace skl
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace skl;
+
+template <class F> static int guarded(F &&f) {
+    try {
+        skl::t_err.clear();
+        return f();
+    } catch (const skl::Error &e) {
+        skl::t_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        skl::t_err = "host allocation failed";
+        return SKL_E_NOMEM;
+    } catch (const std::exception &e) {
+        skl::t_err = e.what();
+        return SKL_E_HIP;
+    }
+}
+
+static skl_opts resolve(const skl_opts *o) {
+    skl_opts r;
+    if (o)
+        r = *o;
+    else
+        skl_default_opts(&r);
+    if (r.graph_chunk <= 0) r.graph_chunk = 16;
+    return r;
+}
+
+namespace skl {
+// Host-pointer variants: inputs are staged in device buffers owned by the call.
+struct DevBuf {
+    void *p = nullptr;
+    explicit DevBuf(size_t bytes) { SKL_HIP(hipMalloc(&p, std::max<size_t>(bytes, 8))); }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <class T> T *as() { return static_cast<T *>(p); }
+};
+
+static int dense_host(long n, long d, const double *A, long lda, const double *b, double *x,
+                      const skl_opts *opts, skl_info *info, bool cpqr) {
+    skl_info tmp;
+    return guarded([&] {
+        if (n < d) throw Error(SKL_E_SHAPE, "Matrix is not overdetermined.");
+        SKL_REQUIRE(A && b && x && lda >= n && d > 0, SKL_E_ARG, "invalid arguments");
+        skl_opts o = resolve(opts);
+        ensure_init();
+        hipStream_t s = ctx().stream;
+        DevBuf dA(sizeof(double) * (size_t)n * d), db(sizeof(double) * n), dx(sizeof(double) * d);
+        SKL_HIP(hipMemcpy2DAsync(dA.p, sizeof(double) * n, A, sizeof(double) * lda,
+                                 sizeof(double) * n, d, hipMemcpyHostToDevice, s));
+        SKL_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        SKL_HIP(hipStreamSynchronize(s));
+        int rc = dense_impl(n, d, dA.as<double>(), n, db.as<double>(), dx.as<double>(), o,
+                            info ? *info : tmp, cpqr);
+        SKL_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * d, hipMemcpyDeviceToHost, s));
+        SKL_HIP(hipStreamSynchronize(s));
+        return rc;
+    });
+}
+
+static Pattern pattern_from(long m, long n, long k, const long *rows, const double *vals) {
+    Pattern P;
+    P.m = m;
+    P.n = n;
+    P.k = k;
+    P.rows.resize((size_t)(n * k));
+    P.vals.assign(vals, vals + n * k);
+    for (long q = 0; q < n * k; ++q) {
+        SKL_REQUIRE(rows[q] >= 0 && rows[q] < m, SKL_E_ARG, "pattern row out of range");
+        P.rows[(size_t)q] = (int32_t)rows[q];
+    }
+    build_buckets(P);
+    return P;
+}
+
+template <class T>
+static void download_as_double(const T *src, long rows, long cols, long lds, double *dst, long ldd) {
+    std::vector<T> h((size_t)rows * cols);
+    SKL_HIP(hipMemcpy2D(h.data(), sizeof(T) * rows, src, sizeof(T) * lds, sizeof(T) * rows, cols,
+                        hipMemcpyDeviceToHost));
+    for (long j = 0; j < cols; ++j)
+        for (long i = 0; i < rows; ++i) dst[i + j * ldd] = (double)h[(size_t)(i + j * rows)];
+}
+
+template <class T>
+static int qr_component(long m, long n, long nfac, double *A, long lda, double *tau, long *jpvt,
+                        bool cpqr) {
+    Arena &ar = ctx().arena;
+    ar.reset();
+    T *W = ar.alloc_n<T>((size_t)m * n);
+    std::vector<T> h((size_t)m * n);
+    for (long j = 0; j < n; ++j)
+        for (long i = 0; i < m; ++i) h[(size_t)(i + j * m)] = (T)A[i + j * lda];
+    SKL_HIP(hipMemcpy(W, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    T *dt = ar.alloc_n<T>((size_t)nfac + 1);
+    int32_t *jp = ar.alloc_n<int32_t>((size_t)nfac + 1);
+    if (cpqr) {
+        hipLaunchKernelGGL(k_iota, dim3(gblocks(nfac)), dim3(256), 0, ctx().stream, nfac, jp);
+        if (std::min(m, nfac) < 1000)
+            cpqr_greedy<T>(m, n, nfac, W, m, jp, dt, ar);
+        else
+            cpqr_hqrrp<T>(m, n, nfac, W, m, jp, dt, gaussian_for(m).data(), ar);
+    } else {
+        qr_householder<T>(m, n, nfac, W, m, dt, ar);
+    }
+    sync();
+    download_as_double(W, m, n, m, A, lda);
+    long kk = std::min(m, nfac);
+    std::vector<T> ht((size_t)kk);
+    SKL_HIP(hipMemcpy(ht.data(), dt, sizeof(T) * kk, hipMemcpyDeviceToHost));
+    for (long i = 0; i < kk; ++i) tau[i] = (double)ht[(size_t)i];
+    if (cpqr && jpvt) {
+        std::vector<int32_t> hj((size_t)nfac);
+        SKL_HIP(hipMemcpy(hj.data(), jp, sizeof(int32_t) * nfac, hipMemcpyDeviceToHost));
+        for (long i = 0; i < nfac; ++i) jpvt[i] = hj[(size_t)i] + 1;
+    }
+    return SKL_OK;
+}
+
+} // namespace skl
+
+extern "C" {
+
+void skl_default_opts(skl_opts *o) {
+    std::memset(o, 0, sizeof *o);
+    // include/bench_config.hpp:7-20 defaults
+    o->gamma = 1.1;
+    o->k = 3;
+    o->abs_tol = 1e-8;
+    o->rcond = 1e-10;
+    o->it_tol = 1e-6;
+    o->max_it = 10000;
+    o->rcond_threshold = 1e-10;
+    o->perturb = 1e-10;
+    o->ordering = 2;
+    o->seed = 1;
+    o->precision = SKL_FP64;
+    o->escalate_cond = 1e6;
+    o->use_graph = 1;
+    o->graph_chunk = 16;
+}
+
+int skl_init(int device) {
+    return guarded([&] {
+        skl::set_device(device);
+        skl::ensure_init();
+        return SKL_OK;
+    });
+}
+
+void skl_finalize(void) {}
+
+const char *skl_error_string(int code) {
+    switch (code) {
+    case SKL_OK: return "ok";
+    case SKL_E_SHAPE: return "Matrix is not overdetermined.";
+    case SKL_E_RANK0: return "SPQR Factorization fails";
+    case SKL_E_NOMEM: return "out of device memory";
+    case SKL_E_HIP: return "HIP runtime error";
+    case SKL_E_ARG: return "invalid argument";
+    case SKL_E_UNSUPPORTED: return "unsupported size";
+    case SKL_E_NODEVICE: return "no GPU available";
+    default: return "unknown error";
+    }
+}
+
+const char *skl_last_error(void) { return skl::t_err.c_str(); }
+
+int skl_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+void skl_set_history(skl_lsqr_step *steps, long max_steps) {
+    skl::t_hist = steps;
+    skl::t_hist_len = steps ? max_steps : 0;
+}
+
+int skl_dense_solve_dev(long n, long d, const double *dA, long lda, const double *db, double *dx,
+                        const skl_opts *opts, skl_info *info) {
+    skl_info tmp;
+    return guarded([&] {
+        skl_opts o = resolve(opts);
+        return dense_impl(n, d, dA, lda, db, dx, o, info ? *info : tmp, true);
+    });
+}
+
+int skl_dense_solve_nocpqr_dev(long n, long d, const double *dA, long lda, const double *db,
+                               double *dx, const skl_opts *opts, skl_info *info) {
+    skl_info tmp;
+    return guarded([&] {
+        skl_opts o = resolve(opts);
+        return dense_impl(n, d, dA, lda, db, dx, o, info ? *info : tmp, false);
+    });
+}
+
+int skl_sparse_solve_dev(long n, long d, long nnz, const long *dcp, const long *dri,
+                         const double *dval, const double *db, double *dx, const skl_opts *opts,
+                         skl_info *info) {
+    skl_info tmp;
+    return guarded([&] {
+        skl_opts o = resolve(opts);
+        return sparse_impl(n, d, nnz, dcp, dri, dval, db, dx, o, info ? *info : tmp);
+    });
+}
+
+int skl_dense_solve(long n, long d, const double *A, long lda, const double *b, double *x,
+                    const skl_opts *opts, skl_info *info) {
+    return dense_host(n, d, A, lda, b, x, opts, info, true);
+}
+int skl_dense_solve_nocpqr(long n, long d, const double *A, long lda, const double *b, double *x,
+                           const skl_opts *opts, skl_info *info) {
+    return dense_host(n, d, A, lda, b, x, opts, info, false);
+}
+
+int skl_sparse_solve(long n, long d, long nnz, const long *colptr, const long *rowind,
+                     const double *val, const double *b, double *x, const skl_opts *opts,
+                     skl_info *info) {
+    skl_info tmp;
+    return guarded([&] {
+        if (n < d) throw Error(SKL_E_SHAPE, "Matrix is not overdetermined.");
+        SKL_REQUIRE(colptr && b && x && d > 0, SKL_E_ARG, "invalid arguments");
+        SKL_REQUIRE(colptr[0] == 0 && colptr[d] == nnz, SKL_E_ARG, "colptr inconsistent with nnz");
+        skl_opts o = resolve(opts);
+        ensure_init();
+        hipStream_t s = ctx().stream;
+        DevBuf dcp(sizeof(long) * (d + 1)), dri(sizeof(long) * nnz), dv(sizeof(double) * nnz),
+            db(sizeof(double) * n), dx(sizeof(double) * d);
+        SKL_HIP(hipMemcpyAsync(dcp.p, colptr, sizeof(long) * (d + 1), hipMemcpyHostToDevice, s));
+        if (nnz > 0) {
+            SKL_HIP(hipMemcpyAsync(dri.p, rowind, sizeof(long) * nnz, hipMemcpyHostToDevice, s));
+            SKL_HIP(hipMemcpyAsync(dv.p, val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+        }
+        SKL_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        SKL_HIP(hipStreamSynchronize(s));
+        int rc = sparse_impl(n, d, nnz, dcp.as<long>(), dri.as<long>(), dv.as<double>(),
+                             db.as<double>(), dx.as<double>(), o, info ? *info : tmp);
+        SKL_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * d, hipMemcpyDeviceToHost, s));
+        SKL_HIP(hipStreamSynchronize(s));
+        return rc;
+    });
+}
+
+// ---- component entry points ------------------------------------------------
+int skl_hash_pattern(unsigned seed, int dense, long m, long n, long k, double *D, long *rows,
+                     double *vals) {
+    return guarded([&] {
+        SKL_REQUIRE(m >= k && k >= 1 && n >= 0, SKL_E_ARG, "invalid pattern shape");
+        Glibc g;
+        g.seed(seed);
+        Pattern P;
+        if (dense) {
+            gen_signs(g, n, P.D);
+            if (D) std::memcpy(D, P.D.data(), sizeof(double) * n);
+        }
+        gen_hashing(g, m, n, k, P);
+        for (long q = 0; q < n * k; ++q) {
+            if (rows) rows[q] = P.rows[(size_t)q];
+            if (vals) vals[q] = P.vals[(size_t)q];
+        }
+        return SKL_OK;
+    });
+}
+
+int skl_dht(long n, long ncols, const double *X, long ldx, double *Y, long ldy, int precision) {
+    return guarded([&] {
+        ensure_init();
+        Arena &ar = ctx().arena;
+        ar.reset();
+        double *dX = ar.alloc_n<double>((size_t)n * ncols);
+        SKL_HIP(hipMemcpy2D(dX, sizeof(double) * n, X, sizeof(double) * ldx, sizeof(double) * n,
+                            ncols, hipMemcpyHostToDevice));
+        if (precision == SKL_MIXED) {
+            float *dY = ar.alloc_n<float>((size_t)n * ncols);
+            dht_cols_from_double<float>(n, ncols, dX, n, dY, n, ar);
+            sync();
+            download_as_double(dY, n, ncols, n, Y, ldy);
+        } else {
+            double *dY = ar.alloc_n<double>((size_t)n * ncols);
+            dht_cols_from_double<double>(n, ncols, dX, n, dY, n, ar);
+            sync();
+            download_as_double(dY, n, ncols, n, Y, ldy);
+        }
+        return SKL_OK;
+    });
+}
+
+int skl_hash_apply(long m, long n, long k, const long *rows, const double *vals, double alpha,
+                   long ncols, const double *X, long ldx, double *Y, long ldy, int precision) {
+    return guarded([&] {
+        ensure_init();
+        Arena &ar = ctx().arena;
+        ar.reset();
+        Pattern P = pattern_from(m, n, k, rows, vals);
+        DevPattern dp = upload_pattern(P, alpha, false, false, precision == SKL_MIXED, ar);
+        double *dX = ar.alloc_n<double>((size_t)n * ncols);
+        SKL_HIP(hipMemcpy2D(dX, sizeof(double) * n, X, sizeof(double) * ldx, sizeof(double) * n,
+                            ncols, hipMemcpyHostToDevice));
+        if (precision == SKL_MIXED) {
+            float *fX = ar.alloc_n<float>((size_t)n * ncols);
+            copy2d<double, float>(n, ncols, dX, n, fX, n);
+            float *dY = ar.alloc_n<float>((size_t)m * ncols);
+            hash_gather<float>(m, ncols, dp.bptr, dp.bsrc, dp.coef_f, fX, n, dY, m);
+            sync();
+            download_as_double(dY, m, ncols, m, Y, ldy);
+        } else {
+            double *dY = ar.alloc_n<double>((size_t)m * ncols);
+            hash_gather<double>(m, ncols, dp.bptr, dp.bsrc, dp.coef_d, dX, n, dY, m);
+            sync();
+            download_as_double(dY, m, ncols, m, Y, ldy);
+        }
+        return SKL_OK;
+    });
+}
+
+int skl_dense_sketch(long n, long d, const double *A, long lda, const double *b, long m, long k,
+                     const double *D, const long *rows, const double *vals, double *SA, long ldsa,
+                     int precision) {
+    return guarded([&] {
+        ensure_init();
+        Arena &ar = ctx().arena;
+        ar.reset();
+        Pattern P = pattern_from(m, n, k, rows, vals);
+        P.D.assign(D, D + n);
+        bool mixed = precision == SKL_MIXED;
+        DevPattern dp = upload_pattern(P, 1 / std::sqrt((double)k), true, false, mixed, ar);
+        double *dA = ar.alloc_n<double>((size_t)n * d);
+        SKL_HIP(hipMemcpy2D(dA, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, d,
+                            hipMemcpyHostToDevice));
+        double *db = nullptr;
+        if (b) {
+            db = ar.alloc_n<double>((size_t)n);
+            SKL_HIP(hipMemcpy(db, b, sizeof(double) * n, hipMemcpyHostToDevice));
+        }
+        long nc = d + (b ? 1 : 0);
+        double scale = 1 / std::sqrt((double)n);
+        if (mixed) {
+            float *W = ar.alloc_n<float>((size_t)m * nc);
+            dht_sketch<float>(n, d, dA, n, dp.D, scale, m, dp.bptr, dp.bsrc, dp.coef_f, W, m, ar);
+            if (b) dht_sketch<float>(n, 1, db, n, dp.D, scale, m, dp.bptr, dp.bsrc, dp.coef_f, W + d * m, m, ar);
+            sync();
+            download_as_double(W, m, nc, m, SA, ldsa);
+        } else {
+            double *W = ar.alloc_n<double>((size_t)m * nc);
+            dht_sketch<double>(n, d, dA, n, dp.D, scale, m, dp.bptr, dp.bsrc, dp.coef_d, W, m, ar);
+            if (b) dht_sketch<double>(n, 1, db, n, dp.D, scale, m, dp.bptr, dp.bsrc, dp.coef_d, W + d * m, m, ar);
+            sync();
+            download_as_double(W, m, nc, m, SA, ldsa);
+        }
+        return SKL_OK;
+    });
+}
+
+int skl_sparse_sketch(long m, long n, long d, long k, const long *rows, const double *vals,
+                      const long *colptr, const long *rowind, const double *val, const double *b,
+                      double *SA, long ldsa, double *bs) {
+    return guarded([&] {
+        ensure_init();
+        Arena &ar = ctx().arena;
+        ar.reset();
+        Pattern P = pattern_from(m, n, k, rows, vals);
+        DevPattern dp = upload_pattern(P, 1.0, false, true, false, ar);
+        long nnz = colptr[d];
+        int32_t *cp = ar.alloc_n<int32_t>((size_t)d + 1);
+        int32_t *ri = ar.alloc_n<int32_t>((size_t)std::max<long>(nnz, 1));
+        double *dv = ar.alloc_n<double>((size_t)std::max<long>(nnz, 1));
+        std::vector<int32_t> hcp(colptr, colptr + d + 1), hri(rowind, rowind + nnz);
+        SKL_HIP(hipMemcpy(cp, hcp.data(), sizeof(int32_t) * (d + 1), hipMemcpyHostToDevice));
+        if (nnz > 0) {
+            SKL_HIP(hipMemcpy(ri, hri.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+            SKL_HIP(hipMemcpy(dv, val, sizeof(double) * nnz, hipMemcpyHostToDevice));
+        }
+        double *W = ar.alloc_n<double>((size_t)m * d);
+        sparse_sketch_t<double>(m, d, k, dp.rows, dp.vals, cp, ri, dv, nnz, W, m, ar);
+        if (b && bs) {
+            double *db = ar.alloc_n<double>((size_t)n);
+            double *dbs = ar.alloc_n<double>((size_t)m);
+            SKL_HIP(hipMemcpy(db, b, sizeof(double) * n, hipMemcpyHostToDevice));
+            hash_gather<double>(m, 1, dp.bptr, dp.bsrc, dp.coef_d, db, n, dbs, m);
+            sync();
+            SKL_HIP(hipMemcpy(bs, dbs, sizeof(double) * m, hipMemcpyDeviceToHost));
+        }
+        sync();
+        download_as_double(W, m, d, m, SA, ldsa);
+        return SKL_OK;
+    });
+}
+
+int skl_qr(long m, long n, long nfac, double *A, long lda, double *tau, int precision) {
+    return guarded([&] {
+        ensure_init();
+        SKL_REQUIRE(nfac <= n && m > 0 && lda >= m, SKL_E_ARG, "invalid QR shape");
+        return precision == SKL_MIXED ? qr_component<float>(m, n, nfac, A, lda, tau, nullptr, false)
+                                      : qr_component<double>(m, n, nfac, A, lda, tau, nullptr, false);
+    });
+}
+
+int skl_cpqr(long m, long n, long nfac, double *A, long lda, long *jpvt, double *tau,
+             int precision) {
+    return guarded([&] {
+        ensure_init();
+        SKL_REQUIRE(nfac <= n && m > 0 && lda >= m, SKL_E_ARG, "invalid QR shape");
+        return precision == SKL_MIXED ? qr_component<float>(m, n, nfac, A, lda, tau, jpvt, true)
+                                      : qr_component<double>(m, n, nfac, A, lda, tau, jpvt, true);
+    });
+}
+
+int skl_trinv(long r, const double *R, long ldr, double perturb, double *Minv, long ldm) {
+    return guarded([&] {
+        ensure_init();
+        Arena &ar = ctx().arena;
+        ar.reset();
+        double *dR = ar.alloc_n<double>((size_t)r * r);
+        double *dM = ar.alloc_n<double>((size_t)r * r);
+        SKL_HIP(hipMemcpy2D(dR, sizeof(double) * r, R, sizeof(double) * ldr, sizeof(double) * r, r,
+                            hipMemcpyHostToDevice));
+        trinv_upper(r, dR, r, perturb, dM, r, ar);
+        sync();
+        SKL_HIP(hipMemcpy2D(Minv, sizeof(double) * ldm, dM, sizeof(double) * r, sizeof(double) * r,
+                            r, hipMemcpyDeviceToHost));
+        return SKL_OK;
+    });
+}
+
+int skl_lsqr_dense(long n, long d, const double *A, long lda, long r, const long *E,
+                   const double *M, long ldm, const double *b, double tol, long max_it, double *y,
+                   int *flag, long *it, skl_lsqr_step *hist) {
+    return guarded([&] {
+        ensure_init();
+        Arena &ar = ctx().arena;
+        ar.reset();
+        double *dA = ar.alloc_n<double>((size_t)n * d);
+        double *dM = ar.alloc_n<double>((size_t)r * r);
+        double *db = ar.alloc_n<double>((size_t)n);
+        double *dy = ar.alloc_n<double>((size_t)r);
+        SKL_HIP(hipMemcpy2D(dA, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, d,
+                            hipMemcpyHostToDevice));
+        // M: upper triangle, zeros below (the triangular GEMVs read the square)
+        std::vector<double> hm((size_t)r * r, 0.0);
+        for (long j = 0; j < r; ++j)
+            for (long i = 0; i <= j; ++i) hm[(size_t)(i + j * r)] = M[i + j * ldm];
+        SKL_HIP(hipMemcpy(dM, hm.data(), sizeof(double) * r * r, hipMemcpyHostToDevice));
+        SKL_HIP(hipMemcpy(db, b, sizeof(double) * n, hipMemcpyHostToDevice));
+        int32_t *dE = nullptr;
+        if (E) {
+            std::vector<int32_t> he(E, E + r);
+            dE = ar.alloc_n<int32_t>((size_t)r);
+            SKL_HIP(hipMemcpy(dE, he.data(), sizeof(int32_t) * r, hipMemcpyHostToDevice));
+        }
+        LsqrProblem p;
+        p.n = n;
+        p.d = d;
+        p.r = r;
+        p.A = dA;
+        p.lda = n;
+        p.E = dE;
+        p.M = dM;
+        p.ldm = r;
+        LsqrResult res = lsqr_solve(p, db, tol, max_it, dy, true, 16, hist, hist ? max_it : 0, ar);
+        SKL_HIP(hipMemcpy(y, dy, sizeof(double) * r, hipMemcpyDeviceToHost));
+        if (flag) *flag = res.flag;
+        if (it) *it = res.it;
+        return SKL_OK;
+    });
+}
+
+} // extern "C"

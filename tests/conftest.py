@@ -1,0 +1,90 @@
+import importlib.util
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TESTS = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(TESTS, "golden")
+sys.path.insert(0, TESTS)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+def load_skl():
+    spec = importlib.util.spec_from_file_location(
+        "skl", os.path.join(ROOT, "ski-lls_amd", "skl.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="session")
+def skl():
+    return load_skl()
+
+
+@pytest.fixture(scope="session")
+def O():
+    import oracle_py
+    oracle_py.lib()
+    return oracle_py
+
+
+@pytest.fixture(scope="session")
+def kat():
+    with open(os.path.join(GOLDEN, "kat.json")) as fh:
+        return json.load(fh)
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if not (np.iscomplexobj(a) or np.iscomplexobj(b)):
+        a = a.astype(np.float64)
+        b = b.astype(np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def read_mtx(path):
+    """MatrixMarket reader returning a dense float64 array; fat matrices are
+    transposed as the reference's drivers do (test/ls_blendenpik_hashing_sparse_input.cpp:43-48)."""
+    import scipy.io
+    import scipy.sparse
+    M = scipy.io.mmread(path)
+    M = M.toarray() if scipy.sparse.issparse(M) else np.asarray(M, dtype=np.float64)
+    if M.shape[0] < M.shape[1]:
+        M = M.T
+    return np.asfortranarray(M)
+
+
+def c1_problem(seed=0, n=4000, d=200):
+    """Dense problem of the 4000x200 configuration: iid N(0,1) entries times
+    column scaling logspace(0,-4,d); b = A x* + 1e-8 noise."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((n, d)) * np.logspace(0, -4, d)
+    xs = rng.standard_normal(d)
+    b = A @ xs + 1e-8 * rng.standard_normal(n)
+    return np.asfortranarray(A), b
+
+
+def sparse_problem(seed=0, n=3000, d=100, density=0.02, decades=8):
+    """Sparse problem of the 120000x5000 family at reduced size: uniform
+    positions, N(0,1) values, columns scaled by logspace(0,-decades,d), unit
+    diagonal on the top d x d block, b = A x* + 1e-8 noise."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    A = sp.random(n, d, density=density, random_state=rng, format="csc",
+                  data_rvs=rng.standard_normal)
+    A = A @ sp.diags(np.logspace(0, -decades, d))
+    A = sp.csc_matrix(A + sp.eye(n, d, format="csc"))
+    A.sort_indices()
+    xs = rng.standard_normal(d)
+    b = A @ xs + 1e-8 * rng.standard_normal(n)
+    return A, b
