@@ -1,0 +1,280 @@
+#!/usr/bin/env python3
+"""Seconds per least-squares solve of the sketch-and-precondition solver
+(ls_dense_hashing_blendenpik) on one MI355X, inputs resident in HBM.
+
+Workload (default): dense 40000 x 4000, A = diag(w) G diag(logspace(0,-6,4000))
+with G iid N(0,1) and w_i lognormal(0,2); b = A x* + 1e-8 N(0,1); hashing
+sketch s=1 with m = 6800 rows after the randomized DHT; LSQR tol 1e-10.
+One step = one full solve (pattern, sketch, CPQR, x0, LSQR, back-transform).
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import importlib.util
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_skl():
+    spec = importlib.util.spec_from_file_location("skl", os.path.join(ROOT, "ski-lls_amd", "skl.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    mod.lib()
+    return mod
+
+
+WORKLOADS = {
+    "dense_40000x4000": dict(kind="dense", n=40000, d=4000, decades=6, lognormal=2.0, gamma=1.7,
+                             k=1),
+    "dense_4000x200": dict(kind="dense", n=4000, d=200, decades=4, lognormal=0.0, gamma=1.7, k=1),
+    "sparse_120000x5000": dict(kind="sparse", n=120000, d=5000, decades=8, density=0.01,
+                               gamma=1.5, k=3),
+}
+
+
+def make_dense(torch, W, seed):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    n, d = W["n"], W["d"]
+    cs = torch.logspace(0, -W["decades"], d, dtype=torch.float64, device="cuda")
+    # column-major A (n x d) is the row-major (d x n) tensor At
+    At = torch.randn(d, n, generator=g, dtype=torch.float64, device="cuda")
+    At *= cs[:, None]
+    if W["lognormal"] > 0:
+        w = torch.exp(W["lognormal"] * torch.randn(n, generator=g, dtype=torch.float64,
+                                                   device="cuda"))
+        At *= w[None, :]
+    xs = torch.randn(d, generator=g, dtype=torch.float64, device="cuda")
+    b = At.t() @ xs + 1e-8 * torch.randn(n, generator=g, dtype=torch.float64, device="cuda")
+    return At.contiguous(), b.contiguous()
+
+
+def make_sparse_host(W, seed):
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    n, d = W["n"], W["d"]
+    A = sp.random(n, d, density=W["density"], random_state=rng, format="csc",
+                  data_rvs=rng.standard_normal)
+    A = A @ sp.diags(np.logspace(0, -W["decades"], d))
+    A = sp.csc_matrix(A + sp.eye(n, d, format="csc"))
+    A.sort_indices()
+    xs = rng.standard_normal(d)
+    b = A @ xs + 1e-8 * rng.standard_normal(n)
+    return A, b
+
+
+def two_norm_est(torch, At, iters=30):
+    """||A||_2 by power iteration on A^T A (A = At^T)."""
+    v = torch.ones(At.shape[0], dtype=torch.float64, device="cuda")
+    v /= v.norm()
+    s = 0.0
+    for _ in range(iters):
+        u = At.t() @ v
+        v = At @ u
+        s = v.norm().item()
+        v /= s
+    return math.sqrt(s)
+
+
+def cpu_baseline_dense(skl_unused, At_host, b_host, W, gpu_iters, budget_s):
+    """The oracle (plain-C restatement, 1 thread) on a bounded sample of the
+    same solve, scaled to one solve: the randomized transform + hashing of a
+    column subset (scaled by column count), the HQRRP CPQR of an m x c_s
+    block (scaled by the 2 m c^2 - 2/3 c^3 flop count) and two LSQR
+    iterations over the full A (times the GPU's iteration count, plus one
+    GEMV for the x0 residual)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    n, d = W["n"], W["d"]
+    m = int(math.ceil(d * W["gamma"]))
+    A = At_host.T  # F-ordered n x d view
+    t0 = time.perf_counter()
+    cs = max(2, min(d, 200))
+    D, rows, vals = O.dense_pattern(1234, n, m, W["k"])
+    ts = time.perf_counter()
+    Y = O.fut(A[:, :cs], D)
+    O.hash_apply(m, rows, vals, W["k"], 1 / math.sqrt(W["k"]), Y)
+    t_sketch = (time.perf_counter() - ts) * d / cs
+    del Y
+    cq = max(2, min(d, 500))
+    rng = np.random.default_rng(0)
+    S = rng.standard_normal((m, cq))
+    tq = time.perf_counter()
+    O.hqrrp(S, 64, 10)
+    flops = lambda c: 2.0 * m * c * c - 2.0 / 3.0 * c ** 3
+    t_qr = (time.perf_counter() - tq) * flops(d) / flops(cq)
+    tl = time.perf_counter()
+    O.lsqr_dense(A, np.eye(d), b_host, 0.0, 2)
+    t_it = (time.perf_counter() - tl) / 2.0
+    t_lsqr = t_it * (gpu_iters + 0.5)
+    total = t_sketch + t_qr + t_lsqr
+    spent = time.perf_counter() - t0
+    return {
+        "value": total, "unit": "sec/solve", "cores": 1, "kind": "port",
+        "sample": (f"oracle on the host: DHT+hash of {cs}/{d} columns (x{d / cs:.0f}), HQRRP of "
+                   f"{m}x{cq} (x{flops(d) / flops(cq):.1f} by flops), 2 LSQR iterations on the "
+                   f"full A (x{gpu_iters + 0.5:.1f}); {spent:.1f}s of CPU work"),
+        "phases_s": {"sketch": t_sketch, "cpqr": t_qr, "lsqr_per_it": t_it},
+    }
+
+
+def run_dense(args, skl, torch, W):
+    n, d = W["n"], W["d"]
+    log(f"generating {n}x{d} dense problem on the GPU")
+    At, b = make_dense(torch, W, seed=2024)
+    x = torch.zeros(d, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    kw = dict(gamma=W["gamma"], k=W["k"], it_tol=1e-10, rcond=1e-12, abs_tol=1e-8, seed=1234)
+
+    def solve(precision):
+        return skl.dense_solve_dev(n, d, At.data_ptr(), n, b.data_ptr(), x.data_ptr(), cpqr=True,
+                                   precision=precision, **kw)
+
+    res = {}
+    for prec, name in [(skl.SKL_FP64, "fp64"), (skl.SKL_MIXED, "mixed")]:
+        for _ in range(args.warmup):
+            solve(prec)
+        if prec == skl.SKL_FP64:
+            skl.profile(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        infos = []
+        lat = []
+        for _ in range(args.steps):
+            ts = time.perf_counter()
+            infos.append(solve(prec))
+            lat.append(time.perf_counter() - ts)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        if prec == skl.SKL_FP64:
+            skl.profile(False, reset=False)
+            res["prof"] = skl.profile_read()
+        info = infos[-1]
+        r = b - At.t() @ x
+        res[name] = {"sec_per_solve": (t1 - t0) / args.steps, "lat": lat, "it": info.it,
+                     "flag": info.flag, "rank": info.rank, "escalated": info.escalated,
+                     "lsqr_est": info.lsqr_nrm_ar / (info.lsqr_nrm_a * info.lsqr_nrm_r),
+                     "r_norm": r.norm().item(), "ATr": (At @ r).norm().item(),
+                     "phases": {k: getattr(info, k) for k in
+                                ["t_pattern", "t_sketch", "t_qr", "t_x0", "t_lsqr"]},
+                     "x": x.clone()}
+        log(f"{name}: {res[name]['sec_per_solve'] * 1e3:.2f} ms/solve, it={info.it}, "
+            f"flag={info.flag}, phases={res[name]['phases']}")
+    nA = two_norm_est(torch, At)
+    for name in ["fp64", "mixed"]:
+        R = res[name]
+        R["ne_residual"] = R["ATr"] / (nA * R["r_norm"])
+    res["x_rel_mixed_vs_fp64"] = ((res["mixed"]["x"] - res["fp64"]["x"]).norm() /
+                                  res["fp64"]["x"].norm()).item()
+    return At, b, res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="dense_40000x4000", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the measurement, copy 1 GiB device-to-device three times "
+                         "(reference kernel for scaling HBM byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the solve shards "
+              "as described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(3)
+    skl = load_skl()
+    W = WORKLOADS[args.workload]
+    if W["kind"] != "dense":
+        print("only dense workloads are wired into this entry point", file=sys.stderr)
+        sys.exit(2)
+    At, b, res = run_dense(args, skl, torch, W)
+    prof = res["prof"]
+    dom = max(prof, key=lambda s: prof[s]["avg_ms"])
+    P = prof[dom]
+    achieved = P["bytes"] / (P["avg_ms"] * 1e-3) / 1e9 if P["avg_ms"] > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tj = json.load(fh)
+        traffic = tj.get(args.workload, {}).get(str(dom))
+    f64 = res["fp64"]
+    mixed = res["mixed"]
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle, 1 thread, bounded sample)")
+        At_h = At.cpu().numpy()
+        b_h = b.cpu().numpy()
+        cpu = cpu_baseline_dense(skl, At_h, b_h, W, f64["it"], args.cpu_budget)
+        log(f"cpu baseline: {cpu['value']:.1f} s/solve ({cpu['sample']})")
+    lat = sorted(f64["lat"])
+    out = {
+        "metric": "sec/solve to rel. normal-eq residual 1e-10, mixed vs FP64 on GPU; p99 latency",
+        "value": f64["sec_per_solve"],
+        "unit": "sec/solve",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": f64["sec_per_solve"] * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "solver": "ls_dense_hashing_blendenpik",
+                   "n": W["n"], "d": W["d"], "m": int(math.ceil(W["d"] * W["gamma"])),
+                   "k": W["k"], "it_tol": 1e-10, "rcond": 1e-12,
+                   "generator": f"diag(w) G diag(logspace(0,-{W['decades']},{W['d']})), "
+                                f"w lognormal(0,{W['lognormal']})"},
+        "lsqr_iterations": f64["it"],
+        "lsqr_estimate": f64["lsqr_est"],
+        "ne_residual_recomputed": f64["ne_residual"],
+        "p50_latency_s": lat[len(lat) // 2],
+        "p99_latency_s": lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)],
+        "phases_s": f64["phases"],
+        "mixed": {"sec_per_solve": mixed["sec_per_solve"], "lsqr_iterations": mixed["it"],
+                  "escalated": mixed["escalated"], "lsqr_estimate": mixed["lsqr_est"],
+                  "ne_residual_recomputed": mixed["ne_residual"],
+                  "speedup_vs_fp64": f64["sec_per_solve"] / mixed["sec_per_solve"],
+                  "x_rel_vs_fp64": res["x_rel_mixed_vs_fp64"], "phases_s": mixed["phases"]},
+        "roofline": {"bound": "hbm", "kernel": P["name"], "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "avg_launch_ms": P["avg_ms"],
+                     "launches_timed": P["launches"], "bytes_per_launch": P["bytes"],
+                     "other_kernels": {v["name"]: {"avg_ms": v["avg_ms"],
+                                                   "GB/s": (v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9
+                                                            if v["avg_ms"] > 0 else 0.0)}
+                                       for s, v in prof.items() if s != dom}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    if args.calibrate_copy:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x 1 GiB device copies done")
+
+
+if __name__ == "__main__":
+    main()

@@ -16,6 +16,8 @@
 #ifndef SKL_H
 #define SKL_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -106,9 +108,36 @@ int skl_sparse_solve_dev(long n, long d, long nnz, const long *dcolptr, const lo
                          const double *dval, const double *db, double *dx,
                          const skl_opts *opts, skl_info *info);
 
+/* Kernel timing of the LSQR operator products on this thread: when enabled,
+ * HIP events on the library stream bracket the four products of the first
+ * iteration of every replayed chunk.  slot 0: A z, 1: A^T u, 2: M v,
+ * 3: M^T t.  bytes_per_launch is the algorithmic traffic of one launch. */
+void skl_profile_enable(int on);
+void skl_profile_reset(void);
+int skl_profile_read(int slot, double *avg_ms, long *launches, double *bytes_per_launch);
+
 /* LSQR per-iteration history of the next solve on this thread (optional):
  * steps must hold max_steps entries; pass NULL to switch it off. */
 void skl_set_history(skl_lsqr_step *steps, long max_steps);
+
+/* ---- serving: continuous batching over worker slots -------------------------
+ * No counterpart in the reference (it solves one problem per call); this is
+ * the request loop that callers with a stream of problems use.  Each slot is
+ * a worker with its own HIP stream and device arena; queued requests are
+ * admitted in arrival order when a slot is free and their device-memory
+ * footprint fits mem_budget_bytes (0: 85% of free device memory).  Input
+ * arrays must stay valid until skl_server_wait returns for the ticket. */
+typedef struct skl_server skl_server;
+skl_server *skl_server_create(int slots, size_t mem_budget_bytes, int device);
+long skl_server_submit_dense(skl_server *s, int cpqr, long n, long d, const double *A, long lda,
+                             const double *b, double *x, const skl_opts *opts);
+long skl_server_submit_sparse(skl_server *s, long n, long d, long nnz, const long *colptr,
+                              const long *rowind, const double *val, const double *b, double *x,
+                              const skl_opts *opts);
+/* blocks until the request finished; queue_s / run_s: time queued / solving */
+int skl_server_wait(skl_server *s, long ticket, skl_info *info, double *queue_s, double *run_s);
+int skl_server_pending(skl_server *s);
+void skl_server_destroy(skl_server *s);
 
 /* ---- component entry points (host pointers; each runs one stage) -----------
  * Hashing pattern in the reference's draw order: when dense != 0 the first n

@@ -417,6 +417,302 @@ static bool smooth(long x) {
     return x == 1;
 }
 
+// ---------------------------------------------------------------------------
+// Generic batched complex FFT (in place), used by Bluestein's algorithm for
+// lengths with a prime factor above 13.  Optional pointwise pre-op
+// (multiply by a table, then conjugate) and post-op (conjugate, scale,
+// multiply by a table; only the first nout outputs are written to `out`).
+// ---------------------------------------------------------------------------
+template <class T> struct CfftPlan {
+    long L = 0;
+    int mode = 0;            // 1 single pass, 2 four-step
+    int L1 = 0, L2 = 0, J2 = 0, K1 = 0;
+    FftDims d1{}, d2{};
+    cplx<T> *tw1 = nullptr, *tw2 = nullptr, *twL = nullptr;
+};
+
+template <class T> struct CfftOps {
+    const cplx<T> *pre_tab = nullptr;
+    int pre_conj = 0;
+    int post_conj = 0;
+    T post_scale = T(1);
+    const cplx<T> *post_tab = nullptr;
+};
+
+template <class T>
+__device__ inline cplx<T> pre_op(cplx<T> v, long j, const CfftOps<T> &o) {
+    if (o.pre_tab) v = cmul(v, o.pre_tab[j]);
+    if (o.pre_conj) v.y = -v.y;
+    return v;
+}
+template <class T>
+__device__ inline cplx<T> post_op(cplx<T> v, long k, const CfftOps<T> &o) {
+    if (o.post_conj) v.y = -v.y;
+    v.x *= o.post_scale;
+    v.y *= o.post_scale;
+    if (o.post_tab) v = cmul(v, o.post_tab[k]);
+    return v;
+}
+
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void k_cfft_single(int L, const cplx<T> *__restrict__ Z, long zs,
+                                                    FftDims dims, const cplx<T> *__restrict__ tw,
+                                                    CfftOps<T> ops, cplx<T> *__restrict__ out,
+                                                    long os, long nout) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const cplx<T> *z = Z + (long)blockIdx.x * zs;
+    for (int j = threadIdx.x; j < L; j += NT) buf[j] = pre_op(z[j], j, ops);
+    __syncthreads();
+    lds_fft<T, NT>(buf, 1, dims, tw);
+    cplx<T> *o = out + (long)blockIdx.x * os;
+    for (long k = threadIdx.x; k < nout; k += NT) o[k] = post_op(buf[k], k, ops);
+}
+
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void k_cfft4_p1(long L, int L1, int L2, int J2,
+                                                 const cplx<T> *__restrict__ Z, long zs, FftDims dims,
+                                                 const cplx<T> *__restrict__ tw1,
+                                                 const cplx<T> *__restrict__ twL, CfftOps<T> ops,
+                                                 cplx<T> *__restrict__ Y) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const long b = blockIdx.y;
+    const int j2b = blockIdx.x * J2;
+    const int nJ = min(J2, L2 - j2b);
+    const cplx<T> *z = Z + b * zs;
+    for (int e = threadIdx.x; e < L1 * J2; e += NT) {
+        int jj = e % J2, j1 = e / J2;
+        cplx<T> v = {T(0), T(0)};
+        if (jj < nJ) {
+            long j = (long)j1 * L2 + j2b + jj;
+            v = pre_op(z[j], j, ops);
+        }
+        buf[j1 * J2 + jj] = v;
+    }
+    __syncthreads();
+    lds_fft<T, NT>(buf, J2, dims, tw1);
+    cplx<T> *y = Y + b * L;
+    for (int e = threadIdx.x; e < L1 * J2; e += NT) {
+        int jj = e % J2, k1 = e / J2;
+        if (jj < nJ) {
+            int j2 = j2b + jj;
+            y[(long)k1 * L2 + j2] = cmul(buf[k1 * J2 + jj], twL[((long)j2 * k1) % L]);
+        }
+    }
+}
+
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void k_cfft4_p2(long L, int L1, int L2, int K1,
+                                                 const cplx<T> *__restrict__ Y, FftDims dims,
+                                                 const cplx<T> *__restrict__ tw2, CfftOps<T> ops,
+                                                 cplx<T> *__restrict__ out, long os, long nout) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const long b = blockIdx.y;
+    const int kb = blockIdx.x * K1;
+    const int nK = min(K1, L1 - kb);
+    const cplx<T> *y = Y + b * L;
+    for (int e = threadIdx.x; e < L2 * K1; e += NT) {
+        int kk = e % K1, j2 = e / K1;
+        cplx<T> v = {T(0), T(0)};
+        if (kk < nK) v = y[(long)(kb + kk) * L2 + j2];
+        buf[j2 * K1 + kk] = v;
+    }
+    __syncthreads();
+    lds_fft<T, NT>(buf, K1, dims, tw2);
+    cplx<T> *o = out + b * os;
+    for (int e = threadIdx.x; e < L2 * K1; e += NT) {
+        int kk = e % K1, k2 = e / K1;
+        if (kk >= nK) continue;
+        long k = (kb + kk) + (long)L1 * k2;
+        if (k < nout) o[k] = post_op(buf[k2 * K1 + kk], k, ops);
+    }
+}
+
+static bool smooth5(long x) {
+    for (int r : {2, 3, 5})
+        while (x % r == 0) x /= r;
+    return x == 1;
+}
+
+template <class T> static bool make_cplan(long L, CfftPlan<T> &p) {
+    const long cap = lds_elems<T>();
+    p.L = L;
+    if (L <= cap) {
+        p.mode = 1;
+        if (!factor_dims(L, p.d1)) return false;
+        p.tw1 = make_twiddles<T>(L);
+        return true;
+    }
+    long best = -1;
+    double bs = 1e300;
+    for (long a = 2; a <= L; ++a) {
+        if (L % a) continue;
+        long b = L / a;
+        if (a > cap / 4 || b > cap / 4) continue;
+        if (!smooth(a) || !smooth(b)) continue;
+        double sc = std::fabs(std::log((double)a) - std::log((double)b));
+        if (sc < bs) {
+            bs = sc;
+            best = a;
+        }
+    }
+    if (best < 0) return false;
+    p.mode = 2;
+    p.L1 = (int)best;
+    p.L2 = (int)(L / best);
+    factor_dims(p.L1, p.d1);
+    factor_dims(p.L2, p.d2);
+    p.tw1 = make_twiddles<T>(p.L1);
+    p.tw2 = make_twiddles<T>(p.L2);
+    p.twL = make_twiddles<T>(L);
+    p.J2 = (int)std::min<long>(32, cap / p.L1);
+    p.K1 = (int)std::min<long>(32, cap / p.L2);
+    return true;
+}
+
+// Batched in-place-capable complex FFT: out[b][k] = post(FFT(pre(Z[b])))[k], k < nout.
+template <class T>
+static void cfft(const CfftPlan<T> &P, long nseq, const cplx<T> *Z, long zs, const CfftOps<T> &ops,
+                 cplx<T> *out, long os, long nout, Arena &ar) {
+    constexpr int NT = fft_nt<T>();
+    hipStream_t s = ctx().stream;
+    if (nseq <= 0) return;
+    if (P.mode == 1) {
+        size_t sh = sizeof(cplx<T>) * (size_t)P.L;
+        k_cfft_single<T, NT><<<(unsigned)nseq, NT, sh, s>>>((int)P.L, Z, zs, P.d1, P.tw1, ops, out,
+                                                            os, nout);
+    } else {
+        cplx<T> *Y = ar.alloc_n<cplx<T>>((size_t)nseq * P.L);
+        dim3 g1((unsigned)((P.L2 + P.J2 - 1) / P.J2), (unsigned)nseq);
+        k_cfft4_p1<T, NT><<<g1, NT, sizeof(cplx<T>) * (size_t)P.L1 * P.J2, s>>>(
+            P.L, P.L1, P.L2, P.J2, Z, zs, P.d1, P.tw1, P.twL, ops, Y);
+        dim3 g2((unsigned)((P.L1 + P.K1 - 1) / P.K1), (unsigned)nseq);
+        k_cfft4_p2<T, NT><<<g2, NT, sizeof(cplx<T>) * (size_t)P.L2 * P.K1, s>>>(
+            P.L, P.L1, P.L2, P.K1, Y, P.d2, P.tw2, ops, out, os, nout);
+    }
+    SKL_CHECK_LAUNCH();
+}
+
+// Bluestein: X_k = w_k conj( FFT_L( conj( FFT_L(a) .* B ) ) )_k / L with
+// a_j = z_j w_j (j < n, zero-padded to L), w_j = exp(-i pi j^2 / n),
+// B = FFT_L(b), b_j = b_{L-j} = conj(w_j).
+template <class T> struct BsPlan {
+    long n = 0, L = 0;
+    CfftPlan<T> cp;
+    cplx<T> *w = nullptr; // n
+    cplx<T> *B = nullptr; // L
+};
+
+template <class T>
+__global__ void k_bs_pack(long n, long L, long ncols, const double *__restrict__ X, long ldx,
+                          const double *__restrict__ D, double scale, long pair0,
+                          const cplx<T> *__restrict__ w, cplx<T> *__restrict__ Z) {
+    const long p = blockIdx.y;
+    const long c0 = 2 * (pair0 + p), c1 = c0 + 1;
+    const bool has1 = c1 < ncols;
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < L; j += (long)gridDim.x * blockDim.x) {
+        cplx<T> v = {T(0), T(0)};
+        if (j < n) {
+            T re = load_scaled<T>(X, j + c0 * ldx, D, j, scale);
+            T im = has1 ? load_scaled<T>(X, j + c1 * ldx, D, j, scale) : T(0);
+            v = cmul(cplx<T>{re, im}, w[j]);
+        }
+        Z[p * L + j] = v;
+    }
+}
+
+// H (two real columns per complex sequence) from X (n per pair): DHT extraction
+template <class T>
+__global__ void k_dht_extract(long n, long ncols, long pair0, long npc, const cplx<T> *__restrict__ X,
+                              T *__restrict__ H, long ldh) {
+    const long p = blockIdx.y;
+    const long c0 = 2 * (pair0 + p), c1 = c0 + 1;
+    const bool has1 = c1 < ncols;
+    const cplx<T> *x = X + p * n;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        long nk = k == 0 ? 0 : n - k;
+        cplx<T> a = x[k], c = x[nk];
+        H[k + c0 * ldh] = T(0.5) * ((a.x + c.x) - (a.y - c.y));
+        if (has1) H[k + c1 * ldh] = T(0.5) * ((a.y + c.y) + (a.x - c.x));
+    }
+}
+
+template <class T> static const BsPlan<T> &get_bs_plan(long n) {
+    static std::map<long, BsPlan<T>> cache;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(n);
+    if (it != cache.end()) return it->second;
+    BsPlan<T> bp;
+    bp.n = n;
+    long L = 2 * n - 1;
+    while (true) {
+        if (smooth5(L) && make_cplan<T>(L, bp.cp)) break;
+        ++L;
+        SKL_REQUIRE(L < 16 * n + 64, SKL_E_UNSUPPORTED, "no Bluestein length found");
+    }
+    bp.L = L;
+    std::vector<cplx<T>> w((size_t)n), b((size_t)L, cplx<T>{T(0), T(0)});
+    const long double PI = 3.14159265358979323846264338327950288L;
+    for (long j = 0; j < n; ++j) {
+        long e = (long)(((__int128)j * j) % (2 * n));
+        long double ang = -PI * (long double)e / (long double)n;
+        w[(size_t)j] = {(T)cosl(ang), (T)sinl(ang)};
+        cplx<T> cj = {(T)cosl(ang), (T)-sinl(ang)};
+        b[(size_t)j] = cj;
+        if (j > 0) b[(size_t)(L - j)] = cj;
+    }
+    SKL_HIP(hipMalloc(&bp.w, sizeof(cplx<T>) * (size_t)n));
+    SKL_HIP(hipMalloc(&bp.B, sizeof(cplx<T>) * (size_t)L));
+    SKL_HIP(hipMemcpy(bp.w, w.data(), sizeof(cplx<T>) * (size_t)n, hipMemcpyHostToDevice));
+    SKL_HIP(hipMemcpy(bp.B, b.data(), sizeof(cplx<T>) * (size_t)L, hipMemcpyHostToDevice));
+    {
+        // B = FFT_L(b) once per length, in place
+        Arena tmp;
+        CfftOps<T> ops;
+        cfft<T>(bp.cp, 1, bp.B, L, ops, bp.B, L, L, tmp);
+        SKL_HIP(hipStreamSynchronize(ctx().stream));
+    }
+    return cache.emplace(n, bp).first->second;
+}
+
+// DHT of scaled column pairs through Bluestein, into H (n x ncols, T)
+template <class T>
+static void dht_bluestein(long n, long ncols, const double *X, long ldx, const double *D,
+                          double scale, T *H, long ldh, Arena &ar) {
+    const BsPlan<T> &bp = get_bs_plan<T>(n);
+    const long L = bp.L;
+    long npairs = (ncols + 1) / 2;
+    // pairs per chunk: about 512 MB of complex scratch
+    long per = (long)std::max<size_t>(1, ((size_t)512 << 20) / (sizeof(cplx<T>) * (size_t)(3 * L + n)));
+    hipStream_t s = ctx().stream;
+    for (long p0 = 0; p0 < npairs; p0 += per) {
+        long npc = std::min(per, npairs - p0);
+        size_t mark = ar.used();
+        cplx<T> *Z = ar.alloc_n<cplx<T>>((size_t)npc * L);
+        cplx<T> *Xc = ar.alloc_n<cplx<T>>((size_t)npc * n);
+        dim3 gp((unsigned)std::min<long>((L + 255) / 256, 1024), (unsigned)npc);
+        k_bs_pack<T><<<gp, 256, 0, s>>>(n, L, ncols, X, ldx, D, scale, p0, bp.w, Z);
+        SKL_CHECK_LAUNCH();
+        CfftOps<T> o1;
+        cfft<T>(bp.cp, npc, Z, L, o1, Z, L, L, ar);
+        CfftOps<T> o2;
+        o2.pre_tab = bp.B;
+        o2.pre_conj = 1;
+        o2.post_conj = 1;
+        o2.post_scale = T(1) / (T)L;
+        o2.post_tab = bp.w;
+        cfft<T>(bp.cp, npc, Z, L, o2, Xc, n, n, ar);
+        dim3 ge((unsigned)std::min<long>((n + 255) / 256, 1024), (unsigned)npc);
+        k_dht_extract<T><<<ge, 256, 0, s>>>(n, ncols, p0, npc, Xc, H, ldh);
+        SKL_CHECK_LAUNCH();
+        SKL_HIP(hipStreamSynchronize(s));
+        ar.rewind(mark);
+    }
+}
+
 template <class T> static const FftPlanDev<T> &get_plan(long n) {
     static std::map<long, FftPlanDev<T>> cache;
     static std::mutex mu;
@@ -426,11 +722,9 @@ template <class T> static const FftPlanDev<T> &get_plan(long n) {
     FftPlanDev<T> p;
     p.n = n;
     const long cap = lds_elems<T>();
-    if (!smooth(n))
-        throw Error(SKL_E_UNSUPPORTED,
-                    "DHT length " + std::to_string(n) +
-                        " has a prime factor above 13 (not implemented)");
-    if (n <= cap) {
+    if (!smooth(n)) {
+        p.mode = 3; // Bluestein
+    } else if (n <= cap) {
         p.mode = 1;
         factor_dims(n, p.d1);
         p.tw1 = make_twiddles<T>(n);
@@ -506,7 +800,10 @@ void dht_sketch(long n, long ncols, const double *X, long ldx, const double *D, 
         SKL_CHECK_LAUNCH();
     } else {
         T *H = ar.alloc_n<T>((size_t)n * ncols);
-        dht_fourstep<T>(P, ncols, X, ldx, D, scale, H, n, ar);
+        if (P.mode == 2)
+            dht_fourstep<T>(P, ncols, X, ldx, D, scale, H, n, ar);
+        else
+            dht_bluestein<T>(n, ncols, X, ldx, D, scale, H, n, ar);
         hash_gather<T>(m, ncols, bptr, bsrc, bcoef, H, n, SA, ldsa);
     }
 }
@@ -529,8 +826,10 @@ void dht_cols_from_double(long n, long ncols, const double *X, long ldx, T *Y, l
         k_dht_single<T, NT><<<(unsigned)npairs, NT, sh, ctx().stream>>>(
             n, ncols, X, ldx, nullptr, 1.0, P.d1, P.tw1, 1, 0, nullptr, nullptr, nullptr, Y, ldy);
         SKL_CHECK_LAUNCH();
-    } else {
+    } else if (P.mode == 2) {
         dht_fourstep<T>(P, ncols, X, ldx, nullptr, 1.0, Y, ldy, ar);
+    } else {
+        dht_bluestein<T>(n, ncols, X, ldx, nullptr, 1.0, Y, ldy, ar);
     }
 }
 template void dht_cols_from_double<double>(long, long, const double *, long, double *, long,

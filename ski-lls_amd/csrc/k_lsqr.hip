@@ -30,6 +30,9 @@ constexpr int GN_THREADS = 256, GN_ROWS = 2 * GN_THREADS, GN_MAXCW = 512;
 
 // P[c * n + i] = sum_{j in chunk c} A[i, j] z[j]; with upper != 0 only the
 // entries j >= i contribute (A upper triangular, zeros not read).
+// ROLE only names the instantiation (0: LSQR A z, 1: triangular M v, 2: other)
+// so that per-kernel profiles do not mix launch shapes.
+template <int ROLE>
 __global__ __launch_bounds__(GN_THREADS) void k_gemv_n_part(
     long n, long d, const double *__restrict__ A, long lda, const double *__restrict__ z, int cw,
     double *__restrict__ P, int upper, const LsqrState *__restrict__ st) {
@@ -87,6 +90,7 @@ constexpr int GT_THREADS = 256, GT_RB = 1024;
 
 // P[rb * d + j] = sum_{i in row block rb} A[i, j] u[i]; with upper != 0 only
 // rows i <= j contribute.
+template <int ROLE>
 __global__ __launch_bounds__(GT_THREADS) void k_gemv_t_part(
     long n, long d, const double *__restrict__ A, long lda, const double *__restrict__ u, int cpw,
     double *__restrict__ P, int upper, const LsqrState *__restrict__ st) {
@@ -412,46 +416,105 @@ struct LsqrBufs {
 };
 
 static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs &B,
-                        hipStream_t s, int init) {
+                        hipStream_t s, int init, hipEvent_t *ev = nullptr) {
+    auto rec = [&](int i) {
+        if (ev) SKL_HIP(hipEventRecord(ev[i], s));
+    };
     // Pt = A^T ut
+    rec(2);
     if (!q.sparse) {
         dim3 g((unsigned)q.nrb_a, (unsigned)((q.d + q.cpw_a - 1) / q.cpw_a));
-        k_gemv_t_part<<<g, GT_THREADS, 0, s>>>(q.n, q.d, p.A, p.lda, B.ut, q.cpw_a, B.Pt, 0, B.st);
+        k_gemv_t_part<0><<<g, GT_THREADS, 0, s>>>(q.n, q.d, p.A, p.lda, B.ut, q.cpw_a, B.Pt, 0, B.st);
     } else {
         k_csc_t<<<(unsigned)((q.d + 3) / 4), 256, 0, s>>>(q.d, p.csc_ptr, p.csc_row, p.csc_val,
                                                           B.ut, B.Pt, B.st);
     }
+    rec(3);
     // t = Pt(E) / beta
     k_reduce_t<<<(unsigned)((q.r + 255) / 256), 256, 0, s>>>(q.d, q.r, q.nrb_a, B.Pt, p.E, B.Sb,
                                                              q.nsb, B.t, B.st);
     // Pm = M^T t
     {
         dim3 g((unsigned)q.nrb_m, (unsigned)((q.r + q.cpw_m - 1) / q.cpw_m));
-        k_gemv_t_part<<<g, GT_THREADS, 0, s>>>(q.r, q.r, p.M, p.ldm, B.t, q.cpw_m, B.Pm, 1, B.st);
+        rec(6);
+        k_gemv_t_part<1><<<g, GT_THREADS, 0, s>>>(q.r, q.r, p.M, p.ldm, B.t, q.cpw_m, B.Pm, 1, B.st);
+        rec(7);
     }
     k_update<<<1, UPD_THREADS, 0, s>>>(q.r, q.nrb_m, B.Pm, B.Sb, q.nsb, B.v, B.w, B.x, B.st,
                                        B.hist, B.hist_cap, init);
 }
 
+// Optional kernel timing: events around the four operator products of one
+// iteration (slots: 0 A z, 1 A^T u, 2 M v, 3 M^T t), recorded on the stream
+// that runs them.
+struct ProfStats {
+    double ms[4] = {0, 0, 0, 0};
+    long cnt[4] = {0, 0, 0, 0};
+    double bytes[4] = {0, 0, 0, 0};
+};
+static thread_local bool g_prof_on = false;
+static thread_local ProfStats g_prof;
+static thread_local hipEvent_t g_ev[8] = {};
+
+static void prof_events() {
+    if (g_ev[0]) return;
+    for (auto &e : g_ev) SKL_HIP(hipEventCreate(&e));
+}
+
+static inline void rec(hipEvent_t *ev, int i, hipStream_t s) {
+    if (ev) SKL_HIP(hipEventRecord(ev[i], s));
+}
+
 static void launch_iteration(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs &B,
-                             hipStream_t s) {
+                             hipStream_t s, hipEvent_t *ev = nullptr) {
     // z = scatter_E(M v)
     {
         dim3 g((unsigned)((q.r + GN_ROWS - 1) / GN_ROWS), (unsigned)q.nck_m);
-        k_gemv_n_part<<<g, GN_THREADS, 0, s>>>(q.r, q.r, p.M, p.ldm, B.v, q.cw_m, B.Pz, 1, B.st);
+        rec(ev, 4, s);
+        k_gemv_n_part<1><<<g, GN_THREADS, 0, s>>>(q.r, q.r, p.M, p.ldm, B.v, q.cw_m, B.Pz, 1, B.st);
+        rec(ev, 5, s);
         k_z_reduce<<<(unsigned)((q.d + 255) / 256), 256, 0, s>>>(q.d, q.r, q.nck_m, B.Pz, B.Einv,
                                                                  B.z, B.st);
     }
     // ut = A z - coef ut ; Sb
     if (!q.sparse) {
         dim3 g((unsigned)((q.n + GN_ROWS - 1) / GN_ROWS), (unsigned)q.nck_a);
-        k_gemv_n_part<<<g, GN_THREADS, 0, s>>>(q.n, q.d, p.A, p.lda, B.z, q.cw_a, B.Pn, 0, B.st);
+        rec(ev, 0, s);
+        k_gemv_n_part<0><<<g, GN_THREADS, 0, s>>>(q.n, q.d, p.A, p.lda, B.z, q.cw_a, B.Pn, 0, B.st);
+        rec(ev, 1, s);
         k_reduce_n<<<(unsigned)q.nsb, 256, 0, s>>>(q.n, q.nck_a, B.Pn, B.ut, B.Sb, B.st, 0);
     } else {
+        rec(ev, 0, s);
         k_spmv_n<8><<<(unsigned)q.nsb, 256, 0, s>>>(q.n, p.csr_ptr, p.csr_col, p.csr_val, B.z, B.ut,
                                                     B.Sb, B.st, 0);
+        rec(ev, 1, s);
     }
-    launch_op_t(p, q, B, s, 0);
+    launch_op_t(p, q, B, s, 0, ev);
+}
+
+static void prof_accumulate(const LsqrPlan &q, long nnz) {
+    const int pairs[4][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}};
+    double r = (double)q.r, n = (double)q.n, d = (double)q.d;
+    double b[4];
+    if (!q.sparse) {
+        b[0] = 8.0 * (n * d + n + d);
+        b[1] = 8.0 * (n * d + n + d);
+    } else {
+        b[0] = 12.0 * (double)nnz + 8.0 * (n + 1) + 8.0 * n + 8.0 * d;
+        b[1] = 12.0 * (double)nnz + 8.0 * (d + 1) + 8.0 * n + 8.0 * d;
+    }
+    b[2] = 8.0 * (r * (r + 1) / 2 + 2 * r);
+    b[3] = b[2];
+    for (int k = 0; k < 4; ++k) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, g_ev[pairs[k][0]], g_ev[pairs[k][1]]) == hipSuccess) {
+            g_prof.ms[k] += ms;
+            g_prof.cnt[k] += 1;
+            g_prof.bytes[k] = b[k];
+        } else {
+            (void)hipGetLastError(); // do not leave a sticky error for later checks
+        }
+    }
 }
 
 __global__ void k_build_einv(long d, long r, const int32_t *E, int32_t *Einv) {
@@ -478,6 +541,11 @@ struct GraphKey {
     bool operator<(const GraphKey &o) const { return k < o.k; }
 };
 static thread_local std::map<GraphKey, hipGraphExec_t> g_graphs;
+
+void lsqr_release_graphs() {
+    for (auto &kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
+    g_graphs.clear();
+}
 
 LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long max_it,
                       double *y, bool use_graph, int chunk, skl_lsqr_step *hist, long hist_len,
@@ -522,6 +590,12 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
     HostFlag *hf = (HostFlag *)C.pinned;
     if (chunk <= 0) chunk = 16;
     bool graph_ok = use_graph;
+    const bool prof = g_prof_on;
+    hipEvent_t *ev = nullptr;
+    if (prof) {
+        prof_events();
+        ev = g_ev;
+    }
     hipGraphExec_t exec = nullptr;
     if (graph_ok) {
         GraphKey key;
@@ -530,14 +604,16 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
                  (long)(intptr_t)p.M, p.ldm, (long)(intptr_t)B.ut, (long)(intptr_t)B.v,
                  (long)(intptr_t)B.x, (long)(intptr_t)B.st, (long)(intptr_t)B.hist,
                  (long)(intptr_t)B.Pn, (long)(intptr_t)B.Pt, (long)(intptr_t)B.Pm,
-                 (long)(intptr_t)B.Einv, chunk, (long)(intptr_t)s};
+                 (long)(intptr_t)B.Einv, chunk, (long)(intptr_t)s, prof ? 1L : 0L};
         auto it = g_graphs.find(key);
         if (it != g_graphs.end()) {
             exec = it->second;
         } else {
             hipGraph_t graph;
             SKL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (int c = 0; c < chunk; ++c) launch_iteration(p, q, B, s);
+            // with timing on, the first iteration of a chunk runs eagerly
+            // (events cannot be timed inside a replayed graph)
+            for (int c = prof ? 1 : 0; c < chunk; ++c) launch_iteration(p, q, B, s);
             hipError_t e1 = hipMemcpyAsync(hf, &B.st->done, sizeof(int), hipMemcpyDeviceToHost, s);
             hipError_t e2 = hipStreamEndCapture(s, &graph);
             if (e1 != hipSuccess || e2 != hipSuccess) {
@@ -551,18 +627,21 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
         }
     }
     // device-side done flag read back once per chunk
+    SKL_HIP(hipMemcpyAsync(hf, &B.st->done, sizeof(int), hipMemcpyDeviceToHost, s));
+    SKL_HIP(hipStreamSynchronize(s));
     long launched = 0;
-    while (true) {
+    while (!hf->done) {
+        if (prof) launch_iteration(p, q, B, s, ev);
         if (graph_ok) {
             SKL_HIP(hipGraphLaunch(exec, s));
         } else {
-            for (int c = 0; c < chunk; ++c) launch_iteration(p, q, B, s);
+            for (int c = prof ? 1 : 0; c < chunk; ++c) launch_iteration(p, q, B, s);
             SKL_CHECK_LAUNCH();
             SKL_HIP(hipMemcpyAsync(hf, &B.st->done, sizeof(int), hipMemcpyDeviceToHost, s));
         }
         launched += chunk;
         SKL_HIP(hipStreamSynchronize(s));
-        if (hf->done) break;
+        if (prof) prof_accumulate(q, p.nnz);
         if (launched > max_it + chunk) break; // defensive: state always sets done by max_it
     }
     LsqrState hs;
@@ -610,9 +689,23 @@ void dev_gemv_n_dense(long n, long d, const double *A, long lda, const double *x
     int nc = (int)((d + cw - 1) / cw);
     double *P = ar.alloc_n<double>((size_t)nc * n);
     dim3 g((unsigned)rowblocks, (unsigned)nc);
-    k_gemv_n_part<<<g, GN_THREADS, 0, s>>>(n, d, A, lda, x, cw, P, 0, nullptr);
+    k_gemv_n_part<2><<<g, GN_THREADS, 0, s>>>(n, d, A, lda, x, cw, P, 0, nullptr);
     k_axpby_part<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, nc, P, alpha, beta, y);
     SKL_CHECK_LAUNCH();
 }
 
 } // namespace skl
+
+// ---- profiling hooks of the C ABI ------------------------------------------------
+extern "C" {
+void skl_profile_enable(int on) { skl::g_prof_on = on != 0; }
+void skl_profile_reset(void) { skl::g_prof = skl::ProfStats(); }
+int skl_profile_read(int slot, double *avg_ms, long *launches, double *bytes_per_launch) {
+    if (slot < 0 || slot > 3) return SKL_E_ARG;
+    const skl::ProfStats &P = skl::g_prof;
+    if (launches) *launches = P.cnt[slot];
+    if (avg_ms) *avg_ms = P.cnt[slot] ? P.ms[slot] / (double)P.cnt[slot] : 0.0;
+    if (bytes_per_launch) *bytes_per_launch = P.bytes[slot];
+    return SKL_OK;
+}
+}

@@ -91,6 +91,20 @@ Context &ctx() {
     return *t_ctx;
 }
 
+void lsqr_release_graphs();
+
+// Frees the calling thread's stream, arena and cached graphs (worker exit).
+void release_thread_context() {
+    if (!t_ctx) return;
+    (void)hipStreamSynchronize(t_ctx->stream);
+    lsqr_release_graphs();
+    t_ctx->arena.release();
+    if (t_ctx->pinned) (void)hipHostFree(t_ctx->pinned);
+    (void)hipStreamDestroy(t_ctx->stream);
+    delete t_ctx;
+    t_ctx = nullptr;
+}
+
 void set_device(int dev) { g_device = dev; }
 
 // ---------------------------------------------------------------------------

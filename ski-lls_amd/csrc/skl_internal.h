@@ -47,6 +47,11 @@ class Arena {
     void *alloc(size_t bytes);         // 256-byte aligned
     template <class T> T *alloc_n(size_t n) { return static_cast<T *>(alloc(n * sizeof(T))); }
     void reset() { used_ = 0; }
+    // release everything carved after `mark` (stream-ordered reuse: later
+    // kernels on the same stream may take the memory over)
+    void rewind(size_t mark) {
+        if (mark <= used_) used_ = mark;
+    }
     size_t used() const { return used_; }
     size_t capacity() const { return cap_; }
     void release();
@@ -159,6 +164,7 @@ struct LsqrProblem {
     const double *csr_val = nullptr;
     const int32_t *csc_ptr = nullptr, *csc_row = nullptr;
     const double *csc_val = nullptr;
+    long nnz = 0;                // sparse: stored entries
     const int32_t *E = nullptr; // r entries or null (identity)
     const double *M = nullptr;  // r x r upper, ldm
     long ldm = 0;

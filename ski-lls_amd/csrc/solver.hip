@@ -261,7 +261,9 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
         finish_qr(W);
         double rc = rcond_from(rank, Rd, rank, Mi, rank, ar);
         cond_est = rc > 0 ? 1.0 / rc : INFINITY;
-        if (!(cond_est <= o.escalate_cond)) {
+        // FP32 cannot tell a tiny column from a dependent one: a rank loss in
+        // FP32 or a large condition estimate sends the factorization to FP64
+        if (rank < std::min(m, d) || !(cond_est <= o.escalate_cond)) {
             escalated = true;
             double *Wd = ar.alloc_n<double>((size_t)m * ncol);
             double *taud = ar.alloc_n<double>((size_t)d);
@@ -466,7 +468,7 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
         trinv_upper(rank, Rd, rank, 0.0, Mi, rank, ar);
         double rc = rcond_from(rank, Rd, rank, Mi, rank, ar);
         cond_est = rc > 0 ? 1.0 / rc : INFINITY;
-        if (!(cond_est <= o.escalate_cond)) {
+        if (rank < std::min(m, d) || !(cond_est <= o.escalate_cond)) {
             escalated = true;
             double *Wd = ar.alloc_n<double>((size_t)m * (d + 1));
             double *taud = ar.alloc_n<double>((size_t)d + 1);
@@ -524,6 +526,7 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
     p.csc_ptr = cp;
     p.csc_row = ri;
     p.csc_val = dval;
+    p.nnz = nnz;
     p.E = jpvt;
     p.M = Mp;
     p.ldm = rank;

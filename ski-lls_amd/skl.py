@@ -67,6 +67,9 @@ EXPORTS = [
     "skl_dense_solve_dev", "skl_dense_solve_nocpqr_dev", "skl_sparse_solve_dev",
     "skl_set_history", "skl_hash_pattern", "skl_dht", "skl_hash_apply", "skl_dense_sketch",
     "skl_sparse_sketch", "skl_qr", "skl_cpqr", "skl_trinv", "skl_lsqr_dense",
+    "skl_profile_enable", "skl_profile_reset", "skl_profile_read",
+    "skl_server_create", "skl_server_submit_dense", "skl_server_submit_sparse",
+    "skl_server_wait", "skl_server_pending", "skl_server_destroy",
 ]
 
 _lib = None
@@ -229,6 +232,99 @@ def ls_sparse_spqr(A, b, gamma, k, abs_tol=ABS_TOL, ordering=SPQR_ORDERING, it_t
                            it_tol=it_tol, max_it=max_it, rcond_threshold=rcond_threshold,
                            perturb=perturb, debug=debug, seed=seed)
     return x, info.rank, info.flag, info.it
+
+
+# ---- serving ----------------------------------------------------------------------
+class Server:
+    """Continuous-batching request loop (skl_server_*).  Submitted arrays are
+    kept alive by the Server until their ticket is waited on."""
+
+    def __init__(self, slots=4, mem_budget_bytes=0, device=0):
+        L = lib()
+        L.skl_server_create.restype = C.c_void_p
+        L.skl_server_submit_dense.restype = C.c_long
+        L.skl_server_submit_sparse.restype = C.c_long
+        self._h = C.c_void_p(L.skl_server_create(C.c_int(slots), C.c_size_t(mem_budget_bytes),
+                                                 C.c_int(device)))
+        self._keep = {}
+
+    def submit_dense(self, A, b, cpqr=True, **opts):
+        A = _f(A)
+        b = _c(b)
+        n, d = A.shape
+        x = np.zeros(d)
+        o = default_opts(**opts)
+        t = lib().skl_server_submit_dense(self._h, C.c_int(1 if cpqr else 0), C.c_long(n),
+                                          C.c_long(d), _d(A), C.c_long(n), _d(b), _d(x),
+                                          C.byref(o))
+        if t < 0:
+            _check(-t)
+        self._keep[t] = (A, b, x, o)
+        return t
+
+    def submit_sparse(self, Acsc, b, **opts):
+        n, d = Acsc.shape
+        Ap = np.ascontiguousarray(Acsc.indptr, dtype=np.int64)
+        Ai = np.ascontiguousarray(Acsc.indices, dtype=np.int64)
+        Ax = _c(Acsc.data)
+        b = _c(b)
+        x = np.zeros(d)
+        o = default_opts(**opts)
+        t = lib().skl_server_submit_sparse(self._h, C.c_long(n), C.c_long(d), C.c_long(Ap[-1]),
+                                           _l(Ap), _l(Ai), _d(Ax), _d(b), _d(x), C.byref(o))
+        if t < 0:
+            _check(-t)
+        self._keep[t] = (Ap, Ai, Ax, b, x, o)
+        return t
+
+    def wait(self, ticket):
+        """-> (x, info, queue_seconds, run_seconds)"""
+        info = Info()
+        q = C.c_double(0)
+        r = C.c_double(0)
+        rc = lib().skl_server_wait(self._h, C.c_long(ticket), C.byref(info), C.byref(q),
+                                   C.byref(r))
+        keep = self._keep.pop(ticket)
+        _check(rc)
+        return keep[-2], info, q.value, r.value
+
+    def pending(self):
+        return lib().skl_server_pending(self._h)
+
+    def close(self):
+        if self._h:
+            lib().skl_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+# ---- kernel timing --------------------------------------------------------------
+PROFILE_SLOTS = {0: "A z (gemv_n / spmv_n)", 1: "A^T u (gemv_t / csc_t)", 2: "M v (trmv_n)",
+                 3: "M^T t (trmv_t)"}
+
+
+def profile(on=True, reset=True):
+    if reset:
+        lib().skl_profile_reset()
+    lib().skl_profile_enable(C.c_int(1 if on else 0))
+
+
+def profile_read():
+    out = {}
+    for s in range(4):
+        ms = C.c_double(0)
+        cnt = C.c_long(0)
+        by = C.c_double(0)
+        _check(lib().skl_profile_read(C.c_int(s), C.byref(ms), C.byref(cnt), C.byref(by)))
+        out[s] = {"name": PROFILE_SLOTS[s], "avg_ms": ms.value, "launches": cnt.value,
+                  "bytes": by.value}
+    return out
 
 
 # ---- components -------------------------------------------------------------

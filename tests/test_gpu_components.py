@@ -1,0 +1,146 @@
+"""Stage-by-stage parity of the HIP kernels with the oracle, through the C ABI.
+
+Tolerances: integer / pattern work and the sketches whose summation order
+follows the reference are bit-exact; FP64 transforms and factorizations are
+compared at 1e-12..1e-13 relative (rounding of a different but equally
+stable evaluation order); FP32 variants at 1e-5."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from conftest import rel, sparse_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_present(skl):
+    assert skl.lib().skl_device_count() >= 1
+    assert skl.lib().skl_init(0) == 0
+
+
+@pytest.mark.parametrize("n", [1, 2, 50, 97, 120, 340, 2798, 4000, 4096, 5533, 12000, 40000,
+                               131072, 199999])
+def test_dht_fp64(skl, O, n):
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, 5))
+    assert rel(skl.dht(X), O.dht_cols(X)) < 1e-13
+
+
+@pytest.mark.parametrize("n", [50, 1399, 4000, 40000])
+def test_dht_fp32(skl, O, n):
+    rng = np.random.default_rng(n + 1)
+    X = rng.standard_normal((n, 4))
+    assert rel(skl.dht(X, skl.SKL_MIXED), O.dht_cols(X)) < 2e-6
+
+
+@pytest.mark.parametrize("k", [1, 3])
+def test_hash_apply_bit_exact(skl, O, k):
+    rng = np.random.default_rng(k)
+    n, m = 4000, 340
+    rows, vals = O.sparse_pattern(3, n, m, k)
+    X = rng.standard_normal((n, 7))
+    a = 1 / np.sqrt(k)
+    assert np.array_equal(skl.hash_apply(m, rows, vals, k, a, X), O.hash_apply(m, rows, vals, k, a, X))
+
+
+@pytest.mark.parametrize("n,d,m,k", [(4000, 20, 340, 1), (4000, 7, 40, 3), (40000, 6, 300, 1)])
+def test_dense_sketch(skl, O, n, d, m, k):
+    rng = np.random.default_rng(n + d)
+    A = rng.standard_normal((n, d))
+    b = rng.standard_normal(n)
+    D, rows, vals = O.dense_pattern(11, n, m, k)
+    SA = skl.dense_sketch(A, b, m, k, D, rows, vals)
+    ref = O.hash_apply(m, rows, vals, k, 1 / np.sqrt(k), O.fut(np.column_stack([A, b]), D))
+    assert rel(SA, ref) < 1e-13
+    SA32 = skl.dense_sketch(A, b, m, k, D, rows, vals, precision=skl.SKL_MIXED)
+    assert rel(SA32, ref) < 1e-5
+
+
+def test_sparse_sketch_bit_exact(skl, O):
+    A, b = sparse_problem(3, n=6000, d=120)
+    m, k = 180, 3
+    rows, vals = O.sparse_pattern(1, 6000, m, k)
+    SA, bs = skl.sparse_sketch(m, rows, vals, k, A, b)
+    Ap = np.ascontiguousarray(A.indptr, dtype=np.int64)
+    Ai = np.ascontiguousarray(A.indices, dtype=np.int64)
+    assert np.array_equal(SA, O.sketch_sparse(m, rows, vals, k, Ap, Ai, A.data, 6000, 120))
+    assert np.array_equal(bs, O.sketch_vec(m, rows, vals, k, b))
+
+
+@pytest.mark.parametrize("m,n", [(60, 40), (300, 120), (680, 400), (1100, 700)])
+def test_qr_fp64(skl, O, m, n):
+    rng = np.random.default_rng(m)
+    A = rng.standard_normal((m, n + 1)) * np.append(np.logspace(0, -4, n), 1.0)
+    W, tau = skl.qr(A, nfac=n)
+    Wo, tauo = O.geqrf(A[:, :n])
+    assert rel(np.triu(W[:n, :n]), np.triu(Wo[:n])) < 1e-12
+    assert rel(tau, tauo) < 1e-12
+    # the carried column receives Q^T c
+    assert rel(W[:, n], O.ormqr_lt(Wo, tauo, A[:, n])) < 1e-12
+
+
+def test_qr_fp32(skl, O):
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((500, 300))
+    W, tau = skl.qr(A, precision=skl.SKL_MIXED)
+    Wo, _ = O.geqrf(A)
+    assert rel(np.triu(W[:300]), np.triu(Wo[:300])) < 1e-5
+
+
+@pytest.mark.parametrize("m,n", [(80, 50), (340, 200), (700, 420)])
+def test_cpqr_greedy_pivots(skl, O, m, n):
+    rng = np.random.default_rng(m + n)
+    A = rng.standard_normal((m, n)) * np.logspace(0, -4, n)
+    W, jpvt, tau = skl.cpqr(A)
+    Wo, jo, to = O.geqp3(A)
+    assert np.array_equal(jpvt, jo)
+    assert rel(np.triu(W[:n]), np.triu(Wo[:n])) < 1e-12
+
+
+def test_cpqr_hqrrp_pivots(skl, O):
+    rng = np.random.default_rng(5)
+    m, n = 1200, 1000
+    A = rng.standard_normal((m, n)) * np.logspace(0, -3, n)
+    W, jpvt, tau = skl.cpqr(A)
+    Wo, jo, to = O.cpqr(A)
+    assert np.array_equal(jpvt, jo)
+    assert rel(np.triu(W[:n]), np.triu(Wo[:n])) < 1e-11
+
+
+@pytest.mark.parametrize("r", [1, 31, 32, 33, 200, 1000])
+def test_trinv(skl, r):
+    rng = np.random.default_rng(r)
+    R = np.triu(rng.standard_normal((r, r))) + 3 * np.eye(r)
+    M = skl.trinv(R)
+    assert rel(M, sl.solve_triangular(R, np.eye(r))) < 1e-12
+    Mp = skl.trinv(R, perturb=0.25)
+    assert rel(Mp, sl.solve_triangular(R + 0.25 * np.eye(r), np.eye(r))) < 1e-12
+
+
+def test_lsqr_steps_match_oracle(skl, O):
+    rng = np.random.default_rng(12)
+    n, d = 5000, 60
+    A = rng.standard_normal((n, d)) * np.logspace(0, -3, d)
+    b = rng.standard_normal(n)
+    # a rough preconditioner so that LSQR takes tens of iterations
+    R = np.linalg.qr(A[:: 7], mode="r")
+    M = sl.solve_triangular(R, np.eye(d))
+    y, flag, it, steps = skl.lsqr_dense(A, M, b, 1e-12, 500, history=True)
+    yo, flo, ito, so, _ = O.lsqr_dense(A, R, b, 1e-12, 500, hist=True)
+    assert flag == flo == 0 and abs(it - ito) <= 1
+    k = min(it, ito) - 1
+    assert rel(steps[:k, :4], so[:k, :4]) < 1e-9
+    assert rel(y, yo) < 1e-9
+
+
+def test_lsqr_permuted_columns(skl, O):
+    rng = np.random.default_rng(13)
+    n, d = 3000, 40
+    A = rng.standard_normal((n, d))
+    b = rng.standard_normal(n)
+    E = rng.permutation(d)[:30]
+    R = np.linalg.qr(A[:, E], mode="r")
+    M = sl.solve_triangular(R, np.eye(30))
+    y, flag, it, _ = skl.lsqr_dense(A, M, b, 1e-12, 200, E=E)
+    yo, *_ = O.lsqr_dense(A, R, b, 1e-12, 200, E=E)
+    assert rel(y, yo) < 1e-10

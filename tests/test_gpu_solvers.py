@@ -1,0 +1,147 @@
+"""End-to-end parity of the three solvers with the oracle, the reference's
+known answers, failure handling, mixed precision and full-size properties.
+
+x is compared to the oracle's x (same algorithm, same seed) at 1e-8
+relative, the north-star tolerance; LSQR's stopping estimate must satisfy
+the reference's criterion (src/IterSolver.cpp:275-278)."""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+import scipy.sparse as sp
+
+from conftest import GOLDEN, c1_problem, read_mtx, rel, sparse_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def ne_residual(A, x, b):
+    r = b - A @ x
+    return np.linalg.norm(A.T @ r) / (np.linalg.norm(A, 2) * np.linalg.norm(r))
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_dense_c1_matches_oracle(skl, O, seed):
+    A, b = c1_problem(seed)
+    kw = dict(gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, abs_tol=1e-8, seed=1234 + seed)
+    x, info = skl.dense_solve(A, b, cpqr=True, **kw)
+    xo, io, _ = O.ls_dense(A, b, **kw)
+    assert info.flag == 0 and info.rank == io.rank == 200
+    assert abs(info.it - io.it) <= 2, (info.it, io.it)
+    assert rel(x, xo) < 1e-8
+    assert info.lsqr_nrm_ar < 1e-10 * info.lsqr_nrm_a * info.lsqr_nrm_r
+    xr = sl.lstsq(A, b, lapack_driver="gelsd")[0]
+    assert rel(x, xr) < 1e-8
+
+
+def test_dense_nocpqr_c1_matches_oracle(skl, O):
+    A, b = c1_problem(2)
+    kw = dict(gamma=1.7, k=1, it_tol=1e-10, seed=77)
+    x, info = skl.dense_solve(A, b, cpqr=False, **kw)
+    xo, io, _ = O.ls_dense_nocpqr(A, b, **kw)
+    assert info.flag == 0 and info.rank == 200
+    assert abs(info.it - io.it) <= 2, (info.it, io.it)
+    assert rel(x, xo) < 1e-8
+
+
+def test_dense_history_matches_oracle(skl, O):
+    A, b = c1_problem(3, n=3000, d=120)
+    kw = dict(gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=5)
+    x, info, st = skl.dense_solve(A, b, cpqr=True, history=400, **kw)
+    xo, io, so = O.ls_dense(A, b, hist=True, **kw)
+    # alpha, beta of the first iterations agree to the rounding of the
+    # preconditioner (R from the two QRs differs at ~kappa eps); later ones
+    # drift apart as finite-precision Lanczos vectors do, while x still agrees
+    assert rel(st[:10, :2], so[:10, :2]) < 1e-5
+    assert abs(info.it - io.it) <= 2
+    assert rel(x, xo) < 1e-8
+
+
+def test_sparse_matches_oracle(skl, O):
+    A, b = sparse_problem(4, n=6000, d=150, density=0.02)
+    kw = dict(gamma=1.5, k=3, it_tol=1e-10, seed=1)
+    x, info = skl.sparse_solve(A, b, **kw)
+    xo, io, _ = O.ls_sparse(A, b, **kw)
+    assert info.flag == 0 and info.rank == io.rank == 150 and info.success == io.success == 1
+    assert abs(info.it - io.it) <= 2, (info.it, io.it)
+    assert rel(x, xo) < 1e-8
+
+
+def test_examples_known_answers(skl, kat):
+    e = kat["dense_example"]
+    A = np.array(e["A"]).reshape(e["n"], e["m"]).T
+    x, info = skl.dense_solve(A, np.array(e["b"]), gamma=1.1, k=3, rcond=1e-10, it_tol=1e-6,
+                              seed=1)
+    assert np.allclose(x, e["x"], atol=1e-6)
+    s = kat["sparse_example"]
+    As = sp.csc_matrix((s["val"], s["rowind"], s["colptr"]), shape=(s["m"], s["n"]))
+    x, info = skl.sparse_solve(As, np.array(s["b"]), gamma=1.1, k=3, it_tol=1e-6)
+    assert np.allclose(x, s["x"], atol=1e-6)
+
+
+def test_full_rank_fixtures(skl, kat):
+    for f, r_ref in kat["full_rank_gelsd_residual"].items():
+        A = read_mtx(os.path.join(GOLDEN, "data", f))
+        b = np.ones(A.shape[0])
+        x, info = skl.dense_solve(A, b, gamma=1.7, k=1, abs_tol=1e-8, rcond=1e-12, it_tol=1e-6,
+                                  seed=1)
+        assert abs(np.linalg.norm(A @ x - b) - r_ref) < 1e-6 * max(1, r_ref), f
+        if "Sparse" in f:
+            As = sp.csc_matrix(A)
+            xs, _ = skl.sparse_solve(As, b, gamma=1.4, k=2, it_tol=1e-6)
+            assert abs(np.linalg.norm(A @ xs - b) - r_ref) < 1e-6 * max(1, r_ref), f
+
+
+def test_rank_deficient_kats(skl, kat):
+    """test/test_solvers.sh:28-31 through the dense robust solver on the
+    densified matrix, as test/ls_blendenpik_hashing_sparse_input.cpp does."""
+    for name, r_ref in kat["rank_deficient"].items():
+        A = read_mtx(os.path.join(GOLDEN, "data", "rank_deficient", name + ".mtx"))
+        b = np.ones(A.shape[0])
+        x, info = skl.dense_solve(A, b, gamma=1.7, k=1, abs_tol=1e-8, rcond=1e-12,
+                                  it_tol=1e-6, seed=1)
+        r = np.linalg.norm(A @ x - b)
+        assert abs(r - r_ref) < 1e-4 * max(1.0, r_ref) + 1e-5, (name, r, r_ref)
+
+
+def test_errors_and_flags(skl):
+    with pytest.raises(RuntimeError, match="Matrix is not overdetermined."):
+        skl.dense_solve(np.ones((3, 5)), np.ones(3))
+    with pytest.raises(RuntimeError, match="Matrix is not overdetermined."):
+        skl.sparse_solve(sp.csc_matrix(np.ones((3, 5))), np.ones(3))
+    A, b = c1_problem(5, n=2000, d=50)
+    x, info = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, max_it=1, rcond=1e-12)
+    assert info.flag == 1 and info.it == 1
+    # b in the range of A: the sketch-and-solve start already meets abs_tol
+    xs = np.random.default_rng(1).standard_normal(50)
+    x, info = skl.dense_solve(A, A @ xs, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12,
+                              abs_tol=1e-6)
+    assert info.it == 0 and info.flag == 0 and rel(x, xs) < 1e-8
+    x, info = skl.dense_solve(A, np.zeros(2000), gamma=1.7, k=1)
+    assert not x.any() and info.flag == 0
+
+
+def test_deterministic(skl):
+    A, b = c1_problem(6)
+    x1, i1 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, seed=3)
+    x2, i2 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, seed=3)
+    assert np.array_equal(x1, x2) and i1.it == i2.it
+
+
+def test_mixed_precision_dense(skl):
+    A, b = c1_problem(7)
+    x64, i64 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=9)
+    x32, i32 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=9,
+                               precision=skl.SKL_MIXED)
+    assert i32.flag == 0 and i32.escalated == 0
+    assert rel(x32, x64) < 1e-8
+    assert i32.lsqr_nrm_ar < 1e-10 * i32.lsqr_nrm_a * i32.lsqr_nrm_r
+
+
+def test_mixed_precision_escalates_on_ill_conditioned_sparse(skl):
+    A, b = sparse_problem(8, n=4000, d=100, density=0.02, decades=10)
+    x, info = skl.sparse_solve(A, b, gamma=1.5, k=3, it_tol=1e-10, precision=skl.SKL_MIXED)
+    assert info.escalated == 1 and info.flag == 0
+    x64, _ = skl.sparse_solve(A, b, gamma=1.5, k=3, it_tol=1e-10)
+    assert rel(x, x64) < 1e-8
