@@ -274,6 +274,7 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copy calibration: 3 x 1 GiB device copies done")
+    skl.lib().skl_finalize()
 
 
 if __name__ == "__main__":

@@ -79,6 +79,8 @@ typedef struct skl_lsqr_step {
 
 void skl_default_opts(skl_opts *o);
 int skl_init(int device);
+/* releases the calling thread's stream, device buffers and cached graphs; the
+   next call on this thread sets them up again */
 void skl_finalize(void);
 const char *skl_error_string(int code);
 const char *skl_last_error(void);

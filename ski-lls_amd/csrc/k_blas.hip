@@ -33,24 +33,48 @@ template <> struct Mfma<float> {
 
 constexpr int GBM = 64, GBN = 64, GBK = 16;
 
+// kchunk > 0 selects split-K: blockIdx.z covers k in [z kchunk, (z+1) kchunk)
+// and writes its raw partial product to C + z sC (alpha, beta applied by the
+// reduction); kchunk == 0 is a strided batch over blockIdx.z.
 template <class T, bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T *__restrict__ A,
                                               long lda, const T *__restrict__ B, long ldb, T beta,
                                               T *__restrict__ C, long ldc, long sA, long sB,
-                                              long sC) {
+                                              long sC, int kchunk) {
     typedef typename Mfma<T>::acc acc_t;
     __shared__ T As[2][GBK][GBM + 1];
     __shared__ T Bs[2][GBK][GBN + 1];
     const long bz = blockIdx.z;
-    A += bz * sA;
-    B += bz * sB;
-    C += bz * sC;
+    if (kchunk > 0) {
+        const long k0 = bz * (long)kchunk;
+        K = (int)min((long)kchunk, (long)K - k0);
+        A += TA ? k0 : k0 * lda;
+        B += TB ? k0 * ldb : k0;
+        C += bz * sC;
+    } else {
+        A += bz * sA;
+        B += bz * sB;
+        C += bz * sC;
+    }
     const int i0 = blockIdx.x * GBM, j0 = blockIdx.y * GBN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     acc_t acc[2][2];
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+    // C is read early so its latency hides behind the products
+    T cold[2][2][4];
+    const bool rd_c = (beta != T(0));
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int gi = i0 + wm + mi * 16 + Mfma<T>::row(lane, r);
+                int gj = j0 + wn + ni * 16 + (lane & 15);
+                cold[mi][ni][r] = (rd_c && gi < M && gj < N) ? C[gi + (long)gj * ldc] : T(0);
+            }
 
     T ra[4], rb[4];
     auto load = [&](int k0) {
@@ -119,33 +143,77 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
                 int gi = i0 + wm + mi * 16 + Mfma<T>::row(lane, r);
                 int gj = j0 + wn + ni * 16 + (lane & 15);
                 if (gi < M && gj < N) {
-                    T *c = C + gi + (long)gj * ldc;
                     T v = alpha * acc[mi][ni][r];
-                    *c = (beta == T(0)) ? v : v + beta * *c;
+                    C[gi + (long)gj * ldc] = rd_c ? v + beta * cold[mi][ni][r] : v;
                 }
             }
+}
+
+// C = alpha sum_z W[z] + beta C over S split-K partials (fixed order)
+template <class T>
+__global__ void k_splitk_reduce(int M, int N, int S, const T *W, T alpha, T beta, T *C, long ldc) {
+    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)M * N) return;
+    int i = (int)(e % M), j = (int)(e / M);
+    T s = W[e];
+    for (int z = 1; z < S; ++z) s += W[e + (long)z * M * N];
+    T *c = C + i + (long)j * ldc;
+    *c = (beta == T(0)) ? alpha * s : alpha * s + beta * *c;
 }
 
 template <class T>
 void gemm(char ta, char tb, long M, long N, long K, T alpha, const T *A, long lda, const T *B,
           long ldb, T beta, T *C, long ldc, long batch, long sA, long sB, long sC) {
     if (M <= 0 || N <= 0 || batch <= 0) return;
-    dim3 grid((unsigned)((M + GBM - 1) / GBM), (unsigned)((N + GBN - 1) / GBN), (unsigned)batch);
+    const long tm = (M + GBM - 1) / GBM, tn = (N + GBN - 1) / GBN;
     hipStream_t s = ctx().stream;
     bool TA = (ta == 'T' || ta == 't'), TB = (tb == 'T' || tb == 't');
+    // Few output tiles over a long K (V^T C, G V, V^T V in the QR): split K so
+    // the launch fills the chip; partials go to the context's split-K scratch
+    // (stream-ordered, so the next GEMM may reuse it).
+    int S = 1;
+    long kchunk = 0;
+    if (batch == 1 && tm * tn < 128 && K >= 512) {
+        long want = std::min<long>((256 + tm * tn - 1) / (tm * tn), K / 256);
+        long cap = (64L << 20) / ((long)sizeof(T) * M * N);
+        S = (int)std::max<long>(1, std::min<long>({want, cap, 64}));
+        if (S > 1) {
+            kchunk = ((K + S - 1) / S + GBK - 1) / GBK * GBK;
+            S = (int)((K + kchunk - 1) / kchunk);
+        }
+    }
+    T *Cw = C;
+    long ldw = ldc, sW = sC, nz = batch;
+    T al = alpha, be = beta;
+    if (S > 1) {
+        Cw = static_cast<T *>(ctx().workspace(sizeof(T) * (size_t)S * M * N));
+        ldw = M;
+        sW = M * N;
+        nz = S;
+        al = T(1);
+        be = T(0);
+    }
+    dim3 grid((unsigned)tm, (unsigned)tn, (unsigned)nz);
+    const int kc = (int)kchunk;
     if (!TA && !TB)
-        k_gemm<T, false, false><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, alpha, A, lda, B, ldb,
-                                                     beta, C, ldc, sA, sB, sC);
+        k_gemm<T, false, false><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, al, A, lda, B, ldb,
+                                                     be, Cw, ldw, sA, sB, sW, kc);
     else if (TA && !TB)
-        k_gemm<T, true, false><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, alpha, A, lda, B, ldb,
-                                                    beta, C, ldc, sA, sB, sC);
+        k_gemm<T, true, false><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, al, A, lda, B, ldb,
+                                                    be, Cw, ldw, sA, sB, sW, kc);
     else if (!TA && TB)
-        k_gemm<T, false, true><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, alpha, A, lda, B, ldb,
-                                                    beta, C, ldc, sA, sB, sC);
+        k_gemm<T, false, true><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, al, A, lda, B, ldb,
+                                                    be, Cw, ldw, sA, sB, sW, kc);
     else
-        k_gemm<T, true, true><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, alpha, A, lda, B, ldb,
-                                                   beta, C, ldc, sA, sB, sC);
+        k_gemm<T, true, true><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, al, A, lda, B, ldb,
+                                                   be, Cw, ldw, sA, sB, sW, kc);
     SKL_CHECK_LAUNCH();
+    if (S > 1) {
+        long tot = M * N;
+        k_splitk_reduce<T><<<(unsigned)((tot + 255) / 256), 256, 0, s>>>((int)M, (int)N, S, Cw,
+                                                                          alpha, beta, C, ldc);
+        SKL_CHECK_LAUNCH();
+    }
 }
 
 template void gemm<double>(char, char, long, long, long, double, const double *, long,

@@ -133,6 +133,112 @@ __global__ __launch_bounds__(GT_THREADS) void k_gemv_t_part(
     }
 }
 
+// ---- one pass over A: u <- A z - coef u and q <- A^T u together -----------------
+// A persistent 1024-thread workgroup per CU walks 16-row tiles of the
+// column-major A (one 128-byte line per column).  Pass 1 streams the tile from
+// HBM for A z: lane l of wave w takes row l%16 of columns
+// 4 w + l/16 + 64 s, so a wave load covers four whole lines; the row sums are
+// reduced over lanes and waves in a fixed order, giving u for the 16 rows.
+// Pass 2 re-reads the same tile, now resident in the caches, with one thread
+// per column (j = tid + 1024 c, c < NQ) summing its 16 rows against u into a
+// register accumulator.  Each workgroup writes its partial A^T u to Q[wg][:]
+// and its partial sum of squares of u to Sb[wg].
+constexpr int FU_T = 1024, FU_R = 16;
+
+template <int NQ>
+__global__ __launch_bounds__(FU_T) void k_fused_dense(long n, long d, const double *__restrict__ A,
+                                                      long lda, const double *__restrict__ z,
+                                                      double *__restrict__ ut,
+                                                      double *__restrict__ Sb,
+                                                      double *__restrict__ Q,
+                                                      const LsqrState *__restrict__ st) {
+    if (st->done) return;
+    extern __shared__ double fsm[];
+    double *zs = fsm;                  // d
+    double *red = fsm + d;             // 16 waves x 16 rows
+    double *utile = red + 16 * FU_R;   // 16
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int row = lane & 15, cg = 4 * w + (lane >> 4);
+    for (long j = tid; j < d; j += FU_T) zs[j] = z[j];
+    const double coef = st->coef;
+    __syncthreads();
+    const long ntiles = (n + FU_R - 1) / FU_R;
+    const long cstride = 64 * lda;
+    double q[NQ];
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) q[c] = 0.0;
+    double sq = 0.0;
+    for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const long i0 = t * FU_R;
+        const long i = i0 + row;
+        // pass 1: partial (A z)_i over columns cg + 64 s
+        double p = 0.0;
+        if (i < n) {
+            const double *ap = A + i + (long)cg * lda;
+            long j = cg;
+            for (; j + 64 * 7 < d; j += 64 * 8) {
+                double a[8];
+#pragma unroll
+                for (int s = 0; s < 8; ++s) a[s] = ap[s * cstride];
+#pragma unroll
+                for (int s = 0; s < 8; ++s) p += a[s] * zs[j + 64 * s];
+                ap += 8 * cstride;
+            }
+            for (; j < d; j += 64) {
+                p += *ap * zs[j];
+                ap += cstride;
+            }
+        }
+        p += __shfl_xor(p, 16, 64);
+        p += __shfl_xor(p, 32, 64);
+        if (lane < FU_R) red[w * FU_R + lane] = p;
+        __syncthreads();
+        if (tid < FU_R) {
+            double s = 0.0;
+            for (int ww = 0; ww < FU_T / 64; ++ww) s += red[ww * FU_R + tid];
+            const long ii = i0 + tid;
+            double v = 0.0;
+            if (ii < n) {
+                v = s - coef * ut[ii];
+                ut[ii] = v;
+                sq += v * v;
+            }
+            utile[tid] = v;
+        }
+        __syncthreads();
+        // pass 2: q_j += A(i0:i0+16, j)^T u
+        double uv[FU_R];
+#pragma unroll
+        for (int r = 0; r < FU_R; ++r) uv[r] = utile[r];
+        const int nr = (int)min((long)FU_R, n - i0);
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            const long j = tid + (long)FU_T * c;
+            if (j < d) {
+                const double *ac = A + i0 + j * lda;
+                double s = 0.0;
+                if (nr == FU_R) {
+#pragma unroll
+                    for (int r = 0; r < FU_R; ++r) s += ac[r] * uv[r];
+                } else {
+                    for (int r = 0; r < nr; ++r) s += ac[r] * utile[r];
+                }
+                q[c] += s;
+            }
+            // one column's 16 loads in flight at a time keeps the register count low
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads(); // utile and red are rewritten by the next tile
+    }
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) {
+        const long j = tid + (long)FU_T * c;
+        if (j < d) Q[(long)blockIdx.x * d + j] = q[c];
+    }
+    sq = block_sum<FU_T>(sq);
+    if (tid == 0) Sb[blockIdx.x] = sq;
+}
+
 // ---- reductions / scatter ----------------------------------------------------
 // z[j] = sum_c Pz[c * r + Einv[j]]  (0 when column j is outside E[0:r])
 __global__ void k_z_reduce(long d, long r, int nck, const double *__restrict__ Pz,
@@ -376,7 +482,16 @@ struct LsqrPlan {
     int nrb_m, cpw_m;  // M^T t
     int nsb;           // squares partials
     bool sparse;
+    bool fused = false; // one pass over A per iteration (k_fused_dense)
+    int G = 0, nq = 0;  // fused: workgroups, columns per thread in pass 2
 };
+
+// the one-pass kernel keeps d / 1024 column sums per thread in registers
+static bool fused_wanted(const LsqrProblem &p) {
+    if (p.csr_ptr || p.d > 4 * FU_T || p.n < 16 * 256) return false;
+    const char *e = getenv("SKL_LSQR_FUSED");
+    return e && e[0] == '1';
+}
 
 static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
     LsqrPlan q;
@@ -405,6 +520,12 @@ static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
     q.cpw_m = (int)std::max<long>(4, (p.r + cchm - 1) / cchm);
     q.nsb = (int)((p.n + RN_PER - 1) / RN_PER);
     if (q.sparse) q.nrb_a = 1;
+    if (fused_wanted(p)) {
+        q.fused = true;
+        q.G = num_cu;
+        long need = (p.d + FU_T - 1) / FU_T;
+        q.nq = need <= 1 ? 1 : need <= 2 ? 2 : 4;
+    }
     return q;
 }
 
@@ -415,24 +536,34 @@ struct LsqrBufs {
     LsqrState *st;
 };
 
+// after_fused: A^T u already sits in Pt as per-workgroup partials (q.G of
+// them) together with the squares in Sb; otherwise A^T u is computed here.
 static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs &B,
-                        hipStream_t s, int init, hipEvent_t *ev = nullptr) {
+                        hipStream_t s, int init, hipEvent_t *ev = nullptr,
+                        bool after_fused = false) {
     auto rec = [&](int i) {
         if (ev) SKL_HIP(hipEventRecord(ev[i], s));
     };
-    // Pt = A^T ut
-    rec(2);
-    if (!q.sparse) {
-        dim3 g((unsigned)q.nrb_a, (unsigned)((q.d + q.cpw_a - 1) / q.cpw_a));
-        k_gemv_t_part<0><<<g, GT_THREADS, 0, s>>>(q.n, q.d, p.A, p.lda, B.ut, q.cpw_a, B.Pt, 0, B.st);
+    int nrb = q.nrb_a, nsb = q.nsb;
+    if (after_fused) {
+        nrb = q.G;
+        nsb = q.G;
     } else {
-        k_csc_t<<<(unsigned)((q.d + 3) / 4), 256, 0, s>>>(q.d, p.csc_ptr, p.csc_row, p.csc_val,
-                                                          B.ut, B.Pt, B.st);
+        // Pt = A^T ut
+        rec(2);
+        if (!q.sparse) {
+            dim3 g((unsigned)q.nrb_a, (unsigned)((q.d + q.cpw_a - 1) / q.cpw_a));
+            k_gemv_t_part<0><<<g, GT_THREADS, 0, s>>>(q.n, q.d, p.A, p.lda, B.ut, q.cpw_a, B.Pt, 0,
+                                                      B.st);
+        } else {
+            k_csc_t<<<(unsigned)((q.d + 3) / 4), 256, 0, s>>>(q.d, p.csc_ptr, p.csc_row,
+                                                              p.csc_val, B.ut, B.Pt, B.st);
+        }
+        rec(3);
     }
-    rec(3);
     // t = Pt(E) / beta
-    k_reduce_t<<<(unsigned)((q.r + 255) / 256), 256, 0, s>>>(q.d, q.r, q.nrb_a, B.Pt, p.E, B.Sb,
-                                                             q.nsb, B.t, B.st);
+    k_reduce_t<<<(unsigned)((q.r + 255) / 256), 256, 0, s>>>(q.d, q.r, nrb, B.Pt, p.E, B.Sb, nsb,
+                                                             B.t, B.st);
     // Pm = M^T t
     {
         dim3 g((unsigned)q.nrb_m, (unsigned)((q.r + q.cpw_m - 1) / q.cpw_m));
@@ -440,8 +571,18 @@ static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs 
         k_gemv_t_part<1><<<g, GT_THREADS, 0, s>>>(q.r, q.r, p.M, p.ldm, B.t, q.cpw_m, B.Pm, 1, B.st);
         rec(7);
     }
-    k_update<<<1, UPD_THREADS, 0, s>>>(q.r, q.nrb_m, B.Pm, B.Sb, q.nsb, B.v, B.w, B.x, B.st,
+    k_update<<<1, UPD_THREADS, 0, s>>>(q.r, q.nrb_m, B.Pm, B.Sb, nsb, B.v, B.w, B.x, B.st,
                                        B.hist, B.hist_cap, init);
+}
+
+static void launch_fused(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs &B,
+                         hipStream_t s) {
+    size_t sh = sizeof(double) * ((size_t)q.d + 16 * FU_R + FU_R);
+    auto fn = q.nq == 1 ? k_fused_dense<1> : q.nq == 2 ? k_fused_dense<2> : k_fused_dense<4>;
+    if (sh > 65536)
+        SKL_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)sh));
+    fn<<<q.G, FU_T, sh, s>>>(q.n, q.d, p.A, p.lda, B.z, B.ut, B.Sb, B.Pt, B.st);
 }
 
 // Optional kernel timing: events around the four operator products of one
@@ -477,6 +618,13 @@ static void launch_iteration(const LsqrProblem &p, const LsqrPlan &q, const Lsqr
                                                                  B.z, B.st);
     }
     // ut = A z - coef ut ; Sb
+    if (q.fused) {
+        rec(ev, 0, s);
+        launch_fused(p, q, B, s);
+        rec(ev, 1, s);
+        launch_op_t(p, q, B, s, 0, ev, true);
+        return;
+    }
     if (!q.sparse) {
         dim3 g((unsigned)((q.n + GN_ROWS - 1) / GN_ROWS), (unsigned)q.nck_a);
         rec(ev, 0, s);
@@ -496,7 +644,11 @@ static void prof_accumulate(const LsqrPlan &q, long nnz) {
     const int pairs[4][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}};
     double r = (double)q.r, n = (double)q.n, d = (double)q.d;
     double b[4];
-    if (!q.sparse) {
+    if (q.fused) {
+        // one read of A, u read and written, z
+        b[0] = 8.0 * (n * d + 2 * n + d);
+        b[1] = 0.0;
+    } else if (!q.sparse) {
         b[0] = 8.0 * (n * d + n + d);
         b[1] = 8.0 * (n * d + n + d);
     } else {
@@ -506,6 +658,7 @@ static void prof_accumulate(const LsqrPlan &q, long nnz) {
     b[2] = 8.0 * (r * (r + 1) / 2 + 2 * r);
     b[3] = b[2];
     for (int k = 0; k < 4; ++k) {
+        if (q.fused && k == 1) continue; // A^T u runs inside the fused pass
         float ms = 0;
         if (hipEventElapsedTime(&ms, g_ev[pairs[k][0]], g_ev[pairs[k][1]]) == hipSuccess) {
             g_prof.ms[k] += ms;
@@ -561,8 +714,8 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
     B.z = ar.alloc_n<double>((size_t)q.d);
     B.Pz = ar.alloc_n<double>((size_t)q.nck_m * q.r);
     B.Pn = q.sparse ? nullptr : ar.alloc_n<double>((size_t)q.nck_a * q.n);
-    B.Sb = ar.alloc_n<double>((size_t)q.nsb);
-    B.Pt = ar.alloc_n<double>((size_t)q.nrb_a * q.d);
+    B.Sb = ar.alloc_n<double>((size_t)std::max(q.nsb, q.G));
+    B.Pt = ar.alloc_n<double>((size_t)std::max(q.nrb_a, q.G) * q.d);
     B.t = ar.alloc_n<double>((size_t)q.r);
     B.Pm = ar.alloc_n<double>((size_t)q.nrb_m * q.r);
     B.st = ar.alloc_n<LsqrState>(1);
@@ -604,7 +757,9 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
                  (long)(intptr_t)p.M, p.ldm, (long)(intptr_t)B.ut, (long)(intptr_t)B.v,
                  (long)(intptr_t)B.x, (long)(intptr_t)B.st, (long)(intptr_t)B.hist,
                  (long)(intptr_t)B.Pn, (long)(intptr_t)B.Pt, (long)(intptr_t)B.Pm,
-                 (long)(intptr_t)B.Einv, chunk, (long)(intptr_t)s, prof ? 1L : 0L};
+                 (long)(intptr_t)B.Einv, (long)(intptr_t)B.z, (long)(intptr_t)B.w,
+                 (long)(intptr_t)B.t, (long)(intptr_t)B.Sb, (long)(intptr_t)B.Pz, B.hist_cap,
+                 q.fused ? 1L : 0L, chunk, (long)(intptr_t)s, prof ? 1L : 0L};
         auto it = g_graphs.find(key);
         if (it != g_graphs.end()) {
             exec = it->second;

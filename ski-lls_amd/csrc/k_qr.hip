@@ -81,10 +81,9 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
     const int npiv = a.npiv;
     const T tol3z = sqrt(eps_half<T>());
 
-    for (long e = tid; e < (long)nr * nc; e += RS_T) {
-        int i = (int)(e % nr), q = (int)(e / nr);
-        S[i + (size_t)q * ldl] = a.W[(r0 + i) + q * a.ldw];
-    }
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int q = wave; q < nc; q += RS_T / 64)
+        for (int i = lane; i < nr; i += 64) S[i + q * ldl] = a.W[(r0 + i) + (long)q * a.ldw];
     __syncthreads();
 
     // all-reduce of vals[q] for q in [q0, nc) (q0 relative), result into Gs
@@ -94,10 +93,16 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
         __threadfence();
         grid.sync();
         if (!a.two_phase) {
-            for (int q = q0 + tid; q < nc; q += RS_T) {
+            // four adjacent lanes per column, each summing every fourth workgroup
+            const int sub = tid & 3;
+            for (int qb = q0; qb < nc; qb += RS_T / 4) {
+                const int q = qb + (tid >> 2);
                 T s = 0;
-                for (int u = 0; u < P; ++u) s += a.part[((size_t)buf * P + u) * nc + q];
-                Gs[q] = s;
+                if (q < nc)
+                    for (int u = sub; u < P; u += 4) s += a.part[((size_t)buf * P + u) * nc + q];
+                s += __shfl_xor(s, 1, 64);
+                s += __shfl_xor(s, 2, 64);
+                if (q < nc && sub == 0) Gs[q] = s;
             }
         } else {
             int ncol = nc - q0;
@@ -139,15 +144,24 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
         const int buf = jj & 1;
         if (npiv > 0) {
             // first index of max |vn1[jj..npiv)| (idamax), identical in all workgroups
-            if (tid == 0) {
-                int best = jj;
-                T bv = fabs(vn1[jj]);
-                for (int q = jj + 1; q < npiv; ++q)
+            if (wave == 0) {
+                int best = 0x7fffffff;
+                T bv = T(-1);
+                for (int q = jj + lane; q < npiv; q += 64)
                     if (fabs(vn1[q]) > bv) {
                         bv = fabs(vn1[q]);
                         best = q;
                     }
-                flags[npiv] = best;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    T ov = __shfl_xor(bv, o, 64);
+                    int ob = __shfl_xor(best, o, 64);
+                    if (ov > bv || (ov == bv && ob < best)) {
+                        bv = ov;
+                        best = ob;
+                    }
+                }
+                if (lane == 0) flags[npiv] = best < npiv ? best : jj; // all NaN: stay
             }
             __syncthreads();
             const int pvt = flags[npiv];
@@ -173,10 +187,21 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
         // partial Gram row: g[q] = sum_{rows > jj} S(:, jj) S(:, q), q >= jj
         T *pw = a.part + ((size_t)buf * P + w) * nc;
         const int ib = (int)max(0L, (long)jj + 1 - r0); // first local row below the diagonal
-        for (int q = jj + tid; q < nc; q += RS_T) {
-            T s = 0;
-            for (int i = ib; i < nr; ++i) s += S[i + (size_t)jj * ldl] * S[i + (size_t)q * ldl];
-            pw[q] = s;
+        {
+            // four adjacent lanes per column, each summing every fourth row
+            const int sub = tid & 3;
+            const T *sj = S + (size_t)jj * ldl;
+            for (int qb = jj; qb < nc; qb += RS_T / 4) {
+                const int q = qb + (tid >> 2);
+                T s = 0;
+                if (q < nc) {
+                    const T *sq = S + (size_t)q * ldl;
+                    for (int i = ib + sub; i < nr; i += 4) s += sj[i] * sq[i];
+                }
+                s += __shfl_xor(s, 1, 64);
+                s += __shfl_xor(s, 2, 64);
+                if (q < nc && sub == 0) pw[q] = s;
+            }
         }
         const bool owner = (jj >= r0 && jj < r1);
         if (owner)
@@ -198,12 +223,11 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
             // w_q = v^T a_q = a(jj, q) + sc * g_q ; C -= tau v w^T
             for (int q = jj + 1 + tid; q < nc; q += RS_T) Gs[q] = -tau * (Rs[q] + sc * Gs[q]);
             __syncthreads();
-            const int nq = nc - jj - 1;
-            const int nrows = nr - ib;
-            for (long e = tid; e < (long)nrows * nq; e += RS_T) {
-                int i = ib + (int)(e % nrows), q = jj + 1 + (int)(e / nrows);
-                T vi = sc * S[i + (size_t)jj * ldl];
-                S[i + (size_t)q * ldl] = S[i + (size_t)q * ldl] + vi * Gs[q];
+            // one wave per column, lanes over rows
+            for (int i = ib + lane; i < nr; i += 64) {
+                const T vi = sc * S[i + (size_t)jj * ldl];
+                for (int q = jj + 1 + wave; q < nc; q += RS_T / 64)
+                    S[i + (size_t)q * ldl] = S[i + (size_t)q * ldl] + vi * Gs[q];
             }
             if (owner)
                 for (int q = jj + 1 + tid; q < nc; q += RS_T)
@@ -276,10 +300,8 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
         }
         __syncthreads();
     }
-    for (long e = tid; e < (long)nr * nc; e += RS_T) {
-        int i = (int)(e % nr), q = (int)(e / nr);
-        a.W[(r0 + i) + q * a.ldw] = S[i + (size_t)q * ldl];
-    }
+    for (int q = wave; q < nc; q += RS_T / 64)
+        for (int i = lane; i < nr; i += 64) a.W[(r0 + i) + (long)q * a.ldw] = S[i + q * ldl];
 }
 
 template <class T>
@@ -363,24 +385,35 @@ __global__ void k_vfull(long mp, int b, const T *W, long ldw, T *Vf) {
     Vf[i + (long)q * mp] = v;
 }
 
-// T (b x b, ld b) from VtV and tau (dlarft forward columnwise)
+// T (b x b, ld b, b <= 64) from VtV and tau (dlarft forward columnwise):
+// T(0:i, i) = T(0:i, 0:i) (-tau_i VtV(0:i, i)), T(i, i) = tau_i.  Built in LDS;
+// four adjacent lanes share one row of each triangular product.
 template <class T>
 __global__ __launch_bounds__(256) void k_build_t(int b, const T *VtV, const T *tau, T *Tm) {
-    __shared__ T tmp[256];
-    for (int e = threadIdx.x; e < b * b; e += 256) Tm[e] = T(0);
+    constexpr int LD = 65;
+    __shared__ T sT[64 * LD], sV[64 * LD], tmp[64], st[64];
+    for (int e = threadIdx.x; e < b * b; e += 256) {
+        int r = e % b, q = e / b;
+        sV[r + q * LD] = VtV[e];
+        sT[r + q * LD] = T(0);
+    }
+    if (threadIdx.x < b) st[threadIdx.x] = tau[threadIdx.x];
     __syncthreads();
+    const int r = threadIdx.x >> 2, part = threadIdx.x & 3;
     for (int i = 0; i < b; ++i) {
-        T ti = tau[i];
-        for (int r = threadIdx.x; r < i; r += 256) tmp[r] = -ti * VtV[r + i * b];
+        const T ti = st[i];
+        if (threadIdx.x < i) tmp[threadIdx.x] = -ti * sV[threadIdx.x + i * LD];
         __syncthreads();
-        for (int r = threadIdx.x; r < i; r += 256) {
-            T s = 0;
-            for (int q = r; q < i; ++q) s += Tm[r + q * b] * tmp[q];
-            Tm[r + i * b] = s;
-        }
-        if (threadIdx.x == 0) Tm[i + i * b] = ti;
+        T s = 0;
+        if (r < i)
+            for (int q = r + part; q < i; q += 4) s += sT[r + q * LD] * tmp[q];
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        if (r < i && part == 0) sT[r + i * LD] = s;
+        if (threadIdx.x == 0) sT[i + i * LD] = ti;
         __syncthreads();
     }
+    for (int e = threadIdx.x; e < b * b; e += 256) Tm[e] = sT[(e % b) + (e / b) * LD];
 }
 
 // C (mp x nc2) := (I - V T V^T)^T C
@@ -445,141 +478,160 @@ template void cpqr_greedy<float>(long, long, long, float *, long, int32_t *, flo
 // ---------------------------------------------------------------------------
 // HQRRP pieces
 // ---------------------------------------------------------------------------
-constexpr int YQ_T = 256;
+constexpr int YQ_W = 16;        // waves per workgroup
+constexpr int YQ_T = 64 * YQ_W;
+constexpr int YQ_MAXROWS = 128; // two rows per lane
 
-// Column-sliced cooperative QRP of V (mY x nv, ld mY) for nsteps pivots; the
-// pivot order is returned as a logical -> physical permutation plus the list
-// of positions that changed.  Uses the HQRRP norm downdating
-// (skint.c:920-990); recomputed norms are local to a column's owner.
+// Column-sliced cooperative QRP of Y (mY x nv, mY <= 128) for nsteps pivots
+// with the HQRRP norm downdating (skint.c:920-990).  Every wave keeps K
+// columns in registers (lane l holds rows l and l + 64), together with their
+// partial norms and current logical positions; Y itself is only read.
+// One grid barrier per pivot: before it each workgroup publishes its best
+// candidate (norm, logical position) and that column's values; after it every
+// wave picks the global winner (largest norm, first position on ties -- the
+// idamax rule), forms the Householder vector itself and updates its columns.
+// perm_out[t] is the column at logical position t after nsteps pivots.
 template <class T> struct YqArgs {
-    int mY, nv, nsteps, cpw;
-    T *V;
-    int32_t *perm_out, *changed, *nchanged;
-    T *cand;      // [2][P]
-    int32_t *candi; // [2][P]
+    int mY, nv, nsteps;
+    const T *Y;
+    long ldy;
+    int32_t *perm_out;
+    T *cand;       // [2][P]
+    int32_t *candp; // [2][P]
+    T *ccol;       // [2][P][YQ_MAXROWS]
 };
 
 template <class T>
+__device__ inline bool yq_better(T v, int p, T bv, int bp) {
+    return v > bv || (v == bv && p < bp);
+}
+
+template <class T, int K>
 __global__ __launch_bounds__(YQ_T) void k_yqrp(YqArgs<T> a) {
     cg::grid_group grid = cg::this_grid();
-    extern __shared__ unsigned char smem_raw[];
-    const int P = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int nv = a.nv, mY = a.mY;
-    int32_t *perm = reinterpret_cast<int32_t *>(smem_raw); // nv
-    int32_t *pos = perm + nv;                              // nv
-    T *dn = reinterpret_cast<T *>(pos + nv + (nv & 1));    // cpw
-    T *en = dn + a.cpw;                                    // cpw
-    T *vv = en + a.cpw;                                    // mY
-    __shared__ T s_best;
-    __shared__ int s_bpos;
-    __shared__ T s_hh[3];
-    const int c0 = w * a.cpw, c1 = min(nv, c0 + a.cpw);
+    __shared__ T s_wv[YQ_W];
+    __shared__ int s_wp[YQ_W];
+    const int P = gridDim.x, w = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int mY = a.mY, nv = a.nv;
+    const int NONE = 0x7fffffff;
     const T tol3z = sqrt(eps_half<T>());
-    for (int t = tid; t < nv; t += YQ_T) {
-        perm[t] = t;
-        pos[t] = t;
+    const int cbase = (w * YQ_W + wave) * K;
+    T x0[K], x1[K], dn[K], en[K];
+    int pos[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int c = cbase + k;
+        const bool ok = c < nv;
+        const T *col = a.Y + (long)(ok ? c : 0) * a.ldy;
+        x0[k] = (ok && lane < mY) ? col[lane] : T(0);
+        x1[k] = (ok && lane + 64 < mY) ? col[lane + 64] : T(0);
+        T s = wave_sum(x0[k] * x0[k] + x1[k] * x1[k]);
+        dn[k] = sqrt(s);
+        en[k] = dn[k];
+        pos[k] = ok ? c : NONE;
     }
-    // initial norms of own columns
-    for (int c = c0 + wave; c < c1; c += YQ_T / 64) {
-        T s = 0;
-        for (int i = lane; i < mY; i += 64) {
-            T v = a.V[i + (long)c * mY];
-            s += v * v;
-        }
-        s = wave_sum(s);
-        if (lane == 0) {
-            dn[c - c0] = sqrt(s);
-            en[c - c0] = dn[c - c0];
-        }
-    }
-    __syncthreads();
     for (int jj = 0; jj < a.nsteps; ++jj) {
         const int buf = jj & 1;
-        if (tid == 0) {
-            T bv = T(-1);
-            int bp = 0x7fffffff;
-            for (int c = c0; c < c1; ++c) {
-                int p = pos[c];
-                if (p < jj) continue;
-                T v = fabs(dn[c - c0]);
-                if (v > bv || (v == bv && p < bp)) {
-                    bv = v;
-                    bp = p;
-                }
+        // best remaining column of this wave, then of this workgroup
+        T bv = T(-1);
+        int bp = NONE, bk = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (pos[k] >= jj && pos[k] != NONE && yq_better(fabs(dn[k]), pos[k], bv, bp)) {
+                bv = fabs(dn[k]);
+                bp = pos[k];
+                bk = k;
             }
-            a.cand[buf * P + w] = bv;
-            a.candi[buf * P + w] = bp;
+        if (lane == 0) {
+            s_wv[wave] = bv;
+            s_wp[wave] = bp;
+        }
+        __syncthreads();
+        T gv = T(-1);
+        int gp = NONE, gw = 0;
+        for (int u = 0; u < YQ_W; ++u)
+            if (yq_better(s_wv[u], s_wp[u], gv, gp)) {
+                gv = s_wv[u];
+                gp = s_wp[u];
+                gw = u;
+            }
+        if (wave == gw) {
+            T c0 = T(0), c1 = T(0);
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (k == bk) {
+                    c0 = x0[k];
+                    c1 = x1[k];
+                }
+            T *dst = a.ccol + ((size_t)buf * P + w) * YQ_MAXROWS;
+            dst[lane] = c0;
+            dst[lane + 64] = c1;
+            if (lane == 0) {
+                a.cand[buf * P + w] = gv;
+                a.candp[buf * P + w] = gp;
+            }
         }
         __threadfence();
         grid.sync();
-        if (tid == 0) {
-            T bv = T(-1);
-            int bp = 0x7fffffff;
-            for (int u = 0; u < P; ++u) {
-                T v = a.cand[buf * P + u];
-                int p = a.candi[buf * P + u];
-                if (v > bv || (v == bv && p < bp)) {
-                    bv = v;
-                    bp = p;
-                }
-            }
-            s_bpos = bp;
-        }
-        __syncthreads();
-        const int pl = s_bpos;
-        if (tid == 0) {
-            int a0 = perm[jj], a1 = perm[pl];
-            perm[jj] = a1;
-            perm[pl] = a0;
-            pos[a1] = jj;
-            pos[a0] = pl;
-        }
-        __syncthreads();
-        const int pc = perm[jj];
-        // Householder vector from the pivot column (rows jj..mY-1)
-        if (wave == 0) {
-            T xs = 0;
-            for (int i = jj + 1 + lane; i < mY; i += 64) {
-                T v = a.V[i + (long)pc * mY];
-                xs += v * v;
-            }
-            xs = wave_sum(xs);
-            if (lane == 0) {
-                T alpha = a.V[jj + (long)pc * mY];
-                T xnorm = sqrt(xs);
-                T tau = 0, sc = 0;
-                if (xnorm != T(0)) {
-                    T beta = -copysign(dlapy2_d(alpha, xnorm), alpha);
-                    tau = (beta - alpha) / beta;
-                    sc = T(1) / (alpha - beta);
-                }
-                s_hh[0] = tau;
-                s_hh[1] = sc;
+        // global winner, computed identically by every wave
+        T v = T(-1);
+        int p = NONE, u = 0;
+        for (int t = lane; t < P; t += 64) {
+            T cv = a.cand[buf * P + t];
+            int cp = a.candp[buf * P + t];
+            if (yq_better(cv, cp, v, p)) {
+                v = cv;
+                p = cp;
+                u = t;
             }
         }
-        __syncthreads();
-        const T tau = s_hh[0], sc = s_hh[1];
-        for (int i = jj + 1 + tid; i < mY; i += YQ_T) vv[i] = sc * a.V[i + (long)pc * mY];
-        if (tid == 0) vv[jj] = T(1);
-        __syncthreads();
-        // apply to own columns with logical position > jj; downdate their norms
-        for (int c = c0 + wave; c < c1; c += YQ_T / 64) {
-            if (pos[c] <= jj) continue;
-            T *col = a.V + (long)c * mY;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            T ov = __shfl_xor(v, o, 64);
+            int op = __shfl_xor(p, o, 64), ou = __shfl_xor(u, o, 64);
+            if (yq_better(ov, op, v, p)) {
+                v = ov;
+                p = op;
+                u = ou;
+            }
+        }
+        const int pl = p;
+        const T *src = a.ccol + ((size_t)buf * P + u) * YQ_MAXROWS;
+        const T p0 = src[lane], p1 = src[lane + 64];
+        const int r0 = lane, r1 = lane + 64;
+        // Householder vector of the pivot column over rows jj..mY-1 (dlarfg)
+        const T alpha = (jj < 64) ? readlane(p0, jj) : readlane(p1, jj - 64);
+        T xs = (r0 > jj ? p0 * p0 : T(0)) + (r1 > jj ? p1 * p1 : T(0));
+        xs = wave_sum(xs);
+        const T xnorm = sqrt(xs);
+        T tau = 0, sc = 0;
+        if (xnorm != T(0)) {
+            T beta = -copysign(dlapy2_d(alpha, xnorm), alpha);
+            tau = (beta - alpha) / beta;
+            sc = T(1) / (alpha - beta);
+        }
+        const T vv0 = r0 > jj ? sc * p0 : (r0 == jj ? T(1) : T(0));
+        const T vv1 = r1 > jj ? sc * p1 : (r1 == jj ? T(1) : T(0));
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            // logical swap of positions jj and pl
+            if (pos[k] == pl)
+                pos[k] = jj;
+            else if (pos[k] == jj)
+                pos[k] = pl;
+            if (pos[k] <= jj || pos[k] == NONE) continue;
             if (tau != T(0)) {
-                T s = 0;
-                for (int i = jj + lane; i < mY; i += 64) s += vv[i] * col[i];
-                s = wave_sum(s);
+                T s = wave_sum(vv0 * x0[k] + vv1 * x1[k]);
                 T f = -tau * s;
-                for (int i = jj + lane; i < mY; i += 64) col[i] = col[i] + vv[i] * f;
+                if (r0 >= jj) x0[k] = x0[k] + vv0 * f;
+                if (r1 >= jj) x1[k] = x1[k] + vv1 * f;
             }
-            // norm downdate (all lanes read the same values)
-            T d0 = dn[c - c0];
-            T nd = d0, ne = en[c - c0];
+            T d0 = dn[k], nd = d0, ne = en[k];
             bool recompute = false;
             if (d0 != T(0)) {
-                T temp = fabs(col[jj]) / d0;
+                T xj = (jj < 64) ? readlane(x0[k], jj) : readlane(x1[k], jj - 64);
+                T temp = fabs(xj) / d0;
                 temp = (T(1) + temp) * (T(1) - temp);
                 if (temp < T(0)) temp = T(0);
                 T t5 = d0 / ne;
@@ -590,28 +642,48 @@ __global__ __launch_bounds__(YQ_T) void k_yqrp(YqArgs<T> a) {
                     nd = d0 * sqrt(temp);
             }
             if (recompute) {
-                T s = 0;
-                for (int i = jj + 1 + lane; i < mY; i += 64) s += col[i] * col[i];
-                s = wave_sum(s);
+                T s = wave_sum((r0 > jj ? x0[k] * x0[k] : T(0)) + (r1 > jj ? x1[k] * x1[k] : T(0)));
                 nd = (mY - jj - 1 > 0) ? sqrt(s) : T(0);
                 ne = nd;
             }
-            if (lane == 0) {
-                dn[c - c0] = nd;
-                en[c - c0] = ne;
-            }
+            dn[k] = nd;
+            en[k] = ne;
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (pos[k] != NONE) a.perm_out[pos[k]] = cbase + k;
+    }
+}
+
+// changed[0..*nch) = ascending positions t with perm[t] != t (one workgroup)
+__global__ __launch_bounds__(1024) void k_perm_changed(int nv, const int32_t *perm,
+                                                       int32_t *changed, int32_t *nch) {
+    __shared__ int s_cnt[16];
+    __shared__ int s_base;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_base = 0;
+    __syncthreads();
+    for (int t0 = 0; t0 < nv; t0 += 1024) {
+        const int t = t0 + threadIdx.x;
+        const bool f = t < nv && perm[t] != t;
+        const unsigned long long bal = __ballot(f);
+        const int before = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) s_cnt[wave] = __popcll(bal);
+        __syncthreads();
+        int off = s_base;
+        for (int u = 0; u < wave; ++u) off += s_cnt[u];
+        if (f) changed[off + before] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int u = 0; u < 16; ++u) tot += s_cnt[u];
+            s_base += tot;
         }
         __syncthreads();
     }
-    if (w == 0) {
-        for (int t = tid; t < nv; t += YQ_T) a.perm_out[t] = perm[t];
-        if (tid == 0) {
-            int cnt = 0;
-            for (int t = 0; t < nv; ++t)
-                if (perm[t] != t) a.changed[cnt++] = t;
-            *a.nchanged = cnt;
-        }
-    }
+    if (threadIdx.x == 0) *nch = s_base;
 }
 
 // gather/scatter of the changed columns: Xout[:, t] = Xin[:, perm[t]]
@@ -643,25 +715,45 @@ __global__ void k_perm_int(int nv, int32_t *jp, const int32_t *perm, const int32
     for (int k = threadIdx.x; k < n; k += blockDim.x) jp[changed[k]] = sj[k];
 }
 
-// apply the panel's column swaps (in order) to extra row blocks
+// Apply the panel's column swaps (s <-> swaps[s], in order, b <= 64) to the
+// columns 0..b-1 of two row blocks.  The swaps are composed into one gather
+// (new column t = old column src[t]); each workgroup moves a 64-row tile
+// through LDS.  Workgroups [0, ceil(r1/64)) cover X1, the rest X2.
 template <class T>
 __global__ __launch_bounds__(256) void k_apply_swaps(int b, const int32_t *swaps, long r1, T *X1,
                                                      long ld1, long r2, T *X2, long ld2) {
-    for (int s = 0; s < b; ++s) {
-        int p = swaps[s];
-        if (p != s) {
-            for (long i = threadIdx.x; i < r1; i += 256) {
-                T t = X1[i + (long)s * ld1];
-                X1[i + (long)s * ld1] = X1[i + (long)p * ld1];
-                X1[i + (long)p * ld1] = t;
-            }
-            for (long i = threadIdx.x; i < r2; i += 256) {
-                T t = X2[i + (long)s * ld2];
-                X2[i + (long)s * ld2] = X2[i + (long)p * ld2];
-                X2[i + (long)p * ld2] = t;
-            }
+    __shared__ int src[64];
+    __shared__ T tile[64 * 65];
+    if (threadIdx.x == 0) {
+        for (int s = 0; s < b; ++s) src[s] = s;
+        for (int s = 0; s < b; ++s) {
+            int p = swaps[s];
+            int t = src[s];
+            src[s] = src[p];
+            src[p] = t;
         }
-        __syncthreads();
+    }
+    const long nb1 = (r1 + 63) / 64;
+    long blk = blockIdx.x;
+    T *X = X1;
+    long ld = ld1, rows = r1;
+    if (blk >= nb1) {
+        blk -= nb1;
+        X = X2;
+        ld = ld2;
+        rows = r2;
+    }
+    const long i0 = blk * 64;
+    const int ni = (int)min(64L, rows - i0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * b; e += 256) {
+        int i = e & 63, t = e >> 6;
+        if (i < ni) tile[i + t * 65] = X[(i0 + i) + (long)t * ld];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * b; e += 256) {
+        int i = e & 63, t = e >> 6;
+        if (i < ni && src[t] != t) X[(i0 + i) + (long)t * ld] = tile[i + src[t] * 65];
     }
 }
 
@@ -695,7 +787,6 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
     }
     T *Y = ar.alloc_n<T>((size_t)mY * n);
     gemm<T>('N', 'N', mY, n, m, T(1), G, mY, W, ldw, T(0), Y, mY);
-    T *V = ar.alloc_n<T>((size_t)mY * n);
     T *Vf = ar.alloc_n<T>((size_t)m * nb);
     T *Tm = ar.alloc_n<T>((size_t)nb * nb);
     int32_t *perm = ar.alloc_n<int32_t>((size_t)n);
@@ -707,36 +798,42 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
     T *BT = ar.alloc_n<T>((size_t)mY * nb);
     T *B = ar.alloc_n<T>((size_t)mY * nb);
     const int P_max = std::max(1, ctx().num_cu);
+    static_assert(mY <= YQ_MAXROWS, "sketch rows exceed the register layout");
+    SKL_REQUIRE(n <= (long)8 * YQ_W * P_max, SKL_E_UNSUPPORTED,
+                "HQRRP pivot kernel: too many columns for one cooperative grid");
     T *cand = ar.alloc_n<T>((size_t)2 * P_max);
-    int32_t *candi = ar.alloc_n<int32_t>((size_t)2 * P_max);
+    int32_t *candp = ar.alloc_n<int32_t>((size_t)2 * P_max);
+    T *ccol = ar.alloc_n<T>((size_t)2 * P_max * YQ_MAXROWS);
     for (long j = 0; j < mn; j += nb) {
         int b = (int)std::min<long>(nb, std::min(n - j, m - j));
         bool last = (j + nb >= m) || (j + nb >= n);
         long nv = n - j;
         if (!last) {
             // pivot block from Y(:, j:n)
-            SKL_HIP(hipMemcpyAsync(V, Y + j * mY, sizeof(T) * (size_t)mY * nv,
-                                   hipMemcpyDeviceToDevice, s));
             YqArgs<T> ya;
             ya.mY = mY;
             ya.nv = (int)nv;
             ya.nsteps = b;
-            int cpw = (int)std::max<long>(4, (nv + P_max - 1) / P_max);
-            ya.cpw = cpw;
-            ya.V = V;
+            ya.Y = Y + j * mY;
+            ya.ldy = mY;
             ya.perm_out = perm;
-            ya.changed = changed;
-            ya.nchanged = nch;
             ya.cand = cand;
-            ya.candi = candi;
-            int P = (int)((nv + cpw - 1) / cpw);
-            size_t sh = sizeof(int32_t) * (2 * (size_t)nv + 2) + sizeof(T) * (2 * (size_t)cpw + mY) + 64;
+            ya.candp = candp;
+            ya.ccol = ccol;
+            // columns per wave: keep the grid near 64 workgroups
+            int K = nv <= 1024 ? 1 : nv <= 2048 ? 2 : nv <= 4096 ? 4 : 8;
+            int P = (int)((nv + (long)YQ_W * K - 1) / ((long)YQ_W * K));
+            if (P > P_max) {
+                K = 8;
+                P = (int)((nv + (long)YQ_W * K - 1) / ((long)YQ_W * K));
+            }
             void *kargs[] = {(void *)&ya};
-            if (sh > 65536)
-                SKL_HIP(hipFuncSetAttribute((const void *)k_yqrp<T>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-            SKL_HIP(hipLaunchCooperativeKernel((const void *)k_yqrp<T>, dim3((unsigned)P),
-                                               dim3(YQ_T), kargs, (unsigned)sh, s));
+            const void *fn = K == 1   ? (const void *)k_yqrp<T, 1>
+                             : K == 2 ? (const void *)k_yqrp<T, 2>
+                             : K == 4 ? (const void *)k_yqrp<T, 4>
+                                      : (const void *)k_yqrp<T, 8>;
+            SKL_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)P), dim3(YQ_T), kargs, 0, s));
+            k_perm_changed<<<1, 1024, 0, s>>>((int)nv, perm, changed, nch);
             // permute W(:, j:n), Y(:, j:n), jpvt(j:n) by the new column order
             dim3 gw((unsigned)((m + 255) / 256), (unsigned)(2 * b));
             k_perm_gather<T><<<gw, 256, 0, s>>>(m, W + j * ldw, ldw, perm, changed, nch, tmpc);
@@ -750,7 +847,11 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
         // panel QRP of W(j:m, j:j+b), pivoting W(0:j, j:j+b) and Y(:, j:j+b)
         long mp = m - j;
         rowslice_qr<T>(mp, b, b, b, W + j + j * ldw, ldw, tau + j, jpvt + j, swaps, 1, ar);
-        k_apply_swaps<T><<<1, 256, 0, s>>>(b, swaps, j, W + j * ldw, ldw, mY, Y + j * mY, mY);
+        {
+            unsigned nblk = (unsigned)((j + 63) / 64 + (mY + 63) / 64);
+            k_apply_swaps<T><<<nblk, 256, 0, s>>>(b, swaps, j, W + j * ldw, ldw, mY, Y + j * mY,
+                                                  mY);
+        }
         SKL_CHECK_LAUNCH();
         panel_t<T>(mp, b, W + j + j * ldw, ldw, tau + j, Vf, Tm, ar);
         long nc2 = ncol - j - b;

@@ -45,6 +45,19 @@ void Arena::reserve(size_t bytes) {
     size_t want = bytes + (bytes >> 3);
     SKL_HIP(hipMalloc(&base_, want));
     cap_ = want;
+    over_ = peak_ = 0;
+}
+
+void Arena::reset() {
+    if (!overflow_.empty()) {
+        // hipFree waits for the device; the previous solve's buffers are dead
+        size_t want = std::max(peak_, cap_);
+        release();
+        reserve(want);
+    }
+    used_ = 0;
+    over_ = 0;
+    peak_ = 0;
 }
 
 void *Arena::alloc(size_t bytes) {
@@ -53,11 +66,14 @@ void *Arena::alloc(size_t bytes) {
     if (used_ + sz <= cap_) {
         void *p = base_ + used_;
         used_ += sz;
+        peak_ = std::max(peak_, used_ + over_);
         return p;
     }
     void *p = nullptr;
     SKL_HIP(hipMalloc(&p, sz));
     overflow_.push_back(p);
+    over_ += sz;
+    peak_ = std::max(peak_, used_ + over_);
     return p;
 }
 
@@ -93,12 +109,25 @@ Context &ctx() {
 
 void lsqr_release_graphs();
 
+void *Context::workspace(size_t bytes) {
+    if (bytes > ws_cap) {
+        SKL_HIP(hipStreamSynchronize(stream));
+        if (ws) SKL_HIP(hipFree(ws));
+        ws = nullptr;
+        ws_cap = 0;
+        SKL_HIP(hipMalloc(&ws, bytes));
+        ws_cap = bytes;
+    }
+    return ws;
+}
+
 // Frees the calling thread's stream, arena and cached graphs (worker exit).
 void release_thread_context() {
     if (!t_ctx) return;
     (void)hipStreamSynchronize(t_ctx->stream);
     lsqr_release_graphs();
     t_ctx->arena.release();
+    if (t_ctx->ws) (void)hipFree(t_ctx->ws);
     if (t_ctx->pinned) (void)hipHostFree(t_ctx->pinned);
     (void)hipStreamDestroy(t_ctx->stream);
     delete t_ctx;

@@ -38,7 +38,10 @@ struct Error : std::runtime_error {
 
 // ---------------------------------------------------------------------------
 // Device memory: a grow-only arena per context; buffers are carved with a
-// bump pointer and released all at once by reset().
+// bump pointer and released all at once by reset().  A solve that outgrows the
+// reservation gets extra blocks; the next reset() replaces base and extras by
+// one block of the high-water size, so repeated solves of one shape allocate
+// nothing after the first.
 // ---------------------------------------------------------------------------
 class Arena {
   public:
@@ -46,7 +49,7 @@ class Arena {
     void reserve(size_t bytes);        // ensure capacity (reallocates when empty)
     void *alloc(size_t bytes);         // 256-byte aligned
     template <class T> T *alloc_n(size_t n) { return static_cast<T *>(alloc(n * sizeof(T))); }
-    void reset() { used_ = 0; }
+    void reset();                      // all buffers dead; caller's stream work is done
     // release everything carved after `mark` (stream-ordered reuse: later
     // kernels on the same stream may take the memory over)
     void rewind(size_t mark) {
@@ -59,6 +62,7 @@ class Arena {
   private:
     char *base_ = nullptr;
     size_t cap_ = 0, used_ = 0;
+    size_t over_ = 0, peak_ = 0;   // bytes in extra blocks; high-water of used_ + over_
     std::vector<void *> overflow_; // extra blocks when the reservation was too small
 };
 
@@ -70,10 +74,15 @@ struct Context {
     int coop_ok = 0;
     size_t lds_max = 0;
     void *pinned = nullptr; // small pinned host buffer for scalars
+    // grow-only scratch for split-K partial products (stream-ordered reuse)
+    void *ws = nullptr;
+    size_t ws_cap = 0;
+    void *workspace(size_t bytes);
 };
 
 Context &ctx();                       // default per-thread context
 void ensure_init();
+void release_thread_context();        // frees the calling thread's context
 
 double wall();
 

@@ -707,7 +707,7 @@ int skl_init(int device) {
     });
 }
 
-void skl_finalize(void) {}
+void skl_finalize(void) { skl::release_thread_context(); }
 
 const char *skl_error_string(int code) {
     switch (code) {

@@ -76,6 +76,13 @@ def test_host_pattern_bit_exact_with_oracle(skl, O, seed, m, n, k, dense):
     assert np.array_equal(rows, ro) and np.array_equal(vals, vo)
 
 
+def test_drivers_built_and_report_usage():
+    bindir = os.path.join(ROOT, "ski-lls_amd", "drivers", "bin")
+    for d in ["ls_blendenpik_hashing", "ls_blendenpik_hashing_sparse_input", "ls_qr"]:
+        out = subprocess.run([os.path.join(bindir, d)], capture_output=True, text=True)
+        assert out.returncode == 2 and "usage: " + d in out.stderr
+
+
 def test_cpp_example_compiles(tmp_path):
     """A caller written against the reference's ski-lls.h API builds and links."""
     src = tmp_path / "ex.cpp"

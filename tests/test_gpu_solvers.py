@@ -140,8 +140,28 @@ def test_mixed_precision_dense(skl):
 
 
 def test_mixed_precision_escalates_on_ill_conditioned_sparse(skl):
-    A, b = sparse_problem(8, n=4000, d=100, density=0.02, decades=10)
+    # no unit diagonal: cond(A) ~ 1e10 sits beyond what an FP32 factor resolves
+    A, b = sparse_problem(8, n=4000, d=100, density=0.02, decades=10, unit_diag=False)
+    assert np.linalg.cond(A.toarray()) > 1e8
     x, info = skl.sparse_solve(A, b, gamma=1.5, k=3, it_tol=1e-10, precision=skl.SKL_MIXED)
     assert info.escalated == 1 and info.flag == 0
     x64, _ = skl.sparse_solve(A, b, gamma=1.5, k=3, it_tol=1e-10)
     assert rel(x, x64) < 1e-8
+
+
+@pytest.mark.parametrize("n,d", [(6000, 300), (5003, 1500), (20000, 2100)])
+def test_fused_lsqr_pass_matches_two_pass(skl, O, monkeypatch, n, d):
+    """The one-pass LSQR kernel (A z and A^T u from one read of A) against the
+    two-pass kernels and the oracle; shapes cover one, two and three column
+    sums per thread and a ragged last row tile."""
+    A, b = c1_problem(11, n=n, d=d)
+    monkeypatch.setenv("SKL_LSQR_FUSED", "0")
+    x2, i2 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=4)
+    monkeypatch.setenv("SKL_LSQR_FUSED", "1")
+    x1, i1 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=4)
+    monkeypatch.delenv("SKL_LSQR_FUSED")
+    assert i1.flag == 0 and abs(i1.it - i2.it) <= 2
+    assert rel(x1, x2) < 1e-8
+    if n * d <= 6000 * 300:
+        xo, io, _ = O.ls_dense(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=4)
+        assert rel(x1, xo) < 1e-8
