@@ -18,6 +18,15 @@ template <class T> __device__ inline T wave_sum(T v) {
     return v;
 }
 
+// K independent wave sums, interleaved level by level (same result per value
+// as wave_sum)
+template <int K, class T> __device__ inline void wave_sum_k(T (&v)[K]) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] += __shfl_xor(v[k], o, 64);
+}
+
 // value of v in lane l (l wave-uniform)
 __device__ inline double readlane(double v, int l) {
     long long b = __double_as_longlong(v);
@@ -27,6 +36,28 @@ __device__ inline double readlane(double v, int l) {
 }
 __device__ inline float readlane(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Barrier over every workgroup of a launch.  The kernels that use it keep
+// their grids far below what the device holds at once (at most a quarter to
+// a half of the CUs, one workgroup per CU), so all workgroups are resident
+// without a cooperative launch.  *ctr is zeroed by the host before the launch;
+// `target` starts at 0 in every workgroup.  Agent-scope fences around the
+// arrival publish this workgroup's global writes and expose the others'.
+// The wait gives up after ~2^28 polls (seconds) rather than hang the device.
+__device__ inline void grid_barrier(unsigned *ctr, unsigned &target) {
+    __syncthreads();
+    target += gridDim.x;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target &&
+               ++spins < (1u << 28))
+            __builtin_amdgcn_s_sleep(1);
+        __threadfence();
+    }
+    __syncthreads();
 }
 
 template <class T> __device__ inline T wave_max(T v) {

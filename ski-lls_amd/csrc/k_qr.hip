@@ -1,8 +1,8 @@
 // k_qr.hip -- Householder QR family of the small sketched matrix SA on gfx950.
 //
 //  * qr_householder: blocked right-looking QR (dgeqrf semantics,
-//    src/dense_qr.cpp:8-33).  Panels are factored by a cooperative kernel in
-//    which each workgroup holds a slice of rows in LDS; the column reductions
+//    src/dense_qr.cpp:8-33).  Panels are factored by a multi-workgroup kernel
+//    in which each workgroup holds a slice of rows in LDS; the column reductions
 //    of one Householder step (||x||^2 and the products with every other panel
 //    column) travel in a single grid-wide all-reduce, so a column costs one
 //    grid barrier.  Trailing updates are GEMMs on the matrix cores.
@@ -11,22 +11,23 @@
 //    partial column norms and makes the identical pivot decision.
 //  * cpqr_hqrrp: HQRRP (hqrrp/lapack_compatible_sources/
 //    NoFLA_HQRRP_WY_blk_var4_skint.c:313-581): block pivots from a Gaussian
-//    sketch Y = G A (column-sliced cooperative kernel), panel QR with pivoting,
+//    sketch Y = G A (column-sliced multi-workgroup kernel), panel QR with pivoting,
 //    GEMM updates and the Y downdate.
-#include <hip/hip_cooperative_groups.h>
-
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "k_common.h"
 
-namespace cg = cooperative_groups;
+// Phase marks for an external timing harness; they compile to nothing here.
+#ifndef SKL_PHASE
+#define SKL_PHASE(kernel, mark)
+#endif
 
 namespace skl {
 
 // ---------------------------------------------------------------------------
-// Row-sliced cooperative Householder kernel
+// Row-sliced multi-workgroup Householder kernel (one grid barrier per column)
 // ---------------------------------------------------------------------------
 template <class T> struct RsArgs {
     long m;        // rows
@@ -46,6 +47,7 @@ template <class T> struct RsArgs {
     T *red;         // [2][nc]
     T *rowv;        // [2][nc]
     T *rcp;         // [P][nc]
+    unsigned *bar;  // barrier counter, zero at launch
 };
 
 template <class T> __device__ inline T dlapy2_d(T x, T y) {
@@ -60,11 +62,11 @@ template <class T> __device__ inline T eps_half();
 template <> __device__ inline double eps_half<double>() { return 1.1102230246251565e-16; }
 template <> __device__ inline float eps_half<float>() { return 5.9604644775390625e-08f; }
 
-constexpr int RS_T = 256;
+constexpr int RS_T = 512;
 
 template <class T>
 __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
-    cg::grid_group grid = cg::this_grid();
+    unsigned bar_target = 0;
     extern __shared__ unsigned char smem_raw[];
     const int P = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
     const int nc = a.nc, ldl = a.ldl;
@@ -86,23 +88,29 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
         for (int i = lane; i < nr; i += 64) S[i + q * ldl] = a.W[(r0 + i) + (long)q * a.ldw];
     __syncthreads();
 
-    // all-reduce of vals[q] for q in [q0, nc) (q0 relative), result into Gs
-    auto allreduce = [&](int buf, int q0) {
+    // all-reduce of vals[q] for q in [q0, nc) (q0 relative), result into Gs;
+    // with_row also copies the published pivot row (rowv[buf]) into Rs, its
+    // loads issued beside the partial sums
+    auto allreduce = [&](int buf, int q0, bool with_row) {
         T *pw = a.part + ((size_t)buf * P + w) * nc;
         // values already in pw[q] for q >= q0
-        __threadfence();
-        grid.sync();
+        grid_barrier(a.bar, bar_target);
         if (!a.two_phase) {
             // four adjacent lanes per column, each summing every fourth workgroup
             const int sub = tid & 3;
             for (int qb = q0; qb < nc; qb += RS_T / 4) {
                 const int q = qb + (tid >> 2);
-                T s = 0;
-                if (q < nc)
+                T s = 0, rv = 0;
+                if (q < nc) {
+                    if (with_row && sub == 0) rv = a.rowv[(size_t)buf * nc + q];
                     for (int u = sub; u < P; u += 4) s += a.part[((size_t)buf * P + u) * nc + q];
+                }
                 s += __shfl_xor(s, 1, 64);
                 s += __shfl_xor(s, 2, 64);
-                if (q < nc && sub == 0) Gs[q] = s;
+                if (q < nc && sub == 0) {
+                    Gs[q] = s;
+                    if (with_row) Rs[q] = rv;
+                }
             }
         } else {
             int ncol = nc - q0;
@@ -113,9 +121,11 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
                 for (int u = 0; u < P; ++u) s += a.part[((size_t)buf * P + u) * nc + q];
                 a.red[(size_t)buf * nc + q] = s;
             }
-            __threadfence();
-            grid.sync();
-            for (int q = q0 + tid; q < nc; q += RS_T) Gs[q] = a.red[(size_t)buf * nc + q];
+            grid_barrier(a.bar, bar_target);
+            for (int q = q0 + tid; q < nc; q += RS_T) {
+                Gs[q] = a.red[(size_t)buf * nc + q];
+                if (with_row) Rs[q] = a.rowv[(size_t)buf * nc + q];
+            }
         }
         (void)pw;
     };
@@ -131,7 +141,7 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
             }
             pw[q] = s;
         }
-        allreduce(1, 0);
+        allreduce(1, 0, false);
         __syncthreads();
         for (int q = tid; q < npiv; q += RS_T) {
             vn1[q] = sqrt(Gs[q]);
@@ -142,6 +152,7 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
 
     for (int jj = 0; jj < a.nsteps; ++jj) {
         const int buf = jj & 1;
+        SKL_PHASE(0, 0);
         if (npiv > 0) {
             // first index of max |vn1[jj..npiv)| (idamax), identical in all workgroups
             if (wave == 0) {
@@ -184,6 +195,7 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
             if (w == 0 && tid == 0 && a.swaps) a.swaps[jj] = pvt;
             __syncthreads();
         }
+        SKL_PHASE(0, 1);
         // partial Gram row: g[q] = sum_{rows > jj} S(:, jj) S(:, q), q >= jj
         T *pw = a.part + ((size_t)buf * P + w) * nc;
         const int ib = (int)max(0L, (long)jj + 1 - r0); // first local row below the diagonal
@@ -207,9 +219,10 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
         if (owner)
             for (int q = jj + tid; q < nc; q += RS_T)
                 a.rowv[(size_t)buf * nc + q] = S[(jj - r0) + (size_t)q * ldl];
-        allreduce(buf, jj);
-        for (int q = jj + tid; q < nc; q += RS_T) Rs[q] = a.rowv[(size_t)buf * nc + q];
+        SKL_PHASE(0, 2);
+        allreduce(buf, jj, true);
         __syncthreads();
+        SKL_PHASE(0, 3);
         // Householder scalars (dlarfg), computed redundantly
         const T alpha = Rs[jj];
         const T xnorm = sqrt(Gs[jj]);
@@ -240,6 +253,7 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
         }
         if (w == 0 && tid == 0) a.tau[jj] = tau;
         __syncthreads();
+        SKL_PHASE(0, 4);
         if (npiv > 0) {
             // partial norm downdating with the new row jj of R
             int any = 0;
@@ -280,8 +294,7 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
                         }
                     rw[q] = s;
                 }
-                __threadfence();
-                grid.sync();
+                grid_barrier(a.bar, bar_target);
                 for (int q = jj + 1 + tid; q < npiv; q += RS_T) {
                     if (!flags[q]) continue;
                     if (jj < a.m - 1) {
@@ -304,17 +317,12 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
         for (int i = lane; i < nr; i += 64) a.W[(r0 + i) + (long)q * a.ldw] = S[i + q * ldl];
 }
 
-template <class T>
-static void launch_coop(void (*fn)(RsArgs<T>), int P, size_t sh, RsArgs<T> &args) {
-    void *kargs[] = {(void *)&args};
-    if (sh > 65536)
-        SKL_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)sh));
-    SKL_HIP(hipLaunchCooperativeKernel((const void *)fn, dim3((unsigned)P), dim3(RS_T), kargs,
-                                       (unsigned)sh, ctx().stream));
-}
+// Workgroups of one grid-barrier launch: one per CU at most, and no more than
+// half the device, so every workgroup is resident even beside a second such
+// launch from another stream.
+static int barrier_grid_cap() { return std::max(1, ctx().num_cu / 2); }
 
-static constexpr size_t RS_LDS_BUDGET = 120 * 1024;
+static constexpr size_t RS_LDS_BUDGET = 150 * 1024;
 
 // Factor (or pivot-factor) the m x nc block at W: nsteps reflectors, pivoting
 // over the first npiv columns when npiv > 0.
@@ -322,11 +330,12 @@ template <class T>
 static void rowslice_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T *tau,
                         int32_t *jpvt, int32_t *swaps, int hq_norms, Arena &ar) {
     if (nsteps <= 0) return;
-    Context &C = ctx();
-    int P_max = std::max(1, C.num_cu);
+    const int P_half = barrier_grid_cap();
+    const int P_max = std::max(1, ctx().num_cu);
     size_t fixed = sizeof(T) * (2 * (size_t)nc + 2 * (size_t)std::max(npiv, 1) + RS_T) +
                    sizeof(int) * ((size_t)npiv + 4) + 64;
-    // smallest P whose slice fits, and no more rows per slice than needed
+    // fewest workgroups whose slices fit in LDS (each step costs one barrier,
+    // whose latency grows with the workgroup count) ...
     long rpw = -1;
     int P = 0;
     for (int p = 1; p <= P_max; ++p) {
@@ -340,10 +349,10 @@ static void rowslice_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T 
         }
     }
     SKL_REQUIRE(rpw > 0, SKL_E_UNSUPPORTED, "QR panel too large for the LDS-resident kernel");
-    // more workgroups shorten each column step when the slices are tall
-    long target_rows = 128;
+    // ... but at least ~16 workgroups so the per-step row work stays short
+    long target_rows = 256;
     if (rpw > target_rows) {
-        int p2 = (int)std::min<long>(P_max, (m + target_rows - 1) / target_rows);
+        int p2 = (int)std::min<long>(std::min(P_half, 16), (m + target_rows - 1) / target_rows);
         if (p2 > P) {
             P = p2;
             rpw = (m + P - 1) / P;
@@ -368,8 +377,18 @@ static void rowslice_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T 
     a.red = ar.alloc_n<T>((size_t)2 * nc);
     a.rowv = ar.alloc_n<T>((size_t)2 * nc);
     a.rcp = ar.alloc_n<T>((size_t)P * nc);
+    a.bar = ar.alloc_n<unsigned>(1);
     size_t sh = sizeof(T) * (size_t)a.ldl * nc + fixed;
-    launch_coop<T>(k_qr_rowslice<T>, P, sh, a);
+    hipStream_t s = ctx().stream;
+    SKL_HIP(hipMemsetAsync(a.bar, 0, sizeof(unsigned), s));
+    if (sh > 65536)
+        SKL_HIP(hipFuncSetAttribute((const void *)k_qr_rowslice<T>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    // grids over more than half the device (wide matrices in the greedy path)
+    // always run alone
+    BarrierLaunch guard(P > P_half);
+    k_qr_rowslice<T><<<P, RS_T, sh, s>>>(a);
+    SKL_CHECK_LAUNCH();
 }
 
 // ---------------------------------------------------------------------------
@@ -478,11 +497,10 @@ template void cpqr_greedy<float>(long, long, long, float *, long, int32_t *, flo
 // ---------------------------------------------------------------------------
 // HQRRP pieces
 // ---------------------------------------------------------------------------
-constexpr int YQ_W = 16;        // waves per workgroup
-constexpr int YQ_T = 64 * YQ_W;
 constexpr int YQ_MAXROWS = 128; // two rows per lane
+constexpr int YQ_MAXP = 128;    // workgroups of one launch
 
-// Column-sliced cooperative QRP of Y (mY x nv, mY <= 128) for nsteps pivots
+// Column-sliced multi-workgroup QRP of Y (mY x nv, mY <= 128) for nsteps pivots
 // with the HQRRP norm downdating (skint.c:920-990).  Every wave keeps K
 // columns in registers (lane l holds rows l and l + 64), together with their
 // partial norms and current logical positions; Y itself is only read.
@@ -496,9 +514,11 @@ template <class T> struct YqArgs {
     const T *Y;
     long ldy;
     int32_t *perm_out;
-    T *cand;       // [2][P]
-    int32_t *candp; // [2][P]
-    T *ccol;       // [2][P][YQ_MAXROWS]
+    T *cand;        // [2][P] candidate norm
+    int32_t *candp; // [2][P] candidate logical position
+    T *ctau;        // [2][P] tau of the candidate's reflector
+    T *cvec;        // [2][P][YQ_MAXROWS] candidate's reflector v (v_jj = 1, zero above)
+    unsigned *bar;  // barrier counter, zero at launch
 };
 
 template <class T>
@@ -506,33 +526,87 @@ __device__ inline bool yq_better(T v, int p, T bv, int bp) {
     return v > bv || (v == bv && p < bp);
 }
 
-template <class T, int K>
-__global__ __launch_bounds__(YQ_T) void k_yqrp(YqArgs<T> a) {
-    cg::grid_group grid = cg::this_grid();
-    __shared__ T s_wv[YQ_W];
-    __shared__ int s_wp[YQ_W];
+template <int K> struct Log2 {
+    static constexpr int v = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : 4;
+};
+
+// Sums each of v[0..K) over the wave.  log2 K halving exchanges (on lane bits
+// 5, 4, ...) leave one value per lane, then plain butterflies finish; every
+// lane ends with the total of value c(lane), whose holder split_lane(c) names.
+template <int K, class T> __device__ inline T wave_sum_split(T (&v)[K], int lane) {
+    constexpr int L = Log2<K>::v;
+#pragma unroll
+    for (int s = 0; s < L; ++s) {
+        const int m = 32 >> s, half = K >> (s + 1);
+        const bool up = (lane & m) != 0;
+#pragma unroll
+        for (int j = 0; j < half; ++j) {
+            T send = up ? v[j] : v[j + half];
+            T keep = up ? v[j + half] : v[j];
+            v[j] = keep + __shfl_xor(send, m, 64);
+        }
+    }
+    T r = v[0];
+#pragma unroll
+    for (int m = 32 >> L; m > 0; m >>= 1) r += __shfl_xor(r, m, 64);
+    return r;
+}
+template <int K> __device__ inline int split_lane(int c) {
+    constexpr int L = Log2<K>::v;
+    int l = 0;
+#pragma unroll
+    for (int b = 0; b < L; ++b)
+        if ((c >> b) & 1) l |= 32 >> (L - 1 - b);
+    return l;
+}
+
+// K columns per wave, W waves per workgroup
+template <class T, int K, int W>
+__global__ __launch_bounds__(64 * W) void k_yqrp(YqArgs<T> a) {
+    __shared__ T s_wv[W];
+    __shared__ int s_wp[W];
+    __shared__ T s_norm[2 * W * K]; // partial norms (dn, en) of each wave's columns
+    extern __shared__ unsigned char yq_smem[];
     const int P = gridDim.x, w = blockIdx.x;
+    T *s_vec = reinterpret_cast<T *>(yq_smem);      // P x YQ_MAXROWS candidate reflectors
+    T *s_cv = s_vec + (size_t)P * YQ_MAXROWS;       // P candidate norms
+    T *s_tau = s_cv + P;                            // P candidate taus
+    int *s_cp = reinterpret_cast<int *>(s_tau + P); // P candidate positions
+    unsigned bar_target = 0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = lane, r1 = lane + 64;
     const int mY = a.mY, nv = a.nv;
     const int NONE = 0x7fffffff;
     const T tol3z = sqrt(eps_half<T>());
-    const int cbase = (w * YQ_W + wave) * K;
-    T x0[K], x1[K], dn[K], en[K];
+    const int cbase = (w * W + wave) * K;
+    T x0[K], x1[K];
+    // dn, en of skint.c:920-990 are wave-uniform: kept in LDS, and the scalar
+    // arithmetic on them runs lane-parallel (lane k works on column k)
+    T *dn = s_norm + 2 * K * wave, *en = dn + K;
     int pos[K];
+    {
+        T sq[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int c = cbase + k;
-        const bool ok = c < nv;
-        const T *col = a.Y + (long)(ok ? c : 0) * a.ldy;
-        x0[k] = (ok && lane < mY) ? col[lane] : T(0);
-        x1[k] = (ok && lane + 64 < mY) ? col[lane + 64] : T(0);
-        T s = wave_sum(x0[k] * x0[k] + x1[k] * x1[k]);
-        dn[k] = sqrt(s);
-        en[k] = dn[k];
-        pos[k] = ok ? c : NONE;
+        for (int k = 0; k < K; ++k) {
+            const int c = cbase + k;
+            const bool ok = c < nv;
+            const T *col = a.Y + (long)(ok ? c : 0) * a.ldy;
+            x0[k] = (ok && r0 < mY) ? col[r0] : T(0);
+            x1[k] = (ok && r1 < mY) ? col[r1] : T(0);
+            sq[k] = x0[k] * x0[k] + x1[k] * x1[k];
+            pos[k] = ok ? c : NONE;
+        }
+        T tot = wave_sum_split<K>(sq, lane);
+        T mine = __shfl(tot, split_lane<K>(lane & (K - 1)), 64);
+        if (lane < K) {
+            T nrm = sqrt(mine);
+            dn[lane] = nrm;
+            en[lane] = nrm;
+        }
     }
     for (int jj = 0; jj < a.nsteps; ++jj) {
         const int buf = jj & 1;
+        SKL_PHASE(1, 0);
         // best remaining column of this wave, then of this workgroup
         T bv = T(-1);
         int bp = NONE, bk = 0;
@@ -550,13 +624,15 @@ __global__ __launch_bounds__(YQ_T) void k_yqrp(YqArgs<T> a) {
         __syncthreads();
         T gv = T(-1);
         int gp = NONE, gw = 0;
-        for (int u = 0; u < YQ_W; ++u)
+        for (int u = 0; u < W; ++u)
             if (yq_better(s_wv[u], s_wp[u], gv, gp)) {
                 gv = s_wv[u];
                 gp = s_wp[u];
                 gw = u;
             }
         if (wave == gw) {
+            // publish the workgroup's candidate as its reflector over rows
+            // jj.. (dlarfg), so the other workgroups need no scalar work
             T c0 = T(0), c1 = T(0);
 #pragma unroll
             for (int k = 0; k < K; ++k)
@@ -564,22 +640,42 @@ __global__ __launch_bounds__(YQ_T) void k_yqrp(YqArgs<T> a) {
                     c0 = x0[k];
                     c1 = x1[k];
                 }
-            T *dst = a.ccol + ((size_t)buf * P + w) * YQ_MAXROWS;
-            dst[lane] = c0;
-            dst[lane + 64] = c1;
+            const T alpha = (jj < 64) ? readlane(c0, jj) : readlane(c1, jj - 64);
+            const T xnorm = sqrt(wave_sum((r0 > jj ? c0 * c0 : T(0)) + (r1 > jj ? c1 * c1 : T(0))));
+            T tau = 0, sc = 0;
+            if (xnorm != T(0)) {
+                T beta = -copysign(dlapy2_d(alpha, xnorm), alpha);
+                tau = (beta - alpha) / beta;
+                sc = T(1) / (alpha - beta);
+            }
+            T *dst = a.cvec + ((size_t)buf * P + w) * YQ_MAXROWS;
+            dst[r0] = r0 > jj ? sc * c0 : (r0 == jj ? T(1) : T(0));
+            dst[r1] = r1 > jj ? sc * c1 : (r1 == jj ? T(1) : T(0));
             if (lane == 0) {
                 a.cand[buf * P + w] = gv;
                 a.candp[buf * P + w] = gp;
+                a.ctau[buf * P + w] = tau;
             }
         }
-        __threadfence();
-        grid.sync();
+        SKL_PHASE(1, 1);
+        grid_barrier(a.bar, bar_target);
+        SKL_PHASE(1, 2);
+        // every workgroup's candidate, staged in LDS in one round trip
+        for (int e = threadIdx.x; e < P * YQ_MAXROWS; e += 64 * W)
+            s_vec[e] = a.cvec[(size_t)buf * P * YQ_MAXROWS + e];
+        for (int t = threadIdx.x; t < P; t += 64 * W) {
+            s_cv[t] = a.cand[buf * P + t];
+            s_cp[t] = a.candp[buf * P + t];
+            s_tau[t] = a.ctau[buf * P + t];
+        }
+        __syncthreads();
+        SKL_PHASE(1, 3);
         // global winner, computed identically by every wave
         T v = T(-1);
         int p = NONE, u = 0;
         for (int t = lane; t < P; t += 64) {
-            T cv = a.cand[buf * P + t];
-            int cp = a.candp[buf * P + t];
+            T cv = s_cv[t];
+            int cp = s_cp[t];
             if (yq_better(cv, cp, v, p)) {
                 v = cv;
                 p = cp;
@@ -597,57 +693,72 @@ __global__ __launch_bounds__(YQ_T) void k_yqrp(YqArgs<T> a) {
             }
         }
         const int pl = p;
-        const T *src = a.ccol + ((size_t)buf * P + u) * YQ_MAXROWS;
-        const T p0 = src[lane], p1 = src[lane + 64];
-        const int r0 = lane, r1 = lane + 64;
-        // Householder vector of the pivot column over rows jj..mY-1 (dlarfg)
-        const T alpha = (jj < 64) ? readlane(p0, jj) : readlane(p1, jj - 64);
-        T xs = (r0 > jj ? p0 * p0 : T(0)) + (r1 > jj ? p1 * p1 : T(0));
-        xs = wave_sum(xs);
-        const T xnorm = sqrt(xs);
-        T tau = 0, sc = 0;
-        if (xnorm != T(0)) {
-            T beta = -copysign(dlapy2_d(alpha, xnorm), alpha);
-            tau = (beta - alpha) / beta;
-            sc = T(1) / (alpha - beta);
-        }
-        const T vv0 = r0 > jj ? sc * p0 : (r0 == jj ? T(1) : T(0));
-        const T vv1 = r1 > jj ? sc * p1 : (r1 == jj ? T(1) : T(0));
+        const T vv0 = s_vec[(size_t)u * YQ_MAXROWS + r0];
+        const T vv1 = s_vec[(size_t)u * YQ_MAXROWS + r1];
+        const T tau = s_tau[u];
+        SKL_PHASE(1, 4);
+        // own columns: logical swap of positions jj and pl, then C -= tau v (v^T C)
+        bool act[K];
+        T sd[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            // logical swap of positions jj and pl
             if (pos[k] == pl)
                 pos[k] = jj;
             else if (pos[k] == jj)
                 pos[k] = pl;
-            if (pos[k] <= jj || pos[k] == NONE) continue;
-            if (tau != T(0)) {
-                T s = wave_sum(vv0 * x0[k] + vv1 * x1[k]);
-                T f = -tau * s;
-                if (r0 >= jj) x0[k] = x0[k] + vv0 * f;
-                if (r1 >= jj) x1[k] = x1[k] + vv1 * f;
+            act[k] = pos[k] > jj && pos[k] != NONE;
+            sd[k] = vv0 * x0[k] + vv1 * x1[k];
+        }
+        if (tau != T(0)) {
+            T tot = wave_sum_split<K>(sd, lane);
+            const T f = -tau * __shfl(tot, split_lane<K>(lane & (K - 1)), 64);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const T fk = readlane(f, k);
+                if (act[k]) {
+                    if (r0 >= jj) x0[k] = x0[k] + vv0 * fk;
+                    if (r1 >= jj) x1[k] = x1[k] + vv1 * fk;
+                }
             }
-            T d0 = dn[k], nd = d0, ne = en[k];
-            bool recompute = false;
+        }
+        // norm downdating, lane k for column k
+        T xjv = T(0);
+        bool av = false;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const T xj = (jj < 64) ? readlane(x0[k], jj) : readlane(x1[k], jj - 64);
+            if (lane == k) {
+                xjv = xj;
+                av = act[k];
+            }
+        }
+        bool rec = false;
+        if (av) {
+            const T d0 = dn[lane];
             if (d0 != T(0)) {
-                T xj = (jj < 64) ? readlane(x0[k], jj) : readlane(x1[k], jj - 64);
-                T temp = fabs(xj) / d0;
+                T temp = fabs(xjv) / d0;
                 temp = (T(1) + temp) * (T(1) - temp);
                 if (temp < T(0)) temp = T(0);
-                T t5 = d0 / ne;
-                T temp2 = temp * t5 * t5;
+                const T t5 = d0 / en[lane];
+                const T temp2 = temp * t5 * t5;
                 if (temp2 <= tol3z)
-                    recompute = true;
+                    rec = true;
                 else
-                    nd = d0 * sqrt(temp);
+                    dn[lane] = d0 * sqrt(temp);
             }
-            if (recompute) {
-                T s = wave_sum((r0 > jj ? x0[k] * x0[k] : T(0)) + (r1 > jj ? x1[k] * x1[k] : T(0)));
-                nd = (mY - jj - 1 > 0) ? sqrt(s) : T(0);
-                ne = nd;
+        }
+        if (__ballot(rec)) { // rare: exact norms of the trailing rows
+            T rs[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                rs[k] = (r0 > jj ? x0[k] * x0[k] : T(0)) + (r1 > jj ? x1[k] * x1[k] : T(0));
+            T tot = wave_sum_split<K>(rs, lane);
+            const T mine = __shfl(tot, split_lane<K>(lane & (K - 1)), 64);
+            if (rec) {
+                const T nrm = (mY - jj - 1 > 0) ? sqrt(mine) : T(0);
+                dn[lane] = nrm;
+                en[lane] = nrm;
             }
-            dn[k] = nd;
-            en[k] = ne;
         }
     }
     if (lane == 0) {
@@ -797,13 +908,15 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
     T *B0 = ar.alloc_n<T>((size_t)mY * nb);
     T *BT = ar.alloc_n<T>((size_t)mY * nb);
     T *B = ar.alloc_n<T>((size_t)mY * nb);
-    const int P_max = std::max(1, ctx().num_cu);
+    const int P_max = std::min(YQ_MAXP, barrier_grid_cap());
     static_assert(mY <= YQ_MAXROWS, "sketch rows exceed the register layout");
-    SKL_REQUIRE(n <= (long)8 * YQ_W * P_max, SKL_E_UNSUPPORTED,
-                "HQRRP pivot kernel: too many columns for one cooperative grid");
+    SKL_REQUIRE(n <= (long)16 * 8 * P_max, SKL_E_UNSUPPORTED,
+                "HQRRP pivot kernel: too many columns for one grid");
     T *cand = ar.alloc_n<T>((size_t)2 * P_max);
     int32_t *candp = ar.alloc_n<int32_t>((size_t)2 * P_max);
-    T *ccol = ar.alloc_n<T>((size_t)2 * P_max * YQ_MAXROWS);
+    T *ctau = ar.alloc_n<T>((size_t)2 * P_max);
+    T *cvec = ar.alloc_n<T>((size_t)2 * P_max * YQ_MAXROWS);
+    unsigned *bar = ar.alloc_n<unsigned>(1);
     for (long j = 0; j < mn; j += nb) {
         int b = (int)std::min<long>(nb, std::min(n - j, m - j));
         bool last = (j + nb >= m) || (j + nb >= n);
@@ -819,20 +932,30 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
             ya.perm_out = perm;
             ya.cand = cand;
             ya.candp = candp;
-            ya.ccol = ccol;
-            // columns per wave: keep the grid near 64 workgroups
-            int K = nv <= 1024 ? 1 : nv <= 2048 ? 2 : nv <= 4096 ? 4 : 8;
-            int P = (int)((nv + (long)YQ_W * K - 1) / ((long)YQ_W * K));
-            if (P > P_max) {
-                K = 8;
-                P = (int)((nv + (long)YQ_W * K - 1) / ((long)YQ_W * K));
+            ya.ctau = ctau;
+            ya.cvec = cvec;
+            ya.bar = bar;
+            // columns per workgroup: at most 32 workgroups (barrier latency
+            // grows with the workgroup count) unless the columns need more
+            // (128-column workgroups: 16 columns x 8 waves in FP64, whose
+            // registers allow it, 8 x 16 in FP32)
+            const bool dbl = sizeof(T) == 8;
+            int K = nv <= 512 ? 1 : nv <= 1024 ? 2 : nv <= 2048 ? 4 : dbl ? 16 : 8;
+            int Wv = (K == 16) ? 8 : 16;
+            int P = (int)((nv + (long)Wv * K - 1) / ((long)Wv * K));
+            SKL_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned), s));
+            constexpr int KW = sizeof(T) == 8 ? 16 : 8, WW = sizeof(T) == 8 ? 8 : 16;
+            auto fn = K == 1 ? k_yqrp<T, 1, 16> : K == 2 ? k_yqrp<T, 2, 16>
+                      : K == 4 ? k_yqrp<T, 4, 16> : k_yqrp<T, KW, WW>;
+            size_t sh = (size_t)P * ((YQ_MAXROWS + 2) * sizeof(T) + sizeof(int));
+            if (sh > 65536)
+                SKL_HIP(hipFuncSetAttribute((const void *)fn,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+            {
+                BarrierLaunch guard;
+                fn<<<P, 64 * Wv, sh, s>>>(ya);
+                SKL_CHECK_LAUNCH();
             }
-            void *kargs[] = {(void *)&ya};
-            const void *fn = K == 1   ? (const void *)k_yqrp<T, 1>
-                             : K == 2 ? (const void *)k_yqrp<T, 2>
-                             : K == 4 ? (const void *)k_yqrp<T, 4>
-                                      : (const void *)k_yqrp<T, 8>;
-            SKL_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)P), dim3(YQ_T), kargs, 0, s));
             k_perm_changed<<<1, 1024, 0, s>>>((int)nv, perm, changed, nch);
             // permute W(:, j:n), Y(:, j:n), jpvt(j:n) by the new column order
             dim3 gw((unsigned)((m + 255) / 256), (unsigned)(2 * b));

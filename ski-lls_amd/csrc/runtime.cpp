@@ -1,5 +1,6 @@
 // runtime.cpp -- context, device arena, errors and the host-side hashing
 // pattern generator of the MI355X least-squares library.
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -80,6 +81,8 @@ void *Arena::alloc(size_t bytes) {
 // ---------------------------------------------------------------------------
 static thread_local Context *t_ctx = nullptr;
 static std::mutex g_init_mu;
+static std::mutex g_barrier_mu;            // serialises grid-barrier launches
+static std::atomic<int> g_live_ctx{0};     // host threads holding a context
 static int g_device = -1;
 
 void ensure_init() {
@@ -100,6 +103,18 @@ void ensure_init() {
     c->lds_max = prop.sharedMemPerBlock;
     SKL_HIP(hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault));
     t_ctx = c;
+    g_live_ctx.fetch_add(1);
+}
+
+BarrierLaunch::BarrierLaunch(bool always) : held_(always || g_live_ctx.load() > 1) {
+    if (held_) g_barrier_mu.lock();
+}
+
+BarrierLaunch::~BarrierLaunch() {
+    if (held_) {
+        (void)hipStreamSynchronize(ctx().stream);
+        g_barrier_mu.unlock();
+    }
 }
 
 Context &ctx() {
@@ -132,6 +147,7 @@ void release_thread_context() {
     (void)hipStreamDestroy(t_ctx->stream);
     delete t_ctx;
     t_ctx = nullptr;
+    g_live_ctx.fetch_sub(1);
 }
 
 void set_device(int dev) { g_device = dev; }

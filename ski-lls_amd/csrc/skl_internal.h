@@ -84,6 +84,22 @@ Context &ctx();                       // default per-thread context
 void ensure_init();
 void release_thread_context();        // frees the calling thread's context
 
+// Kernels with a grid barrier need every workgroup resident at once.  Each
+// keeps its grid to at most half the device, which is safe next to one other
+// such launch; when several host threads drive the device (the serving loop),
+// or a grid is wider (always = true), the guard serialises these launches
+// process-wide and holds the lock until the launch has finished.
+class BarrierLaunch {
+  public:
+    explicit BarrierLaunch(bool always = false);
+    ~BarrierLaunch();
+    BarrierLaunch(const BarrierLaunch &) = delete;
+    BarrierLaunch &operator=(const BarrierLaunch &) = delete;
+
+  private:
+    bool held_;
+};
+
 double wall();
 
 // ---------------------------------------------------------------------------
