@@ -15,6 +15,8 @@
 #include <map>
 #include <tuple>
 
+#include <hip/hip_ext.h>
+
 #include "k_common.h"
 
 namespace skl {
@@ -134,106 +136,125 @@ __global__ __launch_bounds__(GT_THREADS) void k_gemv_t_part(
 }
 
 // ---- one pass over A: u <- A z - coef u and q <- A^T u together -----------------
-// A persistent 1024-thread workgroup per CU walks 16-row tiles of the
-// column-major A (one 128-byte line per column).  Pass 1 streams the tile from
-// HBM for A z: lane l of wave w takes row l%16 of columns
-// 4 w + l/16 + 64 s, so a wave load covers four whole lines; the row sums are
-// reduced over lanes and waves in a fixed order, giving u for the 16 rows.
-// Pass 2 re-reads the same tile, now resident in the caches, with one thread
-// per column (j = tid + 1024 c, c < NQ) summing its 16 rows against u into a
-// register accumulator.  Each workgroup writes its partial A^T u to Q[wg][:]
-// and its partial sum of squares of u to Sb[wg].
-constexpr int FU_T = 1024, FU_R = 16;
+// A persistent 1024-thread workgroup per CU walks 4-row tiles of the
+// column-major A, holding each tile in registers: lane l of wave w takes row
+// l % 4 of the columns cs + 256 k (k < NC), cs = l / 4 + 16 w, so a quad of
+// lanes reads one 32-byte run of a column.  Tiles are loaded one ahead of use.
+// Per tile: the row sums of A z are reduced over the 16 quads of a wave
+// (shuffles) and over the 16 waves (LDS, one barrier, two alternating
+// buffers); every thread then forms u of its row itself, and the column sums
+// A(tile,:)^T u are reduced inside each quad by halving exchanges that leave
+// lane r of the quad with the columns k in [r NC/4, (r+1) NC/4), accumulated
+// in registers across tiles.  The tiles of one XCD (workgroup id mod 8) form
+// one contiguous band of rows, so the four 4-row tiles of a 128-byte line are
+// read at about the same time through the same L2.
+// Per workgroup the partial A^T u goes to Q[wg][:] and the sum of squares of u
+// to Sb[wg].
+constexpr int FU_T = 1024, FU_R = 4, FU_SLOTS = FU_T / FU_R;
 
-template <int NQ>
+template <int NC>
 __global__ __launch_bounds__(FU_T) void k_fused_dense(long n, long d, const double *__restrict__ A,
                                                       long lda, const double *__restrict__ z,
                                                       double *__restrict__ ut,
                                                       double *__restrict__ Sb,
                                                       double *__restrict__ Q,
                                                       const LsqrState *__restrict__ st) {
+    static_assert(NC % 4 == 0, "columns per thread must split over a quad");
     if (st->done) return;
     extern __shared__ double fsm[];
-    double *zs = fsm;                  // d
-    double *red = fsm + d;             // 16 waves x 16 rows
-    double *utile = red + 16 * FU_R;   // 16
+    double *zs = fsm;                    // d
+    double *red = fsm + d;               // [2][16 waves][4 rows]
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int row = lane & 15, cg = 4 * w + (lane >> 4);
+    const int r = lane & 3, cs = (lane >> 2) + 16 * w;
     for (long j = tid; j < d; j += FU_T) zs[j] = z[j];
     const double coef = st->coef;
-    __syncthreads();
+    // this workgroup's tiles: band of XCD x = wg % 8, local index wg / 8
+    const int G = gridDim.x, x = blockIdx.x & 7, nx = min(8, G);
+    const int gl = blockIdx.x >> 3, Gx = (G - x + 7) / 8; // workgroups on this XCD
     const long ntiles = (n + FU_R - 1) / FU_R;
-    const long cstride = 64 * lda;
-    double q[NQ];
+    const long band = (ntiles + nx - 1) / nx;
+    const long t_beg = (long)x * band, t_end = min(ntiles, t_beg + band);
+    bool colok[NC];
+    __syncthreads();
 #pragma unroll
-    for (int c = 0; c < NQ; ++c) q[c] = 0.0;
+    for (int k = 0; k < NC; ++k) colok[k] = cs + (long)FU_SLOTS * k < d;
+    const double *zq = zs + cs;
+    double q[NC / 4];
+#pragma unroll
+    for (int k = 0; k < NC / 4; ++k) q[k] = 0.0;
     double sq = 0.0;
-    for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const long i0 = t * FU_R;
-        const long i = i0 + row;
-        // pass 1: partial (A z)_i over columns cg + 64 s
+    auto load = [&](long t, double (&a)[NC], double &uo) {
+        const long i = t * FU_R + r;
+        const bool rv = (t < t_end) && (i < n);
+        const double *ap = A + (rv ? i : 0) + (long)cs * lda;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) a[k] = (rv && colok[k]) ? ap[(long)FU_SLOTS * k * lda] : 0.0;
+        uo = rv ? ut[i] : 0.0;
+    };
+    double a0[NC], a1[NC], u0 = 0.0, u1 = 0.0;
+    long t = t_beg + gl;
+    load(t, a0, u0);
+    int parity = 0;
+    // one tile: a holds it, uo the old u of this thread's row
+    auto tile = [&](long tt, double (&a)[NC], double uo) {
+        const long i = tt * FU_R + r;
         double p = 0.0;
-        if (i < n) {
-            const double *ap = A + i + (long)cg * lda;
-            long j = cg;
-            for (; j + 64 * 7 < d; j += 64 * 8) {
-                double a[8];
 #pragma unroll
-                for (int s = 0; s < 8; ++s) a[s] = ap[s * cstride];
-#pragma unroll
-                for (int s = 0; s < 8; ++s) p += a[s] * zs[j + 64 * s];
-                ap += 8 * cstride;
-            }
-            for (; j < d; j += 64) {
-                p += *ap * zs[j];
-                ap += cstride;
-            }
-        }
+        for (int k = 0; k < NC; ++k)
+            if (colok[k]) p += a[k] * zq[FU_SLOTS * k];
+        // over the 16 quads of the wave (lane bits 2..5)
+        p += __shfl_xor(p, 4, 64);
+        p += __shfl_xor(p, 8, 64);
         p += __shfl_xor(p, 16, 64);
         p += __shfl_xor(p, 32, 64);
-        if (lane < FU_R) red[w * FU_R + lane] = p;
+        double *rb = red + parity * (16 * FU_R);
+        if (lane < FU_R) rb[w * FU_R + lane] = p;
         __syncthreads();
-        if (tid < FU_R) {
-            double s = 0.0;
-            for (int ww = 0; ww < FU_T / 64; ++ww) s += red[ww * FU_R + tid];
-            const long ii = i0 + tid;
-            double v = 0.0;
-            if (ii < n) {
-                v = s - coef * ut[ii];
-                ut[ii] = v;
-                sq += v * v;
-            }
-            utile[tid] = v;
+        double s = 0.0;
+        for (int ww = 0; ww < FU_T / 64; ++ww) s += rb[ww * FU_R + r];
+        const bool rv = i < n;
+        const double uv = rv ? s - coef * uo : 0.0;
+        if (w == 0 && lane < FU_R && rv) {
+            ut[i] = uv;
+            sq += uv * uv;
         }
-        __syncthreads();
-        // pass 2: q_j += A(i0:i0+16, j)^T u
-        double uv[FU_R];
+        // column sums over the quad: halving exchanges on lane bits 1 and 0
+        double c[NC];
 #pragma unroll
-        for (int r = 0; r < FU_R; ++r) uv[r] = utile[r];
-        const int nr = (int)min((long)FU_R, n - i0);
+        for (int k = 0; k < NC; ++k) c[k] = a[k] * uv;
+        {
+            const bool up = (lane & 2) != 0;
 #pragma unroll
-        for (int c = 0; c < NQ; ++c) {
-            const long j = tid + (long)FU_T * c;
-            if (j < d) {
-                const double *ac = A + i0 + j * lda;
-                double s = 0.0;
-                if (nr == FU_R) {
-#pragma unroll
-                    for (int r = 0; r < FU_R; ++r) s += ac[r] * uv[r];
-                } else {
-                    for (int r = 0; r < nr; ++r) s += ac[r] * utile[r];
-                }
-                q[c] += s;
+            for (int k = 0; k < NC / 2; ++k) {
+                double send = up ? c[k] : c[k + NC / 2];
+                double keep = up ? c[k + NC / 2] : c[k];
+                c[k] = keep + __shfl_xor(send, 2, 64);
             }
-            // one column's 16 loads in flight at a time keeps the register count low
-            __builtin_amdgcn_sched_barrier(0);
         }
-        __syncthreads(); // utile and red are rewritten by the next tile
+        {
+            const bool up = (lane & 1) != 0;
+#pragma unroll
+            for (int k = 0; k < NC / 4; ++k) {
+                double send = up ? c[k] : c[k + NC / 4];
+                double keep = up ? c[k + NC / 4] : c[k];
+                q[k] += keep + __shfl_xor(send, 1, 64);
+            }
+        }
+        parity ^= 1;
+    };
+    for (; t < t_end; t += 2L * Gx) {
+        load(t + Gx, a1, u1);
+        tile(t, a0, u0);
+        if (t + Gx >= t_end) break;
+        load(t + 2L * Gx, a0, u0);
+        tile(t + Gx, a1, u1);
     }
+    // lane r of a quad holds columns k = r NC/4 + j
 #pragma unroll
-    for (int c = 0; c < NQ; ++c) {
-        const long j = tid + (long)FU_T * c;
-        if (j < d) Q[(long)blockIdx.x * d + j] = q[c];
+    for (int j = 0; j < NC / 4; ++j) {
+        const int k = r * (NC / 4) + j;
+        const long c = cs + (long)FU_SLOTS * k;
+        if (c < d) Q[(long)blockIdx.x * d + c] = q[j];
     }
     sq = block_sum<FU_T>(sq);
     if (tid == 0) Sb[blockIdx.x] = sq;
@@ -486,9 +507,9 @@ struct LsqrPlan {
     int G = 0, nq = 0;  // fused: workgroups, columns per thread in pass 2
 };
 
-// the one-pass kernel keeps d / 1024 column sums per thread in registers
+// the one-pass kernel holds d / 256 columns of a tile per thread in registers
 static bool fused_wanted(const LsqrProblem &p) {
-    if (p.csr_ptr || p.d > 4 * FU_T || p.n < 16 * 256) return false;
+    if (p.csr_ptr || p.d > 16 * FU_SLOTS || p.n < 64 * FU_R * 8) return false;
     const char *e = getenv("SKL_LSQR_FUSED");
     return e && e[0] == '1';
 }
@@ -523,8 +544,8 @@ static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
     if (fused_wanted(p)) {
         q.fused = true;
         q.G = num_cu;
-        long need = (p.d + FU_T - 1) / FU_T;
-        q.nq = need <= 1 ? 1 : need <= 2 ? 2 : 4;
+        long need = (p.d + FU_SLOTS - 1) / FU_SLOTS;
+        q.nq = need <= 4 ? 4 : need <= 8 ? 8 : 16;
     }
     return q;
 }
@@ -536,30 +557,43 @@ struct LsqrBufs {
     LsqrState *st;
 };
 
+template <class T> struct Ident {
+    typedef T type;
+};
+
+// Launches k on s.  With timing events (e0 non-null) they are recorded by the
+// dispatch itself (hipExtLaunchKernelGGL), so their interval is the kernel's
+// own execution and not the stream's event commands around it.
+template <class... P>
+static void launch_ev(void (*k)(P...), dim3 g, dim3 b, uint32_t sh, hipStream_t s, hipEvent_t e0,
+                      hipEvent_t e1, typename Ident<P>::type... args) {
+    if (e0)
+        hipExtLaunchKernelGGL(k, g, b, sh, s, e0, e1, 0, args...);
+    else
+        hipLaunchKernelGGL(k, g, b, sh, s, args...);
+}
+
 // after_fused: A^T u already sits in Pt as per-workgroup partials (q.G of
 // them) together with the squares in Sb; otherwise A^T u is computed here.
 static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs &B,
                         hipStream_t s, int init, hipEvent_t *ev = nullptr,
                         bool after_fused = false) {
-    auto rec = [&](int i) {
-        if (ev) SKL_HIP(hipEventRecord(ev[i], s));
-    };
+    hipEvent_t e2 = ev ? ev[2] : nullptr, e3 = ev ? ev[3] : nullptr;
+    hipEvent_t e6 = ev ? ev[6] : nullptr, e7 = ev ? ev[7] : nullptr;
     int nrb = q.nrb_a, nsb = q.nsb;
     if (after_fused) {
         nrb = q.G;
         nsb = q.G;
     } else {
         // Pt = A^T ut
-        rec(2);
         if (!q.sparse) {
             dim3 g((unsigned)q.nrb_a, (unsigned)((q.d + q.cpw_a - 1) / q.cpw_a));
-            k_gemv_t_part<0><<<g, GT_THREADS, 0, s>>>(q.n, q.d, p.A, p.lda, B.ut, q.cpw_a, B.Pt, 0,
-                                                      B.st);
+            launch_ev(k_gemv_t_part<0>, g, dim3(GT_THREADS), 0, s, e2, e3, q.n, q.d, p.A, p.lda,
+                      B.ut, q.cpw_a, B.Pt, 0, B.st);
         } else {
-            k_csc_t<<<(unsigned)((q.d + 3) / 4), 256, 0, s>>>(q.d, p.csc_ptr, p.csc_row,
-                                                              p.csc_val, B.ut, B.Pt, B.st);
+            launch_ev(k_csc_t, dim3((unsigned)((q.d + 3) / 4)), dim3(256), 0, s, e2, e3, q.d,
+                      p.csc_ptr, p.csc_row, p.csc_val, B.ut, B.Pt, B.st);
         }
-        rec(3);
     }
     // t = Pt(E) / beta
     k_reduce_t<<<(unsigned)((q.r + 255) / 256), 256, 0, s>>>(q.d, q.r, nrb, B.Pt, p.E, B.Sb, nsb,
@@ -567,22 +601,22 @@ static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs 
     // Pm = M^T t
     {
         dim3 g((unsigned)q.nrb_m, (unsigned)((q.r + q.cpw_m - 1) / q.cpw_m));
-        rec(6);
-        k_gemv_t_part<1><<<g, GT_THREADS, 0, s>>>(q.r, q.r, p.M, p.ldm, B.t, q.cpw_m, B.Pm, 1, B.st);
-        rec(7);
+        launch_ev(k_gemv_t_part<1>, g, dim3(GT_THREADS), 0, s, e6, e7, q.r, q.r, p.M, p.ldm, B.t,
+                  q.cpw_m, B.Pm, 1, B.st);
     }
     k_update<<<1, UPD_THREADS, 0, s>>>(q.r, q.nrb_m, B.Pm, B.Sb, nsb, B.v, B.w, B.x, B.st,
                                        B.hist, B.hist_cap, init);
 }
 
 static void launch_fused(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs &B,
-                         hipStream_t s) {
-    size_t sh = sizeof(double) * ((size_t)q.d + 16 * FU_R + FU_R);
-    auto fn = q.nq == 1 ? k_fused_dense<1> : q.nq == 2 ? k_fused_dense<2> : k_fused_dense<4>;
+                         hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    size_t sh = sizeof(double) * ((size_t)q.d + 2 * 16 * FU_R);
+    auto fn = q.nq == 4 ? k_fused_dense<4> : q.nq == 8 ? k_fused_dense<8> : k_fused_dense<16>;
     if (sh > 65536)
         SKL_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)sh));
-    fn<<<q.G, FU_T, sh, s>>>(q.n, q.d, p.A, p.lda, B.z, B.ut, B.Sb, B.Pt, B.st);
+    launch_ev(fn, dim3(q.G), dim3(FU_T), (uint32_t)sh, s, e0, e1, q.n, q.d, p.A, p.lda, B.z, B.ut,
+              B.Sb, B.Pt, B.st);
 }
 
 // Optional kernel timing: events around the four operator products of one
@@ -602,40 +636,32 @@ static void prof_events() {
     for (auto &e : g_ev) SKL_HIP(hipEventCreate(&e));
 }
 
-static inline void rec(hipEvent_t *ev, int i, hipStream_t s) {
-    if (ev) SKL_HIP(hipEventRecord(ev[i], s));
-}
+static inline hipEvent_t evi(hipEvent_t *ev, int i) { return ev ? ev[i] : nullptr; }
 
 static void launch_iteration(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs &B,
                              hipStream_t s, hipEvent_t *ev = nullptr) {
     // z = scatter_E(M v)
     {
         dim3 g((unsigned)((q.r + GN_ROWS - 1) / GN_ROWS), (unsigned)q.nck_m);
-        rec(ev, 4, s);
-        k_gemv_n_part<1><<<g, GN_THREADS, 0, s>>>(q.r, q.r, p.M, p.ldm, B.v, q.cw_m, B.Pz, 1, B.st);
-        rec(ev, 5, s);
+        launch_ev(k_gemv_n_part<1>, g, dim3(GN_THREADS), 0, s, evi(ev, 4), evi(ev, 5), q.r, q.r,
+                  p.M, p.ldm, B.v, q.cw_m, B.Pz, 1, B.st);
         k_z_reduce<<<(unsigned)((q.d + 255) / 256), 256, 0, s>>>(q.d, q.r, q.nck_m, B.Pz, B.Einv,
                                                                  B.z, B.st);
     }
     // ut = A z - coef ut ; Sb
     if (q.fused) {
-        rec(ev, 0, s);
-        launch_fused(p, q, B, s);
-        rec(ev, 1, s);
+        launch_fused(p, q, B, s, evi(ev, 0), evi(ev, 1));
         launch_op_t(p, q, B, s, 0, ev, true);
         return;
     }
     if (!q.sparse) {
         dim3 g((unsigned)((q.n + GN_ROWS - 1) / GN_ROWS), (unsigned)q.nck_a);
-        rec(ev, 0, s);
-        k_gemv_n_part<0><<<g, GN_THREADS, 0, s>>>(q.n, q.d, p.A, p.lda, B.z, q.cw_a, B.Pn, 0, B.st);
-        rec(ev, 1, s);
+        launch_ev(k_gemv_n_part<0>, g, dim3(GN_THREADS), 0, s, evi(ev, 0), evi(ev, 1), q.n, q.d,
+                  p.A, p.lda, B.z, q.cw_a, B.Pn, 0, B.st);
         k_reduce_n<<<(unsigned)q.nsb, 256, 0, s>>>(q.n, q.nck_a, B.Pn, B.ut, B.Sb, B.st, 0);
     } else {
-        rec(ev, 0, s);
-        k_spmv_n<8><<<(unsigned)q.nsb, 256, 0, s>>>(q.n, p.csr_ptr, p.csr_col, p.csr_val, B.z, B.ut,
-                                                    B.Sb, B.st, 0);
-        rec(ev, 1, s);
+        launch_ev(k_spmv_n<8>, dim3((unsigned)q.nsb), dim3(256), 0, s, evi(ev, 0), evi(ev, 1), q.n,
+                  p.csr_ptr, p.csr_col, p.csr_val, B.z, B.ut, B.Sb, B.st, 0);
     }
     launch_op_t(p, q, B, s, 0, ev);
 }

@@ -77,6 +77,7 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
     T *vn2 = vn1 + (a.npiv > 0 ? a.npiv : 1);     // npiv
     T *redbuf = vn2 + (a.npiv > 0 ? a.npiv : 1);  // RS_T
     int *flags = reinterpret_cast<int *>(redbuf + RS_T); // npiv + 2
+    int *jps = flags + a.npiv + 2;                       // npiv: jpvt of the columns (workgroup 0)
     const long r0 = (long)w * a.rpw;
     const long r1 = min(a.m, r0 + a.rpw);
     const int nr = (int)max(0L, r1 - r0);
@@ -86,6 +87,10 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
     const int lane = tid & 63, wave = tid >> 6;
     for (int q = wave; q < nc; q += RS_T / 64)
         for (int i = lane; i < nr; i += 64) S[i + q * ldl] = a.W[(r0 + i) + (long)q * a.ldw];
+    // column indices follow their columns in LDS; stored back at the end
+    const bool keep_jp = (w == 0 && a.jpvt != nullptr);
+    if (keep_jp)
+        for (int q = tid; q < npiv; q += RS_T) jps[q] = a.jpvt[q];
     __syncthreads();
 
     // all-reduce of vals[q] for q in [q0, nc) (q0 relative), result into Gs;
@@ -96,17 +101,18 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
         // values already in pw[q] for q >= q0
         grid_barrier(a.bar, bar_target);
         if (!a.two_phase) {
-            // four adjacent lanes per column, each summing every fourth workgroup
-            const int sub = tid & 3;
-            for (int qb = q0; qb < nc; qb += RS_T / 4) {
-                const int q = qb + (tid >> 2);
+            // eight adjacent lanes per column, each summing every eighth workgroup
+            const int sub = tid & 7;
+            for (int qb = q0; qb < nc; qb += RS_T / 8) {
+                const int q = qb + (tid >> 3);
                 T s = 0, rv = 0;
                 if (q < nc) {
                     if (with_row && sub == 0) rv = a.rowv[(size_t)buf * nc + q];
-                    for (int u = sub; u < P; u += 4) s += a.part[((size_t)buf * P + u) * nc + q];
+                    for (int u = sub; u < P; u += 8) s += a.part[((size_t)buf * P + u) * nc + q];
                 }
                 s += __shfl_xor(s, 1, 64);
                 s += __shfl_xor(s, 2, 64);
+                s += __shfl_xor(s, 4, 64);
                 if (q < nc && sub == 0) {
                     Gs[q] = s;
                     if (with_row) Rs[q] = rv;
@@ -186,9 +192,9 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
                     vn1[pvt] = vn1[jj];
                     vn2[pvt] = vn2[jj];
                     if (w == 0 && a.jpvt) {
-                        int32_t t1 = a.jpvt[pvt];
-                        a.jpvt[pvt] = a.jpvt[jj];
-                        a.jpvt[jj] = t1;
+                        int32_t t1 = jps[pvt];
+                        jps[pvt] = jps[jj];
+                        jps[jj] = t1;
                     }
                 }
             }
@@ -200,18 +206,19 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
         T *pw = a.part + ((size_t)buf * P + w) * nc;
         const int ib = (int)max(0L, (long)jj + 1 - r0); // first local row below the diagonal
         {
-            // four adjacent lanes per column, each summing every fourth row
-            const int sub = tid & 3;
+            // eight adjacent lanes per column, each summing every eighth row
+            const int sub = tid & 7;
             const T *sj = S + (size_t)jj * ldl;
-            for (int qb = jj; qb < nc; qb += RS_T / 4) {
-                const int q = qb + (tid >> 2);
+            for (int qb = jj; qb < nc; qb += RS_T / 8) {
+                const int q = qb + (tid >> 3);
                 T s = 0;
                 if (q < nc) {
                     const T *sq = S + (size_t)q * ldl;
-                    for (int i = ib + sub; i < nr; i += 4) s += sj[i] * sq[i];
+                    for (int i = ib + sub; i < nr; i += 8) s += sj[i] * sq[i];
                 }
                 s += __shfl_xor(s, 1, 64);
                 s += __shfl_xor(s, 2, 64);
+                s += __shfl_xor(s, 4, 64);
                 if (q < nc && sub == 0) pw[q] = s;
             }
         }
@@ -315,6 +322,8 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
     }
     for (int q = wave; q < nc; q += RS_T / 64)
         for (int i = lane; i < nr; i += 64) a.W[(r0 + i) + (long)q * a.ldw] = S[i + q * ldl];
+    if (keep_jp)
+        for (int q = tid; q < npiv; q += RS_T) a.jpvt[q] = jps[q];
 }
 
 // Workgroups of one grid-barrier launch: one per CU at most, and no more than
@@ -333,7 +342,7 @@ static void rowslice_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T 
     const int P_half = barrier_grid_cap();
     const int P_max = std::max(1, ctx().num_cu);
     size_t fixed = sizeof(T) * (2 * (size_t)nc + 2 * (size_t)std::max(npiv, 1) + RS_T) +
-                   sizeof(int) * ((size_t)npiv + 4) + 64;
+                   sizeof(int) * (2 * (size_t)npiv + 4) + 64;
     // fewest workgroups whose slices fit in LDS (each step costs one barrier,
     // whose latency grows with the workgroup count) ...
     long rpw = -1;

@@ -42,8 +42,8 @@ def test_alone_equals_mixed(skl):
         out = [S.wait(t) for t in tickets]
     for i, (x, info, q, r) in enumerate(out):
         xa, ita = alone[i % len(P)]
-        assert info.it == ita, i
-        assert np.array_equal(x, xa), i
+        diff = float(np.max(np.abs(x - xa)) / np.max(np.abs(xa)))
+        assert info.it == ita and np.array_equal(x, xa), (i, P[i % len(P)][0], info.it, ita, diff)
         assert q >= 0 and r > 0
 
 

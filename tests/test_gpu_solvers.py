@@ -129,6 +129,21 @@ def test_deterministic(skl):
     assert np.array_equal(x1, x2) and i1.it == i2.it
 
 
+def test_deterministic_sparse_and_after_other_shapes(skl):
+    """Repeated solves in one thread agree bitwise, also after solves of other
+    shapes have reused the device arena in between."""
+    A, b = sparse_problem(20, n=5000, d=120, density=0.02)
+    x1, i1 = skl.sparse_solve(A, b, gamma=1.5, k=3, it_tol=1e-10, seed=1)
+    A2, b2 = c1_problem(12, n=2500, d=25)
+    skl.dense_solve(A2, b2, gamma=1.7, k=1, it_tol=1e-10, seed=3)
+    A3, b3 = sparse_problem(21, n=3000, d=40, density=0.02)
+    skl.sparse_solve(A3, b3, gamma=1.5, k=3, it_tol=1e-10, seed=1)
+    x2, i2 = skl.sparse_solve(A, b, gamma=1.5, k=3, it_tol=1e-10, seed=1)
+    x3, i3 = skl.sparse_solve(A, b, gamma=1.5, k=3, it_tol=1e-10, seed=1)
+    assert i1.it == i2.it == i3.it, (i1.it, i2.it, i3.it)
+    assert np.array_equal(x1, x2) and np.array_equal(x2, x3)
+
+
 def test_mixed_precision_dense(skl):
     A, b = c1_problem(7)
     x64, i64 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=9)
