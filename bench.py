@@ -187,6 +187,110 @@ def run_dense(args, skl, torch, W):
     return At, b, res
 
 
+def run_sparse(args, skl, torch, W):
+    n, d = W["n"], W["d"]
+    log(f"generating {n}x{d} sparse problem on the host")
+    A, bh = make_sparse_host(W, seed=2024)
+    nnz = A.nnz
+    cp = torch.from_numpy(A.indptr.astype(np.int64)).cuda()
+    ri = torch.from_numpy(A.indices.astype(np.int64)).cuda()
+    va = torch.from_numpy(A.data.astype(np.float64)).cuda()
+    b = torch.from_numpy(bh).cuda()
+    x = torch.zeros(d, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    kw = dict(gamma=W["gamma"], k=W["k"], it_tol=1e-10, abs_tol=1e-8, seed=1)
+
+    def solve(precision):
+        return skl.sparse_solve_dev(n, d, nnz, cp.data_ptr(), ri.data_ptr(), va.data_ptr(),
+                                    b.data_ptr(), x.data_ptr(), precision=precision, **kw)
+
+    res = {}
+    for prec, name in [(skl.SKL_FP64, "fp64"), (skl.SKL_MIXED, "mixed")]:
+        for _ in range(args.warmup):
+            solve(prec)
+        if prec == skl.SKL_FP64:
+            skl.profile(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        infos, lat = [], []
+        for _ in range(args.steps):
+            ts = time.perf_counter()
+            infos.append(solve(prec))
+            lat.append(time.perf_counter() - ts)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        if prec == skl.SKL_FP64:
+            skl.profile(False, reset=False)
+            res["prof"] = skl.profile_read()
+        info = infos[-1]
+        res[name] = {"sec_per_solve": (t1 - t0) / args.steps, "lat": lat, "it": info.it,
+                     "flag": info.flag, "rank": info.rank, "escalated": info.escalated,
+                     "lsqr_est": info.lsqr_nrm_ar / (info.lsqr_nrm_a * info.lsqr_nrm_r),
+                     "phases": {k: getattr(info, k) for k in
+                                ["t_pattern", "t_sketch", "t_qr", "t_x0", "t_lsqr"]},
+                     "x": x.clone()}
+        log(f"{name}: {res[name]['sec_per_solve'] * 1e3:.2f} ms/solve, it={info.it}, "
+            f"flag={info.flag}, rank={info.rank}, phases={res[name]['phases']}")
+    # reference solution: dense Householder QR of A on the GPU (4.8 GB in FP64)
+    xq = None
+    try:
+        Ad = torch.sparse_csc_tensor(cp, ri, va, size=(n, d), dtype=torch.float64).to_dense()
+        xq = reference_solution(torch, Ad.t().contiguous(), b)
+        del Ad
+    except Exception as e:
+        log(f"dense reference skipped: {e}")
+    for name in ["fp64", "mixed"]:
+        R = res[name]
+        R["x_rel_vs_qr"] = (None if xq is None else
+                            ((R["x"] - xq).norm() / xq.norm()).item())
+    res["x_rel_mixed_vs_fp64"] = ((res["mixed"]["x"] - res["fp64"]["x"]).norm() /
+                                  res["fp64"]["x"].norm()).item()
+    res["host"] = (A, bh)
+    return res
+
+
+def cpu_baseline_sparse(A, bh, W, gpu_iters, rank):
+    """The oracle (plain-C restatement, 1 thread) on a bounded sample of the
+    sparse solve, scaled to one solve: the hashing sketch of a column subset
+    (scaled by column count), the Householder QR of an m x c_s block of the
+    sketch (scaled by the 2 m c^2 - 2/3 c^3 flop count) and two LSQR iterations
+    with the full A and a rank x rank triangular factor (times the GPU's
+    iteration count)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    n, d = W["n"], W["d"]
+    m = int(math.ceil(d * W["gamma"]))
+    t0 = time.perf_counter()
+    cs = max(2, min(d, 500))
+    rows, vals = O.sparse_pattern(1, n, m, W["k"])
+    As = A[:, :cs].tocsc()
+    ts = time.perf_counter()
+    O.sketch_sparse(m, rows, vals, W["k"], As.indptr.astype(np.int64),
+                    As.indices.astype(np.int64), As.data, n, cs)
+    t_sketch = (time.perf_counter() - ts) * d / cs
+    cq = max(2, min(d, 500))
+    rng = np.random.default_rng(0)
+    S = rng.standard_normal((m, cq))
+    tq = time.perf_counter()
+    O.geqrf(S)
+    flops = lambda c: 2.0 * m * c * c - 2.0 / 3.0 * c ** 3
+    t_qr = (time.perf_counter() - tq) * flops(d) / flops(cq)
+    R = np.triu(rng.standard_normal((rank, rank))) + 10 * np.eye(rank)
+    tl = time.perf_counter()
+    O.lsqr_sparse(A, R, bh, 0.0, 2)
+    t_it = (time.perf_counter() - tl) / 2.0
+    t_lsqr = t_it * (gpu_iters + 0.5)
+    spent = time.perf_counter() - t0
+    return {
+        "value": t_sketch + t_qr + t_lsqr, "unit": "sec/solve", "cores": 1, "kind": "port",
+        "sample": (f"oracle on the host: hashing sketch of {cs}/{d} columns (x{d / cs:.0f}), "
+                   f"Householder QR of {m}x{cq} (x{flops(d) / flops(cq):.1f} by flops), 2 LSQR "
+                   f"iterations with the full A and a {rank}x{rank} R (x{gpu_iters + 0.5:.1f}); "
+                   f"{spent:.1f}s of CPU work"),
+        "phases_s": {"sketch": t_sketch, "qr": t_qr, "lsqr_per_it": t_it},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,10 +313,11 @@ def main():
         sys.exit(3)
     skl = load_skl()
     W = WORKLOADS[args.workload]
-    if W["kind"] != "dense":
-        print("only dense workloads are wired into this entry point", file=sys.stderr)
-        sys.exit(2)
-    At, b, res = run_dense(args, skl, torch, W)
+    sparse = W["kind"] == "sparse"
+    if sparse:
+        res = run_sparse(args, skl, torch, W)
+    else:
+        At, b, res = run_dense(args, skl, torch, W)
     prof = res["prof"]
     dom = max(prof, key=lambda s: prof[s]["avg_ms"])
     P = prof[dom]
@@ -228,10 +333,25 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 1 thread, bounded sample)")
-        At_h = At.cpu().numpy()
-        b_h = b.cpu().numpy()
-        cpu = cpu_baseline_dense(skl, At_h, b_h, W, f64["it"], args.cpu_budget)
+        if sparse:
+            cpu = cpu_baseline_sparse(*res["host"], W, f64["it"], f64["rank"])
+        else:
+            At_h = At.cpu().numpy()
+            b_h = b.cpu().numpy()
+            cpu = cpu_baseline_dense(skl, At_h, b_h, W, f64["it"], args.cpu_budget)
         log(f"cpu baseline: {cpu['value']:.1f} s/solve ({cpu['sample']})")
+    if sparse:
+        config = {"workload": args.workload, "solver": "ls_sparse_spqr", "n": W["n"],
+                  "d": W["d"], "m": int(math.ceil(W["d"] * W["gamma"])), "k": W["k"],
+                  "nnz": int(res["host"][0].nnz), "it_tol": 1e-10,
+                  "generator": f"CSC density {W['density']}, N(0,1) values, columns x "
+                               f"logspace(0,-{W['decades']},{W['d']}), + unit diagonal"}
+    else:
+        config = {"workload": args.workload, "solver": "ls_dense_hashing_blendenpik",
+                  "n": W["n"], "d": W["d"], "m": int(math.ceil(W["d"] * W["gamma"])),
+                  "k": W["k"], "it_tol": 1e-10, "rcond": 1e-12,
+                  "generator": f"diag(w) G diag(logspace(0,-{W['decades']},{W['d']})), "
+                               f"w lognormal(0,{W['lognormal']})"}
     lat = sorted(f64["lat"])
     out = {
         "metric": "sec/solve to rel. normal-eq residual 1e-10, mixed vs FP64 on GPU; p99 latency",
@@ -246,11 +366,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": args.workload, "solver": "ls_dense_hashing_blendenpik",
-                   "n": W["n"], "d": W["d"], "m": int(math.ceil(W["d"] * W["gamma"])),
-                   "k": W["k"], "it_tol": 1e-10, "rcond": 1e-12,
-                   "generator": f"diag(w) G diag(logspace(0,-{W['decades']},{W['d']})), "
-                                f"w lognormal(0,{W['lognormal']})"},
+        "config": config,
         "lsqr_iterations": f64["it"],
         "lsqr_estimate": f64["lsqr_est"],
         "x_rel_vs_qr": f64["x_rel_vs_qr"],

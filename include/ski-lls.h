@@ -13,6 +13,10 @@
  * (src/dense_helpers.cpp:12,92) and leaves the sparse path on the
  * process-wide rand() stream (seed 1 in a fresh process).  These wrappers do
  * the same unless the environment variable SKL_SEED gives a seed.
+ *
+ * Precision: FP64 throughout, as in the reference, unless the environment
+ * variable SKL_PRECISION is "mixed" (FP32 sketch and QR, FP64 LSQR; see
+ * SKL_MIXED in skl.h).
  */
 #ifndef SKILLS_H
 #define SKILLS_H

@@ -35,8 +35,9 @@ extern "C" {
 /* precision modes */
 #define SKL_FP64 0  /* sketch, transform, QR and LSQR all in FP64 */
 #define SKL_MIXED 1 /* sketch, transform and QR in FP32 (FP64 QR fallback when
-                       the condition estimate of R exceeds escalate_cond);
-                       LSQR in FP64 */
+                       the FP32 factor loses rank or the inf-norm condition
+                       number of its column-equilibrated R exceeds
+                       escalate_cond); LSQR in FP64 */
 
 typedef struct skl_opts {
     double gamma;           /* over-sampling ratio, m = ceil(gamma d)          */
@@ -65,7 +66,8 @@ typedef struct skl_info {
     int success;        /* sparse: rcond(R11) > rcond_threshold         */
     int escalated;      /* MIXED: FP64 QR fallback taken                */
     double rcond;       /* reciprocal inf-norm condition number of R11  */
-    double cond_est;    /* condition estimate used for escalation       */
+    double cond_est;    /* MIXED: ||R D^-1||_inf ||D R^-1||_inf of the FP32
+                           factor, D = its column norms (escalation test) */
     double x0_residual; /* ||b - A x0|| of the sketch-and-solve start   */
     double lsqr_nrm_ar, lsqr_nrm_a, lsqr_nrm_r; /* LSQR final estimates */
     double t_pattern, t_sketch, t_qr, t_x0, t_lsqr, t_total; /* seconds */

@@ -307,14 +307,29 @@ double dev_nrm2(long n, const double *x, Arena &ar) {
     return h;
 }
 
-// inf-norm of an upper triangular matrix: max_i sum_{j>=i} |R_ij|
-__global__ __launch_bounds__(256) void k_rowabs_upper(long r, const double *R, long ldr,
-                                                      double *rs) {
-    long i = (long)blockIdx.x * 256 + threadIdx.x;
+// inf-norm of diag(rs) R diag(cs) for upper triangular R (rs, cs may be
+// null): partial row sums over 256-column chunks, then per-row totals and the
+// maximum, so every CU takes part.
+constexpr int RA_CW = 256;
+__global__ __launch_bounds__(256) void k_rowabs_part(long r, const double *R, long ldr,
+                                                     const double *cs, double *part) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    const long c0 = (long)blockIdx.y * RA_CW, c1 = min(r, c0 + RA_CW);
     if (i >= r) return;
     double s = 0.0;
-    for (long j = i; j < r; ++j) s += fabs(R[i + j * ldr]);
-    rs[i] = s;
+    for (long j = max(i, c0); j < c1; ++j) s += fabs(R[i + j * ldr]) * (cs ? cs[j] : 1.0);
+    part[(long)blockIdx.y * r + i] = s;
+}
+__global__ __launch_bounds__(256) void k_rowabs_max(long r, int nch, const double *part,
+                                                    const double *rs, double *bmax) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    double s = 0.0;
+    if (i < r) {
+        for (int c = 0; c < nch; ++c) s += part[(long)c * r + i];
+        if (rs) s *= rs[i];
+    }
+    s = block_max<256>(s);
+    if (threadIdx.x == 0) bmax[blockIdx.x] = s;
 }
 __global__ __launch_bounds__(256) void k_max_final(long n, const double *x, double *out) {
     double s = 0.0;
@@ -322,17 +337,47 @@ __global__ __launch_bounds__(256) void k_max_final(long n, const double *x, doub
     s = block_max<256>(s);
     if (threadIdx.x == 0) *out = s;
 }
-double norm_inf_upper(long r, const double *R, long ldr, Arena &ar) {
+double norm_inf_upper(long r, const double *R, long ldr, Arena &ar, const double *rs,
+                      const double *cs) {
     if (r <= 0) return 0.0;
-    double *rs = ar.alloc_n<double>((size_t)r + 1);
+    const int nch = (int)((r + RA_CW - 1) / RA_CW);
+    const unsigned nb = (unsigned)((r + 255) / 256);
+    const size_t mark = ar.used();
+    double *part = ar.alloc_n<double>((size_t)nch * r);
+    double *bm = ar.alloc_n<double>((size_t)nb + 1);
     hipStream_t s = ctx().stream;
-    k_rowabs_upper<<<(unsigned)((r + 255) / 256), 256, 0, s>>>(r, R, ldr, rs);
-    k_max_final<<<1, 256, 0, s>>>(r, rs, rs + r);
+    k_rowabs_part<<<dim3(nb, (unsigned)nch), 256, 0, s>>>(r, R, ldr, cs, part);
+    k_rowabs_max<<<nb, 256, 0, s>>>(r, nch, part, rs, bm);
+    k_max_final<<<1, 256, 0, s>>>(nb, bm, bm + nb);
     SKL_CHECK_LAUNCH();
     double h = 0;
-    SKL_HIP(hipMemcpyAsync(&h, rs + r, sizeof(double), hipMemcpyDeviceToHost, s));
+    SKL_HIP(hipMemcpyAsync(&h, bm + nb, sizeof(double), hipMemcpyDeviceToHost, s));
     SKL_HIP(hipStreamSynchronize(s));
+    ar.rewind(mark);
     return h;
+}
+
+// D[j] = ||R(0:j, j)||_2 (one wave per column); invD[j] = 1 / D[j] (0 for a zero column)
+__global__ __launch_bounds__(256) void k_colnorm_upper(long r, const double *R, long ldr, double *D,
+                                                       double *invD) {
+    const long j = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= r) return;
+    double s = 0.0;
+    for (long i = lane; i <= j; i += 64) {
+        double v = R[i + j * ldr];
+        s += v * v;
+    }
+    s = wave_sum(s);
+    if (lane == 0) {
+        D[j] = sqrt(s);
+        invD[j] = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
+    }
+}
+void colnorm_upper(long r, const double *R, long ldr, double *D, double *invD) {
+    if (r <= 0) return;
+    k_colnorm_upper<<<(unsigned)((r + 3) / 4), 256, 0, ctx().stream>>>(r, R, ldr, D, invD);
+    SKL_CHECK_LAUNCH();
 }
 
 // ---------------------------------------------------------------------------

@@ -297,13 +297,13 @@ __global__ __launch_bounds__(256) void k_reduce_n(long n, int nck, const double 
     if (threadIdx.x == 0) Sb[blockIdx.x] = sq;
 }
 
-// squares of b into Sb (init), ut = b
-__global__ __launch_bounds__(256) void k_init_u(long n, const double *__restrict__ b,
+// squares of b into Sb (init; one partial per block of rpb rows), ut = b
+__global__ __launch_bounds__(256) void k_init_u(long n, int rpb, const double *__restrict__ b,
                                                 double *__restrict__ ut, double *__restrict__ Sb) {
     double sq = 0.0;
-    for (int q = 0; q < RN_PER / 256; ++q) {
-        long i = (long)blockIdx.x * RN_PER + q * 256 + threadIdx.x;
-        if (i < n) {
+    const long i0 = (long)blockIdx.x * rpb, i1 = min(n, i0 + rpb);
+    for (long i = i0 + threadIdx.x; i < i1; i += 256) {
+        {
             double v = b[i];
             ut[i] = v;
             sq += v * v;
@@ -313,10 +313,12 @@ __global__ __launch_bounds__(256) void k_init_u(long n, const double *__restrict
     if (threadIdx.x == 0) Sb[blockIdx.x] = sq;
 }
 
-__device__ inline double beta_from(const double *Sb, int nsb) {
+// ||u|| from the nsb partial sums of squares, summed by the whole block (BS
+// threads) in a fixed order; the result is valid in every thread
+template <int BS> __device__ inline double beta_from(const double *Sb, int nsb) {
     double s = 0.0;
-    for (int q = 0; q < nsb; ++q) s += Sb[q];
-    return sqrt(s);
+    for (int q = threadIdx.x; q < nsb; q += BS) s += Sb[q];
+    return sqrt(block_sum<BS>(s));
 }
 
 // t[i] = (sum_rb Pt[rb * d + E[i]]) / beta
@@ -324,9 +326,7 @@ __global__ void k_reduce_t(long d, long r, int nrb, const double *__restrict__ P
                            const int32_t *__restrict__ E, const double *__restrict__ Sb, int nsb,
                            double *__restrict__ t, const LsqrState *__restrict__ st) {
     if (st->done) return;
-    __shared__ double sbeta;
-    if (threadIdx.x == 0) sbeta = beta_from(Sb, nsb);
-    __syncthreads();
+    const double sbeta = beta_from<256>(Sb, nsb);
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= r) return;
     long j = E ? (long)E[i] : i;
@@ -344,9 +344,7 @@ __global__ __launch_bounds__(UPD_THREADS) void k_update(
     LsqrState *__restrict__ st, double *__restrict__ hist, long hist_cap, int init) {
     if (st->done) return;
     __shared__ double sh[4];
-    if (threadIdx.x == 0) sh[0] = beta_from(Sb, nsb);
-    __syncthreads();
-    const double beta = sh[0];
+    const double beta = beta_from<UPD_THREADS>(Sb, nsb);
     double alpha_old = init ? 0.0 : st->alpha;
     double alpha = alpha_old;
     double nrm_a = init ? 0.0 : st->nrm_a;
@@ -444,7 +442,10 @@ __global__ __launch_bounds__(UPD_THREADS) void k_update(
 }
 
 // ---- sparse operator kernels ----------------------------------------------------
-// ut[i] = sum_p val[p] z[col[p]] - coef ut[i] over CSR rows; Sb per 1024 rows.
+// ut[i] = sum_p val[p] z[col[p]] - coef ut[i] over CSR rows, LPR lanes per
+// row; Sb per block of SP_ROWS rows (enough blocks to cover every CU several
+// times over at the sparse shapes).
+constexpr int SP_ROWS = 128;
 template <int LPR>
 __global__ __launch_bounds__(256) void k_spmv_n(long n, const int32_t *__restrict__ ptr,
                                                 const int32_t *__restrict__ col,
@@ -457,8 +458,8 @@ __global__ __launch_bounds__(256) void k_spmv_n(long n, const int32_t *__restric
     const int sub = threadIdx.x % LPR, grp = threadIdx.x / LPR;
     constexpr int RPI = 256 / LPR; // rows per pass
     double sq = 0.0;
-    for (int q = 0; q < RN_PER / RPI; ++q) {
-        long i = (long)blockIdx.x * RN_PER + q * RPI + grp;
+    for (int q = 0; q < SP_ROWS / RPI; ++q) {
+        long i = (long)blockIdx.x * SP_ROWS + q * RPI + grp;
         double s = 0.0;
         if (i < n) {
             int p0 = ptr[i], p1 = ptr[i + 1];
@@ -539,7 +540,7 @@ static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
     q.nrb_m = (int)((p.r + GT_RB - 1) / GT_RB);
     long cchm = std::max<long>(1, (target + q.nrb_m - 1) / q.nrb_m);
     q.cpw_m = (int)std::max<long>(4, (p.r + cchm - 1) / cchm);
-    q.nsb = (int)((p.n + RN_PER - 1) / RN_PER);
+    q.nsb = (int)((p.n + (q.sparse ? SP_ROWS : RN_PER) - 1) / (q.sparse ? SP_ROWS : RN_PER));
     if (q.sparse) q.nrb_a = 1;
     if (fused_wanted(p)) {
         q.fused = true;
@@ -758,7 +759,7 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
     SKL_CHECK_LAUNCH();
 
     // init: ut = b, beta = ||b||, v = M^T A^T u / alpha ...
-    k_init_u<<<(unsigned)q.nsb, 256, 0, s>>>(q.n, b, B.ut, B.Sb);
+    k_init_u<<<(unsigned)q.nsb, 256, 0, s>>>(q.n, q.sparse ? SP_ROWS : RN_PER, b, B.ut, B.Sb);
     launch_op_t(p, q, B, s, 1);
     SKL_CHECK_LAUNCH();
 
@@ -778,9 +779,14 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
     hipGraphExec_t exec = nullptr;
     if (graph_ok) {
         GraphKey key;
+        // every pointer and size baked into the captured kernels' arguments:
+        // a replay is only valid when all of them are unchanged (inputs such as
+        // the CSC values may live in a fresh allocation at each call)
         key.k = {(long)q.n, (long)q.d, (long)q.r, (long)(intptr_t)p.A, p.lda,
-                 (long)(intptr_t)p.csr_ptr, (long)(intptr_t)p.csc_ptr, (long)(intptr_t)p.E,
-                 (long)(intptr_t)p.M, p.ldm, (long)(intptr_t)B.ut, (long)(intptr_t)B.v,
+                 (long)(intptr_t)p.csr_ptr, (long)(intptr_t)p.csr_col,
+                 (long)(intptr_t)p.csr_val, (long)(intptr_t)p.csc_ptr,
+                 (long)(intptr_t)p.csc_row, (long)(intptr_t)p.csc_val, p.nnz,
+                 (long)(intptr_t)p.E, (long)(intptr_t)p.M, p.ldm, (long)(intptr_t)B.ut, (long)(intptr_t)B.v,
                  (long)(intptr_t)B.x, (long)(intptr_t)B.st, (long)(intptr_t)B.hist,
                  (long)(intptr_t)B.Pn, (long)(intptr_t)B.Pt, (long)(intptr_t)B.Pm,
                  (long)(intptr_t)B.Einv, (long)(intptr_t)B.z, (long)(intptr_t)B.w,
@@ -803,6 +809,9 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
             } else {
                 SKL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
                 SKL_HIP(hipGraphDestroy(graph));
+                // bounded cache: callers whose buffers move every call (host
+                // inputs staged afresh) would otherwise grow it without end
+                if (g_graphs.size() >= 32) lsqr_release_graphs();
                 g_graphs[key] = exec;
             }
         }

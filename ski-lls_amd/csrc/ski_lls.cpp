@@ -15,6 +15,12 @@ unsigned seed_or(unsigned fallback) {
     return fallback;
 }
 
+// SKL_PRECISION=mixed selects the FP32 sketch/QR with FP64 LSQR
+int precision_from_env() {
+    const char *e = std::getenv("SKL_PRECISION");
+    return (e && std::string(e) == "mixed") ? SKL_MIXED : SKL_FP64;
+}
+
 void raise(int rc) {
     if (rc == SKL_OK) return;
     if (rc == SKL_E_SHAPE) throw std::runtime_error("Matrix is not overdetermined.");
@@ -42,6 +48,7 @@ void ls_dense_hashing_blendenpik(Mat_d &A, Vec_d &b, Vec_d &x, long &rank, int &
     if (A.m() < A.n()) throw std::runtime_error("Matrix is not overdetermined.");
     skl_opts o;
     skl_default_opts(&o);
+    o.precision = precision_from_env();
     o.gamma = gamma;
     o.k = k;
     o.abs_tol = abs_tol;
@@ -68,6 +75,7 @@ void ls_dense_hashing_blendenpik_noCPQR(Mat_d &A, Vec_d &b, Vec_d &x, long &rank
     if (A.m() < A.n()) throw std::runtime_error("Matrix is not overdetermined.");
     skl_opts o;
     skl_default_opts(&o);
+    o.precision = precision_from_env();
     o.gamma = gamma;
     o.k = k;
     o.it_tol = it_tol;
@@ -92,6 +100,7 @@ void ls_sparse_spqr(cs_dl &A, Vec_d &b, Vec_d &x, long &rank, int &flag, long &i
     if (A.m < A.n) throw std::runtime_error("Matrix is not overdetermined.");
     skl_opts o;
     skl_default_opts(&o);
+    o.precision = precision_from_env();
     o.gamma = gamma;
     o.k = k;
     o.abs_tol = abs_tol;

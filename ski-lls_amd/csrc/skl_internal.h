@@ -171,7 +171,11 @@ void trinv_upper(long r, const double *R, long ldr, double perturb, double *M, l
                  Arena &ar);
 template <class T>
 void upcast_upper(long r, const T *R, long ldr, double *Rd, long ldd);
-double norm_inf_upper(long r, const double *R, long ldr, Arena &ar);
+// ||diag(rs) R diag(cs)||_inf of an upper triangular R (rs, cs may be null)
+double norm_inf_upper(long r, const double *R, long ldr, Arena &ar, const double *rs = nullptr,
+                      const double *cs = nullptr);
+// column 2-norms of an upper triangular R and their reciprocals
+void colnorm_upper(long r, const double *R, long ldr, double *D, double *invD);
 
 // GEMM: C = alpha op(A) op(B) + beta C (column-major)
 template <class T>

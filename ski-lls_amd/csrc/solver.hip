@@ -152,10 +152,12 @@ static void sync() { SKL_HIP(hipStreamSynchronize(ctx().stream)); }
 // ---------------------------------------------------------------------------
 // Dense solve (both variants), device pointers
 // ---------------------------------------------------------------------------
+// QR of [SA | c] (c present when db is): column-pivoted when pivot is set
+// (jpvt receives the order), plain Householder otherwise (jpvt = identity).
 template <class T>
 static void dense_sketch_qr(long n, long d, long m, const double *dA, long lda, const double *db,
-                            const DevPattern &dp, bool cpqr, T *W, int32_t *jpvt, T *tau,
-                            double *t_sketch, double *t_qr, Arena &ar) {
+                            const DevPattern &dp, bool pivot, T *W, int32_t *jpvt, T *tau,
+                            double *t_sketch, double *t_qr, Arena &ar, T *sa_copy = nullptr) {
     double t0 = wall();
     const T *coef = std::is_same<T, float>::value ? (const T *)dp.coef_f : (const T *)dp.coef_d;
     double scale = 1 / std::sqrt((double)n);
@@ -166,10 +168,11 @@ static void dense_sketch_qr(long n, long d, long m, const double *dA, long lda, 
         dht_sketch<T>(n, 1, db, n, dp.D, scale, m, dp.bptr, dp.bsrc, coef, W + d * m, m, ar);
         ncol = d + 1;
     }
+    if (sa_copy) copy2d<T, T>(m, d, W, m, sa_copy, m);
     sync();
     double t1 = wall();
-    if (cpqr) {
-        hipLaunchKernelGGL(k_iota, dim3(gblocks(d)), dim3(256), 0, ctx().stream, d, jpvt);
+    hipLaunchKernelGGL(k_iota, dim3(gblocks(d)), dim3(256), 0, ctx().stream, d, jpvt);
+    if (pivot) {
         if (std::min(m, d) < 1000)
             cpqr_greedy<T>(m, ncol, d, W, m, jpvt, tau, ar);
         else
@@ -189,6 +192,60 @@ static double rcond_from(long r, const double *Rd, long ldr, const double *M, lo
     double b = norm_inf_upper(r, M, ldm, ar);
     if (!(a > 0) || !(b > 0) || !std::isfinite(b)) return 0.0;
     return (1.0 / a) / b;
+}
+
+// inf-norm condition number of the column-equilibrated factor R D^{-1},
+// D = diag(||R(:, j)||_2): ||R D^{-1}||_inf ||D M||_inf.  Householder QR in
+// FP32 is columnwise backward stable (the computed R is the exact factor of
+// SA + E with ||E(:, j)|| <= c u_32 ||SA(:, j)||), so the quality of an FP32
+// preconditioner is governed by this number, not by the raw condition number
+// that column scaling alone can inflate.
+static double scaled_cond(long r, const double *Rd, long ldr, const double *M, long ldm,
+                          Arena &ar) {
+    if (r <= 0) return INFINITY;
+    const size_t mark = ar.used();
+    double *D = ar.alloc_n<double>((size_t)r);
+    double *invD = ar.alloc_n<double>((size_t)r);
+    colnorm_upper(r, Rd, ldr, D, invD);
+    double a = norm_inf_upper(r, Rd, ldr, ar, nullptr, invD);
+    double b = norm_inf_upper(r, M, ldm, ar, D, nullptr);
+    ar.rewind(mark);
+    if (!(a > 0) || !(b > 0) || !std::isfinite(a * b)) return INFINITY;
+    return a * b;
+}
+
+// U = I + diag(G - I) / 2 + striu(G): the upper Cholesky factor of G = I + E
+// to first order in E (U^T U = G + O(|E|^2)); zeros below the diagonal
+__global__ void k_chol_first_order(long r, const float *G, double *U) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long j = blockIdx.y;
+    if (i >= r) return;
+    double g = (double)G[i + j * r];
+    U[i + j * r] = (i < j) ? g : (i == j ? 1.0 + 0.5 * (g - 1.0) : 0.0);
+}
+
+// One CholeskyQR-style correction of the FP32 preconditioner of the mixed
+// mode.  With B = SA M and G = B^T B = I + E, right-multiplying M by the
+// inverse of G's upper Cholesky factor makes SA M orthonormal; the factor is
+// taken to first order in E, which FP32 GEMMs resolve well enough (the FP32
+// factor leaves |E| well below 1; the correction cuts it to O(|E|^2)).
+// Mr = M U^{-1}.
+static void refine_precond(long m, long r, const float *SA, long ldsa, const double *M,
+                           double *Mr, Arena &ar) {
+    const size_t mark = ar.used();
+    float *Mf = ar.alloc_n<float>((size_t)r * r);
+    copy2d<double, float>(r, r, M, r, Mf, r);
+    float *B = ar.alloc_n<float>((size_t)m * r);
+    gemm<float>('N', 'N', m, r, r, 1.0f, SA, ldsa, Mf, r, 0.0f, B, m);
+    float *G = ar.alloc_n<float>((size_t)r * r);
+    gemm<float>('T', 'N', r, r, m, 1.0f, B, m, B, m, 0.0f, G, r);
+    double *U = ar.alloc_n<double>((size_t)r * r);
+    hipLaunchKernelGGL(k_chol_first_order, dim3(gblocks(r), (unsigned)r), dim3(256), 0,
+                       ctx().stream, r, G, U);
+    double *Ui = ar.alloc_n<double>((size_t)r * r);
+    trinv_upper(r, U, r, 0.0, Ui, r, ar);
+    gemm<double>('N', 'N', r, r, r, 1.0, M, r, Ui, r, 0.0, Mr, r);
+    ar.rewind(mark);
 }
 
 static int dense_impl(long n, long d, const double *dA, long lda, const double *db, double *dx,
@@ -225,6 +282,7 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     bool escalated = false;
     double cond_est = 0.0;
     double *Mi = ar.alloc_n<double>((size_t)d * d);
+    float *sa32 = nullptr; // mixed: the FP32 sketch, kept for the correction
     auto finish_qr = [&](auto *W) {
         // diagonal -> rank (first |R_ii| <= rcond), R to double, c
         std::vector<double> dg((size_t)d);
@@ -254,15 +312,20 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
                                 &info.t_sketch, &info.t_qr, ar);
         finish_qr(W);
     } else {
+        // Optimistic FP32 preconditioner: an unpivoted Householder QR of the
+        // FP32 sketch.  Pivoting only matters when A is (nearly) rank
+        // deficient, and then the FP32 factor shows it -- a |R_ii| at or below
+        // rcond, or a large condition number of the column-equilibrated R --
+        // and the column-pivoted factorization is redone in FP64.  For a
+        // well-conditioned full-rank A the least-squares solution is unique,
+        // whatever the column order of the preconditioner.
         float *W = ar.alloc_n<float>((size_t)m * ncol);
         float *tau = ar.alloc_n<float>((size_t)d);
-        dense_sketch_qr<float>(n, d, m, dA, lda, cpqr ? db : nullptr, dp, cpqr, W, jpvt, tau,
-                               &info.t_sketch, &info.t_qr, ar);
+        sa32 = ar.alloc_n<float>((size_t)m * d);
+        dense_sketch_qr<float>(n, d, m, dA, lda, cpqr ? db : nullptr, dp, false, W, jpvt, tau,
+                               &info.t_sketch, &info.t_qr, ar, sa32);
         finish_qr(W);
-        double rc = rcond_from(rank, Rd, rank, Mi, rank, ar);
-        cond_est = rc > 0 ? 1.0 / rc : INFINITY;
-        // FP32 cannot tell a tiny column from a dependent one: a rank loss in
-        // FP32 or a large condition estimate sends the factorization to FP64
+        cond_est = scaled_cond(rank, Rd, rank, Mi, rank, ar);
         if (rank < std::min(m, d) || !(cond_est <= o.escalate_cond)) {
             escalated = true;
             double *Wd = ar.alloc_n<double>((size_t)m * ncol);
@@ -276,6 +339,16 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     info.escalated = escalated ? 1 : 0;
     info.cond_est = cond_est;
     info.rank = rank;
+    // LSQR operator M: the factor's inverse, corrected once when it came from
+    // the FP32 factorization (x0 keeps the FP32 sketch-and-solve start)
+    double *Mop = Mi;
+    if (mixed && !escalated && rank == d) {
+        double tr0 = wall();
+        Mop = ar.alloc_n<double>((size_t)d * d);
+        refine_precond(m, d, sa32, m, Mi, Mop, ar);
+        sync();
+        info.t_qr += wall() - tr0;
+    }
     sync();
     double T2 = wall();
     if (rank == 0) throw Error(SKL_E_RANK0, "rank 0: all |R_ii| <= rcond");
@@ -286,7 +359,7 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     p.A = dA;
     p.lda = lda;
     p.E = cpqr ? jpvt : nullptr;
-    p.M = Mi;
+    p.M = Mop;
     p.ldm = rank;
     double *y = ar.alloc_n<double>((size_t)rank);
     if (!cpqr) {
@@ -294,7 +367,7 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
         LsqrResult res = lsqr_solve(p, db, o.it_tol, o.max_it, y, o.use_graph != 0, o.graph_chunk,
                                     t_hist, t_hist_len, ar);
         double *my = ar.alloc_n<double>((size_t)rank);
-        trmv_n(rank, Mi, rank, y, my, ar);
+        trmv_n(rank, Mop, rank, y, my, ar);
         hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank,
                            (const int32_t *)nullptr, my, dx);
         sync();
@@ -332,7 +405,7 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     LsqrResult res = lsqr_solve(p, rs, o.it_tol, o.max_it, y, o.use_graph != 0, o.graph_chunk,
                                 t_hist, t_hist_len, ar);
     double *my = ar.alloc_n<double>((size_t)rank);
-    trmv_n(rank, Mi, rank, y, my, ar);
+    trmv_n(rank, Mop, rank, y, my, ar);
     hipLaunchKernelGGL(k_fill_d, dim3(gblocks(d)), dim3(256), 0, s, d, dx, 0.0);
     hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank, jpvt, my, dx);
     hipLaunchKernelGGL(k_axpy_d, dim3(gblocks(d)), dim3(256), 0, s, d, x0, dx);
@@ -351,10 +424,18 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
 // ---------------------------------------------------------------------------
 // Sparse solve, device CSC with int64 indices
 // ---------------------------------------------------------------------------
+// Sketch [SA | S b] and factor it.  refactor = true follows the reference:
+// natural-order Householder QR, and when any |R_jj| falls to SPQR's default
+// tolerance 20 (m + d) eps max_j ||SA_j|| a column-pivoted refactorisation
+// decides the rank.  refactor = false (the optimistic FP32 pass of the mixed
+// mode) stops after the Householder QR and returns the leading count of
+// |R_jj| above max(m, d) eps max_j ||SA_j||; the caller escalates when it
+// falls short of d.
 template <class T>
 static long sparse_factor(long n, long d, long m, long k, const DevPattern &dp, const int32_t *cp,
                           const int32_t *ri, const double *val, long nnz, const double *db, T *W,
-                          T *tau, int32_t *jpvt, double *tol_out, Arena &ar) {
+                          T *tau, int32_t *jpvt, double *tol_out, Arena &ar, bool refactor = true,
+                          T *sa_copy = nullptr) {
     hipStream_t s = ctx().stream;
     sparse_sketch_t<T>(m, d, k, dp.rows, dp.vals, cp, ri, val, nnz, W, m, ar);
     // b_s = S b in cs_gaxpy order, as the last column
@@ -370,8 +451,9 @@ static long sparse_factor(long n, long d, long m, long k, const DevPattern &dp, 
     double maxn = 0;
     for (double v : hc) maxn = std::max(maxn, std::sqrt(v));
     double eps = std::is_same<T, float>::value ? (double)FLT_EPSILON : DBL_EPSILON;
-    double tol = 20.0 * (double)(m + d) * eps * maxn;
+    double tol = (refactor ? 20.0 * (double)(m + d) : (double)std::max(m, d)) * eps * maxn;
     *tol_out = tol;
+    if (sa_copy) copy2d<T, T>(m, d, W, m, sa_copy, m);
     qr_householder<T>(m, d + 1, d, W, m, tau, ar);
     long kmax = std::min(m, d);
     std::vector<double> dg((size_t)kmax);
@@ -387,6 +469,11 @@ static long sparse_factor(long n, long d, long m, long k, const DevPattern &dp, 
         }
     hipLaunchKernelGGL(k_iota, dim3(gblocks(d)), dim3(256), 0, s, d, jpvt);
     if (full) return kmax;
+    if (!refactor) {
+        long lead = 0;
+        while (lead < kmax && std::fabs(dg[(size_t)lead]) > tol) ++lead;
+        return lead;
+    }
     // rank deficient: column-pivoted refactorisation of a fresh sketch
     sparse_sketch_t<T>(m, d, k, dp.rows, dp.vals, cp, ri, val, nnz, W, m, ar);
     copy2d<double, T>(m, 1, bs, m, W + d * m, m);
@@ -451,6 +538,7 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
     bool escalated = false;
     double cond_est = 0;
     double *Mi = ar.alloc_n<double>((size_t)d * d);
+    float *sa32 = nullptr; // mixed: the FP32 sketch, kept for the correction
     double tq0 = wall();
     if (!mixed) {
         double *W = ar.alloc_n<double>((size_t)m * (d + 1));
@@ -462,13 +550,15 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
     } else {
         float *W = ar.alloc_n<float>((size_t)m * (d + 1));
         float *tau = ar.alloc_n<float>((size_t)d + 1);
-        rank = sparse_factor<float>(n, d, m, o.k, dp, cp, ri, dval, nnz, db, W, tau, jpvt, &tol, ar);
-        if (rank == 0) throw Error(SKL_E_RANK0, "SPQR Factorization fails");
+        sa32 = ar.alloc_n<float>((size_t)m * d);
+        const long r32 = sparse_factor<float>(n, d, m, o.k, dp, cp, ri, dval, nnz, db, W, tau,
+                                              jpvt, &tol, ar, false, sa32);
+        const bool short_rank = r32 < std::min(m, d);
+        rank = std::max(r32, 1L);
         extract(W);
         trinv_upper(rank, Rd, rank, 0.0, Mi, rank, ar);
-        double rc = rcond_from(rank, Rd, rank, Mi, rank, ar);
-        cond_est = rc > 0 ? 1.0 / rc : INFINITY;
-        if (rank < std::min(m, d) || !(cond_est <= o.escalate_cond)) {
+        cond_est = scaled_cond(rank, Rd, rank, Mi, rank, ar);
+        if (short_rank || !(cond_est <= o.escalate_cond)) {
             escalated = true;
             double *Wd = ar.alloc_n<double>((size_t)m * (d + 1));
             double *taud = ar.alloc_n<double>((size_t)d + 1);
@@ -485,11 +575,17 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
     double rc = rcond_from(rank, Rd, rank, Mi, rank, ar);
     info.rcond = rc;
     info.success = rc > o.rcond_threshold ? 1 : 0;
-    double *Mp = Mi;
+    double *Mp = Mi;  // LSQR operator
+    double *Mb = Mi;  // back-transform
     if (!info.success) {
         // LSQR operator uses (R11 + perturb I)^{-1} (sparse_helpers.cpp:138-147)
         Mp = ar.alloc_n<double>((size_t)rank * rank);
         trinv_upper(rank, Rd, rank, o.perturb, Mp, rank, ar);
+    } else if (mixed && !escalated && rank == d) {
+        // corrected FP32 preconditioner; x = x0 + M y with the same M
+        Mp = ar.alloc_n<double>((size_t)rank * rank);
+        refine_precond(m, d, sa32, m, Mi, Mp, ar);
+        Mb = Mp;
     }
     sync();
     double T2 = wall();
@@ -535,7 +631,7 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
                                 t_hist, t_hist_len, ar);
     // y <- R11^{-1} y (no perturbation, ls_sparse_spqr.cpp:188); y += c; x = P y
     double *my = ar.alloc_n<double>((size_t)rank);
-    trmv_n(rank, Mi, rank, y, my, ar);
+    trmv_n(rank, Mb, rank, y, my, ar);
     hipLaunchKernelGGL(k_axpy_d, dim3(gblocks(rank)), dim3(256), 0, s, rank, c1, my);
     hipLaunchKernelGGL(k_fill_d, dim3(gblocks(d)), dim3(256), 0, s, d, dx, 0.0);
     hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank, jpvt, my, dx);

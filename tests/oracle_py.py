@@ -268,6 +268,25 @@ def lsqr_dense(A, R, b, tol, max_it, E=None, hist=False):
 
 
 # ---- solvers -----------------------------------------------------------------
+def lsqr_sparse(Acsc, R, b, tol, max_it, E=None, perturb=0.0):
+    """LSQR on A(:, E) (R + perturb I)^{-1} with A in CSC (scipy)."""
+    n, d = Acsc.shape
+    R = fortran(R)
+    r = R.shape[1]
+    y = np.zeros(r)
+    flag = C.c_int(0)
+    it = C.c_long(0)
+    info = Info()
+    Ep = _l(np.ascontiguousarray(E, dtype=np.int64)) if E is not None else None
+    Ap = np.ascontiguousarray(Acsc.indptr, dtype=np.int64)
+    Ai = np.ascontiguousarray(Acsc.indices, dtype=np.int64)
+    lib().orc_lsqr_sparse(C.c_long(n), C.c_long(d), _l(Ap), _l(Ai), _d(f64(Acsc.data)),
+                          C.c_long(r), Ep, _d(R), C.c_long(R.shape[0]), C.c_double(perturb),
+                          _d(f64(b)), C.c_double(tol), C.c_long(max_it), _d(y), C.byref(flag),
+                          C.byref(it), None, C.byref(info))
+    return y, flag.value, it.value, info
+
+
 def _hist(max_it, want):
     return (LsqrStep * max(max_it, 1))() if want else None
 
