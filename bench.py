@@ -61,14 +61,18 @@ def make_dense(torch, W, seed):
     return At.contiguous(), b.contiguous()
 
 
-def make_sparse_host(W, seed):
+def make_sparse_host(W, seed, decades=None, diag_first=False):
+    """CSC A = S diag(s) + I (diag_first: (S + I) diag(s)), s = logspace(0,-decades,d)."""
     import scipy.sparse as sp
     rng = np.random.default_rng(seed)
     n, d = W["n"], W["d"]
     A = sp.random(n, d, density=W["density"], random_state=rng, format="csc",
                   data_rvs=rng.standard_normal)
-    A = A @ sp.diags(np.logspace(0, -W["decades"], d))
-    A = sp.csc_matrix(A + sp.eye(n, d, format="csc"))
+    scale = sp.diags(np.logspace(0, -(W["decades"] if decades is None else decades), d))
+    if diag_first:
+        A = sp.csc_matrix((A + sp.eye(n, d, format="csc")) @ scale)
+    else:
+        A = sp.csc_matrix(A @ scale + sp.eye(n, d, format="csc"))
     A.sort_indices()
     xs = rng.standard_normal(d)
     b = A @ xs + 1e-8 * rng.standard_normal(n)
@@ -246,7 +250,39 @@ def run_sparse(args, skl, torch, W):
     res["x_rel_mixed_vs_fp64"] = ((res["mixed"]["x"] - res["fp64"]["x"]).norm() /
                                   res["fp64"]["x"].norm()).item()
     res["host"] = (A, bh)
+    if not args.no_escalation_case:
+        res["escalation"] = escalation_case(skl, torch, W, kw)
     return res
+
+
+def escalation_case(skl, torch, W, kw):
+    """Mixed-precision solve of the same generator with the unit diagonal added
+    before the column scaling logspace(0,-10,d): column norms spread over 1e10,
+    which the FP32 factor cannot rank, so the FP64 factorization must run."""
+    n, d = W["n"], W["d"]
+    log(f"generating the {n}x{d} escalation case (logspace(0,-10), diagonal scaled)")
+    A, bh = make_sparse_host(W, seed=2025, decades=10, diag_first=True)
+    cp = torch.from_numpy(A.indptr.astype(np.int64)).cuda()
+    ri = torch.from_numpy(A.indices.astype(np.int64)).cuda()
+    va = torch.from_numpy(A.data.astype(np.float64)).cuda()
+    b = torch.from_numpy(bh).cuda()
+    out = {}
+    xs = {}
+    for prec, name in [(skl.SKL_FP64, "fp64"), (skl.SKL_MIXED, "mixed")]:
+        x = torch.zeros(d, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = skl.sparse_solve_dev(n, d, A.nnz, cp.data_ptr(), ri.data_ptr(), va.data_ptr(),
+                                    b.data_ptr(), x.data_ptr(), precision=prec, **kw)
+        torch.cuda.synchronize()
+        xs[name] = x
+        out[name] = {"sec": time.perf_counter() - t0, "it": info.it, "flag": info.flag,
+                     "escalated": info.escalated,
+                     "lsqr_estimate": info.lsqr_nrm_ar / (info.lsqr_nrm_a * info.lsqr_nrm_r)}
+        log(f"escalation case {name}: {out[name]}")
+    out["x_rel_mixed_vs_fp64"] = ((xs["mixed"] - xs["fp64"]).norm() / xs["fp64"].norm()).item()
+    out["generator"] = f"(S + I) diag(logspace(0,-10,{d})), S as the workload"
+    return out
 
 
 def cpu_baseline_sparse(A, bh, W, gpu_iters, rank):
@@ -299,6 +335,8 @@ def main():
     ap.add_argument("--workload", default="dense_40000x4000", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--no-escalation-case", action="store_true",
+                    help="sparse workload: skip the logspace(0,-10) FP64-fallback check")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the measurement, copy 1 GiB device-to-device three times "
                          "(reference kernel for scaling HBM byte counters)")
@@ -388,6 +426,8 @@ def main():
                                        for s, v in prof.items() if s != dom}},
         "cpu_baseline": cpu,
     }
+    if "escalation" in res:
+        out["mixed"]["escalation_case"] = res["escalation"]
     print(json.dumps(out), flush=True)
     if args.calibrate_copy:
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)

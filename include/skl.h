@@ -34,10 +34,13 @@ extern "C" {
 
 /* precision modes */
 #define SKL_FP64 0  /* sketch, transform, QR and LSQR all in FP64 */
-#define SKL_MIXED 1 /* sketch, transform and QR in FP32 (FP64 QR fallback when
-                       the FP32 factor loses rank or the inf-norm condition
+#define SKL_MIXED 1 /* QR of the sketch in FP32 (FP64 QR fallback when the
+                       FP32 factor loses rank or the inf-norm condition
                        number of its column-equilibrated R exceeds
-                       escalate_cond); LSQR in FP64 */
+                       escalate_cond); LSQR in FP64.  The solvers with a
+                       sketch-and-solve start sketch in FP64 and refine the
+                       start against that sketch; the _nocpqr solver sketches
+                       in FP32. */
 
 typedef struct skl_opts {
     double gamma;           /* over-sampling ratio, m = ceil(gamma d)          */

@@ -107,7 +107,11 @@ __global__ __launch_bounds__(GT_THREADS) void k_gemv_t_part(
         uu[t].y = (ii + 1 < n) ? u[ii + 1] : 0.0;
     }
     const bool vec = ((lda & 1) == 0) && (i0 + GT_RB <= n);
-    const long jbeg = (long)blockIdx.y * cpw, jend = min(d, jbeg + cpw);
+    // the LSQR product walks the columns from the last: the A z pass before it
+    // ends there, so its first reads find that pass's last columns still in
+    // the memory-side cache (and the next A z pass starts where this one ends)
+    const long yb = ROLE == 0 ? (long)(gridDim.y - 1 - blockIdx.y) : (long)blockIdx.y;
+    const long jbeg = yb * cpw, jend = min(d, jbeg + cpw);
     for (long j = jbeg + wave; j < jend; j += 4) {
         double s = 0.0;
         if (!(upper && i0 > j)) {
@@ -881,6 +885,22 @@ void dev_gemv_n_dense(long n, long d, const double *A, long lda, const double *x
     dim3 g((unsigned)rowblocks, (unsigned)nc);
     k_gemv_n_part<2><<<g, GN_THREADS, 0, s>>>(n, d, A, lda, x, cw, P, 0, nullptr);
     k_axpby_part<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, nc, P, alpha, beta, y);
+    SKL_CHECK_LAUNCH();
+}
+
+// y = alpha A^T u + beta y (A n x d); the row-block partials are summed in a
+// fixed order
+void dev_gemv_t_dense(long n, long d, const double *A, long lda, const double *u, double alpha,
+                      double beta, double *y, Arena &ar) {
+    hipStream_t s = ctx().stream;
+    long nrb = (n + GT_RB - 1) / GT_RB;
+    long target = 8L * ctx().num_cu;
+    long cchunks = std::max<long>(1, (target + nrb - 1) / nrb);
+    int cpw = (int)std::max<long>(4, (d + cchunks - 1) / cchunks);
+    double *P = ar.alloc_n<double>((size_t)nrb * d);
+    dim3 g((unsigned)nrb, (unsigned)((d + cpw - 1) / cpw));
+    k_gemv_t_part<2><<<g, GT_THREADS, 0, s>>>(n, d, A, lda, u, cpw, P, 0, nullptr);
+    k_axpby_part<<<(unsigned)((d + 255) / 256), 256, 0, s>>>(d, (int)nrb, P, alpha, beta, y);
     SKL_CHECK_LAUNCH();
 }
 

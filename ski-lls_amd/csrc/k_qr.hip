@@ -15,6 +15,7 @@
 //    GEMM updates and the Y downdate.
 #include <algorithm>
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "k_common.h"
@@ -331,6 +332,319 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
 // launch from another stream.
 static int barrier_grid_cap() { return std::max(1, ctx().num_cu / 2); }
 
+// ---------------------------------------------------------------------------
+// Register-resident panel kernel (at most 64 columns)
+// ---------------------------------------------------------------------------
+// Lane q of every wave holds column q of the panel, wave w the rows w + 8 k
+// (k < K) of its workgroup's slice of 8 K rows, so the column sums of a
+// Householder step (||x||^2 below the diagonal and the products of the
+// reflector column with every column) are plain per-lane sums: no shuffles,
+// no LDS traffic for the matrix itself.  The reflector column's rows are
+// broadcast through a small per-wave LDS vector.  Pivoting (dlaqp2 /
+// HQRRP panel rules, as in k_qr_rowslice) moves no data: each lane keeps the
+// logical position of its column, and the partial norms live in the lane of
+// their column; the pivot is found by a wave-wide argmax that every wave of
+// every workgroup computes identically.  The matrix is read and written in
+// 64-row chunks staged through LDS so that the global accesses run along
+// columns.  One grid barrier per column, none with one workgroup.
+template <class T> struct RpArgs {
+    long m;       // rows
+    int nc;       // columns (<= 64)
+    int npiv;     // leading columns that are pivot candidates (0 = no pivoting)
+    int nsteps;   // reflectors to compute
+    T *W;
+    long ldw;
+    T *tau;
+    int32_t *jpvt;  // (may be null)
+    int32_t *swaps; // pivot column of each step (may be null)
+    int hq_norms;   // 1: HQRRP downdate formula (1+r)(1-r); 0: LAPACK 1 - r^2
+    T *part;        // [2][P][64] workgroup sums
+    T *rowv;        // [2][64] row jj of the panel (workgroup 0)
+    T *rcp;         // [P][64] norm recomputation sums
+    unsigned *bar;  // barrier counter, zero at launch
+};
+
+constexpr int RP_W = 8, RP_T = 64 * RP_W;
+
+template <class T, int K>
+__global__ __launch_bounds__(RP_T) void k_qr_regpanel(RpArgs<T> a) {
+    static_assert(K % 8 == 0, "rows come in 64-row chunks");
+    __shared__ T tile[64 * 65];
+    __shared__ T s_g[2][RP_W][64];
+    __shared__ T s_row[2][64];
+    __shared__ T s_r[RP_W][64];
+    __shared__ T s_gs[RP_W][64];
+    __shared__ T s_col[RP_W][K];
+    const int P = gridDim.x, blk = blockIdx.x, tid = threadIdx.x;
+    const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const long r0 = (long)blk * (8 * K);
+    const int nc = a.nc, npiv = a.npiv;
+    const int BIG = 0x7fffffff;
+    const T tol3z = sqrt(eps_half<T>());
+    unsigned bar_target = 0;
+
+    T x[K];
+#pragma unroll
+    for (int c = 0; c < K / 8; ++c) {
+        const long rb = r0 + 64 * c;
+        for (int e = tid; e < 64 * 64; e += RP_T) {
+            const int r = e & 63, q = e >> 6;
+            const long row = rb + r;
+            tile[r + q * 65] = (q < nc && row < a.m) ? a.W[row + (long)q * a.ldw] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) x[8 * c + kk] = tile[(w + 8 * kk) + lane * 65];
+        __syncthreads();
+    }
+    int pos = lane < nc ? lane : BIG; // logical position of this lane's column
+    int jp = 0;
+    T vn1 = 0, vn2 = 0;
+    if (npiv > 0) {
+        if (a.jpvt && lane < npiv) jp = a.jpvt[lane];
+        T sq = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) sq += x[k] * x[k];
+        s_r[w][lane] = sq;
+        __syncthreads();
+        T tot = 0;
+#pragma unroll
+        for (int u = 0; u < RP_W; ++u) tot += s_r[u][lane];
+        if (P > 1) {
+            if (w == 0) a.rcp[(size_t)blk * 64 + lane] = tot;
+            grid_barrier(a.bar, bar_target);
+            tot = 0;
+            for (int u = 0; u < P; ++u) tot += a.rcp[(size_t)u * 64 + lane];
+        }
+        vn1 = sqrt(tot);
+        vn2 = vn1;
+        __syncthreads();
+    }
+
+    for (int jj = 0; jj < a.nsteps; ++jj) {
+        const int buf = jj & 1;
+        SKL_PHASE(2, 0);
+        // ---- pivot: largest partial norm, first logical position on ties ----
+        int L = jj;
+        if (npiv > 0) {
+            const T av = fabs(vn1);
+            T bv = T(-1);
+            int bp = BIG, bl = -1;
+            if (pos >= jj && pos < npiv && av == av) {
+                bv = av;
+                bp = pos;
+                bl = lane;
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const T ov = __shfl_xor(bv, o, 64);
+                const int op = __shfl_xor(bp, o, 64), ol = __shfl_xor(bl, o, 64);
+                if (ov > bv || (ov == bv && op < bp)) {
+                    bv = ov;
+                    bp = op;
+                    bl = ol;
+                }
+            }
+            const unsigned long long atj = __ballot(pos == jj);
+            const int lj = __builtin_ctzll(atj); // positions 0..nc-1 are all held
+            if (bl < 0) bl = lj; // no finite candidate: the column stays
+            const int pvt = __shfl(pos, bl, 64);
+            if (lane == lj) pos = pvt;
+            if (lane == bl) pos = jj;
+            if (blk == 0 && tid == 0 && a.swaps) a.swaps[jj] = pvt;
+            L = __builtin_amdgcn_readfirstlane(bl); // identical in every lane
+        }
+        SKL_PHASE(2, 1);
+        // ---- column sums over this workgroup's rows below jj ----
+        // rows w + 8 k with k >= kfirst lie below row jj; row jj itself is
+        // register kj of wave jj % 8 of workgroup 0.  Only the first 64 rows
+        // (k < 8) can lie at or above jj (< 64).
+        const int jl = jj - (int)r0;
+        const int kfirst = jl < w ? 0 : (jl - w) / 8 + 1;
+        const int kj = (blk == 0 && w == (jj & 7)) ? (jj >> 3) : -1;
+        if (lane == L) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) s_col[w][k] = x[k];
+        }
+        T g = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) g += (k >= kfirst ? s_col[w][k] : T(0)) * x[k];
+#pragma unroll
+        for (int k = 8; k < K; ++k) g += s_col[w][k] * x[k];
+        s_g[buf][w][lane] = g;
+        if (kj >= 0) {
+            T rv = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) rv = k == kj ? x[k] : rv;
+            s_row[buf][lane] = rv;
+        }
+        __syncthreads();
+        SKL_PHASE(2, 2);
+        T G = 0, Rs;
+        if (P == 1) {
+#pragma unroll
+            for (int u = 0; u < RP_W; ++u) G += s_g[buf][u][lane];
+            Rs = s_row[buf][lane];
+        } else {
+            if (w == 0) {
+                T tot = 0;
+#pragma unroll
+                for (int u = 0; u < RP_W; ++u) tot += s_g[buf][u][lane];
+                a.part[((size_t)buf * P + blk) * 64 + lane] = tot;
+                if (blk == 0) a.rowv[(size_t)buf * 64 + lane] = s_row[buf][lane];
+            }
+            grid_barrier(a.bar, bar_target);
+            // wave w sums the workgroups u = w mod 8, then every wave adds
+            // the eight partial sums in the same order
+            T pt = 0;
+            for (int u = w; u < P; u += RP_W) pt += a.part[((size_t)buf * P + u) * 64 + lane];
+            Rs = a.rowv[(size_t)buf * 64 + lane];
+            s_gs[w][lane] = pt;
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < RP_W; ++u) G += s_gs[u][lane];
+        }
+        SKL_PHASE(2, 3);
+        // ---- Householder scalars (dlarfg), replicated ----
+        const T alpha = readlane(Rs, L);
+        const T xnorm = sqrt(readlane(G, L));
+        T tau = 0, beta = alpha, sc = 0;
+        if (xnorm != T(0)) {
+            beta = -copysign(dlapy2_d(alpha, xnorm), alpha);
+            tau = (beta - alpha) / beta;
+            sc = T(1) / (alpha - beta);
+        }
+        // w_q = v^T a_q = a(jj, q) + sc g_q ; C -= tau v w^T
+        const bool act = pos > jj && pos != BIG;
+        const T f = act ? -tau * (Rs + sc * G) : T(0);
+        const bool isL = lane == L;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const T v = sc * s_col[w][k];
+            const T upd = isL ? v : (act ? x[k] + v * f : x[k]);
+            const T rowj = isL ? beta : (act ? Rs + f : x[k]);
+            x[k] = k >= kfirst ? upd : (k == kj ? rowj : x[k]);
+        }
+#pragma unroll
+        for (int k = 8; k < K; ++k) {
+            const T v = sc * s_col[w][k];
+            x[k] = isL ? v : (act ? x[k] + v * f : x[k]);
+        }
+        if (blk == 0 && tid == 0) a.tau[jj] = tau;
+        SKL_PHASE(2, 4);
+        // ---- partial norm downdating (dlaqp2 / HQRRP) ----
+        if (npiv > 0) {
+            bool fl = false;
+            if (act && pos < npiv && vn1 != T(0)) {
+                const T rn = Rs + f;
+                T temp = fabs(rn) / vn1;
+                if (a.hq_norms)
+                    temp = (T(1) + temp) * (T(1) - temp);
+                else
+                    temp = T(1) - temp * temp;
+                if (temp < T(0)) temp = T(0);
+                const T q5 = vn1 / vn2;
+                const T temp2 = temp * q5 * q5;
+                if (temp2 <= tol3z)
+                    fl = true;
+                else
+                    vn1 = vn1 * sqrt(temp);
+            }
+            if (__ballot(fl)) { // same decision in every wave and workgroup
+                T sq = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) sq += k >= kfirst ? x[k] * x[k] : T(0);
+#pragma unroll
+                for (int k = 8; k < K; ++k) sq += x[k] * x[k];
+                s_r[w][lane] = sq;
+                __syncthreads();
+                T tot = 0;
+#pragma unroll
+                for (int u = 0; u < RP_W; ++u) tot += s_r[u][lane];
+                if (P > 1) {
+                    if (w == 0) a.rcp[(size_t)blk * 64 + lane] = tot;
+                    grid_barrier(a.bar, bar_target);
+                    tot = 0;
+                    for (int u = 0; u < P; ++u) tot += a.rcp[(size_t)u * 64 + lane];
+                }
+                if (fl) {
+                    vn1 = (jj < a.m - 1) ? sqrt(tot) : T(0);
+                    vn2 = vn1;
+                }
+            }
+        }
+    }
+    // ---- store, columns at their logical positions ----
+#pragma unroll
+    for (int c = 0; c < K / 8; ++c) {
+        const long rb = r0 + 64 * c;
+        __syncthreads();
+        if (pos < nc) {
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) tile[(w + 8 * kk) + pos * 65] = x[8 * c + kk];
+        }
+        __syncthreads();
+        for (int e = tid; e < 64 * nc; e += RP_T) {
+            const int r = e & 63, q = e >> 6;
+            const long row = rb + r;
+            if (row < a.m) a.W[row + (long)q * a.ldw] = tile[r + q * 65];
+        }
+    }
+    if (npiv > 0 && a.jpvt && blk == 0 && w == 0 && pos < npiv) a.jpvt[pos] = jp;
+}
+
+template <class T, int K>
+static void launch_regpanel(int P, const RpArgs<T> &a, hipStream_t s) {
+    k_qr_regpanel<T, K><<<P, RP_T, 0, s>>>(a);
+}
+
+// Panel factorization with the register-resident kernel; false when the panel
+// does not fit it (more than 64 columns, or more rows than one grid holds).
+// Slices of 8 K rows: one workgroup when the panel fits one (K as small as
+// that allows), otherwise the largest K that stays in registers (32 rows
+// per lane in FP64, 64 in FP32), which keeps the workgroup count -- and with
+// it the barrier latency -- low.  SKL_QR_PANEL=lds selects the LDS kernel instead.
+template <class T>
+static bool regpanel_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T *tau,
+                        int32_t *jpvt, int32_t *swaps, int hq_norms, Arena &ar) {
+    static const bool off = [] {
+        const char *e = getenv("SKL_QR_PANEL");
+        return e && std::string(e) == "lds";
+    }();
+    if (off || nc > 64 || nsteps > 64) return false;
+    constexpr int KMAX = sizeof(T) == 8 ? 32 : 64;
+    int K = 8;
+    while (K < KMAX && 8L * K < m) K *= 2;
+    const long P = (m + 8L * K - 1) / (8L * K);
+    if (P > barrier_grid_cap()) return false;
+    RpArgs<T> a;
+    a.m = m;
+    a.nc = nc;
+    a.npiv = npiv;
+    a.nsteps = nsteps;
+    a.W = W;
+    a.ldw = ldw;
+    a.tau = tau;
+    a.jpvt = jpvt;
+    a.swaps = swaps;
+    a.hq_norms = hq_norms;
+    a.part = ar.alloc_n<T>((size_t)2 * P * 64);
+    a.rowv = ar.alloc_n<T>((size_t)2 * 64);
+    a.rcp = ar.alloc_n<T>((size_t)P * 64);
+    a.bar = ar.alloc_n<unsigned>(1);
+    hipStream_t s = ctx().stream;
+    SKL_HIP(hipMemsetAsync(a.bar, 0, sizeof(unsigned), s));
+    BarrierLaunch guard(false);
+    switch (K) {
+    case 8: launch_regpanel<T, 8>((int)P, a, s); break;
+    case 16: launch_regpanel<T, 16>((int)P, a, s); break;
+    case 32: launch_regpanel<T, 32>((int)P, a, s); break;
+    default: launch_regpanel<T, KMAX>((int)P, a, s); break;
+    }
+    SKL_CHECK_LAUNCH();
+    return true;
+}
+
 static constexpr size_t RS_LDS_BUDGET = 150 * 1024;
 
 // Factor (or pivot-factor) the m x nc block at W: nsteps reflectors, pivoting
@@ -339,6 +653,7 @@ template <class T>
 static void rowslice_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T *tau,
                         int32_t *jpvt, int32_t *swaps, int hq_norms, Arena &ar) {
     if (nsteps <= 0) return;
+    if (regpanel_qr<T>(m, nc, npiv, nsteps, W, ldw, tau, jpvt, swaps, hq_norms, ar)) return;
     const int P_half = barrier_grid_cap();
     const int P_max = std::max(1, ctx().num_cu);
     size_t fixed = sizeof(T) * (2 * (size_t)nc + 2 * (size_t)std::max(npiv, 1) + RS_T) +
@@ -777,6 +1092,196 @@ __global__ __launch_bounds__(64 * W) void k_yqrp(YqArgs<T> a) {
     }
 }
 
+// Column-per-lane variant of k_yqrp for the HQRRP sketch height (74 rows):
+// lane l of wave w of workgroup b holds column (b W + w) 64 + l of Y entirely
+// in registers, with its partial norms and logical position.  The product of
+// a column with the reflector, its update and the norm downdating -- the exact
+// recomputation included -- are loops inside one lane; the only cross-lane
+// work of a step is the argmax.  One grid barrier per pivot, as in k_yqrp:
+// before it each workgroup publishes its best column as a reflector (dlarfg by
+// the owning wave), after it every wave picks the global winner (largest norm,
+// first position on ties) from the staged candidates and applies it.
+constexpr int YQ_ROWS = 74; // nb + pp of HQRRP (64 + 10)
+
+template <class T, int W>
+__global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
+    __shared__ T s_wv[W];
+    __shared__ int s_wp[W], s_wl[W];
+    __shared__ T s_v[YQ_MAXROWS]; // the workgroup candidate's column
+    extern __shared__ unsigned char yq_smem[];
+    const int P = gridDim.x, blk = blockIdx.x;
+    T *s_vec = reinterpret_cast<T *>(yq_smem);      // P x YQ_MAXROWS candidate reflectors
+    T *s_cv = s_vec + (size_t)P * YQ_MAXROWS;       // P candidate norms
+    T *s_tau = s_cv + P;                            // P candidate taus
+    int *s_cp = reinterpret_cast<int *>(s_tau + P); // P candidate positions
+    unsigned bar_target = 0;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int mY = a.mY, nv = a.nv;
+    const int NONE = 0x7fffffff;
+    const T tol3z = sqrt(eps_half<T>());
+    const int col = (blk * W + wave) * 64 + lane;
+    const bool valid = col < nv;
+    // y[r] is row jj + r of the column at step jj: the rows above the current
+    // pivot row are never read again, so the column shifts up by one row per
+    // step and every access has a fixed register index
+    T y[YQ_ROWS];
+    T dn, en;
+    {
+        const T *src = a.Y + (long)(valid ? col : 0) * a.ldy;
+        T sq = 0;
+#pragma unroll
+        for (int r = 0; r < YQ_ROWS; ++r) {
+            y[r] = (valid && r < mY) ? src[r] : T(0);
+            sq += y[r] * y[r];
+        }
+        dn = sqrt(sq);
+        en = dn;
+    }
+    int pos = valid ? col : NONE;
+    for (int jj = 0; jj < a.nsteps; ++jj) {
+        const int buf = jj & 1;
+        SKL_PHASE(1, 0);
+        // best column of this wave (every lane ends with the same answer) ...
+        T bv = T(-1);
+        int bp = NONE, bl = 0;
+        {
+            const T av = fabs(dn);
+            if (pos >= jj && pos != NONE && av == av) {
+                bv = av;
+                bp = pos;
+                bl = lane;
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const T ov = __shfl_xor(bv, o, 64);
+                const int op = __shfl_xor(bp, o, 64), ol = __shfl_xor(bl, o, 64);
+                if (yq_better(ov, op, bv, bp)) {
+                    bv = ov;
+                    bp = op;
+                    bl = ol;
+                }
+            }
+        }
+        if (lane == 0) {
+            s_wv[wave] = bv;
+            s_wp[wave] = bp;
+            s_wl[wave] = bl;
+        }
+        __syncthreads();
+        // ... and of this workgroup, published as its reflector over rows jj..
+        T gv = T(-1);
+        int gp = NONE, gw = 0;
+        for (int u = 0; u < W; ++u)
+            if (yq_better(s_wv[u], s_wp[u], gv, gp)) {
+                gv = s_wv[u];
+                gp = s_wp[u];
+                gw = u;
+            }
+        if (wave == gw) {
+            if (lane == s_wl[gw]) {
+#pragma unroll
+                for (int r = 0; r < YQ_ROWS; ++r) s_v[r] = y[r];
+            }
+            // reflector over rows jj.. in the same relative indexing
+            const int r0 = lane, r1 = lane + 64;
+            const T c0 = r0 < YQ_ROWS ? s_v[r0] : T(0);
+            const T c1 = r1 < YQ_ROWS ? s_v[r1] : T(0);
+            const T alpha = s_v[0];
+            const T xnorm = sqrt(wave_sum((r0 > 0 ? c0 * c0 : T(0)) + c1 * c1));
+            T tau = 0, sc = 0;
+            if (xnorm != T(0)) {
+                const T beta = -copysign(dlapy2_d(alpha, xnorm), alpha);
+                tau = (beta - alpha) / beta;
+                sc = T(1) / (alpha - beta);
+            }
+            T *dst = a.cvec + ((size_t)buf * P + blk) * YQ_MAXROWS;
+            dst[r0] = r0 > 0 ? sc * c0 : T(1);
+            dst[r1] = sc * c1;
+            if (lane == 0) {
+                a.cand[buf * P + blk] = gv;
+                a.candp[buf * P + blk] = gp;
+                a.ctau[buf * P + blk] = tau;
+            }
+        }
+        SKL_PHASE(1, 1);
+        grid_barrier(a.bar, bar_target);
+        SKL_PHASE(1, 2);
+        // every workgroup's candidate, staged in LDS in one round trip
+        for (int e = threadIdx.x; e < P * YQ_MAXROWS; e += 64 * W)
+            s_vec[e] = a.cvec[(size_t)buf * P * YQ_MAXROWS + e];
+        for (int t = threadIdx.x; t < P; t += 64 * W) {
+            s_cv[t] = a.cand[buf * P + t];
+            s_cp[t] = a.candp[buf * P + t];
+            s_tau[t] = a.ctau[buf * P + t];
+        }
+        __syncthreads();
+        SKL_PHASE(1, 3);
+        // global winner, computed identically by every wave
+        T v = T(-1);
+        int p = NONE, u = 0;
+        for (int t = lane; t < P; t += 64)
+            if (yq_better(s_cv[t], s_cp[t], v, p)) {
+                v = s_cv[t];
+                p = s_cp[t];
+                u = t;
+            }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const T ov = __shfl_xor(v, o, 64);
+            const int op = __shfl_xor(p, o, 64), ou = __shfl_xor(u, o, 64);
+            if (yq_better(ov, op, v, p)) {
+                v = ov;
+                p = op;
+                u = ou;
+            }
+        }
+        const int pl = p;
+        const T tau = s_tau[u];
+        const T *vv = s_vec + (size_t)u * YQ_MAXROWS;
+        SKL_PHASE(1, 4);
+        // logical swap of positions jj and pl, then y -= tau v (v^T y)
+        if (pos == pl)
+            pos = jj;
+        else if (pos == jj)
+            pos = pl;
+        const bool act = pos > jj && pos != NONE;
+        if (act && tau != T(0)) {
+            T g = 0;
+#pragma unroll
+            for (int r = 0; r < YQ_ROWS; ++r) g += vv[r] * y[r];
+            const T f = -tau * g;
+            // read v from LDS again rather than holding all of it in registers
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int r = 0; r < YQ_ROWS; ++r) y[r] += vv[r] * f;
+        }
+        // norm downdating (skint.c:920-990), recomputed in the lane when needed
+        if (act && dn != T(0)) {
+            T temp = fabs(y[0]) / dn;
+            temp = (T(1) + temp) * (T(1) - temp);
+            if (temp < T(0)) temp = T(0);
+            const T t5 = dn / en;
+            const T temp2 = temp * t5 * t5;
+            if (temp2 <= tol3z) {
+                T sq = 0;
+#pragma unroll
+                for (int r = 1; r < YQ_ROWS; ++r) sq += y[r] * y[r];
+                dn = (mY - jj - 1 > 0) ? sqrt(sq) : T(0);
+                en = dn;
+            } else {
+                dn = dn * sqrt(temp);
+            }
+        }
+        // drop row jj
+#pragma unroll
+        for (int r = 0; r + 1 < YQ_ROWS; ++r) y[r] = y[r + 1];
+        y[YQ_ROWS - 1] = T(0);
+        // s_v, s_wv and the staged candidates are rewritten only after the
+        // next step's barriers
+    }
+    if (pos != NONE) a.perm_out[pos] = col;
+}
+
 // changed[0..*nch) = ascending positions t with perm[t] != t (one workgroup)
 __global__ __launch_bounds__(1024) void k_perm_changed(int nv, const int32_t *perm,
                                                        int32_t *changed, int32_t *nch) {
@@ -890,6 +1395,16 @@ __global__ void k_g1_minus(int mY, int b, const T *G1, long ldg, const T *BT, co
     B[i + (long)r * mY] = G1[i + (long)r * ldg] - s;
 }
 
+// the column-per-lane pivot kernel unless SKL_YQRP=rows selects the
+// row-per-lane one
+static bool yqrp_lanes_on() {
+    static const bool on = [] {
+        const char *e = getenv("SKL_YQRP");
+        return !(e && std::string(e) == "rows");
+    }();
+    return on;
+}
+
 template <class T>
 void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *tau,
                 const double *G_host, Arena &ar) {
@@ -944,23 +1459,37 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
             ya.ctau = ctau;
             ya.cvec = cvec;
             ya.bar = bar;
-            // columns per workgroup: at most 32 workgroups (barrier latency
-            // grows with the workgroup count) unless the columns need more
-            // (128-column workgroups: 16 columns x 8 waves in FP64, whose
-            // registers allow it, 8 x 16 in FP32)
-            const bool dbl = sizeof(T) == 8;
-            int K = nv <= 512 ? 1 : nv <= 1024 ? 2 : nv <= 2048 ? 4 : dbl ? 16 : 8;
-            int Wv = (K == 16) ? 8 : 16;
-            int P = (int)((nv + (long)Wv * K - 1) / ((long)Wv * K));
-            SKL_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned), s));
-            constexpr int KW = sizeof(T) == 8 ? 16 : 8, WW = sizeof(T) == 8 ? 8 : 16;
-            auto fn = K == 1 ? k_yqrp<T, 1, 16> : K == 2 ? k_yqrp<T, 2, 16>
-                      : K == 4 ? k_yqrp<T, 4, 16> : k_yqrp<T, KW, WW>;
-            size_t sh = (size_t)P * ((YQ_MAXROWS + 2) * sizeof(T) + sizeof(int));
-            if (sh > 65536)
-                SKL_HIP(hipFuncSetAttribute((const void *)fn,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-            {
+            if (mY == YQ_ROWS && yqrp_lanes_on()) {
+                // 512 columns per workgroup, one per lane
+                constexpr int WL = 8;
+                const int P = (int)((nv + 64 * WL - 1) / (64 * WL));
+                size_t sh = (size_t)P * ((YQ_MAXROWS + 2) * sizeof(T) + sizeof(int));
+                if (sh > 65536)
+                    SKL_HIP(hipFuncSetAttribute((const void *)k_yqrp_lanes<T, WL>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                (int)sh));
+                SKL_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned), s));
+                BarrierLaunch guard;
+                k_yqrp_lanes<T, WL><<<P, 64 * WL, sh, s>>>(ya);
+                SKL_CHECK_LAUNCH();
+            } else {
+                // columns per workgroup: at most 32 workgroups (barrier latency
+                // grows with the workgroup count) unless the columns need more
+                // (128-column workgroups: 16 columns x 8 waves in FP64, whose
+                // registers allow it, 8 x 16 in FP32)
+                const bool dbl = sizeof(T) == 8;
+                int K = nv <= 512 ? 1 : nv <= 1024 ? 2 : nv <= 2048 ? 4 : dbl ? 16 : 8;
+                int Wv = (K == 16) ? 8 : 16;
+                int P = (int)((nv + (long)Wv * K - 1) / ((long)Wv * K));
+                SKL_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned), s));
+                constexpr int KW = sizeof(T) == 8 ? 16 : 8, WW = sizeof(T) == 8 ? 8 : 16;
+                auto fn = K == 1 ? k_yqrp<T, 1, 16> : K == 2 ? k_yqrp<T, 2, 16>
+                          : K == 4 ? k_yqrp<T, 4, 16> : k_yqrp<T, KW, WW>;
+                size_t sh = (size_t)P * ((YQ_MAXROWS + 2) * sizeof(T) + sizeof(int));
+                if (sh > 65536)
+                    SKL_HIP(hipFuncSetAttribute((const void *)fn,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                (int)sh));
                 BarrierLaunch guard;
                 fn<<<P, 64 * Wv, sh, s>>>(ya);
                 SKL_CHECK_LAUNCH();

@@ -213,6 +213,8 @@ void dev_gemv_n_dense(long n, long d, const double *A, long lda, const double *x
 void dev_csc_gemv_n(long n, long d, const int64_t *colptr, const int32_t *rowind,
                     const double *val, const double *x, double alpha, double beta,
                     double *y, Arena &ar);
+void dev_gemv_t_dense(long n, long d, const double *A, long lda, const double *u, double alpha,
+                      double beta, double *y, Arena &ar);
 double dev_nrm2(long n, const double *x, Arena &ar);
 
 } // namespace skl

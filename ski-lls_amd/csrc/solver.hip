@@ -152,25 +152,22 @@ static void sync() { SKL_HIP(hipStreamSynchronize(ctx().stream)); }
 // ---------------------------------------------------------------------------
 // Dense solve (both variants), device pointers
 // ---------------------------------------------------------------------------
-// QR of [SA | c] (c present when db is): column-pivoted when pivot is set
-// (jpvt receives the order), plain Householder otherwise (jpvt = identity).
+// [SA | c] = S H D [A, b] into W (c present when db is)
 template <class T>
-static void dense_sketch_qr(long n, long d, long m, const double *dA, long lda, const double *db,
-                            const DevPattern &dp, bool pivot, T *W, int32_t *jpvt, T *tau,
-                            double *t_sketch, double *t_qr, Arena &ar, T *sa_copy = nullptr) {
-    double t0 = wall();
+static void dense_sketch(long n, long d, long m, const double *dA, long lda, const double *db,
+                         const DevPattern &dp, T *W, Arena &ar) {
     const T *coef = std::is_same<T, float>::value ? (const T *)dp.coef_f : (const T *)dp.coef_d;
     double scale = 1 / std::sqrt((double)n);
-    // SA = S H D A (m x d) into W(:, 0:d)
     dht_sketch<T>(n, d, dA, lda, dp.D, scale, m, dp.bptr, dp.bsrc, coef, W, m, ar);
-    long ncol = d;
-    if (db) {
-        dht_sketch<T>(n, 1, db, n, dp.D, scale, m, dp.bptr, dp.bsrc, coef, W + d * m, m, ar);
-        ncol = d + 1;
-    }
-    if (sa_copy) copy2d<T, T>(m, d, W, m, sa_copy, m);
-    sync();
-    double t1 = wall();
+    if (db) dht_sketch<T>(n, 1, db, n, dp.D, scale, m, dp.bptr, dp.bsrc, coef, W + d * m, m, ar);
+}
+
+// QR of the first d columns of W (m x ncol), the rest receiving Q^T:
+// column-pivoted when pivot is set (jpvt receives the order), plain
+// Householder otherwise (jpvt = identity)
+template <class T>
+static void dense_factor(long m, long ncol, long d, bool pivot, T *W, int32_t *jpvt, T *tau,
+                         Arena &ar) {
     hipLaunchKernelGGL(k_iota, dim3(gblocks(d)), dim3(256), 0, ctx().stream, d, jpvt);
     if (pivot) {
         if (std::min(m, d) < 1000)
@@ -180,6 +177,17 @@ static void dense_sketch_qr(long n, long d, long m, const double *dA, long lda, 
     } else {
         qr_householder<T>(m, ncol, d, W, m, tau, ar);
     }
+}
+
+template <class T>
+static void dense_sketch_qr(long n, long d, long m, const double *dA, long lda, const double *db,
+                            const DevPattern &dp, bool pivot, T *W, int32_t *jpvt, T *tau,
+                            double *t_sketch, double *t_qr, Arena &ar) {
+    double t0 = wall();
+    dense_sketch<T>(n, d, m, dA, lda, db, dp, W, ar);
+    sync();
+    double t1 = wall();
+    dense_factor<T>(m, db ? d + 1 : d, d, pivot, W, jpvt, tau, ar);
     sync();
     double t2 = wall();
     *t_sketch += t1 - t0;
@@ -214,37 +222,27 @@ static double scaled_cond(long r, const double *Rd, long ldr, const double *M, l
     return a * b;
 }
 
-// U = I + diag(G - I) / 2 + striu(G): the upper Cholesky factor of G = I + E
-// to first order in E (U^T U = G + O(|E|^2)); zeros below the diagonal
-__global__ void k_chol_first_order(long r, const float *G, double *U) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long j = blockIdx.y;
-    if (i >= r) return;
-    double g = (double)G[i + j * r];
-    U[i + j * r] = (i < j) ? g : (i == j ? 1.0 + 0.5 * (g - 1.0) : 0.0);
-}
-
-// One CholeskyQR-style correction of the FP32 preconditioner of the mixed
-// mode.  With B = SA M and G = B^T B = I + E, right-multiplying M by the
-// inverse of G's upper Cholesky factor makes SA M orthonormal; the factor is
-// taken to first order in E, which FP32 GEMMs resolve well enough (the FP32
-// factor leaves |E| well below 1; the correction cuts it to O(|E|^2)).
-// Mr = M U^{-1}.
-static void refine_precond(long m, long r, const float *SA, long ldsa, const double *M,
-                           double *Mr, Arena &ar) {
+// Iterative refinement of the sketch-and-solve start: y <- y + M M^T SA^T
+// (c - SA y), `steps` times, with SA and c the FP64 sketch and M the inverse
+// of the FP32 factor.  The start's error contracts by about ||I - (SA M)^T
+// (SA M)|| = O(u_32 cond) per step, so a few steps give the FP64 sketched
+// solution; without them the FP32 start leaves A x0 - b far above the
+// least-squares residual and LSQR spends its first iterations closing that gap.
+static void refine_sketched_solution(long m, long d, const double *SA, long ldsa,
+                                     const double *c, const double *M, long ldm, double *y,
+                                     int steps, Arena &ar) {
+    hipStream_t s = ctx().stream;
     const size_t mark = ar.used();
-    float *Mf = ar.alloc_n<float>((size_t)r * r);
-    copy2d<double, float>(r, r, M, r, Mf, r);
-    float *B = ar.alloc_n<float>((size_t)m * r);
-    gemm<float>('N', 'N', m, r, r, 1.0f, SA, ldsa, Mf, r, 0.0f, B, m);
-    float *G = ar.alloc_n<float>((size_t)r * r);
-    gemm<float>('T', 'N', r, r, m, 1.0f, B, m, B, m, 0.0f, G, r);
-    double *U = ar.alloc_n<double>((size_t)r * r);
-    hipLaunchKernelGGL(k_chol_first_order, dim3(gblocks(r), (unsigned)r), dim3(256), 0,
-                       ctx().stream, r, G, U);
-    double *Ui = ar.alloc_n<double>((size_t)r * r);
-    trinv_upper(r, U, r, 0.0, Ui, r, ar);
-    gemm<double>('N', 'N', r, r, r, 1.0, M, r, Ui, r, 0.0, Mr, r);
+    double *rsk = ar.alloc_n<double>((size_t)m);
+    double *g = ar.alloc_n<double>((size_t)d);
+    double *h = ar.alloc_n<double>((size_t)d);
+    for (int it = 0; it < steps; ++it) {
+        SKL_HIP(hipMemcpyAsync(rsk, c, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+        dev_gemv_n_dense(m, d, SA, ldsa, y, -1.0, 1.0, rsk, ar);
+        dev_gemv_t_dense(m, d, SA, ldsa, rsk, 1.0, 0.0, g, ar);
+        dev_gemv_t_dense(d, d, M, ldm, g, 1.0, 0.0, h, ar);
+        dev_gemv_n_dense(d, d, M, ldm, h, 1.0, 1.0, y, ar);
+    }
     ar.rewind(mark);
 }
 
@@ -282,7 +280,7 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     bool escalated = false;
     double cond_est = 0.0;
     double *Mi = ar.alloc_n<double>((size_t)d * d);
-    float *sa32 = nullptr; // mixed: the FP32 sketch, kept for the correction
+    double *W64 = nullptr; // mixed robust path: the FP64 sketch, kept for the x0 refinement
     auto finish_qr = [&](auto *W) {
         // diagonal -> rank (first |R_ii| <= rcond), R to double, c
         std::vector<double> dg((size_t)d);
@@ -312,43 +310,56 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
                                 &info.t_sketch, &info.t_qr, ar);
         finish_qr(W);
     } else {
-        // Optimistic FP32 preconditioner: an unpivoted Householder QR of the
-        // FP32 sketch.  Pivoting only matters when A is (nearly) rank
-        // deficient, and then the FP32 factor shows it -- a |R_ii| at or below
-        // rcond, or a large condition number of the column-equilibrated R --
-        // and the column-pivoted factorization is redone in FP64.  For a
+        // Optimistic FP32 preconditioner: an unpivoted Householder QR in FP32.
+        // Pivoting only matters when A is (nearly) rank deficient, and then the
+        // FP32 factor shows it -- a |R_ii| at or below rcond, or a large
+        // condition number of the column-equilibrated R -- and the
+        // column-pivoted factorization is redone in FP64.  For a
         // well-conditioned full-rank A the least-squares solution is unique,
-        // whatever the column order of the preconditioner.
+        // whatever the column order of the preconditioner.  The robust path
+        // sketches in FP64 and factors a rounded copy: the FP64 sketch refines
+        // the sketch-and-solve start and serves the fallback without a second
+        // transform.  The fast path (no start) sketches in FP32.
         float *W = ar.alloc_n<float>((size_t)m * ncol);
         float *tau = ar.alloc_n<float>((size_t)d);
-        sa32 = ar.alloc_n<float>((size_t)m * d);
-        dense_sketch_qr<float>(n, d, m, dA, lda, cpqr ? db : nullptr, dp, false, W, jpvt, tau,
-                               &info.t_sketch, &info.t_qr, ar, sa32);
+        if (cpqr) {
+            W64 = ar.alloc_n<double>((size_t)m * ncol);
+            double t0 = wall();
+            dense_sketch<double>(n, d, m, dA, lda, db, dp, W64, ar);
+            sync();
+            double t1 = wall();
+            copy2d<double, float>(m, ncol, W64, m, W, m);
+            dense_factor<float>(m, ncol, d, false, W, jpvt, tau, ar);
+            sync();
+            info.t_sketch += t1 - t0;
+            info.t_qr += wall() - t1;
+        } else {
+            dense_sketch_qr<float>(n, d, m, dA, lda, nullptr, dp, false, W, jpvt, tau,
+                                   &info.t_sketch, &info.t_qr, ar);
+        }
         finish_qr(W);
         cond_est = scaled_cond(rank, Rd, rank, Mi, rank, ar);
         if (rank < std::min(m, d) || !(cond_est <= o.escalate_cond)) {
             escalated = true;
-            double *Wd = ar.alloc_n<double>((size_t)m * ncol);
             double *taud = ar.alloc_n<double>((size_t)d);
-            DevPattern dp2 = dp;
-            dense_sketch_qr<double>(n, d, m, dA, lda, cpqr ? db : nullptr, dp2, cpqr, Wd, jpvt,
-                                    taud, &info.t_sketch, &info.t_qr, ar);
-            finish_qr(Wd);
+            if (W64) {
+                double t1 = wall();
+                dense_factor<double>(m, ncol, d, cpqr, W64, jpvt, taud, ar);
+                sync();
+                info.t_qr += wall() - t1;
+                finish_qr(W64);
+                W64 = nullptr; // factored in place
+            } else {
+                double *Wd = ar.alloc_n<double>((size_t)m * ncol);
+                dense_sketch_qr<double>(n, d, m, dA, lda, nullptr, dp, cpqr, Wd, jpvt, taud,
+                                        &info.t_sketch, &info.t_qr, ar);
+                finish_qr(Wd);
+            }
         }
     }
     info.escalated = escalated ? 1 : 0;
     info.cond_est = cond_est;
     info.rank = rank;
-    // LSQR operator M: the factor's inverse, corrected once when it came from
-    // the FP32 factorization (x0 keeps the FP32 sketch-and-solve start)
-    double *Mop = Mi;
-    if (mixed && !escalated && rank == d) {
-        double tr0 = wall();
-        Mop = ar.alloc_n<double>((size_t)d * d);
-        refine_precond(m, d, sa32, m, Mi, Mop, ar);
-        sync();
-        info.t_qr += wall() - tr0;
-    }
     sync();
     double T2 = wall();
     if (rank == 0) throw Error(SKL_E_RANK0, "rank 0: all |R_ii| <= rcond");
@@ -359,7 +370,7 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     p.A = dA;
     p.lda = lda;
     p.E = cpqr ? jpvt : nullptr;
-    p.M = Mop;
+    p.M = Mi;
     p.ldm = rank;
     double *y = ar.alloc_n<double>((size_t)rank);
     if (!cpqr) {
@@ -367,7 +378,7 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
         LsqrResult res = lsqr_solve(p, db, o.it_tol, o.max_it, y, o.use_graph != 0, o.graph_chunk,
                                     t_hist, t_hist_len, ar);
         double *my = ar.alloc_n<double>((size_t)rank);
-        trmv_n(rank, Mop, rank, y, my, ar);
+        trmv_n(rank, Mi, rank, y, my, ar);
         hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank,
                            (const int32_t *)nullptr, my, dx);
         sync();
@@ -384,6 +395,8 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     // x0 = P M c ; rs = b - A x0
     double *c1 = ar.alloc_n<double>((size_t)rank);
     trmv_n(rank, Mi, rank, cvec, c1, ar);
+    // FP32 factor (unpivoted, full rank): refine the start against the FP64 sketch
+    if (W64) refine_sketched_solution(m, d, W64, m, W64 + d * m, Mi, rank, c1, 3, ar);
     double *x0 = ar.alloc_n<double>((size_t)d);
     hipLaunchKernelGGL(k_fill_d, dim3(gblocks(d)), dim3(256), 0, s, d, x0, 0.0);
     hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank, jpvt, c1, x0);
@@ -405,7 +418,7 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     LsqrResult res = lsqr_solve(p, rs, o.it_tol, o.max_it, y, o.use_graph != 0, o.graph_chunk,
                                 t_hist, t_hist_len, ar);
     double *my = ar.alloc_n<double>((size_t)rank);
-    trmv_n(rank, Mop, rank, y, my, ar);
+    trmv_n(rank, Mi, rank, y, my, ar);
     hipLaunchKernelGGL(k_fill_d, dim3(gblocks(d)), dim3(256), 0, s, d, dx, 0.0);
     hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank, jpvt, my, dx);
     hipLaunchKernelGGL(k_axpy_d, dim3(gblocks(d)), dim3(256), 0, s, d, x0, dx);
@@ -424,24 +437,42 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
 // ---------------------------------------------------------------------------
 // Sparse solve, device CSC with int64 indices
 // ---------------------------------------------------------------------------
-// Sketch [SA | S b] and factor it.  refactor = true follows the reference:
-// natural-order Householder QR, and when any |R_jj| falls to SPQR's default
-// tolerance 20 (m + d) eps max_j ||SA_j|| a column-pivoted refactorisation
-// decides the rank.  refactor = false (the optimistic FP32 pass of the mixed
-// mode) stops after the Householder QR and returns the leading count of
-// |R_jj| above max(m, d) eps max_j ||SA_j||; the caller escalates when it
-// falls short of d.
+// [SA | S b] into W (m x (d + 1)): the CXSparse product orders of
+// sketching.cpp:104-111
+template <class T>
+static void sparse_sketch_full(long n, long d, long m, long k, const DevPattern &dp,
+                               const int32_t *cp, const int32_t *ri, const double *val, long nnz,
+                               const double *db, T *W, Arena &ar) {
+    sparse_sketch_t<T>(m, d, k, dp.rows, dp.vals, cp, ri, val, nnz, W, m, ar);
+    // b_s = S b in cs_gaxpy order, as the last column
+    const size_t mark = ar.used();
+    double *bs = ar.alloc_n<double>((size_t)m);
+    hash_gather<double>(m, 1, dp.bptr, dp.bsrc, dp.coef_d, db, n, bs, m);
+    copy2d<double, T>(m, 1, bs, m, W + d * m, m);
+    sync();
+    ar.rewind(mark);
+}
+
+// Sketch [SA | S b] (or copy it from src, an FP64 sketch made before) and
+// factor it.  refactor = true follows the reference: natural-order
+// Householder QR, and when any |R_jj| falls to SPQR's default tolerance
+// 20 (m + d) eps max_j ||SA_j|| a column-pivoted refactorisation decides the
+// rank.  refactor = false (the optimistic FP32 pass of the mixed mode) stops
+// after the Householder QR and returns the leading count of |R_jj| above
+// max(m, d) eps max_j ||SA_j||; the caller escalates when it falls short of d.
 template <class T>
 static long sparse_factor(long n, long d, long m, long k, const DevPattern &dp, const int32_t *cp,
                           const int32_t *ri, const double *val, long nnz, const double *db, T *W,
                           T *tau, int32_t *jpvt, double *tol_out, Arena &ar, bool refactor = true,
-                          T *sa_copy = nullptr) {
+                          const double *src = nullptr) {
     hipStream_t s = ctx().stream;
-    sparse_sketch_t<T>(m, d, k, dp.rows, dp.vals, cp, ri, val, nnz, W, m, ar);
-    // b_s = S b in cs_gaxpy order, as the last column
-    double *bs = ar.alloc_n<double>((size_t)m);
-    hash_gather<double>(m, 1, dp.bptr, dp.bsrc, dp.coef_d, db, n, bs, m);
-    copy2d<double, T>(m, 1, bs, m, W + d * m, m);
+    auto sketch = [&] {
+        if (src)
+            copy2d<double, T>(m, d + 1, src, m, W, m);
+        else
+            sparse_sketch_full<T>(n, d, m, k, dp, cp, ri, val, nnz, db, W, ar);
+    };
+    sketch();
     // SPQR's default tolerance 20 (m + d) eps max_j ||SA_j||
     double *cn = ar.alloc_n<double>((size_t)d);
     hipLaunchKernelGGL(k_colnorm2<T>, dim3((unsigned)d), dim3(256), 0, s, m, d, W, m, cn);
@@ -453,7 +484,6 @@ static long sparse_factor(long n, long d, long m, long k, const DevPattern &dp, 
     double eps = std::is_same<T, float>::value ? (double)FLT_EPSILON : DBL_EPSILON;
     double tol = (refactor ? 20.0 * (double)(m + d) : (double)std::max(m, d)) * eps * maxn;
     *tol_out = tol;
-    if (sa_copy) copy2d<T, T>(m, d, W, m, sa_copy, m);
     qr_householder<T>(m, d + 1, d, W, m, tau, ar);
     long kmax = std::min(m, d);
     std::vector<double> dg((size_t)kmax);
@@ -474,10 +504,10 @@ static long sparse_factor(long n, long d, long m, long k, const DevPattern &dp, 
         while (lead < kmax && std::fabs(dg[(size_t)lead]) > tol) ++lead;
         return lead;
     }
-    // rank deficient: column-pivoted refactorisation of a fresh sketch
-    sparse_sketch_t<T>(m, d, k, dp.rows, dp.vals, cp, ri, val, nnz, W, m, ar);
-    copy2d<double, T>(m, 1, bs, m, W + d * m, m);
-    cpqr_greedy<T>(m, d + 1, d, W, m, jpvt, tau, ar);
+    // rank deficient: column-pivoted refactorisation of a fresh sketch (the
+    // CPQR of the dense path: dgeqp3 pivoting below 1000 columns, HQRRP above)
+    sketch();
+    dense_factor<T>(m, d + 1, d, true, W, jpvt, tau, ar);
     hipLaunchKernelGGL(k_diag<T>, dim3(gblocks(kmax)), dim3(256), 0, s, kmax, W, m, ddg);
     SKL_HIP(hipMemcpyAsync(dg.data(), ddg, sizeof(double) * kmax, hipMemcpyDeviceToHost, s));
     sync();
@@ -538,7 +568,7 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
     bool escalated = false;
     double cond_est = 0;
     double *Mi = ar.alloc_n<double>((size_t)d * d);
-    float *sa32 = nullptr; // mixed: the FP32 sketch, kept for the correction
+    double *W64 = nullptr; // mixed: the FP64 sketch, kept for the x0 refinement
     double tq0 = wall();
     if (!mixed) {
         double *W = ar.alloc_n<double>((size_t)m * (d + 1));
@@ -548,11 +578,13 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
         extract(W);
         trinv_upper(rank, Rd, rank, 0.0, Mi, rank, ar);
     } else {
+        // FP64 sketch, FP32 factor of its rounded copy (see dense_impl)
+        W64 = ar.alloc_n<double>((size_t)m * (d + 1));
+        sparse_sketch_full<double>(n, d, m, o.k, dp, cp, ri, dval, nnz, db, W64, ar);
         float *W = ar.alloc_n<float>((size_t)m * (d + 1));
         float *tau = ar.alloc_n<float>((size_t)d + 1);
-        sa32 = ar.alloc_n<float>((size_t)m * d);
         const long r32 = sparse_factor<float>(n, d, m, o.k, dp, cp, ri, dval, nnz, db, W, tau,
-                                              jpvt, &tol, ar, false, sa32);
+                                              jpvt, &tol, ar, false, W64);
         const bool short_rank = r32 < std::min(m, d);
         rank = std::max(r32, 1L);
         extract(W);
@@ -563,10 +595,11 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
             double *Wd = ar.alloc_n<double>((size_t)m * (d + 1));
             double *taud = ar.alloc_n<double>((size_t)d + 1);
             rank = sparse_factor<double>(n, d, m, o.k, dp, cp, ri, dval, nnz, db, Wd, taud, jpvt,
-                                         &tol, ar);
+                                         &tol, ar, true, W64);
             if (rank == 0) throw Error(SKL_E_RANK0, "SPQR Factorization fails");
             extract(Wd);
             trinv_upper(rank, Rd, rank, 0.0, Mi, rank, ar);
+            W64 = nullptr;
         }
     }
     info.escalated = escalated;
@@ -575,17 +608,11 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
     double rc = rcond_from(rank, Rd, rank, Mi, rank, ar);
     info.rcond = rc;
     info.success = rc > o.rcond_threshold ? 1 : 0;
-    double *Mp = Mi;  // LSQR operator
-    double *Mb = Mi;  // back-transform
+    double *Mp = Mi; // LSQR operator
     if (!info.success) {
         // LSQR operator uses (R11 + perturb I)^{-1} (sparse_helpers.cpp:138-147)
         Mp = ar.alloc_n<double>((size_t)rank * rank);
         trinv_upper(rank, Rd, rank, o.perturb, Mp, rank, ar);
-    } else if (mixed && !escalated && rank == d) {
-        // corrected FP32 preconditioner; x = x0 + M y with the same M
-        Mp = ar.alloc_n<double>((size_t)rank * rank);
-        refine_precond(m, d, sa32, m, Mi, Mp, ar);
-        Mb = Mp;
     }
     sync();
     double T2 = wall();
@@ -594,6 +621,8 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
     // x0 = P M c (no perturbation, ls_sparse_spqr.cpp:144)
     double *c1 = ar.alloc_n<double>((size_t)rank);
     trmv_n(rank, Mi, rank, cvec, c1, ar);
+    // FP32 factor (unpivoted, rank d): refine the start against the FP64 sketch
+    if (W64 && rank == d) refine_sketched_solution(m, d, W64, m, W64 + d * m, Mi, rank, c1, 3, ar);
     double *x0 = ar.alloc_n<double>((size_t)d);
     hipLaunchKernelGGL(k_fill_d, dim3(gblocks(d)), dim3(256), 0, s, d, x0, 0.0);
     hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank, jpvt, c1, x0);
@@ -631,7 +660,7 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
                                 t_hist, t_hist_len, ar);
     // y <- R11^{-1} y (no perturbation, ls_sparse_spqr.cpp:188); y += c; x = P y
     double *my = ar.alloc_n<double>((size_t)rank);
-    trmv_n(rank, Mb, rank, y, my, ar);
+    trmv_n(rank, Mi, rank, y, my, ar);
     hipLaunchKernelGGL(k_axpy_d, dim3(gblocks(rank)), dim3(256), 0, s, rank, c1, my);
     hipLaunchKernelGGL(k_fill_d, dim3(gblocks(d)), dim3(256), 0, s, d, dx, 0.0);
     hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank, jpvt, my, dx);

@@ -77,15 +77,21 @@ def c1_problem(seed=0, n=4000, d=200):
 def sparse_problem(seed=0, n=3000, d=100, density=0.02, decades=8, unit_diag=True):
     """Sparse problem of the 120000x5000 family at reduced size: uniform
     positions, N(0,1) values, columns scaled by logspace(0,-decades,d), unit
-    diagonal on the top d x d block (unless unit_diag is False, which leaves
-    the column scaling as the condition number), b = A x* + 1e-8 noise."""
+    diagonal on the top d x d block, b = A x* + 1e-8 noise.  unit_diag:
+    True adds the diagonal after the column scaling (columns keep norm >= 1),
+    "scaled" adds it before (A = (S + I) diag(s): full rank, column norms
+    spread over the scaling's range), False leaves it out."""
     import scipy.sparse as sp
     rng = np.random.default_rng(seed)
     A = sp.random(n, d, density=density, random_state=rng, format="csc",
                   data_rvs=rng.standard_normal)
-    A = A @ sp.diags(np.logspace(0, -decades, d))
-    if unit_diag:
-        A = A + sp.eye(n, d, format="csc")
+    scale = sp.diags(np.logspace(0, -decades, d))
+    if unit_diag == "scaled":
+        A = (A + sp.eye(n, d, format="csc")) @ scale
+    else:
+        A = A @ scale
+        if unit_diag:
+            A = A + sp.eye(n, d, format="csc")
     A = sp.csc_matrix(A)
     A.sort_indices()
     xs = rng.standard_normal(d)

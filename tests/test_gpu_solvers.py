@@ -154,12 +154,27 @@ def test_mixed_precision_dense(skl):
     assert i32.lsqr_nrm_ar < 1e-10 * i32.lsqr_nrm_a * i32.lsqr_nrm_r
 
 
-def test_mixed_precision_escalates_on_ill_conditioned_sparse(skl):
-    # no unit diagonal: cond(A) ~ 1e10 sits beyond what an FP32 factor resolves
-    A, b = sparse_problem(8, n=4000, d=100, density=0.02, decades=10, unit_diag=False)
+@pytest.mark.parametrize("unit_diag", ["scaled", False])
+def test_mixed_precision_escalates_on_ill_conditioned_sparse(skl, unit_diag):
+    """Columns scaled by logspace(0,-10): column norms spread over 1e10, so the
+    FP32 factor cannot make the rank decision that the FP64 tolerance
+    20 (m + d) eps max_j ||SA_j|| makes, and the solve falls back to FP64."""
+    A, b = sparse_problem(8, n=4000, d=100, density=0.02, decades=10, unit_diag=unit_diag)
     assert np.linalg.cond(A.toarray()) > 1e8
     x, info = skl.sparse_solve(A, b, gamma=1.5, k=3, it_tol=1e-10, precision=skl.SKL_MIXED)
     assert info.escalated == 1 and info.flag == 0
+    x64, _ = skl.sparse_solve(A, b, gamma=1.5, k=3, it_tol=1e-10)
+    assert rel(x, x64) < 1e-8
+
+
+def test_mixed_precision_keeps_fp32_when_unit_diagonal_dominates(skl):
+    """The same scaling with the unit diagonal added after it leaves every
+    column of norm >= 1 and cond(A) small: the FP32 factor preconditions it and
+    no fallback is taken."""
+    A, b = sparse_problem(8, n=4000, d=100, density=0.02, decades=10, unit_diag=True)
+    assert np.linalg.cond(A.toarray()) < 1e3
+    x, info = skl.sparse_solve(A, b, gamma=1.5, k=3, it_tol=1e-10, precision=skl.SKL_MIXED)
+    assert info.escalated == 0 and info.flag == 0
     x64, _ = skl.sparse_solve(A, b, gamma=1.5, k=3, it_tol=1e-10)
     assert rel(x, x64) < 1e-8
 
@@ -180,3 +195,21 @@ def test_fused_lsqr_pass_matches_two_pass(skl, O, monkeypatch, n, d):
     if n * d <= 6000 * 300:
         xo, io, _ = O.ls_dense(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=4)
         assert rel(x1, xo) < 1e-8
+
+
+def test_sparse_rank_deficient_wide_refactorises_with_hqrrp(skl):
+    """More than 1000 columns with 50 duplicated ones: the natural-order QR
+    meets SPQR's tolerance, and the column-pivoted refactorisation (HQRRP at
+    this width) finds the rank.  Any least-squares solution has the residual
+    of LAPACK's SVD solution (dgelss; dgelsd's divide and conquer leaves a
+    residual 1% above the minimum on this exactly rank-deficient matrix)."""
+    A, b = sparse_problem(21, n=5000, d=1100, density=0.01)
+    A = sp.hstack([A[:, :1050], A[:, :50]]).tocsc()
+    A.sort_indices()
+    b = b + 0.1 * np.random.default_rng(3).standard_normal(b.shape[0])
+    x, info = skl.sparse_solve(A, b, gamma=1.5, k=3, it_tol=1e-10)
+    assert info.rank == 1050 and info.flag == 0
+    Ad = A.toarray()
+    xl = sl.lstsq(Ad, b, lapack_driver="gelss")[0]
+    r, rl = np.linalg.norm(Ad @ x - b), np.linalg.norm(Ad @ xl - b)
+    assert abs(r - rl) <= 1e-8 * rl
