@@ -265,22 +265,33 @@ __global__ __launch_bounds__(FU_T) void k_fused_dense(long n, long d, const doub
 }
 
 // ---- reductions / scatter ----------------------------------------------------
+// Sums of partial products: 64 outputs per 256-thread block, each summed by
+// four threads over every fourth partial, then combined in a fixed order.
+constexpr int RED_OUT = 64, RED_GRP = 4;
+
 // z[j] = sum_c Pz[c * r + Einv[j]]  (0 when column j is outside E[0:r])
-__global__ void k_z_reduce(long d, long r, int nck, const double *__restrict__ Pz,
-                           const int32_t *__restrict__ Einv, double *__restrict__ z,
-                           const LsqrState *__restrict__ st) {
+__global__ __launch_bounds__(256) void k_z_reduce(long d, long r, int nck,
+                                                  const double *__restrict__ Pz,
+                                                  const int32_t *__restrict__ Einv,
+                                                  double *__restrict__ z,
+                                                  const LsqrState *__restrict__ st) {
     if (st && st->done) return;
-    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= d) return;
-    long i = Einv ? (long)Einv[j] : j;
+    __shared__ double sh[RED_GRP][RED_OUT];
+    const int e = threadIdx.x % RED_OUT, g = threadIdx.x / RED_OUT;
+    const long j = (long)blockIdx.x * RED_OUT + e;
     double s = 0.0;
-    if (i >= 0 && i < r)
-        for (int c = 0; c < nck; ++c) s += Pz[(long)c * r + i];
-    z[j] = s;
+    if (j < d) {
+        const long i = Einv ? (long)Einv[j] : j;
+        if (i >= 0 && i < r)
+            for (int c = g; c < nck; c += RED_GRP) s += Pz[(long)c * r + i];
+    }
+    sh[g][e] = s;
+    __syncthreads();
+    if (g == 0 && j < d) z[j] = (sh[0][e] + sh[1][e]) + (sh[2][e] + sh[3][e]);
 }
 
 // ut[i] = sum_c Pn[c * n + i] - coef * ut[i];  Sb[block] = sum of squares
-constexpr int RN_PER = 1024;
+constexpr int RN_PER = 256; // rows per block, one per thread
 __global__ __launch_bounds__(256) void k_reduce_n(long n, int nck, const double *__restrict__ Pn,
                                                   double *__restrict__ ut, double *__restrict__ Sb,
                                                   LsqrState *__restrict__ st, int init) {
@@ -326,17 +337,25 @@ template <int BS> __device__ inline double beta_from(const double *Sb, int nsb) 
 }
 
 // t[i] = (sum_rb Pt[rb * d + E[i]]) / beta
-__global__ void k_reduce_t(long d, long r, int nrb, const double *__restrict__ Pt,
-                           const int32_t *__restrict__ E, const double *__restrict__ Sb, int nsb,
-                           double *__restrict__ t, const LsqrState *__restrict__ st) {
+__global__ __launch_bounds__(256) void k_reduce_t(long d, long r, int nrb,
+                                                  const double *__restrict__ Pt,
+                                                  const int32_t *__restrict__ E,
+                                                  const double *__restrict__ Sb, int nsb,
+                                                  double *__restrict__ t,
+                                                  const LsqrState *__restrict__ st) {
     if (st->done) return;
+    __shared__ double sh[RED_GRP][RED_OUT];
     const double sbeta = beta_from<256>(Sb, nsb);
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= r) return;
-    long j = E ? (long)E[i] : i;
+    const int e = threadIdx.x % RED_OUT, g = threadIdx.x / RED_OUT;
+    const long i = (long)blockIdx.x * RED_OUT + e;
     double s = 0.0;
-    for (int q = 0; q < nrb; ++q) s += Pt[(long)q * d + j];
-    t[i] = s * (1.0 / sbeta);
+    if (i < r) {
+        const long j = E ? (long)E[i] : i;
+        for (int q = g; q < nrb; q += RED_GRP) s += Pt[(long)q * d + j];
+    }
+    sh[g][e] = s;
+    __syncthreads();
+    if (g == 0 && i < r) t[i] = ((sh[0][e] + sh[1][e]) + (sh[2][e] + sh[3][e])) * (1.0 / sbeta);
 }
 
 // ---- scalar recurrences (one workgroup) ---------------------------------------
@@ -601,7 +620,7 @@ static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs 
         }
     }
     // t = Pt(E) / beta
-    k_reduce_t<<<(unsigned)((q.r + 255) / 256), 256, 0, s>>>(q.d, q.r, nrb, B.Pt, p.E, B.Sb, nsb,
+    k_reduce_t<<<(unsigned)((q.r + RED_OUT - 1) / RED_OUT), 256, 0, s>>>(q.d, q.r, nrb, B.Pt, p.E, B.Sb, nsb,
                                                              B.t, B.st);
     // Pm = M^T t
     {
@@ -650,7 +669,7 @@ static void launch_iteration(const LsqrProblem &p, const LsqrPlan &q, const Lsqr
         dim3 g((unsigned)((q.r + GN_ROWS - 1) / GN_ROWS), (unsigned)q.nck_m);
         launch_ev(k_gemv_n_part<1>, g, dim3(GN_THREADS), 0, s, evi(ev, 4), evi(ev, 5), q.r, q.r,
                   p.M, p.ldm, B.v, q.cw_m, B.Pz, 1, B.st);
-        k_z_reduce<<<(unsigned)((q.d + 255) / 256), 256, 0, s>>>(q.d, q.r, q.nck_m, B.Pz, B.Einv,
+        k_z_reduce<<<(unsigned)((q.d + RED_OUT - 1) / RED_OUT), 256, 0, s>>>(q.d, q.r, q.nck_m, B.Pz, B.Einv,
                                                                  B.z, B.st);
     }
     // ut = A z - coef ut ; Sb

@@ -340,7 +340,8 @@ static int barrier_grid_cap() { return std::max(1, ctx().num_cu / 2); }
 // Householder step (||x||^2 below the diagonal and the products of the
 // reflector column with every column) are plain per-lane sums: no shuffles,
 // no LDS traffic for the matrix itself.  The reflector column's rows are
-// broadcast through a small per-wave LDS vector.  Pivoting (dlaqp2 /
+// broadcast through a small per-wave LDS vector (measured faster here than
+// v_readlane, whose scalar results spill at these register counts).  Pivoting (dlaqp2 /
 // HQRRP panel rules, as in k_qr_rowslice) moves no data: each lane keeps the
 // logical position of its column, and the partial norms live in the lane of
 // their column; the pivot is found by a wave-wide argmax that every wave of
@@ -728,35 +729,43 @@ __global__ void k_vfull(long mp, int b, const T *W, long ldw, T *Vf) {
     Vf[i + (long)q * mp] = v;
 }
 
-// T (b x b, ld b, b <= 64) from VtV and tau (dlarft forward columnwise):
-// T(0:i, i) = T(0:i, 0:i) (-tau_i VtV(0:i, i)), T(i, i) = tau_i.  Built in LDS;
-// four adjacent lanes share one row of each triangular product.
+// T (b x b, ld b, b <= 64) of the compact WY form I - V T V^T of the
+// reflectors H_0 .. H_{b-1} (dlarft forward columnwise), from VtV = V^T V and
+// tau through the closed form T^{-1} = diag(1 / tau) + striu(V^T V) (checked
+// for two reflectors: T = [tau_0, -tau_0 tau_1 v_0^T v_1; 0, tau_1]).  A
+// reflector with tau_i = 0 is the identity; its row and column of T are
+// zero, so it is left out of the inverse.  Thread j solves for column j by
+// back substitution; no barrier inside the solve.
 template <class T>
-__global__ __launch_bounds__(256) void k_build_t(int b, const T *VtV, const T *tau, T *Tm) {
+__global__ __launch_bounds__(64) void k_build_t(int b, const T *VtV, const T *tau, T *Tm) {
     constexpr int LD = 65;
-    __shared__ T sT[64 * LD], sV[64 * LD], tmp[64], st[64];
-    for (int e = threadIdx.x; e < b * b; e += 256) {
-        int r = e % b, q = e / b;
-        sV[r + q * LD] = VtV[e];
-        sT[r + q * LD] = T(0);
-    }
-    if (threadIdx.x < b) st[threadIdx.x] = tau[threadIdx.x];
+    __shared__ T sU[64 * LD], sT[64 * LD];
+    __shared__ T st[64];
+    const int j = threadIdx.x;
+    if (j < b) st[j] = tau[j];
     __syncthreads();
-    const int r = threadIdx.x >> 2, part = threadIdx.x & 3;
-    for (int i = 0; i < b; ++i) {
-        const T ti = st[i];
-        if (threadIdx.x < i) tmp[threadIdx.x] = -ti * sV[threadIdx.x + i * LD];
-        __syncthreads();
-        T s = 0;
-        if (r < i)
-            for (int q = r + part; q < i; q += 4) s += sT[r + q * LD] * tmp[q];
-        s += __shfl_xor(s, 1, 64);
-        s += __shfl_xor(s, 2, 64);
-        if (r < i && part == 0) sT[r + i * LD] = s;
-        if (threadIdx.x == 0) sT[i + i * LD] = ti;
-        __syncthreads();
+    for (int e = j; e < b * b; e += 64) {
+        const int r = e % b, q = e / b;
+        T u = T(0);
+        if (st[r] != T(0) && st[q] != T(0)) u = r < q ? VtV[e] : (r == q ? T(1) / st[r] : T(0));
+        else if (r == q) u = T(1);
+        sU[r + q * LD] = u;
     }
-    for (int e = threadIdx.x; e < b * b; e += 256) Tm[e] = sT[(e % b) + (e / b) * LD];
+    __syncthreads();
+    if (j < b) {
+        // U t = e_j, U upper triangular: t_i = -(sum_{k=i+1..j} U_ik t_k) / U_ii
+        T *t = sT + j * LD;
+        for (int i = j + 1; i < b; ++i) t[i] = T(0);
+        t[j] = T(1) / sU[j + j * LD];
+        for (int i = j - 1; i >= 0; --i) {
+            T acc = T(0);
+            for (int k = i + 1; k <= j; ++k) acc += sU[i + k * LD] * t[k];
+            t[i] = -acc / sU[i + i * LD];
+        }
+        if (st[j] == T(0)) t[j] = T(0);
+    }
+    __syncthreads();
+    for (int e = j; e < b * b; e += 64) Tm[e] = sT[(e % b) + (e / b) * LD];
 }
 
 // C (mp x nc2) := (I - V T V^T)^T C
@@ -777,31 +786,70 @@ static void panel_t(long mp, int b, const T *Wp, long ldw, const T *tau, T *Vf, 
     k_vfull<T><<<g, 256, 0, ctx().stream>>>(mp, b, Wp, ldw, Vf);
     T *VtV = ar.alloc_n<T>((size_t)b * b);
     gemm<T>('T', 'N', b, b, mp, T(1), Vf, mp, Vf, mp, T(0), VtV, b);
-    k_build_t<T><<<1, 256, 0, ctx().stream>>>(b, VtV, tau, Tm);
+    k_build_t<T><<<1, 64, 0, ctx().stream>>>(b, VtV, tau, Tm);
     SKL_CHECK_LAUNCH();
 }
 
 // ---------------------------------------------------------------------------
 // Blocked Householder QR: factor the first nfac columns, apply Q^T to the rest
 // ---------------------------------------------------------------------------
+// Look-ahead: after panel p, only the next panel's columns are updated on the
+// main stream; the rest of the trailing matrix is updated on the side stream
+// while the next panel is factored (the panel kernel holds few CUs, the GEMMs
+// take the others).  Hand-offs: the side update of panel p starts after the
+// main stream has built V_p, T_p; the main stream updates the columns of
+// panel p+2 with V_{p+1} only after the side update of panel p (which also
+// reached them) is done.  V and T are double-buffered for that distance.
+// SKL_QR_LOOKAHEAD=0 turns it off.
+static bool lookahead_on() {
+    static const bool on = [] {
+        const char *e = getenv("SKL_QR_LOOKAHEAD");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 template <class T>
 void qr_householder(long m, long ncol, long nfac, T *W, long ldw, T *tau, Arena &ar) {
     long kmax = std::min(m, nfac);
     const int nb = (nfac >= 256) ? 64 : 32;
-    size_t mark = ar.used();
-    (void)mark;
-    T *Vf = ar.alloc_n<T>((size_t)m * nb);
-    T *Tm = ar.alloc_n<T>((size_t)nb * nb);
-    for (long j = 0; j < kmax; j += nb) {
+    const long npan = (kmax + nb - 1) / nb;
+    const bool ahead = npan >= 3 && lookahead_on();
+    T *Vf[2], *Tm[2];
+    Vf[0] = ar.alloc_n<T>((size_t)m * nb);
+    Tm[0] = ar.alloc_n<T>((size_t)nb * nb);
+    Vf[1] = ahead ? ar.alloc_n<T>((size_t)m * nb) : Vf[0];
+    Tm[1] = ahead ? ar.alloc_n<T>((size_t)nb * nb) : Tm[0];
+    Context &c = ctx();
+    if (ahead) c.make_side();
+    bool side_busy = false;
+    for (long j = 0, p = 0; j < kmax; j += nb, ++p) {
         int b = (int)std::min<long>(nb, kmax - j);
         long mp = m - j;
         rowslice_qr<T>(mp, b, 0, b, W + j + j * ldw, ldw, tau + j, nullptr, nullptr, 0, ar);
         long nc2 = ncol - j - b;
-        if (nc2 > 0) {
-            panel_t<T>(mp, b, W + j + j * ldw, ldw, tau + j, Vf, Tm, ar);
-            apply_block_left_t<T>(mp, b, Vf, Tm, nc2, W + j + (j + b) * ldw, ldw, ar);
+        if (nc2 <= 0) continue;
+        T *V = Vf[p & 1], *Tp = Tm[p & 1];
+        panel_t<T>(mp, b, W + j + j * ldw, ldw, tau + j, V, Tp, ar);
+        T *C2 = W + j + (j + b) * ldw;
+        if (!ahead) {
+            apply_block_left_t<T>(mp, b, V, Tp, nc2, C2, ldw, ar);
+            continue;
+        }
+        const long nn = std::min<long>(nb, nc2), nr = nc2 - nn;
+        if (side_busy) SKL_HIP(hipStreamWaitEvent(c.stream, c.ev[1], 0));
+        apply_block_left_t<T>(mp, b, V, Tp, nn, C2, ldw, ar);
+        side_busy = false;
+        if (nr > 0) {
+            SKL_HIP(hipEventRecord(c.ev[0], c.stream));
+            OnSideStream on_side;
+            SKL_HIP(hipStreamWaitEvent(c.stream, c.ev[0], 0));
+            apply_block_left_t<T>(mp, b, V, Tp, nr, C2 + nn * ldw, ldw, ar);
+            SKL_HIP(hipEventRecord(c.ev[1], c.stream));
+            side_busy = true;
         }
     }
+    if (side_busy) SKL_HIP(hipStreamWaitEvent(c.stream, c.ev[1], 0));
 }
 template void qr_householder<double>(long, long, long, double *, long, double *, Arena &);
 template void qr_householder<float>(long, long, long, float *, long, float *, Arena &);
@@ -1237,7 +1285,10 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
         }
         const int pl = p;
         const T tau = s_tau[u];
-        const T *vv = s_vec + (size_t)u * YQ_MAXROWS;
+        // lane l holds v[l] and v[l + 64]; rows reach every lane as scalar
+        // operands (v_readlane) rather than as LDS broadcasts
+        const T vlo = s_vec[(size_t)u * YQ_MAXROWS + lane];
+        const T vhi = s_vec[(size_t)u * YQ_MAXROWS + 64 + lane];
         SKL_PHASE(1, 4);
         // logical swap of positions jj and pl, then y -= tau v (v^T y)
         if (pos == pl)
@@ -1245,15 +1296,19 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
         else if (pos == jj)
             pos = pl;
         const bool act = pos > jj && pos != NONE;
-        if (act && tau != T(0)) {
+        // every lane runs the loops (v_readlane reads lanes outside EXEC, so
+        // the values it reads must have been written by all lanes); retired
+        // columns get f = 0
+        {
             T g = 0;
 #pragma unroll
-            for (int r = 0; r < YQ_ROWS; ++r) g += vv[r] * y[r];
-            const T f = -tau * g;
-            // read v from LDS again rather than holding all of it in registers
-            asm volatile("" ::: "memory");
+            for (int r = 0; r < YQ_ROWS; ++r) g += (r < 64 ? readlane(vlo, r) : readlane(vhi, r - 64)) * y[r];
+            const T f = (act && tau != T(0)) ? -tau * g : T(0);
+            // read the lanes again rather than keeping 2 x 74 scalars live
+            T wlo = vlo, whi = vhi;
+            asm volatile("" : "+v"(wlo), "+v"(whi));
 #pragma unroll
-            for (int r = 0; r < YQ_ROWS; ++r) y[r] += vv[r] * f;
+            for (int r = 0; r < YQ_ROWS; ++r) y[r] += (r < 64 ? readlane(wlo, r) : readlane(whi, r - 64)) * f;
         }
         // norm downdating (skint.c:920-990), recomputed in the lane when needed
         if (act && dn != T(0)) {

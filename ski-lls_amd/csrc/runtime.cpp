@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <utility>
 
 #include "skl_internal.h"
 
@@ -136,13 +137,39 @@ void *Context::workspace(size_t bytes) {
     return ws;
 }
 
-// Frees the calling thread's stream, arena and cached graphs (worker exit).
+void Context::make_side() {
+    if (side) return;
+    SKL_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    for (auto &e : ev) SKL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
+OnSideStream::OnSideStream() {
+    Context &c = ctx();
+    c.make_side();
+    std::swap(c.stream, c.side);
+    std::swap(c.ws, c.ws_side);
+    std::swap(c.ws_cap, c.ws_side_cap);
+}
+
+OnSideStream::~OnSideStream() {
+    Context &c = ctx();
+    std::swap(c.stream, c.side);
+    std::swap(c.ws, c.ws_side);
+    std::swap(c.ws_cap, c.ws_side_cap);
+}
+
+// Frees the calling thread's streams, arena and cached graphs (worker exit).
 void release_thread_context() {
     if (!t_ctx) return;
     (void)hipStreamSynchronize(t_ctx->stream);
+    if (t_ctx->side) (void)hipStreamSynchronize(t_ctx->side);
     lsqr_release_graphs();
     t_ctx->arena.release();
     if (t_ctx->ws) (void)hipFree(t_ctx->ws);
+    if (t_ctx->ws_side) (void)hipFree(t_ctx->ws_side);
+    for (auto &e : t_ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (t_ctx->side) (void)hipStreamDestroy(t_ctx->side);
     if (t_ctx->pinned) (void)hipHostFree(t_ctx->pinned);
     (void)hipStreamDestroy(t_ctx->stream);
     delete t_ctx;

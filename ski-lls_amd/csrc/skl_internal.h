@@ -78,6 +78,24 @@ struct Context {
     void *ws = nullptr;
     size_t ws_cap = 0;
     void *workspace(size_t bytes);
+    // second stream for work that overlaps the first (trailing updates of the
+    // blocked QR), with its own split-K scratch and two events for the
+    // hand-offs between the streams; made on first use (make_side)
+    hipStream_t side = nullptr;
+    void *ws_side = nullptr;
+    size_t ws_side_cap = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    void make_side();
+};
+
+// While alive, ctx().stream (and the split-K scratch) is the context's side
+// stream; everything issued through ctx() goes there.
+class OnSideStream {
+  public:
+    OnSideStream();
+    ~OnSideStream();
+    OnSideStream(const OnSideStream &) = delete;
+    OnSideStream &operator=(const OnSideStream &) = delete;
 };
 
 Context &ctx();                       // default per-thread context

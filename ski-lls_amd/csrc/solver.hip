@@ -149,6 +149,23 @@ static const std::vector<double> &gaussian_for(long m) {
 
 static void sync() { SKL_HIP(hipStreamSynchronize(ctx().stream)); }
 
+// At the end of a solve the arena folds any overflow blocks into one
+// reservation of the solve's high-water size, so that a repeated solve of the
+// same shape allocates nothing -- including the first repeat.
+struct ArenaTrim {
+    Arena &ar;
+    explicit ArenaTrim(Arena &a) : ar(a) {}
+    ~ArenaTrim() {
+        (void)hipStreamSynchronize(ctx().stream);
+        try {
+            ar.reset();
+        } catch (...) {
+            // a failed reallocation leaves the arena empty; the next solve
+            // reserves again
+        }
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Dense solve (both variants), device pointers
 // ---------------------------------------------------------------------------
@@ -259,6 +276,7 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     SKL_REQUIRE(m >= o.k && m >= d, SKL_E_ARG, "sketch must have at least d rows");
     info.m = m;
     ar.reset();
+    ArenaTrim trim(ar);
     double T0 = wall();
     // ---- pattern (libc draw order of dense_helpers.cpp:92 / 12-14) ----
     Glibc g;
@@ -535,6 +553,7 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
     SKL_REQUIRE(m >= o.k && m >= 1, SKL_E_ARG, "invalid sketch parameters");
     info.m = m;
     ar.reset();
+    ArenaTrim trim(ar);
     double T0 = wall();
     int32_t *cp = ar.alloc_n<int32_t>((size_t)d + 1);
     int32_t *ri = ar.alloc_n<int32_t>((size_t)std::max<long>(nnz, 1));
