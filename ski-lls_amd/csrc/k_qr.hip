@@ -731,41 +731,66 @@ __global__ void k_vfull(long mp, int b, const T *W, long ldw, T *Vf) {
 
 // T (b x b, ld b, b <= 64) of the compact WY form I - V T V^T of the
 // reflectors H_0 .. H_{b-1} (dlarft forward columnwise), from VtV = V^T V and
-// tau through the closed form T^{-1} = diag(1 / tau) + striu(V^T V) (checked
-// for two reflectors: T = [tau_0, -tau_0 tau_1 v_0^T v_1; 0, tau_1]).  A
-// reflector with tau_i = 0 is the identity; its row and column of T are
-// zero, so it is left out of the inverse.  Thread j solves for column j by
-// back substitution; no barrier inside the solve.
+// tau, by recursive doubling: splitting the reflectors into halves V = [V1
+// V2], Q = (I - V1 T1 V1^T)(I - V2 T2 V2^T) gives T = [T1, -T1 (V1^T V2) T2;
+// 0, T2].  Eight 8 x 8 diagonal blocks come from the column recurrence
+// T(0:i, i) = -tau_i T(0:i, 0:i) VtV(0:i, i), then three merge levels
+// (8 -> 16 -> 32 -> 64) form the off-diagonal blocks with two small products
+// each.  Reflectors past b have tau = 0 (zero rows and columns).
 template <class T>
-__global__ __launch_bounds__(64) void k_build_t(int b, const T *VtV, const T *tau, T *Tm) {
+__global__ __launch_bounds__(256) void k_build_t(int b, const T *VtV, const T *tau, T *Tm) {
     constexpr int LD = 65;
-    __shared__ T sU[64 * LD], sT[64 * LD];
+    __shared__ T sT[64 * LD], sV[64 * LD], sY[64 * LD];
     __shared__ T st[64];
-    const int j = threadIdx.x;
-    if (j < b) st[j] = tau[j];
-    __syncthreads();
-    for (int e = j; e < b * b; e += 64) {
-        const int r = e % b, q = e / b;
-        T u = T(0);
-        if (st[r] != T(0) && st[q] != T(0)) u = r < q ? VtV[e] : (r == q ? T(1) / st[r] : T(0));
-        else if (r == q) u = T(1);
-        sU[r + q * LD] = u;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int r = e & 63, q = e >> 6;
+        sV[r + q * LD] = (r < b && q < b) ? VtV[r + q * b] : T(0);
+        sT[r + q * LD] = T(0);
     }
+    if (tid < 64) st[tid] = tid < b ? tau[tid] : T(0);
     __syncthreads();
-    if (j < b) {
-        // U t = e_j, U upper triangular: t_i = -(sum_{k=i+1..j} U_ik t_k) / U_ii
-        T *t = sT + j * LD;
-        for (int i = j + 1; i < b; ++i) t[i] = T(0);
-        t[j] = T(1) / sU[j + j * LD];
-        for (int i = j - 1; i >= 0; --i) {
-            T acc = T(0);
-            for (int k = i + 1; k <= j; ++k) acc += sU[i + k * LD] * t[k];
-            t[i] = -acc / sU[i + i * LD];
+    // 8 x 8 diagonal blocks: one thread per block runs the column recurrence
+    if (tid < 8) {
+        const int o = tid * 8;
+        for (int i = 0; i < 8; ++i) {
+            const int c = o + i;
+            const T ti = st[c];
+            T tmp[8];
+            for (int k = 0; k < i; ++k) tmp[k] = -ti * sV[(o + k) + c * LD];
+            for (int r = 0; r < i; ++r) {
+                T acc = T(0);
+                for (int k = r; k < i; ++k) acc += sT[(o + r) + (o + k) * LD] * tmp[k];
+                sT[(o + r) + c * LD] = acc;
+            }
+            sT[c + c * LD] = ti;
         }
-        if (st[j] == T(0)) t[j] = T(0);
     }
     __syncthreads();
-    for (int e = j; e < b * b; e += 64) Tm[e] = sT[(e % b) + (e / b) * LD];
+    for (int sz = 8; sz < 64; sz *= 2) {
+        // pairs (A, C) of sz x sz diagonal blocks at rows a0 and a0 + sz:
+        // Y = VtV(A, C) T_C, then T(A, C) = -T_A Y
+        const int npair = 64 / (2 * sz), per = sz * sz;
+        for (int e = tid; e < npair * per; e += 256) {
+            const int pr = e / per, w = e % per;
+            const int i = w % sz, j = w / sz;
+            const int a0 = pr * 2 * sz, c0 = a0 + sz;
+            T acc = T(0);
+            for (int k = 0; k <= j; ++k) acc += sV[(a0 + i) + (c0 + k) * LD] * sT[(c0 + k) + (c0 + j) * LD];
+            sY[(a0 + i) + (c0 + j) * LD] = acc;
+        }
+        __syncthreads();
+        for (int e = tid; e < npair * per; e += 256) {
+            const int pr = e / per, w = e % per;
+            const int i = w % sz, j = w / sz;
+            const int a0 = pr * 2 * sz, c0 = a0 + sz;
+            T acc = T(0);
+            for (int k = i; k < sz; ++k) acc += sT[(a0 + i) + (a0 + k) * LD] * sY[(a0 + k) + (c0 + j) * LD];
+            sT[(a0 + i) + (c0 + j) * LD] = -acc;
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < b * b; e += 256) Tm[e] = sT[(e % b) + (e / b) * LD];
 }
 
 // C (mp x nc2) := (I - V T V^T)^T C
@@ -786,7 +811,7 @@ static void panel_t(long mp, int b, const T *Wp, long ldw, const T *tau, T *Vf, 
     k_vfull<T><<<g, 256, 0, ctx().stream>>>(mp, b, Wp, ldw, Vf);
     T *VtV = ar.alloc_n<T>((size_t)b * b);
     gemm<T>('T', 'N', b, b, mp, T(1), Vf, mp, Vf, mp, T(0), VtV, b);
-    k_build_t<T><<<1, 64, 0, ctx().stream>>>(b, VtV, tau, Tm);
+    k_build_t<T><<<1, 256, 0, ctx().stream>>>(b, VtV, tau, Tm);
     SKL_CHECK_LAUNCH();
 }
 
@@ -1496,6 +1521,14 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
     T *ctau = ar.alloc_n<T>((size_t)2 * P_max);
     T *cvec = ar.alloc_n<T>((size_t)2 * P_max * YQ_MAXROWS);
     unsigned *bar = ar.alloc_n<unsigned>(1);
+    Context &c = ctx();
+    const bool ahead = lookahead_on();
+    if (ahead) c.make_side();
+    bool side_busy = false;
+    auto join_side = [&] {
+        if (side_busy) SKL_HIP(hipStreamWaitEvent(c.stream, c.ev[1], 0));
+        side_busy = false;
+    };
     for (long j = 0; j < mn; j += nb) {
         int b = (int)std::min<long>(nb, std::min(n - j, m - j));
         bool last = (j + nb >= m) || (j + nb >= n);
@@ -1549,6 +1582,10 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
                 fn<<<P, 64 * Wv, sh, s>>>(ya);
                 SKL_CHECK_LAUNCH();
             }
+            // the previous block's update of the rows below its R12 ran on
+            // the side stream beside this pivot search; the permutation
+            // moves those rows
+            join_side();
             k_perm_changed<<<1, 1024, 0, s>>>((int)nv, perm, changed, nch);
             // permute W(:, j:n), Y(:, j:n), jpvt(j:n) by the new column order
             dim3 gw((unsigned)((m + 255) / 256), (unsigned)(2 * b));
@@ -1561,6 +1598,7 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
             SKL_CHECK_LAUNCH();
         }
         // panel QRP of W(j:m, j:j+b), pivoting W(0:j, j:j+b) and Y(:, j:j+b)
+        join_side();
         long mp = m - j;
         rowslice_qr<T>(mp, b, b, b, W + j + j * ldw, ldw, tau + j, jpvt + j, swaps, 1, ar);
         {
@@ -1571,7 +1609,25 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
         SKL_CHECK_LAUNCH();
         panel_t<T>(mp, b, W + j + j * ldw, ldw, tau + j, Vf, Tm, ar);
         long nc2 = ncol - j - b;
-        if (nc2 > 0) apply_block_left_t<T>(mp, b, Vf, Tm, nc2, W + j + (j + b) * ldw, ldw, ar);
+        T *C2 = W + j + (j + b) * ldw;
+        if (nc2 > 0 && ahead && !last && mp > b) {
+            // C2 -= V (T^T (V^T C2)): the top b rows (R12, which the Y
+            // downdate needs) here, the rows below on the side stream, where
+            // the update overlaps the Y downdate and the next pivot search
+            T *Wt = ar.alloc_n<T>((size_t)b * nc2);
+            T *Wt2 = ar.alloc_n<T>((size_t)b * nc2);
+            gemm<T>('T', 'N', b, nc2, mp, T(1), Vf, mp, C2, ldw, T(0), Wt, b);
+            gemm<T>('T', 'N', b, nc2, b, T(1), Tm, b, Wt, b, T(0), Wt2, b);
+            gemm<T>('N', 'N', b, nc2, b, T(-1), Vf, mp, Wt2, b, T(1), C2, ldw);
+            SKL_HIP(hipEventRecord(c.ev[0], c.stream));
+            OnSideStream on_side;
+            SKL_HIP(hipStreamWaitEvent(c.stream, c.ev[0], 0));
+            gemm<T>('N', 'N', mp - b, nc2, b, T(-1), Vf + b, mp, Wt2, b, T(1), C2 + b, ldw);
+            SKL_HIP(hipEventRecord(c.ev[1], c.stream));
+            side_busy = true;
+        } else if (nc2 > 0) {
+            apply_block_left_t<T>(mp, b, Vf, Tm, nc2, C2, ldw, ar);
+        }
         if (!last) {
             // Y2 -= (G1 - (G V) T U11^T) R12 ; G(:, j:m) -= (G V) T V^T
             T *G1 = G + j * mY;
@@ -1587,6 +1643,7 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
             gemm<T>('N', 'T', mY, mp, b, T(-1), BT, mY, Vf, mp, T(1), G1, mY);
         }
     }
+    join_side();
 }
 template void cpqr_hqrrp<double>(long, long, long, double *, long, int32_t *, double *,
                                  const double *, Arena &);
