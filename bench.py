@@ -41,6 +41,10 @@ WORKLOADS = {
     "dense_4000x200": dict(kind="dense", n=4000, d=200, decades=4, lognormal=0.0, gamma=1.7, k=1),
     "sparse_120000x5000": dict(kind="sparse", n=120000, d=5000, decades=8, density=0.01,
                                gamma=1.5, k=3),
+    # request stream: d log-uniform in [20, 2000], n log-uniform in [10 d, 200000],
+    # half dense (4000x200 family), half CSC at density U[0.5%, 2%] (sparse family)
+    "serving_stream": dict(kind="serving", dmin=20, dmax=2000, nmax=200000, pool=32,
+                           max_dense_elems=120_000_000),
 }
 
 
@@ -337,6 +341,135 @@ def cpu_baseline_sparse(A, bh, W, gpu_iters, rank):
     }
 
 
+def make_pool(W, seed):
+    """Distinct problems the request stream draws from (inputs generated once,
+    host resident, dense in column-major order)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    pool = []
+    for i in range(W["pool"]):
+        d = int(round(math.exp(rng.uniform(math.log(W["dmin"]), math.log(W["dmax"])))))
+        n = int(round(math.exp(rng.uniform(math.log(10 * d), math.log(W["nmax"])))))
+        if i % 2 == 0:
+            n = min(n, max(10 * d, W["max_dense_elems"] // d))
+            A = np.asfortranarray(rng.standard_normal((n, d)) * np.logspace(0, -4, d)[None, :])
+            xs = rng.standard_normal(d)
+            b = A @ xs + 1e-8 * rng.standard_normal(n)
+            pool.append(dict(kind="dense", n=n, d=d, A=A, b=b,
+                             opts=dict(gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=1000 + i)))
+        else:
+            dens = rng.uniform(0.005, 0.02)
+            A = sp.random(n, d, density=dens, random_state=rng, format="csc",
+                          data_rvs=rng.standard_normal)
+            A = sp.csc_matrix(A @ sp.diags(np.logspace(0, -8, d)) + sp.eye(n, d, format="csc"))
+            A.sort_indices()
+            xs = rng.standard_normal(d)
+            b = A @ xs + 1e-8 * rng.standard_normal(n)
+            pool.append(dict(kind="sparse", n=n, d=d, A=A, b=b,
+                             opts=dict(gamma=1.5, k=3, it_tol=1e-10, seed=1)))
+    return pool
+
+
+def run_serving(args, skl, W):
+    """The serving loop (continuous batching over worker slots, FP32
+    preconditioner, FP64 LSQR) against sequential plain-FP64 solves of the same
+    request stream (Poisson arrivals).  Both take host arrays (PCIe included).
+    The sequential figures replay the arrival times against the measured
+    per-problem solve times of one FIFO worker."""
+    log("generating the request pool")
+    pool = make_pool(W, seed=77)
+    rng = np.random.default_rng(78)
+    R = args.requests
+    which = rng.integers(0, len(pool), size=R)
+
+    def solve(P, precision):
+        if P["kind"] == "dense":
+            return skl.dense_solve(P["A"], P["b"], precision=precision, **P["opts"])
+        return skl.sparse_solve(P["A"], P["b"], precision=precision, **P["opts"])
+
+    log("sequential FP64 solves of the pool")
+    t_seq, x_seq = [], []
+    for P in pool:
+        solve(P, skl.SKL_FP64)
+        t0 = time.perf_counter()
+        x, info = solve(P, skl.SKL_FP64)
+        t_seq.append(time.perf_counter() - t0)
+        x_seq.append(x)
+    svc = np.array([t_seq[w] for w in which])
+    lam = args.load / float(svc.mean())
+    arr = np.cumsum(rng.exponential(1.0 / lam, size=R))
+    arr -= arr[0]
+    end = np.empty(R)
+    t = 0.0
+    for i in range(R):
+        t = max(t, arr[i]) + svc[i]
+        end[i] = t
+    lat_seq = end - arr
+    log(f"serving stream: {R} requests, rate {lam:.1f}/s, {args.slots} slots")
+    with skl.Server(slots=args.slots) as S:
+        for P in pool[:4]:  # warm the workers' code paths
+            S.wait(S.submit_dense(P["A"], P["b"], precision=skl.SKL_MIXED, **P["opts"])
+                   if P["kind"] == "dense" else
+                   S.submit_sparse(P["A"], P["b"], precision=skl.SKL_MIXED, **P["opts"]))
+        tickets = []
+        t0 = time.perf_counter()
+        for i in range(R):
+            now = time.perf_counter() - t0
+            if arr[i] > now:
+                time.sleep(arr[i] - now)
+            P = pool[which[i]]
+            if P["kind"] == "dense":
+                tickets.append(S.submit_dense(P["A"], P["b"], precision=skl.SKL_MIXED,
+                                              **P["opts"]))
+            else:
+                tickets.append(S.submit_sparse(P["A"], P["b"], precision=skl.SKL_MIXED,
+                                               **P["opts"]))
+        lat, worst, its = [], 0.0, []
+        for i, tk in enumerate(tickets):
+            x, info, q, r = S.wait(tk)
+            lat.append(q + r)
+            xr = x_seq[which[i]]
+            worst = max(worst, float(np.linalg.norm(x - xr) / np.linalg.norm(xr)))
+            if info.flag != 0:
+                raise RuntimeError(f"request {i}: LSQR flag {info.flag}")
+        span = time.perf_counter() - t0
+        # saturation: the same requests all at once
+        t1 = time.perf_counter()
+        burst = []
+        for i in range(R):
+            P = pool[which[i]]
+            if P["kind"] == "dense":
+                burst.append(S.submit_dense(P["A"], P["b"], precision=skl.SKL_MIXED, **P["opts"]))
+            else:
+                burst.append(S.submit_sparse(P["A"], P["b"], precision=skl.SKL_MIXED,
+                                             **P["opts"]))
+        for tk in burst:
+            S.wait(tk)
+        burst_span = time.perf_counter() - t1
+    lat = np.array(lat)
+    out = {
+        "requests": R, "arrival_rate_per_s": lam, "load_vs_sequential_fp64": args.load,
+        "slots": args.slots,
+        "server_mixed": {"solves_per_s": R / span, "p50_latency_s": float(np.percentile(lat, 50)),
+                         "p99_latency_s": float(np.percentile(lat, 99)),
+                         "saturated_solves_per_s": R / burst_span},
+        "sequential_fp64": {"solves_per_s": R / float(end[-1]),
+                            "capacity_solves_per_s": 1.0 / float(svc.mean()),
+                            "p50_latency_s": float(np.percentile(lat_seq, 50)),
+                            "p99_latency_s": float(np.percentile(lat_seq, 99))},
+        "max_x_rel_server_vs_sequential_fp64": worst,
+        "pool": {"dense": sum(P["kind"] == "dense" for P in pool),
+                 "sparse": sum(P["kind"] == "sparse" for P in pool),
+                 "largest_dense": max(P["n"] * P["d"] for P in pool if P["kind"] == "dense"),
+                 "largest_sparse_nnz": max(P["A"].nnz for P in pool if P["kind"] == "sparse")},
+    }
+    log(f"server p50 {out['server_mixed']['p50_latency_s'] * 1e3:.1f} ms p99 "
+        f"{out['server_mixed']['p99_latency_s'] * 1e3:.1f} ms; sequential p50 "
+        f"{out['sequential_fp64']['p50_latency_s'] * 1e3:.1f} ms p99 "
+        f"{out['sequential_fp64']['p99_latency_s'] * 1e3:.1f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -347,6 +480,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-escalation-case", action="store_true",
                     help="sparse workload: skip the logspace(0,-10) FP64-fallback check")
+    ap.add_argument("--requests", type=int, default=400, help="serving: requests in the stream")
+    ap.add_argument("--slots", type=int, default=4, help="serving: worker slots")
+    ap.add_argument("--load", type=float, default=0.8,
+                    help="serving: arrival rate as a fraction of one sequential FP64 worker's "
+                         "capacity")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the measurement, copy 1 GiB device-to-device three times "
                          "(reference kernel for scaling HBM byte counters)")
@@ -361,6 +499,21 @@ def main():
         sys.exit(3)
     skl = load_skl()
     W = WORKLOADS[args.workload]
+    if W["kind"] == "serving":
+        sv = run_serving(args, skl, W)
+        print(json.dumps({
+            "metric": "sec/solve to rel. normal-eq residual 1e-10, mixed vs FP64 on GPU; p99 latency",
+            "value": sv["server_mixed"]["p99_latency_s"], "unit": "sec (p99 latency)",
+            "n_gpus": 1, "steps": sv["requests"], "warmup": 4,
+            "ms_per_step": 1e3 / sv["server_mixed"]["solves_per_s"], "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": args.workload, "requests": sv["requests"],
+                       "slots": sv["slots"], "generator": "d logU[20,2000], n logU[10d,200000], "
+                       "half dense (logspace(0,-4) columns), half CSC density U[0.5%,2%] "
+                       "(logspace(0,-8) columns + unit diagonal), Poisson arrivals"},
+            "serving": sv, "roofline": None, "cpu_baseline": None}), flush=True)
+        skl.lib().skl_finalize()
+        return
     sparse = W["kind"] == "sparse"
     if sparse:
         res = run_sparse(args, skl, torch, W)

@@ -635,7 +635,7 @@ static bool regpanel_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T 
     a.bar = ar.alloc_n<unsigned>(1);
     hipStream_t s = ctx().stream;
     SKL_HIP(hipMemsetAsync(a.bar, 0, sizeof(unsigned), s));
-    BarrierLaunch guard(false);
+    BarrierLaunch guard((int)P);
     switch (K) {
     case 8: launch_regpanel<T, 8>((int)P, a, s); break;
     case 16: launch_regpanel<T, 16>((int)P, a, s); break;
@@ -711,7 +711,7 @@ static void rowslice_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T 
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     // grids over more than half the device (wide matrices in the greedy path)
     // always run alone
-    BarrierLaunch guard(P > P_half);
+    BarrierLaunch guard(P, P > P_half);
     k_qr_rowslice<T><<<P, RS_T, sh, s>>>(a);
     SKL_CHECK_LAUNCH();
 }
@@ -1557,7 +1557,7 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                                 (int)sh));
                 SKL_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned), s));
-                BarrierLaunch guard;
+                BarrierLaunch guard(P);
                 k_yqrp_lanes<T, WL><<<P, 64 * WL, sh, s>>>(ya);
                 SKL_CHECK_LAUNCH();
             } else {
@@ -1578,7 +1578,7 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
                     SKL_HIP(hipFuncSetAttribute((const void *)fn,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                                 (int)sh));
-                BarrierLaunch guard;
+                BarrierLaunch guard(P);
                 fn<<<P, 64 * Wv, sh, s>>>(ya);
                 SKL_CHECK_LAUNCH();
             }

@@ -6,7 +6,9 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <utility>
+#include <vector>
 
 #include "skl_internal.h"
 
@@ -82,7 +84,6 @@ void *Arena::alloc(size_t bytes) {
 // ---------------------------------------------------------------------------
 static thread_local Context *t_ctx = nullptr;
 static std::mutex g_init_mu;
-static std::mutex g_barrier_mu;            // serialises grid-barrier launches
 static std::atomic<int> g_live_ctx{0};     // host threads holding a context
 static int g_device = -1;
 
@@ -107,14 +108,68 @@ void ensure_init() {
     g_live_ctx.fetch_add(1);
 }
 
-BarrierLaunch::BarrierLaunch(bool always) : held_(always || g_live_ctx.load() > 1) {
-    if (held_) g_barrier_mu.lock();
+namespace {
+struct PendingBarrier {
+    hipEvent_t ev;
+    int tokens;
+};
+std::mutex g_tok_mu;
+int g_tok_cap = 0, g_tok_free = 0;
+std::vector<PendingBarrier> g_tok_pending;
+std::vector<hipEvent_t> g_tok_events;
+
+// returns the tokens of completed launches (caller holds g_tok_mu)
+void reap_barriers() {
+    for (size_t i = 0; i < g_tok_pending.size();) {
+        if (hipEventQuery(g_tok_pending[i].ev) == hipSuccess) {
+            g_tok_free += g_tok_pending[i].tokens;
+            g_tok_events.push_back(g_tok_pending[i].ev);
+            g_tok_pending[i] = g_tok_pending.back();
+            g_tok_pending.pop_back();
+        } else {
+            (void)hipGetLastError(); // hipErrorNotReady is not an error here
+            ++i;
+        }
+    }
+}
+} // namespace
+
+BarrierLaunch::BarrierLaunch(int grid, bool exclusive) {
+    const int num_cu = ctx().num_cu;
+    std::unique_lock<std::mutex> lk(g_tok_mu);
+    if (g_tok_cap == 0) g_tok_cap = g_tok_free = num_cu;
+    const int need = exclusive ? g_tok_cap : std::max(1, std::min(grid, g_tok_cap));
+    while (g_tok_free < need) {
+        reap_barriers();
+        if (g_tok_free >= need) break;
+        lk.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        lk.lock();
+    }
+    g_tok_free -= need;
+    tokens_ = need;
 }
 
 BarrierLaunch::~BarrierLaunch() {
-    if (held_) {
+    hipEvent_t ev = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_tok_mu);
+        if (!g_tok_events.empty()) {
+            ev = g_tok_events.back();
+            g_tok_events.pop_back();
+        }
+    }
+    bool ok = ev != nullptr || hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventRecord(ev, ctx().stream) == hipSuccess;
+    std::lock_guard<std::mutex> lk(g_tok_mu);
+    if (ok) {
+        g_tok_pending.push_back({ev, tokens_});
+    } else {
+        // no event to wait on: hand the tokens back after the stream drains
+        (void)hipGetLastError();
         (void)hipStreamSynchronize(ctx().stream);
-        g_barrier_mu.unlock();
+        g_tok_free += tokens_;
+        if (ev) g_tok_events.push_back(ev);
     }
 }
 

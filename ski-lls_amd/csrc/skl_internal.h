@@ -102,20 +102,22 @@ Context &ctx();                       // default per-thread context
 void ensure_init();
 void release_thread_context();        // frees the calling thread's context
 
-// Kernels with a grid barrier need every workgroup resident at once.  Each
-// keeps its grid to at most half the device, which is safe next to one other
-// such launch; when several host threads drive the device (the serving loop),
-// or a grid is wider (always = true), the guard serialises these launches
-// process-wide and holds the lock until the launch has finished.
+// Kernels with a grid barrier need every workgroup resident at once, and each
+// of their workgroups fills a CU (registers or LDS), so the workgroups of all
+// such kernels in flight on the device must fit on its CUs together.  A launch
+// reserves one token per workgroup from a process-wide pool of num_cu tokens
+// (an exclusive launch, for grids over half the device, reserves them all) and
+// returns them when an event recorded after it completes; no host thread waits
+// for a launch unless the pool is short.
 class BarrierLaunch {
   public:
-    explicit BarrierLaunch(bool always = false);
-    ~BarrierLaunch();
+    explicit BarrierLaunch(int grid, bool exclusive = false);
+    ~BarrierLaunch(); // records the completion event on ctx().stream
     BarrierLaunch(const BarrierLaunch &) = delete;
     BarrierLaunch &operator=(const BarrierLaunch &) = delete;
 
   private:
-    bool held_;
+    int tokens_ = 0;
 };
 
 double wall();
