@@ -83,8 +83,13 @@ def make_sparse_host(W, seed, decades=None, diag_first=False):
     return A, b
 
 
+SKIP_REFERENCE = False
+
+
 def reference_solution(torch, At, b):
     """x = R^{-1} Q^T b from a dense Householder QR of A (A = At^T)."""
+    if SKIP_REFERENCE:
+        return None
     try:
         Q, R = torch.linalg.qr(At.t(), mode="reduced")
         y = Q.t() @ b
@@ -252,6 +257,8 @@ def run_sparse(args, skl, torch, W):
     # reference solution: dense Householder QR of A on the GPU (4.8 GB in FP64)
     xq = None
     try:
+        if SKIP_REFERENCE:
+            raise RuntimeError("--no-reference")
         Ad = torch.sparse_csc_tensor(cp, ri, va, size=(n, d), dtype=torch.float64).to_dense()
         xq = reference_solution(torch, Ad.t().contiguous(), b)
         del Ad
@@ -480,6 +487,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-escalation-case", action="store_true",
                     help="sparse workload: skip the logspace(0,-10) FP64-fallback check")
+    ap.add_argument("--no-reference", action="store_true",
+                    help="skip the dense Householder QR solution that x is compared with "
+                         "(x_rel_vs_qr is then null)")
     ap.add_argument("--requests", type=int, default=400, help="serving: requests in the stream")
     ap.add_argument("--slots", type=int, default=4, help="serving: worker slots")
     ap.add_argument("--load", type=float, default=0.8,
@@ -489,6 +499,8 @@ def main():
                     help="after the measurement, copy 1 GiB device-to-device three times "
                          "(reference kernel for scaling HBM byte counters)")
     args = ap.parse_args()
+    global SKIP_REFERENCE
+    SKIP_REFERENCE = args.no_reference
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the solve shards "
               "as described in DESIGN.md); run with --gpus 1", file=sys.stderr)

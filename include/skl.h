@@ -116,8 +116,9 @@ int skl_sparse_solve_dev(long n, long d, long nnz, const long *dcolptr, const lo
                          const skl_opts *opts, skl_info *info);
 
 /* Kernel timing of the LSQR operator products on this thread: when enabled,
- * HIP events on the library stream bracket the four products of the first
- * iteration of every replayed chunk.  slot 0: A z, 1: A^T u, 2: M v,
+ * the LSQR loop runs without graph replay and HIP events on the library
+ * stream bracket the four products of the second iteration of every chunk
+ * (chunks that end the solve are left out).  slot 0: A z, 1: A^T u, 2: M v,
  * 3: M^T t.  bytes_per_launch is the algorithmic traffic of one launch. */
 void skl_profile_enable(int on);
 void skl_profile_reset(void);

@@ -800,6 +800,10 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
         ev = g_ev;
     }
     hipGraphExec_t exec = nullptr;
+    // with kernel timing on, the iterations run eagerly (events cannot be
+    // timed inside a replayed graph) and the second iteration of every chunk
+    // is timed, on a device already busy with the first
+    if (prof) graph_ok = false;
     if (graph_ok) {
         GraphKey key;
         // every pointer and size baked into the captured kernels' arguments:
@@ -821,9 +825,7 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
         } else {
             hipGraph_t graph;
             SKL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            // with timing on, the first iteration of a chunk runs eagerly
-            // (events cannot be timed inside a replayed graph)
-            for (int c = prof ? 1 : 0; c < chunk; ++c) launch_iteration(p, q, B, s);
+            for (int c = 0; c < chunk; ++c) launch_iteration(p, q, B, s);
             hipError_t e1 = hipMemcpyAsync(hf, &B.st->done, sizeof(int), hipMemcpyDeviceToHost, s);
             hipError_t e2 = hipStreamEndCapture(s, &graph);
             if (e1 != hipSuccess || e2 != hipSuccess) {
@@ -844,17 +846,19 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
     SKL_HIP(hipStreamSynchronize(s));
     long launched = 0;
     while (!hf->done) {
-        if (prof) launch_iteration(p, q, B, s, ev);
         if (graph_ok) {
             SKL_HIP(hipGraphLaunch(exec, s));
         } else {
-            for (int c = prof ? 1 : 0; c < chunk; ++c) launch_iteration(p, q, B, s);
+            for (int c = 0; c < chunk; ++c)
+                launch_iteration(p, q, B, s, (prof && c == 1) ? ev : nullptr);
             SKL_CHECK_LAUNCH();
             SKL_HIP(hipMemcpyAsync(hf, &B.st->done, sizeof(int), hipMemcpyDeviceToHost, s));
         }
         launched += chunk;
         SKL_HIP(hipStreamSynchronize(s));
-        if (prof) prof_accumulate(q, p.nnz);
+        // a chunk that ended the solve may have timed kernels that returned
+        // at once; only chunks that ran every iteration count
+        if (prof && chunk > 1 && !hf->done) prof_accumulate(q, p.nnz);
         if (launched > max_it + chunk) break; // defensive: state always sets done by max_it
     }
     LsqrState hs;
