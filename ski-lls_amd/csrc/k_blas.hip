@@ -40,7 +40,10 @@ template <class T, bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, const T *__restrict__ A,
                                               long lda, const T *__restrict__ B, long ldb, T beta,
                                               T *__restrict__ C, long ldc, long sA, long sB,
-                                              long sC, int kchunk) {
+                                              long sC, int kchunk, int upper) {
+    // upper: only the tiles on or above the diagonal of a square C (A^T A
+    // for a symmetric result whose lower triangle is never read)
+    if (upper && blockIdx.x > blockIdx.y) return;
     typedef typename Mfma<T>::acc acc_t;
     __shared__ T As[2][GBK][GBM + 1];
     __shared__ T Bs[2][GBK][GBN + 1];
@@ -163,7 +166,7 @@ __global__ void k_splitk_reduce(int M, int N, int S, const T *W, T alpha, T beta
 
 template <class T>
 void gemm(char ta, char tb, long M, long N, long K, T alpha, const T *A, long lda, const T *B,
-          long ldb, T beta, T *C, long ldc, long batch, long sA, long sB, long sC) {
+          long ldb, T beta, T *C, long ldc, long batch, long sA, long sB, long sC, bool upper) {
     if (M <= 0 || N <= 0 || batch <= 0) return;
     const long tm = (M + GBM - 1) / GBM, tn = (N + GBN - 1) / GBN;
     hipStream_t s = ctx().stream;
@@ -197,16 +200,16 @@ void gemm(char ta, char tb, long M, long N, long K, T alpha, const T *A, long ld
     const int kc = (int)kchunk;
     if (!TA && !TB)
         k_gemm<T, false, false><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, al, A, lda, B, ldb,
-                                                     be, Cw, ldw, sA, sB, sW, kc);
+                                                     be, Cw, ldw, sA, sB, sW, kc, upper ? 1 : 0);
     else if (TA && !TB)
         k_gemm<T, true, false><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, al, A, lda, B, ldb,
-                                                    be, Cw, ldw, sA, sB, sW, kc);
+                                                    be, Cw, ldw, sA, sB, sW, kc, upper ? 1 : 0);
     else if (!TA && TB)
         k_gemm<T, false, true><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, al, A, lda, B, ldb,
-                                                    be, Cw, ldw, sA, sB, sW, kc);
+                                                    be, Cw, ldw, sA, sB, sW, kc, upper ? 1 : 0);
     else
         k_gemm<T, true, true><<<grid, 256, 0, s>>>((int)M, (int)N, (int)K, al, A, lda, B, ldb,
-                                                   be, Cw, ldw, sA, sB, sW, kc);
+                                                   be, Cw, ldw, sA, sB, sW, kc, upper ? 1 : 0);
     SKL_CHECK_LAUNCH();
     if (S > 1) {
         long tot = M * N;
@@ -217,9 +220,10 @@ void gemm(char ta, char tb, long M, long N, long K, T alpha, const T *A, long ld
 }
 
 template void gemm<double>(char, char, long, long, long, double, const double *, long,
-                           const double *, long, double, double *, long, long, long, long, long);
+                           const double *, long, double, double *, long, long, long, long, long,
+                           bool);
 template void gemm<float>(char, char, long, long, long, float, const float *, long,
-                          const float *, long, float, float *, long, long, long, long, long);
+                          const float *, long, float, float *, long, long, long, long, long, bool);
 
 // ---------------------------------------------------------------------------
 // Small helpers
@@ -448,6 +452,104 @@ void trinv_upper(long r, const double *R, long ldr, double perturb, double *M, l
             gemm<double>('N', 'N', s, s2, s, -1.0, M11, ldm, tmp, s, 0.0, M12, ldm);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky factorization G = R^T R (upper R) of a symmetric positive definite
+// d x d matrix, blocked right-looking with 64-column blocks: the diagonal
+// block in LDS (one workgroup), the block row by forward substitution (one
+// thread per column), the trailing matrix by a GEMM on the matrix cores.
+// ---------------------------------------------------------------------------
+constexpr int CH_B = 64, CH_LD = 65;
+
+// b x b diagonal block (b <= 64) in place, upper factor; a pivot that is not
+// positive and finite sets *fail and is replaced by 1 so the sweep completes
+__global__ __launch_bounds__(256) void k_chol_diag(int b, double *A, long lda, int *fail) {
+    __shared__ double S[CH_B * CH_LD];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < b * b; e += 256) {
+        const int r = e % b, c = e / b;
+        S[r + c * CH_LD] = r <= c ? A[r + (long)c * lda] : 0.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < b; ++k) {
+        const double p = S[k + k * CH_LD];
+        const bool bad = !(p > 0.0) || !(p < INFINITY);
+        const double rk = bad ? 1.0 : sqrt(p);
+        __syncthreads();
+        if (bad && tid == 0) *fail = 1;
+        for (int c = k + 1 + tid; c < b; c += 256) S[k + c * CH_LD] /= rk;
+        if (tid == 0) S[k + k * CH_LD] = rk;
+        __syncthreads();
+        const int w = b - k - 1;
+        for (int e = tid; e < w * w; e += 256) {
+            const int i = k + 1 + e % w, c = k + 1 + e / w;
+            if (i <= c) S[i + c * CH_LD] -= S[k + i * CH_LD] * S[k + c * CH_LD];
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < b * b; e += 256) {
+        const int r = e % b, c = e / b;
+        A[r + (long)c * lda] = r <= c ? S[r + c * CH_LD] : 0.0;
+    }
+}
+
+// X = U^{-T} B for the b x b upper factor U (b <= 64) and B (b x nc), in
+// place: thread c solves U^T x = B(:, c) by forward substitution
+__global__ __launch_bounds__(256) void k_chol_trsm(int b, const double *U, long ldu, double *B,
+                                                   long ldb, long nc) {
+    __shared__ double Us[CH_B * CH_LD];
+    for (int e = threadIdx.x; e < b * b; e += 256) {
+        const int r = e % b, c = e / b;
+        Us[r + c * CH_LD] = r <= c ? U[r + (long)c * ldu] : 0.0;
+    }
+    __syncthreads();
+    const long c = (long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= nc) return;
+    double *col = B + c * ldb;
+    double x[CH_B];
+#pragma unroll
+    for (int i = 0; i < CH_B; ++i) {
+        if (i < b) {
+            double s = col[i];
+#pragma unroll
+            for (int k = 0; k < i; ++k) s -= Us[k + i * CH_LD] * x[k];
+            x[i] = s / Us[i + i * CH_LD];
+        } else {
+            x[i] = 0.0;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < CH_B; ++i)
+        if (i < b) col[i] = x[i];
+}
+
+// R (upper, zeros below, ld ldr) from G (overwritten); false when a pivot
+// was not positive (G not numerically positive definite)
+bool cholesky_upper(long d, double *G, long ldg, double *R, long ldr, Arena &ar) {
+    hipStream_t s = ctx().stream;
+    const size_t mark = ar.used();
+    int *fail = ar.alloc_n<int>(1);
+    SKL_HIP(hipMemsetAsync(fail, 0, sizeof(int), s));
+    for (long j = 0; j < d; j += CH_B) {
+        const int b = (int)std::min<long>(CH_B, d - j);
+        const long nc = d - j - b;
+        double *Gjj = G + j + j * ldg;
+        k_chol_diag<<<1, 256, 0, s>>>(b, Gjj, ldg, fail);
+        if (nc > 0) {
+            double *Bp = G + j + (j + b) * ldg;
+            k_chol_trsm<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(b, Gjj, ldg, Bp, ldg, nc);
+            gemm<double>('T', 'N', nc, nc, b, -1.0, Bp, ldg, Bp, ldg, 1.0,
+                         G + (j + b) + (j + b) * ldg, ldg, 1, 0, 0, 0, true);
+        }
+        SKL_CHECK_LAUNCH();
+    }
+    upcast_upper<double>(d, G, ldg, R, ldr);
+    int hf = 0;
+    SKL_HIP(hipMemcpyAsync(&hf, fail, sizeof(int), hipMemcpyDeviceToHost, s));
+    SKL_HIP(hipStreamSynchronize(s));
+    ar.rewind(mark);
+    return hf == 0;
 }
 
 } // namespace skl

@@ -187,6 +187,9 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
                 const double *G_host, Arena &ar);
 
 // triangular inverse and norms
+// G = R^T R (R upper, zeros below); false when G is not numerically
+// positive definite.  G is overwritten.
+bool cholesky_upper(long d, double *G, long ldg, double *R, long ldr, Arena &ar);
 void trinv_upper(long r, const double *R, long ldr, double perturb, double *M, long ldm,
                  Arena &ar);
 template <class T>
@@ -201,7 +204,7 @@ void colnorm_upper(long r, const double *R, long ldr, double *D, double *invD);
 template <class T>
 void gemm(char ta, char tb, long M, long N, long K, T alpha, const T *A, long lda, const T *B,
           long ldb, T beta, T *C, long ldc, long batch = 1, long sA = 0, long sB = 0,
-          long sC = 0);
+          long sC = 0, bool upper = false);
 
 // LSQR on the preconditioned operator A(:, E) M
 struct LsqrProblem {

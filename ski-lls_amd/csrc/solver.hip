@@ -239,6 +239,82 @@ static double scaled_cond(long r, const double *Rd, long ldr, const double *M, l
     return a * b;
 }
 
+// ---- mixed precision: CholeskyQR of the column-equilibrated sketch ----------
+__global__ void k_norms_from_squares(long d, const double *sq, double *D, double *invD) {
+    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d) return;
+    const double v = sqrt(sq[j]);
+    D[j] = v;
+    invD[j] = v > 0.0 ? 1.0 / v : 0.0;
+}
+template <class T>
+__global__ void k_equil_to_float(long m, long d, const T *SA, long ldsa, const double *invD,
+                                 float *W) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    for (long j = blockIdx.y; j < d; j += gridDim.y)
+        W[i + j * m] = (float)((double)SA[i + j * ldsa] * invD[j]);
+}
+// R = R1 diag(D), M = diag(1/D) M1 (d x d, ld d)
+__global__ void k_unequilibrate(long d, const double *R1, const double *M1, const double *D,
+                                const double *invD, double *R, double *M) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d) return;
+    for (long j = blockIdx.y; j < d; j += gridDim.y) {
+        R[i + j * d] = R1[i + j * d] * D[j];
+        M[i + j * d] = invD[i] * M1[i + j * d];
+    }
+}
+
+// The optimistic factor of the mixed mode.  Only R (for M = R^{-1}) is
+// needed -- the start is refined against the FP64 sketch and LSQR works on
+// A M -- so any QR of SA serves, and CholeskyQR needs no per-column
+// reduction: the Gram matrix of the column-equilibrated sketch W = SA D^{-1}
+// (D = column norms) is one FP32 MFMA GEMM with FP32 accumulation, its
+// Cholesky factor R1 (FP64, blocked) gives R = R1 D.  Equilibration keeps the
+// Gram's condition number at cond(SA D^{-1})^2: FP32 resolves it while
+// cond(SA D^{-1}) stays below about 1/sqrt(u_32) = 4e3; beyond that the
+// Cholesky meets a non-positive pivot and the caller falls back to FP64.
+// Returns false on that breakdown; dg (host) receives diag(R), maxn the
+// largest column norm of SA.
+template <class T>
+static bool mixed_cholqr(long m, long d, const T *SA, long ldsa, double *Rd, double *Mi,
+                         std::vector<double> &dg, double &maxn, Arena &ar) {
+    hipStream_t s = ctx().stream;
+    const size_t mark = ar.used();
+    double *sq = ar.alloc_n<double>((size_t)d);
+    double *D = ar.alloc_n<double>((size_t)d);
+    double *invD = ar.alloc_n<double>((size_t)d);
+    hipLaunchKernelGGL(k_colnorm2<T>, dim3((unsigned)d), dim3(256), 0, s, m, d, SA, ldsa, sq);
+    hipLaunchKernelGGL(k_norms_from_squares, dim3(gblocks(d)), dim3(256), 0, s, d, sq, D, invD);
+    float *W = ar.alloc_n<float>((size_t)m * d);
+    hipLaunchKernelGGL(k_equil_to_float<T>, dim3(gblocks(m), (unsigned)std::min<long>(d, 65535)),
+                       dim3(256), 0, s, m, d, SA, ldsa, invD, W);
+    float *G = ar.alloc_n<float>((size_t)d * d);
+    gemm<float>('T', 'N', d, d, m, 1.0f, W, m, W, m, 0.0f, G, d, 1, 0, 0, 0, true);
+    double *G64 = ar.alloc_n<double>((size_t)d * d);
+    copy2d<float, double>(d, d, G, d, G64, d);
+    double *R1 = ar.alloc_n<double>((size_t)d * d);
+    const bool ok = cholesky_upper(d, G64, d, R1, d, ar);
+    std::vector<double> hD((size_t)d);
+    SKL_HIP(hipMemcpyAsync(hD.data(), D, sizeof(double) * d, hipMemcpyDeviceToHost, s));
+    if (ok) {
+        double *M1 = G64; // the Gram is spent
+        trinv_upper(d, R1, d, 0.0, M1, d, ar);
+        hipLaunchKernelGGL(k_unequilibrate, dim3(gblocks(d), (unsigned)std::min<long>(d, 65535)),
+                           dim3(256), 0, s, d, R1, M1, D, invD, Rd, Mi);
+        double *ddg = ar.alloc_n<double>((size_t)d);
+        hipLaunchKernelGGL(k_diag<double>, dim3(gblocks(d)), dim3(256), 0, s, d, Rd, d, ddg);
+        dg.resize((size_t)d);
+        SKL_HIP(hipMemcpyAsync(dg.data(), ddg, sizeof(double) * d, hipMemcpyDeviceToHost, s));
+    }
+    SKL_HIP(hipStreamSynchronize(s));
+    maxn = 0.0;
+    for (double v : hD) maxn = std::max(maxn, v);
+    ar.rewind(mark);
+    return ok;
+}
+
 // Iterative refinement of the sketch-and-solve start: y <- y + M M^T SA^T
 // (c - SA y), `steps` times, with SA and c the FP64 sketch and M the inverse
 // of the FP32 factor.  The start's error contracts by about ||I - (SA M)^T
@@ -328,43 +404,50 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
                                 &info.t_sketch, &info.t_qr, ar);
         finish_qr(W);
     } else {
-        // Optimistic FP32 preconditioner: an unpivoted Householder QR in FP32.
-        // Pivoting only matters when A is (nearly) rank deficient, and then the
-        // FP32 factor shows it -- a |R_ii| at or below rcond, or a large
-        // condition number of the column-equilibrated R -- and the
+        // Optimistic FP32 preconditioner (mixed_cholqr): R from the FP32 Gram
+        // matrix of the column-equilibrated sketch, no pivoting.  Pivoting
+        // only matters when A is (nearly) rank deficient, and then the factor
+        // shows it -- a Cholesky breakdown, a |R_ii| at or below rcond, or a
+        // large condition number of the column-equilibrated R -- and the
         // column-pivoted factorization is redone in FP64.  For a
         // well-conditioned full-rank A the least-squares solution is unique,
-        // whatever the column order of the preconditioner.  The robust path
-        // sketches in FP64 and factors a rounded copy: the FP64 sketch refines
-        // the sketch-and-solve start and serves the fallback without a second
-        // transform.  The fast path (no start) sketches in FP32.
-        float *W = ar.alloc_n<float>((size_t)m * ncol);
-        float *tau = ar.alloc_n<float>((size_t)d);
+        // whatever the preconditioner.  The robust path sketches in FP64: the
+        // FP64 sketch refines the sketch-and-solve start and serves the
+        // fallback without a second transform.  The fast path (no start)
+        // sketches in FP32.
+        float *W = nullptr;
+        double t0 = wall();
         if (cpqr) {
             W64 = ar.alloc_n<double>((size_t)m * ncol);
-            double t0 = wall();
             dense_sketch<double>(n, d, m, dA, lda, db, dp, W64, ar);
-            sync();
-            double t1 = wall();
-            copy2d<double, float>(m, ncol, W64, m, W, m);
-            dense_factor<float>(m, ncol, d, false, W, jpvt, tau, ar);
-            sync();
-            info.t_sketch += t1 - t0;
-            info.t_qr += wall() - t1;
         } else {
-            dense_sketch_qr<float>(n, d, m, dA, lda, nullptr, dp, false, W, jpvt, tau,
-                                   &info.t_sketch, &info.t_qr, ar);
+            W = ar.alloc_n<float>((size_t)m * d);
+            dense_sketch<float>(n, d, m, dA, lda, nullptr, dp, W, ar);
         }
-        finish_qr(W);
-        cond_est = scaled_cond(rank, Rd, rank, Mi, rank, ar);
-        if (rank < std::min(m, d) || !(cond_est <= o.escalate_cond)) {
+        sync();
+        double t1 = wall();
+        info.t_sketch += t1 - t0;
+        Rd = ar.alloc_n<double>((size_t)d * d);
+        std::vector<double> dg;
+        double maxn = 0.0;
+        const bool ok = cpqr ? mixed_cholqr<double>(m, d, W64, m, Rd, Mi, dg, maxn, ar)
+                             : mixed_cholqr<float>(m, d, W, m, Rd, Mi, dg, maxn, ar);
+        info.t_qr += wall() - t1;
+        rank = d;
+        if (ok && cpqr) {
+            rank = 0;
+            while (rank < d && std::fabs(dg[(size_t)rank]) > o.rcond) ++rank;
+        }
+        cond_est = ok ? scaled_cond(d, Rd, d, Mi, d, ar) : INFINITY;
+        hipLaunchKernelGGL(k_iota, dim3(gblocks(d)), dim3(256), 0, s, d, jpvt);
+        if (!ok || rank < d || !(cond_est <= o.escalate_cond)) {
             escalated = true;
             double *taud = ar.alloc_n<double>((size_t)d);
             if (W64) {
-                double t1 = wall();
+                double t2 = wall();
                 dense_factor<double>(m, ncol, d, cpqr, W64, jpvt, taud, ar);
                 sync();
-                info.t_qr += wall() - t1;
+                info.t_qr += wall() - t2;
                 finish_qr(W64);
                 W64 = nullptr; // factored in place
             } else {
@@ -412,9 +495,14 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     }
     // x0 = P M c ; rs = b - A x0
     double *c1 = ar.alloc_n<double>((size_t)rank);
-    trmv_n(rank, Mi, rank, cvec, c1, ar);
-    // FP32 factor (unpivoted, full rank): refine the start against the FP64 sketch
-    if (W64) refine_sketched_solution(m, d, W64, m, W64 + d * m, Mi, rank, c1, 3, ar);
+    if (W64) {
+        // mixed, no fallback: the sketch-and-solve start from the FP64 sketch
+        // and the optimistic factor, refined from zero
+        hipLaunchKernelGGL(k_fill_d, dim3(gblocks(rank)), dim3(256), 0, s, rank, c1, 0.0);
+        refine_sketched_solution(m, d, W64, m, W64 + d * m, Mi, rank, c1, 4, ar);
+    } else {
+        trmv_n(rank, Mi, rank, cvec, c1, ar);
+    }
     double *x0 = ar.alloc_n<double>((size_t)d);
     hipLaunchKernelGGL(k_fill_d, dim3(gblocks(d)), dim3(256), 0, s, d, x0, 0.0);
     hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank, jpvt, c1, x0);
@@ -472,16 +560,13 @@ static void sparse_sketch_full(long n, long d, long m, long k, const DevPattern 
 }
 
 // Sketch [SA | S b] (or copy it from src, an FP64 sketch made before) and
-// factor it.  refactor = true follows the reference: natural-order
-// Householder QR, and when any |R_jj| falls to SPQR's default tolerance
-// 20 (m + d) eps max_j ||SA_j|| a column-pivoted refactorisation decides the
-// rank.  refactor = false (the optimistic FP32 pass of the mixed mode) stops
-// after the Householder QR and returns the leading count of |R_jj| above
-// max(m, d) eps max_j ||SA_j||; the caller escalates when it falls short of d.
+// factor it as the reference does: natural-order Householder QR, and when any
+// |R_jj| falls to SPQR's default tolerance 20 (m + d) eps max_j ||SA_j|| a
+// column-pivoted refactorisation decides the rank.
 template <class T>
 static long sparse_factor(long n, long d, long m, long k, const DevPattern &dp, const int32_t *cp,
                           const int32_t *ri, const double *val, long nnz, const double *db, T *W,
-                          T *tau, int32_t *jpvt, double *tol_out, Arena &ar, bool refactor = true,
+                          T *tau, int32_t *jpvt, double *tol_out, Arena &ar,
                           const double *src = nullptr) {
     hipStream_t s = ctx().stream;
     auto sketch = [&] {
@@ -500,7 +585,7 @@ static long sparse_factor(long n, long d, long m, long k, const DevPattern &dp, 
     double maxn = 0;
     for (double v : hc) maxn = std::max(maxn, std::sqrt(v));
     double eps = std::is_same<T, float>::value ? (double)FLT_EPSILON : DBL_EPSILON;
-    double tol = (refactor ? 20.0 * (double)(m + d) : (double)std::max(m, d)) * eps * maxn;
+    double tol = 20.0 * (double)(m + d) * eps * maxn;
     *tol_out = tol;
     qr_householder<T>(m, d + 1, d, W, m, tau, ar);
     long kmax = std::min(m, d);
@@ -517,11 +602,6 @@ static long sparse_factor(long n, long d, long m, long k, const DevPattern &dp, 
         }
     hipLaunchKernelGGL(k_iota, dim3(gblocks(d)), dim3(256), 0, s, d, jpvt);
     if (full) return kmax;
-    if (!refactor) {
-        long lead = 0;
-        while (lead < kmax && std::fabs(dg[(size_t)lead]) > tol) ++lead;
-        return lead;
-    }
     // rank deficient: column-pivoted refactorisation of a fresh sketch (the
     // CPQR of the dense path: dgeqp3 pivoting below 1000 columns, HQRRP above)
     sketch();
@@ -597,24 +677,33 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
         extract(W);
         trinv_upper(rank, Rd, rank, 0.0, Mi, rank, ar);
     } else {
-        // FP64 sketch, FP32 factor of its rounded copy (see dense_impl)
+        // FP64 sketch, optimistic CholeskyQR factor (see dense_impl).  The
+        // FP32 rank test: |R_jj| above max(m, d) u_32 max_j ||SA_j||, the
+        // resolution of an FP32 factor, else the FP64 factorization decides.
         W64 = ar.alloc_n<double>((size_t)m * (d + 1));
         sparse_sketch_full<double>(n, d, m, o.k, dp, cp, ri, dval, nnz, db, W64, ar);
-        float *W = ar.alloc_n<float>((size_t)m * (d + 1));
-        float *tau = ar.alloc_n<float>((size_t)d + 1);
-        const long r32 = sparse_factor<float>(n, d, m, o.k, dp, cp, ri, dval, nnz, db, W, tau,
-                                              jpvt, &tol, ar, false, W64);
-        const bool short_rank = r32 < std::min(m, d);
-        rank = std::max(r32, 1L);
-        extract(W);
-        trinv_upper(rank, Rd, rank, 0.0, Mi, rank, ar);
-        cond_est = scaled_cond(rank, Rd, rank, Mi, rank, ar);
+        Rd = ar.alloc_n<double>((size_t)d * d);
+        std::vector<double> dg;
+        double maxn = 0.0;
+        const bool ok = mixed_cholqr<double>(m, d, W64, m, Rd, Mi, dg, maxn, ar);
+        bool short_rank = !ok || m < d;
+        if (!short_rank) {
+            tol = (double)std::max(m, d) * (double)FLT_EPSILON * maxn;
+            for (long j = 0; j < d; ++j)
+                if (!(std::fabs(dg[(size_t)j]) > tol)) {
+                    short_rank = true;
+                    break;
+                }
+        }
+        rank = d;
+        hipLaunchKernelGGL(k_iota, dim3(gblocks(d)), dim3(256), 0, s, d, jpvt);
+        cond_est = short_rank ? INFINITY : scaled_cond(d, Rd, d, Mi, d, ar);
         if (short_rank || !(cond_est <= o.escalate_cond)) {
             escalated = true;
             double *Wd = ar.alloc_n<double>((size_t)m * (d + 1));
             double *taud = ar.alloc_n<double>((size_t)d + 1);
             rank = sparse_factor<double>(n, d, m, o.k, dp, cp, ri, dval, nnz, db, Wd, taud, jpvt,
-                                         &tol, ar, true, W64);
+                                         &tol, ar, W64);
             if (rank == 0) throw Error(SKL_E_RANK0, "SPQR Factorization fails");
             extract(Wd);
             trinv_upper(rank, Rd, rank, 0.0, Mi, rank, ar);
@@ -639,9 +728,13 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
     info.t_qr = T2 - tq0;
     // x0 = P M c (no perturbation, ls_sparse_spqr.cpp:144)
     double *c1 = ar.alloc_n<double>((size_t)rank);
-    trmv_n(rank, Mi, rank, cvec, c1, ar);
-    // FP32 factor (unpivoted, rank d): refine the start against the FP64 sketch
-    if (W64 && rank == d) refine_sketched_solution(m, d, W64, m, W64 + d * m, Mi, rank, c1, 3, ar);
+    if (W64) {
+        // mixed, no fallback: the start from the FP64 sketch, refined from zero
+        hipLaunchKernelGGL(k_fill_d, dim3(gblocks(rank)), dim3(256), 0, s, rank, c1, 0.0);
+        refine_sketched_solution(m, d, W64, m, W64 + d * m, Mi, rank, c1, 4, ar);
+    } else {
+        trmv_n(rank, Mi, rank, cvec, c1, ar);
+    }
     double *x0 = ar.alloc_n<double>((size_t)d);
     hipLaunchKernelGGL(k_fill_d, dim3(gblocks(d)), dim3(256), 0, s, d, x0, 0.0);
     hipLaunchKernelGGL(k_scatter_E, dim3(gblocks(rank)), dim3(256), 0, s, d, rank, jpvt, c1, x0);
