@@ -34,13 +34,15 @@ extern "C" {
 
 /* precision modes */
 #define SKL_FP64 0  /* sketch, transform, QR and LSQR all in FP64 */
-#define SKL_MIXED 1 /* QR of the sketch in FP32 (FP64 QR fallback when the
-                       FP32 factor loses rank or the inf-norm condition
-                       number of its column-equilibrated R exceeds
-                       escalate_cond); LSQR in FP64.  The solvers with a
-                       sketch-and-solve start sketch in FP64 and refine the
-                       start against that sketch; the _nocpqr solver sketches
-                       in FP32. */
+#define SKL_MIXED 1 /* preconditioner from a CholeskyQR of the column-
+                       equilibrated sketch (FP32 MFMA Gram, FP64 Cholesky);
+                       LSQR in FP64.  Falls back to the FP64 factorization of
+                       the same sketch when the Cholesky breaks down, the rank
+                       falls short or the inf-norm condition number of the
+                       column-equilibrated R exceeds escalate_cond.  The
+                       solvers with a sketch-and-solve start sketch in FP64
+                       and refine the start against that sketch; the _nocpqr
+                       solver sketches in FP32. */
 
 typedef struct skl_opts {
     double gamma;           /* over-sampling ratio, m = ceil(gamma d)          */

@@ -13,6 +13,9 @@
 //   against SPQR's default tolerance (column-pivoted refactorisation when it
 //   fails), exact inf-norm condition number of R11, LSQR with the pivot
 //   perturbation when R11 is ill conditioned.
+// Mixed precision (SKL_MIXED): the factorization above is replaced by a
+//   CholeskyQR of the column-equilibrated sketch (mixed_cholqr), with a
+//   fall-back to it when that factor loses rank or conditioning.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
