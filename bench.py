@@ -188,6 +188,7 @@ def run_dense(args, skl, torch, W):
         r = b - At.t() @ x
         res[name] = {"sec_per_solve": (t1 - t0) / args.steps, "lat": lat, "it": info.it,
                      "flag": info.flag, "rank": info.rank, "escalated": info.escalated,
+                     "precond_eig": [info.precond_lmin, info.precond_lmax],
                      "lsqr_est": info.lsqr_nrm_ar / (info.lsqr_nrm_a * info.lsqr_nrm_r),
                      "r_norm": r.norm().item(),
                      "phases": {k: getattr(info, k) for k in
@@ -245,6 +246,7 @@ def run_sparse(args, skl, torch, W):
         info = infos[-1]
         res[name] = {"sec_per_solve": (t1 - t0) / args.steps, "lat": lat, "it": info.it,
                      "flag": info.flag, "rank": info.rank, "escalated": info.escalated,
+                     "precond_eig": [info.precond_lmin, info.precond_lmax],
                      "lsqr_est": info.lsqr_nrm_ar / (info.lsqr_nrm_a * info.lsqr_nrm_r),
                      "phases": {k: getattr(info, k) for k in
                                 ["t_pattern", "t_sketch", "t_qr", "t_x0", "t_lsqr", "t_total"]},
@@ -588,6 +590,7 @@ def main():
         "phases_s": f64["phases"],
         "mixed": {"sec_per_solve": mixed["sec_per_solve"], "lsqr_iterations": mixed["it"],
                   "escalated": mixed["escalated"], "lsqr_estimate": mixed["lsqr_est"],
+                  "precond_eig_estimate": mixed["precond_eig"],
                   "x_rel_vs_qr": mixed["x_rel_vs_qr"],
                   "speedup_vs_fp64": f64["sec_per_solve"] / mixed["sec_per_solve"],
                   "x_rel_vs_fp64": res["x_rel_mixed_vs_fp64"], "phases_s": mixed["phases"]},

@@ -77,6 +77,9 @@ typedef struct skl_info {
     double lsqr_nrm_ar, lsqr_nrm_a, lsqr_nrm_r; /* LSQR final estimates */
     double t_pattern, t_sketch, t_qr, t_x0, t_lsqr, t_total; /* seconds */
     long m;             /* sketch rows                                  */
+    double precond_lmin, precond_lmax; /* MIXED: Lanczos estimate of the extreme
+                           eigenvalues of (SA M)^T (SA M), when the FP32 factor's
+                           pivots called for it (0 when not estimated)  */
 } skl_info;
 
 /* LSQR per-iteration scalars, for step-by-step comparison */

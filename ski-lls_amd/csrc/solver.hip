@@ -269,6 +269,125 @@ __global__ void k_unequilibrate(long d, const double *R1, const double *M1, cons
     }
 }
 
+// One Lanczos step on H = (SA M)^T (SA M), given h = H v_j:
+// alpha_j = v_j^T h, h -= alpha_j v_j + beta_j v_{j-1}, beta_{j+1} = ||h||,
+// v_{j+1} = h / beta_{j+1}.  One workgroup; ab[2j] = alpha_j,
+// ab[2j+1] = beta_{j+1} (ab[-1], the beta_0 of the first step, is not read).
+__global__ __launch_bounds__(1024) void k_lanczos_step(long d, int j, const double *v,
+                                                        const double *vp, double *h, double *vn,
+                                                        double *ab) {
+    double a = 0.0;
+    for (long i = threadIdx.x; i < d; i += 1024) a += v[i] * h[i];
+    a = block_sum<1024>(a);
+    const double b = j > 0 ? ab[2 * j - 1] : 0.0;
+    double q = 0.0;
+    for (long i = threadIdx.x; i < d; i += 1024) {
+        const double t = h[i] - a * v[i] - (j > 0 ? b * vp[i] : 0.0);
+        h[i] = t;
+        q += t * t;
+    }
+    q = sqrt(block_sum<1024>(q));
+    const double inv = q > 0.0 ? 1.0 / q : 0.0;
+    for (long i = threadIdx.x; i < d; i += 1024) vn[i] = h[i] * inv;
+    if (threadIdx.x == 0) {
+        ab[2 * j] = a;
+        ab[2 * j + 1] = q;
+    }
+}
+
+// Extreme eigenvalues of the symmetric tridiagonal with diagonal a and
+// off-diagonal b (b[i] couples i - 1 and i), by bisection on the Sturm count.
+static void tridiag_extremes(const std::vector<double> &a, const std::vector<double> &b,
+                             double &lmin, double &lmax) {
+    const size_t k = a.size();
+    double lo = INFINITY, hi = -INFINITY;
+    for (size_t i = 0; i < k; ++i) {
+        const double r = (i > 0 ? std::fabs(b[i]) : 0.0) + (i + 1 < k ? std::fabs(b[i + 1]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    auto below = [&](double x) { // eigenvalues < x
+        size_t c = 0;
+        double q = 1.0;
+        for (size_t i = 0; i < k; ++i) {
+            q = a[i] - x - (i > 0 ? b[i] * b[i] / q : 0.0);
+            if (q == 0.0) q = -DBL_MIN;
+            if (q < 0.0) ++c;
+        }
+        return c;
+    };
+    auto bisect = [&](size_t target) { // smallest x with `target` eigenvalues below
+        double l = lo, h = hi;
+        for (int it = 0; it < 200 && h - l > 4 * DBL_EPSILON * std::max(std::fabs(l), std::fabs(h));
+             ++it) {
+            const double mid = 0.5 * (l + h);
+            if (below(mid) >= target)
+                h = mid;
+            else
+                l = mid;
+        }
+        return 0.5 * (l + h);
+    };
+    lmin = bisect(1);
+    lmax = bisect(k);
+}
+
+// Lanczos estimate of the extreme eigenvalues of (SA M)^T (SA M) from a fixed
+// pseudo-random start (`steps` products with SA and M, one read-back).  For
+// an exact R both are 1.  CholeskyQR from an FP32 Gram matrix cannot resolve
+// a column of SA that depends on the others: its pivot stops at about
+// sqrt(u_32) instead of 0 and the factor looks full rank, but SA M then has a
+// singular value near 0, which the Ritz values find within a few steps when
+// the rest of the spectrum clusters near 1.
+static void precond_spectrum(long m, long d, const double *SA, long ldsa, const double *M,
+                             long ldm, int steps, double &lmin, double &lmax, Arena &ar) {
+    hipStream_t s = ctx().stream;
+    const size_t mark = ar.used();
+    const int K = (int)std::min<long>(steps, d);
+    double *V = ar.alloc_n<double>((size_t)d * (K + 1));
+    double *w = ar.alloc_n<double>((size_t)d);
+    double *t = ar.alloc_n<double>((size_t)m);
+    double *z = ar.alloc_n<double>((size_t)d);
+    double *h = ar.alloc_n<double>((size_t)d);
+    double *ab = ar.alloc_n<double>((size_t)2 * K);
+    std::vector<double> v0((size_t)d);
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    double nn = 0.0;
+    for (long i = 0; i < d; ++i) { // splitmix64, uniform in (-1, 1)
+        uint64_t x = (st += 0x9E3779B97F4A7C15ull);
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        x ^= x >> 31;
+        v0[(size_t)i] = ((double)(x >> 11) + 0.5) * 0x1.0p-52 - 1.0;
+        nn += v0[(size_t)i] * v0[(size_t)i];
+    }
+    for (double &x : v0) x /= std::sqrt(nn);
+    SKL_HIP(hipMemcpyAsync(V, v0.data(), sizeof(double) * d, hipMemcpyHostToDevice, s));
+    for (int j = 0; j < K; ++j) {
+        const double *vj = V + (size_t)j * d;
+        dev_gemv_n_dense(d, d, M, ldm, vj, 1.0, 0.0, w, ar);
+        dev_gemv_n_dense(m, d, SA, ldsa, w, 1.0, 0.0, t, ar);
+        dev_gemv_t_dense(m, d, SA, ldsa, t, 1.0, 0.0, z, ar);
+        dev_gemv_t_dense(d, d, M, ldm, z, 1.0, 0.0, h, ar);
+        hipLaunchKernelGGL(k_lanczos_step, dim3(1), dim3(1024), 0, s, d, j, vj,
+                           j > 0 ? vj - d : vj, h, V + (size_t)(j + 1) * d, ab);
+    }
+    SKL_CHECK_LAUNCH();
+    std::vector<double> hab((size_t)2 * K);
+    SKL_HIP(hipMemcpyAsync(hab.data(), ab, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, s));
+    SKL_HIP(hipStreamSynchronize(s));
+    ar.rewind(mark);
+    // T_k: alpha on the diagonal, beta_1 .. beta_{k-1} beside it; a beta at
+    // rounding level ends the Krylov space (its Ritz values are exact)
+    std::vector<double> a, b;
+    for (int j = 0; j < K; ++j) {
+        a.push_back(hab[(size_t)2 * j]);
+        b.push_back(j > 0 ? hab[(size_t)2 * j - 1] : 0.0);
+        if (!(hab[(size_t)2 * j + 1] > 1e-12 * std::fabs(hab[(size_t)2 * j]))) break;
+    }
+    tridiag_extremes(a, b, lmin, lmax);
+}
+
 // The optimistic factor of the mixed mode.  Only R (for M = R^{-1}) is
 // needed -- the start is refined against the FP64 sketch and LSQR works on
 // A M -- so any QR of SA serves, and CholeskyQR needs no per-column
@@ -282,7 +401,7 @@ __global__ void k_unequilibrate(long d, const double *R1, const double *M1, cons
 // largest column norm of SA.
 template <class T>
 static bool mixed_cholqr(long m, long d, const T *SA, long ldsa, double *Rd, double *Mi,
-                         std::vector<double> &dg, double &maxn, Arena &ar) {
+                         std::vector<double> &dg, double &maxn, skl_info &info, Arena &ar) {
     hipStream_t s = ctx().stream;
     const size_t mark = ar.used();
     double *sq = ar.alloc_n<double>((size_t)d);
@@ -298,7 +417,7 @@ static bool mixed_cholqr(long m, long d, const T *SA, long ldsa, double *Rd, dou
     double *G64 = ar.alloc_n<double>((size_t)d * d);
     copy2d<float, double>(d, d, G, d, G64, d);
     double *R1 = ar.alloc_n<double>((size_t)d * d);
-    const bool ok = cholesky_upper(d, G64, d, R1, d, ar);
+    bool ok = cholesky_upper(d, G64, d, R1, d, ar);
     std::vector<double> hD((size_t)d);
     SKL_HIP(hipMemcpyAsync(hD.data(), D, sizeof(double) * d, hipMemcpyDeviceToHost, s));
     if (ok) {
@@ -314,6 +433,28 @@ static bool mixed_cholqr(long m, long d, const T *SA, long ldsa, double *Rd, dou
     SKL_HIP(hipStreamSynchronize(s));
     maxn = 0.0;
     for (double v : hD) maxn = std::max(maxn, v);
+    // Pivots of the equilibrated factor below sqrt(max(m, d) u_32) are where
+    // a dependent column and a merely ill-conditioned one look alike; then
+    // the Lanczos estimate of SA M decides (an eigenvalue of (SA M)^T SA M
+    // below 0.1: a dependent column, or an FP32 factor too poor to keep).
+    bool probe = false;
+    const double amb = std::sqrt((double)std::max(m, d) * (double)FLT_EPSILON);
+    for (long j = 0; ok && j < d; ++j)
+        if (!(std::fabs(dg[(size_t)j]) > amb * hD[(size_t)j])) probe = true;
+    if (probe) {
+        const double *S64;
+        long ld64 = ldsa;
+        if constexpr (std::is_same<T, double>::value) {
+            S64 = SA;
+        } else {
+            double *tmp = ar.alloc_n<double>((size_t)m * d);
+            copy2d<T, double>(m, d, SA, ldsa, tmp, m);
+            S64 = tmp;
+            ld64 = m;
+        }
+        precond_spectrum(m, d, S64, ld64, Mi, d, 16, info.precond_lmin, info.precond_lmax, ar);
+        if (!(info.precond_lmin > 0.1)) ok = false;
+    }
     ar.rewind(mark);
     return ok;
 }
@@ -433,8 +574,8 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
         Rd = ar.alloc_n<double>((size_t)d * d);
         std::vector<double> dg;
         double maxn = 0.0;
-        const bool ok = cpqr ? mixed_cholqr<double>(m, d, W64, m, Rd, Mi, dg, maxn, ar)
-                             : mixed_cholqr<float>(m, d, W, m, Rd, Mi, dg, maxn, ar);
+        const bool ok = cpqr ? mixed_cholqr<double>(m, d, W64, m, Rd, Mi, dg, maxn, info, ar)
+                             : mixed_cholqr<float>(m, d, W, m, Rd, Mi, dg, maxn, info, ar);
         info.t_qr += wall() - t1;
         rank = d;
         if (ok && cpqr) {
@@ -688,7 +829,7 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
         Rd = ar.alloc_n<double>((size_t)d * d);
         std::vector<double> dg;
         double maxn = 0.0;
-        const bool ok = mixed_cholqr<double>(m, d, W64, m, Rd, Mi, dg, maxn, ar);
+        const bool ok = mixed_cholqr<double>(m, d, W64, m, Rd, Mi, dg, maxn, info, ar);
         bool short_rank = !ok || m < d;
         if (!short_rank) {
             tol = (double)std::max(m, d) * (double)FLT_EPSILON * maxn;

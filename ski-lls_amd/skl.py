@@ -49,7 +49,7 @@ class Info(C.Structure):
                 ("lsqr_nrm_a", C.c_double), ("lsqr_nrm_r", C.c_double),
                 ("t_pattern", C.c_double), ("t_sketch", C.c_double), ("t_qr", C.c_double),
                 ("t_x0", C.c_double), ("t_lsqr", C.c_double), ("t_total", C.c_double),
-                ("m", C.c_long)]
+                ("m", C.c_long), ("precond_lmin", C.c_double), ("precond_lmax", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

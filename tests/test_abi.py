@@ -2,6 +2,7 @@
 include/skl.h declares plus the three ski-lls.h entry points, the host-side
 pattern generator is bit-exact with the oracle, and the C++ drop-in header
 compiles against a caller written like the reference's examples."""
+import ctypes
 import os
 import re
 import subprocess
@@ -110,3 +111,27 @@ int main() {
     subprocess.check_call(["g++", "-std=c++11", "-O1", "-I", os.path.join(ROOT, "include"),
                            str(src), "-o", str(exe), LIB, "-Wl,-rpath," + os.path.dirname(LIB)])
     assert exe.exists()
+
+
+def test_ctypes_structs_match_c_layout(skl, tmp_path):
+    """Every field of the ctypes mirrors sits at the offset gcc gives it in
+    include/skl.h, and the structs have the same size."""
+    structs = {"skl_opts": skl.Opts, "skl_info": skl.Info, "skl_lsqr_step": skl.LsqrStep}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "skl.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    got = {}
+    for line in subprocess.check_output([str(exe)], text=True).splitlines():
+        cname, f, v = line.split()
+        got[(cname, f)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)

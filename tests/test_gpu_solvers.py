@@ -154,6 +154,86 @@ def test_mixed_precision_dense(skl):
     assert i32.lsqr_nrm_ar < 1e-10 * i32.lsqr_nrm_a * i32.lsqr_nrm_r
 
 
+def test_mixed_precision_dense_iterations_match_fp64(skl):
+    """The mixed factor (CholeskyQR of the equilibrated sketch) preconditions
+    as well as the FP64 CPQR, and the start refined against the FP64 sketch
+    leaves LSQR the same work: iteration counts within 2."""
+    A, b = c1_problem(17, n=8000, d=400)
+    x64, i64 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=2)
+    x32, i32 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=2,
+                               precision=skl.SKL_MIXED)
+    assert i32.escalated == 0 and i32.flag == 0 and i32.rank == 400
+    assert abs(i32.it - i64.it) <= 2, (i32.it, i64.it)
+    assert rel(x32, x64) < 1e-8
+
+
+def test_mixed_precision_dense_rank_deficient_falls_back(skl):
+    """Duplicated columns make the Gram singular: the Cholesky breaks down and
+    the solve runs the FP64 CPQR of the same FP64 sketch, which the FP64 mode
+    also runs -- the same x, bit for bit."""
+    A, b = c1_problem(18, n=3000, d=120)
+    A = np.asfortranarray(np.hstack([A[:, :100], A[:, :20]]))
+    x64, i64 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=4)
+    x32, i32 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=4,
+                               precision=skl.SKL_MIXED)
+    assert i32.escalated == 1 and i32.rank == i64.rank and i64.rank < 120
+    assert np.array_equal(x32, x64)
+
+
+def correlated_problem(seed, n=20000, d=400, decades=3, c=0.5, dep=None):
+    """A = G diag(logspace(0,-decades)) U with U unit upper triangular: the
+    column-equilibrated sketch has condition number about 2e3 for the
+    defaults.  dep = (j, a, b): column j becomes 1e-3 A_a + A_b."""
+    rng = np.random.default_rng(seed)
+    U = np.eye(d) + c * np.triu(rng.standard_normal((d, d)) / np.sqrt(d), 1)
+    A = (rng.standard_normal((n, d)) * np.logspace(0, -decades, d)) @ U
+    if dep is not None:
+        j, a, bb = dep
+        A[:, j] = 1e-3 * A[:, a] + A[:, bb]
+    b = A @ rng.standard_normal(d) + 1e-8 * rng.standard_normal(n)
+    return np.asfortranarray(A), b
+
+
+def test_mixed_precision_probe_keeps_ill_conditioned_factor(skl):
+    """The FP32 factor's smallest equilibrated pivot (about 5e-3) is below
+    sqrt(m u_32), where a dependent column would also land, so the Lanczos
+    estimate of SA M runs; it finds SA M well conditioned and the FP32
+    factor is kept."""
+    A, b = correlated_problem(21)
+    x64, i64 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=3)
+    x32, i32 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=3,
+                               precision=skl.SKL_MIXED)
+    assert i64.precond_lmin == 0.0 and i64.precond_lmax == 0.0
+    assert i32.escalated == 0 and i32.rank == 400, (i32.escalated, i32.cond_est)
+    assert 0.1 < i32.precond_lmin <= i32.precond_lmax < 10.0, (i32.precond_lmin, i32.precond_lmax)
+    assert abs(i32.it - i64.it) <= 3, (i32.it, i64.it)
+    assert rel(x32, x64) < 1e-8
+
+
+def test_mixed_precision_dependent_column_falls_back(skl):
+    """Column 200 = 1e-3 A_3 + A_7: an FP32 Gram can leave this dependence a
+    pivot near sqrt(u_32) instead of 0 (or break down).  Either way the solve
+    falls back to the FP64 CPQR of the same FP64 sketch and returns its rank
+    and x, bit for bit."""
+    A, b = correlated_problem(22, dep=(200, 3, 7))
+    x64, i64 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=5)
+    x32, i32 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=5,
+                               precision=skl.SKL_MIXED)
+    assert i64.rank == 399
+    assert i32.escalated == 1 and i32.rank == i64.rank
+    assert np.array_equal(x32, x64)
+
+
+def test_mixed_precision_nocpqr(skl):
+    A, b = c1_problem(19, n=4000, d=200)
+    x64, i64 = skl.dense_solve(A, b, cpqr=False, gamma=1.7, k=1, it_tol=1e-10, seed=6)
+    x32, i32 = skl.dense_solve(A, b, cpqr=False, gamma=1.7, k=1, it_tol=1e-10, seed=6,
+                               precision=skl.SKL_MIXED)
+    assert i32.escalated == 0 and i32.flag == 0
+    assert abs(i32.it - i64.it) <= 3, (i32.it, i64.it)
+    assert rel(x32, x64) < 1e-8
+
+
 @pytest.mark.parametrize("unit_diag", ["scaled", False])
 def test_mixed_precision_escalates_on_ill_conditioned_sparse(skl, unit_diag):
     """Columns scaled by logspace(0,-10): column norms spread over 1e10, so the
