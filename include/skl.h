@@ -38,8 +38,11 @@ extern "C" {
                        equilibrated sketch (FP32 MFMA Gram, FP64 Cholesky);
                        LSQR in FP64.  Falls back to the FP64 factorization of
                        the same sketch when the Cholesky breaks down, the rank
-                       falls short or the inf-norm condition number of the
-                       column-equilibrated R exceeds escalate_cond.  The
+                       falls short, a Lanczos estimate of (SA M)^T (SA M) (run
+                       when an equilibrated pivot is below sqrt(max(m,d) u_32))
+                       has an eigenvalue <= 0.1, or the inf-norm condition
+                       number of the column-equilibrated R exceeds
+                       escalate_cond.  The
                        solvers with a sketch-and-solve start sketch in FP64
                        and refine the start against that sketch; the _nocpqr
                        solver sketches in FP32. */
