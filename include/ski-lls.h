@@ -175,6 +175,11 @@ void ls_dense_hashing_blendenpik_noCPQR(Mat_d &A, Vec_d &b, Vec_d &x, long &rank
                                         long &it, double gamma, long k, double it_tol,
                                         long max_it, int debug, int wisdom);
 
+/* From blendenpik.hpp:31-33 (the reference declares it there, not in
+ * ski-lls.h; include/blendenpik.hpp of this repository includes this file) */
+void ls_dense_blendenpik(Mat_d &A, Vec_d &b, Vec_d &x, long &rank, int &flag, long &it,
+                         double gamma, double it_tol, long max_it, int debug, int wisdom);
+
 /* From lsex_all_in_c.h */
 void ls_sparse_spqr(cs_dl &A, Vec_d &b, Vec_d &x, long &rank, int &flag, long &it, double gamma,
                     long k, double abs_tol = ABS_TOL, int ordering = SPQR_ORDERING,

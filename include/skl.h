@@ -108,6 +108,12 @@ int skl_dense_solve(long n, long d, const double *A, long lda, const double *b, 
 /* ls_dense_hashing_blendenpik_noCPQR (src/ls_dense_hashing_blendenpik_noCPQR.cpp:23-60) */
 int skl_dense_solve_nocpqr(long n, long d, const double *A, long lda, const double *b,
                            double *x, const skl_opts *opts, skl_info *info);
+/* ls_dense_blendenpik (src/ls_dense_blendenpik.cpp:21-71, declared in
+ * include/blendenpik.hpp:31-33): the randomized DHT, uniform sampling of
+ * m = ceil(gamma d) of its rows (src/dense_helpers.cpp:19-59), Householder QR
+ * and LSQR from zero; opts.k, abs_tol and rcond are not used. */
+int skl_dense_solve_blendenpik(long n, long d, const double *A, long lda, const double *b,
+                               double *x, const skl_opts *opts, skl_info *info);
 /* ls_sparse_spqr (src/ls_sparse_spqr.cpp:79-214); A in CSC (cs_dl layout:
  * colptr d+1, rowind/val nnz) */
 int skl_sparse_solve(long n, long d, long nnz, const long *colptr, const long *rowind,
@@ -119,6 +125,8 @@ int skl_dense_solve_dev(long n, long d, const double *dA, long lda, const double
                         double *dx, const skl_opts *opts, skl_info *info);
 int skl_dense_solve_nocpqr_dev(long n, long d, const double *dA, long lda, const double *db,
                                double *dx, const skl_opts *opts, skl_info *info);
+int skl_dense_solve_blendenpik_dev(long n, long d, const double *dA, long lda, const double *db,
+                                   double *dx, const skl_opts *opts, skl_info *info);
 int skl_sparse_solve_dev(long n, long d, long nnz, const long *dcolptr, const long *drowind,
                          const double *dval, const double *db, double *dx,
                          const skl_opts *opts, skl_info *info);
@@ -161,6 +169,10 @@ void skl_server_destroy(skl_server *s);
  * them), then n*k row draws and n*k sign draws (src/sparse_helpers.cpp:90-116). */
 int skl_hash_pattern(unsigned seed, int dense, long m, long n, long k, double *D, long *rows,
                      double *vals);
+/* Blendenpik's sampling: the n signs D (src/dense_helpers.cpp:7-16), then,
+ * from the same seed again (src/dense_helpers.cpp:25), the m sampled rows idx
+ * of a partial Fisher-Yates shuffle of 0..n-1 (src/sparse_helpers.cpp:74-87) */
+int skl_subsample_pattern(unsigned seed, long m, long n, double *D, long *idx);
 /* Y = DHT(X) per column, unnormalised FFTW_DHT (src/fftw_r2r.cpp:17-54);
  * precision SKL_FP64 or SKL_MIXED (FP32 arithmetic, FP64 in/out) */
 int skl_dht(long n, long ncols, const double *X, long ldx, double *Y, long ldy, int precision);

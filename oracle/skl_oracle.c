@@ -82,6 +82,19 @@ void orc_gen_hashing(orc_rng *g, long m, long n, long k, long *rows, long *colpt
             vals[i * k + t] = 1 / scaling * (double)((orc_rand(g) % 2) * 2 - 1);
 }
 
+void orc_select_k_from_n(orc_rng *g, long n, long k, long *idx) {
+    long *arr = (long *)malloc((size_t)(n > 0 ? n : 1) * sizeof(long));
+    for (long i = 0; i < n; ++i) arr[i] = i; /* gen_0_to_n, sparse_helpers.cpp:66-71 */
+    for (long i = 0; i < k; ++i) {
+        long j = ((long)orc_rand(g) % (n - i)) + i;
+        long tmp = arr[j];
+        arr[j] = arr[i];
+        arr[i] = tmp;
+    }
+    for (long i = 0; i < k; ++i) idx[i] = arr[i];
+    free(arr);
+}
+
 /* NoFLA_Normal_random_number (skint.c:603-624): Marsaglia polar method on
  * rand()/RAND_MAX with an alternate-call cache. */
 static double orc_normal(orc_rng *g) {
@@ -1088,6 +1101,51 @@ int orc_ls_dense_nocpqr(long n, long d, const double *A, long lda, const double 
     info->t_qr = t2 - t1;
     info->t_lsqr = t3 - t2;
     info->t_total = t3 - t0;
+    free(SA);
+    free(tau);
+    return ORC_OK;
+}
+
+int orc_ls_dense_blendenpik(long n, long d, const double *A, long lda, const double *b,
+                            double *x, double gamma, double it_tol, long max_it,
+                            unsigned seed, orc_lsqr_step *hist, orc_info *info) {
+    orc_info loc;
+    if (!info) info = &loc;
+    memset(info, 0, sizeof *info);
+    if (n < d) return ORC_E_SHAPE;
+    double t0 = now_sec();
+    long m = ceil_gamma(d, gamma);
+    if (m > n) return ORC_E_SHAPE; /* select_k_from_n asserts n >= k */
+    /* gen_scaled_rand_signs seeds from the clock, and random_sample seeds
+     * again from the same clock (dense_helpers.cpp:12,25) */
+    orc_rng g;
+    orc_srand(&g, seed);
+    double *D = (double *)malloc((size_t)n * sizeof(double));
+    orc_gen_signs(&g, n, 1.0, D);
+    double *HDA = (double *)malloc((size_t)n * d * sizeof(double));
+    orc_fut(n, d, A, lda, D, HDA, n);
+    long *idx = (long *)malloc((size_t)m * sizeof(long));
+    orc_srand(&g, seed);
+    orc_select_k_from_n(&g, n, m, idx);
+    double *SA = (double *)malloc((size_t)m * d * sizeof(double));
+    for (long j = 0; j < d; ++j)
+        for (long i = 0; i < m; ++i) SA[i + j * m] = HDA[idx[i] + j * n];
+    double t1 = now_sec();
+    double *tau = (double *)calloc((size_t)d, sizeof(double));
+    orc_geqrf(m, d, SA, m, tau); /* dense_qr */
+    double t2 = now_sec();
+    info->rank = d;
+    orc_lsqr_dense(n, d, A, lda, NULL, SA, m, b, it_tol, max_it, x, &info->flag, &info->it,
+                   hist, info);
+    orc_trsv_un(d, SA, m, x, 0.0);
+    double t3 = now_sec();
+    info->t_sketch = t1 - t0;
+    info->t_qr = t2 - t1;
+    info->t_lsqr = t3 - t2;
+    info->t_total = t3 - t0;
+    free(D);
+    free(HDA);
+    free(idx);
     free(SA);
     free(tau);
     return ORC_OK;

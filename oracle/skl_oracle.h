@@ -34,6 +34,9 @@ void orc_gen_signs(orc_rng *g, long n, double scale, double *D);
 /* sparse_helpers.cpp:74-116 (select_k_from_n + gen_hashing_matrix) */
 void orc_gen_hashing(orc_rng *g, long m, long n, long k, long *rows, long *colptr,
                      double *vals);
+/* random_sample (dense_helpers.cpp:19-39): idx (k entries) = the first k
+ * entries of select_k_from_n (sparse_helpers.cpp:74-87) on 0..n-1 */
+void orc_select_k_from_n(orc_rng *g, long n, long k, long *idx);
 /* NoFLA_HQRRP_WY_blk_var4_skint.c:585-624 */
 void orc_gaussian_matrix(orc_rng *g, long rows, long cols, double *G, long ldg);
 
@@ -118,6 +121,11 @@ int orc_ls_dense(long n, long d, double *A, long lda, const double *b, double *x
 int orc_ls_dense_nocpqr(long n, long d, const double *A, long lda, const double *b,
                         double *x, double gamma, long k, double it_tol, long max_it,
                         unsigned seed, orc_lsqr_step *hist, orc_info *info);
+/* ls_dense_blendenpik.cpp:21-71 with rand_subsampled_dct (dense_helpers.cpp:42-59):
+ * signs from seed, rows sampled from the same seed again, dgeqrf, LSQR from 0 */
+int orc_ls_dense_blendenpik(long n, long d, const double *A, long lda, const double *b,
+                            double *x, double gamma, double it_tol, long max_it,
+                            unsigned seed, orc_lsqr_step *hist, orc_info *info);
 /* ls_sparse_spqr.cpp:79-214, SPQR replaced by the dense QR of SA described in
  * orc_sparse_precond */
 int orc_ls_sparse(long n, long d, const long *Ap, const long *Ai, const double *Ax,

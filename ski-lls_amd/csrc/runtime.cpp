@@ -344,6 +344,29 @@ void build_buckets(Pattern &p) {
     }
 }
 
+// Uniform row sampling of Blendenpik (src/dense_helpers.cpp:19-39): a partial
+// Fisher-Yates shuffle of 0..n-1 whose first m entries are the sampled rows
+// (select_k_from_n, src/sparse_helpers.cpp:74-87), as m single-source
+// buckets with coefficient 1: sketch row i is row idx[i] of the transform.
+void gen_subsample(Glibc &g, long n, long m, Pattern &p) {
+    p.m = m;
+    p.n = n;
+    p.k = 1;
+    std::vector<int32_t> arr((size_t)n);
+    for (long i = 0; i < n; ++i) arr[(size_t)i] = (int32_t)i;
+    for (long i = 0; i < m; ++i) {
+        const long j = (long)g.next() % (n - i) + i;
+        std::swap(arr[(size_t)j], arr[(size_t)i]);
+    }
+    p.rows.clear();
+    p.bptr.resize((size_t)m + 1);
+    p.bsrc.assign(arr.begin(), arr.begin() + m);
+    p.bslot.resize((size_t)m);
+    p.vals.assign((size_t)m, 1.0);
+    for (long i = 0; i <= m; ++i) p.bptr[(size_t)i] = (int32_t)i;
+    for (long i = 0; i < m; ++i) p.bslot[(size_t)i] = (int32_t)i;
+}
+
 void gen_gaussian(unsigned seed, long rows, long cols, std::vector<double> &G) {
     Glibc g;
     g.seed(seed);

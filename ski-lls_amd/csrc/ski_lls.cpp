@@ -92,6 +92,30 @@ void ls_dense_hashing_blendenpik_noCPQR(Mat_d &A, Vec_d &b, Vec_d &x, long &rank
     it = info.it;
 }
 
+void ls_dense_blendenpik(Mat_d &A, Vec_d &b, Vec_d &x, long &rank, int &flag, long &it,
+                         double gamma, double it_tol, long max_it, int debug, int wisdom) {
+    assert(A.m() == b.n());
+    assert(A.n() == x.n());
+    if (A.m() < A.n()) throw std::runtime_error("Matrix is not overdetermined.");
+    skl_opts o;
+    skl_default_opts(&o);
+    o.precision = precision_from_env();
+    o.gamma = gamma;
+    o.it_tol = it_tol;
+    o.max_it = max_it;
+    o.debug = debug;
+    o.wisdom = wisdom;
+    o.seed = seed_or((unsigned)std::time(nullptr));
+    std::vector<double> hb = gather(b), hx((size_t)A.n());
+    skl_info info;
+    raise(skl_dense_solve_blendenpik(A.m(), A.n(), A.data(), A.ld(), hb.data(), hx.data(), &o,
+                                     &info));
+    for (long i = 0; i < x.n(); ++i) x(i) = hx[(size_t)i];
+    rank = info.rank;
+    flag = info.flag;
+    it = info.it;
+}
+
 void ls_sparse_spqr(cs_dl &A, Vec_d &b, Vec_d &x, long &rank, int &flag, long &it, double gamma,
                     long k, double abs_tol, int ordering, double it_tol, long max_it,
                     double rcond_threshold, double perturb, int debug) {

@@ -154,6 +154,7 @@ struct Pattern {
 void gen_signs(Glibc &g, long n, std::vector<double> &D);
 void gen_hashing(Glibc &g, long m, long n, long k, Pattern &p);
 void build_buckets(Pattern &p);
+void gen_subsample(Glibc &g, long n, long m, Pattern &p);
 void gen_gaussian(unsigned seed, long rows, long cols, std::vector<double> &G);
 
 // ---------------------------------------------------------------------------

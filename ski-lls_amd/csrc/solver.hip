@@ -108,7 +108,7 @@ static DevPattern upload_pattern(const Pattern &P, double alpha, bool want_D, bo
                                  bool want_float, Arena &ar) {
     hipStream_t s = ctx().stream;
     DevPattern dp;
-    long nk = P.n * P.k;
+    const long nk = (long)P.bsrc.size(); // bucket entries: n k, or m for row sampling
     if (want_D) {
         dp.D = ar.alloc_n<double>((size_t)P.n);
         SKL_HIP(hipMemcpyAsync(dp.D, P.D.data(), sizeof(double) * P.n, hipMemcpyHostToDevice, s));
@@ -130,10 +130,11 @@ static DevPattern upload_pattern(const Pattern &P, double alpha, bool want_D, bo
         SKL_HIP(hipStreamSynchronize(s)); // cf goes out of scope
     }
     if (want_rows) {
-        dp.rows = ar.alloc_n<int32_t>((size_t)std::max<long>(nk, 1));
-        SKL_HIP(hipMemcpyAsync(dp.rows, P.rows.data(), sizeof(int32_t) * nk, hipMemcpyHostToDevice, s));
-        dp.vals = ar.alloc_n<double>((size_t)std::max<long>(nk, 1));
-        SKL_HIP(hipMemcpyAsync(dp.vals, P.vals.data(), sizeof(double) * nk, hipMemcpyHostToDevice, s));
+        const long nr = (long)P.rows.size();
+        dp.rows = ar.alloc_n<int32_t>((size_t)std::max<long>(nr, 1));
+        SKL_HIP(hipMemcpyAsync(dp.rows, P.rows.data(), sizeof(int32_t) * nr, hipMemcpyHostToDevice, s));
+        dp.vals = ar.alloc_n<double>((size_t)std::max<long>(nr, 1));
+        SKL_HIP(hipMemcpyAsync(dp.vals, P.vals.data(), sizeof(double) * nr, hipMemcpyHostToDevice, s));
     }
     SKL_HIP(hipStreamSynchronize(s));
     return dp;
@@ -483,8 +484,12 @@ static void refine_sketched_solution(long m, long d, const double *SA, long ldsa
     ar.rewind(mark);
 }
 
+// cpqr: ls_dense_hashing_blendenpik (sketch-and-solve start, CPQR, rank);
+// otherwise LSQR from zero on the Householder-QR preconditioner, after the
+// hashing sketch (ls_dense_hashing_blendenpik_noCPQR) or, with subsample,
+// after uniform row sampling of the transform (ls_dense_blendenpik).
 static int dense_impl(long n, long d, const double *dA, long lda, const double *db, double *dx,
-                      const skl_opts &o, skl_info &info, bool cpqr) {
+                      const skl_opts &o, skl_info &info, bool cpqr, bool subsample = false) {
     Context &C = ctx();
     Arena &ar = C.arena;
     hipStream_t s = C.stream;
@@ -493,7 +498,8 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     SKL_REQUIRE(d > 0 && n > 0 && lda >= n, SKL_E_ARG, "invalid dense shape");
     SKL_REQUIRE(o.k >= 1 && o.gamma > 0, SKL_E_ARG, "invalid sketch parameters");
     long m = (long)std::ceil((double)d * o.gamma);
-    SKL_REQUIRE(m >= o.k && m >= d, SKL_E_ARG, "sketch must have at least d rows");
+    SKL_REQUIRE((subsample || m >= o.k) && m >= d, SKL_E_ARG, "sketch must have at least d rows");
+    SKL_REQUIRE(!subsample || m <= n, SKL_E_ARG, "row sampling needs at most n sketch rows");
     info.m = m;
     ar.reset();
     ArenaTrim trim(ar);
@@ -503,10 +509,19 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     g.seed(o.seed);
     Pattern P;
     gen_signs(g, n, P.D);
-    gen_hashing(g, m, n, o.k, P);
-    build_buckets(P);
+    if (subsample) {
+        // random_sample seeds again from the same clock (dense_helpers.cpp:25),
+        // so the row draws restart the stream the signs came from
+        Glibc g2;
+        g2.seed(o.seed);
+        gen_subsample(g2, n, m, P);
+    } else {
+        gen_hashing(g, m, n, o.k, P);
+        build_buckets(P);
+    }
     bool mixed = (o.precision == SKL_MIXED);
-    DevPattern dp = upload_pattern(P, 1 / std::sqrt((double)o.k), true, false, mixed, ar);
+    DevPattern dp = upload_pattern(P, subsample ? 1.0 : 1 / std::sqrt((double)o.k), true, false,
+                                   mixed, ar);
     double T1 = wall();
     info.t_pattern = T1 - T0;
     // ---- sketch + QR ----
@@ -977,7 +992,7 @@ struct DevBuf {
 };
 
 static int dense_host(long n, long d, const double *A, long lda, const double *b, double *x,
-                      const skl_opts *opts, skl_info *info, bool cpqr) {
+                      const skl_opts *opts, skl_info *info, bool cpqr, bool subsample = false) {
     skl_info tmp;
     return guarded([&] {
         if (n < d) throw Error(SKL_E_SHAPE, "Matrix is not overdetermined.");
@@ -991,7 +1006,7 @@ static int dense_host(long n, long d, const double *A, long lda, const double *b
         SKL_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
         SKL_HIP(hipStreamSynchronize(s));
         int rc = dense_impl(n, d, dA.as<double>(), n, db.as<double>(), dx.as<double>(), o,
-                            info ? *info : tmp, cpqr);
+                            info ? *info : tmp, cpqr, subsample);
         SKL_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * d, hipMemcpyDeviceToHost, s));
         SKL_HIP(hipStreamSynchronize(s));
         return rc;
@@ -1135,6 +1150,15 @@ int skl_dense_solve_nocpqr_dev(long n, long d, const double *dA, long lda, const
     });
 }
 
+int skl_dense_solve_blendenpik_dev(long n, long d, const double *dA, long lda, const double *db,
+                                   double *dx, const skl_opts *opts, skl_info *info) {
+    skl_info tmp;
+    return guarded([&] {
+        skl_opts o = resolve(opts);
+        return dense_impl(n, d, dA, lda, db, dx, o, info ? *info : tmp, false, true);
+    });
+}
+
 int skl_sparse_solve_dev(long n, long d, long nnz, const long *dcp, const long *dri,
                          const double *dval, const double *db, double *dx, const skl_opts *opts,
                          skl_info *info) {
@@ -1152,6 +1176,10 @@ int skl_dense_solve(long n, long d, const double *A, long lda, const double *b, 
 int skl_dense_solve_nocpqr(long n, long d, const double *A, long lda, const double *b, double *x,
                            const skl_opts *opts, skl_info *info) {
     return dense_host(n, d, A, lda, b, x, opts, info, false);
+}
+int skl_dense_solve_blendenpik(long n, long d, const double *A, long lda, const double *b,
+                               double *x, const skl_opts *opts, skl_info *info) {
+    return dense_host(n, d, A, lda, b, x, opts, info, false, true);
 }
 
 int skl_sparse_solve(long n, long d, long nnz, const long *colptr, const long *rowind,
@@ -1199,6 +1227,23 @@ int skl_hash_pattern(unsigned seed, int dense, long m, long n, long k, double *D
             if (rows) rows[q] = P.rows[(size_t)q];
             if (vals) vals[q] = P.vals[(size_t)q];
         }
+        return SKL_OK;
+    });
+}
+
+int skl_subsample_pattern(unsigned seed, long m, long n, double *D, long *idx) {
+    return guarded([&] {
+        SKL_REQUIRE(m >= 0 && m <= n, SKL_E_ARG, "invalid sampling shape");
+        Glibc g;
+        g.seed(seed);
+        Pattern P;
+        gen_signs(g, n, P.D);
+        if (D) std::memcpy(D, P.D.data(), sizeof(double) * n);
+        Glibc g2;
+        g2.seed(seed);
+        gen_subsample(g2, n, m, P);
+        if (idx)
+            for (long i = 0; i < m; ++i) idx[i] = P.bsrc[(size_t)i];
         return SKL_OK;
     });
 }

@@ -5,6 +5,8 @@
 //       test/ls_blendenpik_hashing.cpp (dense robust solver, k=1, gamma=1.7)
 //   ls_blendenpik_hashing_sparse_input <matrix.mtx> [gamma] [k]
 //       test/ls_blendenpik_hashing_sparse_input.cpp (same on a densified matrix)
+//   ls_blendenpik <matrix.mtx> [gamma]
+//       test/ls_blendenpik.cpp (ls_dense_blendenpik, gamma=2.2)
 //   ls_qr <matrix.mtx> [gamma] [k] [abs_tol] [ordering]
 //       test/ls_qr.cpp (ls_sparse_spqr, k=2, gamma=1.4)
 // The program name (argv[0]'s basename) selects the driver; the first
@@ -59,6 +61,39 @@ static int dense_driver(const char *label, int argc, char **argv) {
     std::cout << "Residual of " << label << " is: " << std::sqrt(r) << std::endl;
     std::cout << "Iteration of " << label << " is: " << it << std::endl;
     std::cout << "Rank of " << label << " is: " << rank << std::endl;
+    return 0;
+}
+
+static int blendenpik_driver(int argc, char **argv) {
+    CscMatrix S = read_matrix_market(argv[1]);
+    std::cout << "number of rows is: " << S.m << std::endl;
+    std::cout << "number of columns is: " << S.n << std::endl;
+    std::cout << "number of non-zeros is: " << S.i.size() << std::endl;
+    std::vector<double> data = to_dense(S);
+    Mat_d A(S.m, S.n, data.data());
+    std::vector<double> bd((size_t)S.m, 1.0);
+    Vec_d b(S.m, bd.data());
+    double gamma = 2.2;
+    long max_it = MAX_IT;
+    double it_tol = IT_TOL;
+    int wisdom = WISDOM, debug = 0;
+    if (argc >= 3) gamma = std::atof(argv[2]);
+    std::cout << "Solver Name: Ls_blendenpik" << std::endl;
+    std::cout << "OVER_SAMPLING_RATIO is: " << gamma << std::endl;
+    long it, rank;
+    int flag;
+    Vec_d x(A.n());
+    double t0 = now();
+    ls_dense_blendenpik(A, b, x, rank, flag, it, gamma, it_tol, max_it, debug, wisdom);
+    double t1 = now();
+    std::cout << "Iteration flag is: " << flag << std::endl;
+    std::cout << "Time taken by ls_blendenpik is: " << (t1 - t0) << std::endl;
+    A.mv('n', 1, x, -1, b);
+    double r = 0;
+    for (long i = 0; i < b.n(); ++i) r += b(i) * b(i);
+    std::cout.precision(6);
+    std::cout << "Residual of ls_blendenpik is: " << std::sqrt(r) << std::endl;
+    std::cout << "Iteration of ls_blendenpik is: " << it << std::endl;
     return 0;
 }
 
@@ -119,6 +154,9 @@ int main(int argc, char **argv) {
     }
     try {
         if (name.find("ls_qr") != std::string::npos) return sparse_driver(argc, argv);
+        if (name.find("hashing") == std::string::npos &&
+            name.find("ls_blendenpik") != std::string::npos)
+            return blendenpik_driver(argc, argv);
         if (name.find("sparse_input") != std::string::npos)
             return dense_driver("ls_blendenpik_hashing", argc, argv);
         return dense_driver("ls_blendenpik_hashing", argc, argv);

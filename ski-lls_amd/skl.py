@@ -65,6 +65,7 @@ EXPORTS = [
     "skl_default_opts", "skl_init", "skl_finalize", "skl_error_string", "skl_last_error",
     "skl_device_count", "skl_dense_solve", "skl_dense_solve_nocpqr", "skl_sparse_solve",
     "skl_dense_solve_dev", "skl_dense_solve_nocpqr_dev", "skl_sparse_solve_dev",
+    "skl_dense_solve_blendenpik", "skl_dense_solve_blendenpik_dev", "skl_subsample_pattern",
     "skl_set_history", "skl_hash_pattern", "skl_dht", "skl_hash_apply", "skl_dense_sketch",
     "skl_sparse_sketch", "skl_qr", "skl_cpqr", "skl_trinv", "skl_lsqr_dense",
     "skl_profile_enable", "skl_profile_reset", "skl_profile_read",
@@ -128,8 +129,9 @@ def _c(a):
 
 
 # ---- solvers ------------------------------------------------------------------
-def dense_solve(A, b, cpqr=True, history=0, **opts):
-    """x, info[, steps] for min ||A x - b|| (host arrays)."""
+def dense_solve(A, b, cpqr=True, history=0, sampling="hashing", **opts):
+    """x, info[, steps] for min ||A x - b|| (host arrays).  sampling="uniform"
+    runs ls_dense_blendenpik (row sampling of the transform, no CPQR)."""
     A = _f(A)
     b = _c(b)
     n, d = A.shape
@@ -140,7 +142,10 @@ def dense_solve(A, b, cpqr=True, history=0, **opts):
     if history:
         H = (LsqrStep * history)()
         lib().skl_set_history(H, C.c_long(history))
-    fn = lib().skl_dense_solve if cpqr else lib().skl_dense_solve_nocpqr
+    if sampling == "uniform":
+        fn = lib().skl_dense_solve_blendenpik
+    else:
+        fn = lib().skl_dense_solve if cpqr else lib().skl_dense_solve_nocpqr
     try:
         rc = fn(C.c_long(n), C.c_long(d), _d(A), C.c_long(n), _d(b), _d(x), C.byref(o),
                 C.byref(info))
@@ -182,11 +187,14 @@ def sparse_solve(Acsc, b, history=0, **opts):
     return x, info
 
 
-def dense_solve_dev(n, d, A_ptr, lda, b_ptr, x_ptr, cpqr=True, **opts):
+def dense_solve_dev(n, d, A_ptr, lda, b_ptr, x_ptr, cpqr=True, sampling="hashing", **opts):
     """Device-resident inputs (raw device pointers, e.g. torch tensor data_ptr())."""
     o = default_opts(**opts)
     info = Info()
-    fn = lib().skl_dense_solve_dev if cpqr else lib().skl_dense_solve_nocpqr_dev
+    if sampling == "uniform":
+        fn = lib().skl_dense_solve_blendenpik_dev
+    else:
+        fn = lib().skl_dense_solve_dev if cpqr else lib().skl_dense_solve_nocpqr_dev
     rc = fn(C.c_long(n), C.c_long(d), C.c_void_p(A_ptr), C.c_long(lda), C.c_void_p(b_ptr),
             C.c_void_p(x_ptr), C.byref(o), C.byref(info))
     _check(rc)
@@ -222,6 +230,16 @@ def ls_dense_hashing_blendenpik_noCPQR(A, b, gamma, k, it_tol, max_it, debug=0, 
         seed = int(time.time()) & 0xFFFFFFFF
     x, info = dense_solve(A, b, cpqr=False, gamma=gamma, k=k, it_tol=it_tol, max_it=max_it,
                           debug=debug, wisdom=wisdom, seed=seed)
+    return x, info.rank, info.flag, info.it
+
+
+def ls_dense_blendenpik(A, b, gamma, it_tol, max_it, debug=0, wisdom=0, seed=None):
+    """include/blendenpik.hpp:31-33: returns (x, rank, flag, it)."""
+    if seed is None:
+        import time
+        seed = int(time.time()) & 0xFFFFFFFF
+    x, info = dense_solve(A, b, cpqr=False, sampling="uniform", gamma=gamma, it_tol=it_tol,
+                          max_it=max_it, debug=debug, wisdom=wisdom, seed=seed)
     return x, info.rank, info.flag, info.it
 
 
@@ -336,6 +354,14 @@ def hash_pattern(seed, m, n, k, dense):
                                   C.c_long(n), C.c_long(k), _d(D) if D is not None else None,
                                   _l(rows), _d(vals)))
     return D, rows, vals
+
+
+def subsample_pattern(seed, m, n):
+    """Signs D and the m sampled rows of ls_dense_blendenpik for a seed."""
+    D = np.zeros(n)
+    idx = np.zeros(m, dtype=np.int64)
+    _check(lib().skl_subsample_pattern(C.c_uint(seed), C.c_long(m), C.c_long(n), _d(D), _l(idx)))
+    return D, idx
 
 
 def dht(X, precision=SKL_FP64):

@@ -109,6 +109,19 @@ def dense_pattern(seed, n, m, k):
     return D, rows, vals
 
 
+def select_k_from_n(g, n, k):
+    idx = np.empty(k, dtype=np.int64)
+    lib().orc_select_k_from_n(C.byref(g), C.c_long(n), C.c_long(k), _l(idx))
+    return idx
+
+
+def subsample_pattern(seed, n, m):
+    """Signs D and sampled rows of ls_dense_blendenpik (dense_helpers.cpp:7-59):
+    both drawn from the stream seeded with `seed` (it is seeded twice)."""
+    D = gen_signs(rng(seed), n)
+    return D, select_k_from_n(rng(seed), n, m)
+
+
 def sparse_pattern(seed, n, m, k):
     g = rng(seed)
     rows, _, vals = gen_hashing(g, m, n, k)
@@ -324,6 +337,21 @@ def ls_dense_nocpqr(A, b, gamma=1.7, k=1, it_tol=1e-10, max_it=10000, seed=1, hi
                                    _d(f64(b)), _d(x), C.c_double(gamma), C.c_long(k),
                                    C.c_double(it_tol), C.c_long(max_it), C.c_uint(seed), H,
                                    C.byref(info))
+    if rc != 0:
+        raise RuntimeError("Matrix is not overdetermined." if rc == 1 else f"oracle error {rc}")
+    return x, info, _steps(H, info.it)
+
+
+def ls_dense_blendenpik(A, b, gamma=1.7, it_tol=1e-10, max_it=10000, seed=1, hist=False):
+    A = fortran(A)
+    n, d = A.shape
+    x = np.zeros(d)
+    info = Info()
+    H = _hist(max_it, hist)
+    rc = lib().orc_ls_dense_blendenpik(C.c_long(n), C.c_long(d), _d(A), C.c_long(n),
+                                       _d(f64(b)), _d(x), C.c_double(gamma),
+                                       C.c_double(it_tol), C.c_long(max_it), C.c_uint(seed), H,
+                                       C.byref(info))
     if rc != 0:
         raise RuntimeError("Matrix is not overdetermined." if rc == 1 else f"oracle error {rc}")
     return x, info, _steps(H, info.it)

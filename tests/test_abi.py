@@ -42,6 +42,8 @@ def test_shim_exports_reference_entry_points():
                  "int&, long&, double, long, double, double, double, long, int, int)",
                  "ls_dense_hashing_blendenpik_noCPQR(Mat<double>&, Vec<double>&, Vec<double>&, "
                  "long&, int&, long&, double, long, double, long, int, int)",
+                 "ls_dense_blendenpik(Mat<double>&, Vec<double>&, Vec<double>&, long&, int&, "
+                 "long&, double, double, long, int, int)",
                  "ls_sparse_spqr(cs_dl_sparse&, Vec<double>&, Vec<double>&, long&, int&, long&, "
                  "double, long, double, int, double, long, double, double, int)"]:
         assert name in demangled, name
@@ -107,10 +109,33 @@ int main() {
     return 0;
 }
 ''')
+    # test/ls_blendenpik.cpp reaches ls_dense_blendenpik through blendenpik.hpp
+    src2 = tmp_path / "ex2.cpp"
+    src2.write_text(r'''
+#include "blendenpik.hpp"
+int main() {
+    double a[8] = {1, 2, 3, 4, 5, 6, 7, 9}, bb[4] = {1, 1, 1, 1};
+    Mat_d A(4, 2, a); Vec_d b(4, bb); Vec_d x(2);
+    long it, rank; int flag;
+    ls_dense_blendenpik(A, b, x, rank, flag, it, 2.0, IT_TOL, MAX_IT, 0, WISDOM);
+    return 0;
+}
+''')
+    exe2 = tmp_path / "ex2"
+    subprocess.check_call(["g++", "-std=c++11", "-O1", "-I", os.path.join(ROOT, "include"),
+                           str(src2), "-o", str(exe2), LIB, "-Wl,-rpath," + os.path.dirname(LIB)])
+    assert exe2.exists()
     exe = tmp_path / "ex"
     subprocess.check_call(["g++", "-std=c++11", "-O1", "-I", os.path.join(ROOT, "include"),
                            str(src), "-o", str(exe), LIB, "-Wl,-rpath," + os.path.dirname(LIB)])
     assert exe.exists()
+
+
+@pytest.mark.parametrize("seed,m,n", [(5, 340, 4000), (1, 6800, 40000), (77, 300, 300)])
+def test_host_subsample_bit_exact_with_oracle(skl, O, seed, m, n):
+    D, idx = skl.subsample_pattern(seed, m, n)
+    Do, io = O.subsample_pattern(seed, n, m)
+    assert np.array_equal(D, Do) and np.array_equal(idx, io)
 
 
 def test_ctypes_structs_match_c_layout(skl, tmp_path):

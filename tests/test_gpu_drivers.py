@@ -1,4 +1,5 @@
-"""Command-line drivers (reference test/ls_blendenpik_hashing*.cpp, test/ls_qr.cpp)
+"""Command-line drivers (reference test/ls_blendenpik_hashing*.cpp, test/ls_qr.cpp,
+test/ls_blendenpik.cpp)
 on the reference's fixtures: the printed residual matches the known answer."""
 import os
 import re
@@ -25,6 +26,13 @@ def residual(driver, path, *args):
 def test_dense_driver_full_rank(kat):
     for f in ["50by15DenseFullRank.mtx", "120by20DenseFullRank.mtx"]:
         r = residual("ls_blendenpik_hashing", os.path.join(GOLDEN, "data", f))
+        ref = kat["full_rank_gelsd_residual"][f]
+        assert abs(r - ref) < 1e-5 * ref, (f, r, ref)
+
+
+def test_blendenpik_driver_full_rank(kat):
+    for f in ["50by15DenseFullRank.mtx", "120by20DenseFullRank.mtx"]:
+        r = residual("ls_blendenpik", os.path.join(GOLDEN, "data", f))
         ref = kat["full_rank_gelsd_residual"][f]
         assert abs(r - ref) < 1e-5 * ref, (f, r, ref)
 

@@ -45,6 +45,30 @@ def test_dense_nocpqr_c1_matches_oracle(skl, O):
     assert rel(x, xo) < 1e-8
 
 
+@pytest.mark.parametrize("n,d,gamma", [(4000, 200, 4.0), (40000, 400, 2.0)])
+def test_dense_blendenpik_matches_oracle(skl, O, n, d, gamma):
+    """ls_dense_blendenpik: rows of the randomized DHT sampled uniformly
+    (n = 4000 takes the single-pass transform, n = 40000 the four-step one),
+    Householder QR, LSQR from zero."""
+    A, b = c1_problem(4, n=n, d=d)
+    kw = dict(gamma=gamma, it_tol=1e-10, seed=91)
+    x, info = skl.dense_solve(A, b, cpqr=False, sampling="uniform", **kw)
+    xo, io, _ = O.ls_dense_blendenpik(A, b, **kw)
+    assert info.flag == 0 and info.rank == d and info.m == int(np.ceil(gamma * d))
+    assert abs(info.it - io.it) <= 2, (info.it, io.it)
+    assert rel(x, xo) < 1e-8
+    assert info.lsqr_nrm_ar < 1e-10 * info.lsqr_nrm_a * info.lsqr_nrm_r
+    xm, im = skl.dense_solve(A, b, cpqr=False, sampling="uniform", precision=skl.SKL_MIXED, **kw)
+    assert im.flag == 0 and im.escalated == 0 and abs(im.it - info.it) <= 3
+    assert rel(xm, x) < 1e-8
+
+
+def test_dense_blendenpik_rejects_more_samples_than_rows(skl):
+    A, b = c1_problem(5, n=300, d=200)
+    with pytest.raises(RuntimeError, match="row sampling"):
+        skl.dense_solve(A, b, cpqr=False, sampling="uniform", gamma=1.7)
+
+
 def test_dense_history_matches_oracle(skl, O):
     A, b = c1_problem(3, n=3000, d=120)
     kw = dict(gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=5)

@@ -276,6 +276,54 @@ def test_dense_solver_matches_gelsd(O):
     assert rel(x2, xr) < 1e-8 and info2.flag == 0
 
 
+def test_subsample_is_the_hashing_draw_for_one_column(O):
+    """select_k_from_n is the step gen_hashing_matrix takes per column
+    (sparse_helpers.cpp:98-104); with one column the two draw the same rows
+    from the same stream, and gen_hashing is pinned by the golden patterns."""
+    for seed, n, k in [(1, 4000, 340), (7, 500, 500), (123, 40000, 6800)]:
+        idx = O.select_k_from_n(O.rng(seed), n, k)
+        rows, _ = O.sparse_pattern(seed, 1, n, k)
+        assert np.array_equal(idx, rows), seed
+        assert len(set(idx.tolist())) == k and idx.min() >= 0 and idx.max() < n
+
+
+def test_subsample_pattern_restarts_the_stream(O):
+    """ls_dense_blendenpik seeds twice from the same clock: the signs and the
+    sampled rows both start at the first draw of the stream."""
+    seed, n, m = 5, 1000, 120
+    D, idx = O.subsample_pattern(seed, n, m)
+    r = O.rand_seq(seed, m)
+    assert np.array_equal(D[:m], (r % 2) * 2.0 - 1.0)
+    arr = list(range(n))
+    for i in range(m):  # brute-force Fisher-Yates on the libc values
+        j = int(r[i]) % (n - i) + i
+        arr[i], arr[j] = arr[j], arr[i]
+    assert np.array_equal(idx, arr[:m])
+
+
+def test_blendenpik_solver_matches_gelsd(O):
+    A, b = c1_problem(2, n=2000, d=100)
+    xr = sl.lstsq(A, b, lapack_driver="gelsd")[0]
+    x, info, _ = O.ls_dense_blendenpik(A, b, gamma=4.0, it_tol=1e-10, seed=3)
+    assert rel(x, xr) < 1e-8 and info.flag == 0 and info.rank == 100
+    with pytest.raises(RuntimeError):   # more sampled rows than A has
+        O.ls_dense_blendenpik(A[:150], b[:150], gamma=1.7)
+
+
+def test_blendenpik_sketch_is_sampled_transform(O):
+    """The R of the oracle's Blendenpik preconditioner is the R of the sampled
+    rows of the randomized DHT: R^T R = (P F D A)^T (P F D A)."""
+    rng = np.random.default_rng(4)
+    n, d, m = 300, 12, 40
+    A = rng.standard_normal((n, d))
+    D, idx = O.subsample_pattern(9, n, m)
+    F = O.fut(np.eye(n), D)              # the transform applied to the identity
+    SA = (F @ A)[idx]
+    assert np.allclose(SA, O.fut(A, D)[idx], atol=1e-13)
+    R = sl.qr(SA, mode="r")[0][:d]
+    assert np.allclose(R.T @ R, SA.T @ SA, rtol=1e-10, atol=1e-10)
+
+
 def test_sparse_solver_matches_gelsd(O):
     A, b = sparse_problem(0, n=3000, d=80)
     xr = sl.lstsq(A.toarray(), b, lapack_driver="gelsd")[0]
