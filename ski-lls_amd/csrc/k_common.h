@@ -8,6 +8,17 @@ namespace skl {
 template <class T> void fill2d(long m, long n, T *A, long lda, T v);
 template <class Ts, class Td> void copy2d(long m, long n, const Ts *A, long lda, Td *B, long ldb);
 
+// MI355X dispatches workgroup i of a grid to XCD i mod 8, and each XCD has
+// its own L2.  This maps the hardware index to a logical one under which
+// consecutive logical workgroups run on the same XCD (XCD x takes a
+// contiguous range of the logical indices), so tiles that share cache lines
+// (the blocks of one column) meet in one L2 instead of fetching the lines
+// once per XCD.  A bijection on [0, nblocks) for any nblocks.
+__device__ inline long xcd_local_block(long bid, long nblocks) {
+    const long q = nblocks / 8, r = nblocks % 8, x = bid % 8, slot = bid / 8;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + slot;
+}
+
 __device__ inline double shfl_xor_d(double v, int mask) {
     return __shfl_xor(v, mask, 64);
 }

@@ -13,6 +13,14 @@
 //  * larger smooth n = n1 n2: four-step FFT (pass 1: length-n1 FFTs on
 //    strided rows + twiddles; pass 2: length-n2 FFTs on pairs of mirrored
 //    rows, which also yields the DHT), then the bucket gather.
+//  * other n (a prime factor above 5): Bluestein's algorithm on a
+//    {2,3,5}-smooth length of at least 2n - 1.
+//
+// The LDS FFT uses radices 2, 3, 4, 5 and 8, eight complex elements per
+// thread per stage, and 512 (FP64) / 1024 (FP32) threads per workgroup: at
+// most about 140 VGPRs, so several waves per SIMD hide the memory latency.
+// Loads are batched (all of a thread's elements in flight before the first
+// use), and workgroups that share cache lines are mapped to one XCD.
 //
 // Bucket sums run with contraction off, in increasing source row, matching
 // the reference's scatter order (src/SpMat.cpp:53-55).
@@ -26,7 +34,7 @@
 
 namespace skl {
 
-constexpr int FFT_MAXE = 16;   // complex elements per thread held per stage
+constexpr int FFT_MAXE = 8;    // complex elements per thread held per stage
 constexpr int MAX_STAGES = 24;
 
 struct FftDims {
@@ -146,10 +154,7 @@ __device__ void lds_fft(cplx<T> *buf, int S, const FftDims &D, const cplx<T> *__
         case 3: fft_stage<T, 3, NT>(buf, D.L, S, len, stride, tw); break;
         case 4: fft_stage<T, 4, NT>(buf, D.L, S, len, stride, tw); break;
         case 5: fft_stage<T, 5, NT>(buf, D.L, S, len, stride, tw); break;
-        case 7: fft_stage<T, 7, NT>(buf, D.L, S, len, stride, tw); break;
         case 8: fft_stage<T, 8, NT>(buf, D.L, S, len, stride, tw); break;
-        case 11: fft_stage<T, 11, NT>(buf, D.L, S, len, stride, tw); break;
-        case 13: fft_stage<T, 13, NT>(buf, D.L, S, len, stride, tw); break;
         default: break;
         }
         len /= r;
@@ -186,10 +191,22 @@ __global__ __launch_bounds__(NT) void k_dht_single(long n, long ncols, const dou
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const long c0 = 2L * blockIdx.x, c1 = c0 + 1;
     const bool has1 = c1 < ncols;
-    for (long j = threadIdx.x; j < n; j += NT) {
-        T re = load_scaled<T>(X, j + c0 * ldx, D, j, scale);
-        T im = has1 ? load_scaled<T>(X, j + c1 * ldx, D, j, scale) : T(0);
-        buf[j] = {re, im};
+    {
+        cplx<T> v[FFT_MAXE]; // n <= NT * FFT_MAXE; all loads in flight together
+#pragma unroll
+        for (int u = 0; u < FFT_MAXE; ++u) {
+            const long j = threadIdx.x + (long)u * NT;
+            v[u] = {T(0), T(0)};
+            if (j < n) {
+                v[u].x = load_scaled<T>(X, j + c0 * ldx, D, j, scale);
+                v[u].y = has1 ? load_scaled<T>(X, j + c1 * ldx, D, j, scale) : T(0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < FFT_MAXE; ++u) {
+            const long j = threadIdx.x + (long)u * NT;
+            if (j < n) buf[j] = v[u];
+        }
     }
     __syncthreads();
     lds_fft<T, NT>(buf, 1, dims, tw);
@@ -240,7 +257,7 @@ __global__ __launch_bounds__(NT) void k_dht_single(long n, long ncols, const dou
 // Y[k1*n2 + j2] = W_n^{j2 k1} * FFT_{n1}( x[j1*n2 + j2] )[k1]
 // ---------------------------------------------------------------------------
 template <class T, int NT>
-__global__ __launch_bounds__(NT) void k_fft4_pass1(long n, int n1, int n2, int J2, long ncols,
+__global__ __launch_bounds__(NT, 2048 / NT) void k_fft4_pass1(long n, int n1, int n2, int J2, long ncols,
                                                    const double *__restrict__ X, long ldx,
                                                    const double *__restrict__ D, double scale,
                                                    FftDims dims, const cplx<T> *__restrict__ tw1,
@@ -248,32 +265,50 @@ __global__ __launch_bounds__(NT) void k_fft4_pass1(long n, int n1, int n2, int J
                                                    cplx<T> *__restrict__ Y) {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
-    const long pair = blockIdx.y;
+    const long lb = xcd_local_block(blockIdx.x + (long)gridDim.x * blockIdx.y,
+                                    (long)gridDim.x * gridDim.y);
+    const long pair = lb / gridDim.x;
     const long c0 = 2 * pair, c1 = c0 + 1;
     const bool has1 = c1 < ncols;
-    const int j2b = blockIdx.x * J2;
+    const int j2b = (int)(lb % gridDim.x) * J2;
     const int nJ = min(J2, n2 - j2b);
-    const int tot = n1 * J2;
-    for (int e = threadIdx.x; e < tot; e += NT) {
-        int jj = e % J2, j1 = e / J2;
-        cplx<T> v = {T(0), T(0)};
-        if (jj < nJ) {
-            long j = (long)j1 * n2 + j2b + jj;
-            v.x = load_scaled<T>(X, j + c0 * ldx, D, j, scale);
-            v.y = has1 ? load_scaled<T>(X, j + c1 * ldx, D, j, scale) : T(0);
+    const int tot = n1 * J2; // <= NT * FFT_MAXE
+    // every load of the thread is issued before the first is used: the
+    // elements are independent, and a loop that consumed each load before
+    // issuing the next would pay the memory latency once per element
+    cplx<T> v[FFT_MAXE];
+#pragma unroll
+    for (int u = 0; u < FFT_MAXE; ++u) {
+        const int e = threadIdx.x + u * NT;
+        const int jj = e % J2, j1 = e / J2;
+        v[u] = {T(0), T(0)};
+        if (e < tot && jj < nJ) {
+            const long j = (long)j1 * n2 + j2b + jj;
+            v[u].x = load_scaled<T>(X, j + c0 * ldx, D, j, scale);
+            v[u].y = has1 ? load_scaled<T>(X, j + c1 * ldx, D, j, scale) : T(0);
         }
-        buf[j1 * J2 + jj] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < FFT_MAXE; ++u) {
+        const int e = threadIdx.x + u * NT;
+        if (e < tot) buf[e] = v[u]; // e = j1 J2 + jj
     }
     __syncthreads();
     lds_fft<T, NT>(buf, J2, dims, tw1);
     cplx<T> *Yp = Y + pair * n;
-    for (int e = threadIdx.x; e < tot; e += NT) {
-        int jj = e % J2, k1 = e / J2;
-        if (jj < nJ) {
-            int j2 = j2b + jj;
-            long ex = ((long)j2 * k1) % n;
-            Yp[(long)k1 * n2 + j2] = cmul(buf[k1 * J2 + jj], twn[ex]);
-        }
+    // twiddles first (all in flight together), then the products
+#pragma unroll
+    for (int u = 0; u < FFT_MAXE; ++u) {
+        const int e = threadIdx.x + u * NT;
+        const int jj = e % J2, k1 = e / J2;
+        // j2 k1 <= (n2 - 1)(n1 - 1) < n: no reduction needed
+        v[u] = (e < tot && jj < nJ) ? twn[(j2b + jj) * k1] : cplx<T>{T(0), T(0)};
+    }
+#pragma unroll
+    for (int u = 0; u < FFT_MAXE; ++u) {
+        const int e = threadIdx.x + u * NT;
+        const int jj = e % J2, k1 = e / J2;
+        if (e < tot && jj < nJ) Yp[(long)k1 * n2 + j2b + jj] = cmul(buf[k1 * J2 + jj], v[u]);
     }
 }
 
@@ -289,25 +324,36 @@ __global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K
                                                    T *__restrict__ H, long ldh) {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
-    const long pair = blockIdx.y;
+    const long lb = xcd_local_block(blockIdx.x + (long)gridDim.x * blockIdx.y,
+                                    (long)gridDim.x * gridDim.y);
+    const long pair = lb / gridDim.x;
     const long c0 = 2 * pair, c1 = c0 + 1;
     const bool has1 = c1 < ncols;
     const int half = n1 / 2;                 // primary rows 0..half
-    const int kb = blockIdx.x * K1;
+    const int kb = (int)(lb % gridDim.x) * K1;
     const int nK = min(K1, half + 1 - kb);   // primary rows in this block
     const int S = 2 * K1;                    // sequence slots: [0,K1) primary, [K1,2K1) mirror
     const cplx<T> *Yp = Y + pair * n;
-    const int tot = n2 * S;
-    for (int e = threadIdx.x; e < tot; e += NT) {
-        int sl = e % S, j2 = e / S;
-        int kk = sl % K1;
-        cplx<T> v = {T(0), T(0)};
-        if (kk < nK) {
-            int k1 = kb + kk;
-            int row = (sl < K1) ? k1 : (n1 - k1) % n1;
-            v = Yp[(long)row * n2 + j2];
+    const int tot = n2 * S; // <= NT * FFT_MAXE
+    {
+        cplx<T> v[FFT_MAXE]; // all loads in flight before the first use
+#pragma unroll
+        for (int u = 0; u < FFT_MAXE; ++u) {
+            const int e = threadIdx.x + u * NT;
+            const int sl = e % S, j2 = e / S;
+            const int kk = sl % K1;
+            v[u] = {T(0), T(0)};
+            if (e < tot && kk < nK) {
+                const int k1 = kb + kk;
+                const int row = (sl < K1) ? k1 : (n1 - k1) % n1;
+                v[u] = Yp[(long)row * n2 + j2];
+            }
         }
-        buf[j2 * S + sl] = v;
+#pragma unroll
+        for (int u = 0; u < FFT_MAXE; ++u) {
+            const int e = threadIdx.x + u * NT;
+            if (e < tot) buf[e] = v[u]; // e = j2 S + sl
+        }
     }
     __syncthreads();
     lds_fft<T, NT>(buf, S, dims, tw2);
@@ -345,32 +391,50 @@ __global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K
 // ---------------------------------------------------------------------------
 // Bucket gather: Y[h, c] = sum_{q in bucket h} coef[q] X[bsrc[q], c]
 // ---------------------------------------------------------------------------
+// One workgroup per (column, 256 buckets); the workgroups of a column run on
+// one XCD, so the column's random reads are served by that XCD's L2 after
+// one fetch of the column.
 template <class T>
-__global__ __launch_bounds__(256) void k_hash_gather(long m, long ncols,
+__global__ __launch_bounds__(256) void k_hash_gather(long m, long ncols, long nbh,
                                                      const int32_t *__restrict__ bptr,
                                                      const int32_t *__restrict__ bsrc,
                                                      const T *__restrict__ bcoef,
                                                      const T *__restrict__ X, long ldx,
                                                      T *__restrict__ Y, long ldy) {
 #pragma clang fp contract(off)
-    long h = (long)blockIdx.x * 256 + threadIdx.x;
-    if (h >= m) return;
+    const long lb = xcd_local_block(blockIdx.x, gridDim.x);
+    const long c = lb / nbh;
+    const long h = (lb % nbh) * 256 + threadIdx.x;
+    if (h >= m || c >= ncols) return;
     const int q0 = bptr[h], q1 = bptr[h + 1];
-    for (long c = blockIdx.y; c < ncols; c += gridDim.y) {
-        const T *x = X + c * ldx;
-        T s = 0;
-        for (int q = q0; q < q1; ++q) s = s + bcoef[q] * x[bsrc[q]];
-        Y[h + c * ldy] = s;
+    const T *x = X + c * ldx;
+    T s = 0;
+    int q = q0;
+    for (; q + 4 <= q1; q += 4) { // four reads in flight, summed in source order
+        const int i0 = bsrc[q], i1 = bsrc[q + 1], i2 = bsrc[q + 2], i3 = bsrc[q + 3];
+        const T w0 = bcoef[q], w1 = bcoef[q + 1], w2 = bcoef[q + 2], w3 = bcoef[q + 3];
+        const T x0 = x[i0], x1 = x[i1], x2 = x[i2], x3 = x[i3];
+        s = s + w0 * x0;
+        s = s + w1 * x1;
+        s = s + w2 * x2;
+        s = s + w3 * x3;
     }
+    for (; q < q1; ++q) s = s + bcoef[q] * x[bsrc[q]];
+    Y[h + c * ldy] = s;
 }
 
 template <class T>
 void hash_gather(long m, long ncols, const int32_t *bptr, const int32_t *bsrc, const T *bcoef,
                  const T *X, long ldx, T *Y, long ldy) {
     if (m <= 0 || ncols <= 0) return;
-    dim3 g((unsigned)((m + 255) / 256), (unsigned)std::min<long>(ncols, 4096));
-    k_hash_gather<T><<<g, 256, 0, ctx().stream>>>(m, ncols, bptr, bsrc, bcoef, X, ldx, Y, ldy);
-    SKL_CHECK_LAUNCH();
+    const long nbh = (m + 255) / 256;
+    for (long c0 = 0; c0 < ncols;) { // 1-D grids of at most 2^31 - 1 workgroups
+        const long nc = std::min<long>(ncols - c0, 2147483647L / nbh);
+        k_hash_gather<T><<<(unsigned)(nbh * nc), 256, 0, ctx().stream>>>(
+            m, nc, nbh, bptr, bsrc, bcoef, X + c0 * ldx, ldx, Y + c0 * ldy, ldy);
+        SKL_CHECK_LAUNCH();
+        c0 += nc;
+    }
 }
 template void hash_gather<double>(long, long, const int32_t *, const int32_t *, const double *,
                                   const double *, long, double *, long);
@@ -384,7 +448,7 @@ static bool factor_dims(long L, FftDims &d) {
     d.L = (int)L;
     d.nst = 0;
     long x = L;
-    const int pref[] = {8, 4, 2, 3, 5, 7, 11, 13};
+    const int pref[] = {8, 4, 2, 3, 5};
     for (int r : pref) {
         while (x % r == 0) {
             if (d.nst >= MAX_STAGES) return false;
@@ -408,11 +472,15 @@ template <class T> static cplx<T> *make_twiddles(long L) {
     return dptr;
 }
 
-template <class T> constexpr int fft_nt() { return sizeof(T) == 8 ? 256 : 512; }
+template <class T> constexpr int fft_nt() { return sizeof(T) == 8 ? 512 : 1024; }
 template <class T> constexpr long lds_elems() { return (long)fft_nt<T>() * FFT_MAXE; } // 4096 / 8192
 
+// Lengths the LDS FFT runs directly: radices 2, 3, 4, 5, 8 only.  Larger
+// radices (a generic R-point DFT per butterfly) would set the register
+// allocation of every FFT kernel -- one kernel serves all lengths -- and cut
+// its occupancy; lengths with other prime factors go through Bluestein.
 static bool smooth(long x) {
-    for (int r : {2, 3, 5, 7, 11, 13})
+    for (int r : {2, 3, 5})
         while (x % r == 0) x /= r;
     return x == 1;
 }

@@ -43,7 +43,8 @@ def test_hash_apply_bit_exact(skl, O, k):
     assert np.array_equal(skl.hash_apply(m, rows, vals, k, a, X), O.hash_apply(m, rows, vals, k, a, X))
 
 
-@pytest.mark.parametrize("n,d,m,k", [(4000, 20, 340, 1), (4000, 7, 40, 3), (40000, 6, 300, 1)])
+@pytest.mark.parametrize("n,d,m,k", [(4000, 20, 340, 1), (4000, 7, 40, 3), (40000, 6, 300, 1),
+                                     (40000, 37, 600, 2)])
 def test_dense_sketch(skl, O, n, d, m, k):
     rng = np.random.default_rng(n + d)
     A = rng.standard_normal((n, d))
