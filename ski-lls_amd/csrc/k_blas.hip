@@ -457,40 +457,64 @@ void trinv_upper(long r, const double *R, long ldr, double perturb, double *M, l
 // ---------------------------------------------------------------------------
 // Cholesky factorization G = R^T R (upper R) of a symmetric positive definite
 // d x d matrix, blocked right-looking with 64-column blocks: the diagonal
-// block in LDS (one workgroup), the block row by forward substitution (one
-// thread per column), the trailing matrix by a GEMM on the matrix cores.
+// block in registers (one workgroup), the block row by forward substitution
+// (one thread per column), the trailing matrix by a GEMM on the matrix cores
+// (upper tiles only).
 // ---------------------------------------------------------------------------
 constexpr int CH_B = 64, CH_LD = 65;
 
-// b x b diagonal block (b <= 64) in place, upper factor; a pivot that is not
-// positive and finite sets *fail and is replaced by 1 so the sweep completes
-__global__ __launch_bounds__(256) void k_chol_diag(int b, double *A, long lda, int *fail) {
-    __shared__ double S[CH_B * CH_LD];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < b * b; e += 256) {
-        const int r = e % b, c = e / b;
-        S[r + c * CH_LD] = r <= c ? A[r + (long)c * lda] : 0.0;
+// b x b diagonal block (b <= 64, padded with the identity), in registers:
+// thread (i, g) of 64 x 4 holds entries (i, g + 4q), q < 16.  Step k
+// publishes row k to LDS (double-buffered, so one barrier per step); every
+// thread takes the pivot and row k from there; row k becomes R_k. and the
+// rows below subtract R_ki R_k. over all 64 columns, so the lower triangle is
+// unused scratch (never read as data, finite: products of R entries).
+// Writes U (zeros below the diagonal) over A.  A pivot that is not positive
+// and finite sets *fail and is replaced by 1 so the sweep completes.
+constexpr int CH_NT = 256, CH_Q = CH_B * CH_B / CH_NT; // 16 entries per thread
+__global__ __launch_bounds__(CH_NT) void k_chol_diag(int b, double *A, long lda, int *fail) {
+    constexpr int G = CH_NT / CH_B;
+    __shared__ double rows[2][CH_B];
+    const int i = threadIdx.x % CH_B, g = threadIdx.x / CH_B;
+    double a[CH_Q];
+#pragma unroll
+    for (int q = 0; q < CH_Q; ++q) {
+        const int c = g + G * q;
+        a[q] = (i < b && c < b) ? (i <= c ? A[i + (long)c * lda] : 0.0) : (i == c ? 1.0 : 0.0);
     }
-    __syncthreads();
+    bool bad = false;
     for (int k = 0; k < b; ++k) {
-        const double p = S[k + k * CH_LD];
-        const bool bad = !(p > 0.0) || !(p < INFINITY);
-        const double rk = bad ? 1.0 : sqrt(p);
+        double *rw = rows[k & 1];
+        if (i == k)
+#pragma unroll
+            for (int q = 0; q < CH_Q; ++q) rw[g + G * q] = a[q];
         __syncthreads();
-        if (bad && tid == 0) *fail = 1;
-        for (int c = k + 1 + tid; c < b; c += 256) S[k + c * CH_LD] /= rk;
-        if (tid == 0) S[k + k * CH_LD] = rk;
-        __syncthreads();
-        const int w = b - k - 1;
-        for (int e = tid; e < w * w; e += 256) {
-            const int i = k + 1 + e % w, c = k + 1 + e / w;
-            if (i <= c) S[i + c * CH_LD] -= S[k + i * CH_LD] * S[k + c * CH_LD];
-        }
-        __syncthreads();
+        double p = rw[k];
+        const bool pb = !(p > 0.0) || !(p < INFINITY);
+        bad = bad || pb;
+        if (pb) p = 1.0;
+        // A dependent FP64 operation costs tens of cycles on one wave, and
+        // the pivot is on the critical path of every step: 1/sqrt(p) by the
+        // hardware estimate and two Newton steps, then R_kc = a_kc / sqrt(p)
+        // and a_ic -= (a_ki / p) a_kc, one dependent operation each.
+        double y = __builtin_amdgcn_rsq(p);
+        const double h = 0.5 * p;
+        y = y * fma(-h, y * y, 1.5);
+        y = y * fma(-h, y * y, 1.5);
+        const double ip = y * y;     // 1 / p
+        const double wki = rw[i] * ip; // R_ki / sqrt(p) (used by rows i > k)
+        double rc[CH_Q]; // all reads before the arithmetic
+#pragma unroll
+        for (int q = 0; q < CH_Q; ++q) rc[q] = rw[g + G * q];
+#pragma unroll
+        for (int q = 0; q < CH_Q; ++q)
+            a[q] = i == k ? rc[q] * y : (i > k ? fma(-wki, rc[q], a[q]) : a[q]);
     }
-    for (int e = tid; e < b * b; e += 256) {
-        const int r = e % b, c = e / b;
-        A[r + (long)c * lda] = r <= c ? S[r + c * CH_LD] : 0.0;
+    if (bad && threadIdx.x == 0) *fail = 1;
+#pragma unroll
+    for (int q = 0; q < CH_Q; ++q) {
+        const int c = g + G * q;
+        if (i < b && c < b) A[i + (long)c * lda] = i <= c ? a[q] : 0.0;
     }
 }
 
@@ -535,7 +559,7 @@ bool cholesky_upper(long d, double *G, long ldg, double *R, long ldr, Arena &ar)
         const int b = (int)std::min<long>(CH_B, d - j);
         const long nc = d - j - b;
         double *Gjj = G + j + j * ldg;
-        k_chol_diag<<<1, 256, 0, s>>>(b, Gjj, ldg, fail);
+        k_chol_diag<<<1, CH_NT, 0, s>>>(b, Gjj, ldg, fail);
         if (nc > 0) {
             double *Bp = G + j + (j + b) * ldg;
             k_chol_trsm<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(b, Gjj, ldg, Bp, ldg, nc);
