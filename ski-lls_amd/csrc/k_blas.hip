@@ -458,10 +458,10 @@ void trinv_upper(long r, const double *R, long ldr, double perturb, double *M, l
 // Cholesky factorization G = R^T R (upper R) of a symmetric positive definite
 // d x d matrix, blocked right-looking with 64-column blocks: the diagonal
 // block in registers (one workgroup), the block row by forward substitution
-// (one thread per column), the trailing matrix by a GEMM on the matrix cores
-// (upper tiles only).
+// (one wave per four columns), the trailing matrix by a GEMM on the matrix
+// cores (upper tiles only).
 // ---------------------------------------------------------------------------
-constexpr int CH_B = 64, CH_LD = 65;
+constexpr int CH_B = 64;
 
 // b x b diagonal block (b <= 64, padded with the identity), in registers:
 // thread (i, g) of 64 x 4 holds entries (i, g + 4q), q < 16.  Step k
@@ -519,33 +519,43 @@ __global__ __launch_bounds__(CH_NT) void k_chol_diag(int b, double *A, long lda,
 }
 
 // X = U^{-T} B for the b x b upper factor U (b <= 64) and B (b x nc), in
-// place: thread c solves U^T x = B(:, c) by forward substitution
-__global__ __launch_bounds__(256) void k_chol_trsm(int b, const double *U, long ldu, double *B,
-                                                   long ldb, long nc) {
-    __shared__ double Us[CH_B * CH_LD];
-    for (int e = threadIdx.x; e < b * b; e += 256) {
-        const int r = e % b, c = e / b;
-        Us[r + c * CH_LD] = r <= c ? U[r + (long)c * ldu] : 0.0;
+// place, by forward substitution.  One wave per CH_TC columns, lane p = row p:
+// lane p keeps U(0:p, p) in registers, and step i takes x_i = b_i / U_ii from
+// lane i (v_readlane; every lane ran the previous step) and subtracts
+// U_ip x_i in every lane (U_ip = 0 for p <= i).  The dependent chain is a
+// few operations per row, where one thread per column would chain one FMA
+// per entry, and the loads and stores of B are coalesced along the rows.
+constexpr int CH_TC = 4;
+__global__ __launch_bounds__(64) void k_chol_trsm(int b, const double *U, long ldu, double *B,
+                                                  long ldb, long nc) {
+    const int p = threadIdx.x;
+    const int pc = p < b ? p : b - 1; // clamped: every load stays inside the block
+    double u[CH_B];
+#pragma unroll
+    for (int i = 0; i < CH_B; ++i) {
+        const double v = U[(i < b ? i : b - 1) + (long)pc * ldu];
+        u[i] = (i < p && p < b) ? v : 0.0; // U_ip, strictly upper
     }
-    __syncthreads();
-    const long c = (long)blockIdx.x * 256 + threadIdx.x;
-    if (c >= nc) return;
-    double *col = B + c * ldb;
-    double x[CH_B];
+    const double rdp = 1.0 / U[pc + (long)pc * ldu];
+    const long c0 = (long)blockIdx.x * CH_TC;
+    double x[CH_TC];
+#pragma unroll
+    for (int t = 0; t < CH_TC; ++t) {
+        const long c = c0 + t < nc ? c0 + t : nc - 1;
+        const double v = B[pc + c * ldb];
+        x[t] = (p < b && c0 + t < nc) ? v : 0.0;
+    }
 #pragma unroll
     for (int i = 0; i < CH_B; ++i) {
         if (i < b) {
-            double s = col[i];
+            const double ri = readlane(rdp, i);
 #pragma unroll
-            for (int k = 0; k < i; ++k) s -= Us[k + i * CH_LD] * x[k];
-            x[i] = s / Us[i + i * CH_LD];
-        } else {
-            x[i] = 0.0;
+            for (int t = 0; t < CH_TC; ++t) x[t] = fma(-u[i], readlane(x[t], i) * ri, x[t]);
         }
     }
 #pragma unroll
-    for (int i = 0; i < CH_B; ++i)
-        if (i < b) col[i] = x[i];
+    for (int t = 0; t < CH_TC; ++t)
+        if (p < b && c0 + t < nc) B[p + (c0 + t) * ldb] = x[t] * rdp;
 }
 
 // R (upper, zeros below, ld ldr) from G (overwritten); false when a pivot
@@ -562,7 +572,8 @@ bool cholesky_upper(long d, double *G, long ldg, double *R, long ldr, Arena &ar)
         k_chol_diag<<<1, CH_NT, 0, s>>>(b, Gjj, ldg, fail);
         if (nc > 0) {
             double *Bp = G + j + (j + b) * ldg;
-            k_chol_trsm<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(b, Gjj, ldg, Bp, ldg, nc);
+            k_chol_trsm<<<(unsigned)((nc + CH_TC - 1) / CH_TC), 64, 0, s>>>(b, Gjj, ldg, Bp, ldg,
+                                                                            nc);
             gemm<double>('T', 'N', nc, nc, b, -1.0, Bp, ldg, Bp, ldg, 1.0,
                          G + (j + b) + (j + b) * ldg, ldg, 1, 0, 0, 0, true);
         }
