@@ -18,15 +18,15 @@ def test_device_present(skl):
     assert skl.lib().skl_init(0) == 0
 
 
-@pytest.mark.parametrize("n", [1, 2, 50, 97, 120, 340, 2798, 4000, 4096, 5533, 12000, 40000,
-                               131072, 199999])
+@pytest.mark.parametrize("n", [1, 2, 50, 97, 120, 340, 1001, 2798, 4000, 4096, 5533, 7000, 12000,
+                               40000, 131072, 199999])
 def test_dht_fp64(skl, O, n):
     rng = np.random.default_rng(n)
     X = rng.standard_normal((n, 5))
     assert rel(skl.dht(X), O.dht_cols(X)) < 1e-13
 
 
-@pytest.mark.parametrize("n", [50, 1399, 4000, 40000])
+@pytest.mark.parametrize("n", [50, 1399, 4000, 7000, 40000])
 def test_dht_fp32(skl, O, n):
     rng = np.random.default_rng(n + 1)
     X = rng.standard_normal((n, 4))

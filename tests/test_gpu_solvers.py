@@ -248,6 +248,19 @@ def test_mixed_precision_dependent_column_falls_back(skl):
     assert np.array_equal(x32, x64)
 
 
+@pytest.mark.parametrize("d", [1, 7, 64, 65, 129])
+def test_mixed_precision_small_and_ragged_widths(skl, d):
+    """Widths around the 64-column blocks of the Cholesky (a single column, a
+    partial first block, exactly one block, one column more, two blocks and
+    one more): the mixed factor must give the FP64 solution."""
+    A, b = c1_problem(30 + d, n=max(50 * d, 400), d=d)
+    x64, i64 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=8)
+    x32, i32 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=8,
+                               precision=skl.SKL_MIXED)
+    assert i32.escalated == 0 and i32.rank == d and i32.flag == 0
+    assert rel(x32, x64) < 1e-8
+
+
 def test_mixed_precision_nocpqr(skl):
     A, b = c1_problem(19, n=4000, d=200)
     x64, i64 = skl.dense_solve(A, b, cpqr=False, gamma=1.7, k=1, it_tol=1e-10, seed=6)
