@@ -210,7 +210,40 @@ def run_dense(args, skl, torch, W):
                             ((R["x"] - xq).norm() / xq.norm()).item())
     res["x_rel_mixed_vs_fp64"] = ((res["mixed"]["x"] - res["fp64"]["x"]).norm() /
                                   res["fp64"]["x"].norm()).item()
+    if not args.no_other_solvers:
+        res["other"] = other_dense_solvers(args, skl, torch, W, At, b, res["fp64"]["x"], xq)
     return At, b, res
+
+
+def other_dense_solvers(args, skl, torch, W, At, b, x_ref, xq):
+    """The two other dense entry points on the same matrix (no sketch-and-solve
+    start, LSQR from zero): ls_dense_hashing_blendenpik_noCPQR and
+    ls_dense_blendenpik, each in FP64 and mixed precision, one warm-up and two
+    timed solves."""
+    n, d = W["n"], W["d"]
+    x = torch.zeros(d, dtype=torch.float64, device="cuda")
+    out = {}
+    for label, sampling in [("ls_dense_hashing_blendenpik_noCPQR", "hashing"),
+                            ("ls_dense_blendenpik", "uniform")]:
+        out[label] = {}
+        for prec, pname in [(skl.SKL_FP64, "fp64"), (skl.SKL_MIXED, "mixed")]:
+            def solve():
+                return skl.dense_solve_dev(n, d, At.data_ptr(), n, b.data_ptr(), x.data_ptr(),
+                                           cpqr=False, sampling=sampling, precision=prec,
+                                           gamma=W["gamma"], k=W["k"], it_tol=1e-10, seed=1234)
+            solve()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(2):
+                info = solve()
+            torch.cuda.synchronize()
+            sec = (time.perf_counter() - t0) / 2
+            ref = xq if xq is not None else x_ref
+            out[label][pname] = {"sec_per_solve": sec, "it": info.it, "flag": info.flag,
+                                 "escalated": info.escalated,
+                                 "x_rel_vs_qr": ((x - ref).norm() / ref.norm()).item()}
+            log(f"{label} {pname}: {sec * 1e3:.2f} ms/solve, it={info.it}, flag={info.flag}")
+    return out
 
 
 def run_sparse(args, skl, torch, W):
@@ -489,6 +522,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-escalation-case", action="store_true",
                     help="sparse workload: skip the logspace(0,-10) FP64-fallback check")
+    ap.add_argument("--no-other-solvers", action="store_true",
+                    help="dense workloads: skip the noCPQR and Blendenpik entry points")
     ap.add_argument("--no-reference", action="store_true",
                     help="skip the dense Householder QR solution that x is compared with "
                          "(x_rel_vs_qr is then null)")
@@ -594,6 +629,7 @@ def main():
                   "x_rel_vs_qr": mixed["x_rel_vs_qr"],
                   "speedup_vs_fp64": f64["sec_per_solve"] / mixed["sec_per_solve"],
                   "x_rel_vs_fp64": res["x_rel_mixed_vs_fp64"], "phases_s": mixed["phases"]},
+        **({"other_entry_points": res["other"]} if "other" in res else {}),
         "roofline": {"bound": "hbm", "kernel": P["name"], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "avg_launch_ms": P["avg_ms"],
