@@ -500,6 +500,47 @@ __global__ __launch_bounds__(256) void k_spmv_n(long n, const int32_t *__restric
     if (threadIdx.x == 0) Sb[blockIdx.x] = sq;
 }
 
+// The same product with z staged in LDS: a persistent grid of workgroups,
+// each copies z (d values) once and then serves its rows' gathers from LDS
+// instead of the cache hierarchy; Sb keeps one partial per SP_ROWS rows.
+template <int LPR>
+__global__ __launch_bounds__(256) void k_spmv_n_lds(long n, long d, int nrb,
+                                                    const int32_t *__restrict__ ptr,
+                                                    const int32_t *__restrict__ col,
+                                                    const double *__restrict__ val,
+                                                    const double *__restrict__ z,
+                                                    double *__restrict__ ut,
+                                                    double *__restrict__ Sb,
+                                                    LsqrState *__restrict__ st, int init) {
+    if (st->done) return;
+    extern __shared__ double zs[];
+    for (long j = threadIdx.x; j < d; j += 256) zs[j] = z[j];
+    __syncthreads();
+    const double coef = init ? 0.0 : st->coef;
+    const int sub = threadIdx.x % LPR, grp = threadIdx.x / LPR;
+    constexpr int RPI = 256 / LPR; // rows per pass
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+        double sq = 0.0;
+        for (int q = 0; q < SP_ROWS / RPI; ++q) {
+            long i = (long)rb * SP_ROWS + q * RPI + grp;
+            double s = 0.0;
+            if (i < n) {
+                int p0 = ptr[i], p1 = ptr[i + 1];
+                for (int p = p0 + sub; p < p1; p += LPR) s += val[p] * zs[col[p]];
+            }
+#pragma unroll
+            for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+            if (sub == 0 && i < n) {
+                double vv = s - coef * ut[i];
+                ut[i] = vv;
+                sq += vv * vv;
+            }
+        }
+        sq = block_sum<256>(sq);
+        if (threadIdx.x == 0) Sb[rb] = sq;
+    }
+}
+
 // Pt[j] = sum_p val[p] ut[row[p]] over CSC column j (one wave per column)
 __global__ __launch_bounds__(256) void k_csc_t(long d, const int32_t *__restrict__ ptr,
                                                const int32_t *__restrict__ row,
@@ -528,6 +569,7 @@ struct LsqrPlan {
     int nsb;           // squares partials
     bool sparse;
     bool fused = false; // one pass over A per iteration (k_fused_dense)
+    int spg = 0;        // sparse A z with z in LDS: workgroups (0: z from memory)
     int G = 0, nq = 0;  // fused: workgroups, columns per thread in pass 2
 };
 
@@ -564,7 +606,14 @@ static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
     long cchm = std::max<long>(1, (target + q.nrb_m - 1) / q.nrb_m);
     q.cpw_m = (int)std::max<long>(4, (p.r + cchm - 1) / cchm);
     q.nsb = (int)((p.n + (q.sparse ? SP_ROWS : RN_PER) - 1) / (q.sparse ? SP_ROWS : RN_PER));
-    if (q.sparse) q.nrb_a = 1;
+    if (q.sparse) {
+        q.nrb_a = 1;
+        // z in LDS when it fits four workgroups per CU (or at least one)
+        const size_t zb = sizeof(double) * (size_t)p.d;
+        const int per_cu = zb <= 40960 ? 4 : zb <= 81920 ? 2 : zb <= 163840 ? 1 : 0;
+        const char *e = getenv("SKL_SPMV_LDS");
+        if (per_cu > 0 && !(e && e[0] == '0')) q.spg = std::min(q.nsb, per_cu * num_cu);
+    }
     if (fused_wanted(p)) {
         q.fused = true;
         q.G = num_cu;
@@ -684,8 +733,13 @@ static void launch_iteration(const LsqrProblem &p, const LsqrPlan &q, const Lsqr
                   p.A, p.lda, B.z, q.cw_a, B.Pn, 0, B.st);
         k_reduce_n<<<(unsigned)q.nsb, 256, 0, s>>>(q.n, q.nck_a, B.Pn, B.ut, B.Sb, B.st, 0);
     } else {
-        launch_ev(k_spmv_n<8>, dim3((unsigned)q.nsb), dim3(256), 0, s, evi(ev, 0), evi(ev, 1), q.n,
-                  p.csr_ptr, p.csr_col, p.csr_val, B.z, B.ut, B.Sb, B.st, 0);
+        if (q.spg > 0)
+            launch_ev(k_spmv_n_lds<8>, dim3((unsigned)q.spg), dim3(256),
+                      (uint32_t)(sizeof(double) * q.d), s, evi(ev, 0), evi(ev, 1), q.n, q.d, q.nsb,
+                      p.csr_ptr, p.csr_col, p.csr_val, B.z, B.ut, B.Sb, B.st, 0);
+        else
+            launch_ev(k_spmv_n<8>, dim3((unsigned)q.nsb), dim3(256), 0, s, evi(ev, 0), evi(ev, 1),
+                      q.n, p.csr_ptr, p.csr_col, p.csr_val, B.z, B.ut, B.Sb, B.st, 0);
     }
     launch_op_t(p, q, B, s, 0, ev);
 }
@@ -756,6 +810,10 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
     Context &C = ctx();
     hipStream_t s = C.stream;
     LsqrPlan q = make_plan(p, C.num_cu);
+    if (q.spg > 0 && sizeof(double) * q.d > 65536)
+        SKL_HIP(hipFuncSetAttribute((const void *)k_spmv_n_lds<8>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(sizeof(double) * q.d)));
     LsqrBufs B;
     B.ut = ar.alloc_n<double>((size_t)q.n);
     B.v = ar.alloc_n<double>((size_t)q.r);
@@ -818,7 +876,8 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
                  (long)(intptr_t)B.Pn, (long)(intptr_t)B.Pt, (long)(intptr_t)B.Pm,
                  (long)(intptr_t)B.Einv, (long)(intptr_t)B.z, (long)(intptr_t)B.w,
                  (long)(intptr_t)B.t, (long)(intptr_t)B.Sb, (long)(intptr_t)B.Pz, B.hist_cap,
-                 q.fused ? 1L : 0L, chunk, (long)(intptr_t)s, prof ? 1L : 0L};
+                 q.fused ? 1L : 0L, chunk, (long)(intptr_t)s, prof ? 1L : 0L,
+                 (long)q.spg};
         auto it = g_graphs.find(key);
         if (it != g_graphs.end()) {
             exec = it->second;
