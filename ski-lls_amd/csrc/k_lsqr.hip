@@ -1,10 +1,13 @@
 // k_lsqr.hip -- LSQR (src/IterSolver.cpp:160-284) on the preconditioned
 // operator A(:, E[0:r]) M with M = R^{-1}, in FP64 on the device.
 //
-// Each iteration is seven kernels that read A once for A z and once for
-// A^T u (column-major A is streamed with coalesced 16-byte loads), apply M
+// Each iteration is a handful of kernels that form A z and A^T u, apply M
 // and M^T as triangular GEMVs, and keep every scalar of the Golub-Kahan /
-// Givens recurrences in device memory.  Partial sums are reduced in a fixed
+// Givens recurrences in device memory.  Dense A (d <= 8192, when the device
+// has room for a second copy) is read once per iteration, from a row-major
+// copy made at the start of the solve, by one kernel that forms u = A z and
+// A^T u together; otherwise A z and A^T u are two passes over the
+// column-major A (coalesced 16-byte loads) or over CSR and CSC for sparse A.  Partial sums are reduced in a fixed
 // order, so a solve is bitwise reproducible run to run.  The iteration
 // sequence is captured once in a hipGraph and replayed until the device-side
 // convergence flag is set; kernels of iterations past convergence exit at
@@ -139,129 +142,132 @@ __global__ __launch_bounds__(GT_THREADS) void k_gemv_t_part(
     }
 }
 
-// ---- one pass over A: u <- A z - coef u and q <- A^T u together -----------------
-// A persistent 1024-thread workgroup per CU walks 4-row tiles of the
-// column-major A, holding each tile in registers: lane l of wave w takes row
-// l % 4 of the columns cs + 256 k (k < NC), cs = l / 4 + 16 w, so a quad of
-// lanes reads one 32-byte run of a column.  Tiles are loaded one ahead of use.
-// Per tile: the row sums of A z are reduced over the 16 quads of a wave
-// (shuffles) and over the 16 waves (LDS, one barrier, two alternating
-// buffers); every thread then forms u of its row itself, and the column sums
-// A(tile,:)^T u are reduced inside each quad by halving exchanges that leave
-// lane r of the quad with the columns k in [r NC/4, (r+1) NC/4), accumulated
-// in registers across tiles.  The tiles of one XCD (workgroup id mod 8) form
-// one contiguous band of rows, so the four 4-row tiles of a 128-byte line are
-// read at about the same time through the same L2.
-// Per workgroup the partial A^T u goes to Q[wg][:] and the sum of squares of u
-// to Sb[wg].
-constexpr int FU_T = 1024, FU_R = 4, FU_SLOTS = FU_T / FU_R;
+// ---- one pass over A: u <- A z - coef u and A^T u together ---------------------
+// The pass reads a row-major copy AT of A (rows of ldt doubles, ldt even, the
+// padding column zero), made once per solve by k_to_rows.  A 256-thread
+// workgroup takes the rows blockIdx.x + k G; its four waves share each row:
+// thread t holds the column pairs 2t + 512c (c < NC) of the row in registers,
+// so one load instruction of a wave reads 1 KB of the row, and the next row is
+// loaded while this one is used.  Per row the dot with z (z in LDS, zero past
+// d) is reduced over the wave by shuffles and over the four waves through LDS
+// (one barrier, two alternating slots); every thread then forms u of the row
+// and adds u times its columns to its register sums of A^T u.  Per workgroup
+// the partial A^T u goes to Q[wg][:] and the sum of squares of u to Sb[wg].
+constexpr int FR_T = 256, FR_W = FR_T / 64, FR_COLS = 2 * FR_T;
 
 template <int NC>
-__global__ __launch_bounds__(FU_T) void k_fused_dense(long n, long d, const double *__restrict__ A,
-                                                      long lda, const double *__restrict__ z,
-                                                      double *__restrict__ ut,
-                                                      double *__restrict__ Sb,
-                                                      double *__restrict__ Q,
-                                                      const LsqrState *__restrict__ st) {
-    static_assert(NC % 4 == 0, "columns per thread must split over a quad");
+__global__ __launch_bounds__(FR_T) void k_fused_rows(long n, long d, const double *__restrict__ AT,
+                                                     long ldt, const double *__restrict__ z,
+                                                     double *__restrict__ ut,
+                                                     double *__restrict__ Sb,
+                                                     double *__restrict__ Q,
+                                                     const LsqrState *__restrict__ st) {
     if (st->done) return;
     extern __shared__ double fsm[];
-    double *zs = fsm;                    // d
-    double *red = fsm + d;               // [2][16 waves][4 rows]
+    double *zs = fsm;                // NC * FR_COLS
+    double *red = fsm + NC * FR_COLS; // [2][FR_W]
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int r = lane & 3, cs = (lane >> 2) + 16 * w;
-    for (long j = tid; j < d; j += FU_T) zs[j] = z[j];
+    for (int j = tid; j < NC * FR_COLS; j += FR_T) zs[j] = j < d ? z[j] : 0.0;
     const double coef = st->coef;
-    // this workgroup's tiles: band of XCD x = wg % 8, local index wg / 8
-    const int G = gridDim.x, x = blockIdx.x & 7, nx = min(8, G);
-    const int gl = blockIdx.x >> 3, Gx = (G - x + 7) / 8; // workgroups on this XCD
-    const long ntiles = (n + FU_R - 1) / FU_R;
-    const long band = (ntiles + nx - 1) / nx;
-    const long t_beg = (long)x * band, t_end = min(ntiles, t_beg + band);
-    bool colok[NC];
-    __syncthreads();
+    // loads are unconditional (a load under a branch makes every wait on a
+    // row wait for the prefetched one too): column pairs past d read pair 0
+    // and meet zeros of z; rows past n read row n - 1 and are never used
+    int off[NC];
+    double2 q[NC];
 #pragma unroll
-    for (int k = 0; k < NC; ++k) colok[k] = cs + (long)FU_SLOTS * k < d;
-    const double *zq = zs + cs;
-    double q[NC / 4];
-#pragma unroll
-    for (int k = 0; k < NC / 4; ++k) q[k] = 0.0;
+    for (int c = 0; c < NC; ++c) {
+        off[c] = 2L * tid + (long)FR_COLS * c < d ? tid + FR_T * c : 0;
+        q[c] = make_double2(0.0, 0.0);
+    }
     double sq = 0.0;
-    auto load = [&](long t, double (&a)[NC], double &uo) {
-        const long i = t * FU_R + r;
-        const bool rv = (t < t_end) && (i < n);
-        const double *ap = A + (rv ? i : 0) + (long)cs * lda;
+    __syncthreads();
+    const long G = gridDim.x;
+    auto load = [&](long i, double2(&a)[NC], double &uo) {
+        const long ic = i < n ? i : n - 1;
+        const double2 *ap = (const double2 *)(AT + ic * ldt);
 #pragma unroll
-        for (int k = 0; k < NC; ++k) a[k] = (rv && colok[k]) ? ap[(long)FU_SLOTS * k * lda] : 0.0;
-        uo = rv ? ut[i] : 0.0;
+        for (int c = 0; c < NC; ++c) a[c] = ap[off[c]];
+        uo = ut[ic];
     };
-    double a0[NC], a1[NC], u0 = 0.0, u1 = 0.0;
-    long t = t_beg + gl;
-    load(t, a0, u0);
-    int parity = 0;
-    // one tile: a holds it, uo the old u of this thread's row
-    auto tile = [&](long tt, double (&a)[NC], double uo) {
-        const long i = tt * FU_R + r;
-        double p = 0.0;
+    int par = 0;
+    auto row = [&](long i, const double2(&a)[NC], double uo) {
+        const double2 *z2 = (const double2 *)zs + tid;
+        double p0 = 0.0, p1 = 0.0;
 #pragma unroll
-        for (int k = 0; k < NC; ++k)
-            if (colok[k]) p += a[k] * zq[FU_SLOTS * k];
-        // over the 16 quads of the wave (lane bits 2..5)
-        p += __shfl_xor(p, 4, 64);
-        p += __shfl_xor(p, 8, 64);
-        p += __shfl_xor(p, 16, 64);
-        p += __shfl_xor(p, 32, 64);
-        double *rb = red + parity * (16 * FU_R);
-        if (lane < FU_R) rb[w * FU_R + lane] = p;
+        for (int c = 0; c < NC; ++c) {
+            const double2 zz = z2[FR_T * c];
+            p0 = fma(a[c].x, zz.x, p0);
+            p1 = fma(a[c].y, zz.y, p1);
+        }
+        const double p = wave_sum(p0 + p1);
+        double *rb = red + par * FR_W;
+        if (lane == 0) rb[w] = p;
         __syncthreads();
-        double s = 0.0;
-        for (int ww = 0; ww < FU_T / 64; ++ww) s += rb[ww * FU_R + r];
-        const bool rv = i < n;
-        const double uv = rv ? s - coef * uo : 0.0;
-        if (w == 0 && lane < FU_R && rv) {
+        const double s = (rb[0] + rb[1]) + (rb[2] + rb[3]);
+        const double uv = i < n ? s - coef * uo : 0.0;
+        if (tid == 0 && i < n) {
             ut[i] = uv;
             sq += uv * uv;
         }
-        // column sums over the quad: halving exchanges on lane bits 1 and 0
-        double c[NC];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) c[k] = a[k] * uv;
-        {
-            const bool up = (lane & 2) != 0;
-#pragma unroll
-            for (int k = 0; k < NC / 2; ++k) {
-                double send = up ? c[k] : c[k + NC / 2];
-                double keep = up ? c[k + NC / 2] : c[k];
-                c[k] = keep + __shfl_xor(send, 2, 64);
-            }
+        for (int c = 0; c < NC; ++c) {
+            q[c].x = fma(a[c].x, uv, q[c].x);
+            q[c].y = fma(a[c].y, uv, q[c].y);
         }
-        {
-            const bool up = (lane & 1) != 0;
-#pragma unroll
-            for (int k = 0; k < NC / 4; ++k) {
-                double send = up ? c[k] : c[k + NC / 4];
-                double keep = up ? c[k + NC / 4] : c[k];
-                q[k] += keep + __shfl_xor(send, 1, 64);
-            }
-        }
-        parity ^= 1;
+        par ^= 1;
     };
-    for (; t < t_end; t += 2L * Gx) {
-        load(t + Gx, a1, u1);
-        tile(t, a0, u0);
-        if (t + Gx >= t_end) break;
-        load(t + 2L * Gx, a0, u0);
-        tile(t + Gx, a1, u1);
+    double2 a0[NC], a1[NC];
+    double u0, u1;
+    long i = blockIdx.x;
+    load(i, a0, u0);
+    for (; i < n; i += 2 * G) {
+        load(i + G, a1, u1);
+        row(i, a0, u0);
+        if (i + G >= n) break;
+        load(i + 2 * G, a0, u0);
+        row(i + G, a1, u1);
     }
-    // lane r of a quad holds columns k = r NC/4 + j
+    double *qo = Q + (long)blockIdx.x * d;
 #pragma unroll
-    for (int j = 0; j < NC / 4; ++j) {
-        const int k = r * (NC / 4) + j;
-        const long c = cs + (long)FU_SLOTS * k;
-        if (c < d) Q[(long)blockIdx.x * d + c] = q[j];
+    for (int c = 0; c < NC; ++c) {
+        const long j = 2L * tid + (long)FR_COLS * c;
+        if (j < d) qo[j] = q[c].x;
+        if (j + 1 < d) qo[j + 1] = q[c].y;
     }
-    sq = block_sum<FU_T>(sq);
     if (tid == 0) Sb[blockIdx.x] = sq;
+}
+
+// AT[i * ldt + j] = A[i + j * lda] for i < n, j < ldt (zero for d <= j < ldt),
+// through 64 x 64 tiles in LDS: the reads run down columns, the writes along rows
+__global__ __launch_bounds__(256) void k_to_rows(long n, long d, const double *__restrict__ A,
+                                                 long lda, double *__restrict__ AT, long ldt) {
+    __shared__ double tile[64][65];
+    const long i0 = (long)blockIdx.x * 64, j0 = (long)blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int jj = ty; jj < 64; jj += 4) {
+        const long i = i0 + tx, j = j0 + jj;
+        tile[jj][tx] = (i < n && j < d) ? A[i + j * lda] : 0.0;
+    }
+    __syncthreads();
+    for (int ii = ty; ii < 64; ii += 4) {
+        const long i = i0 + ii, j = j0 + tx;
+        if (i < n && j < ldt) AT[i * ldt + j] = tile[tx][ii];
+    }
+}
+
+// F[g * d + j] = sum of P[p * d + j] over the partials p of group g (FOLD_GRP
+// consecutive ones, the last group ragged): fewer partials for k_reduce_t
+constexpr int FOLD_GRP = 32;
+__global__ __launch_bounds__(256) void k_fold_parts(long d, int np, const double *__restrict__ P,
+                                                    double *__restrict__ F,
+                                                    const LsqrState *__restrict__ st) {
+    if (st->done) return;
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= d) return;
+    const int p0 = blockIdx.y * FOLD_GRP, p1 = min(np, p0 + FOLD_GRP);
+    double s = 0.0;
+    for (int p = p0; p < p1; ++p) s += P[(long)p * d + j];
+    F[(long)blockIdx.y * d + j] = s;
 }
 
 // ---- reductions / scatter ----------------------------------------------------
@@ -568,16 +574,19 @@ struct LsqrPlan {
     int nrb_m, cpw_m;  // M^T t
     int nsb;           // squares partials
     bool sparse;
-    bool fused = false; // one pass over A per iteration (k_fused_dense)
+    bool fused = false; // one pass over a row-major copy of A per iteration (k_fused_rows)
     int spg = 0;        // sparse A z with z in LDS: workgroups (0: z from memory)
-    int G = 0, nq = 0;  // fused: workgroups, columns per thread in pass 2
+    int G = 0, nc = 0;  // fused: workgroups, column pairs per thread (over 512 columns)
+    int nfold = 0;      // fused: partial groups after k_fold_parts
+    long ldt = 0;       // fused: row stride of the copy
 };
 
-// the one-pass kernel holds d / 256 columns of a tile per thread in registers
+// the one-pass kernel holds up to 16 column pairs of a row per thread in
+// registers (d <= 8192); small problems keep the two-pass kernels
 static bool fused_wanted(const LsqrProblem &p) {
-    if (p.csr_ptr || p.d > 16 * FU_SLOTS || p.n < 64 * FU_R * 8) return false;
+    if (p.csr_ptr || p.d > 16 * FR_COLS || p.n < 4096) return false;
     const char *e = getenv("SKL_LSQR_FUSED");
-    return e && e[0] == '1';
+    return !(e && e[0] == '0');
 }
 
 static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
@@ -616,15 +625,24 @@ static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
     }
     if (fused_wanted(p)) {
         q.fused = true;
-        q.G = num_cu;
-        long need = (p.d + FU_SLOTS - 1) / FU_SLOTS;
-        q.nq = need <= 4 ? 4 : need <= 8 ? 8 : 16;
+        const long need = (p.d + FR_COLS - 1) / FR_COLS;
+        q.nc = need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : need <= 8 ? 8 : 16;
+        // two workgroups per CU (the registers allow four up to nc = 8, but
+        // two keep as many bytes in flight and halve the partials: 0.222 vs
+        // 0.236 ms per pass at 40000 x 4000)
+        int wpc = 2;
+        const char *e = getenv("SKL_FUSED_WPC");
+        if (e && atoi(e) > 0) wpc = atoi(e);
+        q.G = (int)std::min<long>((long)wpc * num_cu, p.n);
+        q.nfold = (q.G + FOLD_GRP - 1) / FOLD_GRP;
+        q.ldt = p.d + (p.d & 1);
     }
     return q;
 }
 
 struct LsqrBufs {
     double *ut, *v, *w, *x, *z, *Pz, *Pn, *Sb, *Pt, *t, *Pm, *hist;
+    double *AT, *Pf; // fused: row-major copy of A, folded partials of A^T u
     long hist_cap;
     int32_t *Einv;
     LsqrState *st;
@@ -646,17 +664,20 @@ static void launch_ev(void (*k)(P...), dim3 g, dim3 b, uint32_t sh, hipStream_t 
         hipLaunchKernelGGL(k, g, b, sh, s, args...);
 }
 
-// after_fused: A^T u already sits in Pt as per-workgroup partials (q.G of
-// them) together with the squares in Sb; otherwise A^T u is computed here.
+// after_fused: A^T u already sits in Pf as folded per-workgroup partials
+// (q.nfold of them) and the squares in Sb (q.G); otherwise A^T u is computed
+// here.
 static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs &B,
                         hipStream_t s, int init, hipEvent_t *ev = nullptr,
                         bool after_fused = false) {
     hipEvent_t e2 = ev ? ev[2] : nullptr, e3 = ev ? ev[3] : nullptr;
     hipEvent_t e6 = ev ? ev[6] : nullptr, e7 = ev ? ev[7] : nullptr;
     int nrb = q.nrb_a, nsb = q.nsb;
+    const double *Pt = B.Pt;
     if (after_fused) {
-        nrb = q.G;
+        nrb = q.nfold;
         nsb = q.G;
+        Pt = B.Pf;
     } else {
         // Pt = A^T ut
         if (!q.sparse) {
@@ -669,7 +690,7 @@ static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs 
         }
     }
     // t = Pt(E) / beta
-    k_reduce_t<<<(unsigned)((q.r + RED_OUT - 1) / RED_OUT), 256, 0, s>>>(q.d, q.r, nrb, B.Pt, p.E, B.Sb, nsb,
+    k_reduce_t<<<(unsigned)((q.r + RED_OUT - 1) / RED_OUT), 256, 0, s>>>(q.d, q.r, nrb, Pt, p.E, B.Sb, nsb,
                                                              B.t, B.st);
     // Pm = M^T t
     {
@@ -681,15 +702,28 @@ static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs 
                                        B.hist, B.hist_cap, init);
 }
 
+template <int NC> static void launch_fused_nc(const LsqrProblem &p, const LsqrPlan &q,
+                                              const LsqrBufs &B, hipStream_t s, hipEvent_t e0,
+                                              hipEvent_t e1) {
+    const size_t sh = sizeof(double) * ((size_t)NC * FR_COLS + 2 * FR_W);
+    if (sh > 65536)
+        SKL_HIP(hipFuncSetAttribute((const void *)k_fused_rows<NC>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    launch_ev(k_fused_rows<NC>, dim3(q.G), dim3(FR_T), (uint32_t)sh, s, e0, e1, q.n, q.d, B.AT,
+              q.ldt, B.z, B.ut, B.Sb, B.Pt, B.st);
+}
+
 static void launch_fused(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs &B,
                          hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
-    size_t sh = sizeof(double) * ((size_t)q.d + 2 * 16 * FU_R);
-    auto fn = q.nq == 4 ? k_fused_dense<4> : q.nq == 8 ? k_fused_dense<8> : k_fused_dense<16>;
-    if (sh > 65536)
-        SKL_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)sh));
-    launch_ev(fn, dim3(q.G), dim3(FU_T), (uint32_t)sh, s, e0, e1, q.n, q.d, p.A, p.lda, B.z, B.ut,
-              B.Sb, B.Pt, B.st);
+    switch (q.nc) {
+    case 1: launch_fused_nc<1>(p, q, B, s, e0, e1); break;
+    case 2: launch_fused_nc<2>(p, q, B, s, e0, e1); break;
+    case 4: launch_fused_nc<4>(p, q, B, s, e0, e1); break;
+    case 8: launch_fused_nc<8>(p, q, B, s, e0, e1); break;
+    default: launch_fused_nc<16>(p, q, B, s, e0, e1); break;
+    }
+    dim3 g((unsigned)((q.d + 255) / 256), (unsigned)q.nfold);
+    k_fold_parts<<<g, 256, 0, s>>>(q.d, q.G, B.Pt, B.Pf, B.st);
 }
 
 // Optional kernel timing: events around the four operator products of one
@@ -810,6 +844,15 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
     Context &C = ctx();
     hipStream_t s = C.stream;
     LsqrPlan q = make_plan(p, C.num_cu);
+    if (q.fused) {
+        // the row-major copy doubles A's footprint: only when the device has
+        // room for it (free memory plus what the arena holds unused)
+        size_t fr = 0, tot = 0;
+        SKL_HIP(hipMemGetInfo(&fr, &tot));
+        const double need = 8.0 * (double)q.n * (double)q.ldt;
+        const double avail = (double)fr + (double)(ar.capacity() - ar.used());
+        if (need * 1.25 + 256.0 * (1 << 20) > avail) q.fused = false;
+    }
     if (q.spg > 0 && sizeof(double) * q.d > 65536)
         SKL_HIP(hipFuncSetAttribute((const void *)k_spmv_n_lds<8>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -826,6 +869,13 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
     B.Pt = ar.alloc_n<double>((size_t)std::max(q.nrb_a, q.G) * q.d);
     B.t = ar.alloc_n<double>((size_t)q.r);
     B.Pm = ar.alloc_n<double>((size_t)q.nrb_m * q.r);
+    B.AT = B.Pf = nullptr;
+    if (q.fused) {
+        B.AT = ar.alloc_n<double>((size_t)q.n * q.ldt);
+        B.Pf = ar.alloc_n<double>((size_t)q.nfold * q.d);
+        dim3 g((unsigned)((q.n + 63) / 64), (unsigned)((q.ldt + 63) / 64));
+        k_to_rows<<<g, 256, 0, s>>>(q.n, q.d, p.A, p.lda, B.AT, q.ldt);
+    }
     B.st = ar.alloc_n<LsqrState>(1);
     long hl = hist ? std::min(hist_len, max_it) : 0;
     B.hist = hl > 0 ? ar.alloc_n<double>((size_t)5 * hl) : nullptr;
@@ -877,7 +927,7 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
                  (long)(intptr_t)B.Einv, (long)(intptr_t)B.z, (long)(intptr_t)B.w,
                  (long)(intptr_t)B.t, (long)(intptr_t)B.Sb, (long)(intptr_t)B.Pz, B.hist_cap,
                  q.fused ? 1L : 0L, chunk, (long)(intptr_t)s, prof ? 1L : 0L,
-                 (long)q.spg};
+                 (long)q.spg, (long)(intptr_t)B.AT, (long)(intptr_t)B.Pf, (long)q.G};
         auto it = g_graphs.find(key);
         if (it != g_graphs.end()) {
             exec = it->second;

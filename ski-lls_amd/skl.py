@@ -323,7 +323,7 @@ class Server:
 
 
 # ---- kernel timing --------------------------------------------------------------
-PROFILE_SLOTS = {0: "A z (gemv_n / spmv_n)", 1: "A^T u (gemv_t / csc_t)", 2: "M v (trmv_n)",
+PROFILE_SLOTS = {0: "A z (gemv_n / spmv_n; fused_rows: A z with A^T u)", 1: "A^T u (gemv_t / csc_t)", 2: "M v (trmv_n)",
                  3: "M^T t (trmv_t)"}
 
 

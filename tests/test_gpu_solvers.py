@@ -296,11 +296,13 @@ def test_mixed_precision_keeps_fp32_when_unit_diagonal_dominates(skl):
     assert rel(x, x64) < 1e-8
 
 
-@pytest.mark.parametrize("n,d", [(6000, 300), (5003, 1500), (20000, 2100)])
+@pytest.mark.parametrize("n,d", [(6000, 300), (5003, 1500), (20000, 2100), (9001, 5001)])
 def test_fused_lsqr_pass_matches_two_pass(skl, O, monkeypatch, n, d):
-    """The one-pass LSQR kernel (A z and A^T u from one read of A) against the
-    two-pass kernels and the oracle; shapes cover one, two and three column
-    sums per thread and a ragged last row tile."""
+    """The one-pass LSQR kernel (A z and A^T u from one read of a row-major
+    copy of A) against the two-pass kernels and the oracle; the widths take
+    1, 4, 8 and 16 column pairs per thread (the last one odd, with a padding
+    column in the copy), and the row counts leave workgroups with unequal
+    row counts."""
     A, b = c1_problem(11, n=n, d=d)
     monkeypatch.setenv("SKL_LSQR_FUSED", "0")
     x2, i2 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=4)
