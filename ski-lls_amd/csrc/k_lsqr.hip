@@ -589,6 +589,16 @@ static bool fused_wanted(const LsqrProblem &p) {
     return !(e && e[0] == '0');
 }
 
+// the row-major copy doubles A's footprint: only when the device has room for
+// it (free memory plus what the arena holds unused)
+static bool rows_fit(const LsqrProblem &p, Arena &ar) {
+    size_t fr = 0, tot = 0;
+    SKL_HIP(hipMemGetInfo(&fr, &tot));
+    const double need = 8.0 * (double)p.n * (double)(p.d + (p.d & 1));
+    const double avail = (double)fr + (double)(ar.capacity() - ar.used());
+    return need * 1.25 + 256.0 * (1 << 20) <= avail;
+}
+
 static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
     LsqrPlan q;
     q.n = p.n;
@@ -627,10 +637,10 @@ static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
         q.fused = true;
         const long need = (p.d + FR_COLS - 1) / FR_COLS;
         q.nc = need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : need <= 8 ? 8 : 16;
-        // two workgroups per CU (the registers allow four up to nc = 8, but
-        // two keep as many bytes in flight and halve the partials: 0.222 vs
-        // 0.236 ms per pass at 40000 x 4000)
-        int wpc = 2;
+        // one workgroup per CU (the registers allow four up to nc = 8, but one
+        // keeps as many bytes in flight with the fewest partials: 0.210, 0.222
+        // and 0.236 ms per pass at 40000 x 4000 for one, two and four)
+        int wpc = 1;
         const char *e = getenv("SKL_FUSED_WPC");
         if (e && atoi(e) > 0) wpc = atoi(e);
         q.G = (int)std::min<long>((long)wpc * num_cu, p.n);
@@ -838,21 +848,31 @@ void lsqr_release_graphs() {
     g_graphs.clear();
 }
 
+bool lsqr_start_rows(LsqrProblem &p, Arena &ar) {
+    if (!fused_wanted(p) || !rows_fit(p, ar)) return false;
+    Context &C = ctx();
+    C.make_aux();
+    const long ldt = p.d + (p.d & 1);
+    double *AT = ar.alloc_n<double>((size_t)p.n * ldt);
+    SKL_HIP(hipEventRecord(C.aux_ev[0], C.stream));
+    SKL_HIP(hipStreamWaitEvent(C.aux, C.aux_ev[0], 0));
+    dim3 g((unsigned)((p.n + 63) / 64), (unsigned)((ldt + 63) / 64));
+    k_to_rows<<<g, 256, 0, C.aux>>>(p.n, p.d, p.A, p.lda, AT, ldt);
+    SKL_CHECK_LAUNCH();
+    SKL_HIP(hipEventRecord(C.aux_ev[1], C.aux));
+    p.AT = AT;
+    p.ldt = ldt;
+    p.AT_ready = C.aux_ev[1];
+    return true;
+}
+
 LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long max_it,
                       double *y, bool use_graph, int chunk, skl_lsqr_step *hist, long hist_len,
                       Arena &ar) {
     Context &C = ctx();
     hipStream_t s = C.stream;
     LsqrPlan q = make_plan(p, C.num_cu);
-    if (q.fused) {
-        // the row-major copy doubles A's footprint: only when the device has
-        // room for it (free memory plus what the arena holds unused)
-        size_t fr = 0, tot = 0;
-        SKL_HIP(hipMemGetInfo(&fr, &tot));
-        const double need = 8.0 * (double)q.n * (double)q.ldt;
-        const double avail = (double)fr + (double)(ar.capacity() - ar.used());
-        if (need * 1.25 + 256.0 * (1 << 20) > avail) q.fused = false;
-    }
+    if (q.fused && !p.AT && !rows_fit(p, ar)) q.fused = false;
     if (q.spg > 0 && sizeof(double) * q.d > 65536)
         SKL_HIP(hipFuncSetAttribute((const void *)k_spmv_n_lds<8>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -871,10 +891,17 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
     B.Pm = ar.alloc_n<double>((size_t)q.nrb_m * q.r);
     B.AT = B.Pf = nullptr;
     if (q.fused) {
-        B.AT = ar.alloc_n<double>((size_t)q.n * q.ldt);
+        if (p.AT) {
+            // made on the aux stream (lsqr_start_rows) with the same row stride
+            SKL_REQUIRE(p.ldt == q.ldt, SKL_E_ARG, "row-major copy of A has the wrong stride");
+            B.AT = const_cast<double *>(p.AT);
+            SKL_HIP(hipStreamWaitEvent(s, p.AT_ready, 0));
+        } else {
+            B.AT = ar.alloc_n<double>((size_t)q.n * q.ldt);
+            dim3 g((unsigned)((q.n + 63) / 64), (unsigned)((q.ldt + 63) / 64));
+            k_to_rows<<<g, 256, 0, s>>>(q.n, q.d, p.A, p.lda, B.AT, q.ldt);
+        }
         B.Pf = ar.alloc_n<double>((size_t)q.nfold * q.d);
-        dim3 g((unsigned)((q.n + 63) / 64), (unsigned)((q.ldt + 63) / 64));
-        k_to_rows<<<g, 256, 0, s>>>(q.n, q.d, p.A, p.lda, B.AT, q.ldt);
     }
     B.st = ar.alloc_n<LsqrState>(1);
     long hl = hist ? std::min(hist_len, max_it) : 0;

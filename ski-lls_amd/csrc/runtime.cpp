@@ -198,6 +198,12 @@ void Context::make_side() {
     for (auto &e : ev) SKL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
+void Context::make_aux() {
+    if (aux) return;
+    SKL_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+    for (auto &e : aux_ev) SKL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
 OnSideStream::OnSideStream() {
     Context &c = ctx();
     c.make_side();
@@ -218,6 +224,7 @@ void release_thread_context() {
     if (!t_ctx) return;
     (void)hipStreamSynchronize(t_ctx->stream);
     if (t_ctx->side) (void)hipStreamSynchronize(t_ctx->side);
+    if (t_ctx->aux) (void)hipStreamSynchronize(t_ctx->aux);
     lsqr_release_graphs();
     t_ctx->arena.release();
     if (t_ctx->ws) (void)hipFree(t_ctx->ws);
@@ -225,6 +232,9 @@ void release_thread_context() {
     for (auto &e : t_ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (t_ctx->side) (void)hipStreamDestroy(t_ctx->side);
+    for (auto &e : t_ctx->aux_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (t_ctx->aux) (void)hipStreamDestroy(t_ctx->aux);
     if (t_ctx->pinned) (void)hipHostFree(t_ctx->pinned);
     (void)hipStreamDestroy(t_ctx->stream);
     delete t_ctx;

@@ -86,6 +86,12 @@ struct Context {
     size_t ws_side_cap = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
     void make_side();
+    // third stream for the row-major copy of A that the one-pass LSQR kernel
+    // reads, made while the QR runs on the other two; its events mark the
+    // copy's start (after A is on the device) and end
+    hipStream_t aux = nullptr;
+    hipEvent_t aux_ev[2] = {nullptr, nullptr};
+    void make_aux();
 };
 
 // While alive, ctx().stream (and the split-K scratch) is the context's side
@@ -221,6 +227,12 @@ struct LsqrProblem {
     const int32_t *E = nullptr; // r entries or null (identity)
     const double *M = nullptr;  // r x r upper, ldm
     long ldm = 0;
+    // dense: a row-major copy of A (row stride ldt) under way on the aux
+    // stream, complete when AT_ready fires (lsqr_start_rows); when null,
+    // lsqr_solve makes the copy itself if its one-pass kernel is used
+    const double *AT = nullptr;
+    long ldt = 0;
+    hipEvent_t AT_ready = nullptr;
 };
 struct LsqrResult {
     int flag = 0;
@@ -230,6 +242,11 @@ struct LsqrResult {
 LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long max_it,
                       double *y, bool use_graph, int chunk, skl_lsqr_step *hist,
                       long hist_len, Arena &ar);
+// Starts the row-major copy of dense A (p.n, p.d, p.A, p.lda) for the one-pass
+// LSQR kernel on the aux stream, after the work queued so far on the main
+// stream, and sets p.AT / p.ldt / p.AT_ready; false (p unchanged) when the
+// one-pass kernel will not be used.  The copy lives in the arena.
+bool lsqr_start_rows(LsqrProblem &p, Arena &ar);
 
 // small vector helpers
 void dev_gemv_n_dense(long n, long d, const double *A, long lda, const double *x, double alpha,

@@ -17,6 +17,7 @@
 //   CholeskyQR of the column-equilibrated sketch (mixed_cholqr), with a
 //   fall-back to it when that factor loses rank or conditioning.
 #include <algorithm>
+#include <functional>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -161,6 +162,8 @@ struct ArenaTrim {
     explicit ArenaTrim(Arena &a) : ar(a) {}
     ~ArenaTrim() {
         (void)hipStreamSynchronize(ctx().stream);
+        // an early exit can leave the row-major copy of A still running
+        if (ctx().aux) (void)hipStreamSynchronize(ctx().aux);
         try {
             ar.reset();
         } catch (...) {
@@ -203,11 +206,13 @@ static void dense_factor(long m, long ncol, long d, bool pivot, T *W, int32_t *j
 template <class T>
 static void dense_sketch_qr(long n, long d, long m, const double *dA, long lda, const double *db,
                             const DevPattern &dp, bool pivot, T *W, int32_t *jpvt, T *tau,
-                            double *t_sketch, double *t_qr, Arena &ar) {
+                            double *t_sketch, double *t_qr, Arena &ar,
+                            const std::function<void()> &after_sketch = {}) {
     double t0 = wall();
     dense_sketch<T>(n, d, m, dA, lda, db, dp, W, ar);
     sync();
     double t1 = wall();
+    if (after_sketch) after_sketch();
     dense_factor<T>(m, db ? d + 1 : d, d, pivot, W, jpvt, tau, ar);
     sync();
     double t2 = wall();
@@ -534,6 +539,14 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     double cond_est = 0.0;
     double *Mi = ar.alloc_n<double>((size_t)d * d);
     double *W64 = nullptr; // mixed robust path: the FP64 sketch, kept for the x0 refinement
+    // the row-major copy of A for LSQR's one-pass kernel is made on the aux
+    // stream while the QR runs (after the sketch, which is bandwidth-bound too)
+    LsqrProblem pre;
+    pre.n = n;
+    pre.d = d;
+    pre.A = dA;
+    pre.lda = lda;
+    auto start_rows = [&]() { lsqr_start_rows(pre, ar); };
     auto finish_qr = [&](auto *W) {
         // diagonal -> rank (first |R_ii| <= rcond), R to double, c
         std::vector<double> dg((size_t)d);
@@ -560,7 +573,7 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
         double *W = ar.alloc_n<double>((size_t)m * ncol);
         double *tau = ar.alloc_n<double>((size_t)d);
         dense_sketch_qr<double>(n, d, m, dA, lda, cpqr ? db : nullptr, dp, cpqr, W, jpvt, tau,
-                                &info.t_sketch, &info.t_qr, ar);
+                                &info.t_sketch, &info.t_qr, ar, start_rows);
         finish_qr(W);
     } else {
         // Optimistic FP32 preconditioner (mixed_cholqr): R from the FP32 Gram
@@ -586,6 +599,7 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
         sync();
         double t1 = wall();
         info.t_sketch += t1 - t0;
+        start_rows();
         Rd = ar.alloc_n<double>((size_t)d * d);
         std::vector<double> dg;
         double maxn = 0.0;
@@ -632,6 +646,9 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     p.E = cpqr ? jpvt : nullptr;
     p.M = Mi;
     p.ldm = rank;
+    p.AT = pre.AT;
+    p.ldt = pre.ldt;
+    p.AT_ready = pre.AT_ready;
     double *y = ar.alloc_n<double>((size_t)rank);
     if (!cpqr) {
         // LSQR on b directly, x = M y
