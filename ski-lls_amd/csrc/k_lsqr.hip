@@ -142,6 +142,53 @@ __global__ __launch_bounds__(GT_THREADS) void k_gemv_t_part(
     }
 }
 
+// ---- M^T t for upper-triangular M: one wave per (row block, column) item ------
+// P[rb * r + j] = sum over rows i in block rb (GT_RB rows), i <= j, of
+// M[i, j] t[i].  The items are only those with j >= rb * GT_RB, numbered row
+// block by row block, so no workgroup is empty; entries with j < rb * GT_RB
+// are never written (the caller zeroes them once).
+__global__ __launch_bounds__(256) void k_trmv_t_items(long r, const double *__restrict__ M, long ldm,
+                                                      const double *__restrict__ t,
+                                                      double *__restrict__ P, int nrb,
+                                                      const LsqrState *__restrict__ st) {
+    if (st && st->done) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    long item = (long)blockIdx.x * 4 + w;
+    int rb = 0;
+    for (; rb < nrb; ++rb) {
+        const long cnt = r - (long)rb * GT_RB;
+        if (item < cnt) break;
+        item -= cnt;
+    }
+    if (rb >= nrb) return;
+    const long j = (long)rb * GT_RB + item;
+    const long i0 = (long)rb * GT_RB, i1 = min(j + 1, i0 + GT_RB);
+    const double *c = M + j * ldm;
+    constexpr int U = GT_RB / 64;
+    double s[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const long i = i0 + lane + 64 * u;
+        s[u] = i < i1 ? c[i] * t[i] : 0.0;
+    }
+#pragma unroll
+    for (int h = U / 2; h > 0; h >>= 1)
+#pragma unroll
+        for (int u = 0; u < h; ++u) s[u] += s[u + h];
+    const double a = wave_sum(s[0]);
+    if (lane == 0) P[(long)rb * r + j] = a;
+}
+
+__global__ void k_zero_d(long n, double *__restrict__ p) {
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = 0.0;
+}
+
+static long trmv_t_items(long r) {
+    long n = 0;
+    for (long i0 = 0; i0 < r; i0 += GT_RB) n += r - i0;
+    return n;
+}
+
 // ---- one pass over A: u <- A z - coef u and A^T u together ---------------------
 // The pass reads a row-major copy AT of A (rows of ldt doubles, ldt even, the
 // padding column zero), made once per solve by k_to_rows.  A 256-thread
@@ -255,21 +302,6 @@ __global__ __launch_bounds__(256) void k_to_rows(long n, long d, const double *_
     }
 }
 
-// F[g * d + j] = sum of P[p * d + j] over the partials p of group g (FOLD_GRP
-// consecutive ones, the last group ragged): fewer partials for k_reduce_t
-constexpr int FOLD_GRP = 32;
-__global__ __launch_bounds__(256) void k_fold_parts(long d, int np, const double *__restrict__ P,
-                                                    double *__restrict__ F,
-                                                    const LsqrState *__restrict__ st) {
-    if (st->done) return;
-    const long j = (long)blockIdx.x * 256 + threadIdx.x;
-    if (j >= d) return;
-    const int p0 = blockIdx.y * FOLD_GRP, p1 = min(np, p0 + FOLD_GRP);
-    double s = 0.0;
-    for (int p = p0; p < p1; ++p) s += P[(long)p * d + j];
-    F[(long)blockIdx.y * d + j] = s;
-}
-
 // ---- reductions / scatter ----------------------------------------------------
 // Sums of partial products: 64 outputs per 256-thread block, each summed by
 // four threads over every fourth partial, then combined in a fixed order.
@@ -362,6 +394,36 @@ __global__ __launch_bounds__(256) void k_reduce_t(long d, long r, int nrb,
     sh[g][e] = s;
     __syncthreads();
     if (g == 0 && i < r) t[i] = ((sh[0][e] + sh[1][e]) + (sh[2][e] + sh[3][e])) * (1.0 / sbeta);
+}
+
+// The same over many partials (the one-pass kernel's, one per workgroup): 16
+// outputs per block, each summed by 16 threads over every 16th partial, then
+// combined in a fixed order.
+constexpr int RW_OUT = 16, RW_GRP = 256 / RW_OUT;
+__global__ __launch_bounds__(256) void k_reduce_t_wide(long d, long r, int np,
+                                                       const double *__restrict__ P,
+                                                       const int32_t *__restrict__ E,
+                                                       const double *__restrict__ Sb, int nsb,
+                                                       double *__restrict__ t,
+                                                       const LsqrState *__restrict__ st) {
+    if (st->done) return;
+    __shared__ double sh[RW_GRP][RW_OUT + 1];
+    const int e = threadIdx.x % RW_OUT, g = threadIdx.x / RW_OUT;
+    const long i = (long)blockIdx.x * RW_OUT + e;
+    double s = 0.0;
+    if (i < r) {
+        const long j = E ? (long)E[i] : i;
+        for (int q = g; q < np; q += RW_GRP) s += P[(long)q * d + j];
+    }
+    sh[g][e] = s;
+    // beta after the partial sums (their loads in flight meanwhile)
+    const double sbeta = beta_from<256>(Sb, nsb);
+    if (g == 0 && i < r) {
+        double tot = 0.0;
+#pragma unroll
+        for (int k = 0; k < RW_GRP; ++k) tot += sh[k][e];
+        t[i] = tot * (1.0 / sbeta);
+    }
 }
 
 // ---- scalar recurrences (one workgroup) ---------------------------------------
@@ -467,6 +529,115 @@ __global__ __launch_bounds__(UPD_THREADS) void k_update(
             st->flag = 1;
             st->done = 1;
         }
+    }
+}
+
+// The same iteration step (init = 0) over ceil(r / 256) workgroups in two
+// kernels.  k_update_v: v = A^T u - beta v (unnormalised) and one partial sum
+// of squares per workgroup in Sv.  k_update_wx: every workgroup forms alpha
+// from Sv and the Givens scalars from the state (the same values everywhere),
+// normalises its part of v and updates w and x; the last workgroup to finish
+// (a counter in device memory, reset by that workgroup) writes the new state,
+// when every other workgroup has read the old one.
+constexpr int UV_T = 256;
+
+__global__ __launch_bounds__(UV_T) void k_update_v(long r, int nrbm, const double *__restrict__ Pm,
+                                                   const double *__restrict__ Sb, int nsb,
+                                                   double *__restrict__ v,
+                                                   double *__restrict__ Sv,
+                                                   const LsqrState *__restrict__ st) {
+    if (st->done) return;
+    const double beta = beta_from<UV_T>(Sb, nsb);
+    const long j = (long)blockIdx.x * UV_T + threadIdx.x;
+    double sq = 0.0;
+    if (beta > 0 && j < r) {
+        double s = 0.0;
+        for (int q = 0; q < nrbm; ++q) s += Pm[(long)q * r + j];
+        const double vv = (-beta) * v[j] + s;
+        v[j] = vv;
+        sq = vv * vv;
+    }
+    sq = block_sum<UV_T>(sq);
+    if (threadIdx.x == 0) Sv[blockIdx.x] = sq;
+}
+
+__global__ __launch_bounds__(UV_T) void k_update_wx(long r, int nsv, const double *__restrict__ Sv,
+                                                    const double *__restrict__ Sb, int nsb,
+                                                    double *__restrict__ v, double *__restrict__ w,
+                                                    double *__restrict__ x,
+                                                    LsqrState *__restrict__ st, int *cnt,
+                                                    double *__restrict__ hist, long hist_cap) {
+    if (st->done) return;
+    // this thread's entries first, so that their loads overlap the sums
+    const long j = (long)blockIdx.x * UV_T + threadIdx.x;
+    double vj = 0.0, wj = 0.0, xj = 0.0;
+    if (j < r) {
+        vj = v[j];
+        wj = w[j];
+        xj = x[j];
+    }
+    const double beta = beta_from<UV_T>(Sb, nsb);
+    const double alpha_old = st->alpha;
+    double alpha = alpha_old, nrm_a = st->nrm_a;
+    const bool upd = beta > 0;
+    if (upd) {
+        double s = 0.0;
+        for (int q = threadIdx.x; q < nsv; q += UV_T) s += Sv[q];
+        alpha = sqrt(block_sum<UV_T>(s));
+        nrm_a = sqrt(nrm_a * nrm_a + alpha_old * alpha_old + beta * beta);
+    }
+    const bool scale = upd && alpha > 0;
+    const double sc = scale ? 1.0 / alpha : 1.0;
+    double rhobar = st->rhobar, phibar = st->phibar;
+    const double rho = sqrt(rhobar * rhobar + beta * beta);
+    const double cs = rhobar / rho;
+    const double sn = beta / rho;
+    const double theta = sn * alpha;
+    rhobar = -cs * alpha;
+    const double phi = cs * phibar;
+    phibar = sn * phibar;
+    const double c1 = phi / rho, c2 = -theta / rho;
+    if (j < r) {
+        if (scale) {
+            vj = sc * vj;
+            v[j] = vj;
+        }
+        x[j] = xj + c1 * wj;
+        w[j] = c2 * wj + 1.0 * vj;
+    }
+    __shared__ int last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(cnt, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    *cnt = 0;
+    const long it = st->it + 1;
+    const double nrm_r = phibar;
+    const double nrm_ar = phibar * alpha * fabs(cs);
+    st->alpha = alpha;
+    st->beta = beta;
+    st->rhobar = rhobar;
+    st->phibar = phibar;
+    st->nrm_a = nrm_a;
+    st->nrm_r = nrm_r;
+    st->nrm_ar = nrm_ar;
+    st->coef = (beta > 0) ? alpha / beta : 0.0;
+    st->it = it;
+    if (hist && it <= hist_cap) {
+        double *h = hist + 5 * (it - 1);
+        h[0] = alpha;
+        h[1] = beta;
+        h[2] = rhobar;
+        h[3] = phibar;
+        h[4] = nrm_a;
+    }
+    if (nrm_ar < st->tol * nrm_a * nrm_r) {
+        st->flag = 0;
+        st->done = 1;
+    } else if (it >= st->max_it) {
+        st->flag = 1;
+        st->done = 1;
     }
 }
 
@@ -577,7 +748,6 @@ struct LsqrPlan {
     bool fused = false; // one pass over a row-major copy of A per iteration (k_fused_rows)
     int spg = 0;        // sparse A z with z in LDS: workgroups (0: z from memory)
     int G = 0, nc = 0;  // fused: workgroups, column pairs per thread (over 512 columns)
-    int nfold = 0;      // fused: partial groups after k_fold_parts
     long ldt = 0;       // fused: row stride of the copy
 };
 
@@ -644,7 +814,6 @@ static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
         const char *e = getenv("SKL_FUSED_WPC");
         if (e && atoi(e) > 0) wpc = atoi(e);
         q.G = (int)std::min<long>((long)wpc * num_cu, p.n);
-        q.nfold = (q.G + FOLD_GRP - 1) / FOLD_GRP;
         q.ldt = p.d + (p.d & 1);
     }
     return q;
@@ -652,7 +821,9 @@ static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
 
 struct LsqrBufs {
     double *ut, *v, *w, *x, *z, *Pz, *Pn, *Sb, *Pt, *t, *Pm, *hist;
-    double *AT, *Pf; // fused: row-major copy of A, folded partials of A^T u
+    double *AT;      // fused: row-major copy of A
+    double *Sv;      // squares of v, one per k_update_v workgroup
+    int *cnt;        // k_update_wx's finished-workgroup counter (zero between launches)
     long hist_cap;
     int32_t *Einv;
     LsqrState *st;
@@ -674,20 +845,19 @@ static void launch_ev(void (*k)(P...), dim3 g, dim3 b, uint32_t sh, hipStream_t 
         hipLaunchKernelGGL(k, g, b, sh, s, args...);
 }
 
-// after_fused: A^T u already sits in Pf as folded per-workgroup partials
-// (q.nfold of them) and the squares in Sb (q.G); otherwise A^T u is computed
-// here.
+// after_fused: A^T u already sits in Pt as per-workgroup partials and the
+// squares in Sb (q.G of each); otherwise A^T u is computed here.
 static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs &B,
                         hipStream_t s, int init, hipEvent_t *ev = nullptr,
                         bool after_fused = false) {
     hipEvent_t e2 = ev ? ev[2] : nullptr, e3 = ev ? ev[3] : nullptr;
     hipEvent_t e6 = ev ? ev[6] : nullptr, e7 = ev ? ev[7] : nullptr;
     int nrb = q.nrb_a, nsb = q.nsb;
-    const double *Pt = B.Pt;
     if (after_fused) {
-        nrb = q.nfold;
+        // t = Pt(E) / beta over the workgroup partials
         nsb = q.G;
-        Pt = B.Pf;
+        k_reduce_t_wide<<<(unsigned)((q.r + RW_OUT - 1) / RW_OUT), 256, 0, s>>>(
+            q.d, q.r, q.G, B.Pt, p.E, B.Sb, nsb, B.t, B.st);
     } else {
         // Pt = A^T ut
         if (!q.sparse) {
@@ -698,18 +868,27 @@ static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs 
             launch_ev(k_csc_t, dim3((unsigned)((q.d + 3) / 4)), dim3(256), 0, s, e2, e3, q.d,
                       p.csc_ptr, p.csc_row, p.csc_val, B.ut, B.Pt, B.st);
         }
+        // t = Pt(E) / beta
+        k_reduce_t<<<(unsigned)((q.r + RED_OUT - 1) / RED_OUT), 256, 0, s>>>(
+            q.d, q.r, nrb, B.Pt, p.E, B.Sb, nsb, B.t, B.st);
     }
-    // t = Pt(E) / beta
-    k_reduce_t<<<(unsigned)((q.r + RED_OUT - 1) / RED_OUT), 256, 0, s>>>(q.d, q.r, nrb, Pt, p.E, B.Sb, nsb,
-                                                             B.t, B.st);
-    // Pm = M^T t
-    {
+    const unsigned nv = (unsigned)((q.r + UV_T - 1) / UV_T);
+    if (init) {
+        // Pm = M^T t, then the first step on one workgroup
         dim3 g((unsigned)q.nrb_m, (unsigned)((q.r + q.cpw_m - 1) / q.cpw_m));
-        launch_ev(k_gemv_t_part<1>, g, dim3(GT_THREADS), 0, s, e6, e7, q.r, q.r, p.M, p.ldm, B.t,
-                  q.cpw_m, B.Pm, 1, B.st);
+        k_gemv_t_part<1><<<g, GT_THREADS, 0, s>>>(q.r, q.r, p.M, p.ldm, B.t, q.cpw_m, B.Pm, 1,
+                                                   B.st);
+        k_update<<<1, UPD_THREADS, 0, s>>>(q.r, q.nrb_m, B.Pm, B.Sb, nsb, B.v, B.w, B.x, B.st,
+                                           B.hist, B.hist_cap, init);
+    } else {
+        // Pm = M^T t (row-block partials; those below the diagonal blocks stay zero)
+        const unsigned g = (unsigned)((trmv_t_items(q.r) + 3) / 4);
+        launch_ev(k_trmv_t_items, dim3(g), dim3(256), 0, s, e6, e7, q.r, p.M, p.ldm, B.t, B.Pm,
+                  q.nrb_m, B.st);
+        k_update_v<<<nv, UV_T, 0, s>>>(q.r, q.nrb_m, B.Pm, B.Sb, nsb, B.v, B.Sv, B.st);
+        k_update_wx<<<nv, UV_T, 0, s>>>(q.r, (int)nv, B.Sv, B.Sb, nsb, B.v, B.w, B.x, B.st, B.cnt,
+                                        B.hist, B.hist_cap);
     }
-    k_update<<<1, UPD_THREADS, 0, s>>>(q.r, q.nrb_m, B.Pm, B.Sb, nsb, B.v, B.w, B.x, B.st,
-                                       B.hist, B.hist_cap, init);
 }
 
 template <int NC> static void launch_fused_nc(const LsqrProblem &p, const LsqrPlan &q,
@@ -732,8 +911,6 @@ static void launch_fused(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs
     case 8: launch_fused_nc<8>(p, q, B, s, e0, e1); break;
     default: launch_fused_nc<16>(p, q, B, s, e0, e1); break;
     }
-    dim3 g((unsigned)((q.d + 255) / 256), (unsigned)q.nfold);
-    k_fold_parts<<<g, 256, 0, s>>>(q.d, q.G, B.Pt, B.Pf, B.st);
 }
 
 // Optional kernel timing: events around the four operator products of one
@@ -826,7 +1003,8 @@ __global__ void k_build_einv2(long r, const int32_t *E, int32_t *Einv) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < r) Einv[E[i]] = (int32_t)i;
 }
-__global__ void k_set_state(LsqrState *st, double tol, long max_it) {
+__global__ void k_set_state(LsqrState *st, double tol, long max_it, int *cnt) {
+    *cnt = 0;
     st->alpha = st->beta = st->rhobar = st->phibar = 0.0;
     st->nrm_a = st->nrm_r = st->nrm_ar = st->coef = 0.0;
     st->tol = tol;
@@ -889,7 +1067,8 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
     B.Pt = ar.alloc_n<double>((size_t)std::max(q.nrb_a, q.G) * q.d);
     B.t = ar.alloc_n<double>((size_t)q.r);
     B.Pm = ar.alloc_n<double>((size_t)q.nrb_m * q.r);
-    B.AT = B.Pf = nullptr;
+    k_zero_d<<<256, 256, 0, s>>>((long)q.nrb_m * q.r, B.Pm);
+    B.AT = nullptr;
     if (q.fused) {
         if (p.AT) {
             // made on the aux stream (lsqr_start_rows) with the same row stride
@@ -901,9 +1080,10 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
             dim3 g((unsigned)((q.n + 63) / 64), (unsigned)((q.ldt + 63) / 64));
             k_to_rows<<<g, 256, 0, s>>>(q.n, q.d, p.A, p.lda, B.AT, q.ldt);
         }
-        B.Pf = ar.alloc_n<double>((size_t)q.nfold * q.d);
     }
     B.st = ar.alloc_n<LsqrState>(1);
+    B.Sv = ar.alloc_n<double>((size_t)(q.r + UV_T - 1) / UV_T);
+    B.cnt = ar.alloc_n<int>(1);
     long hl = hist ? std::min(hist_len, max_it) : 0;
     B.hist = hl > 0 ? ar.alloc_n<double>((size_t)5 * hl) : nullptr;
     B.hist_cap = hl;
@@ -913,7 +1093,7 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
         k_build_einv<<<(unsigned)((q.d + 255) / 256), 256, 0, s>>>(q.d, q.r, p.E, B.Einv);
         k_build_einv2<<<(unsigned)((q.r + 255) / 256), 256, 0, s>>>(q.r, p.E, B.Einv);
     }
-    k_set_state<<<1, 1, 0, s>>>(B.st, tol, max_it);
+    k_set_state<<<1, 1, 0, s>>>(B.st, tol, max_it, B.cnt);
     SKL_CHECK_LAUNCH();
 
     // init: ut = b, beta = ||b||, v = M^T A^T u / alpha ...
@@ -953,8 +1133,9 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
                  (long)(intptr_t)B.Pn, (long)(intptr_t)B.Pt, (long)(intptr_t)B.Pm,
                  (long)(intptr_t)B.Einv, (long)(intptr_t)B.z, (long)(intptr_t)B.w,
                  (long)(intptr_t)B.t, (long)(intptr_t)B.Sb, (long)(intptr_t)B.Pz, B.hist_cap,
+                 (long)(intptr_t)B.Sv, (long)(intptr_t)B.cnt,
                  q.fused ? 1L : 0L, chunk, (long)(intptr_t)s, prof ? 1L : 0L,
-                 (long)q.spg, (long)(intptr_t)B.AT, (long)(intptr_t)B.Pf, (long)q.G};
+                 (long)q.spg, (long)(intptr_t)B.AT, (long)q.G};
         auto it = g_graphs.find(key);
         if (it != g_graphs.end()) {
             exec = it->second;
