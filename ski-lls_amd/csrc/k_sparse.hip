@@ -1,6 +1,7 @@
 // k_sparse.hip -- sparse-input kernels: hashing sketch S A of a CSC matrix,
 // CSC -> CSR transpose, index narrowing and CSR products.
 #include <algorithm>
+#include <climits>
 
 #include "k_common.h"
 
@@ -98,21 +99,36 @@ __global__ __launch_bounds__(256) void k_csc_scatter(long d, const int32_t *colp
         cval[pos] = val[p];
     }
 }
-__global__ void k_row_sort(long n, const int32_t *ptr, int32_t *ccol, double *cval) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per row: every entry goes to the position given by its rank in the
+// row (the number of entries with a smaller column, or an equal column and a
+// smaller scatter index, so duplicates keep a fixed order).  The keys are read
+// 64 at a time into registers and broadcast with readlane.
+__global__ __launch_bounds__(256) void k_row_sort_rank(long n, const int32_t *__restrict__ ptr,
+                                                       const int32_t *__restrict__ tcol,
+                                                       const double *__restrict__ tval,
+                                                       int32_t *__restrict__ ccol,
+                                                       double *__restrict__ cval) {
+    const long i = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= n) return;
-    int p0 = ptr[i], p1 = ptr[i + 1];
-    for (int a = p0 + 1; a < p1; ++a) {
-        int kc = ccol[a];
-        double kv = cval[a];
-        int b = a - 1;
-        while (b >= p0 && ccol[b] > kc) {
-            ccol[b + 1] = ccol[b];
-            cval[b + 1] = cval[b];
-            --b;
+    const int lane = threadIdx.x & 63;
+    const int p0 = ptr[i], L = ptr[i + 1] - p0;
+    for (int e0 = 0; e0 < L; e0 += 64) {
+        const int e = e0 + lane;
+        const int key = e < L ? tcol[p0 + e] : INT_MAX;
+        const double v = e < L ? tval[p0 + e] : 0.0;
+        int rank = 0;
+        for (int q0 = 0; q0 < L; q0 += 64) {
+            const int kl = q0 + lane < L ? tcol[p0 + q0 + lane] : INT_MAX;
+            const int qn = min(64, L - q0);
+            for (int q = 0; q < qn; ++q) {
+                const int kq = __builtin_amdgcn_readlane(kl, q);
+                rank += (kq < key) || (kq == key && q0 + q < e);
+            }
         }
-        ccol[b + 1] = kc;
-        cval[b + 1] = kv;
+        if (e < L) {
+            ccol[p0 + rank] = key;
+            cval[p0 + rank] = v;
+        }
     }
 }
 
@@ -125,9 +141,14 @@ void csc_to_csr(long n, long d, long nnz, const int32_t *colptr, const int32_t *
     exclusive_scan(n, cnt, rptr, ar);
     int32_t *cur = ar.alloc_n<int32_t>((size_t)n + 1);
     SKL_HIP(hipMemcpyAsync(cur, rptr, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyDeviceToDevice, s));
-    if (d > 0) k_csc_scatter<<<(unsigned)((d + 3) / 4), 256, 0, s>>>(d, colptr, rowind, val, cur, ccol, cval);
-    k_row_sort<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, rptr, ccol, cval);
+    // scatter in arrival order into scratch, then place each entry by its rank
+    const size_t mark = ar.used();
+    int32_t *tcol = ar.alloc_n<int32_t>((size_t)std::max<long>(nnz, 1));
+    double *tval = ar.alloc_n<double>((size_t)std::max<long>(nnz, 1));
+    if (d > 0) k_csc_scatter<<<(unsigned)((d + 3) / 4), 256, 0, s>>>(d, colptr, rowind, val, cur, tcol, tval);
+    k_row_sort_rank<<<(unsigned)((n + 3) / 4), 256, 0, s>>>(n, rptr, tcol, tval, ccol, cval);
     SKL_CHECK_LAUNCH();
+    ar.rewind(mark);
 }
 
 // y = beta y + alpha A x  (CSR, one thread-group of 8 lanes per row)

@@ -1,5 +1,6 @@
 // runtime.cpp -- context, device arena, errors and the host-side hashing
 // pattern generator of the MI355X least-squares library.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -287,9 +288,102 @@ double Glibc::normal() {
     return 0.0 + 1.0 * (c1 * factor);
 }
 
+// ---- jump-ahead for the generator -------------------------------------------
+// With y_0..y_30 the generator's last 31 raw values (oldest first), the raw
+// sequence obeys y_i = y_{i-31} + y_{i-3} (mod 2^32), a linear recurrence with
+// characteristic polynomial t^31 - t^28 - 1.  If t^N = sum_j c_j t^j modulo
+// that polynomial, then y_{N+m} = sum_j c_j y_{j+m} exactly in uint32
+// arithmetic, so any later window follows from the current one.
+namespace {
+typedef std::vector<uint32_t> Poly; // 31 coefficients, degree < 31
+
+// reduce a polynomial of degree < 61 modulo t^31 - t^28 - 1 (t^31 = t^28 + 1)
+Poly poly_reduce(std::vector<uint32_t> a) {
+    for (int k = (int)a.size() - 1; k >= 31; --k) {
+        const uint32_t c = a[(size_t)k];
+        if (!c) continue;
+        a[(size_t)k] = 0;
+        a[(size_t)k - 3] += c;
+        a[(size_t)k - 31] += c;
+    }
+    a.resize(31);
+    return a;
+}
+
+Poly poly_mul(const Poly &x, const Poly &y) {
+    std::vector<uint32_t> a(61, 0u);
+    for (int i = 0; i < 31; ++i)
+        if (x[(size_t)i])
+            for (int j = 0; j < 31; ++j) a[(size_t)(i + j)] += x[(size_t)i] * y[(size_t)j];
+    return poly_reduce(std::move(a));
+}
+
+// t^N modulo the characteristic polynomial
+Poly poly_tpow(uint64_t N) {
+    Poly r(31, 0u), b(31, 0u);
+    r[0] = 1;
+    b[1] = 1;
+    while (N) {
+        if (N & 1) r = poly_mul(r, b);
+        N >>= 1;
+        if (N) b = poly_mul(b, b);
+    }
+    return r;
+}
+
+// the window y_N..y_{N+30} from y_0..y_60 (first 31 = current window)
+void jump_window(const std::vector<uint32_t> &y, uint64_t N, uint32_t *w) {
+    Poly c = poly_tpow(N);
+    for (int m = 0; m < 31; ++m) {
+        uint32_t v = 0;
+        for (int j = 0; j < 31; ++j) v += c[(size_t)j] * y[(size_t)(j + m)];
+        w[m] = v;
+    }
+}
+} // namespace
+
+void Glibc::fill(int *out, long count) {
+    if (count <= 0) return;
+    const long chunk = 1L << 16;
+    unsigned hw = std::thread::hardware_concurrency();
+    long P = std::min<long>(std::min<long>(8, hw ? (long)hw : 1), count / chunk);
+    if (P <= 1) {
+        for (long i = 0; i < count; ++i) out[i] = next();
+        return;
+    }
+    // the current window and 30 values beyond it (for shifted windows)
+    std::vector<uint32_t> y(61);
+    for (int j = 0; j < 31; ++j) y[(size_t)j] = (uint32_t)s[(f + j) % 31];
+    for (int j = 31; j < 61; ++j) y[(size_t)j] = y[(size_t)j - 31] + y[(size_t)j - 3];
+    const long per = (count + P - 1) / P;
+    std::vector<Glibc> part((size_t)P);
+    std::vector<std::thread> th;
+    for (long p = 0; p < P; ++p) {
+        th.emplace_back([&, p] {
+            const long b = p * per, e = std::min(count, b + per);
+            // output i is raw y_{31 + i}: the window before it starts at y_i
+            uint32_t w[31];
+            jump_window(y, (uint64_t)b, w);
+            Glibc &G = part[(size_t)p];
+            for (int j = 0; j < 31; ++j) G.s[j] = (int32_t)w[j];
+            G.f = 0;
+            G.r = 28;
+            for (long i = b; i < e; ++i) out[i] = G.next();
+        });
+    }
+    for (auto &t : th) t.join();
+    // continue from the state after the last chunk
+    const Glibc &L = part[(size_t)P - 1];
+    std::memcpy(s, L.s, sizeof s);
+    f = L.f;
+    r = L.r;
+}
+
 void gen_signs(Glibc &g, long n, std::vector<double> &D) {
     D.resize((size_t)n);
-    for (long i = 0; i < n; ++i) D[(size_t)i] = 1.0 * (double)((g.next() % 2) * 2 - 1);
+    std::vector<int> R((size_t)n);
+    g.fill(R.data(), n);
+    for (long i = 0; i < n; ++i) D[(size_t)i] = 1.0 * (double)((R[(size_t)i] % 2) * 2 - 1);
 }
 
 // Division by a runtime constant without the hardware divider: the generator
@@ -320,9 +414,14 @@ void gen_hashing(Glibc &g, long m, long n, long k, Pattern &p) {
     for (long t = 0; t < k; ++t) mods.emplace_back((uint64_t)(m - t));
     int32_t *rows = p.rows.data();
     int32_t *a = arr.data();
+    // all draws first (k per source row, then one sign per entry), so that the
+    // generator runs in parallel chunks and only the swaps stay sequential
+    std::vector<int> R((size_t)(2 * n * k));
+    g.fill(R.data(), 2 * n * k);
+    const int *rr = R.data();
     for (long i = 0; i < n; ++i) {
         for (long t = 0; t < k; ++t) {
-            long j = (long)mods[(size_t)t].mod((uint64_t)g.next()) + t;
+            long j = (long)mods[(size_t)t].mod((uint64_t)rr[k * i + t]) + t;
             int32_t tmp = a[j];
             a[j] = a[t];
             a[t] = tmp;
@@ -332,7 +431,8 @@ void gen_hashing(Glibc &g, long m, long n, long k, Pattern &p) {
     double scaling = std::sqrt((double)k);
     double pos = 1 / scaling * 1.0, neg = 1 / scaling * -1.0;
     double *v = p.vals.data();
-    for (long q = 0; q < n * k; ++q) v[q] = (g.next() % 2) ? pos : neg;
+    const int *rs = rr + n * k;
+    for (long q = 0; q < n * k; ++q) v[q] = (rs[q] % 2) ? pos : neg;
 }
 
 // Bucket lists: a stable counting sort of the pattern by target row, so the

@@ -145,6 +145,10 @@ struct Glibc {
         return (int)(v >> 1);
     }
     double normal();
+    // out[0..count) = the next count outputs of next(), generated in parallel
+    // chunks for large counts (each chunk's starting window by jump-ahead);
+    // the generator is left as after count calls of next()
+    void fill(int *out, long count);
 };
 
 struct Pattern {

@@ -68,7 +68,8 @@ def test_error_strings(skl):
 
 
 @pytest.mark.parametrize("seed,m,n,k,dense", [(5, 340, 4000, 1, True), (1, 7500, 120000, 3, False),
-                                              (77, 50, 300, 3, True), (3, 8, 100, 8, False)])
+                                              (77, 50, 300, 3, True), (3, 8, 100, 8, False),
+                                              (9, 60, 200003, 2, True), (4, 900, 70001, 1, False)])
 def test_host_pattern_bit_exact_with_oracle(skl, O, seed, m, n, k, dense):
     D, rows, vals = skl.hash_pattern(seed, m, n, k, dense)
     if dense:

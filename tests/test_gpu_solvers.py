@@ -332,3 +332,25 @@ def test_sparse_rank_deficient_wide_refactorises_with_hqrrp(skl):
     xl = sl.lstsq(Ad, b, lapack_driver="gelss")[0]
     r, rl = np.linalg.norm(Ad @ x - b), np.linalg.norm(Ad @ xl - b)
     assert abs(r - rl) <= 1e-8 * rl
+
+
+def test_sparse_long_rows_csr_order(skl):
+    """Rows with more entries than a wave has lanes (65, 100, 129 and all 300
+    columns) go through the CSR row placement in several 64-entry rounds: the
+    solution matches the dense least-squares one and repeats bitwise."""
+    import scipy.sparse as sp
+    A, b = sparse_problem(31, n=3000, d=300, density=0.01)
+    A = A.tolil()
+    rng = np.random.default_rng(5)
+    for i, cnt in [(17, 65), (500, 100), (1201, 129), (2999, 300)]:
+        cols = rng.choice(300, size=cnt, replace=False)
+        A[i, cols] = rng.standard_normal(cnt)
+    A = A.tocsc()
+    A.sort_indices()
+    b = A @ rng.standard_normal(300) + 1e-3 * rng.standard_normal(3000)
+    x1, i1 = skl.sparse_solve(A, b, gamma=1.5, k=3, it_tol=1e-12, seed=1)
+    x2, _ = skl.sparse_solve(A, b, gamma=1.5, k=3, it_tol=1e-12, seed=1)
+    xl = np.linalg.lstsq(A.toarray(), b, rcond=None)[0]
+    assert i1.flag == 0 and i1.rank == 300
+    assert rel(x1, xl) < 1e-8
+    assert np.array_equal(x1, x2)
