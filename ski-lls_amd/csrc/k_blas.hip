@@ -565,6 +565,15 @@ bool cholesky_upper(long d, double *G, long ldg, double *R, long ldr, Arena &ar)
     const size_t mark = ar.used();
     int *fail = ar.alloc_n<int>(1);
     SKL_HIP(hipMemsetAsync(fail, 0, sizeof(int), s));
+    // Look-ahead: after block j's row B_j is solved, the main stream updates
+    // only the next block row (what block j + 1 factors and solves next); the
+    // rest of the trailing update runs on the context's second stream beside
+    // block j + 1's diagonal factor and solve, and the main stream waits for
+    // it only before it updates the following block row (which that update
+    // also touches).
+    Context &c = ctx();
+    c.make_side();
+    bool pending = false; // a trailing update on the side stream not yet waited for
     for (long j = 0; j < d; j += CH_B) {
         const int b = (int)std::min<long>(CH_B, d - j);
         const long nc = d - j - b;
@@ -574,11 +583,27 @@ bool cholesky_upper(long d, double *G, long ldg, double *R, long ldr, Arena &ar)
             double *Bp = G + j + (j + b) * ldg;
             k_chol_trsm<<<(unsigned)((nc + CH_TC - 1) / CH_TC), 64, 0, s>>>(b, Gjj, ldg, Bp, ldg,
                                                                             nc);
-            gemm<double>('T', 'N', nc, nc, b, -1.0, Bp, ldg, Bp, ldg, 1.0,
+            if (pending) SKL_HIP(hipStreamWaitEvent(s, c.ev[1], 0));
+            pending = false;
+            // next block row: rows j + b .. j + b + b2 of the trailing matrix
+            const long b2 = std::min<long>(CH_B, nc), nc2 = nc - b2;
+            gemm<double>('T', 'N', b2, nc, b, -1.0, Bp, ldg, Bp, ldg, 1.0,
                          G + (j + b) + (j + b) * ldg, ldg, 1, 0, 0, 0, true);
+            if (nc2 > 0) {
+                SKL_HIP(hipEventRecord(c.ev[0], s));
+                OnSideStream on_side;
+                hipStream_t ss = ctx().stream;
+                SKL_HIP(hipStreamWaitEvent(ss, c.ev[0], 0));
+                const double *B2 = Bp + b2 * ldg;
+                gemm<double>('T', 'N', nc2, nc2, b, -1.0, B2, ldg, B2, ldg, 1.0,
+                             G + (j + b + b2) + (j + b + b2) * ldg, ldg, 1, 0, 0, 0, true);
+                SKL_HIP(hipEventRecord(c.ev[1], ss));
+                pending = true;
+            }
         }
         SKL_CHECK_LAUNCH();
     }
+    if (pending) SKL_HIP(hipStreamWaitEvent(s, c.ev[1], 0));
     upcast_upper<double>(d, G, ldg, R, ldr);
     int hf = 0;
     SKL_HIP(hipMemcpyAsync(&hf, fail, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -162,8 +162,10 @@ struct ArenaTrim {
     explicit ArenaTrim(Arena &a) : ar(a) {}
     ~ArenaTrim() {
         (void)hipStreamSynchronize(ctx().stream);
-        // an early exit can leave the row-major copy of A still running
+        // an early exit can leave work on the other streams still running
+        // (the row-major copy of A, a look-ahead update)
         if (ctx().aux) (void)hipStreamSynchronize(ctx().aux);
+        if (ctx().side) (void)hipStreamSynchronize(ctx().side);
         try {
             ar.reset();
         } catch (...) {
