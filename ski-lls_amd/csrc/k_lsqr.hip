@@ -702,8 +702,17 @@ __global__ __launch_bounds__(256) void k_spmv_n_lds(long n, long d, int nrb,
             long i = (long)rb * SP_ROWS + q * RPI + grp;
             double s = 0.0;
             if (i < n) {
-                int p0 = ptr[i], p1 = ptr[i + 1];
-                for (int p = p0 + sub; p < p1; p += LPR) s += val[p] * zs[col[p]];
+                const int p0 = ptr[i], p1 = ptr[i + 1];
+                double s1 = 0.0;
+                int p = p0 + sub;
+                for (; p + LPR < p1; p += 2 * LPR) {
+                    const int c0 = col[p], c1 = col[p + LPR];
+                    const double v0 = val[p], v1 = val[p + LPR];
+                    s = fma(v0, zs[c0], s);
+                    s1 = fma(v1, zs[c1], s1);
+                }
+                if (p < p1) s = fma(val[p], zs[col[p]], s);
+                s += s1;
             }
 #pragma unroll
             for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
@@ -729,7 +738,9 @@ __global__ __launch_bounds__(256) void k_csc_t(long d, const int32_t *__restrict
     const int lane = threadIdx.x & 63;
     long j = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (j >= d) return;
-    int p0 = ptr[j], p1 = ptr[j + 1];
+    const int p0 = ptr[j], p1 = ptr[j + 1];
+    // one running sum (four independent ones measured slower: 32 vs 27 us at
+    // 120000 x 5000, density 1%)
     double s = 0.0;
     for (int p = p0 + lane; p < p1; p += 64) s += val[p] * ut[row[p]];
     s = wave_sum(s);
