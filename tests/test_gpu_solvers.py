@@ -354,3 +354,20 @@ def test_sparse_long_rows_csr_order(skl):
     assert i1.flag == 0 and i1.rank == 300
     assert rel(x1, xl) < 1e-8
     assert np.array_equal(x1, x2)
+
+
+def test_fused_lsqr_rank_deficient_matches_two_pass(skl, monkeypatch):
+    """Twenty duplicated columns: CPQR finds rank 280 and LSQR runs on the
+    pivoted columns E[0:280] of A (z scattered through E, A^T u gathered
+    through it).  The one-pass kernel gives the two-pass x, and the residual
+    is LAPACK's least-squares minimum."""
+    A, b = c1_problem(41, n=6000, d=280)
+    A = np.asfortranarray(np.hstack([A, A[:, :20]]))
+    monkeypatch.setenv("SKL_LSQR_FUSED", "0")
+    x2, i2 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-12, rcond=1e-10, seed=6)
+    monkeypatch.delenv("SKL_LSQR_FUSED")
+    x1, i1 = skl.dense_solve(A, b, gamma=1.7, k=1, it_tol=1e-12, rcond=1e-10, seed=6)
+    assert i1.rank == i2.rank == 280 and i1.flag == 0
+    assert rel(x1, x2) < 1e-8
+    r_min = np.linalg.norm(A @ sl.lstsq(A, b, lapack_driver="gelss")[0] - b)
+    assert abs(np.linalg.norm(A @ x1 - b) - r_min) < 1e-8 * r_min
