@@ -637,7 +637,8 @@ def main():
                      "other_kernels": {v["name"]: {"avg_ms": v["avg_ms"],
                                                    "GB/s": (v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9
                                                             if v["avg_ms"] > 0 else 0.0)}
-                                       for s, v in prof.items() if s != dom}},
+                                       for s, v in prof.items()
+                                       if s != dom and v["launches"] > 0}},
         "cpu_baseline": cpu,
     }
     if "escalation" in res:
