@@ -140,6 +140,14 @@ void skl_profile_enable(int on);
 void skl_profile_reset(void);
 int skl_profile_read(int slot, double *avg_ms, long *launches, double *bytes_per_launch);
 
+/* Poll limit of the factorization kernels' grid barriers on this thread (no
+ * counterpart in the reference).  A barrier whose wait exceeds it gives up,
+ * the kernels of that factorization stop early, and the call that launched
+ * them returns SKL_E_HIP instead of a result computed from stale data.
+ * spins < 0 restores the default (2^28 polls, seconds); 0 makes a barrier give
+ * up unless every workgroup has already arrived, which tests that error path. */
+void skl_set_barrier_spins(long spins);
+
 /* LSQR per-iteration history of the next solve on this thread (optional):
  * steps must hold max_steps entries; pass NULL to switch it off. */
 void skl_set_history(skl_lsqr_step *steps, long max_steps);

@@ -52,23 +52,53 @@ __device__ inline float readlane(float v, int l) {
 // Barrier over every workgroup of a launch.  The kernels that use it keep
 // their grids far below what the device holds at once (at most a quarter to
 // a half of the CUs, one workgroup per CU), so all workgroups are resident
-// without a cooperative launch.  *ctr is zeroed by the host before the launch;
+// without a cooperative launch.  ctr is zeroed by the host before the launch;
 // `target` starts at 0 in every workgroup.  Agent-scope fences around the
 // arrival publish this workgroup's global writes and expose the others'.
-// The wait gives up after ~2^28 polls (seconds) rather than hang the device.
-__device__ inline void grid_barrier(unsigned *ctr, unsigned &target) {
+//
+// The wait is bounded: after `spins` polls it gives up, sets the error word
+// *err (kept per host context, read back by the host after the
+// factorization) and returns false.  A workgroup that finds *err set while
+// waiting gives up at once, so after one failure every workgroup of the grid
+// leaves its barriers within a few polls.  The result is uniform over the
+// workgroup; a kernel returns as soon as it is false (all its threads
+// together), and the kernels that consume indices the failed launch should
+// have written check *err before using them.
+__device__ inline bool grid_barrier(const GridSync &g, unsigned &target) {
+    __shared__ int s_ok;
     __syncthreads();
     target += gridDim.x;
     if (threadIdx.x == 0) {
         __threadfence();
-        __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned spins = 0;
-        while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target &&
-               ++spins < (1u << 28))
-            __builtin_amdgcn_s_sleep(1);
+        const unsigned old =
+            __hip_atomic_fetch_add(g.ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int ok = 1;
+        if (old + 1u < target) {
+            unsigned polls = 0;
+            for (;;) {
+                if (__hip_atomic_load(g.ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target)
+                    break;
+                if (polls >= g.spins ||
+                    ((polls & 63u) == 63u &&
+                     __hip_atomic_load(g.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)) {
+                    ok = 0;
+                    break;
+                }
+                ++polls;
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        if (!ok) __hip_atomic_fetch_or(g.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
+        s_ok = ok;
     }
     __syncthreads();
+    return s_ok != 0;
+}
+
+// true when a grid barrier of an earlier launch gave up (see GridSync)
+__device__ inline bool grid_failed(const unsigned *err) {
+    return __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
 }
 
 template <class T> __device__ inline T wave_max(T v) {

@@ -48,7 +48,7 @@ template <class T> struct RsArgs {
     T *red;         // [2][nc]
     T *rowv;        // [2][nc]
     T *rcp;         // [P][nc]
-    unsigned *bar;  // barrier counter, zero at launch
+    GridSync bar;   // barrier counter (zero at launch), error word, poll limit
 };
 
 template <class T> __device__ inline T dlapy2_d(T x, T y) {
@@ -97,10 +97,10 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
     // all-reduce of vals[q] for q in [q0, nc) (q0 relative), result into Gs;
     // with_row also copies the published pivot row (rowv[buf]) into Rs, its
     // loads issued beside the partial sums
-    auto allreduce = [&](int buf, int q0, bool with_row) {
+    auto allreduce = [&](int buf, int q0, bool with_row) -> bool {
         T *pw = a.part + ((size_t)buf * P + w) * nc;
         // values already in pw[q] for q >= q0
-        grid_barrier(a.bar, bar_target);
+        if (!grid_barrier(a.bar, bar_target)) return false;
         if (!a.two_phase) {
             // eight adjacent lanes per column, each summing every eighth workgroup
             const int sub = tid & 7;
@@ -128,13 +128,14 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
                 for (int u = 0; u < P; ++u) s += a.part[((size_t)buf * P + u) * nc + q];
                 a.red[(size_t)buf * nc + q] = s;
             }
-            grid_barrier(a.bar, bar_target);
+            if (!grid_barrier(a.bar, bar_target)) return false;
             for (int q = q0 + tid; q < nc; q += RS_T) {
                 Gs[q] = a.red[(size_t)buf * nc + q];
                 if (with_row) Rs[q] = a.rowv[(size_t)buf * nc + q];
             }
         }
         (void)pw;
+        return true;
     };
 
     // initial column norms for pivoting (over all rows)
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
             }
             pw[q] = s;
         }
-        allreduce(1, 0, false);
+        if (!allreduce(1, 0, false)) return;
         __syncthreads();
         for (int q = tid; q < npiv; q += RS_T) {
             vn1[q] = sqrt(Gs[q]);
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
             for (int q = jj + tid; q < nc; q += RS_T)
                 a.rowv[(size_t)buf * nc + q] = S[(jj - r0) + (size_t)q * ldl];
         SKL_PHASE(0, 2);
-        allreduce(buf, jj, true);
+        if (!allreduce(buf, jj, true)) return;
         __syncthreads();
         SKL_PHASE(0, 3);
         // Householder scalars (dlarfg), computed redundantly
@@ -302,7 +303,7 @@ __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
                         }
                     rw[q] = s;
                 }
-                grid_barrier(a.bar, bar_target);
+                if (!grid_barrier(a.bar, bar_target)) return;
                 for (int q = jj + 1 + tid; q < npiv; q += RS_T) {
                     if (!flags[q]) continue;
                     if (jj < a.m - 1) {
@@ -362,7 +363,7 @@ template <class T> struct RpArgs {
     T *part;        // [2][P][64] workgroup sums
     T *rowv;        // [2][64] row jj of the panel (workgroup 0)
     T *rcp;         // [P][64] norm recomputation sums
-    unsigned *bar;  // barrier counter, zero at launch
+    GridSync bar;   // barrier counter (zero at launch), error word, poll limit
 };
 
 constexpr int RP_W = 8, RP_T = 64 * RP_W;
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(RP_T) void k_qr_regpanel(RpArgs<T> a) {
         for (int u = 0; u < RP_W; ++u) tot += s_r[u][lane];
         if (P > 1) {
             if (w == 0) a.rcp[(size_t)blk * 64 + lane] = tot;
-            grid_barrier(a.bar, bar_target);
+            if (!grid_barrier(a.bar, bar_target)) return;
             tot = 0;
             for (int u = 0; u < P; ++u) tot += a.rcp[(size_t)u * 64 + lane];
         }
@@ -494,7 +495,7 @@ __global__ __launch_bounds__(RP_T) void k_qr_regpanel(RpArgs<T> a) {
                 a.part[((size_t)buf * P + blk) * 64 + lane] = tot;
                 if (blk == 0) a.rowv[(size_t)buf * 64 + lane] = s_row[buf][lane];
             }
-            grid_barrier(a.bar, bar_target);
+            if (!grid_barrier(a.bar, bar_target)) return;
             // wave w sums the workgroups u = w mod 8, then every wave adds
             // the eight partial sums in the same order
             T pt = 0;
@@ -564,7 +565,7 @@ __global__ __launch_bounds__(RP_T) void k_qr_regpanel(RpArgs<T> a) {
                 for (int u = 0; u < RP_W; ++u) tot += s_r[u][lane];
                 if (P > 1) {
                     if (w == 0) a.rcp[(size_t)blk * 64 + lane] = tot;
-                    grid_barrier(a.bar, bar_target);
+                    if (!grid_barrier(a.bar, bar_target)) return;
                     tot = 0;
                     for (int u = 0; u < P; ++u) tot += a.rcp[(size_t)u * 64 + lane];
                 }
@@ -632,9 +633,9 @@ static bool regpanel_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T 
     a.part = ar.alloc_n<T>((size_t)2 * P * 64);
     a.rowv = ar.alloc_n<T>((size_t)2 * 64);
     a.rcp = ar.alloc_n<T>((size_t)P * 64);
-    a.bar = ar.alloc_n<unsigned>(1);
+    a.bar = grid_sync(ar.alloc_n<unsigned>(1));
     hipStream_t s = ctx().stream;
-    SKL_HIP(hipMemsetAsync(a.bar, 0, sizeof(unsigned), s));
+    SKL_HIP(hipMemsetAsync(a.bar.ctr, 0, sizeof(unsigned), s));
     BarrierLaunch guard((int)P);
     switch (K) {
     case 8: launch_regpanel<T, 8>((int)P, a, s); break;
@@ -702,10 +703,10 @@ static void rowslice_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T 
     a.red = ar.alloc_n<T>((size_t)2 * nc);
     a.rowv = ar.alloc_n<T>((size_t)2 * nc);
     a.rcp = ar.alloc_n<T>((size_t)P * nc);
-    a.bar = ar.alloc_n<unsigned>(1);
+    a.bar = grid_sync(ar.alloc_n<unsigned>(1));
     size_t sh = sizeof(T) * (size_t)a.ldl * nc + fixed;
     hipStream_t s = ctx().stream;
-    SKL_HIP(hipMemsetAsync(a.bar, 0, sizeof(unsigned), s));
+    SKL_HIP(hipMemsetAsync(a.bar.ctr, 0, sizeof(unsigned), s));
     if (sh > 65536)
         SKL_HIP(hipFuncSetAttribute((const void *)k_qr_rowslice<T>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
@@ -915,7 +916,7 @@ template <class T> struct YqArgs {
     int32_t *candp; // [2][P] candidate logical position
     T *ctau;        // [2][P] tau of the candidate's reflector
     T *cvec;        // [2][P][YQ_MAXROWS] candidate's reflector v (v_jj = 1, zero above)
-    unsigned *bar;  // barrier counter, zero at launch
+    GridSync bar;   // barrier counter (zero at launch), error word, poll limit
 };
 
 template <class T>
@@ -1055,7 +1056,7 @@ __global__ __launch_bounds__(64 * W) void k_yqrp(YqArgs<T> a) {
             }
         }
         SKL_PHASE(1, 1);
-        grid_barrier(a.bar, bar_target);
+        if (!grid_barrier(a.bar, bar_target)) return;
         SKL_PHASE(1, 2);
         // every workgroup's candidate, staged in LDS in one round trip
         for (int e = threadIdx.x; e < P * YQ_MAXROWS; e += 64 * W)
@@ -1277,7 +1278,7 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
             }
         }
         SKL_PHASE(1, 1);
-        grid_barrier(a.bar, bar_target);
+        if (!grid_barrier(a.bar, bar_target)) return;
         SKL_PHASE(1, 2);
         // every workgroup's candidate, staged in LDS in one round trip
         for (int e = threadIdx.x; e < P * YQ_MAXROWS; e += 64 * W)
@@ -1363,10 +1364,16 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
 }
 
 // changed[0..*nch) = ascending positions t with perm[t] != t (one workgroup)
+// (none when a grid barrier gave up: perm may then hold stale entries)
 __global__ __launch_bounds__(1024) void k_perm_changed(int nv, const int32_t *perm,
-                                                       int32_t *changed, int32_t *nch) {
+                                                       int32_t *changed, int32_t *nch,
+                                                       const unsigned *err) {
     __shared__ int s_cnt[16];
     __shared__ int s_base;
+    if (grid_failed(err)) {
+        if (threadIdx.x == 0) *nch = 0;
+        return;
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (threadIdx.x == 0) s_base = 0;
     __syncthreads();
@@ -1423,12 +1430,15 @@ __global__ void k_perm_int(int nv, int32_t *jp, const int32_t *perm, const int32
 // Apply the panel's column swaps (s <-> swaps[s], in order, b <= 64) to the
 // columns 0..b-1 of two row blocks.  The swaps are composed into one gather
 // (new column t = old column src[t]); each workgroup moves a 64-row tile
-// through LDS.  Workgroups [0, ceil(r1/64)) cover X1, the rest X2.
+// through LDS.  Workgroups [0, ceil(r1/64)) cover X1, the rest X2.  Nothing
+// moves when a grid barrier gave up (the swaps may be stale).
 template <class T>
 __global__ __launch_bounds__(256) void k_apply_swaps(int b, const int32_t *swaps, long r1, T *X1,
-                                                     long ld1, long r2, T *X2, long ld2) {
+                                                     long ld1, long r2, T *X2, long ld2,
+                                                     const unsigned *err) {
     __shared__ int src[64];
     __shared__ T tile[64 * 65];
+    if (grid_failed(err)) return;
     if (threadIdx.x == 0) {
         for (int s = 0; s < b; ++s) src[s] = s;
         for (int s = 0; s < b; ++s) {
@@ -1520,7 +1530,7 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
     int32_t *candp = ar.alloc_n<int32_t>((size_t)2 * P_max);
     T *ctau = ar.alloc_n<T>((size_t)2 * P_max);
     T *cvec = ar.alloc_n<T>((size_t)2 * P_max * YQ_MAXROWS);
-    unsigned *bar = ar.alloc_n<unsigned>(1);
+    const GridSync bar = grid_sync(ar.alloc_n<unsigned>(1));
     Context &c = ctx();
     const bool ahead = lookahead_on();
     if (ahead) c.make_side();
@@ -1556,7 +1566,7 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
                     SKL_HIP(hipFuncSetAttribute((const void *)k_yqrp_lanes<T, WL>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                                 (int)sh));
-                SKL_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned), s));
+                SKL_HIP(hipMemsetAsync(bar.ctr, 0, sizeof(unsigned), s));
                 BarrierLaunch guard(P);
                 k_yqrp_lanes<T, WL><<<P, 64 * WL, sh, s>>>(ya);
                 SKL_CHECK_LAUNCH();
@@ -1569,7 +1579,7 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
                 int K = nv <= 512 ? 1 : nv <= 1024 ? 2 : nv <= 2048 ? 4 : dbl ? 16 : 8;
                 int Wv = (K == 16) ? 8 : 16;
                 int P = (int)((nv + (long)Wv * K - 1) / ((long)Wv * K));
-                SKL_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned), s));
+                SKL_HIP(hipMemsetAsync(bar.ctr, 0, sizeof(unsigned), s));
                 constexpr int KW = sizeof(T) == 8 ? 16 : 8, WW = sizeof(T) == 8 ? 8 : 16;
                 auto fn = K == 1 ? k_yqrp<T, 1, 16> : K == 2 ? k_yqrp<T, 2, 16>
                           : K == 4 ? k_yqrp<T, 4, 16> : k_yqrp<T, KW, WW>;
@@ -1586,7 +1596,7 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
             // the side stream beside this pivot search; the permutation
             // moves those rows
             join_side();
-            k_perm_changed<<<1, 1024, 0, s>>>((int)nv, perm, changed, nch);
+            k_perm_changed<<<1, 1024, 0, s>>>((int)nv, perm, changed, nch, bar.err);
             // permute W(:, j:n), Y(:, j:n), jpvt(j:n) by the new column order
             dim3 gw((unsigned)((m + 255) / 256), (unsigned)(2 * b));
             k_perm_gather<T><<<gw, 256, 0, s>>>(m, W + j * ldw, ldw, perm, changed, nch, tmpc);
@@ -1604,7 +1614,7 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
         {
             unsigned nblk = (unsigned)((j + 63) / 64 + (mY + 63) / 64);
             k_apply_swaps<T><<<nblk, 256, 0, s>>>(b, swaps, j, W + j * ldw, ldw, mY, Y + j * mY,
-                                                  mY);
+                                                  mY, bar.err);
         }
         SKL_CHECK_LAUNCH();
         panel_t<T>(mp, b, W + j + j * ldw, ldw, tau + j, Vf, Tm, ar);

@@ -105,6 +105,8 @@ void ensure_init() {
     c->coop_ok = prop.cooperativeLaunch;
     c->lds_max = prop.sharedMemPerBlock;
     SKL_HIP(hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault));
+    SKL_HIP(hipMalloc(&c->dev_err, 256));
+    SKL_HIP(hipMemset(c->dev_err, 0, 256));
     t_ctx = c;
     g_live_ctx.fetch_add(1);
 }
@@ -237,6 +239,7 @@ void release_thread_context() {
         if (e) (void)hipEventDestroy(e);
     if (t_ctx->aux) (void)hipStreamDestroy(t_ctx->aux);
     if (t_ctx->pinned) (void)hipHostFree(t_ctx->pinned);
+    if (t_ctx->dev_err) (void)hipFree(t_ctx->dev_err);
     (void)hipStreamDestroy(t_ctx->stream);
     delete t_ctx;
     t_ctx = nullptr;
@@ -244,6 +247,26 @@ void release_thread_context() {
 }
 
 void set_device(int dev) { g_device = dev; }
+
+GridSync grid_sync(unsigned *ctr) {
+    Context &c = ctx();
+    return GridSync{ctr, c.dev_err, c.barrier_spins};
+}
+
+void check_grid_fault() {
+    Context &c = ctx();
+    // the last 64 bytes of the pinned block are this check's
+    unsigned *h = reinterpret_cast<unsigned *>(static_cast<char *>(c.pinned) + 4096 - 64);
+    *h = 0;
+    SKL_HIP(hipMemcpyAsync(h, c.dev_err, sizeof(unsigned), hipMemcpyDeviceToHost, c.stream));
+    SKL_HIP(hipStreamSynchronize(c.stream));
+    if (*h != 0) {
+        SKL_HIP(hipMemsetAsync(c.dev_err, 0, sizeof(unsigned), c.stream));
+        SKL_HIP(hipStreamSynchronize(c.stream));
+        throw Error(SKL_E_HIP, "grid barrier gave up waiting for the other workgroups of a "
+                               "factorization kernel; its results were discarded");
+    }
+}
 
 // ---------------------------------------------------------------------------
 // glibc srandom_r / random_r (TYPE_3: degree 31, separation 3): the generator

@@ -66,10 +66,21 @@ class Arena {
     std::vector<void *> overflow_; // extra blocks when the reservation was too small
 };
 
+// Arguments of the bounded grid barrier (k_common.h:grid_barrier)
+struct GridSync {
+    unsigned *ctr;  // arrival counter, zero at launch
+    unsigned *err;  // nonzero once a wait gave up (sticky until the host clears it)
+    unsigned spins; // polls before giving up
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     Arena arena;
+    // error word of the grid barriers launched from this context, and the
+    // number of polls after which a barrier wait gives up
+    unsigned *dev_err = nullptr;
+    unsigned barrier_spins = 1u << 28;
     int num_cu = 0;
     int coop_ok = 0;
     size_t lds_max = 0;
@@ -105,6 +116,11 @@ class OnSideStream {
 };
 
 Context &ctx();                       // default per-thread context
+// GridSync for a launch from the calling thread's context, counter ctr
+GridSync grid_sync(unsigned *ctr);
+// Waits for ctx().stream and throws Error(SKL_E_HIP) when a grid barrier
+// launched from this context gave up since the last check (clearing the word).
+void check_grid_fault();
 void ensure_init();
 void release_thread_context();        // frees the calling thread's context
 

@@ -555,7 +555,7 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
         double *ddg = ar.alloc_n<double>((size_t)d);
         hipLaunchKernelGGL(k_diag, dim3(gblocks(d)), dim3(256), 0, s, d, W, m, ddg);
         SKL_HIP(hipMemcpyAsync(dg.data(), ddg, sizeof(double) * d, hipMemcpyDeviceToHost, s));
-        sync();
+        check_grid_fault(); // before R and the pivots are used
         rank = d;
         if (cpqr) {
             rank = 0;
@@ -771,7 +771,7 @@ static long sparse_factor(long n, long d, long m, long k, const DevPattern &dp, 
     double *ddg = ar.alloc_n<double>((size_t)kmax);
     hipLaunchKernelGGL(k_diag<T>, dim3(gblocks(kmax)), dim3(256), 0, s, kmax, W, m, ddg);
     SKL_HIP(hipMemcpyAsync(dg.data(), ddg, sizeof(double) * kmax, hipMemcpyDeviceToHost, s));
-    sync();
+    check_grid_fault();
     bool full = true;
     for (long i = 0; i < kmax; ++i)
         if (!(std::fabs(dg[(size_t)i]) > tol)) {
@@ -786,7 +786,7 @@ static long sparse_factor(long n, long d, long m, long k, const DevPattern &dp, 
     dense_factor<T>(m, d + 1, d, true, W, jpvt, tau, ar);
     hipLaunchKernelGGL(k_diag<T>, dim3(gblocks(kmax)), dim3(256), 0, s, kmax, W, m, ddg);
     SKL_HIP(hipMemcpyAsync(dg.data(), ddg, sizeof(double) * kmax, hipMemcpyDeviceToHost, s));
-    sync();
+    check_grid_fault();
     long rank = 0;
     for (long i = 0; i < kmax; ++i) {
         if (std::fabs(dg[(size_t)i]) > tol)
@@ -1077,7 +1077,7 @@ static int qr_component(long m, long n, long nfac, double *A, long lda, double *
     } else {
         qr_householder<T>(m, n, nfac, W, m, dt, ar);
     }
-    sync();
+    check_grid_fault();
     download_as_double(W, m, n, m, A, lda);
     long kk = std::min(m, nfac);
     std::vector<T> ht((size_t)kk);
@@ -1144,6 +1144,14 @@ int skl_device_count(void) {
     int c = 0;
     if (hipGetDeviceCount(&c) != hipSuccess) return 0;
     return c;
+}
+
+void skl_set_barrier_spins(long spins) {
+    try {
+        skl::ctx().barrier_spins =
+            spins < 0 ? (1u << 28) : (unsigned)std::min<long>(spins, 0xffffffffL);
+    } catch (...) {
+    }
 }
 
 void skl_set_history(skl_lsqr_step *steps, long max_steps) {

@@ -68,7 +68,7 @@ EXPORTS = [
     "skl_dense_solve_blendenpik", "skl_dense_solve_blendenpik_dev", "skl_subsample_pattern",
     "skl_set_history", "skl_hash_pattern", "skl_dht", "skl_hash_apply", "skl_dense_sketch",
     "skl_sparse_sketch", "skl_qr", "skl_cpqr", "skl_trinv", "skl_lsqr_dense",
-    "skl_profile_enable", "skl_profile_reset", "skl_profile_read",
+    "skl_profile_enable", "skl_profile_reset", "skl_profile_read", "skl_set_barrier_spins",
     "skl_server_create", "skl_server_submit_dense", "skl_server_submit_sparse",
     "skl_server_wait", "skl_server_pending", "skl_server_destroy",
 ]
@@ -102,6 +102,13 @@ def _check(rc):
         raise SklError(rc, "SPQR Factorization fails")
     L = lib()
     raise SklError(rc, f"{L.skl_error_string(rc).decode()}: {L.skl_last_error().decode()}")
+
+
+def set_barrier_spins(spins):
+    """Poll limit of the factorization kernels' grid barriers on this thread;
+    negative restores the default.  A barrier that gives up makes the call
+    that launched it fail with SklError (code 4)."""
+    lib().skl_set_barrier_spins(C.c_long(int(spins)))
 
 
 def default_opts(**kw):

@@ -145,3 +145,34 @@ def test_lsqr_permuted_columns(skl, O):
     y, flag, it, _ = skl.lsqr_dense(A, M, b, 1e-12, 200, E=E)
     yo, *_ = O.lsqr_dense(A, R, b, 1e-12, 200, E=E)
     assert rel(y, yo) < 1e-10
+
+
+def test_grid_barrier_give_up_is_reported(skl, O):
+    """A grid barrier whose bounded wait gives up must surface as SKL_E_HIP,
+    never as a factor computed from stale data.  The poll limit is forced to
+    zero from the host, so the first workgroup to arrive at a barrier gives
+    up and every kernel still exits normally (k_common.h:grid_barrier)."""
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((3000, 300)) * np.logspace(0, -3, 300)  # multi-workgroup panels
+    Ah = rng.standard_normal((1200, 1000)) * np.logspace(0, -3, 1000)  # HQRRP
+    x_star = rng.standard_normal(300)
+    b = A @ x_star
+    skl.set_barrier_spins(0)
+    try:
+        for call in (lambda: skl.cpqr(A), lambda: skl.qr(A), lambda: skl.cpqr(Ah),
+                     lambda: skl.dense_solve(np.vstack([A, A]), np.concatenate([b, b]),
+                                             cpqr=True, it_tol=1e-10, seed=3)):
+            with pytest.raises(skl.SklError) as ei:
+                call()
+            assert ei.value.code == 4
+            assert "grid barrier" in str(ei.value)
+    finally:
+        skl.set_barrier_spins(-1)
+    # the error word was cleared: the same factorizations succeed again
+    W, jpvt, tau = skl.cpqr(A)
+    Wo, jo, to = O.geqp3(A)
+    assert np.array_equal(jpvt, jo)
+    assert rel(np.triu(W[:300]), np.triu(Wo[:300])) < 1e-12
+    W, jpvt, tau = skl.cpqr(Ah)
+    Wo, jo, to = O.cpqr(Ah)
+    assert np.array_equal(jpvt, jo)
