@@ -10,6 +10,15 @@
  */
 #include "skl_oracle.h"
 
+/* ORC_PLANT (default 0) compiles a deliberate mistake into the HQRRP Y
+ * downdate (1: wrong sign of the (G V) T U11^T term, 2: G not updated by the
+ * block reflector, 3: the last row of R12 dropped).  Only the tests build
+ * these variants, to show that the per-block pivot check in
+ * tests/test_oracle.py fails on each of them. */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+
 #include <float.h>
 #include <math.h>
 #include <stdlib.h>
@@ -657,18 +666,28 @@ static void orc_downdate_y(long b, const double *U, long ldu, long mu, const dou
         for (long r = b - 1; r >= 0; --r) {
             double s = B[i + r * my];
             for (long q = 0; q < r; ++q) s += B[i + q * my] * U[r + q * ldu];
+#if ORC_PLANT == 1
+            B[i + r * my] = s; /* planted: wrong sign of the (G V) T U11^T term */
+#else
             B[i + r * my] = -s;
+#endif
         }
     for (long r = 0; r < b; ++r)
         for (long i = 0; i < my; ++i) B[i + r * my] += G1[i + r * ldg];
     for (long c = 0; c < nr; ++c)
         for (long i = 0; i < my; ++i) {
             double s = 0.0;
+#if ORC_PLANT == 3
+            for (long r = 0; r + 1 < b; ++r) s += B[i + r * my] * R12[r + c * ldr]; /* planted: last row of R12 dropped */
+#else
             for (long r = 0; r < b; ++r) s += B[i + r * my] * R12[r + c * ldr];
+#endif
             Y2[i + c * ldy] -= s;
         }
     free(B);
+#if ORC_PLANT != 2
     orc_larfb_right_n(mu, b, U, ldu, T, ldt, mg, G1, ldg);
+#endif /* ORC_PLANT == 2 (planted): G is not carried along with Q */
 }
 
 void orc_hqrrp(long m, long n, double *A, long lda, long *jpvt, double *tau, long nb,

@@ -97,3 +97,51 @@ def sparse_problem(seed=0, n=3000, d=100, density=0.02, decades=8, unit_diag=Tru
     xs = rng.standard_normal(d)
     b = A @ xs + 1e-8 * rng.standard_normal(n)
     return A, b
+
+
+def hqrrp_block_mismatches(O, A, QR, jpvt, tau, nb, pp):
+    """Per-block check of an HQRRP factorization against the mathematics it
+    rests on (hqrrp/lapack_compatible_sources/NoFLA_HQRRP_WY_blk_var4_skint.c:
+    422-568, Downdate_Y :627-718).  After j blocks the downdated sketch of the
+    remaining columns is, in exact arithmetic,
+        Y_j = (G Q_1 ... Q_j)[:, j:m] (Q_j^T ... Q_1^T A)[j:m, remaining],
+    with G the pinned Gaussian (srand(12)) and Q_i the block reflectors the
+    factorization stored.  Block j + 1 must pick the first b greedy (dgeqp3)
+    pivots of Y_j, and its panel must order them as the greedy QRP of the
+    trailing A restricted to them.  Y_j is rebuilt here from the final
+    reflectors by numpy/LAPACK, independently of the oracle's running
+    downdate.  Returns the blocks that disagree."""
+    import scipy.linalg
+    m, n = A.shape
+    mY = nb + pp
+    Acur = np.array(A, dtype=np.float64)              # Q^T A, original column order
+    Gcur = O.gaussian_matrix(O.rng(12), mY, m)         # G Q
+    jp = np.asarray(jpvt) - 1
+    bad = []
+    for j in range(0, min(m, n), nb):
+        b = min(nb, n - j, m - j)
+        last = (j + nb >= m) or (j + nb >= n)
+        cand = np.sort(jp[j:])
+        if not last:
+            Yh = Gcur[:, j:] @ Acur[j:, cand]
+            _, P = scipy.linalg.qr(Yh, pivoting=True, mode="r")
+            chosen = cand[P[:b]]
+            _, Pp = scipy.linalg.qr(Acur[j:, chosen], pivoting=True, mode="r")
+            expect = chosen[Pp]
+        else:
+            _, Pp = scipy.linalg.qr(Acur[j:, cand], pivoting=True, mode="r")
+            expect = cand[Pp[:b]]
+        if not np.array_equal(expect, jp[j:j + b]):
+            bad.append(j // nb)
+        # apply this block's reflectors (compact WY, dlarft forward columnwise)
+        V = np.zeros((m, b))
+        for i in range(b):
+            V[j + i, i] = 1.0
+            V[j + i + 1:, i] = QR[j + i + 1:, j + i]
+        T = np.zeros((b, b))
+        for i in range(b):
+            T[i, i] = tau[j + i]
+            T[:i, i] = -tau[j + i] * (T[:i, :i] @ (V[:, :i].T @ V[:, i]))
+        Acur -= V @ (T.T @ (V.T @ Acur))
+        Gcur -= (Gcur @ V) @ T @ V.T
+    return bad

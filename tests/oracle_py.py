@@ -45,6 +45,22 @@ def lib():
     return _lib
 
 
+def load_variant(path):
+    """Binds an oracle library built elsewhere (a planted-mistake variant of
+    skl_oracle.c); returns a function hqrrp(A, nb, pp) over it."""
+    L = C.CDLL(path)
+
+    def hqrrp_v(A, nb=64, pp=10):
+        A = fcopy(A)
+        m, n = A.shape
+        jpvt = np.zeros(n, dtype=np.int64)
+        tau = np.zeros(n)
+        L.orc_hqrrp(C.c_long(m), C.c_long(n), _d(A), C.c_long(m), _l(jpvt), _d(tau),
+                    C.c_long(nb), C.c_long(pp))
+        return A, jpvt, tau
+    return hqrrp_v
+
+
 def _d(a):
     return a.ctypes.data_as(dp)
 

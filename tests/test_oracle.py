@@ -12,7 +12,7 @@ import scipy.linalg as sl
 import scipy.sparse as sp
 import scipy.sparse.linalg as spl
 
-from conftest import GOLDEN, c1_problem, read_mtx, rel, sparse_problem
+from conftest import GOLDEN, c1_problem, hqrrp_block_mismatches, read_mtx, rel, sparse_problem
 
 libc = ctypes.CDLL("libc.so.6")
 
@@ -200,6 +200,38 @@ def test_hqrrp_vs_dgeqp3_on_large(O):
     assert np.allclose(np.linalg.norm(R, axis=0), np.linalg.norm(A[:, jpvt - 1], axis=0),
                        rtol=1e-12)
     assert d.min() > 0
+
+
+@pytest.mark.parametrize("m,n,nb,seed", [(400, 300, 16, 7), (600, 520, 32, 8),
+                                         (1100, 1000, 64, 9)])
+def test_hqrrp_every_block_pivots(O, m, n, nb, seed):
+    """Pins the oracle's HQRRP beyond its first block: every block's pivot set
+    and panel order follow from the exactly downdated sketch."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((m, n)) * np.logspace(0, -3, n)
+    QR, jpvt, tau = O.hqrrp(A, nb=nb, pp=10)
+    assert hqrrp_block_mismatches(O, A, QR, jpvt, tau, nb, 10) == []
+
+
+@pytest.mark.parametrize("plant", [1, 2, 3])
+def test_hqrrp_block_check_catches_planted_downdate_mistakes(O, tmp_path, plant):
+    """Each planted mistake in the Y downdate still yields a valid
+    factorization A P = Q R (the invariants alone cannot see it), but the
+    per-block pivot check above fails on it."""
+    import subprocess
+    import oracle_py
+    src = os.path.join(os.path.dirname(oracle_py.LIB_PATH), "skl_oracle.c")
+    so = str(tmp_path / f"liboracle_plant{plant}.so")
+    subprocess.check_call(["gcc", "-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off",
+                           f"-DORC_PLANT={plant}", "-shared", "-o", so, src, "-lm"])
+    hq = oracle_py.load_variant(so)
+    rng = np.random.default_rng(7)
+    m, n, nb = 400, 300, 16
+    A = rng.standard_normal((m, n)) * np.logspace(0, -3, n)
+    QR, jpvt, tau = hq(A, nb=nb, pp=10)
+    Q = q_from(QR, tau)
+    assert rel(A[:, jpvt - 1], Q[:, :n] @ np.triu(QR[:n])) < 1e-13
+    assert len(hqrrp_block_mismatches(O, A, QR, jpvt, tau, nb, 10)) > 0
 
 
 def test_ormqr_and_trsv(O):
