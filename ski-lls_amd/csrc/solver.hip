@@ -467,28 +467,68 @@ static bool mixed_cholqr(long m, long d, const T *SA, long ldsa, double *Rd, dou
     return ok;
 }
 
-// Iterative refinement of the sketch-and-solve start: y <- y + M M^T SA^T
-// (c - SA y), `steps` times, with SA and c the FP64 sketch and M the inverse
-// of the FP32 factor.  The start's error contracts by about ||I - (SA M)^T
-// (SA M)|| = O(u_32 cond) per step, so a few steps give the FP64 sketched
-// solution; without them the FP32 start leaves A x0 - b far above the
-// least-squares residual and LSQR spends its first iterations closing that gap.
-static void refine_sketched_solution(long m, long d, const double *SA, long ldsa,
-                                     const double *c, const double *M, long ldm, double *y,
-                                     int steps, Arena &ar) {
+__global__ void k_axpy_s(long n, double a, const double *x, double *y) { // y += a x
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+__global__ void k_xpby(long n, const double *x, double b, double *y) { // y = x + b y
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = x[i] + b * y[i];
+}
+
+// The sketch-and-solve start from the FP64 sketch [SA | c] and the inverse M
+// of the FP32 factor: CGLS on min ||c - (SA M) z|| from z = 0, y = M z.  SA M
+// is nearly orthonormal (its normal matrix is I + E with ||E|| = O(u_32
+// cond(SA D^-1)^2)), so the iteration converges in a few steps to the FP64
+// sketched solution; it stops when the normal-equation residual ||M^T SA^T
+// r|| has fallen by `tol` or after max_steps.  Without it the FP32 factor's
+// own solution would leave A x0 - b far above the least-squares residual and
+// LSQR would spend its first iterations closing the gap.  Returns the steps
+// taken.
+static int refine_sketched_solution(long m, long d, const double *SA, long ldsa,
+                                    const double *c, const double *M, long ldm, double *y,
+                                    int max_steps, double tol, Arena &ar) {
     hipStream_t s = ctx().stream;
     const size_t mark = ar.used();
-    double *rsk = ar.alloc_n<double>((size_t)m);
-    double *g = ar.alloc_n<double>((size_t)d);
+    double *r = ar.alloc_n<double>((size_t)m);
+    double *q = ar.alloc_n<double>((size_t)m);
+    double *t = ar.alloc_n<double>((size_t)d);
+    double *sv = ar.alloc_n<double>((size_t)d);
+    double *p = ar.alloc_n<double>((size_t)d);
     double *h = ar.alloc_n<double>((size_t)d);
-    for (int it = 0; it < steps; ++it) {
-        SKL_HIP(hipMemcpyAsync(rsk, c, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
-        dev_gemv_n_dense(m, d, SA, ldsa, y, -1.0, 1.0, rsk, ar);
-        dev_gemv_t_dense(m, d, SA, ldsa, rsk, 1.0, 0.0, g, ar);
-        dev_gemv_t_dense(d, d, M, ldm, g, 1.0, 0.0, h, ar);
-        dev_gemv_n_dense(d, d, M, ldm, h, 1.0, 1.0, y, ar);
+    double *z = ar.alloc_n<double>((size_t)d);
+    const size_t mark2 = ar.used();
+    SKL_HIP(hipMemcpyAsync(r, c, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_fill_d, dim3(gblocks(d)), dim3(256), 0, s, d, z, 0.0);
+    dev_gemv_t_dense(m, d, SA, ldsa, r, 1.0, 0.0, t, ar);
+    dev_gemv_t_dense(d, d, M, ldm, t, 1.0, 0.0, sv, ar);
+    SKL_HIP(hipMemcpyAsync(p, sv, sizeof(double) * d, hipMemcpyDeviceToDevice, s));
+    double gamma = dev_nrm2(d, sv, ar);
+    gamma *= gamma;
+    const double stop = tol * tol * gamma;
+    int k = 0;
+    while (k < max_steps && gamma > stop) {
+        ar.rewind(mark2);
+        dev_gemv_n_dense(d, d, M, ldm, p, 1.0, 0.0, h, ar);
+        dev_gemv_n_dense(m, d, SA, ldsa, h, 1.0, 0.0, q, ar);
+        double qn = dev_nrm2(m, q, ar);
+        if (!(qn > 0.0)) break;
+        const double alpha = gamma / (qn * qn);
+        hipLaunchKernelGGL(k_axpy_s, dim3(gblocks(d)), dim3(256), 0, s, d, alpha, p, z);
+        hipLaunchKernelGGL(k_axpy_s, dim3(gblocks(m)), dim3(256), 0, s, m, -alpha, q, r);
+        dev_gemv_t_dense(m, d, SA, ldsa, r, 1.0, 0.0, t, ar);
+        dev_gemv_t_dense(d, d, M, ldm, t, 1.0, 0.0, sv, ar);
+        double gn = dev_nrm2(d, sv, ar);
+        gn *= gn;
+        hipLaunchKernelGGL(k_xpby, dim3(gblocks(d)), dim3(256), 0, s, d, sv, gn / gamma, p);
+        gamma = gn;
+        ++k;
     }
+    ar.rewind(mark2);
+    dev_gemv_n_dense(d, d, M, ldm, z, 1.0, 0.0, y, ar);
+    SKL_CHECK_LAUNCH();
     ar.rewind(mark);
+    return k;
 }
 
 // cpqr: ls_dense_hashing_blendenpik (sketch-and-solve start, CPQR, rank);
@@ -676,8 +716,7 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     if (W64) {
         // mixed, no fallback: the sketch-and-solve start from the FP64 sketch
         // and the optimistic factor, refined from zero
-        hipLaunchKernelGGL(k_fill_d, dim3(gblocks(rank)), dim3(256), 0, s, rank, c1, 0.0);
-        refine_sketched_solution(m, d, W64, m, W64 + d * m, Mi, rank, c1, 4, ar);
+        refine_sketched_solution(m, d, W64, m, W64 + d * m, Mi, rank, c1, 40, 1e-11, ar);
     } else {
         trmv_n(rank, Mi, rank, cvec, c1, ar);
     }
@@ -908,8 +947,7 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
     double *c1 = ar.alloc_n<double>((size_t)rank);
     if (W64) {
         // mixed, no fallback: the start from the FP64 sketch, refined from zero
-        hipLaunchKernelGGL(k_fill_d, dim3(gblocks(rank)), dim3(256), 0, s, rank, c1, 0.0);
-        refine_sketched_solution(m, d, W64, m, W64 + d * m, Mi, rank, c1, 4, ar);
+        refine_sketched_solution(m, d, W64, m, W64 + d * m, Mi, rank, c1, 40, 1e-11, ar);
     } else {
         trmv_n(rank, Mi, rank, cvec, c1, ar);
     }

@@ -24,8 +24,22 @@ def load_skl():
     return mod
 
 
+def _torch_runtime_first():
+    """torch ships its own HIP and HSA runtime libraries beside the ROCm ones
+    libskl.so links; two runtimes share the process only when torch's starts
+    first (once libskl's holds the device, torch reports no GPU).  Tests that
+    take device memory from torch therefore need torch initialised before the
+    library's first call."""
+    try:
+        import torch
+        torch.cuda.is_available()
+    except Exception:
+        pass
+
+
 @pytest.fixture(scope="session")
 def skl():
+    _torch_runtime_first()
     return load_skl()
 
 
@@ -134,14 +148,30 @@ def hqrrp_block_mismatches(O, A, QR, jpvt, tau, nb, pp):
         if not np.array_equal(expect, jp[j:j + b]):
             bad.append(j // nb)
         # apply this block's reflectors (compact WY, dlarft forward columnwise)
-        V = np.zeros((m, b))
+        # to the rows j: of the columns still to be factored, and to G(:, j:)
+        V = np.zeros((m - j, b))
         for i in range(b):
-            V[j + i, i] = 1.0
-            V[j + i + 1:, i] = QR[j + i + 1:, j + i]
+            V[i, i] = 1.0
+            V[i + 1:, i] = QR[j + i + 1:, j + i]
         T = np.zeros((b, b))
         for i in range(b):
             T[i, i] = tau[j + i]
             T[:i, i] = -tau[j + i] * (T[:i, :i] @ (V[:, :i].T @ V[:, i]))
-        Acur -= V @ (T.T @ (V.T @ Acur))
-        Gcur -= (Gcur @ V) @ T @ V.T
+        rest = np.sort(jp[j + b:])
+        if rest.size:
+            sub = Acur[j:, rest]
+            Acur[j:, rest] = sub - V @ (T.T @ (V.T @ sub))
+        Gcur[:, j:] -= (Gcur[:, j:] @ V) @ T @ V.T
     return bad
+
+
+def c2_problem(seed=0, n=40000, d=4000, lognormal=2.0, decades=6):
+    """Dense problem of the 40000x4000 configuration family: A = diag(w) G
+    diag(logspace(0,-decades,d)), G iid N(0,1), w_i lognormal(0, lognormal)
+    (coherent rows); b = A x* + 1e-8 noise."""
+    rng = np.random.default_rng(seed)
+    w = np.exp(lognormal * rng.standard_normal(n))
+    A = (rng.standard_normal((n, d)) * np.logspace(0, -decades, d)) * w[:, None]
+    xs = rng.standard_normal(d)
+    b = A @ xs + 1e-8 * rng.standard_normal(n)
+    return np.asfortranarray(A), b
