@@ -467,6 +467,7 @@ template <class T> static cplx<T> *make_twiddles(long L) {
         h[(size_t)e] = {(T)cosl(ang), (T)sinl(ang)};
     }
     cplx<T> *dptr = nullptr;
+    AllocLock lk(capture_mutex());
     SKL_HIP(hipMalloc(&dptr, sizeof(cplx<T>) * (size_t)std::max<long>(L, 1)));
     SKL_HIP(hipMemcpy(dptr, h.data(), sizeof(cplx<T>) * (size_t)L, hipMemcpyHostToDevice));
     return dptr;
@@ -732,10 +733,13 @@ template <class T> static const BsPlan<T> &get_bs_plan(long n) {
         b[(size_t)j] = cj;
         if (j > 0) b[(size_t)(L - j)] = cj;
     }
-    SKL_HIP(hipMalloc(&bp.w, sizeof(cplx<T>) * (size_t)n));
-    SKL_HIP(hipMalloc(&bp.B, sizeof(cplx<T>) * (size_t)L));
-    SKL_HIP(hipMemcpy(bp.w, w.data(), sizeof(cplx<T>) * (size_t)n, hipMemcpyHostToDevice));
-    SKL_HIP(hipMemcpy(bp.B, b.data(), sizeof(cplx<T>) * (size_t)L, hipMemcpyHostToDevice));
+    {
+        AllocLock lk(capture_mutex());
+        SKL_HIP(hipMalloc(&bp.w, sizeof(cplx<T>) * (size_t)n));
+        SKL_HIP(hipMalloc(&bp.B, sizeof(cplx<T>) * (size_t)L));
+        SKL_HIP(hipMemcpy(bp.w, w.data(), sizeof(cplx<T>) * (size_t)n, hipMemcpyHostToDevice));
+        SKL_HIP(hipMemcpy(bp.B, b.data(), sizeof(cplx<T>) * (size_t)L, hipMemcpyHostToDevice));
+    }
     {
         // B = FFT_L(b) once per length, in place
         Arena tmp;

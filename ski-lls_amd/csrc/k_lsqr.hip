@@ -906,11 +906,27 @@ template <int NC> static void launch_fused_nc(const LsqrProblem &p, const LsqrPl
                                               const LsqrBufs &B, hipStream_t s, hipEvent_t e0,
                                               hipEvent_t e1) {
     const size_t sh = sizeof(double) * ((size_t)NC * FR_COLS + 2 * FR_W);
+    launch_ev(k_fused_rows<NC>, dim3(q.G), dim3(FR_T), (uint32_t)sh, s, e0, e1, q.n, q.d, B.AT,
+              q.ldt, B.z, B.ut, B.Sb, B.Pt, B.st);
+}
+
+// The one-pass kernel's dynamic LDS exceeds the default 64 KB at 16 column
+// pairs per thread; the attribute is set before the first launch (never
+// inside a graph capture).
+template <int NC> static void fused_attr_nc() {
+    const size_t sh = sizeof(double) * ((size_t)NC * FR_COLS + 2 * FR_W);
     if (sh > 65536)
         SKL_HIP(hipFuncSetAttribute((const void *)k_fused_rows<NC>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    launch_ev(k_fused_rows<NC>, dim3(q.G), dim3(FR_T), (uint32_t)sh, s, e0, e1, q.n, q.d, B.AT,
-              q.ldt, B.z, B.ut, B.Sb, B.Pt, B.st);
+}
+static void fused_attr(int nc) {
+    switch (nc) {
+    case 1: fused_attr_nc<1>(); break;
+    case 2: fused_attr_nc<2>(); break;
+    case 4: fused_attr_nc<4>(); break;
+    case 8: fused_attr_nc<8>(); break;
+    default: fused_attr_nc<16>(); break;
+    }
 }
 
 static void launch_fused(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs &B,
@@ -938,6 +954,7 @@ static thread_local hipEvent_t g_ev[8] = {};
 
 static void prof_events() {
     if (g_ev[0]) return;
+    AllocLock lk(capture_mutex());
     for (auto &e : g_ev) SKL_HIP(hipEventCreate(&e));
 }
 
@@ -1033,7 +1050,10 @@ struct GraphKey {
 static thread_local std::map<GraphKey, hipGraphExec_t> g_graphs;
 
 void lsqr_release_graphs() {
-    for (auto &kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
+    if (!g_graphs.empty()) {
+        AllocLock lk(capture_mutex());
+        for (auto &kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
+    }
     g_graphs.clear();
 }
 
@@ -1062,6 +1082,7 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
     hipStream_t s = C.stream;
     LsqrPlan q = make_plan(p, C.num_cu);
     if (q.fused && !p.AT && !rows_fit(p, ar)) q.fused = false;
+    if (q.fused) fused_attr(q.nc);
     if (q.spg > 0 && sizeof(double) * q.d > 65536)
         SKL_HIP(hipFuncSetAttribute((const void *)k_spmv_n_lds<8>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1151,20 +1172,35 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
         if (it != g_graphs.end()) {
             exec = it->second;
         } else {
-            hipGraph_t graph;
-            SKL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (int c = 0; c < chunk; ++c) launch_iteration(p, q, B, s);
-            hipError_t e1 = hipMemcpyAsync(hf, &B.st->done, sizeof(int), hipMemcpyDeviceToHost, s);
-            hipError_t e2 = hipStreamEndCapture(s, &graph);
+            hipGraph_t graph = nullptr;
+            hipError_t e1, e2;
+            {
+                // no other thread allocates or frees device memory meanwhile
+                CaptureLock lk(capture_mutex());
+                SKL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                for (int c = 0; c < chunk; ++c) launch_iteration(p, q, B, s);
+                e1 = hipMemcpyAsync(hf, &B.st->done, sizeof(int), hipMemcpyDeviceToHost, s);
+                e2 = hipStreamEndCapture(s, &graph);
+            }
             if (e1 != hipSuccess || e2 != hipSuccess) {
+                // run this solve without a graph: make sure the stream has
+                // left capture mode and drop what was recorded
                 graph_ok = false;
+                hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+                if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+                    hipGraph_t g2 = nullptr;
+                    (void)hipStreamEndCapture(s, &g2);
+                    if (g2) (void)hipGraphDestroy(g2);
+                }
+                if (graph) (void)hipGraphDestroy(graph);
                 (void)hipGetLastError();
             } else {
-                SKL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-                SKL_HIP(hipGraphDestroy(graph));
                 // bounded cache: callers whose buffers move every call (host
                 // inputs staged afresh) would otherwise grow it without end
                 if (g_graphs.size() >= 32) lsqr_release_graphs();
+                AllocLock lk(capture_mutex());
+                SKL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                SKL_HIP(hipGraphDestroy(graph));
                 g_graphs[key] = exec;
             }
         }

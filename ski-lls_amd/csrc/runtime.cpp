@@ -32,8 +32,11 @@ double wall() {
 Arena::~Arena() { release(); }
 
 void Arena::release() {
-    if (base_) (void)hipFree(base_);
-    for (void *p : overflow_) (void)hipFree(p);
+    if (base_ || !overflow_.empty()) {
+        AllocLock lk(capture_mutex());
+        if (base_) (void)hipFree(base_);
+        for (void *p : overflow_) (void)hipFree(p);
+    }
     overflow_.clear();
     base_ = nullptr;
     cap_ = used_ = 0;
@@ -42,6 +45,7 @@ void Arena::release() {
 void Arena::reserve(size_t bytes) {
     if (bytes <= cap_) return;
     if (used_ != 0) return; // cannot move live buffers; alloc() falls back
+    AllocLock lk(capture_mutex());
     if (base_) SKL_HIP(hipFree(base_));
     base_ = nullptr;
     cap_ = 0;
@@ -75,7 +79,10 @@ void *Arena::alloc(size_t bytes) {
         return p;
     }
     void *p = nullptr;
-    SKL_HIP(hipMalloc(&p, sz));
+    {
+        AllocLock lk(capture_mutex());
+        SKL_HIP(hipMalloc(&p, sz));
+    }
     overflow_.push_back(p);
     over_ += sz;
     peak_ = std::max(peak_, used_ + over_);
@@ -96,6 +103,7 @@ void ensure_init() {
     if (e != hipSuccess || count == 0) throw Error(SKL_E_NODEVICE, "no HIP device available");
     int dev = g_device >= 0 ? g_device : 0;
     SKL_HIP(hipSetDevice(dev));
+    AllocLock alk(capture_mutex());
     Context *c = new Context();
     c->device = dev;
     SKL_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -162,7 +170,11 @@ BarrierLaunch::~BarrierLaunch() {
             g_tok_events.pop_back();
         }
     }
-    bool ok = ev != nullptr || hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+    bool ok = ev != nullptr;
+    if (!ok) {
+        AllocLock alk(capture_mutex());
+        ok = hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+    }
     ok = ok && hipEventRecord(ev, ctx().stream) == hipSuccess;
     std::lock_guard<std::mutex> lk(g_tok_mu);
     if (ok) {
@@ -185,6 +197,7 @@ void lsqr_release_graphs();
 
 void *Context::workspace(size_t bytes) {
     if (bytes > ws_cap) {
+        AllocLock lk(capture_mutex());
         SKL_HIP(hipStreamSynchronize(stream));
         if (ws) SKL_HIP(hipFree(ws));
         ws = nullptr;
@@ -197,12 +210,14 @@ void *Context::workspace(size_t bytes) {
 
 void Context::make_side() {
     if (side) return;
+    AllocLock lk(capture_mutex());
     SKL_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     for (auto &e : ev) SKL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 void Context::make_aux() {
     if (aux) return;
+    AllocLock lk(capture_mutex());
     SKL_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
     for (auto &e : aux_ev) SKL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
@@ -229,7 +244,8 @@ void release_thread_context() {
     if (t_ctx->side) (void)hipStreamSynchronize(t_ctx->side);
     if (t_ctx->aux) (void)hipStreamSynchronize(t_ctx->aux);
     lsqr_release_graphs();
-    t_ctx->arena.release();
+    t_ctx->arena.release(); // (takes the capture lock itself)
+    AllocLock alk(capture_mutex());
     if (t_ctx->ws) (void)hipFree(t_ctx->ws);
     if (t_ctx->ws_side) (void)hipFree(t_ctx->ws_side);
     for (auto &e : t_ctx->ev)
@@ -241,12 +257,18 @@ void release_thread_context() {
     if (t_ctx->pinned) (void)hipHostFree(t_ctx->pinned);
     if (t_ctx->dev_err) (void)hipFree(t_ctx->dev_err);
     (void)hipStreamDestroy(t_ctx->stream);
+    alk.unlock(); // the arena's destructor below owns nothing any more
     delete t_ctx;
     t_ctx = nullptr;
     g_live_ctx.fetch_sub(1);
 }
 
 void set_device(int dev) { g_device = dev; }
+
+std::shared_mutex &capture_mutex() {
+    static std::shared_mutex mu;
+    return mu;
+}
 
 GridSync grid_sync(unsigned *ctr) {
     Context &c = ctx();

@@ -11,6 +11,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <string>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -40,6 +41,7 @@ struct Request {
     size_t footprint = 0;
     clk::time_point submitted, started, finished;
     int rc = -1;
+    std::string err; // skl_last_error() of the worker when rc != SKL_OK
     skl_info info{};
     bool done = false;
 };
@@ -110,6 +112,7 @@ static void worker_main(skl_server *S) {
             break;
         }
         r->finished = clk::now();
+        if (r->rc != SKL_OK) r->err = skl_last_error();
         {
             std::lock_guard<std::mutex> lk(S->mu);
             S->in_use -= r->footprint;
@@ -210,6 +213,7 @@ int skl_server_wait(skl_server *S, long ticket, skl_info *info, double *queue_s,
     if (queue_s) *queue_s = sec(r->started - r->submitted).count();
     if (run_s) *run_s = sec(r->finished - r->started).count();
     int rc = r->rc;
+    if (rc != SKL_OK) skl::set_last_error(r->err);
     delete r;
     return rc;
 }

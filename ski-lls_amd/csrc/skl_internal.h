@@ -5,6 +5,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
+#include <shared_mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -35,6 +37,18 @@ struct Error : std::runtime_error {
     do {                                             \
         if (!(cond)) throw ::skl::Error((code), (msg)); \
     } while (0)
+
+// ---------------------------------------------------------------------------
+// Graph capture vs. device allocation.  The runtime invalidates a stream
+// capture in progress on one thread when another thread frees device memory
+// (hipFree waits for every stream of the device) or otherwise allocates, so
+// the LSQR graph captures hold this lock shared and every device or pinned
+// allocation, free, stream/event creation and graph instantiation holds it
+// exclusively.
+// ---------------------------------------------------------------------------
+std::shared_mutex &capture_mutex();
+using AllocLock = std::unique_lock<std::shared_mutex>;
+using CaptureLock = std::shared_lock<std::shared_mutex>;
 
 // ---------------------------------------------------------------------------
 // Device memory: a grow-only arena per context; buffers are carved with a
@@ -121,6 +135,8 @@ GridSync grid_sync(unsigned *ctr);
 // Waits for ctx().stream and throws Error(SKL_E_HIP) when a grid barrier
 // launched from this context gave up since the last check (clearing the word).
 void check_grid_fault();
+// sets the message skl_last_error() returns on the calling thread
+void set_last_error(const std::string &msg);
 void ensure_init();
 void release_thread_context();        // frees the calling thread's context
 

@@ -44,6 +44,7 @@ void csr_gemv(long n, const int32_t *ptr, const int32_t *col, const double *val,
 void set_device(int dev);
 
 static thread_local std::string t_err;
+void set_last_error(const std::string &msg) { t_err = msg; }
 static thread_local skl_lsqr_step *t_hist = nullptr;
 static thread_local long t_hist_len = 0;
 
@@ -1041,9 +1042,15 @@ namespace skl {
 // Host-pointer variants: inputs are staged in device buffers owned by the call.
 struct DevBuf {
     void *p = nullptr;
-    explicit DevBuf(size_t bytes) { SKL_HIP(hipMalloc(&p, std::max<size_t>(bytes, 8))); }
+    explicit DevBuf(size_t bytes) {
+        AllocLock lk(capture_mutex());
+        SKL_HIP(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+    }
     ~DevBuf() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            AllocLock lk(capture_mutex());
+            (void)hipFree(p);
+        }
     }
     template <class T> T *as() { return static_cast<T *>(p); }
 };
