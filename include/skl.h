@@ -131,6 +131,25 @@ int skl_sparse_solve_dev(long n, long d, long nnz, const long *dcolptr, const lo
                          const double *dval, const double *db, double *dx,
                          const skl_opts *opts, skl_info *info);
 
+/* ls_dense_hashing_blendenpik without device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory or create
+ * streams (code being captured into a HIP graph, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and errors as
+ * skl_dense_solve_dev, computed by the same kernels, with every buffer carved
+ * from a static 64 MiB device array of the library (one per GPU, held until
+ * the process ends) instead of the per-thread arena; the LSQR operator runs
+ * as its two-pass kernels, without graph replay, and the QR without its
+ * second stream.  skl_init must have been called on the calling thread.
+ * Limits: opts->precision SKL_FP64 only (SKL_E_UNSUPPORTED otherwise);
+ * problems whose scratch exceeds 64 MiB (roughly 8 (m (d + 1) + 4 d^2 + 8 n)
+ * bytes, m = ceil(gamma d)) are refused with SKL_E_UNSUPPORTED before any
+ * work, as are transform lengths n with a prime factor above 5 that no
+ * earlier call of the library has planned.  Calls share the array: calls
+ * from several threads of one process take turns (they would race on it
+ * otherwise), and each call leaves its scratch contents behind. */
+int skl_dense_solve_dev_noalloc(long n, long d, const double *dA, long lda, const double *db,
+                                double *dx, const skl_opts *opts, skl_info *info);
+
 /* Kernel timing of the LSQR operator products on this thread: when enabled,
  * the LSQR loop runs without graph replay and HIP events on the library
  * stream bracket the four products of the second iteration of every chunk

@@ -203,7 +203,7 @@ void gemm(char ta, char tb, long M, long N, long K, T alpha, const T *A, long ld
     // (stream-ordered, so the next GEMM may reuse it).
     int S = 1;
     long kchunk = 0;
-    if (batch == 1 && tm * tn < 128 && K >= 512) {
+    if (batch == 1 && tm * tn < 128 && K >= 512 && !ctx().no_alloc) {
         long want = std::min<long>((256 + tm * tn - 1) / (tm * tn), K / 256);
         long cap = (64L << 20) / ((long)sizeof(T) * M * N);
         S = (int)std::max<long>(1, std::min<long>({want, cap, 64}));

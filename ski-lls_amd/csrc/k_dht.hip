@@ -459,17 +459,32 @@ static bool factor_dims(long L, FftDims &d) {
     return x == 1;
 }
 
-template <class T> static cplx<T> *make_twiddles(long L) {
+template <class T> static std::vector<cplx<T>> twiddle_values(long L) {
     std::vector<cplx<T>> h((size_t)L);
     for (long e = 0; e < L; ++e) {
         long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)e /
                           (long double)L;
         h[(size_t)e] = {(T)cosl(ang), (T)sinl(ang)};
     }
+    return h;
+}
+
+template <class T> static cplx<T> *make_twiddles(long L) {
+    std::vector<cplx<T>> h = twiddle_values<T>(L);
     cplx<T> *dptr = nullptr;
     AllocLock lk(capture_mutex());
     SKL_HIP(hipMalloc(&dptr, sizeof(cplx<T>) * (size_t)std::max<long>(L, 1)));
     SKL_HIP(hipMemcpy(dptr, h.data(), sizeof(cplx<T>) * (size_t)L, hipMemcpyHostToDevice));
+    return dptr;
+}
+
+// the same table in arena memory (no-allocation mode; not cached)
+template <class T> static cplx<T> *arena_twiddles(long L, Arena &ar) {
+    std::vector<cplx<T>> h = twiddle_values<T>(L);
+    cplx<T> *dptr = ar.alloc_n<cplx<T>>((size_t)std::max<long>(L, 1));
+    hipStream_t s = ctx().stream;
+    SKL_HIP(hipMemcpyAsync(dptr, h.data(), sizeof(cplx<T>) * (size_t)L, hipMemcpyHostToDevice, s));
+    SKL_HIP(hipStreamSynchronize(s)); // h goes out of scope
     return dptr;
 }
 
@@ -785,12 +800,22 @@ static void dht_bluestein(long n, long ncols, const double *X, long ldx, const d
     }
 }
 
-template <class T> static const FftPlanDev<T> &get_plan(long n) {
+// Transform plan of length n, cached per length with its tables in device
+// memory of its own.  In no-allocation mode a length not cached yet gets an
+// uncached plan whose tables live in the arena (lengths with a prime factor
+// above 5, whose Bluestein tables are larger, are refused there).
+template <class T> static const FftPlanDev<T> &get_plan(long n, Arena &ar) {
     static std::map<long, FftPlanDev<T>> cache;
     static std::mutex mu;
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find(n);
     if (it != cache.end()) return it->second;
+    const bool scratch = ctx().no_alloc;
+    if (scratch && !smooth(n))
+        throw Error(SKL_E_UNSUPPORTED,
+                    "no-allocation entry point: transform length " + std::to_string(n) +
+                        " has a prime factor above 5");
+    auto tables = [&](long L) { return scratch ? arena_twiddles<T>(L, ar) : make_twiddles<T>(L); };
     FftPlanDev<T> p;
     p.n = n;
     const long cap = lds_elems<T>();
@@ -799,7 +824,7 @@ template <class T> static const FftPlanDev<T> &get_plan(long n) {
     } else if (n <= cap) {
         p.mode = 1;
         factor_dims(n, p.d1);
-        p.tw1 = make_twiddles<T>(n);
+        p.tw1 = tables(n);
     } else {
         // n = n1 n2, both smooth, n1 <= cap/4 (J2 >= 4), n2 <= cap/8 (K1 >= 4)
         long best1 = -1;
@@ -822,11 +847,16 @@ template <class T> static const FftPlanDev<T> &get_plan(long n) {
         p.n2 = (int)(n / best1);
         factor_dims(p.n1, p.d1);
         factor_dims(p.n2, p.d2);
-        p.tw1 = make_twiddles<T>(p.n1);
-        p.tw2 = make_twiddles<T>(p.n2);
-        p.twn = make_twiddles<T>(n);
+        p.tw1 = tables(p.n1);
+        p.tw2 = tables(p.n2);
+        p.twn = tables(n);
         p.J2 = (int)std::min<long>(32, cap / p.n1);
         p.K1 = (int)std::min<long>(16, cap / (2L * p.n2));
+    }
+    if (scratch) {
+        thread_local FftPlanDev<T> tmp;
+        tmp = p;
+        return tmp;
     }
     auto res = cache.emplace(n, p);
     return res.first->second;
@@ -860,7 +890,7 @@ template <class T>
 void dht_sketch(long n, long ncols, const double *X, long ldx, const double *D, double scale,
                 long m, const int32_t *bptr, const int32_t *bsrc, const T *bcoef, T *SA,
                 long ldsa, Arena &ar) {
-    const FftPlanDev<T> &P = get_plan<T>(n);
+    const FftPlanDev<T> &P = get_plan<T>(n, ar);
     constexpr int NT = fft_nt<T>();
     hipStream_t s = ctx().stream;
     long npairs = (ncols + 1) / 2;
@@ -871,12 +901,27 @@ void dht_sketch(long n, long ncols, const double *X, long ldx, const double *D, 
                                                              ldsa);
         SKL_CHECK_LAUNCH();
     } else {
-        T *H = ar.alloc_n<T>((size_t)n * ncols);
-        if (P.mode == 2)
-            dht_fourstep<T>(P, ncols, X, ldx, D, scale, H, n, ar);
-        else
-            dht_bluestein<T>(n, ncols, X, ldx, D, scale, H, n, ar);
-        hash_gather<T>(m, ncols, bptr, bsrc, bcoef, H, n, SA, ldsa);
+        // the transformed columns pass through H (n x cols) and the four-step
+        // pass's complex pairs; in no-allocation mode the columns go in
+        // chunks of an even count (the pairs stay as in one pass, so the
+        // result is the same) sized to a quarter of the scratch left
+        long chunk = ncols;
+        if (ctx().no_alloc) {
+            const size_t per_col = 2 * sizeof(T) * (size_t)n + 64;
+            const size_t room = (ar.capacity() - std::min(ar.used(), ar.capacity())) / 4;
+            chunk = std::max<long>(2, (long)(room / per_col) & ~1L);
+        }
+        for (long c0 = 0; c0 < ncols; c0 += chunk) {
+            const long nc = std::min(chunk, ncols - c0);
+            const size_t mark = ar.used();
+            T *H = ar.alloc_n<T>((size_t)n * nc);
+            if (P.mode == 2)
+                dht_fourstep<T>(P, nc, X + c0 * ldx, ldx, D, scale, H, n, ar);
+            else
+                dht_bluestein<T>(n, nc, X + c0 * ldx, ldx, D, scale, H, n, ar);
+            hash_gather<T>(m, nc, bptr, bsrc, bcoef, H, n, SA + c0 * ldsa, ldsa);
+            if (chunk < ncols) ar.rewind(mark);
+        }
     }
 }
 template void dht_sketch<double>(long, long, const double *, long, const double *, double, long,
@@ -890,7 +935,7 @@ template void dht_sketch<float>(long, long, const double *, long, const double *
 template <class T>
 void dht_cols_from_double(long n, long ncols, const double *X, long ldx, T *Y, long ldy,
                           Arena &ar) {
-    const FftPlanDev<T> &P = get_plan<T>(n);
+    const FftPlanDev<T> &P = get_plan<T>(n, ar);
     constexpr int NT = fft_nt<T>();
     long npairs = (ncols + 1) / 2;
     if (P.mode == 1) {

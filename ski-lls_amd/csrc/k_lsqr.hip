@@ -765,7 +765,7 @@ struct LsqrPlan {
 // the one-pass kernel holds up to 16 column pairs of a row per thread in
 // registers (d <= 8192); small problems keep the two-pass kernels
 static bool fused_wanted(const LsqrProblem &p) {
-    if (p.csr_ptr || p.d > 16 * FR_COLS || p.n < 4096) return false;
+    if (p.csr_ptr || p.d > 16 * FR_COLS || p.n < 4096 || ctx().no_alloc) return false;
     const char *e = getenv("SKL_LSQR_FUSED");
     return !(e && e[0] == '0');
 }
@@ -1139,8 +1139,9 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
     };
     HostFlag *hf = (HostFlag *)C.pinned;
     if (chunk <= 0) chunk = 16;
-    bool graph_ok = use_graph;
-    const bool prof = g_prof_on;
+    // no graphs and no timing events in no-allocation mode
+    bool graph_ok = use_graph && !C.no_alloc;
+    const bool prof = g_prof_on && !C.no_alloc;
     hipEvent_t *ev = nullptr;
     if (prof) {
         prof_events();

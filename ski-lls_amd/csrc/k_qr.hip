@@ -840,7 +840,7 @@ void qr_householder(long m, long ncol, long nfac, T *W, long ldw, T *tau, Arena 
     long kmax = std::min(m, nfac);
     const int nb = (nfac >= 256) ? 64 : 32;
     const long npan = (kmax + nb - 1) / nb;
-    const bool ahead = npan >= 3 && lookahead_on();
+    const bool ahead = npan >= 3 && lookahead_on() && !ctx().no_alloc;
     T *Vf[2], *Tm[2];
     Vf[0] = ar.alloc_n<T>((size_t)m * nb);
     Tm[0] = ar.alloc_n<T>((size_t)nb * nb);
@@ -1532,7 +1532,7 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
     T *cvec = ar.alloc_n<T>((size_t)2 * P_max * YQ_MAXROWS);
     const GridSync bar = grid_sync(ar.alloc_n<unsigned>(1));
     Context &c = ctx();
-    const bool ahead = lookahead_on();
+    const bool ahead = lookahead_on() && !ctx().no_alloc;
     if (ahead) c.make_side();
     bool side_busy = false;
     auto join_side = [&] {

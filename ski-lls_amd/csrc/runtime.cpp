@@ -31,7 +31,19 @@ double wall() {
 // ---------------------------------------------------------------------------
 Arena::~Arena() { release(); }
 
+void Arena::attach_fixed(void *base, size_t bytes) {
+    if (!fixed_) release();
+    fixed_ = true;
+    base_ = static_cast<char *>(base);
+    cap_ = bytes;
+    used_ = over_ = peak_ = 0;
+}
+
 void Arena::release() {
+    if (fixed_) {
+        used_ = 0;
+        return;
+    }
     if (base_ || !overflow_.empty()) {
         AllocLock lk(capture_mutex());
         if (base_) (void)hipFree(base_);
@@ -43,7 +55,7 @@ void Arena::release() {
 }
 
 void Arena::reserve(size_t bytes) {
-    if (bytes <= cap_) return;
+    if (bytes <= cap_ || fixed_) return;
     if (used_ != 0) return; // cannot move live buffers; alloc() falls back
     AllocLock lk(capture_mutex());
     if (base_) SKL_HIP(hipFree(base_));
@@ -58,7 +70,7 @@ void Arena::reserve(size_t bytes) {
 }
 
 void Arena::reset() {
-    if (!overflow_.empty()) {
+    if (!fixed_ && !overflow_.empty()) {
         // hipFree waits for the device; the previous solve's buffers are dead
         size_t want = std::max(peak_, cap_);
         release();
@@ -78,6 +90,10 @@ void *Arena::alloc(size_t bytes) {
         peak_ = std::max(peak_, used_ + over_);
         return p;
     }
+    if (fixed_)
+        throw Error(SKL_E_UNSUPPORTED,
+                    "problem needs more device scratch than the no-allocation entry point's " +
+                        std::to_string(cap_ >> 20) + " MiB");
     void *p = nullptr;
     {
         AllocLock lk(capture_mutex());
@@ -115,6 +131,7 @@ void ensure_init() {
     SKL_HIP(hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault));
     SKL_HIP(hipMalloc(&c->dev_err, 256));
     SKL_HIP(hipMemset(c->dev_err, 0, 256));
+    noalloc_setup(dev);
     t_ctx = c;
     g_live_ctx.fetch_add(1);
 }
@@ -170,8 +187,10 @@ BarrierLaunch::~BarrierLaunch() {
             g_tok_events.pop_back();
         }
     }
+    // (no event is created in no-allocation mode: without a pooled one the
+    // tokens come back after the stream drains)
     bool ok = ev != nullptr;
-    if (!ok) {
+    if (!ok && !ctx().no_alloc) {
         AllocLock alk(capture_mutex());
         ok = hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
     }
@@ -193,10 +212,18 @@ Context &ctx() {
     return *t_ctx;
 }
 
+bool has_context() { return t_ctx != nullptr; }
+
+std::shared_mutex &capture_mutex() {
+    static std::shared_mutex mu;
+    return mu;
+}
+
 void lsqr_release_graphs();
 
 void *Context::workspace(size_t bytes) {
     if (bytes > ws_cap) {
+        if (no_alloc) throw Error(SKL_E_UNSUPPORTED, "split-K scratch in no-allocation mode");
         AllocLock lk(capture_mutex());
         SKL_HIP(hipStreamSynchronize(stream));
         if (ws) SKL_HIP(hipFree(ws));
@@ -210,6 +237,7 @@ void *Context::workspace(size_t bytes) {
 
 void Context::make_side() {
     if (side) return;
+    if (no_alloc) throw Error(SKL_E_UNSUPPORTED, "second stream in no-allocation mode");
     AllocLock lk(capture_mutex());
     SKL_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     for (auto &e : ev) SKL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -217,6 +245,7 @@ void Context::make_side() {
 
 void Context::make_aux() {
     if (aux) return;
+    if (no_alloc) throw Error(SKL_E_UNSUPPORTED, "third stream in no-allocation mode");
     AllocLock lk(capture_mutex());
     SKL_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
     for (auto &e : aux_ev) SKL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -257,18 +286,13 @@ void release_thread_context() {
     if (t_ctx->pinned) (void)hipHostFree(t_ctx->pinned);
     if (t_ctx->dev_err) (void)hipFree(t_ctx->dev_err);
     (void)hipStreamDestroy(t_ctx->stream);
-    alk.unlock(); // the arena's destructor below owns nothing any more
+    alk.unlock(); // the arenas' destructors below own nothing any more
     delete t_ctx;
     t_ctx = nullptr;
     g_live_ctx.fetch_sub(1);
 }
 
 void set_device(int dev) { g_device = dev; }
-
-std::shared_mutex &capture_mutex() {
-    static std::shared_mutex mu;
-    return mu;
-}
 
 GridSync grid_sync(unsigned *ctr) {
     Context &c = ctx();

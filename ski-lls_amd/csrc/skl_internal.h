@@ -72,8 +72,14 @@ class Arena {
     size_t used() const { return used_; }
     size_t capacity() const { return cap_; }
     void release();
+    // Fixed mode: the arena carves from [base, base + bytes), memory it does
+    // not own; it never allocates, and a request beyond the end throws
+    // Error(SKL_E_UNSUPPORTED).
+    void attach_fixed(void *base, size_t bytes);
+    bool fixed() const { return fixed_; }
 
   private:
+    bool fixed_ = false;
     char *base_ = nullptr;
     size_t cap_ = 0, used_ = 0;
     size_t over_ = 0, peak_ = 0;   // bytes in extra blocks; high-water of used_ + over_
@@ -95,6 +101,12 @@ struct Context {
     // number of polls after which a barrier wait gives up
     unsigned *dev_err = nullptr;
     unsigned barrier_spins = 1u << 28;
+    // no-allocation mode (skl_dense_solve_dev_noalloc): solver scratch comes
+    // from `fixed`, attached to the device's static scratch array; no device
+    // memory, stream, event or graph is created while it is set
+    bool no_alloc = false;
+    Arena fixed;
+    Arena &cur_arena() { return no_alloc ? fixed : arena; }
     int num_cu = 0;
     int coop_ok = 0;
     size_t lds_max = 0;
@@ -135,6 +147,12 @@ GridSync grid_sync(unsigned *ctr);
 // Waits for ctx().stream and throws Error(SKL_E_HIP) when a grid barrier
 // launched from this context gave up since the last check (clearing the word).
 void check_grid_fault();
+// Resolves the device address of the static scratch array of the
+// no-allocation entry point for the current device (once per device; called
+// from the context set-up with that device current).
+void noalloc_setup(int device);
+// true when the calling thread holds a context (skl_init or an earlier call)
+bool has_context();
 // sets the message skl_last_error() returns on the calling thread
 void set_last_error(const std::string &msg);
 void ensure_init();

@@ -539,7 +539,7 @@ static int refine_sketched_solution(long m, long d, const double *SA, long ldsa,
 static int dense_impl(long n, long d, const double *dA, long lda, const double *db, double *dx,
                       const skl_opts &o, skl_info &info, bool cpqr, bool subsample = false) {
     Context &C = ctx();
-    Arena &ar = C.arena;
+    Arena &ar = C.cur_arena();
     hipStream_t s = C.stream;
     std::memset(&info, 0, sizeof info);
     if (n < d) throw Error(SKL_E_SHAPE, "Matrix is not overdetermined.");
@@ -1136,6 +1136,45 @@ static int qr_component(long m, long n, long nfac, double *A, long lda, double *
     return SKL_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation entry point (skl_dense_solve_dev_noalloc): a
+// static device array, one copy per GPU the library runs on, held until the
+// process ends.  Host code never uses the array's name as an address: its
+// device address comes from hipGetSymbolAddress at context set-up, once per
+// device, and that pointer is what the solver carves its buffers from.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+// the calls of this process share each device's array: one at a time
+static std::mutex g_noalloc_call_mu;
+
+void noalloc_setup(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES || g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    SKL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = p;
+}
+
+static void *noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < NOALLOC_MAX_DEVICES) ? g_scratch_ptr[device] : nullptr;
+}
+
+// Device scratch of the robust dense FP64 solve without the row-major copy
+// of A (two-pass LSQR), a bound used to refuse a problem before any work
+// is queued: the sketch [SA | c], M and R, the pattern, the CPQR's panel and
+// update buffers, the LSQR vectors and partial sums, and the transform's
+// column chunks.
+static size_t noalloc_footprint(long n, long d, long m, long k) {
+    const double b = 8.0 * ((double)m * (d + 1) + 4.0 * d * d + 6.0 * n + 2.0 * n * k + 80.0 * m) +
+                     16.0 * d * ((double)n / 1024.0 + 64.0) + 32.0 * n + (4 << 20);
+    return (size_t)std::min(b, 1e18);
+}
+
 } // namespace skl
 
 extern "C" {
@@ -1209,6 +1248,41 @@ int skl_dense_solve_dev(long n, long d, const double *dA, long lda, const double
     skl_info tmp;
     return guarded([&] {
         skl_opts o = resolve(opts);
+        return dense_impl(n, d, dA, lda, db, dx, o, info ? *info : tmp, true);
+    });
+}
+
+int skl_dense_solve_dev_noalloc(long n, long d, const double *dA, long lda, const double *db,
+                                double *dx, const skl_opts *opts, skl_info *info) {
+    using namespace skl;
+    skl_info tmp;
+    return guarded([&] {
+        if (!has_context())
+            throw Error(SKL_E_ARG, "skl_dense_solve_dev_noalloc: call skl_init on this thread first");
+        Context &C = ctx();
+        skl_opts o = resolve(opts);
+        SKL_REQUIRE(o.precision == SKL_FP64, SKL_E_UNSUPPORTED,
+                    "skl_dense_solve_dev_noalloc runs the FP64 solver only");
+        if (n < d) throw Error(SKL_E_SHAPE, "Matrix is not overdetermined.");
+        SKL_REQUIRE(d > 0 && n > 0 && lda >= n && o.gamma > 0 && o.k >= 1, SKL_E_ARG,
+                    "invalid dense shape or sketch parameters");
+        const long m = (long)std::ceil((double)d * o.gamma);
+        const size_t need = noalloc_footprint(n, d, m, o.k);
+        if (need > NOALLOC_SCRATCH_BYTES)
+            throw Error(SKL_E_UNSUPPORTED,
+                        "skl_dense_solve_dev_noalloc: " + std::to_string(n) + " x " +
+                            std::to_string(d) + " needs about " + std::to_string(need >> 20) +
+                            " MiB of device scratch, above its " +
+                            std::to_string(NOALLOC_SCRATCH_BYTES >> 20) + " MiB");
+        void *p = noalloc_scratch(C.device);
+        if (!p) throw Error(SKL_E_HIP, "no-allocation scratch not set up on this device");
+        std::lock_guard<std::mutex> lk(g_noalloc_call_mu);
+        struct Mode {
+            Context &c;
+            ~Mode() { c.no_alloc = false; }
+        } mode{C};
+        C.fixed.attach_fixed(p, NOALLOC_SCRATCH_BYTES);
+        C.no_alloc = true;
         return dense_impl(n, d, dA, lda, db, dx, o, info ? *info : tmp, true);
     });
 }

@@ -69,6 +69,7 @@ EXPORTS = [
     "skl_set_history", "skl_hash_pattern", "skl_dht", "skl_hash_apply", "skl_dense_sketch",
     "skl_sparse_sketch", "skl_qr", "skl_cpqr", "skl_trinv", "skl_lsqr_dense",
     "skl_profile_enable", "skl_profile_reset", "skl_profile_read", "skl_set_barrier_spins",
+    "skl_dense_solve_dev_noalloc",
     "skl_server_create", "skl_server_submit_dense", "skl_server_submit_sparse",
     "skl_server_wait", "skl_server_pending", "skl_server_destroy",
 ]
@@ -204,6 +205,19 @@ def dense_solve_dev(n, d, A_ptr, lda, b_ptr, x_ptr, cpqr=True, sampling="hashing
         fn = lib().skl_dense_solve_dev if cpqr else lib().skl_dense_solve_nocpqr_dev
     rc = fn(C.c_long(n), C.c_long(d), C.c_void_p(A_ptr), C.c_long(lda), C.c_void_p(b_ptr),
             C.c_void_p(x_ptr), C.byref(o), C.byref(info))
+    _check(rc)
+    return info
+
+
+def dense_solve_dev_noalloc(n, d, A_ptr, lda, b_ptr, x_ptr, **opts):
+    """skl_dense_solve_dev_noalloc: the robust dense FP64 solver on device
+    pointers with no device allocation (scratch from the library's static
+    array; skl_init must have run on this thread)."""
+    o = default_opts(**opts)
+    info = Info()
+    rc = lib().skl_dense_solve_dev_noalloc(C.c_long(n), C.c_long(d), C.c_void_p(A_ptr),
+                                           C.c_long(lda), C.c_void_p(b_ptr), C.c_void_p(x_ptr),
+                                           C.byref(o), C.byref(info))
     _check(rc)
     return info
 
