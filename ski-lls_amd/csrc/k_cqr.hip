@@ -1,0 +1,443 @@
+// k_cqr.hip -- Householder panel factorization by CholeskyQR2 and Householder
+// reconstruction, with the HQRRP panel pivoting rule.
+//
+// The panel kernels of k_qr.hip factor a tall panel A_p (mp x b, b <= 64) one
+// column at a time, one grid-wide exchange per column.  Here the panel's
+// Householder factors come from b-column-wide steps instead:
+//
+//   1. G = A_p^T A_p (GEMM), then a pivoted Cholesky of G in one workgroup:
+//      pivots by the largest partial norm with HQRRP's downdating
+//      (hqrrp/lapack_compatible_sources/NoFLA_HQRRP_WY_blk_var4_skint.c:
+//      920-990: vn1 *= sqrt((1 + t)(1 - t)), t = |r_kj| / vn1, recomputed
+//      when the downdate loses accuracy), so the column order is the one the
+//      column-by-column panel QRP picks, in exact arithmetic;
+//   2. Q1 = A_p P R1^-1 (the triangle inverted in LDS, then a 64-column
+//      product), then one more CholeskyQR pass, Q = Q1 R2^-1, R = R2 R1:
+//      orthonormal to working precision while cond(A_p) stays far below
+//      1/sqrt(u);
+//   3. Householder reconstruction (Ballard, Demmel, Grigori, Jacquelin,
+//      Knight, Nguyen, "Reconstructing Householder vectors from tall-skinny
+//      QR", JPDC 85, 2015): the LU factorization without pivoting of
+//      Q - [S; 0], with S_kk = -sign of the k-th pivot, gives the unit lower
+//      trapezoidal V = L, U and tau_k = -U_kk S_kk, and the R of the
+//      Householder QR is S R -- the factors the panel's dgeqrf / QRP would
+//      store, in exact arithmetic.
+//
+// A panel whose Cholesky meets a partial norm below 1e-4 of the column's own
+// norm (near dependence: the Gram matrix no longer resolves it) or a pivot
+// that is not positive sets a device flag; the fast path's later kernels then
+// do nothing and the column-by-column kernel, launched after them with the
+// flag as its condition, factors the untouched panel instead.
+#include "k_common.h"
+
+namespace skl {
+
+constexpr int CQ_T = 256;
+constexpr int CQ_LD = 65;
+
+// 1/sqrt(d) and 1/x from the hardware estimates with two Newton steps each
+// (quadratic convergence from the estimate's 2^-22 or better): a chain of a
+// few fused multiply-adds instead of the IEEE square-root and division
+// sequences on every step's critical path.  Within an ulp or two of the
+// correctly rounded values.
+__device__ inline double rsqrt_nr(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+    y = y * fma(-0.5 * d * y, y, 1.5);
+    y = y * fma(-0.5 * d * y, y, 1.5);
+    return y;
+}
+__device__ inline double rcp_nr(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    y = y * fma(-x, y, 2.0);
+    y = y * fma(-x, y, 2.0);
+    return y;
+}
+
+// X = R^-1 for the upper triangular 64 x 64 R in LDS (ld CQ_LD; rows and
+// columns past the panel width hold the identity), by recursive doubling:
+// the 8 x 8 diagonal blocks by back substitution, one thread per column,
+// then three merge levels with X(A, C) = -X_A R(A, C) X_C.  Y is LDS
+// scratch of the same shape.  All CQ_T threads call it; it ends with a
+// barrier.
+__device__ void tri_inv_upper_lds(const double *Rs, double *X, double *Y) {
+    const int tid = threadIdx.x;
+    for (int e = tid; e < 64 * 64; e += CQ_T) X[(e & 63) + (e >> 6) * CQ_LD] = 0.0;
+    __syncthreads();
+    if (tid < 64) {
+        const int o = (tid >> 3) * 8, c = o + (tid & 7);
+        X[c + c * CQ_LD] = 1.0 / Rs[c + c * CQ_LD];
+        for (int i = c - 1; i >= o; --i) {
+            double acc = 0.0;
+            for (int l = i + 1; l <= c; ++l) acc += Rs[i + l * CQ_LD] * X[l + c * CQ_LD];
+            X[i + c * CQ_LD] = -acc / Rs[i + i * CQ_LD];
+        }
+    }
+    __syncthreads();
+    for (int lg = 3; lg < 6; ++lg) {
+        const int sz = 1 << lg, npair = 64 >> (lg + 1), per = sz * sz;
+        // Y(A, C) = R(A, C) X_C
+        for (int e = tid; e < npair * per; e += CQ_T) {
+            const int pr = e >> (2 * lg), w = e & (per - 1);
+            const int i = w & (sz - 1), j = w >> lg;
+            const int a0 = pr * 2 * sz, c0 = a0 + sz;
+            double acc = 0.0;
+            for (int k = 0; k <= j; ++k) acc += Rs[(a0 + i) + (c0 + k) * CQ_LD] * X[(c0 + k) + (c0 + j) * CQ_LD];
+            Y[(a0 + i) + (c0 + j) * CQ_LD] = acc;
+        }
+        __syncthreads();
+        // X(A, C) = -X_A Y(A, C)
+        for (int e = tid; e < npair * per; e += CQ_T) {
+            const int pr = e >> (2 * lg), w = e & (per - 1);
+            const int i = w & (sz - 1), j = w >> lg;
+            const int a0 = pr * 2 * sz, c0 = a0 + sz;
+            double acc = 0.0;
+            for (int k = i; k < sz; ++k) acc += X[(a0 + i) + (a0 + k) * CQ_LD] * Y[(a0 + k) + (c0 + j) * CQ_LD];
+            X[(a0 + i) + (c0 + j) * CQ_LD] = -acc;
+        }
+        __syncthreads();
+    }
+}
+
+// Pivoted (PIV) or plain Cholesky of the b x b Gram matrix G (ld ldg) into
+// R (upper, ld 64) and Z = P R^-1 (b x b, ld b; A_p Z = A_p P R^-1).  pass 1:
+// perm (final position -> panel column) and the sequential swaps; flag = 0 /
+// 1 (failed).  pass 2 (plain): runs only when flag is 0 and sets it on
+// failure.
+// Thread t keeps entries (p, g + 4 m), m < 16, of the Gram matrix in
+// registers, p = t & 63 its row and g = t >> 6; rows and columns stay where
+// they are (p is the panel column; wave 0 keeps each column's logical
+// position, partial norms and Schur diagonal).  A step: wave 0 picks the
+// pivot, the pivot row's owners publish it in LDS (double buffered), every
+// thread updates its entries -- two barriers.
+template <bool PIV>
+__global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double *G, int ldg,
+                                                  double *R, double *Z, int32_t *perm,
+                                                  int32_t *swp, int *flag, int hq_norms,
+                                                  int pass) {
+    __shared__ double Rb[64 * CQ_LD]; // row k of R by panel column; later X = R^-1
+    __shared__ double Rr[64 * CQ_LD]; // R in logical order
+    __shared__ double Ys[64 * CQ_LD];
+    __shared__ double rowbuf[2][64];
+    __shared__ double s_g0[64];
+    __shared__ int s_pk[2], s_perm[64];
+    const int tid = threadIdx.x, p = tid & 63, g = tid >> 6;
+    if (pass == 2 && *flag != 0) return;
+    double sv[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int q = g + 4 * m;
+        sv[m] = (p < b && q < b) ? G[p + (long)q * ldg] : 0.0;
+    }
+    for (int e = tid; e < 64 * CQ_LD; e += CQ_T) Rb[e] = 0.0;
+    // wave 0: per column p its Gram diagonal, partial norms and position
+    double dg = 0.0, vn1 = 0.0, vn2 = 0.0;
+    int pos = p < b ? p : 0x7fffffff;
+    if (g == 0) {
+        dg = p < b ? G[p + (long)p * ldg] : 0.0;
+        vn1 = sqrt(fmax(dg, 0.0));
+        vn2 = vn1;
+        s_g0[p] = dg;
+    }
+    unsigned long long act = b >= 64 ? ~0ull : ((1ull << b) - 1ull);
+    const double tol3z = sqrt(1.1102230246251565e-16);
+    __syncthreads();
+    int k = 0;
+    for (; k < b; ++k) {
+        const int buf = k & 1;
+        if (g == 0) {
+            int pk = k;
+            if (PIV) {
+                // largest partial norm among the columns left, first logical
+                // position on ties
+                double bv = -1.0;
+                int bpos = 0x7fffffff, bl = -1;
+                if (((act >> p) & 1ull) && vn1 == vn1) {
+                    bv = fabs(vn1);
+                    bpos = pos;
+                    bl = p;
+                }
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    const double ov = __shfl_xor(bv, o, 64);
+                    const int opos = __shfl_xor(bpos, o, 64), ol = __shfl_xor(bl, o, 64);
+                    if (ov > bv || (ov == bv && opos < bpos)) {
+                        bv = ov;
+                        bpos = opos;
+                        bl = ol;
+                    }
+                }
+                if (bl < 0) bl = __builtin_ctzll(__ballot(pos == k)); // no finite candidate
+                pk = bl;
+                const int lp = __shfl(pos, pk, 64); // logical position of the pivot
+                const bool atk = pos == k;
+                if (p == pk)
+                    pos = k;
+                else if (atk)
+                    pos = lp;
+                if (p == 0 && swp) swp[k] = lp;
+            }
+            if (p == 0) s_pk[buf] = pk;
+        }
+        __syncthreads();
+        const int pk = s_pk[buf];
+        if (p == pk) {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) rowbuf[buf][g + 4 * m] = sv[m];
+        }
+        __syncthreads();
+        const double d = rowbuf[buf][pk];
+        // not positive, or the column's partial norm below 1e-4 of its norm
+        if (!(d > 0.0) || !(d >= 1e-8 * s_g0[pk])) break; // the same in every thread
+        const double inv = rsqrt_nr(d), rkk = d * inv;
+        act &= ~(1ull << pk);
+        const bool mine = (act >> p) & 1ull;
+        const double rp = rowbuf[buf][p] * inv; // R(k, column p)
+        {
+            // branch-free, so that the sixteen LDS reads are issued together
+            double rq[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) rq[m] = rowbuf[buf][g + 4 * m];
+            const double rpm = mine ? rp : 0.0;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int q = g + 4 * m;
+                const double f = ((act >> q) & 1ull) ? rq[m] * inv : 0.0;
+                sv[m] -= rpm * f;
+            }
+        }
+        if (g == 0) {
+            Rb[k + p * CQ_LD] = p == pk ? rkk : (mine ? rp : 0.0);
+            if (mine) {
+                dg -= rp * rp;
+                if (PIV && vn1 != 0.0) {
+                    // partial norm downdating with the new row k of R
+                    double temp = fabs(rp) / vn1;
+                    if (hq_norms)
+                        temp = (1.0 + temp) * (1.0 - temp);
+                    else
+                        temp = 1.0 - temp * temp;
+                    if (temp < 0.0) temp = 0.0;
+                    const double q5 = vn1 / vn2;
+                    const double temp2 = temp * q5 * q5;
+                    if (temp2 <= tol3z) {
+                        vn1 = (mp - k - 1 > 0) ? sqrt(fmax(dg, 0.0)) : 0.0;
+                        vn2 = vn1;
+                    } else {
+                        vn1 = vn1 * sqrt(temp);
+                    }
+                }
+            }
+        }
+    }
+    const int fail = k < b ? 1 : 0;
+    if (tid == 0) {
+        if (pass == 1)
+            *flag = fail;
+        else if (fail)
+            *flag = 1;
+    }
+    if (fail) return;
+    if (g == 0 && p < b) s_perm[pos] = p; // logical position -> panel column
+    __syncthreads();
+    for (int e = tid; e < 64 * 64; e += CQ_T) {
+        const int i = e & 63, j = e >> 6;
+        const double v = (i <= j && j < b) ? Rb[i + s_perm[j] * CQ_LD] : (i == j ? 1.0 : 0.0);
+        Rr[i + j * CQ_LD] = v;
+        R[i + j * 64] = (j < b) ? ((i <= j) ? v : 0.0) : 0.0;
+    }
+    if (PIV && perm && tid < b) perm[tid] = s_perm[tid];
+    __syncthreads();
+    // Z = P R^-1: row perm[t] of Z is row t of R^-1
+    tri_inv_upper_lds(Rr, Rb, Ys);
+    for (int e = tid; e < b * b; e += CQ_T) {
+        const int t = e % b, j = e / b;
+        Z[s_perm[t] + (long)j * b] = Rb[t + j * CQ_LD];
+    }
+}
+
+// Out(i, :) = In(i, :) X for rows i < rows (X b x b, ld b): 64-row tiles of
+// In in LDS, thread t computes row t & 63 for every fourth column from
+// t >> 6.  Runs only when *flag is 0.
+__global__ __launch_bounds__(CQ_T) void k_cqr_rmul(long rows, int b, const double *In, long ldi,
+                                                   const double *X, double *Out, long ldo,
+                                                   const int *flag) {
+    __shared__ double sI[64 * CQ_LD];
+    __shared__ double sX[64 * 64];
+    if (*flag != 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, grp = tid >> 6;
+    const long r0 = (long)blockIdx.x * 64;
+    for (int e = tid; e < 64 * 64; e += CQ_T) {
+        const int i = e & 63, j = e >> 6;
+        sX[e] = (i < b && j < b) ? X[i + (long)j * b] : 0.0;
+        sI[i + j * CQ_LD] = (j < b && r0 + i < rows) ? In[(r0 + i) + (long)j * ldi] : 0.0;
+    }
+    __syncthreads();
+    const long row = r0 + lane;
+    // the row in registers (zero past b), then 16 columns per thread
+    double xi[64];
+#pragma unroll
+    for (int k = 0; k < 64; ++k) xi[k] = sI[lane + k * CQ_LD];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int j = grp + 4 * m;
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 64; ++k) acc += xi[k] * sX[k + j * 64];
+        if (row < rows && j < b) Out[row + (long)j * ldo] = acc;
+    }
+}
+
+// Householder reconstruction of the panel from Q (mp x b, ld ldq) and
+// R = R2 R1: the LU of the top block of Q - [S; 0] (S_kk = -sign of the k-th
+// pivot) gives L1 (below the diagonal of the panel's top block), U^-1 (the
+// rows below are V2 = Q2 U^-1) and tau_k = -U_kk S_kk; the top block's upper
+// triangle receives S R.  The panel's column order goes into jpvt (entries
+// permuted) and swaps.  Runs only when *flag is 0.  Thread t owns row t & 63
+// and every fourth column from t >> 6 on: one barrier per step.
+__global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q, long ldq,
+                                                 const double *R1, const double *R2,
+                                                 const int32_t *perm, const int32_t *swp,
+                                                 double *Wp, long ldw, double *tau, double *Uinv,
+                                                 int32_t *jpvt, int32_t *swaps, const int *flag) {
+    // thread t keeps entries (i, g + 4 m) of the top block in registers,
+    // i = t & 63, g = t >> 6; step k publishes row k and column k (double
+    // buffered): one barrier per step
+    __shared__ double Ub[64 * CQ_LD];
+    __shared__ double Lb[64 * CQ_LD];
+    __shared__ double Ys[64 * CQ_LD];
+    __shared__ double rowbuf[2][64], colbuf[2][64];
+    __shared__ double sg[64];
+    __shared__ int jp[64];
+    if (*flag != 0) return;
+    const int tid = threadIdx.x, i = tid & 63, g = tid >> 6;
+    double a[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int j = g + 4 * m;
+        a[m] = (i < b && j < b) ? Q[i + (long)j * ldq] : 0.0;
+    }
+    for (int e = tid; e < 64 * CQ_LD; e += CQ_T) {
+        Ub[e] = 0.0;
+        Lb[e] = 0.0;
+    }
+    if (i == 0) {
+#pragma unroll
+        for (int m = 0; m < 16; ++m) rowbuf[0][g + 4 * m] = a[m];
+    }
+    if (g == 0) colbuf[0][i] = a[0];
+    __syncthreads();
+    for (int k = 0; k < b; ++k) {
+        const int buf = k & 1, nb = buf ^ 1;
+        const double akk = rowbuf[buf][k];
+        const double sk = akk >= 0.0 ? -1.0 : 1.0;
+        const double piv = akk - sk;
+        const double l = colbuf[buf][i] * rcp_nr(piv);
+        {
+            // branch-free, so that the sixteen LDS reads are issued together
+            double rj[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) rj[m] = rowbuf[buf][g + 4 * m];
+            const double li = (i > k && i < b) ? l : 0.0;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int j = g + 4 * m;
+                a[m] -= li * ((j > k && j < b) ? rj[m] : 0.0);
+            }
+        }
+        if (g == 0) {
+            if (i > k && i < b) Lb[i + k * CQ_LD] = l;
+            if (i >= k && i < b) Ub[k + i * CQ_LD] = i == k ? piv : rowbuf[buf][i];
+            if (i == 0) sg[k] = sk;
+        }
+        if (k + 1 < b) {
+            if (i == k + 1) {
+#pragma unroll
+                for (int m = 0; m < 16; ++m) rowbuf[nb][g + 4 * m] = a[m];
+            }
+            if (g == ((k + 1) & 3)) {
+                double c = 0.0;
+#pragma unroll
+                for (int m = 0; m < 16; ++m)
+                    if (m == ((k + 1) >> 2)) c = a[m];
+                colbuf[nb][i] = c;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < b) tau[tid] = -Ub[tid + tid * CQ_LD] * sg[tid];
+    // top block: S R2 R1 on and above the diagonal, L1 below; R2 and R1 go
+    // through LDS (Ys, and rowbuf's space is too small: R1 into a tile of Ys
+    // rows 0..63, R2 read once per row into registers)
+    for (int e = tid; e < 64 * 64; e += CQ_T) Ys[(e & 63) + (e >> 6) * CQ_LD] = R1[e];
+    __syncthreads();
+    {
+        // thread (r = tid & 63, g) forms row r for columns g + 4 m
+        const int r = tid & 63;
+        double r2[64];
+#pragma unroll
+        for (int q = 0; q < 64; ++q) r2[q] = R2[r + q * 64];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int c = g + 4 * m;
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < 64; ++q) s += r2[q] * Ys[q + c * CQ_LD]; // R2, R1 upper: zeros elsewhere
+            const double v = r <= c ? sg[r] * s : Lb[r + c * CQ_LD];
+            if (r < b && c < b) Wp[r + (long)c * ldw] = v;
+        }
+    }
+    if (jpvt) {
+        if (tid < b) jp[tid] = jpvt[perm[tid]];
+        __syncthreads();
+        if (tid < b) jpvt[tid] = jp[tid];
+    }
+    if (swaps && tid < b) swaps[tid] = swp[tid];
+    // U^-1 (b x b, ld b) for the rows below the top block: V2 = Q2 U^-1
+    for (int e = tid; e < 64 * 64; e += CQ_T) {
+        const int r = e & 63, c = e >> 6;
+        if (c >= b) Ub[r + c * CQ_LD] = (r == c) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    tri_inv_upper_lds(Ub, Lb, Ys);
+    for (int e = tid; e < b * b; e += CQ_T) Uinv[e] = Lb[(e % b) + (e / b) * CQ_LD];
+}
+
+// Queues the fast path for the panel Wp (mp x b, ld ldw): pivot != 0 uses the
+// HQRRP panel pivoting (hq_norms) and fills jpvt (may be null) and swaps.
+// Returns the device flag that is nonzero at run time when the fast path gave
+// up and left the panel untouched.
+int *cqr_panel(long mp, int b, int pivot, double *Wp, long ldw, double *tau, int32_t *jpvt,
+               int32_t *swaps, int hq_norms, Arena &ar) {
+    hipStream_t s = ctx().stream;
+    double *G = ar.alloc_n<double>(64 * 64);
+    double *R1 = ar.alloc_n<double>(64 * 64);
+    double *R2 = ar.alloc_n<double>(64 * 64);
+    double *Z1 = ar.alloc_n<double>(64 * 64);
+    double *Z2 = ar.alloc_n<double>(64 * 64);
+    double *Ui = ar.alloc_n<double>(64 * 64);
+    double *Q1 = ar.alloc_n<double>((size_t)mp * b);
+    double *Q2 = ar.alloc_n<double>((size_t)mp * b);
+    int32_t *perm = ar.alloc_n<int32_t>(64);
+    int32_t *swp = ar.alloc_n<int32_t>(64);
+    int *flag = ar.alloc_n<int>(1);
+    const unsigned gr = (unsigned)((mp + 63) / 64);
+    gemm<double>('T', 'N', b, b, mp, 1.0, Wp, ldw, Wp, ldw, 0.0, G, b);
+    if (pivot)
+        k_cqr_chol<true><<<1, CQ_T, 0, s>>>(b, mp, G, b, R1, Z1, perm, swp, flag, hq_norms, 1);
+    else
+        k_cqr_chol<false><<<1, CQ_T, 0, s>>>(b, mp, G, b, R1, Z1, nullptr, nullptr, flag, 0, 1);
+    k_cqr_rmul<<<gr, CQ_T, 0, s>>>(mp, b, Wp, ldw, Z1, Q1, mp, flag);
+    gemm<double>('T', 'N', b, b, mp, 1.0, Q1, mp, Q1, mp, 0.0, G, b);
+    k_cqr_chol<false><<<1, CQ_T, 0, s>>>(b, mp, G, b, R2, Z2, nullptr, nullptr, flag, 0, 2);
+    k_cqr_rmul<<<gr, CQ_T, 0, s>>>(mp, b, Q1, mp, Z2, Q2, mp, flag);
+    k_cqr_hr<<<1, CQ_T, 0, s>>>(b, Q2, mp, R1, R2, perm, swp, Wp, ldw, tau, Ui,
+                                 pivot ? jpvt : nullptr, pivot ? swaps : nullptr, flag);
+    if (mp > b) {
+        const unsigned g2 = (unsigned)((mp - b + 63) / 64);
+        k_cqr_rmul<<<g2, CQ_T, 0, s>>>(mp - b, b, Q2 + b, mp, Ui, Wp + b, ldw, flag);
+    }
+    SKL_CHECK_LAUNCH();
+    return flag;
+}
+
+} // namespace skl

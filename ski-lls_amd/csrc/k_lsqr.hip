@@ -16,9 +16,15 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <tuple>
+#include <vector>
+#include <type_traits>
+#include <utility>
 
 #include <hip/hip_ext.h>
+
+#include <cstdio>
 
 #include "k_common.h"
 
@@ -844,12 +850,53 @@ template <class T> struct Ident {
     typedef T type;
 };
 
-// Launches k on s.  With timing events (e0 non-null) they are recorded by the
-// dispatch itself (hipExtLaunchKernelGGL), so their interval is the kernel's
-// own execution and not the stream's event commands around it.
+// Graph under construction on this thread: while `on`, launch_ev appends a
+// kernel node (after the previous one) instead of launching.  The LSQR chunk
+// graphs are built this way rather than by stream capture, which another
+// thread's runtime calls can invalidate mid-recording (and the stream stays
+// in the invalidated capture state afterwards).
+struct GraphRec {
+    bool on = false;
+    hipGraph_t graph = nullptr;
+    hipGraphNode_t last = nullptr;
+    hipError_t err = hipSuccess;
+    // argument values of the nodes, kept until the graph is instantiated
+    std::vector<std::shared_ptr<void>> keep;
+};
+static thread_local GraphRec t_rec;
+
+template <class Tup, size_t... I>
+static void arg_ptrs(Tup &t, void **ptr, std::index_sequence<I...>) {
+    ((ptr[I] = (void *)&std::get<I>(t)), ...);
+}
+
+// Launches k on s (or appends it to the graph being built).  With timing
+// events (e0 non-null) they are recorded by the dispatch itself
+// (hipExtLaunchKernelGGL), so their interval is the kernel's own execution
+// and not the stream's event commands around it.
 template <class... P>
 static void launch_ev(void (*k)(P...), dim3 g, dim3 b, uint32_t sh, hipStream_t s, hipEvent_t e0,
                       hipEvent_t e1, typename Ident<P>::type... args) {
+    if (t_rec.on) {
+        if (t_rec.err != hipSuccess) return;
+        using Tup = std::tuple<typename std::decay<P>::type...>;
+        auto vals = std::make_shared<Tup>(args...);
+        t_rec.keep.push_back(vals);
+        void *ptr[sizeof...(P) > 0 ? sizeof...(P) : 1];
+        arg_ptrs(*vals, ptr, std::index_sequence_for<P...>{});
+        hipKernelNodeParams np{};
+        np.func = (void *)k;
+        np.gridDim = g;
+        np.blockDim = b;
+        np.sharedMemBytes = sh;
+        np.kernelParams = ptr;
+        np.extra = nullptr;
+        hipGraphNode_t node = nullptr;
+        t_rec.err = hipGraphAddKernelNode(&node, t_rec.graph, t_rec.last ? &t_rec.last : nullptr,
+                                          t_rec.last ? 1 : 0, &np);
+        if (t_rec.err == hipSuccess) t_rec.last = node;
+        return;
+    }
     if (e0)
         hipExtLaunchKernelGGL(k, g, b, sh, s, e0, e1, 0, args...);
     else
@@ -867,8 +914,8 @@ static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs 
     if (after_fused) {
         // t = Pt(E) / beta over the workgroup partials
         nsb = q.G;
-        k_reduce_t_wide<<<(unsigned)((q.r + RW_OUT - 1) / RW_OUT), 256, 0, s>>>(
-            q.d, q.r, q.G, B.Pt, p.E, B.Sb, nsb, B.t, B.st);
+        launch_ev(k_reduce_t_wide, dim3((unsigned)((q.r + RW_OUT - 1) / RW_OUT)), dim3(256), 0, s,
+                  nullptr, nullptr, q.d, q.r, q.G, B.Pt, p.E, B.Sb, nsb, B.t, B.st);
     } else {
         // Pt = A^T ut
         if (!q.sparse) {
@@ -880,8 +927,8 @@ static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs 
                       p.csc_ptr, p.csc_row, p.csc_val, B.ut, B.Pt, B.st);
         }
         // t = Pt(E) / beta
-        k_reduce_t<<<(unsigned)((q.r + RED_OUT - 1) / RED_OUT), 256, 0, s>>>(
-            q.d, q.r, nrb, B.Pt, p.E, B.Sb, nsb, B.t, B.st);
+        launch_ev(k_reduce_t, dim3((unsigned)((q.r + RED_OUT - 1) / RED_OUT)), dim3(256), 0, s,
+                  nullptr, nullptr, q.d, q.r, nrb, B.Pt, p.E, B.Sb, nsb, B.t, B.st);
     }
     const unsigned nv = (unsigned)((q.r + UV_T - 1) / UV_T);
     if (init) {
@@ -896,8 +943,9 @@ static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs 
         const unsigned g = (unsigned)((trmv_t_items(q.r) + 3) / 4);
         launch_ev(k_trmv_t_items, dim3(g), dim3(256), 0, s, e6, e7, q.r, p.M, p.ldm, B.t, B.Pm,
                   q.nrb_m, B.st);
-        k_update_v<<<nv, UV_T, 0, s>>>(q.r, q.nrb_m, B.Pm, B.Sb, nsb, B.v, B.Sv, B.st);
-        k_update_wx<<<nv, UV_T, 0, s>>>(q.r, (int)nv, B.Sv, B.Sb, nsb, B.v, B.w, B.x, B.st, B.cnt,
+        launch_ev(k_update_v, dim3(nv), dim3(UV_T), 0, s, nullptr, nullptr, q.r, q.nrb_m, B.Pm,
+                  B.Sb, nsb, B.v, B.Sv, B.st);
+        launch_ev(k_update_wx, dim3(nv), dim3(UV_T), 0, s, nullptr, nullptr, q.r, (int)nv, B.Sv, B.Sb, nsb, B.v, B.w, B.x, B.st, B.cnt,
                                         B.hist, B.hist_cap);
     }
 }
@@ -967,7 +1015,7 @@ static void launch_iteration(const LsqrProblem &p, const LsqrPlan &q, const Lsqr
         dim3 g((unsigned)((q.r + GN_ROWS - 1) / GN_ROWS), (unsigned)q.nck_m);
         launch_ev(k_gemv_n_part<1>, g, dim3(GN_THREADS), 0, s, evi(ev, 4), evi(ev, 5), q.r, q.r,
                   p.M, p.ldm, B.v, q.cw_m, B.Pz, 1, B.st);
-        k_z_reduce<<<(unsigned)((q.d + RED_OUT - 1) / RED_OUT), 256, 0, s>>>(q.d, q.r, q.nck_m, B.Pz, B.Einv,
+        launch_ev(k_z_reduce, dim3((unsigned)((q.d + RED_OUT - 1) / RED_OUT)), dim3(256), 0, s, nullptr, nullptr, q.d, q.r, q.nck_m, B.Pz, B.Einv,
                                                                  B.z, B.st);
     }
     // ut = A z - coef ut ; Sb
@@ -980,7 +1028,8 @@ static void launch_iteration(const LsqrProblem &p, const LsqrPlan &q, const Lsqr
         dim3 g((unsigned)((q.n + GN_ROWS - 1) / GN_ROWS), (unsigned)q.nck_a);
         launch_ev(k_gemv_n_part<0>, g, dim3(GN_THREADS), 0, s, evi(ev, 0), evi(ev, 1), q.n, q.d,
                   p.A, p.lda, B.z, q.cw_a, B.Pn, 0, B.st);
-        k_reduce_n<<<(unsigned)q.nsb, 256, 0, s>>>(q.n, q.nck_a, B.Pn, B.ut, B.Sb, B.st, 0);
+        launch_ev(k_reduce_n, dim3((unsigned)q.nsb), dim3(256), 0, s, nullptr, nullptr, q.n,
+                  q.nck_a, B.Pn, B.ut, B.Sb, B.st, 0);
     } else {
         if (q.spg > 0)
             launch_ev(k_spmv_n_lds<8>, dim3((unsigned)q.spg), dim3(256),
@@ -1173,36 +1222,41 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
         if (it != g_graphs.end()) {
             exec = it->second;
         } else {
+            // the chunk as an explicit graph: `chunk` iterations, then the
+            // device-side done flag copied to the pinned host word
             hipGraph_t graph = nullptr;
-            hipError_t e1, e2;
-            {
-                // no other thread allocates or frees device memory meanwhile
-                CaptureLock lk(capture_mutex());
-                SKL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipError_t e1 = hipGraphCreate(&graph, 0);
+            if (e1 == hipSuccess) {
+                t_rec.on = true;
+                t_rec.graph = graph;
+                t_rec.last = nullptr;
+                t_rec.err = hipSuccess;
                 for (int c = 0; c < chunk; ++c) launch_iteration(p, q, B, s);
-                e1 = hipMemcpyAsync(hf, &B.st->done, sizeof(int), hipMemcpyDeviceToHost, s);
-                e2 = hipStreamEndCapture(s, &graph);
-            }
-            if (e1 != hipSuccess || e2 != hipSuccess) {
-                // run this solve without a graph: make sure the stream has
-                // left capture mode and drop what was recorded
-                graph_ok = false;
-                hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-                if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
-                    hipGraph_t g2 = nullptr;
-                    (void)hipStreamEndCapture(s, &g2);
-                    if (g2) (void)hipGraphDestroy(g2);
+                t_rec.on = false;
+                e1 = t_rec.err;
+                if (e1 == hipSuccess) {
+                    hipGraphNode_t cp = nullptr;
+                    e1 = hipGraphAddMemcpyNode1D(&cp, graph, t_rec.last ? &t_rec.last : nullptr,
+                                                 t_rec.last ? 1 : 0, hf, &B.st->done, sizeof(int),
+                                                 hipMemcpyDeviceToHost);
                 }
-                if (graph) (void)hipGraphDestroy(graph);
-                (void)hipGetLastError();
-            } else {
+            }
+            if (e1 == hipSuccess) {
                 // bounded cache: callers whose buffers move every call (host
                 // inputs staged afresh) would otherwise grow it without end
                 if (g_graphs.size() >= 32) lsqr_release_graphs();
                 AllocLock lk(capture_mutex());
-                SKL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-                SKL_HIP(hipGraphDestroy(graph));
+                e1 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            }
+            if (graph) (void)hipGraphDestroy(graph);
+            t_rec.keep.clear();
+            if (e1 == hipSuccess) {
                 g_graphs[key] = exec;
+            } else {
+                // run this solve's iterations without a graph
+                graph_ok = false;
+                exec = nullptr;
+                (void)hipGetLastError();
             }
         }
     }

@@ -49,6 +49,7 @@ template <class T> struct RsArgs {
     T *rowv;        // [2][nc]
     T *rcp;         // [P][nc]
     GridSync bar;   // barrier counter (zero at launch), error word, poll limit
+    const int *only_if; // when set, the kernel runs only if *only_if != 0
 };
 
 template <class T> __device__ inline T dlapy2_d(T x, T y) {
@@ -67,6 +68,7 @@ constexpr int RS_T = 512;
 
 template <class T>
 __global__ __launch_bounds__(RS_T) void k_qr_rowslice(RsArgs<T> a) {
+    if (a.only_if && *a.only_if == 0) return;
     unsigned bar_target = 0;
     extern __shared__ unsigned char smem_raw[];
     const int P = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
@@ -364,6 +366,7 @@ template <class T> struct RpArgs {
     T *rowv;        // [2][64] row jj of the panel (workgroup 0)
     T *rcp;         // [P][64] norm recomputation sums
     GridSync bar;   // barrier counter (zero at launch), error word, poll limit
+    const int *only_if; // when set, the kernel runs only if *only_if != 0
 };
 
 constexpr int RP_W = 8, RP_T = 64 * RP_W;
@@ -371,6 +374,7 @@ constexpr int RP_W = 8, RP_T = 64 * RP_W;
 template <class T, int K>
 __global__ __launch_bounds__(RP_T) void k_qr_regpanel(RpArgs<T> a) {
     static_assert(K % 8 == 0, "rows come in 64-row chunks");
+    if (a.only_if && *a.only_if == 0) return;
     __shared__ T tile[64 * 65];
     __shared__ T s_g[2][RP_W][64];
     __shared__ T s_row[2][64];
@@ -608,7 +612,8 @@ static void launch_regpanel(int P, const RpArgs<T> &a, hipStream_t s) {
 // it the barrier latency -- low.  SKL_QR_PANEL=lds selects the LDS kernel instead.
 template <class T>
 static bool regpanel_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T *tau,
-                        int32_t *jpvt, int32_t *swaps, int hq_norms, Arena &ar) {
+                        int32_t *jpvt, int32_t *swaps, int hq_norms, const int *only_if,
+                        Arena &ar) {
     static const bool off = [] {
         const char *e = getenv("SKL_QR_PANEL");
         return e && std::string(e) == "lds";
@@ -630,6 +635,7 @@ static bool regpanel_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T 
     a.jpvt = jpvt;
     a.swaps = swaps;
     a.hq_norms = hq_norms;
+    a.only_if = only_if;
     a.part = ar.alloc_n<T>((size_t)2 * P * 64);
     a.rowv = ar.alloc_n<T>((size_t)2 * 64);
     a.rcp = ar.alloc_n<T>((size_t)P * 64);
@@ -651,11 +657,34 @@ static constexpr size_t RS_LDS_BUDGET = 150 * 1024;
 
 // Factor (or pivot-factor) the m x nc block at W: nsteps reflectors, pivoting
 // over the first npiv columns when npiv > 0.
+// The CholeskyQR2 + Householder-reconstruction panel (k_cqr.hip) for FP64
+// panels of at most 64 columns that are factored whole (all pivoting
+// columns), tall enough that the column-by-column kernel needs several
+// workgroups; SKL_QR_CQR=0 turns it off.
+int *cqr_panel(long mp, int b, int pivot, double *Wp, long ldw, double *tau, int32_t *jpvt,
+               int32_t *swaps, int hq_norms, Arena &ar);
+static bool cqr_wanted(long m, int nc, int npiv, int nsteps) {
+    static const bool on = [] {
+        const char *e = getenv("SKL_QR_CQR");
+        return !(e && e[0] == '0');
+    }();
+    return on && !ctx().no_alloc && nc == nsteps && nc <= 64 && (npiv == 0 || npiv == nc) &&
+           m >= 512;
+}
+
 template <class T>
 static void rowslice_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T *tau,
                         int32_t *jpvt, int32_t *swaps, int hq_norms, Arena &ar) {
     if (nsteps <= 0) return;
-    if (regpanel_qr<T>(m, nc, npiv, nsteps, W, ldw, tau, jpvt, swaps, hq_norms, ar)) return;
+    // with the fast panel queued first, the kernels below run only when it
+    // gave up (its flag nonzero)
+    const int *only_if = nullptr;
+    if constexpr (std::is_same<T, double>::value) {
+        if (cqr_wanted(m, nc, npiv, nsteps))
+            only_if = cqr_panel(m, nc, npiv > 0 ? 1 : 0, W, ldw, tau, jpvt, swaps, hq_norms, ar);
+    }
+    if (regpanel_qr<T>(m, nc, npiv, nsteps, W, ldw, tau, jpvt, swaps, hq_norms, only_if, ar))
+        return;
     const int P_half = barrier_grid_cap();
     const int P_max = std::max(1, ctx().num_cu);
     size_t fixed = sizeof(T) * (2 * (size_t)nc + 2 * (size_t)std::max(npiv, 1) + RS_T) +
@@ -697,6 +726,7 @@ static void rowslice_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T 
     a.swaps = swaps;
     a.hq_norms = hq_norms;
     a.rpw = (int)rpw;
+    a.only_if = only_if;
     a.ldl = (int)(rpw | 1);
     a.two_phase = ((long)P * nc > 32768) ? 1 : 0;
     a.part = ar.alloc_n<T>((size_t)2 * P * nc);

@@ -39,16 +39,14 @@ struct Error : std::runtime_error {
     } while (0)
 
 // ---------------------------------------------------------------------------
-// Graph capture vs. device allocation.  The runtime invalidates a stream
-// capture in progress on one thread when another thread frees device memory
-// (hipFree waits for every stream of the device) or otherwise allocates, so
-// the LSQR graph captures hold this lock shared and every device or pinned
-// allocation, free, stream/event creation and graph instantiation holds it
-// exclusively.
+// Device allocations, frees, stream/event creation and graph instantiation
+// of all worker threads take this lock, one at a time (hipFree waits for
+// every stream of the device).  No stream is ever captured: the LSQR graphs
+// are built node by node (k_lsqr.hip), because another thread's runtime calls
+// invalidated stream captures mid-recording and left the stream unusable.
 // ---------------------------------------------------------------------------
 std::shared_mutex &capture_mutex();
 using AllocLock = std::unique_lock<std::shared_mutex>;
-using CaptureLock = std::shared_lock<std::shared_mutex>;
 
 // ---------------------------------------------------------------------------
 // Device memory: a grow-only arena per context; buffers are carved with a

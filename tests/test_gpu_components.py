@@ -159,7 +159,8 @@ def test_grid_barrier_give_up_is_reported(skl, O):
     b = A @ x_star
     skl.set_barrier_spins(0)
     try:
-        for call in (lambda: skl.cpqr(A), lambda: skl.qr(A), lambda: skl.cpqr(Ah),
+        for call in (lambda: skl.cpqr(A), lambda: skl.qr(A, precision=skl.SKL_MIXED),
+                     lambda: skl.cpqr(Ah),
                      lambda: skl.dense_solve(np.vstack([A, A]), np.concatenate([b, b]),
                                              cpqr=True, it_tol=1e-10, seed=3)):
             with pytest.raises(skl.SklError) as ei:
@@ -176,3 +177,38 @@ def test_grid_barrier_give_up_is_reported(skl, O):
     W, jpvt, tau = skl.cpqr(Ah)
     Wo, jo, to = O.cpqr(Ah)
     assert np.array_equal(jpvt, jo)
+
+
+def _q_from(W, tau):
+    m, n = W.shape
+    Q = np.eye(m)
+    for i in range(len(tau) - 1, -1, -1):
+        v = np.zeros(m)
+        v[i] = 1.0
+        v[i + 1:] = W[i + 1:, i]
+        Q = Q - tau[i] * np.outer(v, v @ Q)
+    return Q
+
+
+@pytest.mark.parametrize("pivot", [False, True])
+def test_qr_panel_fallback_on_dependent_columns(skl, pivot):
+    """A panel with a repeated column: the CholeskyQR2 panel gives up on it
+    (the repeated column's partial norm vanishes) and the column-by-column
+    kernel factors the untouched panel; the factorization is still exact:
+    Q orthonormal and A P = Q R."""
+    rng = np.random.default_rng(17)
+    m, n = 1500, 1040 if pivot else 160
+    A = rng.standard_normal((m, n)) * np.logspace(0, -2, n)
+    A[:, 37] = A[:, 5]
+    A[:, n - 3] = 2.0 * A[:, 100]
+    if pivot:
+        W, jpvt, tau = skl.cpqr(A)
+        P = jpvt - 1
+        assert sorted(P) == list(range(n))
+    else:
+        W, tau = skl.qr(A)
+        P = np.arange(n)
+    Q = _q_from(W, tau)
+    R = np.triu(W[:n])
+    assert np.abs(Q.T @ Q - np.eye(m)).max() < 1e-12
+    assert rel(A[:, P], Q[:, :n] @ R) < 1e-13
