@@ -20,7 +20,10 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# MI355X published peaks: HBM3E bandwidth, dense FP64 and FP32 matrix rates
 HBM_PEAK_GBS = 8000.0
+MFMA_F64_TFLOPS = 78.6
+MFMA_F32_TFLOPS = 157.3
 
 
 def log(*a):
@@ -102,46 +105,217 @@ def reference_solution(torch, At, b):
         return None
 
 
-def cpu_baseline_dense(skl_unused, At_host, b_host, W, gpu_iters, budget_s):
-    """The oracle (plain-C restatement, 1 thread) on a bounded sample of the
-    same solve, scaled to one solve: the randomized transform + hashing of a
-    column subset (scaled by column count), the HQRRP CPQR of an m x c_s
-    block (scaled by the 2 m c^2 - 2/3 c^3 flop count) and two LSQR
-    iterations over the full A (times the GPU's iteration count, plus one
-    GEMV for the x0 residual)."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_py as O
-    n, d = W["n"], W["d"]
+# ---------------------------------------------------------------------------
+# Accuracy: the relative normal-equation residual ||A^T r|| / (||A||_2 ||r||)
+# recomputed on the host with r = b - A x and A^T r accumulated in extended
+# precision (x86 long double, 64-bit significand), so that the rounding of r
+# itself does not set the floor; the same for the dense-QR solution beside it.
+# ---------------------------------------------------------------------------
+def norm2_power(matvec, rmatvec, d, iters=30):
+    v = np.random.default_rng(0).standard_normal(d)
+    v /= np.linalg.norm(v)
+    s = 0.0
+    for _ in range(iters):
+        w = rmatvec(matvec(v))
+        s = np.linalg.norm(w)
+        v = w / s
+    return math.sqrt(s)
+
+
+def ne_residual_dense(A, b, xs):
+    """A: n x d float64 (any layout); xs: name -> x (host).  -> name -> value."""
+    A_ld = np.asarray(A, dtype=np.longdouble)
+    b_ld = np.asarray(b, dtype=np.longdouble)
+    nA = norm2_power(lambda v: A @ v, lambda u: A.T @ u, A.shape[1])
+    out = {}
+    for name, x in xs.items():
+        if x is None:
+            out[name] = None
+            continue
+        r = b_ld - A_ld @ np.asarray(x, dtype=np.longdouble)
+        g = (A_ld.T @ r).astype(np.float64)
+        out[name] = float(np.linalg.norm(g) / (nA * np.linalg.norm(r.astype(np.float64))))
+    return out, nA
+
+
+def ne_residual_sparse(A, b, xs):
+    """A: scipy CSC; extended-precision r = b - A x and A^T r per column."""
+    data = A.data.astype(np.longdouble)
+    idx, ptr = A.indices, A.indptr
+    cols = np.repeat(np.arange(A.shape[1]), np.diff(ptr))
+    nA = norm2_power(lambda v: A @ v, lambda u: A.T @ u, A.shape[1])
+    nz = np.diff(ptr) > 0
+    out = {}
+    for name, x in xs.items():
+        if x is None:
+            out[name] = None
+            continue
+        ax = np.zeros(A.shape[0], dtype=np.longdouble)
+        np.add.at(ax, idx, data * np.asarray(x, dtype=np.longdouble)[cols])
+        r = np.asarray(b, dtype=np.longdouble) - ax
+        g = np.zeros(A.shape[1], dtype=np.longdouble)
+        g[nz] = np.add.reduceat(data * r[idx], ptr[:-1][nz])
+        out[name] = float(np.linalg.norm(g.astype(np.float64)) /
+                          (nA * np.linalg.norm(r.astype(np.float64))))
+    return out, nA
+
+
+# ---------------------------------------------------------------------------
+# Per-phase rooflines of one solve: algorithmic bytes (or flops) of the phase
+# over its measured seconds, against the MI355X peak that bounds it.
+#   sketch   8 n d + 8 n (A and b read once) + 8 m (d + 1) (sketch written)
+#   QR       2 m d^2 - 2/3 d^3 flop (+ 2 (nb + pp) m d for the HQRRP Gaussian
+#            sketch when d >= 1000), FP64 matrix peak; mixed: the FP32 Gram
+#            m d^2 (symmetric half) against the FP32 matrix peak
+#   x0       8 n d (the residual's pass over A)
+#   LSQR     per iteration 8 n d + 8 r (r + 1) + 64 (n + d) (one-pass kernel)
+# Sparse: 12 B per nonzero for one pass over A, the sketch writes 8 m (d + 1).
+# ---------------------------------------------------------------------------
+def phase_rooflines(n, d, m, phases, it, nnz=None, mixed=False, hqrrp=False):
+    t = phases
+    out = {}
+    pass_a = 12.0 * nnz if nnz is not None else 8.0 * n * d
+    if t.get("t_sketch", 0) > 0:
+        by = pass_a + 8.0 * n + 8.0 * m * (d + 1)
+        gbs = by / t["t_sketch"] / 1e9
+        out["sketch"] = {"bytes": by, "GB/s": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
+    if t.get("t_qr", 0) > 0:
+        if mixed:
+            fl = m * d * d + d ** 3 / 3.0
+            tf = fl / t["t_qr"] / 1e12
+            out["qr"] = {"flop": fl, "TFLOP/s": tf, "frac_mfma_f32": tf / MFMA_F32_TFLOPS}
+        else:
+            fl = 2.0 * m * d * d - 2.0 / 3.0 * d ** 3 + (2.0 * 74 * m * d if hqrrp else 0.0)
+            tf = fl / t["t_qr"] / 1e12
+            out["qr"] = {"flop": fl, "TFLOP/s": tf, "frac_mfma_f64": tf / MFMA_F64_TFLOPS}
+    if t.get("t_x0", 0) > 0:
+        gbs = pass_a / t["t_x0"] / 1e9
+        out["x0"] = {"bytes": pass_a, "GB/s": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
+    if t.get("t_lsqr", 0) > 0 and it > 0:
+        by = it * (pass_a + 8.0 * d * (d + 1) + 64.0 * (n + d))
+        gbs = by / t["t_lsqr"] / 1e9
+        out["lsqr"] = {"bytes_per_iteration": by / it, "GB/s": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
+    return out
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: the BLAS/LAPACK restatement oracle/cpu_ref.py on this host,
+# at one thread (the reference's configuration) and at all cores.
+# ---------------------------------------------------------------------------
+def _cpu_ref():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import cpu_ref
+    return cpu_ref
+
+
+def _median3(f):
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        f()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[1]
+
+
+def cpu_baseline_dense(A, b, W, gpu_iters, budget_s):
+    """The 4000 x 200 configuration solved in full (median of 3) at one
+    thread and at all cores; the 40000 x 4000 one in full at all cores when
+    its estimate fits the budget, and at one thread from a bounded sample:
+    the transform + hash of 400 columns (x d/400), dgeqp3 of m x 1000 (x by
+    the 2 m c^2 - 2/3 c^3 flop count) and 3 LSQR iterations (x the GPU's
+    iteration count)."""
+    from threadpoolctl import threadpool_limits
+    cr = _cpu_ref()
+    cores = cr.host_cores()
+    n, d = A.shape
     m = int(math.ceil(d * W["gamma"]))
-    A = At_host.T  # F-ordered n x d view
-    t0 = time.perf_counter()
-    cs = max(2, min(d, 200))
-    D, rows, vals = O.dense_pattern(1234, n, m, W["k"])
-    ts = time.perf_counter()
-    Y = O.fut(A[:, :cs], D)
-    O.hash_apply(m, rows, vals, W["k"], 1 / math.sqrt(W["k"]), Y)
-    t_sketch = (time.perf_counter() - ts) * d / cs
-    del Y
-    cq = max(2, min(d, 500))
-    rng = np.random.default_rng(0)
-    S = rng.standard_normal((m, cq))
-    tq = time.perf_counter()
-    O.hqrrp(S, 64, 10)
+    kw = dict(gamma=W["gamma"], k=W["k"], abs_tol=1e-8, rcond=1e-12, it_tol=1e-10, seed=1234)
+    t_start = time.perf_counter()
+    rng = np.random.default_rng(7)
+    A1 = np.asfortranarray(rng.standard_normal((4000, 200)) * np.logspace(0, -4, 200))
+    b1 = A1 @ rng.standard_normal(200) + 1e-8 * rng.standard_normal(4000)
+    c1 = {}
+    for label, th in [("1_thread", 1), ("all_cores", cores)]:
+        with threadpool_limits(th):
+            c1[label] = _median3(lambda: cr.dense_solve(A1, b1, workers=th, **kw))
+    # 40000 x 4000: one thread from samples
     flops = lambda c: 2.0 * m * c * c - 2.0 / 3.0 * c ** 3
-    t_qr = (time.perf_counter() - tq) * flops(d) / flops(cq)
-    tl = time.perf_counter()
-    O.lsqr_dense(A, np.eye(d), b_host, 0.0, 2)
-    t_it = (time.perf_counter() - tl) / 2.0
-    t_lsqr = t_it * (gpu_iters + 0.5)
-    total = t_sketch + t_qr + t_lsqr
-    spent = time.perf_counter() - t0
+    cs, cq, nit = min(d, 400), min(d, 1000), 3
+    with threadpool_limits(1):
+        ts = sorted(cr.dense_solve(np.asfortranarray(A[:, :cs]), b, lsqr_it_cap=0,
+                                   **kw)[1]["t_sketch"] for _ in range(3))[1]
+        Sq = np.asfortranarray(rng.standard_normal((m, cq)))
+        tq = _median3(lambda: __import__("scipy.linalg").linalg.qr(Sq, pivoting=True,
+                                                                  mode="economic"))
+        Ad = np.asfortranarray(A)
+        R = np.triu(rng.standard_normal((d, d))) + 10 * np.eye(d)
+        import scipy.linalg as sl
+        mv = lambda v: Ad @ sl.solve_triangular(R, v)
+        rmv = lambda u: sl.solve_triangular(R, Ad.T @ u, trans="T")
+        tl = _median3(lambda: cr.lsqr(mv, rmv, b.copy(), d, 0.0, nit)) / nit
+    one = {"sketch": ts * d / cs, "cpqr": tq * flops(d) / flops(cq), "lsqr_per_it": tl}
+    one_total = one["sketch"] + one["cpqr"] + tl * (gpu_iters + 1)
+    # all cores: the full solve when it fits
+    allc = None
+    with threadpool_limits(cores):
+        # a rough estimate (memory-bound GEMVs scale with a few threads only)
+        est = one_total / min(8.0, max(1.0, cores / 2.0))
+        if est <= budget_s:
+            x, info = cr.dense_solve(A, b, workers=cores, **kw)
+            allc = {"sec_per_solve": info["t_total"], "lsqr_iterations": info["it"],
+                    "phases_s": {k: info[k] for k in ["t_sketch", "t_qr", "t_x0", "t_lsqr"]},
+                    "measured": "full solve, once"}
+    spent = time.perf_counter() - t_start
+    value, cores_used = (allc["sec_per_solve"], cores) if allc else (one_total, 1)
     return {
-        "value": total, "unit": "sec/solve", "cores": 1, "kind": "port",
-        "sample": (f"oracle on the host: DHT+hash of {cs}/{d} columns (x{d / cs:.0f}), HQRRP of "
-                   f"{m}x{cq} (x{flops(d) / flops(cq):.1f} by flops), 2 LSQR iterations on the "
-                   f"full A (x{gpu_iters + 0.5:.1f}); {spent:.1f}s of CPU work"),
-        "phases_s": {"sketch": t_sketch, "cpqr": t_qr, "lsqr_per_it": t_it},
+        "value": value, "unit": "sec/solve", "cores": cores_used, "kind": "port",
+        "sample": ("oracle/cpu_ref.py (numpy/scipy BLAS + LAPACK dgeqp3 in place of HQRRP, "
+                   "which the reference's hqrrp/README.md:93-95 puts about 3x faster on one "
+                   "core): " + (f"the full {n}x{d} solve at all cores" if allc else
+                                "one thread from samples") + f"; {spent:.0f}s of CPU work"),
+        "cpu_model": cr.cpu_model(), "nproc": cores,
+        "one_thread_sampled": {"sec_per_solve": one_total, "phases_s": one,
+                               "sample": f"transform+hash of {cs}/{d} columns (x{d / cs:.0f}), "
+                                         f"dgeqp3 of {m}x{cq} (x{flops(d) / flops(cq):.1f} by "
+                                         f"flops), {nit} LSQR iterations (x{gpu_iters + 1}), "
+                                         "each the median of 3"},
+        "all_cores_full": allc,
+        "dense_4000x200_full_solve_median_of_3_s": c1,
     }
+
+
+def cpu_baseline_sparse(A, b, W, gpu_iters):
+    """The sparse solve restated on the host (dgeqrf in place of SPQR) at one
+    thread from samples: the hashing sketch in full, dgeqrf of m x 1000 (x by
+    flops) and 3 LSQR iterations (x the GPU's iteration count)."""
+    from threadpoolctl import threadpool_limits
+    import scipy.linalg as sl
+    cr = _cpu_ref()
+    n, d = A.shape
+    m = int(math.ceil(d * W["gamma"]))
+    rng = np.random.default_rng(8)
+    flops = lambda c: 2.0 * m * c * c - 2.0 / 3.0 * c ** 3
+    cq, nit = min(d, 1000), 3
+    t0 = time.perf_counter()
+    with threadpool_limits(1):
+        rows, vals = cr.O.sparse_pattern(1, n, m, W["k"])
+        S = cr.hashing_matrix(m, n, rows, vals, W["k"])
+        ts = _median3(lambda: (S @ A).toarray())
+        Sq = np.asfortranarray(rng.standard_normal((m, cq)))
+        tq = _median3(lambda: sl.qr(Sq, mode="economic"))
+        R = np.triu(rng.standard_normal((d, d))) + 10 * np.eye(d)
+        Ar = A.tocsr()
+        mv = lambda v: Ar @ sl.solve_triangular(R, v)
+        rmv = lambda u: sl.solve_triangular(R, Ar.T @ u, trans="T")
+        tl = _median3(lambda: cr.lsqr(mv, rmv, b.copy(), d, 0.0, nit)) / nit
+    total = ts + tq * flops(d) / flops(cq) + tl * (gpu_iters + 1)
+    return {"value": total, "unit": "sec/solve", "cores": 1, "kind": "port",
+            "sample": (f"oracle/cpu_ref.py at one thread: the hashing sketch in full, dgeqrf of "
+                       f"{m}x{cq} (x{flops(d) / flops(cq):.1f} by flops; SuiteSparseQR is "
+                       f"absent), {nit} LSQR iterations (x{gpu_iters + 1}), each the median of "
+                       f"3; {time.perf_counter() - t0:.0f}s of CPU work"),
+            "cpu_model": cr.cpu_model(), "nproc": cr.host_cores(),
+            "phases_s": {"sketch": ts, "qr": tq * flops(d) / flops(cq), "lsqr_per_it": tl}}
 
 
 def profile_pass(skl, torch, solve, nsolves=3):
@@ -204,6 +378,7 @@ def run_dense(args, skl, torch, W):
     # 1e-5 even for LAPACK's own least-squares solution (rounding in r and the
     # row scaling set its floor), so accuracy is judged against the QR solution.
     xq = reference_solution(torch, At, b)
+    res["xq"] = xq
     for name in ["fp64", "mixed"]:
         R = res[name]
         R["x_rel_vs_qr"] = (None if xq is None else
@@ -299,6 +474,7 @@ def run_sparse(args, skl, torch, W):
         del Ad
     except Exception as e:
         log(f"dense reference skipped: {e}")
+    res["xq"] = xq
     for name in ["fp64", "mixed"]:
         R = res[name]
         R["x_rel_vs_qr"] = (None if xq is None else
@@ -341,48 +517,6 @@ def escalation_case(skl, torch, W, kw):
     return out
 
 
-def cpu_baseline_sparse(A, bh, W, gpu_iters, rank):
-    """The oracle (plain-C restatement, 1 thread) on a bounded sample of the
-    sparse solve, scaled to one solve: the hashing sketch of a column subset
-    (scaled by column count), the Householder QR of an m x c_s block of the
-    sketch (scaled by the 2 m c^2 - 2/3 c^3 flop count) and two LSQR iterations
-    with the full A and a rank x rank triangular factor (times the GPU's
-    iteration count)."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_py as O
-    n, d = W["n"], W["d"]
-    m = int(math.ceil(d * W["gamma"]))
-    t0 = time.perf_counter()
-    cs = max(2, min(d, 500))
-    rows, vals = O.sparse_pattern(1, n, m, W["k"])
-    As = A[:, :cs].tocsc()
-    ts = time.perf_counter()
-    O.sketch_sparse(m, rows, vals, W["k"], As.indptr.astype(np.int64),
-                    As.indices.astype(np.int64), As.data, n, cs)
-    t_sketch = (time.perf_counter() - ts) * d / cs
-    cq = max(2, min(d, 500))
-    rng = np.random.default_rng(0)
-    S = rng.standard_normal((m, cq))
-    tq = time.perf_counter()
-    O.geqrf(S)
-    flops = lambda c: 2.0 * m * c * c - 2.0 / 3.0 * c ** 3
-    t_qr = (time.perf_counter() - tq) * flops(d) / flops(cq)
-    R = np.triu(rng.standard_normal((rank, rank))) + 10 * np.eye(rank)
-    tl = time.perf_counter()
-    O.lsqr_sparse(A, R, bh, 0.0, 2)
-    t_it = (time.perf_counter() - tl) / 2.0
-    t_lsqr = t_it * (gpu_iters + 0.5)
-    spent = time.perf_counter() - t0
-    return {
-        "value": t_sketch + t_qr + t_lsqr, "unit": "sec/solve", "cores": 1, "kind": "port",
-        "sample": (f"oracle on the host: hashing sketch of {cs}/{d} columns (x{d / cs:.0f}), "
-                   f"Householder QR of {m}x{cq} (x{flops(d) / flops(cq):.1f} by flops), 2 LSQR "
-                   f"iterations with the full A and a {rank}x{rank} R (x{gpu_iters + 0.5:.1f}); "
-                   f"{spent:.1f}s of CPU work"),
-        "phases_s": {"sketch": t_sketch, "qr": t_qr, "lsqr_per_it": t_it},
-    }
-
-
 def make_pool(W, seed):
     """Distinct problems the request stream draws from (inputs generated once,
     host resident, dense in column-major order)."""
@@ -412,24 +546,30 @@ def make_pool(W, seed):
     return pool
 
 
-def run_serving(args, skl, W):
+def run_serving(args, skl, W, requests, rates):
     """The serving loop (continuous batching over worker slots, FP32
     preconditioner, FP64 LSQR) against sequential plain-FP64 solves of the same
-    request stream (Poisson arrivals).  Both take host arrays (PCIe included).
-    The sequential figures replay the arrival times against the measured
-    per-problem solve times of one FIFO worker."""
+    request stream, both measured: Poisson arrivals at each rate (a multiple of
+    one sequential FP64 worker's capacity), host inputs (PCIe included).  The
+    sequential run is one thread that takes the requests in arrival order and
+    solves each when it has arrived and the previous one is done; latency is
+    completion minus arrival in both runs."""
     log("generating the request pool")
     pool = make_pool(W, seed=77)
     rng = np.random.default_rng(78)
-    R = args.requests
-    which = rng.integers(0, len(pool), size=R)
+    which = rng.integers(0, len(pool), size=requests)
 
     def solve(P, precision):
         if P["kind"] == "dense":
             return skl.dense_solve(P["A"], P["b"], precision=precision, **P["opts"])
         return skl.sparse_solve(P["A"], P["b"], precision=precision, **P["opts"])
 
-    log("sequential FP64 solves of the pool")
+    def submit(S, P, precision):
+        if P["kind"] == "dense":
+            return S.submit_dense(P["A"], P["b"], precision=precision, **P["opts"])
+        return S.submit_sparse(P["A"], P["b"], precision=precision, **P["opts"])
+
+    log("sequential FP64 solves of the pool (service times, reference x)")
     t_seq, x_seq = [], []
     for P in pool:
         solve(P, skl.SKL_FP64)
@@ -438,78 +578,117 @@ def run_serving(args, skl, W):
         t_seq.append(time.perf_counter() - t0)
         x_seq.append(x)
     svc = np.array([t_seq[w] for w in which])
-    lam = args.load / float(svc.mean())
-    arr = np.cumsum(rng.exponential(1.0 / lam, size=R))
-    arr -= arr[0]
-    end = np.empty(R)
-    t = 0.0
-    for i in range(R):
-        t = max(t, arr[i]) + svc[i]
-        end[i] = t
-    lat_seq = end - arr
-    log(f"serving stream: {R} requests, rate {lam:.1f}/s, {args.slots} slots")
+    capacity = 1.0 / float(svc.mean())
+    pct = lambda a, q: float(np.percentile(np.asarray(a), q))
+    out = {"requests": requests, "slots": args.slots,
+           "sequential_fp64_capacity_solves_per_s": capacity, "rates": []}
     with skl.Server(slots=args.slots) as S:
         for P in pool[:4]:  # warm the workers' code paths
-            S.wait(S.submit_dense(P["A"], P["b"], precision=skl.SKL_MIXED, **P["opts"])
-                   if P["kind"] == "dense" else
-                   S.submit_sparse(P["A"], P["b"], precision=skl.SKL_MIXED, **P["opts"]))
-        tickets = []
-        t0 = time.perf_counter()
-        for i in range(R):
-            now = time.perf_counter() - t0
-            if arr[i] > now:
-                time.sleep(arr[i] - now)
-            P = pool[which[i]]
-            if P["kind"] == "dense":
-                tickets.append(S.submit_dense(P["A"], P["b"], precision=skl.SKL_MIXED,
-                                              **P["opts"]))
-            else:
-                tickets.append(S.submit_sparse(P["A"], P["b"], precision=skl.SKL_MIXED,
-                                               **P["opts"]))
-        lat, worst, its = [], 0.0, []
-        for i, tk in enumerate(tickets):
-            x, info, q, r = S.wait(tk)
-            lat.append(q + r)
-            xr = x_seq[which[i]]
-            worst = max(worst, float(np.linalg.norm(x - xr) / np.linalg.norm(xr)))
-            if info.flag != 0:
-                raise RuntimeError(f"request {i}: LSQR flag {info.flag}")
-        span = time.perf_counter() - t0
-        # saturation: the same requests all at once
+            S.wait(submit(S, P, skl.SKL_MIXED))
+        for load in rates:
+            lam = load * capacity
+            arr = np.cumsum(rng.exponential(1.0 / lam, size=requests))
+            arr -= arr[0]
+            log(f"rate {lam:.1f}/s ({load} x sequential capacity): sequential FP64")
+            lat_seq = []
+            t0 = time.perf_counter()
+            for i in range(requests):
+                now = time.perf_counter() - t0
+                if arr[i] > now:
+                    time.sleep(arr[i] - now)
+                solve(pool[which[i]], skl.SKL_FP64)
+                lat_seq.append(time.perf_counter() - t0 - arr[i])
+            span_seq = time.perf_counter() - t0
+            log("  server, mixed precision")
+            tickets = []
+            t0 = time.perf_counter()
+            for i in range(requests):
+                now = time.perf_counter() - t0
+                if arr[i] > now:
+                    time.sleep(arr[i] - now)
+                tickets.append(submit(S, pool[which[i]], skl.SKL_MIXED))
+            lat, worst = [], 0.0
+            for i, tk in enumerate(tickets):
+                x, info, q, r = S.wait(tk)
+                lat.append(q + r)
+                xr = x_seq[which[i]]
+                worst = max(worst, float(np.linalg.norm(x - xr) / np.linalg.norm(xr)))
+                if info.flag != 0:
+                    raise RuntimeError(f"request {i}: LSQR flag {info.flag}")
+            span = time.perf_counter() - t0
+            out["rates"].append({
+                "load_vs_sequential_capacity": load, "arrival_rate_per_s": lam,
+                "server_mixed": {"solves_per_s": requests / span, "p50_latency_s": pct(lat, 50),
+                                 "p99_latency_s": pct(lat, 99)},
+                "sequential_fp64": {"solves_per_s": requests / span_seq,
+                                    "p50_latency_s": pct(lat_seq, 50),
+                                    "p99_latency_s": pct(lat_seq, 99)},
+                "max_x_rel_server_vs_sequential_fp64": worst})
+            r = out["rates"][-1]
+            log(f"  server p50 {r['server_mixed']['p50_latency_s'] * 1e3:.1f} ms p99 "
+                f"{r['server_mixed']['p99_latency_s'] * 1e3:.1f} ms; sequential p50 "
+                f"{r['sequential_fp64']['p50_latency_s'] * 1e3:.1f} ms p99 "
+                f"{r['sequential_fp64']['p99_latency_s'] * 1e3:.1f} ms")
+        # saturation: the whole stream at once
         t1 = time.perf_counter()
-        burst = []
-        for i in range(R):
-            P = pool[which[i]]
-            if P["kind"] == "dense":
-                burst.append(S.submit_dense(P["A"], P["b"], precision=skl.SKL_MIXED, **P["opts"]))
-            else:
-                burst.append(S.submit_sparse(P["A"], P["b"], precision=skl.SKL_MIXED,
-                                             **P["opts"]))
+        burst = [submit(S, pool[which[i]], skl.SKL_MIXED) for i in range(requests)]
         for tk in burst:
             S.wait(tk)
-        burst_span = time.perf_counter() - t1
-    lat = np.array(lat)
-    out = {
-        "requests": R, "arrival_rate_per_s": lam, "load_vs_sequential_fp64": args.load,
-        "slots": args.slots,
-        "server_mixed": {"solves_per_s": R / span, "p50_latency_s": float(np.percentile(lat, 50)),
-                         "p99_latency_s": float(np.percentile(lat, 99)),
-                         "saturated_solves_per_s": R / burst_span},
-        "sequential_fp64": {"solves_per_s": R / float(end[-1]),
-                            "capacity_solves_per_s": 1.0 / float(svc.mean()),
-                            "p50_latency_s": float(np.percentile(lat_seq, 50)),
-                            "p99_latency_s": float(np.percentile(lat_seq, 99))},
-        "max_x_rel_server_vs_sequential_fp64": worst,
-        "pool": {"dense": sum(P["kind"] == "dense" for P in pool),
-                 "sparse": sum(P["kind"] == "sparse" for P in pool),
-                 "largest_dense": max(P["n"] * P["d"] for P in pool if P["kind"] == "dense"),
-                 "largest_sparse_nnz": max(P["A"].nnz for P in pool if P["kind"] == "sparse")},
-    }
-    log(f"server p50 {out['server_mixed']['p50_latency_s'] * 1e3:.1f} ms p99 "
-        f"{out['server_mixed']['p99_latency_s'] * 1e3:.1f} ms; sequential p50 "
-        f"{out['sequential_fp64']['p50_latency_s'] * 1e3:.1f} ms p99 "
-        f"{out['sequential_fp64']['p99_latency_s'] * 1e3:.1f} ms")
+        out["saturated_server_mixed_solves_per_s"] = requests / (time.perf_counter() - t1)
+    out["pool"] = {"problems": len(pool),
+                   "dense": sum(P["kind"] == "dense" for P in pool),
+                   "sparse": sum(P["kind"] == "sparse" for P in pool),
+                   "largest_dense_entries": max(P["n"] * P["d"] for P in pool
+                                                if P["kind"] == "dense"),
+                   "largest_sparse_nnz": max(P["A"].nnz for P in pool if P["kind"] == "sparse")}
     return out
+
+
+def dominant_roofline(res, workload):
+    """The LSQR operator product with the largest average launch time, timed
+    live with HIP events (profile_pass), against the HBM peak."""
+    prof = res["prof"]
+    dom = max(prof, key=lambda s: prof[s]["avg_ms"])
+    P = prof[dom]
+    achieved = P["bytes"] / (P["avg_ms"] * 1e-3) / 1e9 if P["avg_ms"] > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get(workload, {}).get(str(dom))
+    return {"bound": "hbm", "kernel": P["name"], "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "avg_launch_ms": P["avg_ms"], "launches_timed": P["launches"],
+            "bytes_per_launch": P["bytes"],
+            "other_kernels": {v["name"]: {"avg_ms": v["avg_ms"],
+                                          "GB/s": (v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9
+                                                   if v["avg_ms"] > 0 else 0.0)}
+                              for s, v in prof.items() if s != dom and v["launches"] > 0}}
+
+
+def pct_lat(lat, q):
+    lat = sorted(lat)
+    return lat[min(len(lat) - 1, max(0, int(math.ceil(q / 100.0 * len(lat))) - 1))]
+
+
+def solver_summary(res, n, d, m, nnz=None, hqrrp=False):
+    f64, mixed = res["fp64"], res["mixed"]
+    return {
+        "fp64": {"sec_per_solve": f64["sec_per_solve"], "lsqr_iterations": f64["it"],
+                 "lsqr_estimate": f64["lsqr_est"], "x_rel_vs_qr": f64["x_rel_vs_qr"],
+                 "p50_latency_s": pct_lat(f64["lat"], 50), "p99_latency_s": pct_lat(f64["lat"], 99),
+                 "phases_s": f64["phases"],
+                 "phase_rooflines": phase_rooflines(n, d, m, f64["phases"], f64["it"], nnz,
+                                                    hqrrp=hqrrp)},
+        "mixed": {"sec_per_solve": mixed["sec_per_solve"], "lsqr_iterations": mixed["it"],
+                  "escalated": mixed["escalated"], "lsqr_estimate": mixed["lsqr_est"],
+                  "precond_eig_estimate": mixed["precond_eig"],
+                  "x_rel_vs_qr": mixed["x_rel_vs_qr"],
+                  "speedup_vs_fp64": f64["sec_per_solve"] / mixed["sec_per_solve"],
+                  "x_rel_vs_fp64": res["x_rel_mixed_vs_fp64"], "phases_s": mixed["phases"],
+                  "phase_rooflines": phase_rooflines(n, d, m, mixed["phases"], mixed["it"], nnz,
+                                                     mixed=True)},
+    }
 
 
 def main():
@@ -519,7 +698,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="dense_40000x4000", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--cpu-budget", type=float, default=30.0,
+                    help="seconds the all-cores CPU solve may take before it is sampled")
     ap.add_argument("--no-escalation-case", action="store_true",
                     help="sparse workload: skip the logspace(0,-10) FP64-fallback check")
     ap.add_argument("--no-other-solvers", action="store_true",
@@ -527,10 +707,16 @@ def main():
     ap.add_argument("--no-reference", action="store_true",
                     help="skip the dense Householder QR solution that x is compared with "
                          "(x_rel_vs_qr is then null)")
-    ap.add_argument("--requests", type=int, default=400, help="serving: requests in the stream")
+    ap.add_argument("--no-sparse", action="store_true",
+                    help="default workload: skip the 120000x5000 sparse configuration")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="default workload: skip the short serving stream")
+    ap.add_argument("--no-residual", action="store_true",
+                    help="skip the extended-precision normal-equation residuals")
+    ap.add_argument("--requests", type=int, default=2000, help="serving: requests in the stream")
     ap.add_argument("--slots", type=int, default=4, help="serving: worker slots")
-    ap.add_argument("--load", type=float, default=0.8,
-                    help="serving: arrival rate as a fraction of one sequential FP64 worker's "
+    ap.add_argument("--rates", default="0.5,0.8,1.2",
+                    help="serving: arrival rates as multiples of one sequential FP64 worker's "
                          "capacity")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the measurement, copy 1 GiB device-to-device three times "
@@ -548,13 +734,15 @@ def main():
         sys.exit(3)
     skl = load_skl()
     W = WORKLOADS[args.workload]
+    metric = "sec/solve to rel. normal-eq residual 1e-10, mixed vs FP64 on GPU; p99 latency"
     if W["kind"] == "serving":
-        sv = run_serving(args, skl, W)
+        rates = [float(r) for r in args.rates.split(",")]
+        sv = run_serving(args, skl, W, args.requests, rates)
+        mid = sv["rates"][len(rates) // 2]
         print(json.dumps({
-            "metric": "sec/solve to rel. normal-eq residual 1e-10, mixed vs FP64 on GPU; p99 latency",
-            "value": sv["server_mixed"]["p99_latency_s"], "unit": "sec (p99 latency)",
-            "n_gpus": 1, "steps": sv["requests"], "warmup": 4,
-            "ms_per_step": 1e3 / sv["server_mixed"]["solves_per_s"], "higher_is_better": False,
+            "metric": metric, "value": mid["server_mixed"]["p99_latency_s"],
+            "unit": "sec (p99 latency)", "n_gpus": 1, "steps": sv["requests"], "warmup": 4,
+            "ms_per_step": 1e3 / mid["server_mixed"]["solves_per_s"], "higher_is_better": False,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": args.workload, "requests": sv["requests"],
                        "slots": sv["slots"], "generator": "d logU[20,2000], n logU[10d,200000], "
@@ -564,85 +752,63 @@ def main():
         skl.lib().skl_finalize()
         return
     sparse = W["kind"] == "sparse"
-    if sparse:
-        res = run_sparse(args, skl, torch, W)
-    else:
+    out = {"metric": metric, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic"}
+    if not sparse:
+        n, d = W["n"], W["d"]
+        m = int(math.ceil(d * W["gamma"]))
         At, b, res = run_dense(args, skl, torch, W)
-    prof = res["prof"]
-    dom = max(prof, key=lambda s: prof[s]["avg_ms"])
-    P = prof[dom]
-    achieved = P["bytes"] / (P["avg_ms"] * 1e-3) / 1e9 if P["avg_ms"] > 0 else 0.0
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            tj = json.load(fh)
-        traffic = tj.get(args.workload, {}).get(str(dom))
-    f64 = res["fp64"]
-    mixed = res["mixed"]
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("cpu baseline (oracle, 1 thread, bounded sample)")
-        if sparse:
-            cpu = cpu_baseline_sparse(*res["host"], W, f64["it"], f64["rank"])
-        else:
-            At_h = At.cpu().numpy()
-            b_h = b.cpu().numpy()
-            cpu = cpu_baseline_dense(skl, At_h, b_h, W, f64["it"], args.cpu_budget)
-        log(f"cpu baseline: {cpu['value']:.1f} s/solve ({cpu['sample']})")
-    if sparse:
-        config = {"workload": args.workload, "solver": "ls_sparse_spqr", "n": W["n"],
-                  "d": W["d"], "m": int(math.ceil(W["d"] * W["gamma"])), "k": W["k"],
-                  "nnz": int(res["host"][0].nnz), "it_tol": 1e-10,
-                  "generator": f"CSC density {W['density']}, N(0,1) values, columns x "
-                               f"logspace(0,-{W['decades']},{W['d']}), + unit diagonal"}
+        summ = solver_summary(res, n, d, m, hqrrp=d >= 1000)
+        f64 = res["fp64"]
+        out.update({
+            "value": f64["sec_per_solve"], "unit": "sec/solve",
+            "ms_per_step": f64["sec_per_solve"] * 1e3,
+            "config": {"workload": args.workload, "solver": "ls_dense_hashing_blendenpik",
+                       "n": n, "d": d, "m": m, "k": W["k"], "it_tol": 1e-10, "rcond": 1e-12,
+                       "generator": f"diag(w) G diag(logspace(0,-{W['decades']},{d})), "
+                                    f"w lognormal(0,{W['lognormal']})"},
+            "lsqr_iterations": f64["it"], "lsqr_estimate": f64["lsqr_est"],
+            "x_rel_vs_qr": f64["x_rel_vs_qr"],
+            "p50_latency_s": summ["fp64"]["p50_latency_s"],
+            "p99_latency_s": summ["fp64"]["p99_latency_s"],
+            "phases_s": f64["phases"], "phase_rooflines": summ["fp64"]["phase_rooflines"],
+            "mixed": summ["mixed"],
+            **({"other_entry_points": res["other"]} if "other" in res else {}),
+            "roofline": dominant_roofline(res, args.workload),
+        })
+        A_h = At.cpu().numpy().T  # column-major n x d view
+        b_h = b.cpu().numpy()
+        if not args.no_residual:
+            log("normal-equation residuals in extended precision")
+            xs = {"fp64": f64["x"].cpu().numpy(), "mixed": res["mixed"]["x"].cpu().numpy(),
+                  "dense_qr": None if res["xq"] is None else res["xq"].cpu().numpy()}
+            ne, nA = ne_residual_dense(A_h, b_h, xs)
+            out["ne_residual"] = {"definition": "||A^T r|| / (||A||_2 ||r||), r = b - A x and "
+                                                "A^T r in long double", **ne}
+        it64 = f64["it"]
+        del At, b, res, f64
+        torch.cuda.empty_cache()
+        cpu = None
+        if not args.no_cpu_baseline:
+            log("cpu baseline (oracle/cpu_ref.py)")
+            cpu = cpu_baseline_dense(A_h, b_h, W, it64, args.cpu_budget)
+            log(f"cpu baseline: {cpu['value']:.2f} s/solve at {cpu['cores']} cores ({cpu['sample']})")
+        out["cpu_baseline"] = cpu
+        del A_h, b_h
+        if not args.no_sparse:
+            out["sparse_120000x5000"] = sparse_section(args, skl, torch,
+                                                       WORKLOADS["sparse_120000x5000"])
+        if not args.no_serving:
+            Ws = dict(WORKLOADS["serving_stream"], pool=12, max_dense_elems=20_000_000)
+            log("short serving stream")
+            out["serving_stream_short"] = run_serving(args, skl, Ws, 100, [0.8])
     else:
-        config = {"workload": args.workload, "solver": "ls_dense_hashing_blendenpik",
-                  "n": W["n"], "d": W["d"], "m": int(math.ceil(W["d"] * W["gamma"])),
-                  "k": W["k"], "it_tol": 1e-10, "rcond": 1e-12,
-                  "generator": f"diag(w) G diag(logspace(0,-{W['decades']},{W['d']})), "
-                               f"w lognormal(0,{W['lognormal']})"}
-    lat = sorted(f64["lat"])
-    out = {
-        "metric": "sec/solve to rel. normal-eq residual 1e-10, mixed vs FP64 on GPU; p99 latency",
-        "value": f64["sec_per_solve"],
-        "unit": "sec/solve",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": f64["sec_per_solve"] * 1e3,
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": config,
-        "lsqr_iterations": f64["it"],
-        "lsqr_estimate": f64["lsqr_est"],
-        "x_rel_vs_qr": f64["x_rel_vs_qr"],
-        "p50_latency_s": lat[len(lat) // 2],
-        "p99_latency_s": lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)],
-        "phases_s": f64["phases"],
-        "mixed": {"sec_per_solve": mixed["sec_per_solve"], "lsqr_iterations": mixed["it"],
-                  "escalated": mixed["escalated"], "lsqr_estimate": mixed["lsqr_est"],
-                  "precond_eig_estimate": mixed["precond_eig"],
-                  "x_rel_vs_qr": mixed["x_rel_vs_qr"],
-                  "speedup_vs_fp64": f64["sec_per_solve"] / mixed["sec_per_solve"],
-                  "x_rel_vs_fp64": res["x_rel_mixed_vs_fp64"], "phases_s": mixed["phases"]},
-        **({"other_entry_points": res["other"]} if "other" in res else {}),
-        "roofline": {"bound": "hbm", "kernel": P["name"], "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "avg_launch_ms": P["avg_ms"],
-                     "launches_timed": P["launches"], "bytes_per_launch": P["bytes"],
-                     "other_kernels": {v["name"]: {"avg_ms": v["avg_ms"],
-                                                   "GB/s": (v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9
-                                                            if v["avg_ms"] > 0 else 0.0)}
-                                       for s, v in prof.items()
-                                       if s != dom and v["launches"] > 0}},
-        "cpu_baseline": cpu,
-    }
-    if "escalation" in res:
-        out["mixed"]["escalation_case"] = res["escalation"]
+        sec = sparse_section(args, skl, torch, W)
+        out.update({"value": sec["fp64"]["sec_per_solve"], "unit": "sec/solve",
+                    "ms_per_step": sec["fp64"]["sec_per_solve"] * 1e3, "config": sec["config"],
+                    "roofline": sec.pop("roofline"), "cpu_baseline": sec.pop("cpu_baseline"),
+                    "sparse": sec})
     print(json.dumps(out), flush=True)
     if args.calibrate_copy:
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
@@ -652,6 +818,40 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 3 x 1 GiB device copies done")
     skl.lib().skl_finalize()
+
+
+def sparse_section(args, skl, torch, W):
+    """The 120000 x 5000 CSC configuration: FP64 and mixed solves, the
+    escalation case, residuals, rooflines and the CPU baseline."""
+    n, d = W["n"], W["d"]
+    m = int(math.ceil(d * W["gamma"]))
+    res = run_sparse(args, skl, torch, W)
+    A, bh = res["host"]
+    summ = solver_summary(res, n, d, m, nnz=A.nnz)
+    sec = {"config": {"workload": "sparse_120000x5000", "solver": "ls_sparse_spqr", "n": n,
+                      "d": d, "m": m, "k": W["k"], "nnz": int(A.nnz), "it_tol": 1e-10,
+                      "generator": f"CSC density {W['density']}, N(0,1) values, columns x "
+                                   f"logspace(0,-{W['decades']},{d}), + unit diagonal"},
+           **summ, "roofline": dominant_roofline(res, "sparse_120000x5000")}
+    if "escalation" in res:
+        sec["mixed"]["escalation_case"] = res["escalation"]
+    res.pop("prof", None)
+    if not args.no_residual:
+        log("sparse normal-equation residuals in extended precision")
+        xs = {"fp64": res["fp64"]["x"].cpu().numpy(), "mixed": res["mixed"]["x"].cpu().numpy(),
+              "dense_qr": None if res["xq"] is None else res["xq"].cpu().numpy()}
+        ne, _ = ne_residual_sparse(A, bh, xs)
+        sec["ne_residual"] = {"definition": "||A^T r|| / (||A||_2 ||r||), r = b - A x and "
+                                            "A^T r in long double", **ne}
+    it = res["fp64"]["it"]
+    del res
+    torch.cuda.empty_cache()
+    sec["cpu_baseline"] = None
+    if not args.no_cpu_baseline:
+        log("sparse cpu baseline (oracle/cpu_ref.py)")
+        sec["cpu_baseline"] = cpu_baseline_sparse(A, bh, W, it)
+        log(f"sparse cpu baseline: {sec['cpu_baseline']['value']:.2f} s/solve")
+    return sec
 
 
 if __name__ == "__main__":

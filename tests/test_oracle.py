@@ -5,6 +5,7 @@ import hashlib
 import json
 import math
 import os
+import sys
 
 import numpy as np
 import pytest
@@ -405,3 +406,24 @@ def test_overdetermined_error(O):
         O.ls_dense(A, np.ones(3))
     with pytest.raises(RuntimeError, match="Matrix is not overdetermined."):
         O.ls_sparse(sp.csc_matrix(A), np.ones(3))
+
+
+def test_cpu_ref_matches_oracle(O):
+    """The BLAS/LAPACK restatement that serves as the CPU baseline computes
+    what the oracle computes: dense (4000 x 200 configuration) and sparse."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "oracle"))
+    import cpu_ref
+    A, b = c1_problem(3)
+    kw = dict(gamma=1.7, k=1, it_tol=1e-10, rcond=1e-12, seed=5)
+    xo, io, _ = O.ls_dense(A, b, **kw)
+    x, info = cpu_ref.dense_solve(A, b, **kw)
+    assert info["rank"] == io.rank and info["flag"] == 0
+    assert abs(info["it"] - io.it) <= 2
+    assert rel(x, xo) < 1e-8
+    As, bs = sparse_problem(4, n=3000, d=80, density=0.03)
+    kws = dict(gamma=1.5, k=3, it_tol=1e-10, seed=1)
+    xo, io, _ = O.ls_sparse(As, bs, **kws)
+    x, info = cpu_ref.sparse_solve(As, bs, **kws)
+    assert info["flag"] == 0 and abs(info["it"] - io.it) <= 2
+    assert rel(x, xo) < 1e-8
