@@ -171,6 +171,39 @@ void skl_set_barrier_spins(long spins);
  * steps must hold max_steps entries; pass NULL to switch it off. */
 void skl_set_history(skl_lsqr_step *steps, long max_steps);
 
+/* ---- HBM-resident problems (handles) ----------------------------------------
+ * No counterpart in the reference, whose calls take A each time; these are the
+ * "keep A resident" entry points for callers that solve on one matrix more
+ * than once or feed the serving loop.  An upload copies A to the calling
+ * thread's device once, together with what every solve would rebuild (dense:
+ * the row-major copy of A for the one-pass LSQR kernel when LSQR would use it
+ * and it fits; CSC: int32 indices and a CSR copy).  Each solve on a handle
+ * computes exactly what the corresponding solver computes on the same A, b
+ * and opts (same sketch for the same seed, same results).  Solves only read
+ * the handle: several may run on it at once, from any threads of the
+ * process that use its device.  skl_release frees it; no solve may still be
+ * running on it. */
+typedef struct skl_handle skl_handle;
+#define SKL_METHOD_DENSE_CPQR 0       /* ls_dense_hashing_blendenpik */
+#define SKL_METHOD_DENSE_NOCPQR 1     /* ls_dense_hashing_blendenpik_noCPQR */
+#define SKL_METHOD_DENSE_BLENDENPIK 2 /* ls_dense_blendenpik */
+#define SKL_METHOD_SPARSE_SPQR 3      /* ls_sparse_spqr (CSC handles only) */
+/* dense A (host or device pointer, n x d, lda >= n) */
+int skl_upload_dense(long n, long d, const double *A, long lda, skl_handle **h);
+int skl_upload_dense_dev(long n, long d, const double *dA, long lda, skl_handle **h);
+/* CSC A (host arrays in the cs_dl layout, checked: colptr monotone from 0 to
+ * nnz, row indices in [0, n)) */
+int skl_upload_csc(long n, long d, long nnz, const long *colptr, const long *rowind,
+                   const double *val, skl_handle **h);
+/* shape, stored entries (-1 for dense) and device bytes held */
+int skl_handle_query(const skl_handle *h, long *n, long *d, long *nnz, size_t *device_bytes);
+/* b (n) and x (d) in host memory / device memory of the handle's device */
+int skl_solve_handle(skl_handle *h, int method, const double *b, double *x, const skl_opts *opts,
+                     skl_info *info);
+int skl_solve_handle_dev(skl_handle *h, int method, const double *db, double *dx,
+                         const skl_opts *opts, skl_info *info);
+int skl_release(skl_handle *h);
+
 /* ---- serving: continuous batching over worker slots -------------------------
  * No counterpart in the reference (it solves one problem per call); this is
  * the request loop that callers with a stream of problems use.  Each slot is
@@ -185,6 +218,17 @@ long skl_server_submit_dense(skl_server *s, int cpqr, long n, long d, const doub
 long skl_server_submit_sparse(skl_server *s, long n, long d, long nnz, const long *colptr,
                               const long *rowind, const double *val, const double *b, double *x,
                               const skl_opts *opts);
+/* device-pointer requests (the server's device); the footprint leaves out
+ * the inputs, which are already resident */
+long skl_server_submit_dense_dev(skl_server *s, int cpqr, long n, long d, const double *dA,
+                                 long lda, const double *db, double *dx, const skl_opts *opts);
+long skl_server_submit_sparse_dev(skl_server *s, long n, long d, long nnz, const long *dcolptr,
+                                  const long *drowind, const double *dval, const double *db,
+                                  double *dx, const skl_opts *opts);
+/* a solve on a handle (b, x host pointers); the handle must stay alive until
+ * skl_server_wait returns for the ticket */
+long skl_server_submit_handle(skl_server *s, skl_handle *h, int method, const double *b,
+                              double *x, const skl_opts *opts);
 /* blocks until the request finished; queue_s / run_s: time queued / solving */
 int skl_server_wait(skl_server *s, long ticket, skl_info *info, double *queue_s, double *run_s);
 int skl_server_pending(skl_server *s);

@@ -770,15 +770,31 @@ struct LsqrPlan {
 
 // the one-pass kernel holds up to 16 column pairs of a row per thread in
 // registers (d <= 8192); small problems keep the two-pass kernels
-static bool fused_wanted(const LsqrProblem &p) {
-    if (p.csr_ptr || p.d > 16 * FR_COLS || p.n < 4096 || ctx().no_alloc) return false;
+bool lsqr_rows_wanted(long n, long d) {
+    if (d > 16 * FR_COLS || n < 4096) return false;
     const char *e = getenv("SKL_LSQR_FUSED");
     return !(e && e[0] == '0');
 }
+static bool fused_wanted(const LsqrProblem &p) {
+    return !p.csr_ptr && !ctx().no_alloc && lsqr_rows_wanted(p.n, p.d);
+}
+
+long lsqr_rows_ld(long d) { return d + (d & 1); }
+
+void lsqr_make_rows(long n, long d, const double *A, long lda, double *AT, hipStream_t s) {
+    const long ldt = lsqr_rows_ld(d);
+    dim3 g((unsigned)((n + 63) / 64), (unsigned)((ldt + 63) / 64));
+    k_to_rows<<<g, 256, 0, s>>>(n, d, A, lda, AT, ldt);
+    SKL_CHECK_LAUNCH();
+}
 
 // the row-major copy doubles A's footprint: only when the device has room for
-// it (free memory plus what the arena holds unused)
+// it (free memory plus what the arena holds unused).  Under a memory
+// allowance (a server slot) the admitted footprint, which counts the copy,
+// decides instead of the device's free memory, which other slots change.
 static bool rows_fit(const LsqrProblem &p, Arena &ar) {
+    if (ctx().mem_allowance > 0)
+        return 8.0 * (double)p.n * (double)lsqr_rows_ld(p.d) <= (double)ctx().mem_allowance;
     size_t fr = 0, tot = 0;
     SKL_HIP(hipMemGetInfo(&fr, &tot));
     const double need = 8.0 * (double)p.n * (double)(p.d + (p.d & 1));
@@ -1155,7 +1171,7 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
             // made on the aux stream (lsqr_start_rows) with the same row stride
             SKL_REQUIRE(p.ldt == q.ldt, SKL_E_ARG, "row-major copy of A has the wrong stride");
             B.AT = const_cast<double *>(p.AT);
-            SKL_HIP(hipStreamWaitEvent(s, p.AT_ready, 0));
+            if (p.AT_ready) SKL_HIP(hipStreamWaitEvent(s, p.AT_ready, 0));
         } else {
             B.AT = ar.alloc_n<double>((size_t)q.n * q.ldt);
             dim3 g((unsigned)((q.n + 63) / 64), (unsigned)((q.ldt + 63) / 64));

@@ -105,6 +105,8 @@ void *Arena::alloc(size_t bytes) {
     return p;
 }
 
+size_t Context::retained_bytes() const { return arena.capacity() + ws_cap + ws_side_cap; }
+
 // ---------------------------------------------------------------------------
 static thread_local Context *t_ctx = nullptr;
 static std::mutex g_init_mu;

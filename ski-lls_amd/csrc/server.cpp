@@ -6,8 +6,20 @@
 // when a slot is free and their device-memory footprint fits the budget;
 // completion frees both.  Every solve reduces in a fixed order on its own
 // buffers, so a result does not depend on what else is running.
+//
+// Memory accounting: a slot holds device memory between requests (its arena
+// keeps the high-water size of the solves it ran, and its split-K scratch),
+// so each slot is charged max(what it retains, the footprint of the request
+// it runs).  A request is admitted by a free slot when the charges of the
+// other slots plus its own charge fit the budget (a request that fits no
+// budget runs when every other slot is idle).  The admitted footprint is
+// also the solve's memory allowance: choices that depend on free memory
+// (the row-major copy for one-pass LSQR) follow it instead of asking the
+// device, whose free memory the other slots change meanwhile.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -27,14 +39,19 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 
+enum Kind { DENSE_CPQR = 0, DENSE_NOCPQR = 1, SPARSE = 2, HANDLE = 3 };
+
 struct Request {
     long ticket = 0;
-    int kind = 0; // 0 dense robust, 1 dense noCPQR, 2 sparse
+    int kind = 0;
+    bool dev = false; // device pointers
     long n = 0, d = 0, nnz = 0;
     const double *A = nullptr;
     long lda = 0;
     const long *colptr = nullptr, *rowind = nullptr;
     const double *val = nullptr;
+    skl_handle *h = nullptr;
+    int method = 0;
     const double *b = nullptr;
     double *x = nullptr;
     skl_opts opts{};
@@ -46,18 +63,29 @@ struct Request {
     bool done = false;
 };
 
-// device bytes one solve needs: inputs + sketch/QR workspace + LSQR buffers
+// Device bytes one solve needs beyond what is already resident: the inputs
+// (host requests), per-solve copies of A (int32 CSC + CSR for sparse A, the
+// row-major copy for one-pass dense LSQR), the sketch and its QR, R and its
+// inverse, the LSQR vectors and partial sums, and the dense transform's
+// output.  A little above the solvers' own high-water marks.
 size_t footprint_of(const Request &r) {
-    double m = std::ceil((double)r.d * r.opts.gamma);
-    double b = 0;
-    if (r.kind == 2)
-        b = 36.0 * r.nnz + 16.0 * r.n + 16.0 * (r.d + 1);
-    else
-        b = 8.0 * r.n * r.d + 16.0 * r.n;
-    b += 8.0 * m * (r.d + 1) * 2;            // sketch + QR copies
-    b += 8.0 * r.d * r.d * 3;                // R, R^{-1}, products
-    b += 8.0 * r.n * 24 + 8.0 * r.d * 64;    // LSQR vectors and partials
-    if (r.kind != 2) b += 8.0 * r.n * (r.d + 1) * 2; // transform scratch
+    const bool sparse = r.kind == SPARSE || (r.kind == HANDLE && r.method == SKL_METHOD_SPARSE_SPQR);
+    const bool handle = r.kind == HANDLE;
+    const double n = (double)r.n, d = (double)r.d, nnz = (double)r.nnz;
+    const double m = std::ceil(d * r.opts.gamma);
+    double b = 16.0 * n + 16.0 * (d + 1); // b, x and the residual copy
+    if (sparse) {
+        if (!r.dev && !handle) b += 16.0 * nnz + 8.0 * (d + 1); // int64 indices + values
+        if (!handle) b += 16.0 * nnz + 4.0 * (n + d + 2);       // int32 CSC + CSR copy
+    } else {
+        if (!r.dev && !handle) b += 8.0 * n * d;
+        b += 8.0 * n * (d + 1) * 2; // transform output and scratch
+        if (!handle && skl::lsqr_rows_wanted(r.n, r.d))
+            b += 8.0 * n * (double)skl::lsqr_rows_ld(r.d);
+    }
+    b += 8.0 * m * (d + 1) * 2;         // sketch + QR copies
+    b += 8.0 * d * d * 3;               // R, R^{-1}, products
+    b += 8.0 * n * 24 + 8.0 * d * 64;   // LSQR vectors and partials
     return (size_t)b + ((size_t)64 << 20);
 }
 
@@ -65,7 +93,10 @@ size_t footprint_of(const Request &r) {
 
 struct skl_server {
     int slots = 1;
-    size_t budget = 0, in_use = 0;
+    size_t budget = 0;
+    // per slot: device bytes retained between requests, footprint of the
+    // request it runs (0 when idle)
+    std::vector<size_t> retained, running;
     int busy = 0;
     bool stop = false;
     long next_ticket = 1;
@@ -77,7 +108,35 @@ struct skl_server {
     int device = 0;
 };
 
-static void worker_main(skl_server *S) {
+// device bytes charged to the slots other than `self`
+static size_t charged_except(const skl_server *S, int self) {
+    size_t t = 0;
+    for (int i = 0; i < S->slots; ++i)
+        if (i != self) t += std::max(S->retained[(size_t)i], S->running[(size_t)i]);
+    return t;
+}
+
+static int run_request(Request *r) {
+    switch (r->kind) {
+    case DENSE_CPQR:
+        return r->dev ? skl_dense_solve_dev(r->n, r->d, r->A, r->lda, r->b, r->x, &r->opts, &r->info)
+                      : skl_dense_solve(r->n, r->d, r->A, r->lda, r->b, r->x, &r->opts, &r->info);
+    case DENSE_NOCPQR:
+        return r->dev ? skl_dense_solve_nocpqr_dev(r->n, r->d, r->A, r->lda, r->b, r->x, &r->opts,
+                                                   &r->info)
+                      : skl_dense_solve_nocpqr(r->n, r->d, r->A, r->lda, r->b, r->x, &r->opts,
+                                               &r->info);
+    case SPARSE:
+        return r->dev ? skl_sparse_solve_dev(r->n, r->d, r->nnz, r->colptr, r->rowind, r->val, r->b,
+                                             r->x, &r->opts, &r->info)
+                      : skl_sparse_solve(r->n, r->d, r->nnz, r->colptr, r->rowind, r->val, r->b,
+                                         r->x, &r->opts, &r->info);
+    default:
+        return skl_solve_handle(r->h, r->method, r->b, r->x, &r->opts, &r->info);
+    }
+}
+
+static void worker_main(skl_server *S, int self) {
     skl_init(S->device);
     while (true) {
         Request *r = nullptr;
@@ -89,33 +148,26 @@ static void worker_main(skl_server *S) {
                 Request *h = S->queue.front();
                 // admit in arrival order when the footprint fits (an oversized
                 // request runs alone)
-                return S->in_use + h->footprint <= S->budget || S->busy == 0;
+                const size_t mine = std::max(S->retained[(size_t)self], h->footprint);
+                return charged_except(S, self) + mine <= S->budget || S->busy == 0;
             });
             if (S->stop && S->queue.empty()) break;
             if (S->queue.empty()) continue;
             r = S->queue.front();
             S->queue.pop_front();
-            S->in_use += r->footprint;
+            S->running[(size_t)self] = r->footprint;
             S->busy++;
         }
         r->started = clk::now();
-        switch (r->kind) {
-        case 0:
-            r->rc = skl_dense_solve(r->n, r->d, r->A, r->lda, r->b, r->x, &r->opts, &r->info);
-            break;
-        case 1:
-            r->rc = skl_dense_solve_nocpqr(r->n, r->d, r->A, r->lda, r->b, r->x, &r->opts, &r->info);
-            break;
-        default:
-            r->rc = skl_sparse_solve(r->n, r->d, r->nnz, r->colptr, r->rowind, r->val, r->b, r->x,
-                                     &r->opts, &r->info);
-            break;
-        }
+        skl::ctx().mem_allowance = r->footprint;
+        r->rc = run_request(r);
+        skl::ctx().mem_allowance = 0;
         r->finished = clk::now();
         if (r->rc != SKL_OK) r->err = skl_last_error();
         {
             std::lock_guard<std::mutex> lk(S->mu);
-            S->in_use -= r->footprint;
+            S->retained[(size_t)self] = skl::ctx().retained_bytes();
+            S->running[(size_t)self] = 0;
             S->busy--;
             r->done = true;
         }
@@ -155,15 +207,17 @@ skl_server *skl_server_create(int slots, size_t mem_budget_bytes, int device) {
             mem_budget_bytes = (size_t)64 << 30;
     }
     S->budget = mem_budget_bytes;
-    for (int i = 0; i < slots; ++i) S->workers.emplace_back(worker_main, S);
+    S->retained.assign((size_t)slots, 0);
+    S->running.assign((size_t)slots, 0);
+    for (int i = 0; i < slots; ++i) S->workers.emplace_back(worker_main, S, i);
     return S;
 }
 
-long skl_server_submit_dense(skl_server *S, int cpqr, long n, long d, const double *A, long lda,
-                             const double *b, double *x, const skl_opts *opts) {
-    if (!S || !A || !b || !x) return -SKL_E_ARG;
+static Request *dense_request(int cpqr, bool dev, long n, long d, const double *A, long lda,
+                              const double *b, double *x, const skl_opts *opts) {
     Request *r = new Request();
-    r->kind = cpqr ? 0 : 1;
+    r->kind = cpqr ? DENSE_CPQR : DENSE_NOCPQR;
+    r->dev = dev;
     r->n = n;
     r->d = d;
     r->A = A;
@@ -174,21 +228,68 @@ long skl_server_submit_dense(skl_server *S, int cpqr, long n, long d, const doub
         r->opts = *opts;
     else
         skl_default_opts(&r->opts);
-    return submit(S, r);
+    return r;
 }
 
-long skl_server_submit_sparse(skl_server *S, long n, long d, long nnz, const long *colptr,
-                              const long *rowind, const double *val, const double *b, double *x,
-                              const skl_opts *opts) {
-    if (!S || !colptr || !b || !x) return -SKL_E_ARG;
+static Request *sparse_request(bool dev, long n, long d, long nnz, const long *colptr,
+                               const long *rowind, const double *val, const double *b, double *x,
+                               const skl_opts *opts) {
     Request *r = new Request();
-    r->kind = 2;
+    r->kind = SPARSE;
+    r->dev = dev;
     r->n = n;
     r->d = d;
     r->nnz = nnz;
     r->colptr = colptr;
     r->rowind = rowind;
     r->val = val;
+    r->b = b;
+    r->x = x;
+    if (opts)
+        r->opts = *opts;
+    else
+        skl_default_opts(&r->opts);
+    return r;
+}
+
+long skl_server_submit_dense(skl_server *S, int cpqr, long n, long d, const double *A, long lda,
+                             const double *b, double *x, const skl_opts *opts) {
+    if (!S || !A || !b || !x) return -SKL_E_ARG;
+    return submit(S, dense_request(cpqr, false, n, d, A, lda, b, x, opts));
+}
+
+long skl_server_submit_sparse(skl_server *S, long n, long d, long nnz, const long *colptr,
+                              const long *rowind, const double *val, const double *b, double *x,
+                              const skl_opts *opts) {
+    if (!S || !colptr || !b || !x) return -SKL_E_ARG;
+    return submit(S, sparse_request(false, n, d, nnz, colptr, rowind, val, b, x, opts));
+}
+
+long skl_server_submit_dense_dev(skl_server *S, int cpqr, long n, long d, const double *dA,
+                                 long lda, const double *db, double *dx, const skl_opts *opts) {
+    if (!S || !dA || !db || !dx) return -SKL_E_ARG;
+    return submit(S, dense_request(cpqr, true, n, d, dA, lda, db, dx, opts));
+}
+
+long skl_server_submit_sparse_dev(skl_server *S, long n, long d, long nnz, const long *dcolptr,
+                                  const long *drowind, const double *dval, const double *db,
+                                  double *dx, const skl_opts *opts) {
+    if (!S || !dcolptr || !db || !dx) return -SKL_E_ARG;
+    return submit(S, sparse_request(true, n, d, nnz, dcolptr, drowind, dval, db, dx, opts));
+}
+
+long skl_server_submit_handle(skl_server *S, skl_handle *h, int method, const double *b,
+                              double *x, const skl_opts *opts) {
+    if (!S || !h || !b || !x) return -SKL_E_ARG;
+    long n = 0, d = 0, nnz = 0;
+    skl_handle_query(h, &n, &d, &nnz, nullptr);
+    Request *r = new Request();
+    r->kind = HANDLE;
+    r->h = h;
+    r->method = method;
+    r->n = n;
+    r->d = d;
+    r->nnz = nnz < 0 ? 0 : nnz;
     r->b = b;
     r->x = x;
     if (opts)

@@ -105,6 +105,12 @@ struct Context {
     bool no_alloc = false;
     Arena fixed;
     Arena &cur_arena() { return no_alloc ? fixed : arena; }
+    // device bytes the current solve was admitted with (a server slot; 0:
+    // no allowance, decisions that depend on free memory ask the device)
+    size_t mem_allowance = 0;
+    // device bytes this context keeps between solves (arena reservation and
+    // split-K scratch)
+    size_t retained_bytes() const;
     int num_cu = 0;
     int coop_ok = 0;
     size_t lds_max = 0;
@@ -299,6 +305,12 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
 // stream, and sets p.AT / p.ldt / p.AT_ready; false (p unchanged) when the
 // one-pass kernel will not be used.  The copy lives in the arena.
 bool lsqr_start_rows(LsqrProblem &p, Arena &ar);
+// whether LSQR on a dense n x d A runs its one-pass kernel over a row-major
+// copy (by shape; no device query); the copy's row stride; the copy itself (AT: n x lsqr_rows_ld(d)),
+// queued on stream s
+bool lsqr_rows_wanted(long n, long d);
+long lsqr_rows_ld(long d);
+void lsqr_make_rows(long n, long d, const double *A, long lda, double *AT, hipStream_t s);
 
 // small vector helpers
 void dev_gemv_n_dense(long n, long d, const double *A, long lda, const double *x, double alpha,

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <ctime>
 #include <map>
+#include <memory>
 #include <mutex>
 
 #include "k_common.h"
@@ -536,8 +537,11 @@ static int refine_sketched_solution(long m, long d, const double *SA, long ldsa,
 // otherwise LSQR from zero on the Householder-QR preconditioner, after the
 // hashing sketch (ls_dense_hashing_blendenpik_noCPQR) or, with subsample,
 // after uniform row sampling of the transform (ls_dense_blendenpik).
+// AT_pre: a row-major copy of A (lsqr_make_rows) that outlives the call, or
+// null (the copy is made per solve when the one-pass LSQR kernel is used)
 static int dense_impl(long n, long d, const double *dA, long lda, const double *db, double *dx,
-                      const skl_opts &o, skl_info &info, bool cpqr, bool subsample = false) {
+                      const skl_opts &o, skl_info &info, bool cpqr, bool subsample = false,
+                      const double *AT_pre = nullptr) {
     Context &C = ctx();
     Arena &ar = C.cur_arena();
     hipStream_t s = C.stream;
@@ -589,7 +593,13 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
     pre.d = d;
     pre.A = dA;
     pre.lda = lda;
-    auto start_rows = [&]() { lsqr_start_rows(pre, ar); };
+    if (AT_pre) {
+        pre.AT = AT_pre;
+        pre.ldt = lsqr_rows_ld(d);
+    }
+    auto start_rows = [&]() {
+        if (!pre.AT) lsqr_start_rows(pre, ar);
+    };
     auto finish_qr = [&](auto *W) {
         // diagonal -> rank (first |R_ii| <= rcond), R to double, c
         std::vector<double> dg((size_t)d);
@@ -837,9 +847,17 @@ static long sparse_factor(long n, long d, long m, long k, const DevPattern &dp, 
     return rank;
 }
 
+// The int32 CSC indices and the CSR copy of a sparse A, when they outlive the
+// call (a handle); otherwise sparse_impl builds them per solve.
+struct SparsePrep {
+    const int32_t *cp = nullptr, *ri = nullptr; // CSC column pointers / row indices
+    const int32_t *rp = nullptr, *cc = nullptr; // CSR row pointers / column indices
+    const double *cv = nullptr;                 // CSR values
+};
+
 static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dri,
                        const double *dval, const double *db, double *dx, const skl_opts &o,
-                       skl_info &info) {
+                       skl_info &info, const SparsePrep *prep = nullptr) {
     Context &C = ctx();
     Arena &ar = C.arena;
     hipStream_t s = C.stream;
@@ -853,14 +871,29 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
     ar.reset();
     ArenaTrim trim(ar);
     double T0 = wall();
-    int32_t *cp = ar.alloc_n<int32_t>((size_t)d + 1);
-    int32_t *ri = ar.alloc_n<int32_t>((size_t)std::max<long>(nnz, 1));
-    narrow_indices(d + 1, dcp, cp);
-    narrow_indices(nnz, dri, ri);
-    int32_t *rp = ar.alloc_n<int32_t>((size_t)n + 1);
-    int32_t *cc = ar.alloc_n<int32_t>((size_t)std::max<long>(nnz, 1));
-    double *cv = ar.alloc_n<double>((size_t)std::max<long>(nnz, 1));
-    csc_to_csr(n, d, nnz, cp, ri, dval, rp, cc, cv, ar);
+    const int32_t *cp, *ri, *rp, *cc;
+    const double *cv;
+    if (prep) {
+        cp = prep->cp;
+        ri = prep->ri;
+        rp = prep->rp;
+        cc = prep->cc;
+        cv = prep->cv;
+    } else {
+        int32_t *cp_ = ar.alloc_n<int32_t>((size_t)d + 1);
+        int32_t *ri_ = ar.alloc_n<int32_t>((size_t)std::max<long>(nnz, 1));
+        narrow_indices(d + 1, dcp, cp_);
+        narrow_indices(nnz, dri, ri_);
+        int32_t *rp_ = ar.alloc_n<int32_t>((size_t)n + 1);
+        int32_t *cc_ = ar.alloc_n<int32_t>((size_t)std::max<long>(nnz, 1));
+        double *cv_ = ar.alloc_n<double>((size_t)std::max<long>(nnz, 1));
+        csc_to_csr(n, d, nnz, cp_, ri_, dval, rp_, cc_, cv_, ar);
+        cp = cp_;
+        ri = ri_;
+        rp = rp_;
+        cc = cc_;
+        cv = cv_;
+    }
     // pattern: no srand on the sparse path (sketching.cpp:100); seed = the
     // state of the process-wide generator, 1 for a fresh process
     Glibc g;
@@ -1353,6 +1386,206 @@ int skl_sparse_solve(long n, long d, long nnz, const long *colptr, const long *r
         SKL_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * d, hipMemcpyDeviceToHost, s));
         SKL_HIP(hipStreamSynchronize(s));
         return rc;
+    });
+}
+
+// ---- HBM-resident problems --------------------------------------------------
+// A handle owns a device copy of A together with what every solve on it would
+// otherwise rebuild: for dense A the row-major copy read by the one-pass LSQR
+// kernel, for CSC A the int32 indices and the CSR copy.  Solves only read it,
+// so any number may run on one handle at once (from server slots or threads).
+struct skl_handle {
+    int device = 0;
+    int sparse = 0;
+    long n = 0, d = 0, nnz = 0;
+    double *A = nullptr;  // dense, n x d, ld n
+    double *AT = nullptr; // dense, row-major copy (ld lsqr_rows_ld(d)) or null
+    int32_t *cp = nullptr, *ri = nullptr, *rp = nullptr, *cc = nullptr;
+    double *val = nullptr, *cv = nullptr;
+    size_t bytes = 0;
+    std::vector<void *> blocks;
+    // a device block of `sz` bytes, freed with the handle
+    void *take(size_t sz) {
+        void *p = nullptr;
+        sz = std::max<size_t>(sz, 8);
+        {
+            AllocLock lk(capture_mutex());
+            SKL_HIP(hipMalloc(&p, sz));
+        }
+        blocks.push_back(p);
+        bytes += sz;
+        return p;
+    }
+    ~skl_handle() {
+        if (blocks.empty()) return;
+        AllocLock lk(capture_mutex());
+        for (void *p : blocks) (void)hipFree(p);
+    }
+};
+
+namespace skl {
+// the dense handle's row-major copy, when LSQR would use it and it fits
+static void handle_rows(skl_handle *h) {
+    if (ctx().no_alloc || !lsqr_rows_wanted(h->n, h->d)) return;
+    size_t fr = 0, tot = 0;
+    SKL_HIP(hipMemGetInfo(&fr, &tot));
+    const double need = 8.0 * (double)h->n * (double)lsqr_rows_ld(h->d);
+    if (need * 1.25 + 256.0 * (1 << 20) > (double)fr) return;
+    h->AT = (double *)h->take(sizeof(double) * (size_t)h->n * lsqr_rows_ld(h->d));
+    lsqr_make_rows(h->n, h->d, h->A, h->n, h->AT, ctx().stream);
+}
+
+static int upload_dense(long n, long d, const double *A, long lda, skl_handle **out,
+                        hipMemcpyKind kind) {
+    return guarded([&] {
+        SKL_REQUIRE(out && A && n > 0 && d > 0 && lda >= n, SKL_E_ARG, "invalid arguments");
+        *out = nullptr;
+        if (n < d) throw Error(SKL_E_SHAPE, "Matrix is not overdetermined.");
+        ensure_init();
+        hipStream_t s = ctx().stream;
+        std::unique_ptr<skl_handle> h(new skl_handle());
+        h->device = ctx().device;
+        h->n = n;
+        h->d = d;
+        h->A = (double *)h->take(sizeof(double) * (size_t)n * d);
+        SKL_HIP(hipMemcpy2DAsync(h->A, sizeof(double) * n, A, sizeof(double) * lda,
+                                 sizeof(double) * n, d, kind, s));
+        handle_rows(h.get());
+        SKL_HIP(hipStreamSynchronize(s));
+        *out = h.release();
+        return SKL_OK;
+    });
+}
+
+static int solve_on_handle(skl_handle *h, int method, const double *db, double *dx,
+                           const skl_opts &o, skl_info &info) {
+    SKL_REQUIRE(h->device == ctx().device, SKL_E_ARG,
+                "the handle lives on device " + std::to_string(h->device) +
+                    ", the calling thread uses device " + std::to_string(ctx().device));
+    if (h->sparse) {
+        SKL_REQUIRE(method == SKL_METHOD_SPARSE_SPQR, SKL_E_ARG,
+                    "a CSC handle takes SKL_METHOD_SPARSE_SPQR");
+        SparsePrep sp;
+        sp.cp = h->cp;
+        sp.ri = h->ri;
+        sp.rp = h->rp;
+        sp.cc = h->cc;
+        sp.cv = h->cv;
+        return sparse_impl(h->n, h->d, h->nnz, nullptr, nullptr, h->val, db, dx, o, info, &sp);
+    }
+    switch (method) {
+    case SKL_METHOD_DENSE_CPQR:
+        return dense_impl(h->n, h->d, h->A, h->n, db, dx, o, info, true, false, h->AT);
+    case SKL_METHOD_DENSE_NOCPQR:
+        return dense_impl(h->n, h->d, h->A, h->n, db, dx, o, info, false, false, h->AT);
+    case SKL_METHOD_DENSE_BLENDENPIK:
+        return dense_impl(h->n, h->d, h->A, h->n, db, dx, o, info, false, true, h->AT);
+    default:
+        throw Error(SKL_E_ARG, "a dense handle takes SKL_METHOD_DENSE_CPQR, _NOCPQR or _BLENDENPIK");
+    }
+}
+} // namespace skl
+
+int skl_upload_dense(long n, long d, const double *A, long lda, skl_handle **h) {
+    return upload_dense(n, d, A, lda, h, hipMemcpyHostToDevice);
+}
+
+int skl_upload_dense_dev(long n, long d, const double *dA, long lda, skl_handle **h) {
+    return upload_dense(n, d, dA, lda, h, hipMemcpyDeviceToDevice);
+}
+
+int skl_upload_csc(long n, long d, long nnz, const long *colptr, const long *rowind,
+                   const double *val, skl_handle **out) {
+    return guarded([&] {
+        SKL_REQUIRE(out && colptr && n > 0 && d > 0 && nnz >= 0, SKL_E_ARG, "invalid arguments");
+        *out = nullptr;
+        if (n < d) throw Error(SKL_E_SHAPE, "Matrix is not overdetermined.");
+        SKL_REQUIRE(nnz == 0 || (rowind && val), SKL_E_ARG, "invalid arguments");
+        SKL_REQUIRE(nnz < 2147483647L && n < 2147483647L, SKL_E_UNSUPPORTED,
+                    "nnz must fit in int32");
+        SKL_REQUIRE(colptr[0] == 0 && colptr[d] == nnz, SKL_E_ARG, "colptr inconsistent with nnz");
+        for (long j = 0; j < d; ++j)
+            SKL_REQUIRE(colptr[j] <= colptr[j + 1], SKL_E_ARG, "colptr not monotone");
+        for (long e = 0; e < nnz; ++e)
+            SKL_REQUIRE(rowind[e] >= 0 && rowind[e] < n, SKL_E_ARG, "row index out of range");
+        ensure_init();
+        Context &C = ctx();
+        hipStream_t s = C.stream;
+        std::unique_ptr<skl_handle> h(new skl_handle());
+        h->device = C.device;
+        h->sparse = 1;
+        h->n = n;
+        h->d = d;
+        h->nnz = nnz;
+        std::vector<int32_t> cp32((size_t)d + 1), ri32((size_t)std::max<long>(nnz, 1));
+        for (long j = 0; j <= d; ++j) cp32[(size_t)j] = (int32_t)colptr[j];
+        for (long e = 0; e < nnz; ++e) ri32[(size_t)e] = (int32_t)rowind[e];
+        h->cp = (int32_t *)h->take(sizeof(int32_t) * ((size_t)d + 1));
+        h->ri = (int32_t *)h->take(sizeof(int32_t) * (size_t)std::max<long>(nnz, 1));
+        h->val = (double *)h->take(sizeof(double) * (size_t)std::max<long>(nnz, 1));
+        h->rp = (int32_t *)h->take(sizeof(int32_t) * ((size_t)n + 1));
+        h->cc = (int32_t *)h->take(sizeof(int32_t) * (size_t)std::max<long>(nnz, 1));
+        h->cv = (double *)h->take(sizeof(double) * (size_t)std::max<long>(nnz, 1));
+        SKL_HIP(hipMemcpyAsync(h->cp, cp32.data(), sizeof(int32_t) * (d + 1),
+                               hipMemcpyHostToDevice, s));
+        if (nnz > 0) {
+            SKL_HIP(hipMemcpyAsync(h->ri, ri32.data(), sizeof(int32_t) * nnz,
+                                   hipMemcpyHostToDevice, s));
+            SKL_HIP(hipMemcpyAsync(h->val, val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+        }
+        Arena &ar = C.arena;
+        ar.reset();
+        ArenaTrim trim(ar);
+        csc_to_csr(n, d, nnz, h->cp, h->ri, h->val, h->rp, h->cc, h->cv, ar);
+        SKL_HIP(hipStreamSynchronize(s));
+        *out = h.release();
+        return SKL_OK;
+    });
+}
+
+int skl_handle_query(const skl_handle *h, long *n, long *d, long *nnz, size_t *device_bytes) {
+    if (!h) return SKL_E_ARG;
+    if (n) *n = h->n;
+    if (d) *d = h->d;
+    if (nnz) *nnz = h->sparse ? h->nnz : -1;
+    if (device_bytes) *device_bytes = h->bytes;
+    return SKL_OK;
+}
+
+int skl_solve_handle_dev(skl_handle *h, int method, const double *db, double *dx,
+                         const skl_opts *opts, skl_info *info) {
+    skl_info tmp;
+    return guarded([&] {
+        SKL_REQUIRE(h && db && dx, SKL_E_ARG, "invalid arguments");
+        skl_opts o = resolve(opts);
+        ensure_init();
+        return solve_on_handle(h, method, db, dx, o, info ? *info : tmp);
+    });
+}
+
+int skl_solve_handle(skl_handle *h, int method, const double *b, double *x, const skl_opts *opts,
+                     skl_info *info) {
+    skl_info tmp;
+    return guarded([&] {
+        SKL_REQUIRE(h && b && x, SKL_E_ARG, "invalid arguments");
+        skl_opts o = resolve(opts);
+        ensure_init();
+        hipStream_t s = ctx().stream;
+        DevBuf db(sizeof(double) * h->n), dx(sizeof(double) * h->d);
+        SKL_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * h->n, hipMemcpyHostToDevice, s));
+        int rc = solve_on_handle(h, method, db.as<double>(), dx.as<double>(), o,
+                                 info ? *info : tmp);
+        SKL_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * h->d, hipMemcpyDeviceToHost, s));
+        SKL_HIP(hipStreamSynchronize(s));
+        return rc;
+    });
+}
+
+int skl_release(skl_handle *h) {
+    if (!h) return SKL_E_ARG;
+    return guarded([&] {
+        delete h;
+        return SKL_OK;
     });
 }
 

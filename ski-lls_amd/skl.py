@@ -70,9 +70,15 @@ EXPORTS = [
     "skl_sparse_sketch", "skl_qr", "skl_cpqr", "skl_trinv", "skl_lsqr_dense",
     "skl_profile_enable", "skl_profile_reset", "skl_profile_read", "skl_set_barrier_spins",
     "skl_dense_solve_dev_noalloc",
+    "skl_upload_dense", "skl_upload_dense_dev", "skl_upload_csc", "skl_handle_query",
+    "skl_solve_handle", "skl_solve_handle_dev", "skl_release",
     "skl_server_create", "skl_server_submit_dense", "skl_server_submit_sparse",
+    "skl_server_submit_dense_dev", "skl_server_submit_sparse_dev", "skl_server_submit_handle",
     "skl_server_wait", "skl_server_pending", "skl_server_destroy",
 ]
+
+# solver of a handle solve (skl.h SKL_METHOD_*)
+METHOD_DENSE_CPQR, METHOD_DENSE_NOCPQR, METHOD_DENSE_BLENDENPIK, METHOD_SPARSE_SPQR = 0, 1, 2, 3
 
 _lib = None
 
@@ -273,6 +279,78 @@ def ls_sparse_spqr(A, b, gamma, k, abs_tol=ABS_TOL, ordering=SPQR_ORDERING, it_t
     return x, info.rank, info.flag, info.it
 
 
+# ---- device-resident problems -----------------------------------------------------
+class Handle:
+    """A problem kept in device memory (skl_upload_dense / skl_upload_csc):
+    A is copied once; solve() takes b and returns x like dense_solve /
+    sparse_solve on the same A.  Use as a context manager or call release()."""
+
+    def __init__(self, A=None, Acsc=None, dev=None):
+        L = lib()
+        h = C.c_void_p()
+        if Acsc is not None:
+            n, d = Acsc.shape
+            Ap = np.ascontiguousarray(Acsc.indptr, dtype=np.int64)
+            Ai = np.ascontiguousarray(Acsc.indices, dtype=np.int64)
+            Ax = _c(Acsc.data)
+            rc = L.skl_upload_csc(C.c_long(n), C.c_long(d), C.c_long(Ap[-1]), _l(Ap), _l(Ai),
+                                  _d(Ax), C.byref(h))
+            self.sparse = True
+        elif dev is not None:
+            n, d, ptr, lda = dev
+            rc = L.skl_upload_dense_dev(C.c_long(n), C.c_long(d), C.c_void_p(ptr), C.c_long(lda),
+                                        C.byref(h))
+            self.sparse = False
+        else:
+            A = _f(A)
+            n, d = A.shape
+            rc = L.skl_upload_dense(C.c_long(n), C.c_long(d), _d(A), C.c_long(n), C.byref(h))
+            self.sparse = False
+        _check(rc)
+        self._h = h
+        self.n, self.d = n, d
+
+    def query(self):
+        """-> (n, d, nnz or -1, device bytes held)"""
+        n, d, nnz, by = C.c_long(), C.c_long(), C.c_long(), C.c_size_t()
+        _check(lib().skl_handle_query(self._h, C.byref(n), C.byref(d), C.byref(nnz), C.byref(by)))
+        return n.value, d.value, nnz.value, by.value
+
+    def default_method(self, cpqr=True):
+        if self.sparse:
+            return METHOD_SPARSE_SPQR
+        return METHOD_DENSE_CPQR if cpqr else METHOD_DENSE_NOCPQR
+
+    def solve(self, b, method=None, **opts):
+        b = _c(b)
+        x = np.zeros(self.d)
+        o = default_opts(**opts)
+        info = Info()
+        m = self.default_method() if method is None else method
+        _check(lib().skl_solve_handle(self._h, C.c_int(m), _d(b), _d(x), C.byref(o),
+                                      C.byref(info)))
+        return x, info
+
+    def solve_dev(self, b_ptr, x_ptr, method=None, **opts):
+        o = default_opts(**opts)
+        info = Info()
+        m = self.default_method() if method is None else method
+        _check(lib().skl_solve_handle_dev(self._h, C.c_int(m), C.c_void_p(b_ptr), C.c_void_p(x_ptr),
+                                          C.byref(o), C.byref(info)))
+        return info
+
+    def release(self):
+        if self._h:
+            _check(lib().skl_release(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.release()
+
+
 # ---- serving ----------------------------------------------------------------------
 class Server:
     """Continuous-batching request loop (skl_server_*).  Submitted arrays are
@@ -283,6 +361,9 @@ class Server:
         L.skl_server_create.restype = C.c_void_p
         L.skl_server_submit_dense.restype = C.c_long
         L.skl_server_submit_sparse.restype = C.c_long
+        L.skl_server_submit_handle.restype = C.c_long
+        L.skl_server_submit_dense_dev.restype = C.c_long
+        L.skl_server_submit_sparse_dev.restype = C.c_long
         self._h = C.c_void_p(L.skl_server_create(C.c_int(slots), C.c_size_t(mem_budget_bytes),
                                                  C.c_int(device)))
         self._keep = {}
@@ -314,6 +395,43 @@ class Server:
         if t < 0:
             _check(-t)
         self._keep[t] = (Ap, Ai, Ax, b, x, o)
+        return t
+
+    def submit_handle(self, handle, b, method=None, **opts):
+        """A solve on a Handle (its A stays resident); the Handle must outlive
+        the wait."""
+        b = _c(b)
+        x = np.zeros(handle.d)
+        o = default_opts(**opts)
+        m = handle.default_method() if method is None else method
+        t = lib().skl_server_submit_handle(self._h, handle._h, C.c_int(m), _d(b), _d(x),
+                                           C.byref(o))
+        if t < 0:
+            _check(-t)
+        self._keep[t] = (handle, b, x, o)
+        return t
+
+    def submit_dense_dev(self, n, d, A_ptr, lda, b_ptr, x_ptr, cpqr=True, **opts):
+        """Device pointers (on the server's device); wait() returns x = None."""
+        o = default_opts(**opts)
+        t = lib().skl_server_submit_dense_dev(self._h, C.c_int(1 if cpqr else 0), C.c_long(n),
+                                              C.c_long(d), C.c_void_p(A_ptr), C.c_long(lda),
+                                              C.c_void_p(b_ptr), C.c_void_p(x_ptr), C.byref(o))
+        if t < 0:
+            _check(-t)
+        self._keep[t] = (o, None, o)
+        return t
+
+    def submit_sparse_dev(self, n, d, nnz, cp_ptr, ri_ptr, val_ptr, b_ptr, x_ptr, **opts):
+        """Device pointers (int64 CSC arrays, on the server's device)."""
+        o = default_opts(**opts)
+        t = lib().skl_server_submit_sparse_dev(self._h, C.c_long(n), C.c_long(d), C.c_long(nnz),
+                                               C.c_void_p(cp_ptr), C.c_void_p(ri_ptr),
+                                               C.c_void_p(val_ptr), C.c_void_p(b_ptr),
+                                               C.c_void_p(x_ptr), C.byref(o))
+        if t < 0:
+            _check(-t)
+        self._keep[t] = (o, None, o)
         return t
 
     def wait(self, ticket):
