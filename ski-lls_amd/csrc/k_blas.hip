@@ -201,10 +201,14 @@ void gemm(char ta, char tb, long M, long N, long K, T alpha, const T *A, long ld
     // Few output tiles over a long K (V^T C, G V, V^T V in the QR): split K so
     // the launch fills the chip; partials go to the context's split-K scratch
     // (stream-ordered, so the next GEMM may reuse it).
+    // Target: about four workgroups per CU (a single workgroup per CU left
+    // each one a long chain of dependent tile loads: V^T C at 6300 x 3456
+    // read C at 1.3 TB/s), and no chunk shorter than 128 (eight k-steps).
     int S = 1;
     long kchunk = 0;
-    if (batch == 1 && tm * tn < 128 && K >= 512 && !ctx().no_alloc) {
-        long want = std::min<long>((256 + tm * tn - 1) / (tm * tn), K / 256);
+    if (batch == 1 && tm * tn < 4L * ctx().num_cu && K >= 256 && !ctx().no_alloc) {
+        const long wgs = 4L * ctx().num_cu;
+        long want = std::min<long>((wgs + tm * tn - 1) / (tm * tn), K / 128);
         long cap = (64L << 20) / ((long)sizeof(T) * M * N);
         S = (int)std::max<long>(1, std::min<long>({want, cap, 64}));
         if (S > 1) {

@@ -105,10 +105,10 @@ __device__ void tri_inv_upper_lds(const double *Rs, double *X, double *Y) {
 // failure.
 // Thread t keeps entries (p, g + 4 m), m < 16, of the Gram matrix in
 // registers, p = t & 63 its row and g = t >> 6; rows and columns stay where
-// they are (p is the panel column; wave 0 keeps each column's logical
-// position, partial norms and Schur diagonal).  A step: wave 0 picks the
-// pivot, the pivot row's owners publish it in LDS (double buffered), every
-// thread updates its entries -- two barriers.
+// they are (p is the panel column; lane p of every wave keeps that column's
+// logical position, partial norms and Schur diagonal, alike in all waves).  A
+// step: every wave picks the pivot, the pivot row's owners publish it in LDS
+// (double buffered), every thread updates its entries -- one barrier.
 template <bool PIV>
 __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double *G, int ldg,
                                                   double *R, double *Z, int32_t *perm,
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
     __shared__ double Ys[64 * CQ_LD];
     __shared__ double rowbuf[2][64];
     __shared__ double s_g0[64];
-    __shared__ int s_pk[2], s_perm[64];
+    __shared__ int s_perm[64];
     const int tid = threadIdx.x, p = tid & 63, g = tid >> 6;
     if (pass == 2 && *flag != 0) return;
     double sv[16];
@@ -129,57 +129,50 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
         sv[m] = (p < b && q < b) ? G[p + (long)q * ldg] : 0.0;
     }
     for (int e = tid; e < 64 * CQ_LD; e += CQ_T) Rb[e] = 0.0;
-    // wave 0: per column p its Gram diagonal, partial norms and position
-    double dg = 0.0, vn1 = 0.0, vn2 = 0.0;
+    // every wave keeps, for column p in lane p, its Gram diagonal, partial
+    // norms and logical position (the same values in every wave), so each wave
+    // finds the pivot itself: one barrier per step
+    double dg = p < b ? G[p + (long)p * ldg] : 0.0;
+    double vn1 = sqrt(fmax(dg, 0.0)), vn2 = vn1;
     int pos = p < b ? p : 0x7fffffff;
-    if (g == 0) {
-        dg = p < b ? G[p + (long)p * ldg] : 0.0;
-        vn1 = sqrt(fmax(dg, 0.0));
-        vn2 = vn1;
-        s_g0[p] = dg;
-    }
+    if (g == 0) s_g0[p] = dg;
     unsigned long long act = b >= 64 ? ~0ull : ((1ull << b) - 1ull);
     const double tol3z = sqrt(1.1102230246251565e-16);
     __syncthreads();
     int k = 0;
     for (; k < b; ++k) {
         const int buf = k & 1;
-        if (g == 0) {
-            int pk = k;
-            if (PIV) {
-                // largest partial norm among the columns left, first logical
-                // position on ties
-                double bv = -1.0;
-                int bpos = 0x7fffffff, bl = -1;
-                if (((act >> p) & 1ull) && vn1 == vn1) {
-                    bv = fabs(vn1);
-                    bpos = pos;
-                    bl = p;
-                }
-#pragma unroll
-                for (int o = 32; o > 0; o >>= 1) {
-                    const double ov = __shfl_xor(bv, o, 64);
-                    const int opos = __shfl_xor(bpos, o, 64), ol = __shfl_xor(bl, o, 64);
-                    if (ov > bv || (ov == bv && opos < bpos)) {
-                        bv = ov;
-                        bpos = opos;
-                        bl = ol;
-                    }
-                }
-                if (bl < 0) bl = __builtin_ctzll(__ballot(pos == k)); // no finite candidate
-                pk = bl;
-                const int lp = __shfl(pos, pk, 64); // logical position of the pivot
-                const bool atk = pos == k;
-                if (p == pk)
-                    pos = k;
-                else if (atk)
-                    pos = lp;
-                if (p == 0 && swp) swp[k] = lp;
+        int pk = k;
+        if (PIV) {
+            // largest partial norm among the columns left, first logical
+            // position on ties
+            double bv = -1.0;
+            int bpos = 0x7fffffff, bl = -1;
+            if (((act >> p) & 1ull) && vn1 == vn1) {
+                bv = fabs(vn1);
+                bpos = pos;
+                bl = p;
             }
-            if (p == 0) s_pk[buf] = pk;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ov = __shfl_xor(bv, o, 64);
+                const int opos = __shfl_xor(bpos, o, 64), ol = __shfl_xor(bl, o, 64);
+                if (ov > bv || (ov == bv && opos < bpos)) {
+                    bv = ov;
+                    bpos = opos;
+                    bl = ol;
+                }
+            }
+            if (bl < 0) bl = __builtin_ctzll(__ballot(pos == k)); // no finite candidate
+            pk = bl;
+            const int lp = __shfl(pos, pk, 64); // logical position of the pivot
+            const bool atk = pos == k;
+            if (p == pk)
+                pos = k;
+            else if (atk)
+                pos = lp;
+            if (tid == 0 && swp) swp[k] = lp;
         }
-        __syncthreads();
-        const int pk = s_pk[buf];
         if (p == pk) {
 #pragma unroll
             for (int m = 0; m < 16; ++m) rowbuf[buf][g + 4 * m] = sv[m];
@@ -205,26 +198,24 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
                 sv[m] -= rpm * f;
             }
         }
-        if (g == 0) {
-            Rb[k + p * CQ_LD] = p == pk ? rkk : (mine ? rp : 0.0);
-            if (mine) {
-                dg -= rp * rp;
-                if (PIV && vn1 != 0.0) {
-                    // partial norm downdating with the new row k of R
-                    double temp = fabs(rp) / vn1;
-                    if (hq_norms)
-                        temp = (1.0 + temp) * (1.0 - temp);
-                    else
-                        temp = 1.0 - temp * temp;
-                    if (temp < 0.0) temp = 0.0;
-                    const double q5 = vn1 / vn2;
-                    const double temp2 = temp * q5 * q5;
-                    if (temp2 <= tol3z) {
-                        vn1 = (mp - k - 1 > 0) ? sqrt(fmax(dg, 0.0)) : 0.0;
-                        vn2 = vn1;
-                    } else {
-                        vn1 = vn1 * sqrt(temp);
-                    }
+        if (g == 0) Rb[k + p * CQ_LD] = p == pk ? rkk : (mine ? rp : 0.0);
+        if (mine) {
+            dg -= rp * rp;
+            if (PIV && vn1 != 0.0) {
+                // partial norm downdating with the new row k of R
+                double temp = fabs(rp) / vn1;
+                if (hq_norms)
+                    temp = (1.0 + temp) * (1.0 - temp);
+                else
+                    temp = 1.0 - temp * temp;
+                if (temp < 0.0) temp = 0.0;
+                const double q5 = vn1 / vn2;
+                const double temp2 = temp * q5 * q5;
+                if (temp2 <= tol3z) {
+                    vn1 = (mp - k - 1 > 0) ? sqrt(fmax(dg, 0.0)) : 0.0;
+                    vn2 = vn1;
+                } else {
+                    vn1 = vn1 * sqrt(temp);
                 }
             }
         }

@@ -1502,17 +1502,52 @@ __global__ __launch_bounds__(256) void k_apply_swaps(int b, const int32_t *swaps
     }
 }
 
-// B = G1 - BT U11^T  (b x b unit lower U11 = Vf top rows); mY x b
+// The small middle of the Y downdate, one workgroup per 16 rows of Y:
+// BT = B0 Tm (B0 mY x b, Tm b x b upper, ld ldt) and B = G1(:, 0:b) - BT U11^T
+// (U11 = the unit lower top b x b block of Vf, ld mp); both mY x b, ld mY.
+// Tm and U11 pass through LDS 16 columns (rows) at a time, so the workgroup
+// needs 24 KB of LDS in FP64 and fits beside the GEMM tiles of another stream.
+// Thread (i = t & 15, c = t >> 4) forms entry (i, 16 s + c) of slice s.
 template <class T>
-__global__ void k_g1_minus(int mY, int b, const T *G1, long ldg, const T *BT, const T *Vf,
-                           long mp, T *B) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int r = blockIdx.y;
-    if (i >= mY) return;
-    // (BT U11^T)(i, r) = sum_{q <= r} BT(i, q) U11(r, q), U11(r, r) = 1
-    T s = BT[i + (long)r * mY];
-    for (int q = 0; q < r; ++q) s += BT[i + (long)q * mY] * Vf[r + (long)q * mp];
-    B[i + (long)r * mY] = G1[i + (long)r * ldg] - s;
+__global__ __launch_bounds__(256) void k_yd_small(int mY, int b, const T *B0, const T *Tm, int ldt,
+                                                  const T *G1, long ldg, const T *Vf, long mp,
+                                                  T *BT, T *B) {
+    // (row, col) of the 16 x 64 row blocks at row + 16 col; sS: the slice of
+    // Tm (q + 64 c: Tm(q, 16 s + c)) or of U11 (c + 16 q: U11(16 s + c, q))
+    __shared__ T sB0[16 * 64], sBT[16 * 64], sS[64 * 16];
+    const int tid = threadIdx.x, i = tid & 15, c = tid >> 4;
+    const int r0 = blockIdx.x * 16;
+    const int rows = min(16, mY - r0);
+    for (int e = tid; e < 16 * 64; e += 256) {
+        const int ii = e & 15, q = e >> 4;
+        sB0[ii + q * 16] = (ii < rows && q < b) ? B0[(r0 + ii) + (long)q * mY] : T(0);
+    }
+    for (int s0 = 0; s0 < b; s0 += 16) {
+        __syncthreads(); // sB0 stored / previous slice consumed
+        for (int e = tid; e < 64 * 16; e += 256) {
+            const int q = e & 63, cc = e >> 6, r = s0 + cc;
+            sS[q + 64 * cc] = (r < b && q <= r) ? Tm[q + (long)r * ldt] : T(0);
+        }
+        __syncthreads();
+        const int r = s0 + c;
+        T acc = 0;
+        for (int q = 0; q <= r && q < b; ++q) acc += sB0[i + q * 16] * sS[q + 64 * c];
+        sBT[i + r * 16] = acc;
+        if (i < rows && r < b) BT[(r0 + i) + (long)r * mY] = acc;
+    }
+    for (int s0 = 0; s0 < b; s0 += 16) {
+        __syncthreads(); // sBT complete / previous slice consumed
+        for (int e = tid; e < 64 * 16; e += 256) {
+            const int cc = e & 15, q = e >> 4, r = s0 + cc;
+            sS[cc + 16 * q] = (r < b && q <= r) ? (q == r ? T(1) : Vf[r + (long)q * mp]) : T(0);
+        }
+        __syncthreads();
+        // (BT U11^T)(i, r) = sum_{q <= r} BT(i, q) U11(r, q)
+        const int r = s0 + c;
+        T acc = 0;
+        for (int q = 0; q <= r && q < b; ++q) acc += sBT[i + q * 16] * sS[c + 16 * q];
+        if (i < rows && r < b) B[(r0 + i) + (long)r * mY] = G1[(r0 + i) + (long)r * ldg] - acc;
+    }
 }
 
 // the column-per-lane pivot kernel unless SKL_YQRP=rows selects the
@@ -1650,21 +1685,15 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
         panel_t<T>(mp, b, W + j + j * ldw, ldw, tau + j, Vf, Tm, ar);
         long nc2 = ncol - j - b;
         T *C2 = W + j + (j + b) * ldw;
+        T *Wt2 = nullptr; // T^T V^T C2 when the rows below R12 are left to the side stream
         if (nc2 > 0 && ahead && !last && mp > b) {
             // C2 -= V (T^T (V^T C2)): the top b rows (R12, which the Y
-            // downdate needs) here, the rows below on the side stream, where
-            // the update overlaps the Y downdate and the next pivot search
+            // downdate needs) here; the rows below on the side stream (below)
             T *Wt = ar.alloc_n<T>((size_t)b * nc2);
-            T *Wt2 = ar.alloc_n<T>((size_t)b * nc2);
+            Wt2 = ar.alloc_n<T>((size_t)b * nc2);
             gemm<T>('T', 'N', b, nc2, mp, T(1), Vf, mp, C2, ldw, T(0), Wt, b);
             gemm<T>('T', 'N', b, nc2, b, T(1), Tm, b, Wt, b, T(0), Wt2, b);
             gemm<T>('N', 'N', b, nc2, b, T(-1), Vf, mp, Wt2, b, T(1), C2, ldw);
-            SKL_HIP(hipEventRecord(c.ev[0], c.stream));
-            OnSideStream on_side;
-            SKL_HIP(hipStreamWaitEvent(c.stream, c.ev[0], 0));
-            gemm<T>('N', 'N', mp - b, nc2, b, T(-1), Vf + b, mp, Wt2, b, T(1), C2 + b, ldw);
-            SKL_HIP(hipEventRecord(c.ev[1], c.stream));
-            side_busy = true;
         } else if (nc2 > 0) {
             apply_block_left_t<T>(mp, b, Vf, Tm, nc2, C2, ldw, ar);
         }
@@ -1672,15 +1701,26 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
             // Y2 -= (G1 - (G V) T U11^T) R12 ; G(:, j:m) -= (G V) T V^T
             T *G1 = G + j * mY;
             gemm<T>('N', 'N', mY, b, mp, T(1), G1, mY, Vf, mp, T(0), B0, mY);
-            gemm<T>('N', 'N', mY, b, b, T(1), B0, mY, Tm, nb, T(0), BT, mY);
-            dim3 gb((unsigned)((mY + 255) / 256), (unsigned)b);
-            k_g1_minus<T><<<gb, 256, 0, s>>>(mY, b, G1, mY, BT, Vf, mp, B);
+            k_yd_small<T><<<(unsigned)((mY + 15) / 16), 256, 0, s>>>(mY, b, B0, Tm, nb, G1, mY,
+                                                                    Vf, mp, BT, B);
             SKL_CHECK_LAUNCH();
             long n2 = n - j - b;
             if (n2 > 0)
                 gemm<T>('N', 'N', mY, n2, b, T(-1), B, mY, W + j + (j + b) * ldw, ldw, T(1),
                         Y + (j + b) * mY, mY);
             gemm<T>('N', 'T', mY, mp, b, T(-1), BT, mY, Vf, mp, T(1), G1, mY);
+        }
+        if (Wt2) {
+            // the rows below R12 on the side stream, queued after the Y
+            // downdate: the update (every CU's worth of GEMM tiles) then
+            // overlaps the next pivot search, which holds a few CUs, instead
+            // of keeping the downdate's small kernels from being scheduled
+            SKL_HIP(hipEventRecord(c.ev[0], c.stream));
+            OnSideStream on_side;
+            SKL_HIP(hipStreamWaitEvent(c.stream, c.ev[0], 0));
+            gemm<T>('N', 'N', mp - b, nc2, b, T(-1), Vf + b, mp, Wt2, b, T(1), C2 + b, ldw);
+            SKL_HIP(hipEventRecord(c.ev[1], c.stream));
+            side_busy = true;
         }
     }
     join_side();
