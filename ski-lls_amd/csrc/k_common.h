@@ -101,6 +101,53 @@ __device__ inline bool grid_failed(const unsigned *err) {
     return __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
 }
 
+// Lane moves within rows of 16 (DPP, VALU ops without an LDS round trip):
+// quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror, row_mirror.  After the
+// four the value combined over its row of 16 sits in every lane of the row.
+template <int CTRL> __device__ inline int dpp_move(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ inline float dpp_move(float v) {
+    return __int_as_float(dpp_move<CTRL>(__float_as_int(v)));
+}
+template <int CTRL> __device__ inline double dpp_move(double v) {
+    const int lo = dpp_move<CTRL>(__double2loint(v)), hi = dpp_move<CTRL>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
+
+// max over the wave, the same value in every lane (rows by DPP, then the four
+// row results by readlane)
+template <class T> __device__ inline T wave_max_bcast(T v) {
+    v = fmax(v, dpp_move<DPP_XOR1>(v));
+    v = fmax(v, dpp_move<DPP_XOR2>(v));
+    v = fmax(v, dpp_move<DPP_HALF_MIRROR>(v));
+    v = fmax(v, dpp_move<DPP_MIRROR>(v));
+    return fmax(fmax(readlane(v, 0), readlane(v, 16)), fmax(readlane(v, 32), readlane(v, 48)));
+}
+__device__ inline int wave_min_bcast(int k) {
+    k = min(k, dpp_move<DPP_XOR1>(k));
+    k = min(k, dpp_move<DPP_XOR2>(k));
+    k = min(k, dpp_move<DPP_HALF_MIRROR>(k));
+    k = min(k, dpp_move<DPP_MIRROR>(k));
+    return min(min(__builtin_amdgcn_readlane(k, 0), __builtin_amdgcn_readlane(k, 16)),
+               min(__builtin_amdgcn_readlane(k, 32), __builtin_amdgcn_readlane(k, 48)));
+}
+
+// Lane of the largest v over the wave's candidates (lanes with v >= 0; v < 0
+// marks a lane that is not one, so NaN must be mapped below 0 by the caller),
+// ties to the smallest key; -1 when no lane is a candidate.  Wave uniform.
+// The same choice as a butterfly over (v, key) pairs, from one max, one
+// ballot and, only on ties, one min over the keys.
+template <class T> __device__ inline int wave_argmax(T v, int key) {
+    const T m = wave_max_bcast(v);
+    if (m < T(0)) return -1;
+    const unsigned long long hit = __ballot(v == m);
+    if (__popcll(hit) == 1) return __ffsll((long long)hit) - 1;
+    const int kmin = wave_min_bcast(v == m ? key : 0x7fffffff);
+    return __ffsll((long long)__ballot(v == m && key == kmin)) - 1;
+}
+
 template <class T> __device__ inline T wave_max(T v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));

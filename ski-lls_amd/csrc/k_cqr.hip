@@ -21,7 +21,9 @@
 //      Q - [S; 0], with S_kk = -sign of the k-th pivot, gives the unit lower
 //      trapezoidal V = L, U and tau_k = -U_kk S_kk, and the R of the
 //      Householder QR is S R -- the factors the panel's dgeqrf / QRP would
-//      store, in exact arithmetic.
+//      store, in exact arithmetic -- and the compact WY factor of the
+//      reflectors is T = -U S L1^-T (L1 the top block of V), so the
+//      trailing update needs no V^T V product.
 //
 // A panel whose Cholesky meets a partial norm below 1e-4 of the column's own
 // norm (near dependence: the Gram matrix no longer resolves it) or a pivot
@@ -51,6 +53,40 @@ __device__ inline double rcp_nr(double x) {
     y = y * fma(-x, y, 2.0);
     y = y * fma(-x, y, 2.0);
     return y;
+}
+
+// C = A B for 64 x 64 matrices in LDS (ld CQ_LD) on the matrix cores: wave w
+// of the CQ_T threads forms the 32 x 32 quadrant (32 (w & 1), 32 (w >> 1)) as
+// 2 x 2 blocks of v_mfma_f64_16x16x4f64 (A(i, k) from lane i + 16 k, B(k, j)
+// from lane j + 16 k; accumulator element r of lane l holds row l / 16 + 4 r,
+// column l % 16 of its block).  Each entry goes to store(row, col, value).
+// No barrier inside: A and B must be ready, and the caller synchronises
+// before anything store wrote is read.
+template <class F> __device__ inline void mm64_lds(const double *A, const double *B, F store) {
+    typedef double acc4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int qr = (w & 1) * 32, qc = (w >> 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    acc4 c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
+#pragma unroll 4
+    for (int k0 = 0; k0 < 64; k0 += 4) {
+        const double a0 = A[(qr + li) + (k0 + lk) * CQ_LD];
+        const double a1 = A[(qr + 16 + li) + (k0 + lk) * CQ_LD];
+        const double b0 = B[(k0 + lk) + (qc + li) * CQ_LD];
+        const double b1 = B[(k0 + lk) + (qc + 16 + li) * CQ_LD];
+        c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c00, 0, 0, 0);
+        c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c01, 0, 0, 0);
+        c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c10, 0, 0, 0);
+        c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c11, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = lk + 4 * r, col = li;
+        store(qr + row, qc + col, c00[r]);
+        store(qr + row, qc + 16 + col, c01[r]);
+        store(qr + 16 + row, qc + col, c10[r]);
+        store(qr + 16 + row, qc + 16 + col, c11[r]);
+    }
 }
 
 // X = R^-1 for the upper triangular 64 x 64 R in LDS (ld CQ_LD; rows and
@@ -146,23 +182,8 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
         if (PIV) {
             // largest partial norm among the columns left, first logical
             // position on ties
-            double bv = -1.0;
-            int bpos = 0x7fffffff, bl = -1;
-            if (((act >> p) & 1ull) && vn1 == vn1) {
-                bv = fabs(vn1);
-                bpos = pos;
-                bl = p;
-            }
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const double ov = __shfl_xor(bv, o, 64);
-                const int opos = __shfl_xor(bpos, o, 64), ol = __shfl_xor(bl, o, 64);
-                if (ov > bv || (ov == bv && opos < bpos)) {
-                    bv = ov;
-                    bpos = opos;
-                    bl = ol;
-                }
-            }
+            const double bv = (((act >> p) & 1ull) && vn1 == vn1) ? fabs(vn1) : -1.0;
+            int bl = wave_argmax(bv, pos);
             if (bl < 0) bl = __builtin_ctzll(__ballot(pos == k)); // no finite candidate
             pk = bl;
             const int lp = __shfl(pos, pk, 64); // logical position of the pivot
@@ -278,17 +299,23 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_rmul(long rows, int b, const doubl
     }
 }
 
-// Householder reconstruction of the panel from Q (mp x b, ld ldq) and
-// R = R2 R1: the LU of the top block of Q - [S; 0] (S_kk = -sign of the k-th
-// pivot) gives L1 (below the diagonal of the panel's top block), U^-1 (the
-// rows below are V2 = Q2 U^-1) and tau_k = -U_kk S_kk; the top block's upper
-// triangle receives S R.  The panel's column order goes into jpvt (entries
-// permuted) and swaps.  Runs only when *flag is 0.  Thread t owns row t & 63
-// and every fourth column from t >> 6 on: one barrier per step.
-__global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q, long ldq,
-                                                 const double *R1, const double *R2,
-                                                 const int32_t *perm, const int32_t *swp,
-                                                 double *Wp, long ldw, double *tau, double *Uinv,
+// Householder reconstruction of the panel from Q = Q1 Z2 (Q1 mp x b, ld ldq;
+// Z2 = R2^-1, b x b upper, ld b) and R = R2 R1: only the top block of Q is
+// formed here (the rows below never are: the final product applies Z2 U^-1
+// to Q1).  The LU of the top block of Q - [S; 0] (S_kk = -sign of the k-th
+// pivot) gives L1 (below the diagonal of the panel's top block), U and
+// tau_k = -U_kk S_kk; the top block's upper triangle receives S R.  Outputs:
+// Zc = Z2 U^-1 (b x b, ld b) for V2 = Q1(b:, :) Zc, and, when Tout is not
+// null, the compact WY factor of the reflectors, T = -U S L1^-T (upper, b x b,
+// ld b): (I - V T V^T)[S; 0] = Q means Q - [S; 0] = V (-T L1^T S), an LU
+// factorization with unit lower V.  The panel's column order goes into jpvt
+// (entries permuted) and swaps.  Runs only when *flag is 0.  Thread t owns
+// row t & 63 and every fourth column from t >> 6 on: one barrier per step.
+__global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long ldq,
+                                                 const double *Z2, const double *R1,
+                                                 const double *R2, const int32_t *perm,
+                                                 const int32_t *swp, double *Wp, long ldw,
+                                                 double *tau, double *Tout, double *Zc,
                                                  int32_t *jpvt, int32_t *swaps, const int *flag) {
     // thread t keeps entries (i, g + 4 m) of the top block in registers,
     // i = t & 63, g = t >> 6; step k publishes row k and column k (double
@@ -296,17 +323,26 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q, long ld
     __shared__ double Ub[64 * CQ_LD];
     __shared__ double Lb[64 * CQ_LD];
     __shared__ double Ys[64 * CQ_LD];
+    __shared__ double Xs[64 * CQ_LD];
     __shared__ double rowbuf[2][64], colbuf[2][64];
     __shared__ double sg[64];
     __shared__ int jp[64];
     if (*flag != 0) return;
     const int tid = threadIdx.x, i = tid & 63, g = tid >> 6;
+    // the top block of Q = Q1(0:64, :) Z2 (Q1's top block in Ys, Z2 in Lb,
+    // the product into Xs)
+    for (int e = tid; e < 64 * 64; e += CQ_T) {
+        const int r = e & 63, c = e >> 6;
+        Ys[r + c * CQ_LD] = (r < b && c < b) ? Q1[r + (long)c * ldq] : 0.0;
+        Lb[r + c * CQ_LD] = (r < b && c < b && r <= c) ? Z2[r + (long)c * b] : 0.0;
+    }
+    __syncthreads();
+    mm64_lds(Ys, Lb, [&](int r, int c, double v) { Xs[r + c * CQ_LD] = v; });
+    __syncthreads();
     double a[16];
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const int j = g + 4 * m;
-        a[m] = (i < b && j < b) ? Q[i + (long)j * ldq] : 0.0;
-    }
+    for (int m = 0; m < 16; ++m) a[m] = Xs[i + (g + 4 * m) * CQ_LD];
+    __syncthreads();
     for (int e = tid; e < 64 * CQ_LD; e += CQ_T) {
         Ub[e] = 0.0;
         Lb[e] = 0.0;
@@ -356,58 +392,75 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q, long ld
         __syncthreads();
     }
     if (tid < b) tau[tid] = -Ub[tid + tid * CQ_LD] * sg[tid];
-    // top block: S R2 R1 on and above the diagonal, L1 below; R2 and R1 go
-    // through LDS (Ys, and rowbuf's space is too small: R1 into a tile of Ys
-    // rows 0..63, R2 read once per row into registers)
-    for (int e = tid; e < 64 * 64; e += CQ_T) Ys[(e & 63) + (e >> 6) * CQ_LD] = R1[e];
-    __syncthreads();
-    {
-        // thread (r = tid & 63, g) forms row r for columns g + 4 m
-        const int r = tid & 63;
-        double r2[64];
-#pragma unroll
-        for (int q = 0; q < 64; ++q) r2[q] = R2[r + q * 64];
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            const int c = g + 4 * m;
-            double s = 0.0;
-#pragma unroll
-            for (int q = 0; q < 64; ++q) s += r2[q] * Ys[q + c * CQ_LD]; // R2, R1 upper: zeros elsewhere
-            const double v = r <= c ? sg[r] * s : Lb[r + c * CQ_LD];
-            if (r < b && c < b) Wp[r + (long)c * ldw] = v;
-        }
+    // top block: S R2 R1 on and above the diagonal, L1 below (R2 in Xs, R1
+    // in Ys; both upper with zeros elsewhere, ld 64)
+    for (int e = tid; e < 64 * 64; e += CQ_T) {
+        Xs[(e & 63) + (e >> 6) * CQ_LD] = R2[e];
+        Ys[(e & 63) + (e >> 6) * CQ_LD] = R1[e];
     }
+    __syncthreads();
+    mm64_lds(Xs, Ys, [&](int r, int c, double v) {
+        if (r < b && c < b) Wp[r + (long)c * ldw] = r <= c ? sg[r] * v : Lb[r + c * CQ_LD];
+    });
     if (jpvt) {
         if (tid < b) jp[tid] = jpvt[perm[tid]];
         __syncthreads();
         if (tid < b) jpvt[tid] = jp[tid];
     }
     if (swaps && tid < b) swaps[tid] = swp[tid];
-    // U^-1 (b x b, ld b) for the rows below the top block: V2 = Q2 U^-1
+    __syncthreads(); // L1 (Lb) and R1 (Ys) read
+    if (Tout) {
+        // T = -U S X with X = L1^-T: L1^T (unit upper) into Ys, its inverse
+        // into Xs (Lb as the inversion's scratch)
+        for (int e = tid; e < 64 * 64; e += CQ_T) {
+            const int r = e & 63, c = e >> 6;
+            Ys[r + c * CQ_LD] = r < c ? Lb[c + r * CQ_LD] : (r == c ? 1.0 : 0.0);
+        }
+        __syncthreads();
+        tri_inv_upper_lds(Ys, Xs, Lb);
+        // S X into Ys (rows scaled), then T = -U (S X)
+        for (int e = tid; e < 64 * 64; e += CQ_T) {
+            const int r = e & 63, c = e >> 6;
+            Ys[r + c * CQ_LD] = (r < b ? sg[r] : 1.0) * Xs[r + c * CQ_LD];
+        }
+        __syncthreads();
+        mm64_lds(Ub, Ys, [&](int r, int c, double v) {
+            if (r < b && c < b) Tout[r + (long)c * b] = r <= c ? -v : 0.0;
+        });
+        __syncthreads();
+    }
+    // U^-1, then Zc = Z2 U^-1 (Z2 into Ys)
     for (int e = tid; e < 64 * 64; e += CQ_T) {
         const int r = e & 63, c = e >> 6;
         if (c >= b) Ub[r + c * CQ_LD] = (r == c) ? 1.0 : 0.0;
     }
     __syncthreads();
     tri_inv_upper_lds(Ub, Lb, Ys);
-    for (int e = tid; e < b * b; e += CQ_T) Uinv[e] = Lb[(e % b) + (e / b) * CQ_LD];
+    for (int e = tid; e < 64 * 64; e += CQ_T) {
+        const int r = e & 63, c = e >> 6;
+        Ys[r + c * CQ_LD] = (r < b && c < b && r <= c) ? Z2[r + (long)c * b] : 0.0;
+    }
+    __syncthreads();
+    mm64_lds(Ys, Lb, [&](int r, int c, double v) {
+        if (r < b && c < b) Zc[r + (long)c * b] = r <= c ? v : 0.0;
+    });
 }
 
 // Queues the fast path for the panel Wp (mp x b, ld ldw): pivot != 0 uses the
-// HQRRP panel pivoting (hq_norms) and fills jpvt (may be null) and swaps.
+// HQRRP panel pivoting (hq_norms) and fills jpvt (may be null) and swaps;
+// Tout (b x b, ld b, may be null) receives the reflectors' T factor.
 // Returns the device flag that is nonzero at run time when the fast path gave
 // up and left the panel untouched.
 int *cqr_panel(long mp, int b, int pivot, double *Wp, long ldw, double *tau, int32_t *jpvt,
-               int32_t *swaps, int hq_norms, Arena &ar) {
+               int32_t *swaps, int hq_norms, double *Tout, Arena &ar) {
     hipStream_t s = ctx().stream;
     double *G = ar.alloc_n<double>(64 * 64);
     double *R1 = ar.alloc_n<double>(64 * 64);
     double *R2 = ar.alloc_n<double>(64 * 64);
     double *Z1 = ar.alloc_n<double>(64 * 64);
     double *Z2 = ar.alloc_n<double>(64 * 64);
-    double *Ui = ar.alloc_n<double>(64 * 64);
+    double *Zc = ar.alloc_n<double>(64 * 64);
     double *Q1 = ar.alloc_n<double>((size_t)mp * b);
-    double *Q2 = ar.alloc_n<double>((size_t)mp * b);
     int32_t *perm = ar.alloc_n<int32_t>(64);
     int32_t *swp = ar.alloc_n<int32_t>(64);
     int *flag = ar.alloc_n<int>(1);
@@ -420,12 +473,11 @@ int *cqr_panel(long mp, int b, int pivot, double *Wp, long ldw, double *tau, int
     k_cqr_rmul<<<gr, CQ_T, 0, s>>>(mp, b, Wp, ldw, Z1, Q1, mp, flag);
     gemm<double>('T', 'N', b, b, mp, 1.0, Q1, mp, Q1, mp, 0.0, G, b);
     k_cqr_chol<false><<<1, CQ_T, 0, s>>>(b, mp, G, b, R2, Z2, nullptr, nullptr, flag, 0, 2);
-    k_cqr_rmul<<<gr, CQ_T, 0, s>>>(mp, b, Q1, mp, Z2, Q2, mp, flag);
-    k_cqr_hr<<<1, CQ_T, 0, s>>>(b, Q2, mp, R1, R2, perm, swp, Wp, ldw, tau, Ui,
+    k_cqr_hr<<<1, CQ_T, 0, s>>>(b, Q1, mp, Z2, R1, R2, perm, swp, Wp, ldw, tau, Tout, Zc,
                                  pivot ? jpvt : nullptr, pivot ? swaps : nullptr, flag);
     if (mp > b) {
         const unsigned g2 = (unsigned)((mp - b + 63) / 64);
-        k_cqr_rmul<<<g2, CQ_T, 0, s>>>(mp - b, b, Q2 + b, mp, Ui, Wp + b, ldw, flag);
+        k_cqr_rmul<<<g2, CQ_T, 0, s>>>(mp - b, b, Q1 + b, mp, Zc, Wp + b, ldw, flag);
     }
     SKL_CHECK_LAUNCH();
     return flag;

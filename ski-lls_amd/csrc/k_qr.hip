@@ -639,9 +639,8 @@ static bool regpanel_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T 
     a.part = ar.alloc_n<T>((size_t)2 * P * 64);
     a.rowv = ar.alloc_n<T>((size_t)2 * 64);
     a.rcp = ar.alloc_n<T>((size_t)P * 64);
-    a.bar = grid_sync(ar.alloc_n<unsigned>(1));
+    a.bar = grid_sync_fresh(ar);
     hipStream_t s = ctx().stream;
-    SKL_HIP(hipMemsetAsync(a.bar.ctr, 0, sizeof(unsigned), s));
     BarrierLaunch guard((int)P);
     switch (K) {
     case 8: launch_regpanel<T, 8>((int)P, a, s); break;
@@ -662,7 +661,7 @@ static constexpr size_t RS_LDS_BUDGET = 150 * 1024;
 // columns), tall enough that the column-by-column kernel needs several
 // workgroups; SKL_QR_CQR=0 turns it off.
 int *cqr_panel(long mp, int b, int pivot, double *Wp, long ldw, double *tau, int32_t *jpvt,
-               int32_t *swaps, int hq_norms, Arena &ar);
+               int32_t *swaps, int hq_norms, double *Tout, Arena &ar);
 static bool cqr_wanted(long m, int nc, int npiv, int nsteps) {
     static const bool on = [] {
         const char *e = getenv("SKL_QR_CQR");
@@ -672,19 +671,24 @@ static bool cqr_wanted(long m, int nc, int npiv, int nsteps) {
            m >= 512;
 }
 
+// Returns the CholeskyQR2 panel's device flag when that panel was queued (it
+// then wrote the reflectors' T factor to Tout, when given, unless the flag
+// ends nonzero), else null.
 template <class T>
-static void rowslice_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T *tau,
-                        int32_t *jpvt, int32_t *swaps, int hq_norms, Arena &ar) {
-    if (nsteps <= 0) return;
+static const int *rowslice_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T *tau,
+                              int32_t *jpvt, int32_t *swaps, int hq_norms, Arena &ar,
+                              T *Tout = nullptr) {
+    if (nsteps <= 0) return nullptr;
     // with the fast panel queued first, the kernels below run only when it
     // gave up (its flag nonzero)
     const int *only_if = nullptr;
     if constexpr (std::is_same<T, double>::value) {
         if (cqr_wanted(m, nc, npiv, nsteps))
-            only_if = cqr_panel(m, nc, npiv > 0 ? 1 : 0, W, ldw, tau, jpvt, swaps, hq_norms, ar);
+            only_if = cqr_panel(m, nc, npiv > 0 ? 1 : 0, W, ldw, tau, jpvt, swaps, hq_norms,
+                                Tout, ar);
     }
     if (regpanel_qr<T>(m, nc, npiv, nsteps, W, ldw, tau, jpvt, swaps, hq_norms, only_if, ar))
-        return;
+        return only_if;
     const int P_half = barrier_grid_cap();
     const int P_max = std::max(1, ctx().num_cu);
     size_t fixed = sizeof(T) * (2 * (size_t)nc + 2 * (size_t)std::max(npiv, 1) + RS_T) +
@@ -733,10 +737,9 @@ static void rowslice_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T 
     a.red = ar.alloc_n<T>((size_t)2 * nc);
     a.rowv = ar.alloc_n<T>((size_t)2 * nc);
     a.rcp = ar.alloc_n<T>((size_t)P * nc);
-    a.bar = grid_sync(ar.alloc_n<unsigned>(1));
+    a.bar = grid_sync_fresh(ar);
     size_t sh = sizeof(T) * (size_t)a.ldl * nc + fixed;
     hipStream_t s = ctx().stream;
-    SKL_HIP(hipMemsetAsync(a.bar.ctr, 0, sizeof(unsigned), s));
     if (sh > 65536)
         SKL_HIP(hipFuncSetAttribute((const void *)k_qr_rowslice<T>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
@@ -745,6 +748,7 @@ static void rowslice_qr(long m, int nc, int npiv, int nsteps, T *W, long ldw, T 
     BarrierLaunch guard(P, P > P_half);
     k_qr_rowslice<T><<<P, RS_T, sh, s>>>(a);
     SKL_CHECK_LAUNCH();
+    return only_if;
 }
 
 // ---------------------------------------------------------------------------
@@ -769,14 +773,45 @@ __global__ void k_vfull(long mp, int b, const T *W, long ldw, T *Vf) {
 // (8 -> 16 -> 32 -> 64) form the off-diagonal blocks with two small products
 // each.  Reflectors past b have tau = 0 (zero rows and columns).
 template <class T>
-__global__ __launch_bounds__(256) void k_build_t(int b, const T *VtV, const T *tau, T *Tm) {
+// only_if: run only when *only_if is nonzero, with VtV null: V^T V is then
+// formed here from Vf (mp x b), 64 rows at a time through LDS.
+__global__ __launch_bounds__(256) void k_build_t(int b, const T *VtV, const T *tau, T *Tm,
+                                                 const int *only_if, long mp, const T *Vf) {
     constexpr int LD = 65;
     __shared__ T sT[64 * LD], sV[64 * LD], sY[64 * LD];
     __shared__ T st[64];
+    if (only_if && *only_if == 0) return;
     const int tid = threadIdx.x;
+    if (VtV) {
+        for (int e = tid; e < 64 * 64; e += 256) {
+            const int r = e & 63, q = e >> 6;
+            sV[r + q * LD] = (r < b && q < b) ? VtV[r + q * b] : T(0);
+        }
+    } else {
+        // thread (r = t & 63, g = t >> 6) sums entries (r, g + 4 m) over the
+        // rows; chunks of 64 rows of Vf staged in sY
+        T acc[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) acc[m] = T(0);
+        const int r = tid & 63, g = tid >> 6;
+        for (long i0 = 0; i0 < mp; i0 += 64) {
+            __syncthreads();
+            for (int e = tid; e < 64 * 64; e += 256) {
+                const int ii = e & 63, q = e >> 6;
+                sY[ii + q * LD] = (i0 + ii < mp && q < b) ? Vf[(i0 + ii) + (long)q * mp] : T(0);
+            }
+            __syncthreads();
+            for (int ii = 0; ii < 64; ++ii) {
+                const T vr = sY[ii + r * LD];
+#pragma unroll
+                for (int m = 0; m < 16; ++m) acc[m] += vr * sY[ii + (g + 4 * m) * LD];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 16; ++m) sV[r + (g + 4 * m) * LD] = acc[m];
+    }
     for (int e = tid; e < 64 * 64; e += 256) {
         const int r = e & 63, q = e >> 6;
-        sV[r + q * LD] = (r < b && q < b) ? VtV[r + q * b] : T(0);
         sT[r + q * LD] = T(0);
     }
     if (tid < 64) st[tid] = tid < b ? tau[tid] : T(0);
@@ -836,13 +871,21 @@ static void apply_block_left_t(long mp, int b, const T *Vf, const T *Tm, long nc
     gemm<T>('N', 'N', mp, nc2, b, T(-1), Vf, mp, Wt2, b, T(1), Cm, ldc);
 }
 
+// Vf and the T factor of a factored panel.  cqr_flag: the CholeskyQR2 panel's
+// flag (rowslice_qr), whose kernels wrote T already unless the flag is
+// nonzero; k_build_t then forms V^T V from Vf itself (the rare case).
 template <class T>
-static void panel_t(long mp, int b, const T *Wp, long ldw, const T *tau, T *Vf, T *Tm, Arena &ar) {
+static void panel_t(long mp, int b, const T *Wp, long ldw, const T *tau, T *Vf, T *Tm, Arena &ar,
+                    const int *cqr_flag = nullptr) {
     dim3 g((unsigned)((mp + 255) / 256), (unsigned)b);
     k_vfull<T><<<g, 256, 0, ctx().stream>>>(mp, b, Wp, ldw, Vf);
-    T *VtV = ar.alloc_n<T>((size_t)b * b);
-    gemm<T>('T', 'N', b, b, mp, T(1), Vf, mp, Vf, mp, T(0), VtV, b);
-    k_build_t<T><<<1, 256, 0, ctx().stream>>>(b, VtV, tau, Tm);
+    if (cqr_flag) {
+        k_build_t<T><<<1, 256, 0, ctx().stream>>>(b, nullptr, tau, Tm, cqr_flag, mp, Vf);
+    } else {
+        T *VtV = ar.alloc_n<T>((size_t)b * b);
+        gemm<T>('T', 'N', b, b, mp, T(1), Vf, mp, Vf, mp, T(0), VtV, b);
+        k_build_t<T><<<1, 256, 0, ctx().stream>>>(b, VtV, tau, Tm, nullptr, 0, nullptr);
+    }
     SKL_CHECK_LAUNCH();
 }
 
@@ -882,11 +925,12 @@ void qr_householder(long m, long ncol, long nfac, T *W, long ldw, T *tau, Arena 
     for (long j = 0, p = 0; j < kmax; j += nb, ++p) {
         int b = (int)std::min<long>(nb, kmax - j);
         long mp = m - j;
-        rowslice_qr<T>(mp, b, 0, b, W + j + j * ldw, ldw, tau + j, nullptr, nullptr, 0, ar);
+        T *V = Vf[p & 1], *Tp = Tm[p & 1];
+        const int *cf = rowslice_qr<T>(mp, b, 0, b, W + j + j * ldw, ldw, tau + j, nullptr,
+                                       nullptr, 0, ar, Tp);
         long nc2 = ncol - j - b;
         if (nc2 <= 0) continue;
-        T *V = Vf[p & 1], *Tp = Tm[p & 1];
-        panel_t<T>(mp, b, W + j + j * ldw, ldw, tau + j, V, Tp, ar);
+        panel_t<T>(mp, b, W + j + j * ldw, ldw, tau + j, V, Tp, ar, cf);
         T *C2 = W + j + (j + b) * ldw;
         if (!ahead) {
             apply_block_left_t<T>(mp, b, V, Tp, nc2, C2, ldw, ar);
@@ -1202,16 +1246,16 @@ __global__ __launch_bounds__(64 * W) void k_yqrp(YqArgs<T> a) {
 // a column with the reflector, its update and the norm downdating -- the exact
 // recomputation included -- are loops inside one lane; the only cross-lane
 // work of a step is the argmax.  One grid barrier per pivot, as in k_yqrp:
-// before it each workgroup publishes its best column as a reflector (dlarfg by
-// the owning wave), after it every wave picks the global winner (largest norm,
-// first position on ties) from the staged candidates and applies it.
+// before it each workgroup publishes its best column as a reflector (dlarfg in
+// the lane that holds the column), after it every wave picks the global winner
+// (largest norm, first position on ties) from the staged candidates and
+// applies it.
 constexpr int YQ_ROWS = 74; // nb + pp of HQRRP (64 + 10)
 
 template <class T, int W>
 __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
     __shared__ T s_wv[W];
     __shared__ int s_wp[W], s_wl[W];
-    __shared__ T s_v[YQ_MAXROWS]; // the workgroup candidate's column
     extern __shared__ unsigned char yq_smem[];
     const int P = gridDim.x, blk = blockIdx.x;
     T *s_vec = reinterpret_cast<T *>(yq_smem);      // P x YQ_MAXROWS candidate reflectors
@@ -1245,25 +1289,31 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
     for (int jj = 0; jj < a.nsteps; ++jj) {
         const int buf = jj & 1;
         SKL_PHASE(1, 0);
+        // sum of squares of the column's rows below the current one (four
+        // partial sums), for the reflector if this column wins
+        T xs;
+        {
+            T q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+#pragma unroll
+            for (int r = 1; r < YQ_ROWS; r += 4) {
+                q0 += y[r] * y[r];
+                if (r + 1 < YQ_ROWS) q1 += y[r + 1] * y[r + 1];
+                if (r + 2 < YQ_ROWS) q2 += y[r + 2] * y[r + 2];
+                if (r + 3 < YQ_ROWS) q3 += y[r + 3] * y[r + 3];
+            }
+            xs = (q0 + q1) + (q2 + q3);
+        }
         // best column of this wave (every lane ends with the same answer) ...
         T bv = T(-1);
         int bp = NONE, bl = 0;
         {
             const T av = fabs(dn);
-            if (pos >= jj && pos != NONE && av == av) {
-                bv = av;
-                bp = pos;
-                bl = lane;
-            }
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const T ov = __shfl_xor(bv, o, 64);
-                const int op = __shfl_xor(bp, o, 64), ol = __shfl_xor(bl, o, 64);
-                if (yq_better(ov, op, bv, bp)) {
-                    bv = ov;
-                    bp = op;
-                    bl = ol;
-                }
+            const T cand = (pos >= jj && pos != NONE && av == av) ? av : T(-1);
+            const int wl = wave_argmax(cand, pos);
+            if (wl >= 0) {
+                bv = readlane(cand, wl);
+                bp = __builtin_amdgcn_readlane(pos, wl);
+                bl = wl;
             }
         }
         if (lane == 0) {
@@ -1282,26 +1332,23 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
                 gw = u;
             }
         if (wave == gw) {
-            if (lane == s_wl[gw]) {
-#pragma unroll
-                for (int r = 0; r < YQ_ROWS; ++r) s_v[r] = y[r];
-            }
-            // reflector over rows jj.. in the same relative indexing
-            const int r0 = lane, r1 = lane + 64;
-            const T c0 = r0 < YQ_ROWS ? s_v[r0] : T(0);
-            const T c1 = r1 < YQ_ROWS ? s_v[r1] : T(0);
-            const T alpha = s_v[0];
-            const T xnorm = sqrt(wave_sum((r0 > 0 ? c0 * c0 : T(0)) + c1 * c1));
+            // the reflector (dlarfg) of the winning column over rows jj..,
+            // formed in the lane that holds it: ||y(1:)|| from xs, the sum of
+            // squares every lane keeps of its column's rows below the first
+            const int wl = s_wl[gw];
+            const T alpha = readlane(y[0], wl);
+            const T xnorm = sqrt(readlane(xs, wl));
             T tau = 0, sc = 0;
             if (xnorm != T(0)) {
                 const T beta = -copysign(dlapy2_d(alpha, xnorm), alpha);
                 tau = (beta - alpha) / beta;
                 sc = T(1) / (alpha - beta);
             }
-            T *dst = a.cvec + ((size_t)buf * P + blk) * YQ_MAXROWS;
-            dst[r0] = r0 > 0 ? sc * c0 : T(1);
-            dst[r1] = sc * c1;
-            if (lane == 0) {
+            if (lane == wl) {
+                T *dst = a.cvec + ((size_t)buf * P + blk) * YQ_MAXROWS;
+                dst[0] = T(1);
+#pragma unroll
+                for (int r = 1; r < YQ_ROWS; ++r) dst[r] = sc * y[r];
                 a.cand[buf * P + blk] = gv;
                 a.candp[buf * P + blk] = gp;
                 a.ctau[buf * P + blk] = tau;
@@ -1329,14 +1376,14 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
                 p = s_cp[t];
                 u = t;
             }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const T ov = __shfl_xor(v, o, 64);
-            const int op = __shfl_xor(p, o, 64), ou = __shfl_xor(u, o, 64);
-            if (yq_better(ov, op, v, p)) {
-                v = ov;
-                p = op;
-                u = ou;
+        {
+            const int wl = wave_argmax(v, p);
+            if (wl >= 0) {
+                p = __builtin_amdgcn_readlane(p, wl);
+                u = __builtin_amdgcn_readlane(u, wl);
+            } else {
+                p = NONE;
+                u = 0;
             }
         }
         const int pl = p;
@@ -1428,34 +1475,43 @@ __global__ __launch_bounds__(1024) void k_perm_changed(int nv, const int32_t *pe
     if (threadIdx.x == 0) *nch = s_base;
 }
 
-// gather/scatter of the changed columns: Xout[:, t] = Xin[:, perm[t]]
+// The block permutation of W (m rows), Y (mY rows) and jpvt in one gather
+// and one scatter launch: blocks x < gw of row tile x take W, the others Y;
+// block (0, k) also moves jpvt's entry k.
 template <class T>
-__global__ void k_perm_gather(long rows, const T *X, long ldx, const int32_t *perm,
-                              const int32_t *changed, const int32_t *nch, T *tmp) {
-    int k = blockIdx.y;
+__global__ void k_perm_gather_all(long m, const T *Wc, long ldw, int mY, const T *Yc,
+                                  const int32_t *jp, const int32_t *perm, const int32_t *changed,
+                                  const int32_t *nch, unsigned gw, T *tmpW, T *tmpY,
+                                  int32_t *tmpJ) {
+    const int k = blockIdx.y;
     if (k >= *nch) return;
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows) return;
-    tmp[i + (long)k * rows] = X[i + (long)perm[changed[k]] * ldx];
+    const int src = perm[changed[k]];
+    if (blockIdx.x < gw) {
+        const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < m) tmpW[i + (long)k * m] = Wc[i + (long)src * ldw];
+        if (blockIdx.x == 0 && threadIdx.x == 0) tmpJ[k] = jp[src];
+    } else {
+        const int i = (int)(blockIdx.x - gw) * blockDim.x + threadIdx.x;
+        if (i < mY) tmpY[i + (long)k * mY] = Yc[i + (long)src * mY];
+    }
 }
 template <class T>
-__global__ void k_perm_scatter(long rows, T *X, long ldx, const int32_t *changed,
-                               const int32_t *nch, const T *tmp) {
-    int k = blockIdx.y;
+__global__ void k_perm_scatter_all(long m, T *Wc, long ldw, int mY, T *Yc, int32_t *jp,
+                                   const int32_t *changed, const int32_t *nch, unsigned gw,
+                                   const T *tmpW, const T *tmpY, const int32_t *tmpJ) {
+    const int k = blockIdx.y;
     if (k >= *nch) return;
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows) return;
-    X[i + (long)changed[k] * ldx] = tmp[i + (long)k * rows];
+    const int dst = changed[k];
+    if (blockIdx.x < gw) {
+        const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < m) Wc[i + (long)dst * ldw] = tmpW[i + (long)k * m];
+        if (blockIdx.x == 0 && threadIdx.x == 0) jp[dst] = tmpJ[k];
+    } else {
+        const int i = (int)(blockIdx.x - gw) * blockDim.x + threadIdx.x;
+        if (i < mY) Yc[i + (long)dst * mY] = tmpY[i + (long)k * mY];
+    }
 }
-__global__ void k_perm_int(int nv, int32_t *jp, const int32_t *perm, const int32_t *changed,
-                           const int32_t *nch) {
-    // single block: gather then scatter through shared memory
-    extern __shared__ int32_t sj[];
-    int n = *nch;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) sj[k] = jp[perm[changed[k]]];
-    __syncthreads();
-    for (int k = threadIdx.x; k < n; k += blockDim.x) jp[changed[k]] = sj[k];
-}
+
 
 // Apply the panel's column swaps (s <-> swaps[s], in order, b <= 64) to the
 // columns 0..b-1 of two row blocks.  The swaps are composed into one gather
@@ -1584,6 +1640,8 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
     int32_t *nch = ar.alloc_n<int32_t>(4);
     int32_t *swaps = ar.alloc_n<int32_t>(nb);
     T *tmpc = ar.alloc_n<T>((size_t)std::max<long>(m, mY) * 2 * nb);
+    T *tmpy = ar.alloc_n<T>((size_t)mY * 2 * nb);
+    int32_t *tmpj = ar.alloc_n<int32_t>((size_t)2 * nb);
     T *B0 = ar.alloc_n<T>((size_t)mY * nb);
     T *BT = ar.alloc_n<T>((size_t)mY * nb);
     T *B = ar.alloc_n<T>((size_t)mY * nb);
@@ -1595,7 +1653,7 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
     int32_t *candp = ar.alloc_n<int32_t>((size_t)2 * P_max);
     T *ctau = ar.alloc_n<T>((size_t)2 * P_max);
     T *cvec = ar.alloc_n<T>((size_t)2 * P_max * YQ_MAXROWS);
-    const GridSync bar = grid_sync(ar.alloc_n<unsigned>(1));
+    GridSync bar = grid_sync_fresh(ar); // (a fresh counter per pivot launch below)
     Context &c = ctx();
     const bool ahead = lookahead_on() && !ctx().no_alloc;
     if (ahead) c.make_side();
@@ -1621,6 +1679,7 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
             ya.candp = candp;
             ya.ctau = ctau;
             ya.cvec = cvec;
+            bar = grid_sync_fresh(ar);
             ya.bar = bar;
             if (mY == YQ_ROWS && yqrp_lanes_on()) {
                 // 512 columns per workgroup, one per lane
@@ -1631,7 +1690,6 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
                     SKL_HIP(hipFuncSetAttribute((const void *)k_yqrp_lanes<T, WL>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                                 (int)sh));
-                SKL_HIP(hipMemsetAsync(bar.ctr, 0, sizeof(unsigned), s));
                 BarrierLaunch guard(P);
                 k_yqrp_lanes<T, WL><<<P, 64 * WL, sh, s>>>(ya);
                 SKL_CHECK_LAUNCH();
@@ -1644,7 +1702,6 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
                 int K = nv <= 512 ? 1 : nv <= 1024 ? 2 : nv <= 2048 ? 4 : dbl ? 16 : 8;
                 int Wv = (K == 16) ? 8 : 16;
                 int P = (int)((nv + (long)Wv * K - 1) / ((long)Wv * K));
-                SKL_HIP(hipMemsetAsync(bar.ctr, 0, sizeof(unsigned), s));
                 constexpr int KW = sizeof(T) == 8 ? 16 : 8, WW = sizeof(T) == 8 ? 8 : 16;
                 auto fn = K == 1 ? k_yqrp<T, 1, 16> : K == 2 ? k_yqrp<T, 2, 16>
                           : K == 4 ? k_yqrp<T, 4, 16> : k_yqrp<T, KW, WW>;
@@ -1663,26 +1720,27 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
             join_side();
             k_perm_changed<<<1, 1024, 0, s>>>((int)nv, perm, changed, nch, bar.err);
             // permute W(:, j:n), Y(:, j:n), jpvt(j:n) by the new column order
-            dim3 gw((unsigned)((m + 255) / 256), (unsigned)(2 * b));
-            k_perm_gather<T><<<gw, 256, 0, s>>>(m, W + j * ldw, ldw, perm, changed, nch, tmpc);
-            k_perm_scatter<T><<<gw, 256, 0, s>>>(m, W + j * ldw, ldw, changed, nch, tmpc);
-            dim3 gy((unsigned)((mY + 255) / 256), (unsigned)(2 * b));
-            k_perm_gather<T><<<gy, 256, 0, s>>>(mY, Y + j * mY, mY, perm, changed, nch, tmpc);
-            k_perm_scatter<T><<<gy, 256, 0, s>>>(mY, Y + j * mY, mY, changed, nch, tmpc);
-            k_perm_int<<<1, 256, sizeof(int32_t) * 2 * b, s>>>((int)nv, jpvt + j, perm, changed, nch);
+            // (at most 2 b columns move: the b pivots and the ones they displace)
+            const unsigned gwx = (unsigned)((m + 255) / 256), gyx = (unsigned)((mY + 255) / 256);
+            dim3 gp(gwx + gyx, (unsigned)(2 * b));
+            k_perm_gather_all<T><<<gp, 256, 0, s>>>(m, W + j * ldw, ldw, mY, Y + j * mY, jpvt + j,
+                                                    perm, changed, nch, gwx, tmpc, tmpy, tmpj);
+            k_perm_scatter_all<T><<<gp, 256, 0, s>>>(m, W + j * ldw, ldw, mY, Y + j * mY, jpvt + j,
+                                                     changed, nch, gwx, tmpc, tmpy, tmpj);
             SKL_CHECK_LAUNCH();
         }
         // panel QRP of W(j:m, j:j+b), pivoting W(0:j, j:j+b) and Y(:, j:j+b)
         join_side();
         long mp = m - j;
-        rowslice_qr<T>(mp, b, b, b, W + j + j * ldw, ldw, tau + j, jpvt + j, swaps, 1, ar);
+        const int *cf = rowslice_qr<T>(mp, b, b, b, W + j + j * ldw, ldw, tau + j, jpvt + j,
+                                       swaps, 1, ar, Tm);
         {
             unsigned nblk = (unsigned)((j + 63) / 64 + (mY + 63) / 64);
             k_apply_swaps<T><<<nblk, 256, 0, s>>>(b, swaps, j, W + j * ldw, ldw, mY, Y + j * mY,
                                                   mY, bar.err);
         }
         SKL_CHECK_LAUNCH();
-        panel_t<T>(mp, b, W + j + j * ldw, ldw, tau + j, Vf, Tm, ar);
+        panel_t<T>(mp, b, W + j + j * ldw, ldw, tau + j, Vf, Tm, ar, cf);
         long nc2 = ncol - j - b;
         T *C2 = W + j + (j + b) * ldw;
         T *Wt2 = nullptr; // T^T V^T C2 when the rows below R12 are left to the side stream

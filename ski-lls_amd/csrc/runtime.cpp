@@ -287,6 +287,7 @@ void release_thread_context() {
     if (t_ctx->aux) (void)hipStreamDestroy(t_ctx->aux);
     if (t_ctx->pinned) (void)hipHostFree(t_ctx->pinned);
     if (t_ctx->dev_err) (void)hipFree(t_ctx->dev_err);
+    if (t_ctx->ctr_pool) (void)hipFree(t_ctx->ctr_pool);
     (void)hipStreamDestroy(t_ctx->stream);
     alk.unlock(); // the arenas' destructors below own nothing any more
     delete t_ctx;
@@ -299,6 +300,32 @@ void set_device(int dev) { g_device = dev; }
 GridSync grid_sync(unsigned *ctr) {
     Context &c = ctx();
     return GridSync{ctr, c.dev_err, c.barrier_spins};
+}
+
+// Counters of the pool are handed out once each; a barrier launch leaves its
+// counter at the grid's arrival total, so the pool is zeroed again (after the
+// context's streams drain) when every counter has been used.
+static constexpr int CTR_POOL = 8192;
+
+GridSync grid_sync_fresh(Arena &ar) {
+    Context &c = ctx();
+    if (c.no_alloc) {
+        unsigned *p = ar.alloc_n<unsigned>(1);
+        SKL_HIP(hipMemsetAsync(p, 0, sizeof(unsigned), c.stream));
+        return grid_sync(p);
+    }
+    if (!c.ctr_pool) {
+        AllocLock alk(capture_mutex());
+        SKL_HIP(hipMalloc(&c.ctr_pool, sizeof(unsigned) * CTR_POOL));
+        SKL_HIP(hipMemsetAsync(c.ctr_pool, 0, sizeof(unsigned) * CTR_POOL, c.stream));
+        c.ctr_next = 0;
+    } else if (c.ctr_next == CTR_POOL) {
+        if (c.side) SKL_HIP(hipStreamSynchronize(c.side));
+        if (c.aux) SKL_HIP(hipStreamSynchronize(c.aux));
+        SKL_HIP(hipMemsetAsync(c.ctr_pool, 0, sizeof(unsigned) * CTR_POOL, c.stream));
+        c.ctr_next = 0;
+    }
+    return grid_sync(c.ctr_pool + c.ctr_next++);
 }
 
 void check_grid_fault() {

@@ -115,6 +115,9 @@ struct Context {
     int coop_ok = 0;
     size_t lds_max = 0;
     void *pinned = nullptr; // small pinned host buffer for scalars
+    // zeroed grid-barrier counters, one per launch (grid_sync_fresh)
+    unsigned *ctr_pool = nullptr;
+    int ctr_next = 0;
     // grow-only scratch for split-K partial products (stream-ordered reuse)
     void *ws = nullptr;
     size_t ws_cap = 0;
@@ -148,6 +151,10 @@ class OnSideStream {
 Context &ctx();                       // default per-thread context
 // GridSync for a launch from the calling thread's context, counter ctr
 GridSync grid_sync(unsigned *ctr);
+// A grid barrier whose counter is already zero: taken from the context's pool
+// of zeroed counters (re-zeroed in one fill when used up), or, in
+// no-allocation mode, carved from `ar` and zeroed on the stream.
+GridSync grid_sync_fresh(Arena &ar);
 // Waits for ctx().stream and throws Error(SKL_E_HIP) when a grid barrier
 // launched from this context gave up since the last check (clearing the word).
 void check_grid_fault();
