@@ -332,6 +332,36 @@ def profile_pass(skl, torch, solve, nsolves=3):
     return skl.profile_read()
 
 
+def precision_ablation(args, skl, torch, solve, x, res):
+    """SKL_CHOLQR64: the mixed path's algorithm (unpivoted CholeskyQR of the
+    column-equilibrated FP64 sketch, the same start refinement and fallbacks)
+    with the Gram matrix in FP64, so that mixed vs this isolates what FP32
+    arithmetic buys; FP64 vs this is the algorithm swap (pivoted HQRRP or the
+    Householder QR against CholeskyQR)."""
+    for _ in range(max(1, args.warmup)):
+        solve(skl.SKL_CHOLQR64)
+    torch.cuda.synchronize()
+    steps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        info = solve(skl.SKL_CHOLQR64)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    xr = res["fp64"]["x"]
+    out = {"sec_per_solve": sec, "it": info.it, "flag": info.flag, "escalated": info.escalated,
+           "phases_s": {k: getattr(info, k) for k in
+                        ["t_pattern", "t_sketch", "t_qr", "t_x0", "t_lsqr", "t_total"]},
+           "x_rel_vs_fp64": ((x - xr).norm() / xr.norm()).item()}
+    mixed = res["mixed"]
+    out["mixed_speedup_vs_fp64_gram"] = sec / mixed["sec_per_solve"]
+    out["factor_phase_speedup_vs_fp64_gram"] = (info.t_qr / mixed["phases"]["t_qr"]
+                                                if mixed["phases"]["t_qr"] > 0 else None)
+    out["fp64_speedup_from_algorithm"] = res["fp64"]["sec_per_solve"] / sec
+    log(f"precision ablation (CholeskyQR, FP64 Gram): {sec * 1e3:.2f} ms/solve, it={info.it}, "
+        f"qr {info.t_qr * 1e3:.2f} ms; mixed is {out['mixed_speedup_vs_fp64_gram']:.3f}x faster")
+    return out
+
+
 def run_dense(args, skl, torch, W):
     n, d = W["n"], W["d"]
     log(f"generating {n}x{d} dense problem on the GPU")
@@ -372,6 +402,7 @@ def run_dense(args, skl, torch, W):
             f"flag={info.flag}, phases={res[name]['phases']}")
         log(f"{name} per step (wall ms, in-solver ms): "
             f"{[(round(a * 1e3, 1), round(b.t_total * 1e3, 1)) for a, b in zip(lat, infos)]}")
+    res["cholqr64"] = precision_ablation(args, skl, torch, solve, x, res)
     res["prof"] = profile_pass(skl, torch, solve)
     # reference solution by Householder QR (rocSOLVER through torch).  For this
     # row-weighted problem the recomputed ||A^T r|| / (||A|| ||r||) sits near
@@ -463,6 +494,7 @@ def run_sparse(args, skl, torch, W):
             f"flag={info.flag}, rank={info.rank}, phases={res[name]['phases']}")
         log(f"{name} per step (wall ms, in-solver ms): "
             f"{[(round(a * 1e3, 1), round(b.t_total * 1e3, 1)) for a, b in zip(lat, infos)]}")
+    res["cholqr64"] = precision_ablation(args, skl, torch, solve, x, res)
     res["prof"] = profile_pass(skl, torch, solve)
     # reference solution: dense Householder QR of A on the GPU (4.8 GB in FP64)
     xq = None
@@ -688,6 +720,9 @@ def solver_summary(res, n, d, m, nnz=None, hqrrp=False):
                   "x_rel_vs_fp64": res["x_rel_mixed_vs_fp64"], "phases_s": mixed["phases"],
                   "phase_rooflines": phase_rooflines(n, d, m, mixed["phases"], mixed["it"], nnz,
                                                      mixed=True)},
+        # the same CholeskyQR with an FP64 Gram: mixed vs this is the precision
+        # effect alone, fp64 vs this the algorithm swap
+        "precision_ablation_cholqr_fp64_gram": {k: v for k, v in res["cholqr64"].items()},
     }
 
 
@@ -774,6 +809,7 @@ def main():
             "p99_latency_s": summ["fp64"]["p99_latency_s"],
             "phases_s": f64["phases"], "phase_rooflines": summ["fp64"]["phase_rooflines"],
             "mixed": summ["mixed"],
+            "precision_ablation_cholqr_fp64_gram": summ["precision_ablation_cholqr_fp64_gram"],
             **({"other_entry_points": res["other"]} if "other" in res else {}),
             "roofline": dominant_roofline(res, args.workload),
         })

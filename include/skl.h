@@ -46,6 +46,11 @@ extern "C" {
                        solvers with a sketch-and-solve start sketch in FP64
                        and refine the start against that sketch; the _nocpqr
                        solver sketches in FP32. */
+#define SKL_CHOLQR64 2 /* ablation of SKL_MIXED's precision: the same
+                          CholeskyQR preconditioner, start refinement and
+                          fallbacks, with the equilibrated sketch and its Gram
+                          matrix in FP64 (FP64 MFMA) and an FP64 sketch on
+                          every solver.  No counterpart in the reference. */
 
 typedef struct skl_opts {
     double gamma;           /* over-sampling ratio, m = ceil(gamma d)          */
@@ -60,7 +65,7 @@ typedef struct skl_opts {
     int debug;              /* accepted for compatibility, ignored             */
     int wisdom;             /* accepted for compatibility, ignored             */
     unsigned seed;          /* seed of the libc-compatible generator           */
-    int precision;          /* SKL_FP64 or SKL_MIXED                           */
+    int precision;          /* SKL_FP64, SKL_MIXED or SKL_CHOLQR64             */
     double escalate_cond;   /* MIXED: condition estimate that forces FP64 QR   */
     int use_graph;          /* capture the LSQR loop in a hipGraph (1)         */
     int graph_chunk;        /* LSQR iterations per graph launch                */

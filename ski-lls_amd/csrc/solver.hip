@@ -268,6 +268,13 @@ __global__ void k_equil_to_float(long m, long d, const T *SA, long ldsa, const d
     for (long j = blockIdx.y; j < d; j += gridDim.y)
         W[i + j * m] = (float)((double)SA[i + j * ldsa] * invD[j]);
 }
+template <class T>
+__global__ void k_equil_to_double(long m, long d, const T *SA, long ldsa, const double *invD,
+                                  double *W) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    for (long j = blockIdx.y; j < d; j += gridDim.y) W[i + j * m] = (double)SA[i + j * ldsa] * invD[j];
+}
 // R = R1 diag(D), M = diag(1/D) M1 (d x d, ld d)
 __global__ void k_unequilibrate(long d, const double *R1, const double *M1, const double *D,
                                 const double *invD, double *R, double *M) {
@@ -410,8 +417,11 @@ static void precond_spectrum(long m, long d, const double *SA, long ldsa, const 
 // Returns false on that breakdown; dg (host) receives diag(R), maxn the
 // largest column norm of SA.
 template <class T>
+// gram64: the Gram matrix of the equilibrated sketch in FP64 (the precision
+// ablation SKL_CHOLQR64) instead of FP32; everything else alike.
 static bool mixed_cholqr(long m, long d, const T *SA, long ldsa, double *Rd, double *Mi,
-                         std::vector<double> &dg, double &maxn, skl_info &info, Arena &ar) {
+                         std::vector<double> &dg, double &maxn, skl_info &info, Arena &ar,
+                         bool gram64 = false) {
     hipStream_t s = ctx().stream;
     const size_t mark = ar.used();
     double *sq = ar.alloc_n<double>((size_t)d);
@@ -419,13 +429,19 @@ static bool mixed_cholqr(long m, long d, const T *SA, long ldsa, double *Rd, dou
     double *invD = ar.alloc_n<double>((size_t)d);
     hipLaunchKernelGGL(k_colnorm2<T>, dim3((unsigned)d), dim3(256), 0, s, m, d, SA, ldsa, sq);
     hipLaunchKernelGGL(k_norms_from_squares, dim3(gblocks(d)), dim3(256), 0, s, d, sq, D, invD);
-    float *W = ar.alloc_n<float>((size_t)m * d);
-    hipLaunchKernelGGL(k_equil_to_float<T>, dim3(gblocks(m), (unsigned)std::min<long>(d, 65535)),
-                       dim3(256), 0, s, m, d, SA, ldsa, invD, W);
-    float *G = ar.alloc_n<float>((size_t)d * d);
-    gemm<float>('T', 'N', d, d, m, 1.0f, W, m, W, m, 0.0f, G, d, 1, 0, 0, 0, true);
     double *G64 = ar.alloc_n<double>((size_t)d * d);
-    copy2d<float, double>(d, d, G, d, G64, d);
+    const dim3 eg(gblocks(m), (unsigned)std::min<long>(d, 65535));
+    if (gram64) {
+        double *W = ar.alloc_n<double>((size_t)m * d);
+        hipLaunchKernelGGL(k_equil_to_double<T>, eg, dim3(256), 0, s, m, d, SA, ldsa, invD, W);
+        gemm<double>('T', 'N', d, d, m, 1.0, W, m, W, m, 0.0, G64, d, 1, 0, 0, 0, true);
+    } else {
+        float *W = ar.alloc_n<float>((size_t)m * d);
+        hipLaunchKernelGGL(k_equil_to_float<T>, eg, dim3(256), 0, s, m, d, SA, ldsa, invD, W);
+        float *G = ar.alloc_n<float>((size_t)d * d);
+        gemm<float>('T', 'N', d, d, m, 1.0f, W, m, W, m, 0.0f, G, d, 1, 0, 0, 0, true);
+        copy2d<float, double>(d, d, G, d, G64, d);
+    }
     double *R1 = ar.alloc_n<double>((size_t)d * d);
     bool ok = cholesky_upper(d, G64, d, R1, d, ar);
     std::vector<double> hD((size_t)d);
@@ -571,7 +587,8 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
         gen_hashing(g, m, n, o.k, P);
         build_buckets(P);
     }
-    bool mixed = (o.precision == SKL_MIXED);
+    const bool gram64 = (o.precision == SKL_CHOLQR64);
+    bool mixed = (o.precision == SKL_MIXED) || gram64;
     DevPattern dp = upload_pattern(P, subsample ? 1.0 : 1 / std::sqrt((double)o.k), true, false,
                                    mixed, ar);
     double T1 = wall();
@@ -642,9 +659,9 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
         // sketches in FP32.
         float *W = nullptr;
         double t0 = wall();
-        if (cpqr) {
+        if (cpqr || gram64) {
             W64 = ar.alloc_n<double>((size_t)m * ncol);
-            dense_sketch<double>(n, d, m, dA, lda, db, dp, W64, ar);
+            dense_sketch<double>(n, d, m, dA, lda, cpqr ? db : nullptr, dp, W64, ar);
         } else {
             W = ar.alloc_n<float>((size_t)m * d);
             dense_sketch<float>(n, d, m, dA, lda, nullptr, dp, W, ar);
@@ -656,8 +673,8 @@ static int dense_impl(long n, long d, const double *dA, long lda, const double *
         Rd = ar.alloc_n<double>((size_t)d * d);
         std::vector<double> dg;
         double maxn = 0.0;
-        const bool ok = cpqr ? mixed_cholqr<double>(m, d, W64, m, Rd, Mi, dg, maxn, info, ar)
-                             : mixed_cholqr<float>(m, d, W, m, Rd, Mi, dg, maxn, info, ar);
+        const bool ok = W64 ? mixed_cholqr<double>(m, d, W64, m, Rd, Mi, dg, maxn, info, ar, gram64)
+                            : mixed_cholqr<float>(m, d, W, m, Rd, Mi, dg, maxn, info, ar);
         info.t_qr += wall() - t1;
         rank = d;
         if (ok && cpqr) {
@@ -901,7 +918,8 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
     Pattern P;
     gen_hashing(g, m, n, o.k, P);
     build_buckets(P);
-    bool mixed = (o.precision == SKL_MIXED);
+    const bool gram64 = (o.precision == SKL_CHOLQR64);
+    bool mixed = (o.precision == SKL_MIXED) || gram64;
     DevPattern dp = upload_pattern(P, 1.0, false, true, false, ar);
     double T1 = wall();
     info.t_pattern = T1 - T0;
@@ -936,7 +954,7 @@ static int sparse_impl(long n, long d, long nnz, const long *dcp, const long *dr
         Rd = ar.alloc_n<double>((size_t)d * d);
         std::vector<double> dg;
         double maxn = 0.0;
-        const bool ok = mixed_cholqr<double>(m, d, W64, m, Rd, Mi, dg, maxn, info, ar);
+        const bool ok = mixed_cholqr<double>(m, d, W64, m, Rd, Mi, dg, maxn, info, ar, gram64);
         bool short_rank = !ok || m < d;
         if (!short_rank) {
             tol = (double)std::max(m, d) * (double)FLT_EPSILON * maxn;

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "libskl.so")
 
 SKL_FP64 = 0
 SKL_MIXED = 1
+SKL_CHOLQR64 = 2  # precision ablation: SKL_MIXED's algorithm with an FP64 Gram
 
 # include/bench_config.hpp:7-20
 IT_TOL = 1e-6

@@ -164,3 +164,20 @@ def test_sparse_ill_conditioned_r11_perturbed_operator(skl, O):
     r_min = np.linalg.norm(Ad @ sl.lstsq(Ad, b, lapack_driver="gelss")[0] - b)
     r, ro = np.linalg.norm(Ad @ x - b), np.linalg.norm(Ad @ xo - b)
     assert r > r_min * (1 + 1e-3) and abs(r - ro) <= 1e-6 * ro
+
+
+def test_precision_ablation_cholqr64(skl, O):
+    """SKL_CHOLQR64 (the mixed solver's CholeskyQR with its Gram in FP64, a
+    benchmarking ablation): the same x as the oracle on the 40000x4000 and
+    120000x5000 families at reduced size, with the oracle's iteration count."""
+    A, b = c2_problem(3, n=8000, d=800)
+    xo, io, _ = O.ls_dense(A, b, seed=17, **DENSE_KW)
+    x, info = skl.dense_solve(A, b, cpqr=True, seed=17, precision=skl.SKL_CHOLQR64, **DENSE_KW)
+    assert info.flag == 0 and info.escalated == 0 and abs(info.it - io.it) <= 2
+    assert rel(x, xo) < 1e-8
+    As, bs = sparse_problem(7, n=24000, d=1000, density=0.01, decades=8)
+    kw = dict(gamma=1.5, k=3, it_tol=1e-10, seed=1)
+    xo, io, _ = O.ls_sparse(As, bs, **kw)
+    x, info = skl.sparse_solve(As, bs, precision=skl.SKL_CHOLQR64, **kw)
+    assert info.flag == 0 and info.escalated == 0 and abs(info.it - io.it) <= 2
+    assert rel(x, xo) < 1e-8
