@@ -578,25 +578,37 @@ def make_pool(W, seed):
     return pool
 
 
-def run_serving(args, skl, W, requests, rates):
+def run_serving(args, skl, W, requests, rates, resident=False, pool=None):
     """The serving loop (continuous batching over worker slots, FP32
     preconditioner, FP64 LSQR) against sequential plain-FP64 solves of the same
     request stream, both measured: Poisson arrivals at each rate (a multiple of
-    one sequential FP64 worker's capacity), host inputs (PCIe included).  The
-    sequential run is one thread that takes the requests in arrival order and
-    solves each when it has arrived and the previous one is done; latency is
-    completion minus arrival in both runs."""
-    log("generating the request pool")
-    pool = make_pool(W, seed=77)
+    one sequential FP64 worker's capacity).  Inputs: host arrays (PCIe
+    included), or with `resident` every pool problem uploaded once as a handle
+    (skl_upload_dense / skl_upload_csc) that both runs solve on, so only b and
+    x cross PCIe.  The sequential run is one thread that takes the requests in
+    arrival order and solves each when it has arrived and the previous one is
+    done; latency is completion minus arrival in both runs."""
+    if pool is None:
+        log("generating the request pool")
+        pool = make_pool(W, seed=77)
     rng = np.random.default_rng(78)
     which = rng.integers(0, len(pool), size=requests)
+    handles = []
+    if resident:
+        for P in pool:
+            P["h"] = skl.Handle(P["A"]) if P["kind"] == "dense" else skl.Handle(Acsc=P["A"])
+            handles.append(P["h"])
 
     def solve(P, precision):
+        if resident:
+            return P["h"].solve(P["b"], precision=precision, **P["opts"])
         if P["kind"] == "dense":
             return skl.dense_solve(P["A"], P["b"], precision=precision, **P["opts"])
         return skl.sparse_solve(P["A"], P["b"], precision=precision, **P["opts"])
 
     def submit(S, P, precision):
+        if resident:
+            return S.submit_handle(P["h"], P["b"], precision=precision, **P["opts"])
         if P["kind"] == "dense":
             return S.submit_dense(P["A"], P["b"], precision=precision, **P["opts"])
         return S.submit_sparse(P["A"], P["b"], precision=precision, **P["opts"])
@@ -667,6 +679,12 @@ def run_serving(args, skl, W, requests, rates):
         for tk in burst:
             S.wait(tk)
         out["saturated_server_mixed_solves_per_s"] = requests / (time.perf_counter() - t1)
+    for h in handles:
+        h.release()
+    for P in pool:
+        P.pop("h", None)
+    out["inputs"] = ("resident: A uploaded once per problem (skl_upload_*), b and x host"
+                     if resident else "host arrays, A copied to the device per request")
     out["pool"] = {"problems": len(pool),
                    "dense": sum(P["kind"] == "dense" for P in pool),
                    "sparse": sum(P["kind"] == "sparse" for P in pool),
@@ -749,6 +767,9 @@ def main():
     ap.add_argument("--no-residual", action="store_true",
                     help="skip the extended-precision normal-equation residuals")
     ap.add_argument("--requests", type=int, default=2000, help="serving: requests in the stream")
+    ap.add_argument("--inputs", default="both", choices=["host", "resident", "both"],
+                    help="serving: host arrays per request, problems resident on the device "
+                         "(handles), or both runs")
     ap.add_argument("--slots", type=int, default=4, help="serving: worker slots")
     ap.add_argument("--rates", default="0.5,0.8,1.2",
                     help="serving: arrival rates as multiples of one sequential FP64 worker's "
@@ -772,7 +793,16 @@ def main():
     metric = "sec/solve to rel. normal-eq residual 1e-10, mixed vs FP64 on GPU; p99 latency"
     if W["kind"] == "serving":
         rates = [float(r) for r in args.rates.split(",")]
-        sv = run_serving(args, skl, W, args.requests, rates)
+        log("generating the request pool")
+        pool = make_pool(W, seed=77)
+        if args.inputs in ("host", "both"):
+            sv = run_serving(args, skl, W, args.requests, rates, pool=pool)
+            if args.inputs == "both":
+                log("the same stream with every problem resident on the device (handles)")
+                sv["resident_inputs"] = run_serving(args, skl, W, args.requests, rates,
+                                                    resident=True, pool=pool)
+        else:
+            sv = run_serving(args, skl, W, args.requests, rates, resident=True, pool=pool)
         mid = sv["rates"][len(rates) // 2]
         print(json.dumps({
             "metric": metric, "value": mid["server_mixed"]["p99_latency_s"],
