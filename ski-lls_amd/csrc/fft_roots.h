@@ -1,4 +1,4 @@
-// fft_roots.h -- exp(-2 pi i e / R) for the generic small-radix butterflies,
+// fft_roots.h -- exp(-2 pi i e / R) for the radix-3 and radix-5 butterflies,
 // as exact-to-rounding double literals (generated with 30-digit arithmetic).
 #pragma once
 
@@ -11,17 +11,5 @@ template <> struct RootTab<3> {
 template <> struct RootTab<5> {
     static constexpr double c[5] = {1, 0.30901699437494745, -0.80901699437494745, -0.80901699437494745, 0.30901699437494745};
     static constexpr double s[5] = {0, -0.95105651629515353, -0.58778525229247314, 0.58778525229247314, 0.95105651629515353};
-};
-template <> struct RootTab<7> {
-    static constexpr double c[7] = {1, 0.62348980185873348, -0.22252093395631439, -0.90096886790241915, -0.90096886790241915, -0.22252093395631439, 0.62348980185873348};
-    static constexpr double s[7] = {0, -0.7818314824680298, -0.97492791218182362, -0.43388373911755812, 0.43388373911755812, 0.97492791218182362, 0.7818314824680298};
-};
-template <> struct RootTab<11> {
-    static constexpr double c[11] = {1, 0.84125353283118121, 0.41541501300188644, -0.14231483827328514, -0.6548607339452851, -0.95949297361449737, -0.95949297361449737, -0.6548607339452851, -0.14231483827328514, 0.41541501300188644, 0.84125353283118121};
-    static constexpr double s[11] = {0, -0.54064081745559756, -0.90963199535451833, -0.98982144188093268, -0.75574957435425827, -0.28173255684142967, 0.28173255684142967, 0.75574957435425827, 0.98982144188093268, 0.90963199535451833, 0.54064081745559756};
-};
-template <> struct RootTab<13> {
-    static constexpr double c[13] = {1, 0.88545602565320991, 0.56806474673115581, 0.12053668025532305, -0.35460488704253562, -0.74851074817110108, -0.97094181742605201, -0.97094181742605201, -0.74851074817110108, -0.35460488704253562, 0.12053668025532305, 0.56806474673115581, 0.88545602565320991};
-    static constexpr double s[13] = {0, -0.46472317204376856, -0.82298386589365635, -0.99270887409805397, -0.93501624268541483, -0.66312265824079519, -0.23931566428755777, 0.23931566428755777, 0.66312265824079519, 0.93501624268541483, 0.99270887409805397, 0.82298386589365635, 0.46472317204376856};
 };
 } // namespace skl

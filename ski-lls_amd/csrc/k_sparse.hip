@@ -88,24 +88,32 @@ __global__ void k_row_count(long nnz, const int32_t *rowind, int32_t *cnt) {
     long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p < nnz) atomicAdd(&cnt[rowind[p]], 1);
 }
+// Entries land in their row's slots in arrival order (atomics); each keeps its
+// CSC position so that the ranking below can order duplicates.
 __global__ __launch_bounds__(256) void k_csc_scatter(long d, const int32_t *colptr,
                                                      const int32_t *rowind, const double *val,
-                                                     int32_t *cur, int32_t *ccol, double *cval) {
+                                                     int32_t *cur, int32_t *ccol, double *cval,
+                                                     int32_t *csrc) {
     long j = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (j >= d) return;
     for (int p = colptr[j] + (threadIdx.x & 63); p < colptr[j + 1]; p += 64) {
         int pos = atomicAdd(&cur[rowind[p]], 1);
         ccol[pos] = (int32_t)j;
         cval[pos] = val[p];
+        csrc[pos] = p;
     }
 }
 // One wave per row: every entry goes to the position given by its rank in the
-// row (the number of entries with a smaller column, or an equal column and a
-// smaller scatter index, so duplicates keep a fixed order).  The keys are read
-// 64 at a time into registers and broadcast with readlane.
+// row: the number of entries with a smaller column, or with the same column
+// (a duplicate (i, j) of the CSC input) and a smaller CSC position.  The
+// result depends on the input only, not on the order the scatter's atomics
+// happened to take.  The keys are read 64 at a time into registers and
+// broadcast with readlane; the positions are compared only in rows that hold
+// a duplicate.
 __global__ __launch_bounds__(256) void k_row_sort_rank(long n, const int32_t *__restrict__ ptr,
                                                        const int32_t *__restrict__ tcol,
                                                        const double *__restrict__ tval,
+                                                       const int32_t *__restrict__ tsrc,
                                                        int32_t *__restrict__ ccol,
                                                        double *__restrict__ cval) {
     const long i = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -116,13 +124,28 @@ __global__ __launch_bounds__(256) void k_row_sort_rank(long n, const int32_t *__
         const int e = e0 + lane;
         const int key = e < L ? tcol[p0 + e] : INT_MAX;
         const double v = e < L ? tval[p0 + e] : 0.0;
-        int rank = 0;
+        int rank = 0, eq = 0;
         for (int q0 = 0; q0 < L; q0 += 64) {
             const int kl = q0 + lane < L ? tcol[p0 + q0 + lane] : INT_MAX;
             const int qn = min(64, L - q0);
             for (int q = 0; q < qn; ++q) {
                 const int kq = __builtin_amdgcn_readlane(kl, q);
-                rank += (kq < key) || (kq == key && q0 + q < e);
+                rank += kq < key;
+                eq += (kq == key) && (q0 + q != e);
+            }
+        }
+        if (__ballot(e < L && eq > 0)) {
+            // duplicates in this row: order equal columns by CSC position
+            const int src = e < L ? tsrc[p0 + e] : INT_MAX;
+            for (int q0 = 0; q0 < L; q0 += 64) {
+                const int kl = q0 + lane < L ? tcol[p0 + q0 + lane] : INT_MAX;
+                const int sl = q0 + lane < L ? tsrc[p0 + q0 + lane] : INT_MAX;
+                const int qn = min(64, L - q0);
+                for (int q = 0; q < qn; ++q) {
+                    const int kq = __builtin_amdgcn_readlane(kl, q);
+                    const int sq = __builtin_amdgcn_readlane(sl, q);
+                    rank += (kq == key) && (sq < src);
+                }
             }
         }
         if (e < L) {
@@ -145,8 +168,11 @@ void csc_to_csr(long n, long d, long nnz, const int32_t *colptr, const int32_t *
     const size_t mark = ar.used();
     int32_t *tcol = ar.alloc_n<int32_t>((size_t)std::max<long>(nnz, 1));
     double *tval = ar.alloc_n<double>((size_t)std::max<long>(nnz, 1));
-    if (d > 0) k_csc_scatter<<<(unsigned)((d + 3) / 4), 256, 0, s>>>(d, colptr, rowind, val, cur, tcol, tval);
-    k_row_sort_rank<<<(unsigned)((n + 3) / 4), 256, 0, s>>>(n, rptr, tcol, tval, ccol, cval);
+    int32_t *tsrc = ar.alloc_n<int32_t>((size_t)std::max<long>(nnz, 1));
+    if (d > 0)
+        k_csc_scatter<<<(unsigned)((d + 3) / 4), 256, 0, s>>>(d, colptr, rowind, val, cur, tcol,
+                                                              tval, tsrc);
+    k_row_sort_rank<<<(unsigned)((n + 3) / 4), 256, 0, s>>>(n, rptr, tcol, tval, tsrc, ccol, cval);
     SKL_CHECK_LAUNCH();
     ar.rewind(mark);
 }

@@ -212,3 +212,35 @@ def test_qr_panel_fallback_on_dependent_columns(skl, pivot):
     R = np.triu(W[:n])
     assert np.abs(Q.T @ Q - np.eye(m)).max() < 1e-12
     assert rel(A[:, P], Q[:, :n] @ R) < 1e-13
+
+
+def test_csc_duplicate_entries_repeatable(skl, O):
+    """CSC input with duplicate (i, j) entries (each split value stored twice in
+    its column): the CSR copy orders duplicates by CSC position, so repeated
+    solves agree bit for bit; the solution is the one of the matrix the
+    duplicates sum to."""
+    import scipy.sparse as sp
+    from conftest import rel, sparse_problem
+    A, b = sparse_problem(31, n=6000, d=200, density=0.02)
+    A = sp.csc_matrix(A)
+    rng = np.random.default_rng(2)
+    cols, rows, vals = [], [], []
+    for j in range(A.shape[1]):
+        r = A.indices[A.indptr[j]:A.indptr[j + 1]]
+        v = A.data[A.indptr[j]:A.indptr[j + 1]]
+        split = rng.random(len(r)) < 0.3
+        w = rng.uniform(0.2, 0.8, len(r))
+        rr = np.concatenate([r, r[split]])
+        vv = np.concatenate([np.where(split, v * w, v), (v * (1 - w))[split]])
+        rows.append(rr)
+        vals.append(vv)
+        cols.append(len(rr))
+    indptr = np.concatenate([[0], np.cumsum(cols)])
+    Ad = sp.csc_matrix((np.concatenate(vals), np.concatenate(rows), indptr), shape=A.shape)
+    assert Ad.nnz > A.nnz  # duplicates kept (no sum_duplicates)
+    kw = dict(gamma=1.5, k=3, it_tol=1e-10, seed=1)
+    x1, i1 = skl.sparse_solve(Ad, b, **kw)
+    x2, i2 = skl.sparse_solve(Ad, b, **kw)
+    assert np.array_equal(x1, x2) and i1.it == i2.it
+    xo, io, _ = O.ls_sparse(A, b, **kw)
+    assert rel(x1, xo) < 1e-8

@@ -81,3 +81,50 @@ def test_test_solvers_script(kat):
             assert abs(r - ref) <= 1e-5 * max(1.0, ref) + 1e-10, (blk, r, ref)
         checked += 1
     assert checked == 12
+
+
+def test_cpp_examples_run(kat, tmp_path):
+    """A C++ caller written against the reference's ski-lls.h API (the two
+    API examples' data: ski-llsDenseExample.cpp, ski-llsSparseExample.cpp)
+    builds against include/ and libskl.so and prints the examples' known
+    answers."""
+    e, s = kat["dense_example"], kat["sparse_example"]
+    fmt = lambda v: ",".join(repr(float(t)) for t in v)
+    src = tmp_path / "examples.cpp"
+    src.write_text(f"""
+#include <cstdio>
+#include "ski-lls.h"
+int main() {{
+    double a[] = {{{fmt(e["A"])}}}, b[] = {{{fmt(e["b"])}}};
+    Mat_d A({e["m"]}, {e["n"]}, a);
+    Vec_d bv({e["m"]}, b), x(A.n());
+    long it, rank; int flag;
+    ls_dense_hashing_blendenpik(A, bv, x, rank, flag, it, 1.1, 3, ABS_TOL, 1e-10, 1e-6, MAX_IT, 0, 0);
+    for (long i = 0; i < x.n(); ++i) std::printf("dense %.17g\\n", x(i));
+    long cp[] = {{{",".join(map(str, s["colptr"]))}}}, ri[] = {{{",".join(map(str, s["rowind"]))}}};
+    double va[] = {{{fmt(s["val"])}}}, bs[] = {{{fmt(s["b"])}}};
+    cs_dl S = {{{len(s["val"])}, {s["m"]}, {s["n"]}, cp, ri, va, -1}};
+    Vec_d bsv({s["m"]}, bs), xs({s["n"]});
+    ls_sparse_spqr(S, bsv, xs, rank, flag, it, 1.1, 3, ABS_TOL, SPQR_ORDERING, 1e-6);
+    for (long i = 0; i < xs.n(); ++i) std::printf("sparse %.17g\\n", xs(i));
+    return 0;
+}}
+""")
+    exe = tmp_path / "examples"
+    lib = os.path.join(ROOT, "ski-lls_amd")
+    subprocess.run(["g++", "-std=c++11", "-O1", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-o", str(exe), os.path.join(lib, "libskl.so"), f"-Wl,-rpath,{lib}"],
+                   check=True, timeout=120)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120,
+                         env=dict(os.environ, SKL_SEED="1"))
+    assert out.returncode == 0, out.stderr
+    got = {"dense": [], "sparse": []}
+    for line in out.stdout.split("\n"):
+        if line.strip():
+            k, v = line.split()
+            got[k].append(float(v))
+    assert len(got["dense"]) == e["n"] and len(got["sparse"]) == s["n"], out.stdout
+    for u, v in zip(got["dense"], e["x"]):
+        assert abs(u - v) < 1e-6, (got["dense"], e["x"])
+    for u, v in zip(got["sparse"], s["x"]):
+        assert abs(u - v) < 1e-6, (got["sparse"], s["x"])
