@@ -767,6 +767,9 @@ def main():
     ap.add_argument("--no-residual", action="store_true",
                     help="skip the extended-precision normal-equation residuals")
     ap.add_argument("--requests", type=int, default=2000, help="serving: requests in the stream")
+    ap.add_argument("--autotune", action="store_true",
+                    help="serving: tune the LSQR launch shapes of the pool's buckets first "
+                         "(both runs then use them)")
     ap.add_argument("--inputs", default="both", choices=["host", "resident", "both"],
                     help="serving: host arrays per request, problems resident on the device "
                          "(handles), or both runs")
@@ -795,6 +798,24 @@ def main():
         rates = [float(r) for r in args.rates.split(",")]
         log("generating the request pool")
         pool = make_pool(W, seed=77)
+        tuned = None
+        if args.autotune:
+            # LSQR launch shapes for the pool's buckets, before the stream
+            log("autotuning the LSQR launch shapes of the pool's buckets")
+            skl.autotune_clear()
+            t0 = time.perf_counter()
+            per = []
+            for P in pool:
+                nnz = P["A"].nnz if P["kind"] == "sparse" else None
+                best, dflt = skl.autotune_lsqr(P["n"], P["d"], nnz)
+                per.append({"kind": P["kind"], "n": P["n"], "d": P["d"], "nnz": nnz,
+                            "ms_per_it_tuned": best, "ms_per_it_default": dflt})
+            tuned = {"buckets": skl.autotune_entries(), "tuning_s": time.perf_counter() - t0,
+                     "per_problem": per,
+                     "geomean_speedup_per_iteration": float(np.exp(np.mean(
+                         [np.log(q["ms_per_it_default"] / q["ms_per_it_tuned"]) for q in per])))}
+            log(f"  {tuned['buckets']} buckets in {tuned['tuning_s']:.1f} s, LSQR iteration "
+                f"{tuned['geomean_speedup_per_iteration']:.3f}x faster (geometric mean)")
         if args.inputs in ("host", "both"):
             sv = run_serving(args, skl, W, args.requests, rates, pool=pool)
             if args.inputs == "both":
@@ -803,6 +824,8 @@ def main():
                                                     resident=True, pool=pool)
         else:
             sv = run_serving(args, skl, W, args.requests, rates, resident=True, pool=pool)
+        if tuned is not None:
+            sv["autotune"] = tuned
         mid = sv["rates"][len(rates) // 2]
         print(json.dumps({
             "metric": metric, "value": mid["server_mixed"]["p99_latency_s"],

@@ -209,6 +209,22 @@ int skl_solve_handle_dev(skl_handle *h, int method, const double *db, double *dx
                          const skl_opts *opts, skl_info *info);
 int skl_release(skl_handle *h);
 
+/* ---- LSQR launch shapes: autotuner --------------------------------------------
+ * No counterpart in the reference.  The LSQR kernels' launch shapes (the
+ * workgroup count the two-pass products aim at, workgroups per CU of the
+ * one-pass dense kernel, z staged in LDS for the sparse product) come from a
+ * process-wide table keyed by problem-shape bucket (half octaves of n, d and,
+ * for CSC, nnz/n); buckets not in it use the defaults.  skl_autotune_lsqr
+ * times every candidate shape on a synthetic problem of the given shape (32
+ * LSQR iterations replayed as graphs, on the calling thread's device) and
+ * records the fastest; best_ms / default_ms: ms per iteration of the chosen
+ * and the default shape.  Shapes change the order of partial sums, so x can
+ * differ in the last bits (within the LSQR tolerance) between a tuned and an
+ * untuned bucket; with a fixed table solves stay bitwise reproducible. */
+int skl_autotune_lsqr(int sparse, long n, long d, long nnz, double *best_ms, double *default_ms);
+long skl_autotune_entries(void);
+void skl_autotune_clear(void);
+
 /* ---- serving: continuous batching over worker slots -------------------------
  * No counterpart in the reference (it solves one problem per call); this is
  * the request loop that callers with a stream of problems use.  Each slot is
@@ -237,6 +253,10 @@ long skl_server_submit_handle(skl_server *s, skl_handle *h, int method, const do
 /* blocks until the request finished; queue_s / run_s: time queued / solving */
 int skl_server_wait(skl_server *s, long ticket, skl_info *info, double *queue_s, double *run_s);
 int skl_server_pending(skl_server *s);
+/* on != 0: a slot that takes a request whose LSQR shape bucket the table lacks
+ * runs skl_autotune_lsqr for it first (once per bucket; requests of a bucket
+ * being tuned wait for it), so every request of a bucket uses one shape */
+void skl_server_set_autotune(skl_server *s, int on);
 void skl_server_destroy(skl_server *s);
 
 /* ---- component entry points (host pointers; each runs one stage) -----------

@@ -13,10 +13,12 @@
 // convergence flag is set; kernels of iterations past convergence exit at
 // entry.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <tuple>
 #include <vector>
 #include <type_traits>
@@ -25,6 +27,11 @@
 #include <hip/hip_ext.h>
 
 #include <cstdio>
+
+#include <array>
+#include <cmath>
+#include <map>
+#include <mutex>
 
 #include "k_common.h"
 
@@ -802,15 +809,63 @@ static bool rows_fit(const LsqrProblem &p, Arena &ar) {
     return need * 1.25 + 256.0 * (1 << 20) <= avail;
 }
 
+// ---- launch shapes (autotuned per problem-shape bucket) ----------------------
+// target_mult: the workgroup count the two-pass products aim at, in multiples
+// of the CU count; wpc: workgroups per CU of the one-pass kernel; spmv_lds:
+// the sparse A z with z staged in LDS.  Defaults: 8, 1, 1.
+struct LsqrShape {
+    int target_mult = 8, wpc = 1, spmv_lds = 1;
+};
+using ShapeKey = std::array<int, 4>;
+static std::mutex g_shape_mu;
+static std::map<ShapeKey, LsqrShape> g_shapes;
+static thread_local const LsqrShape *t_shape_override = nullptr;
+
+// buckets of half an octave in n, d and (sparse) the nonzeros per row
+static ShapeKey shape_key(bool sparse, long n, long d, long nnz) {
+    auto lg = [](double v) { return (int)std::floor(2.0 * std::log2(std::max(v, 1.0))); };
+    return {sparse ? 1 : 0, lg((double)n), lg((double)d),
+            sparse ? lg((double)nnz / (double)std::max(n, 1L)) : 0};
+}
+
+static LsqrShape shape_for(const LsqrProblem &p) {
+    if (t_shape_override) return *t_shape_override;
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    auto it = g_shapes.find(shape_key(p.csr_ptr != nullptr, p.n, p.d, p.nnz));
+    return it == g_shapes.end() ? LsqrShape() : it->second;
+}
+
+bool lsqr_shape_tuned(bool sparse, long n, long d, long nnz) {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    return g_shapes.count(shape_key(sparse, n, d, nnz)) != 0;
+}
+
+long lsqr_shape_entries() {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    return (long)g_shapes.size();
+}
+
+void lsqr_shape_clear() {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    g_shapes.clear();
+}
+
+long long lsqr_shape_bucket(bool sparse, long n, long d, long nnz) {
+    const ShapeKey k = shape_key(sparse, n, d, nnz);
+    return ((long long)k[0] << 48) | ((long long)(k[1] & 0xffff) << 32) |
+           ((long long)(k[2] & 0xffff) << 16) | (long long)(k[3] & 0xffff);
+}
+
 static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
     LsqrPlan q;
     q.n = p.n;
     q.d = p.d;
     q.r = p.r;
     q.sparse = (p.csr_ptr != nullptr);
+    const LsqrShape shape = shape_for(p);
     long rowblocks = (p.n + GN_ROWS - 1) / GN_ROWS;
     // enough workgroups to fill the chip several times over
-    long target = 8L * num_cu;
+    long target = (long)shape.target_mult * num_cu;
     long nck = std::max<long>(1, std::min<long>((target + rowblocks - 1) / rowblocks,
                                                 (p.d + 63) / 64));
     q.cw_a = (int)std::min<long>(GN_MAXCW, (p.d + nck - 1) / nck);
@@ -834,7 +889,8 @@ static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
         const size_t zb = sizeof(double) * (size_t)p.d;
         const int per_cu = zb <= 40960 ? 4 : zb <= 81920 ? 2 : zb <= 163840 ? 1 : 0;
         const char *e = getenv("SKL_SPMV_LDS");
-        if (per_cu > 0 && !(e && e[0] == '0')) q.spg = std::min(q.nsb, per_cu * num_cu);
+        if (per_cu > 0 && shape.spmv_lds && !(e && e[0] == '0'))
+            q.spg = std::min(q.nsb, per_cu * num_cu);
     }
     if (fused_wanted(p)) {
         q.fused = true;
@@ -843,7 +899,7 @@ static LsqrPlan make_plan(const LsqrProblem &p, int num_cu) {
         // one workgroup per CU (the registers allow four up to nc = 8, but one
         // keeps as many bytes in flight with the fewest partials: 0.210, 0.222
         // and 0.236 ms per pass at 40000 x 4000 for one, two and four)
-        int wpc = 1;
+        int wpc = shape.wpc;
         const char *e = getenv("SKL_FUSED_WPC");
         if (e && atoi(e) > 0) wpc = atoi(e);
         q.G = (int)std::min<long>((long)wpc * num_cu, p.n);
@@ -1233,7 +1289,10 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
                  (long)(intptr_t)B.t, (long)(intptr_t)B.Sb, (long)(intptr_t)B.Pz, B.hist_cap,
                  (long)(intptr_t)B.Sv, (long)(intptr_t)B.cnt,
                  q.fused ? 1L : 0L, chunk, (long)(intptr_t)s, prof ? 1L : 0L,
-                 (long)q.spg, (long)(intptr_t)B.AT, (long)q.G};
+                 (long)q.spg, (long)(intptr_t)B.AT, (long)q.G,
+                 // and the launch shape
+                 (long)q.cw_a, (long)q.nck_a, (long)q.cw_m, (long)q.nck_m, (long)q.nrb_a,
+                 (long)q.cpw_a, (long)q.nrb_m, (long)q.cpw_m, (long)q.nsb, (long)q.nc};
         auto it = g_graphs.find(key);
         if (it != g_graphs.end()) {
             exec = it->second;
@@ -1360,6 +1419,148 @@ void dev_gemv_t_dense(long n, long d, const double *A, long lda, const double *u
     k_gemv_t_part<2><<<g, GT_THREADS, 0, s>>>(n, d, A, lda, u, cpw, P, 0, nullptr);
     k_axpby_part<<<(unsigned)((d + 255) / 256), 256, 0, s>>>(d, (int)nrb, P, alpha, beta, y);
     SKL_CHECK_LAUNCH();
+}
+
+// ---- the autotuner -------------------------------------------------------------
+// deterministic fill in [-1, 1) (splitmix64 of seed + index)
+__global__ void k_fill_hash(long count, double *x, unsigned long long seed) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    x[i] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+// M: upper triangular r x r with unit diagonal and small entries above it
+__global__ void k_tune_upper(long r, double *M) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long j = blockIdx.y;
+    for (; j < r; j += gridDim.y)
+        if (i < r) M[i + j * r] = i < j ? M[i + j * r] * (0.1 / (double)r) : (i == j ? 1.0 : 0.0);
+}
+
+// Times every candidate launch shape on a synthetic problem of the given shape
+// (the LSQR iteration as a solve runs it: `iters` iterations replayed as
+// graphs, the operator A M with a dense upper triangular M) and stores the
+// fastest for the shape's bucket.  Returns the chosen shape's time per
+// iteration (ms); *default_ms: the default shape's.
+double lsqr_autotune(bool sparse, long n, long d, long nnz, int iters, double *default_ms) {
+    Context &C = ctx();
+    SKL_REQUIRE(!C.no_alloc, SKL_E_UNSUPPORTED, "autotuning allocates device memory");
+    SKL_REQUIRE(n >= d && d > 0 && (!sparse || (nnz > 0 && nnz <= n * d && nnz < 2147483647L)),
+                SKL_E_ARG, "invalid autotuning shape");
+    Arena &ar = C.arena;
+    ar.reset();
+    hipStream_t s = C.stream;
+    LsqrProblem p;
+    p.n = n;
+    p.d = d;
+    p.r = d;
+    double *M = ar.alloc_n<double>((size_t)d * d);
+    k_fill_hash<<<(unsigned)(((long)d * d + 255) / 256), 256, 0, s>>>((long)d * d, M, 11);
+    k_tune_upper<<<dim3((unsigned)((d + 255) / 256), (unsigned)std::min<long>(d, 4096)), 256, 0, s>>>(
+        d, M);
+    p.M = M;
+    p.ldm = d;
+    if (!sparse) {
+        double *A = ar.alloc_n<double>((size_t)n * d);
+        k_fill_hash<<<(unsigned)(((long)n * d + 255) / 256), 256, 0, s>>>((long)n * d, A, 7);
+        p.A = A;
+        p.lda = n;
+    } else {
+        // nnz / d distinct rows per column, by a multiplicative hash of the
+        // column, sorted
+        const long per = std::max<long>(1, nnz / d);
+        std::vector<int32_t> cp((size_t)d + 1), ri;
+        ri.reserve((size_t)(per * d));
+        std::vector<int32_t> col;
+        for (long j = 0; j < d; ++j) {
+            cp[(size_t)j] = (int32_t)ri.size();
+            col.clear();
+            unsigned long long h = 0x2545F4914F6CDD1Dull * (unsigned long long)(j + 1);
+            for (long t = 0; t < per; ++t) {
+                h ^= h << 13;
+                h ^= h >> 7;
+                h ^= h << 17;
+                col.push_back((int32_t)(h % (unsigned long long)n));
+            }
+            std::sort(col.begin(), col.end());
+            col.erase(std::unique(col.begin(), col.end()), col.end());
+            ri.insert(ri.end(), col.begin(), col.end());
+        }
+        cp[(size_t)d] = (int32_t)ri.size();
+        const long nz = (long)ri.size();
+        int32_t *dcp = ar.alloc_n<int32_t>((size_t)d + 1);
+        int32_t *dri = ar.alloc_n<int32_t>((size_t)nz);
+        double *dv = ar.alloc_n<double>((size_t)nz);
+        SKL_HIP(hipMemcpyAsync(dcp, cp.data(), sizeof(int32_t) * (d + 1), hipMemcpyHostToDevice, s));
+        SKL_HIP(hipMemcpyAsync(dri, ri.data(), sizeof(int32_t) * nz, hipMemcpyHostToDevice, s));
+        k_fill_hash<<<(unsigned)((nz + 255) / 256), 256, 0, s>>>(nz, dv, 5);
+        int32_t *rp = ar.alloc_n<int32_t>((size_t)n + 1);
+        int32_t *cc = ar.alloc_n<int32_t>((size_t)nz);
+        double *cv = ar.alloc_n<double>((size_t)nz);
+        csc_to_csr(n, d, nz, dcp, dri, dv, rp, cc, cv, ar);
+        p.csr_ptr = rp;
+        p.csr_col = cc;
+        p.csr_val = cv;
+        p.csc_ptr = dcp;
+        p.csc_row = dri;
+        p.csc_val = dv;
+        p.nnz = nz;
+        SKL_HIP(hipStreamSynchronize(s)); // the host arrays go out of scope
+    }
+    double *b = ar.alloc_n<double>((size_t)n);
+    k_fill_hash<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, b, 3);
+    double *y = ar.alloc_n<double>((size_t)d);
+    SKL_CHECK_LAUNCH();
+    std::vector<LsqrShape> cand;
+    const bool fused = !sparse && lsqr_rows_wanted(n, d);
+    for (int tm : {4, 8, 16, 2}) {
+        if (fused) {
+            if (tm == 2) continue;
+            for (int w : {1, 2}) cand.push_back({tm, w, 1});
+        } else if (sparse) {
+            if (tm == 2) continue;
+            for (int l : {1, 0}) cand.push_back({tm, 1, l});
+        } else {
+            cand.push_back({tm, 1, 1});
+        }
+    }
+    const size_t mark = ar.used();
+    double best = INFINITY, def = INFINITY;
+    LsqrShape chosen;
+    for (const LsqrShape &c : cand) {
+        t_shape_override = &c;
+        double ms = INFINITY;
+        try {
+            for (int rep = 0; rep < 2; ++rep) {
+                ar.rewind(mark);
+                SKL_HIP(hipStreamSynchronize(s));
+                const double t0 = wall();
+                lsqr_solve(p, b, 0.0, iters, y, true, 16, nullptr, 0, ar);
+                SKL_HIP(hipStreamSynchronize(s));
+                ms = (wall() - t0) * 1e3 / iters; // the second repetition is kept
+            }
+        } catch (...) {
+            t_shape_override = nullptr;
+            throw;
+        }
+        t_shape_override = nullptr;
+        if (c.target_mult == 8 && c.wpc == 1 && c.spmv_lds == 1) def = ms;
+        if (ms < best) {
+            best = ms;
+            chosen = c;
+        }
+    }
+    ar.rewind(mark);
+    lsqr_release_graphs(); // the synthetic problem's graphs
+    {
+        std::lock_guard<std::mutex> lk(g_shape_mu);
+        g_shapes[shape_key(sparse, n, d, sparse ? nnz : 0)] = chosen;
+    }
+    if (default_ms) *default_ms = def;
+    return best;
 }
 
 } // namespace skl

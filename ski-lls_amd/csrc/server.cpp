@@ -26,6 +26,7 @@
 #include <string>
 #include <map>
 #include <mutex>
+#include <set>
 #include <thread>
 #include <vector>
 
@@ -98,6 +99,9 @@ struct skl_server {
     // request it runs (0 when idle)
     std::vector<size_t> retained, running;
     int busy = 0;
+    // LSQR shape autotuning of new buckets before their first request
+    bool autotune = false;
+    std::set<long long> tuning; // buckets a slot is tuning now
     bool stop = false;
     long next_ticket = 1;
     std::mutex mu;
@@ -136,6 +140,29 @@ static int run_request(Request *r) {
     }
 }
 
+// The request's LSQR shape bucket is tuned before it runs: by this slot when
+// no slot is tuning it, else after waiting for the slot that is.
+static void tune_for(skl_server *S, Request *r) {
+    const bool sparse = r->kind == SPARSE || (r->kind == HANDLE && r->method == SKL_METHOD_SPARSE_SPQR);
+    if (skl::lsqr_shape_tuned(sparse, r->n, r->d, r->nnz)) return;
+    const long long key = skl::lsqr_shape_bucket(sparse, r->n, r->d, r->nnz);
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        if (S->tuning.count(key)) {
+            S->cv_done.wait(lk, [&] { return S->tuning.count(key) == 0; });
+            return;
+        }
+        S->tuning.insert(key);
+    }
+    double best = 0, def = 0;
+    (void)skl_autotune_lsqr(sparse ? 1 : 0, r->n, r->d, r->nnz, &best, &def);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->tuning.erase(key);
+    }
+    S->cv_done.notify_all();
+}
+
 static void worker_main(skl_server *S, int self) {
     skl_init(S->device);
     while (true) {
@@ -158,6 +185,7 @@ static void worker_main(skl_server *S, int self) {
             S->running[(size_t)self] = r->footprint;
             S->busy++;
         }
+        if (S->autotune) tune_for(S, r);
         r->started = clk::now();
         skl::ctx().mem_allowance = r->footprint;
         r->rc = run_request(r);
@@ -317,6 +345,12 @@ int skl_server_wait(skl_server *S, long ticket, skl_info *info, double *queue_s,
     if (rc != SKL_OK) skl::set_last_error(r->err);
     delete r;
     return rc;
+}
+
+void skl_server_set_autotune(skl_server *S, int on) {
+    if (!S) return;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->autotune = on != 0;
 }
 
 int skl_server_pending(skl_server *S) {

@@ -316,6 +316,18 @@ bool lsqr_start_rows(LsqrProblem &p, Arena &ar);
 // copy (by shape; no device query); the copy's row stride; the copy itself (AT: n x lsqr_rows_ld(d)),
 // queued on stream s
 bool lsqr_rows_wanted(long n, long d);
+// Launch shapes of the LSQR kernels per problem-shape bucket (half octaves of
+// n, d and, sparse, nnz / n): lsqr_autotune times the candidates on a
+// synthetic problem of the shape and records the fastest (ms per iteration
+// returned; *default_ms the default shape's); later solves in the bucket use
+// it.  lsqr_shape_bucket: the bucket's id.
+double lsqr_autotune(bool sparse, long n, long d, long nnz, int iters, double *default_ms);
+bool lsqr_shape_tuned(bool sparse, long n, long d, long nnz);
+long lsqr_shape_entries();
+void lsqr_shape_clear();
+long long lsqr_shape_bucket(bool sparse, long n, long d, long nnz);
+void csc_to_csr(long n, long d, long nnz, const int32_t *colptr, const int32_t *rowind,
+                const double *val, int32_t *rptr, int32_t *ccol, double *cval, Arena &ar);
 long lsqr_rows_ld(long d);
 void lsqr_make_rows(long n, long d, const double *A, long lda, double *AT, hipStream_t s);
 

@@ -1607,6 +1607,23 @@ int skl_release(skl_handle *h) {
     });
 }
 
+// ---- LSQR launch-shape autotuner --------------------------------------------
+int skl_autotune_lsqr(int sparse, long n, long d, long nnz, double *best_ms, double *default_ms) {
+    return guarded([&] {
+        ensure_init();
+        ArenaTrim trim(ctx().arena);
+        double def = 0.0;
+        const double best = lsqr_autotune(sparse != 0, n, d, nnz, 32, &def);
+        if (best_ms) *best_ms = best;
+        if (default_ms) *default_ms = def;
+        return SKL_OK;
+    });
+}
+
+long skl_autotune_entries(void) { return lsqr_shape_entries(); }
+
+void skl_autotune_clear(void) { lsqr_shape_clear(); }
+
 // ---- component entry points ------------------------------------------------
 int skl_hash_pattern(unsigned seed, int dense, long m, long n, long k, double *D, long *rows,
                      double *vals) {

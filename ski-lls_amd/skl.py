@@ -73,6 +73,7 @@ EXPORTS = [
     "skl_dense_solve_dev_noalloc",
     "skl_upload_dense", "skl_upload_dense_dev", "skl_upload_csc", "skl_handle_query",
     "skl_solve_handle", "skl_solve_handle_dev", "skl_release",
+    "skl_autotune_lsqr", "skl_autotune_entries", "skl_autotune_clear", "skl_server_set_autotune",
     "skl_server_create", "skl_server_submit_dense", "skl_server_submit_sparse",
     "skl_server_submit_dense_dev", "skl_server_submit_sparse_dev", "skl_server_submit_handle",
     "skl_server_wait", "skl_server_pending", "skl_server_destroy",
@@ -280,6 +281,26 @@ def ls_sparse_spqr(A, b, gamma, k, abs_tol=ABS_TOL, ordering=SPQR_ORDERING, it_t
     return x, info.rank, info.flag, info.it
 
 
+# ---- LSQR launch-shape autotuner ---------------------------------------------------
+def autotune_lsqr(n, d, nnz=None):
+    """Tune the LSQR launch shape of the (n, d[, nnz]) bucket (CSC when nnz is
+    given) -> (ms per iteration chosen, ms per iteration of the default)."""
+    best, dflt = C.c_double(0), C.c_double(0)
+    _check(lib().skl_autotune_lsqr(C.c_int(0 if nnz is None else 1), C.c_long(n), C.c_long(d),
+                                   C.c_long(0 if nnz is None else nnz), C.byref(best),
+                                   C.byref(dflt)))
+    return best.value, dflt.value
+
+
+def autotune_entries():
+    lib().skl_autotune_entries.restype = C.c_long
+    return lib().skl_autotune_entries()
+
+
+def autotune_clear():
+    lib().skl_autotune_clear()
+
+
 # ---- device-resident problems -----------------------------------------------------
 class Handle:
     """A problem kept in device memory (skl_upload_dense / skl_upload_csc):
@@ -448,6 +469,10 @@ class Server:
 
     def pending(self):
         return lib().skl_server_pending(self._h)
+
+    def set_autotune(self, on=True):
+        """Tune each new LSQR shape bucket before its first request runs."""
+        lib().skl_server_set_autotune(self._h, C.c_int(1 if on else 0))
 
     def close(self):
         if self._h:
