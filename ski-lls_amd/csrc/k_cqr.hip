@@ -153,8 +153,9 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
     __shared__ double Rb[64 * CQ_LD]; // row k of R by panel column; later X = R^-1
     __shared__ double Rr[64 * CQ_LD]; // R in logical order
     __shared__ double Ys[64 * CQ_LD];
-    __shared__ double rowbuf[2][64];
-    __shared__ double s_g0[64];
+    // rowbuf[buf]: the pivot row with zeros in the retired columns and in the
+    // pivot's own, its diagonal at [64]
+    __shared__ double rowbuf[2][65];
     __shared__ int s_perm[64];
     const int tid = threadIdx.x, p = tid & 63, g = tid >> 6;
     if (pass == 2 && *flag != 0) return;
@@ -166,15 +167,18 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
     }
     for (int e = tid; e < 64 * CQ_LD; e += CQ_T) Rb[e] = 0.0;
     // every wave keeps, for column p in lane p, its Gram diagonal, partial
-    // norms and logical position (the same values in every wave), so each wave
-    // finds the pivot itself: one barrier per step
+    // norms, logical position and whether it is still active (the same values
+    // in every wave), so each wave finds the pivot itself: one barrier per step
     double dg = p < b ? G[p + (long)p * ldg] : 0.0;
+    const double g0 = dg; // the column's squared norm (lane p)
     double vn1 = sqrt(fmax(dg, 0.0)), vn2 = vn1;
     int pos = p < b ? p : 0x7fffffff;
-    if (g == 0) s_g0[p] = dg;
-    unsigned long long act = b >= 64 ? ~0ull : ((1ull << b) - 1ull);
+    bool mine = p < b;
+    // active flags of this thread's columns g + 4 m, bit m
+    unsigned amask = 0;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) amask |= (g + 4 * m < b) ? (1u << m) : 0u;
     const double tol3z = sqrt(1.1102230246251565e-16);
-    __syncthreads();
     int k = 0;
     for (; k < b; ++k) {
         const int buf = k & 1;
@@ -182,7 +186,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
         if (PIV) {
             // largest partial norm among the columns left, first logical
             // position on ties
-            const double bv = (((act >> p) & 1ull) && vn1 == vn1) ? fabs(vn1) : -1.0;
+            const double bv = (mine && vn1 == vn1) ? fabs(vn1) : -1.0;
             int bl = wave_argmax(bv, pos);
             if (bl < 0) bl = __builtin_ctzll(__ballot(pos == k)); // no finite candidate
             pk = bl;
@@ -194,31 +198,35 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
                 pos = lp;
             if (tid == 0 && swp) swp[k] = lp;
         }
+        // retire the pivot column: its flag in lane pk, its bit in the threads
+        // that hold it
+        const double g0k = readlane(g0, pk);
+        if (p == pk) mine = false;
+        if ((pk & 3) == g) amask &= ~(1u << (pk >> 2));
         if (p == pk) {
 #pragma unroll
-            for (int m = 0; m < 16; ++m) rowbuf[buf][g + 4 * m] = sv[m];
-        }
-        __syncthreads();
-        const double d = rowbuf[buf][pk];
-        // not positive, or the column's partial norm below 1e-4 of its norm
-        if (!(d > 0.0) || !(d >= 1e-8 * s_g0[pk])) break; // the same in every thread
-        const double inv = rsqrt_nr(d), rkk = d * inv;
-        act &= ~(1ull << pk);
-        const bool mine = (act >> p) & 1ull;
-        const double rp = rowbuf[buf][p] * inv; // R(k, column p)
-        {
-            // branch-free, so that the sixteen LDS reads are issued together
-            double rq[16];
+            for (int m = 0; m < 16; ++m) rowbuf[buf][g + 4 * m] = ((amask >> m) & 1u) ? sv[m] : 0.0;
+            if ((pk & 3) == g) {
+                double dk = 0.0;
 #pragma unroll
-            for (int m = 0; m < 16; ++m) rq[m] = rowbuf[buf][g + 4 * m];
-            const double rpm = mine ? rp : 0.0;
-#pragma unroll
-            for (int m = 0; m < 16; ++m) {
-                const int q = g + 4 * m;
-                const double f = ((act >> q) & 1ull) ? rq[m] * inv : 0.0;
-                sv[m] -= rpm * f;
+                for (int m = 0; m < 16; ++m)
+                    if (m == (pk >> 2)) dk = sv[m];
+                rowbuf[buf][64] = dk;
             }
         }
+        __syncthreads();
+        const double d = rowbuf[buf][64];
+        const double rowp = rowbuf[buf][p];
+        double rq[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) rq[m] = rowbuf[buf][g + 4 * m];
+        // not positive, or the column's partial norm below 1e-4 of its norm
+        if (!(d > 0.0) || !(d >= 1e-8 * g0k)) break; // the same in every thread
+        const double inv = rsqrt_nr(d), rkk = d * inv;
+        const double rp = rowp * inv; // R(k, column p) for an active p
+        const double c = mine ? rp * inv : 0.0;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) sv[m] = fma(-c, rq[m], sv[m]);
         if (g == 0) Rb[k + p * CQ_LD] = p == pk ? rkk : (mine ? rp : 0.0);
         if (mine) {
             dg -= rp * rp;
