@@ -212,7 +212,9 @@ int skl_release(skl_handle *h);
 /* ---- LSQR launch shapes: autotuner --------------------------------------------
  * No counterpart in the reference.  The LSQR kernels' launch shapes (the
  * workgroup count the two-pass products aim at, workgroups per CU of the
- * one-pass dense kernel, z staged in LDS for the sparse product) come from a
+ * one-pass dense kernel, z staged in LDS for the sparse product, and for
+ * rank <= 256 with at most 128 rows per workgroup on half the CUs whether the
+ * iterations run as one persistent launch and on how many workgroups) come from a
  * process-wide table keyed by problem-shape bucket (half octaves of n, d and,
  * for CSC, nnz/n); buckets not in it use the defaults.  skl_autotune_lsqr
  * times every candidate shape on a synthetic problem of the given shape (32

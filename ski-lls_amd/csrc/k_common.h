@@ -125,6 +125,15 @@ template <class T> __device__ inline T wave_max_bcast(T v) {
     v = fmax(v, dpp_move<DPP_MIRROR>(v));
     return fmax(fmax(readlane(v, 0), readlane(v, 16)), fmax(readlane(v, 32), readlane(v, 48)));
 }
+// sum over the wave, the same value in every lane (same lane moves as
+// wave_max_bcast; a fixed order, so equal inputs give equal sums everywhere)
+template <class T> __device__ inline T wave_sum_bcast(T v) {
+    v += dpp_move<DPP_XOR1>(v);
+    v += dpp_move<DPP_XOR2>(v);
+    v += dpp_move<DPP_HALF_MIRROR>(v);
+    v += dpp_move<DPP_MIRROR>(v);
+    return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
+}
 __device__ inline int wave_min_bcast(int k) {
     k = min(k, dpp_move<DPP_XOR1>(k));
     k = min(k, dpp_move<DPP_XOR2>(k));

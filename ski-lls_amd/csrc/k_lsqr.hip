@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -654,6 +655,299 @@ __global__ __launch_bounds__(UV_T) void k_update_wx(long r, int nsv, const doubl
     }
 }
 
+// ---- persistent LSQR (small problems) ----------------------------------------------
+// For problems whose preconditioned operator B = A(:, E) M (n x r) fits in
+// the registers of at most half the CUs (r <= 256, at most 128 rows per
+// workgroup), every iteration of a solve runs in one launch of P workgroups
+// with ONE grid barrier per iteration:
+//   each workgroup holds its rows of B in registers (lane = row, wave w =
+//   columns w, w + 16, ...) and its rows of ut; before the barrier it forms
+//   ut = B v - coef ut for its rows, their sum of squares (Sb) and its
+//   partial B^T ut over its rows (Pt); after the barrier every workgroup sums
+//   the P partials in the same order and so holds the same beta, t = B^T u,
+//   v' = t - beta v, alpha = ||v'||, the Givens step and x, w, v (kept in
+//   LDS, identical bits in every workgroup; workgroup 0 writes x and the
+//   state at the end).  Pt and Sb alternate between two halves by iteration
+//   parity: a workgroup writing the next iteration's partials never touches
+//   those that a slower one may still read.
+// B is formed once per solve (k_precond_op), so the iteration reads no M and,
+// beyond the P x r partials, no global memory.  The recurrences, the stopping
+// test and the history are those of the launch-per-kernel iteration; the
+// operator is applied as one product with B in place of M then A, which moves
+// the rounding by about one unit of B's entries.
+constexpr int PS_T = 1024, PS_W = PS_T / 64, PS_KPW = 16, PS_RMAX = PS_W * PS_KPW;
+struct PersistArgs {
+    long n, r, rb; // rows, rank, rows per workgroup (<= 64 * RCH)
+    const double *Bm;
+    long ldb;
+    double *ut, *v, *w, *x, *Pt, *Sb, *hist;
+    long hist_cap;
+    LsqrState *st;
+    GridSync bar;
+};
+
+// Sum over the 64 lanes of each of 16 values; lane l returns the sum of value
+// (l >> 2) & 15.  Halving exchanges: a lane keeps the half of its values that
+// its bit selects and adds the partner's copy of that half; the partners are
+// lane ^ 32 and lane ^ 16 (LDS permutes), then lane mirrored within its 16
+// and within its 8 (DPP; both differ in the bit of the step and agree in the
+// bits already used), then two butterfly steps over the quad.  The order is
+// fixed, so every call sums the same way.
+template <int H, int BIT, int CTRL>
+__device__ inline void halve_step(double (&p)[PS_KPW], int lane) {
+    const bool up = (lane & BIT) != 0;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const double keep = up ? p[i + H] : p[i];
+        const double send = up ? p[i] : p[i + H];
+        p[i] = keep + (CTRL < 0 ? __shfl_xor(send, BIT, 64) : dpp_move<(CTRL < 0 ? 0 : CTRL)>(send));
+    }
+}
+__device__ inline double wave_sum16(double (&p)[PS_KPW], int lane) {
+    static_assert(PS_KPW == 16, "wave_sum16 reduces 16 values");
+    halve_step<8, 32, -1>(p, lane);
+    halve_step<4, 16, -1>(p, lane);
+    halve_step<2, 8, DPP_MIRROR>(p, lane);
+    halve_step<1, 4, DPP_HALF_MIRROR>(p, lane);
+    double s = p[0];
+    s += dpp_move<DPP_XOR2>(s);
+    s += dpp_move<DPP_XOR1>(s);
+    return s;
+}
+
+template <int RCH>
+__global__ __launch_bounds__(PS_T) void k_lsqr_persist(PersistArgs a) {
+    __shared__ double vs[PS_RMAX], wl[PS_RMAX], xl[PS_RMAX], tl[4][PS_RMAX];
+    __shared__ double red[RCH][PS_W][64], wred[2][PS_W];
+    const int P = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+    const int wv = tid >> 6, lane = tid & 63;
+    const long r = a.r;
+    const LsqrState s0 = *a.st;
+    if (s0.done) return;
+    // this workgroup's rows of B and ut
+    double bb[RCH][PS_KPW], u[RCH];
+    bool ok[RCH];
+#pragma unroll
+    for (int ch = 0; ch < RCH; ++ch) {
+        const long lr = (long)ch * 64 + lane, i = (long)g * a.rb + lr;
+        ok[ch] = lr < a.rb && i < a.n;
+#pragma unroll
+        for (int c = 0; c < PS_KPW; ++c) {
+            const long k = wv + (long)PS_W * c;
+            bb[ch][c] = (ok[ch] && k < r) ? a.Bm[i + k * a.ldb] : 0.0;
+        }
+        u[ch] = ok[ch] ? a.ut[i] : 0.0;
+    }
+    for (int k = tid; k < PS_RMAX; k += PS_T) {
+        vs[k] = k < r ? a.v[k] : 0.0;
+        wl[k] = k < r ? a.w[k] : 0.0;
+        xl[k] = k < r ? a.x[k] : 0.0;
+    }
+    __syncthreads();
+    double alpha = s0.alpha, beta = s0.beta, rhobar = s0.rhobar, phibar = s0.phibar;
+    double nrm_a = s0.nrm_a, nrm_r = s0.nrm_r, nrm_ar = s0.nrm_ar, coef = s0.coef;
+    long it = s0.it;
+    int flag = 1;
+    unsigned target = 0;
+    for (int par = 0;; par ^= 1) {
+        double *Pt = a.Pt + (size_t)par * P * r;
+        double *Sb = a.Sb + (size_t)par * P;
+        // ut = B v - coef ut over this workgroup's rows
+#pragma unroll
+        for (int ch = 0; ch < RCH; ++ch) {
+            double f = 0.0;
+#pragma unroll
+            for (int c = 0; c < PS_KPW; ++c) f += bb[ch][c] * vs[wv + PS_W * c];
+            red[ch][wv][lane] = f;
+        }
+        __syncthreads();
+        double sq = 0.0;
+#pragma unroll
+        for (int ch = 0; ch < RCH; ++ch) {
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < PS_W; ++q) s += red[ch][q][lane];
+            u[ch] = ok[ch] ? s - coef * u[ch] : 0.0;
+            sq += u[ch] * u[ch];
+        }
+        // partial B^T ut over the rows, one column per four lanes
+        double p[PS_KPW];
+#pragma unroll
+        for (int c = 0; c < PS_KPW; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int ch = 0; ch < RCH; ++ch) s += bb[ch][c] * u[ch];
+            p[c] = s;
+        }
+        const double pc = wave_sum16(p, lane);
+        const long kc = wv + (long)PS_W * ((lane >> 2) & 15);
+        if ((lane & 3) == 0 && kc < r) Pt[(long)g * r + kc] = pc;
+        if (wv == 0) {
+            sq = wave_sum_bcast(sq);
+            if (lane == 0) Sb[g] = sq;
+        }
+        if (!grid_barrier(a.bar, target)) return;
+        // every workgroup: beta, t = B^T u, v', alpha (same order everywhere);
+        // the partial sums of squares load in the same round as those of B^T ut
+        const double sbv = tid < P ? Sb[tid] : 0.0;
+        {
+            const int k = tid & (PS_RMAX - 1), h = tid / PS_RMAX;
+            double s = 0.0;
+            if (k < r) {
+                const double *pt = Pt + k;
+                const int rr = (int)r;
+#pragma unroll 8
+                for (int q = h; q < P; q += PS_T / PS_RMAX) s += pt[q * rr];
+            }
+            tl[h][k] = s;
+        }
+        {
+            const double ws = wave_sum_bcast(sbv);
+            if (lane == 0) wred[0][wv] = ws;
+        }
+        __syncthreads();
+        double sb = 0.0;
+#pragma unroll
+        for (int q = 0; q < PS_W; ++q) sb += wred[0][q];
+        beta = sqrt(sb);
+        const bool upd = beta > 0;
+        double vp = 0.0, sv = 0.0;
+        if (tid < r) {
+            const double t = ((tl[0][tid] + tl[1][tid]) + (tl[2][tid] + tl[3][tid])) / beta;
+            vp = upd ? t - beta * vs[tid] : vs[tid];
+            sv = upd ? vp * vp : 0.0;
+        }
+        {
+            const double ws = wave_sum_bcast(sv);
+            if (lane == 0) wred[1][wv] = ws;
+        }
+        __syncthreads();
+        sv = 0.0;
+#pragma unroll
+        for (int q = 0; q < PS_W; ++q) sv += wred[1][q];
+        const double alpha_old = alpha;
+        if (upd) {
+            alpha = sqrt(sv);
+            nrm_a = sqrt(nrm_a * nrm_a + alpha_old * alpha_old + beta * beta);
+        }
+        const double sc = (upd && alpha > 0) ? 1.0 / alpha : 1.0;
+        const double rho = sqrt(rhobar * rhobar + beta * beta);
+        const double cs = rhobar / rho;
+        const double sn = beta / rho;
+        const double theta = sn * alpha;
+        rhobar = -cs * alpha;
+        const double phi = cs * phibar;
+        phibar = sn * phibar;
+        const double c1 = phi / rho, c2 = -theta / rho;
+        if (tid < r) {
+            const double vn = sc * vp, wj = wl[tid];
+            vs[tid] = vn;
+            xl[tid] = xl[tid] + c1 * wj;
+            wl[tid] = c2 * wj + 1.0 * vn;
+        }
+        ++it;
+        nrm_r = phibar;
+        nrm_ar = phibar * alpha * fabs(cs);
+        coef = upd ? alpha / beta : 0.0;
+        if (g == 0 && tid == 0 && a.hist && it <= a.hist_cap) {
+            double *hh = a.hist + 5 * (it - 1);
+            hh[0] = alpha;
+            hh[1] = beta;
+            hh[2] = rhobar;
+            hh[3] = phibar;
+            hh[4] = nrm_a;
+        }
+        bool done = false;
+        if (nrm_ar < s0.tol * nrm_a * nrm_r) {
+            flag = 0;
+            done = true;
+        } else if (it >= s0.max_it) {
+            flag = 1;
+            done = true;
+        }
+        __syncthreads(); // v, x, w complete before the next product reads them
+        if (done) break;
+    }
+    if (g != 0) return;
+    for (long k = tid; k < r; k += PS_T) {
+        a.x[k] = xl[k];
+        a.v[k] = vs[k];
+        a.w[k] = wl[k];
+    }
+    if (tid == 0) {
+        LsqrState *st = a.st;
+        st->alpha = alpha;
+        st->beta = beta;
+        st->rhobar = rhobar;
+        st->phibar = phibar;
+        st->nrm_a = nrm_a;
+        st->nrm_r = nrm_r;
+        st->nrm_ar = nrm_ar;
+        st->coef = coef;
+        st->it = it;
+        st->flag = flag;
+        st->done = 1;
+    }
+}
+
+// Bm = A(:, E) M (n x r, column-major, ld n) with M upper triangular (its
+// entries below the diagonal are not read); E null is the identity.  64 x 64
+// output tiles, 16-deep steps of A's columns and M's rows through LDS.
+__global__ __launch_bounds__(256) void k_precond_op(long n, long r, const double *__restrict__ A,
+                                                    long lda, const int32_t *__restrict__ E,
+                                                    const double *__restrict__ M, long ldm,
+                                                    double *__restrict__ Bm, long ldb) {
+    __shared__ double As[16][64], Ms[16][65];
+    const long i0 = (long)blockIdx.x * 64, k0 = (long)blockIdx.y * 64;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc[4][4] = {};
+    const long jend = min(r, k0 + 64); // M[j, k] = 0 for j > k
+    for (long j0 = 0; j0 < jend; j0 += 16) {
+        for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+            const int jj = e >> 6, ii = e & 63;
+            const long j = j0 + jj, i = i0 + ii, k = k0 + ii;
+            As[jj][ii] = (j < r && i < n) ? A[i + (E ? (long)E[j] : j) * lda] : 0.0;
+            Ms[jj][ii] = (j < r && k < r && j <= k) ? M[j + k * ldm] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+            double av[4], mv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                av[q] = As[jj][tx + 16 * q];
+                mv[q] = Ms[jj][ty + 16 * q];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) acc[q][s] = fma(av[q], mv[s], acc[q][s]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const long i = i0 + tx + 16 * q, k = k0 + ty + 16 * s;
+            if (i < n && k < r) Bm[i + k * ldb] = acc[q][s];
+        }
+}
+
+// D (n x r, zeroed by the caller) = the columns E of CSR A, compacted:
+// D[i, Einv[j]] = A[i, j] for the columns j that E selects (Einv null: all).
+// Duplicate entries add up in CSR order.  One thread per row.
+__global__ void k_csr_to_dense(long n, const int32_t *__restrict__ rp,
+                               const int32_t *__restrict__ cc, const double *__restrict__ cv,
+                               const int32_t *__restrict__ Einv, double *__restrict__ D) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int q = rp[i]; q < rp[i + 1]; ++q) {
+        const long k = Einv ? (long)Einv[cc[q]] : (long)cc[q];
+        if (k >= 0) D[i + k * n] += cv[q];
+    }
+}
+
 // ---- sparse operator kernels ----------------------------------------------------
 // ut[i] = sum_p val[p] z[col[p]] - coef ut[i] over CSR rows, LPR lanes per
 // row; Sb per block of SP_ROWS rows (enough blocks to cover every CU several
@@ -812,9 +1106,13 @@ static bool rows_fit(const LsqrProblem &p, Arena &ar) {
 // ---- launch shapes (autotuned per problem-shape bucket) ----------------------
 // target_mult: the workgroup count the two-pass products aim at, in multiples
 // of the CU count; wpc: workgroups per CU of the one-pass kernel; spmv_lds:
-// the sparse A z with z staged in LDS.  Defaults: 8, 1, 1.
+// the sparse A z with z staged in LDS; persist: the workgroups of the
+// persistent iteration (k_lsqr_persist), 0 for the launch-per-kernel one.
+// Defaults: 8, 1, 1, and the persistent iteration with its smallest grid
+// (ceil(n / 128) workgroups) for every problem that it takes.
 struct LsqrShape {
     int target_mult = 8, wpc = 1, spmv_lds = 1;
+    int persist = -1; // workgroups of the persistent iteration, 0 off, -1 by size
 };
 using ShapeKey = std::array<int, 4>;
 static std::mutex g_shape_mu;
@@ -833,6 +1131,27 @@ static LsqrShape shape_for(const LsqrProblem &p) {
     std::lock_guard<std::mutex> lk(g_shape_mu);
     auto it = g_shapes.find(shape_key(p.csr_ptr != nullptr, p.n, p.d, p.nnz));
     return it == g_shapes.end() ? LsqrShape() : it->second;
+}
+
+// The persistent iteration takes ranks up to 256 and at most 128 rows per
+// workgroup on at most half the CUs; it runs with the smallest such grid
+// unless the launch shape (or SKL_LSQR_PERSIST) asks for more workgroups.
+static long persist_min_groups(const LsqrProblem &p) { return (p.n + 127) / 128; }
+static bool persist_eligible(const LsqrProblem &p) {
+    return p.r >= 1 && p.r <= PS_RMAX && p.n >= 1 &&
+           persist_min_groups(p) <= std::max(1, ctx().num_cu / 2);
+}
+static int persist_groups(const LsqrProblem &p, const LsqrShape &sh) {
+    if (!persist_eligible(p) || ctx().no_alloc) return 0;
+    // SKL_LSQR_PERSIST=0 turns the persistent iteration off, =P (> 0) asks
+    // for P workgroups on every problem that it takes
+    const char *e = getenv("SKL_LSQR_PERSIST");
+    const long env_p = e ? strtol(e, nullptr, 10) : -1;
+    if (e && env_p <= 0) return 0;
+    long P = env_p > 0 ? env_p : (long)sh.persist;
+    if (P == 0) return 0;
+    P = std::max(P, persist_min_groups(p));
+    return (int)std::min<long>(P, std::max(1, ctx().num_cu / 2));
 }
 
 bool lsqr_shape_tuned(bool sparse, long n, long d, long nnz) {
@@ -1273,6 +1592,53 @@ LsqrResult lsqr_solve(const LsqrProblem &p, const double *b, double tol, long ma
     // timed inside a replayed graph) and the second iteration of every chunk
     // is timed, on a device already busy with the first
     if (prof) graph_ok = false;
+    // small problems: every iteration in one persistent launch
+    int pg = (prof || max_it <= 0) ? 0 : persist_groups(p, shape_for(p));
+    // the barrier count of the launch stays within the 32-bit counter
+    if ((double)max_it * pg >= 2.0e9) pg = 0;
+    if (pg > 0) {
+        graph_ok = false;
+        // the preconditioned operator B = A(:, E) M, formed once
+        double *Bm = ar.alloc_n<double>((size_t)q.n * q.r);
+        const dim3 gb((unsigned)((q.n + 63) / 64), (unsigned)((q.r + 63) / 64));
+        if (q.sparse) {
+            double *D = ar.alloc_n<double>((size_t)q.n * q.r);
+            k_zero_d<<<256, 256, 0, s>>>(q.n * q.r, D);
+            k_csr_to_dense<<<(unsigned)((q.n + 255) / 256), 256, 0, s>>>(
+                q.n, p.csr_ptr, p.csr_col, p.csr_val, B.Einv, D);
+            k_precond_op<<<gb, 256, 0, s>>>(q.n, q.r, D, q.n, nullptr, p.M, p.ldm, Bm, q.n);
+        } else {
+            k_precond_op<<<gb, 256, 0, s>>>(q.n, q.r, p.A, p.lda, p.E, p.M, p.ldm, Bm, q.n);
+        }
+        SKL_CHECK_LAUNCH();
+        PersistArgs pa;
+        pa.n = q.n;
+        pa.r = q.r;
+        pa.rb = (q.n + pg - 1) / pg;
+        pa.Bm = Bm;
+        pa.ldb = q.n;
+        pa.ut = B.ut;
+        pa.v = B.v;
+        pa.w = B.w;
+        pa.x = B.x;
+        pa.Pt = ar.alloc_n<double>((size_t)2 * pg * q.r);
+        pa.Sb = ar.alloc_n<double>((size_t)2 * pg);
+        pa.hist = B.hist;
+        pa.hist_cap = B.hist_cap;
+        pa.st = B.st;
+        pa.bar = grid_sync_fresh(ar);
+        SKL_REQUIRE(pa.rb <= 128, SKL_E_ARG, "persistent LSQR: more than 128 rows per workgroup");
+        {
+            BarrierLaunch guard(pg);
+            if (pa.rb <= 64)
+                k_lsqr_persist<1><<<pg, PS_T, 0, s>>>(pa);
+            else
+                k_lsqr_persist<2><<<pg, PS_T, 0, s>>>(pa);
+            SKL_CHECK_LAUNCH();
+        }
+        SKL_HIP(hipStreamSynchronize(s));
+        check_grid_fault(); // a barrier that gave up leaves the state unfinished
+    }
     if (graph_ok) {
         GraphKey key;
         // every pointer and size baked into the captured kernels' arguments:
@@ -1514,18 +1880,28 @@ double lsqr_autotune(bool sparse, long n, long d, long nnz, int iters, double *d
     k_fill_hash<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, b, 3);
     double *y = ar.alloc_n<double>((size_t)d);
     SKL_CHECK_LAUNCH();
-    std::vector<LsqrShape> cand;
+    // the default shape first, then the launch-per-kernel iteration's
+    // shapes and, for the problems it takes, the persistent iteration's
+    // workgroup counts
+    std::vector<LsqrShape> cand(1);
     const bool fused = !sparse && lsqr_rows_wanted(n, d);
     for (int tm : {4, 8, 16, 2}) {
         if (fused) {
             if (tm == 2) continue;
-            for (int w : {1, 2}) cand.push_back({tm, w, 1});
+            for (int w : {1, 2}) cand.push_back({tm, w, 1, 0});
         } else if (sparse) {
             if (tm == 2) continue;
-            for (int l : {1, 0}) cand.push_back({tm, 1, l});
+            for (int l : {1, 0}) cand.push_back({tm, 1, l, 0});
         } else {
-            cand.push_back({tm, 1, 1});
+            cand.push_back({tm, 1, 1, 0});
         }
+    }
+    if (persist_eligible(p)) {
+        const long cap = std::max(1, C.num_cu / 2), lo = persist_min_groups(p);
+        std::vector<long> pw;
+        for (long P : {lo, (n + 63) / 64, 2 * ((n + 63) / 64)})
+            if (P <= cap && std::find(pw.begin(), pw.end(), P) == pw.end()) pw.push_back(P);
+        for (long P : pw) cand.push_back({8, 1, 1, (int)P});
     }
     const size_t mark = ar.used();
     double best = INFINITY, def = INFINITY;
@@ -1547,7 +1923,7 @@ double lsqr_autotune(bool sparse, long n, long d, long nnz, int iters, double *d
             throw;
         }
         t_shape_override = nullptr;
-        if (c.target_mult == 8 && c.wpc == 1 && c.spmv_lds == 1) def = ms;
+        if (&c == &cand[0]) def = ms;
         if (ms < best) {
             best = ms;
             chosen = c;

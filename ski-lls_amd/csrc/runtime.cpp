@@ -339,7 +339,7 @@ void check_grid_fault() {
         SKL_HIP(hipMemsetAsync(c.dev_err, 0, sizeof(unsigned), c.stream));
         SKL_HIP(hipStreamSynchronize(c.stream));
         throw Error(SKL_E_HIP, "grid barrier gave up waiting for the other workgroups of a "
-                               "factorization kernel; its results were discarded");
+                               "factorization or LSQR kernel; its results were discarded");
     }
 }
 
