@@ -157,6 +157,12 @@ template <class T> __device__ inline int wave_argmax(T v, int key) {
     return __ffsll((long long)__ballot(v == m && key == kmin)) - 1;
 }
 
+__device__ inline int wave_max_int(int k) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) k = max(k, __shfl_xor(k, o, 64));
+    return k;
+}
+
 template <class T> __device__ inline T wave_max(T v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));

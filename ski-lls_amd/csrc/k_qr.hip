@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "k_common.h"
@@ -991,6 +992,7 @@ template <class T> struct YqArgs {
     T *ctau;        // [2][P] tau of the candidate's reflector
     T *cvec;        // [2][P][YQ_MAXROWS] candidate's reflector v (v_jj = 1, zero above)
     GridSync bar;   // barrier counter (zero at launch), error word, poll limit
+    const int32_t *skip = nullptr; // k_yqrp_lanes: nonzero *skip = nothing to do
 };
 
 template <class T>
@@ -1240,23 +1242,144 @@ __global__ __launch_bounds__(64 * W) void k_yqrp(YqArgs<T> a) {
     }
 }
 
-// Column-per-lane variant of k_yqrp for the HQRRP sketch height (74 rows):
-// lane l of wave w of workgroup b holds column (b W + w) 64 + l of Y entirely
-// in registers, with its partial norms and logical position.  The product of
-// a column with the reflector, its update and the norm downdating -- the exact
-// recomputation included -- are loops inside one lane; the only cross-lane
-// work of a step is the argmax.  One grid barrier per pivot, as in k_yqrp:
-// before it each workgroup publishes its best column as a reflector (dlarfg in
-// the lane that holds the column), after it every wave picks the global winner
-// (largest norm, first position on ties) from the staged candidates and
-// applies it.
+// Column-per-lane pivot search for the HQRRP sketch height (74 rows): lane l
+// of a wave holds one column of Y entirely in registers, with its partial
+// norms and logical position.  The product of a column with the reflector,
+// its update and the norm downdating -- the exact recomputation included --
+// are loops inside one lane; the only cross-lane work of a step is the
+// argmax.  Two kernels run these steps with the same per-column arithmetic
+// (so they choose the same pivots bit for bit):
+//  * k_yqrp_top: one workgroup over the (at most 512) columns of largest
+//    initial norm, no grid barrier;
+//  * k_yqrp_lanes: every column, 512 per workgroup, one grid barrier per pivot.
 constexpr int YQ_ROWS = 74; // nb + pp of HQRRP (64 + 10)
+
+// Register layout: during group g of eight pivot steps, y[i] holds row 8 g + i
+// of the column, so at step jj = 8 g + s the pivot row is y[s] (s static), the
+// rows above it are retired and never read again, and the rows at or beyond
+// live = 74 - 8 g are zero.  The column shifts up by eight rows between
+// groups, so every access has a fixed register index.  The loops run over
+// chunks of eight rows and skip the chunks that start at or beyond `live` (a
+// wave-uniform branch; skipping terms that are exact zeros changes no sum).
+constexpr int YQ_CH = 8, YQ_NCH = (YQ_ROWS + YQ_CH - 1) / YQ_CH;
+#define YQ_ROWS_FROM(S, body)                                                                 \
+    _Pragma("unroll") for (int c_ = 0; c_ < YQ_NCH; ++c_) {                                   \
+        if (YQ_CH * c_ + YQ_CH > (S) && YQ_CH * c_ < live) {                                  \
+            _Pragma("unroll") for (int i_ = 0; i_ < YQ_CH; ++i_) {                            \
+                const int r = YQ_CH * c_ + i_;                                                \
+                if (r >= (S) && r < YQ_ROWS) { body }                                         \
+            }                                                                                 \
+        }                                                                                     \
+    }
+
+template <int S> using yq_sub = std::integral_constant<int, S>;
+
+// step(yq_sub<s>{}, jj, live) for jj = 8 g + s < nsteps, then regroup(live)
+// after each group; false when a step says stop
+template <class F, class G>
+__device__ inline bool yq_for_steps(int nsteps, F &&step, G &&regroup) {
+    for (int g = 0; 8 * g < nsteps; ++g) {
+        const int live = YQ_ROWS - 8 * g;
+#define YQ_SUB(S)                                                                             \
+    if (8 * g + (S) < nsteps && !step(yq_sub<(S)>{}, 8 * g + (S), live)) return false;
+        YQ_SUB(0) YQ_SUB(1) YQ_SUB(2) YQ_SUB(3) YQ_SUB(4) YQ_SUB(5) YQ_SUB(6) YQ_SUB(7)
+#undef YQ_SUB
+        regroup(live);
+    }
+    return true;
+}
+
+// sum of squares of the rows below the pivot row (four partial sums)
+template <int S, class T> __device__ inline T yq_tail_sumsq(const T (&y)[YQ_ROWS], int live) {
+    T q[4] = {0, 0, 0, 0};
+    YQ_ROWS_FROM(S + 1, q[(r - S - 1) & 3] = fma(y[r], y[r], q[(r - S - 1) & 3]);)
+    return (q[0] + q[1]) + (q[2] + q[3]);
+}
+
+// y -= tau v (v^T y) for an active column (f = 0 otherwise); lane l holds
+// v[l] and v[l + 64] (rows of the group's frame), and the rows reach every
+// lane as scalar operands (v_readlane).  Every lane runs the loops
+// (v_readlane reads lanes outside EXEC, so the values it reads must have been
+// written by all lanes).
+template <int S, class T>
+__device__ inline void yq_apply(T (&y)[YQ_ROWS], T vlo, T vhi, T tau, bool act, int live) {
+    T g[2] = {0, 0};
+    YQ_ROWS_FROM(S, g[(r - S) & 1] = fma(r < 64 ? readlane(vlo, r) : readlane(vhi, r - 64), y[r],
+                                         g[(r - S) & 1]);)
+    const T f = (act && tau != T(0)) ? -tau * (g[0] + g[1]) : T(0);
+    // read the lanes again rather than keeping 2 x 74 scalars live
+    T wlo = vlo, whi = vhi;
+    asm volatile("" : "+v"(wlo), "+v"(whi));
+    YQ_ROWS_FROM(S, y[r] = fma(r < 64 ? readlane(wlo, r) : readlane(whi, r - 64), f, y[r]);)
+}
+
+// norm downdating of skint.c:920-990 (NoFLA_QRP_downdate_partial_norms) with
+// the updated pivot-row entry y[S], recomputed in the lane when needed
+template <int S, class T>
+__device__ inline void yq_downdate(const T (&y)[YQ_ROWS], T &dn, T &en, bool act, int mY, int jj,
+                                   T tol3z, int live) {
+    if (act && dn != T(0)) {
+        T temp = fabs(y[S]) / dn;
+        temp = (T(1) + temp) * (T(1) - temp);
+        if (temp < T(0)) temp = T(0);
+        const T t5 = dn / en;
+        const T temp2 = temp * t5 * t5;
+        if (temp2 <= tol3z) {
+            T sq = 0;
+            YQ_ROWS_FROM(S + 1, sq = fma(y[r], y[r], sq);)
+            dn = (mY - jj - 1 > 0) ? sqrt(sq) : T(0);
+            en = dn;
+        } else {
+            dn = dn * sqrt(temp);
+        }
+    }
+}
+
+// next group: drop the eight retired rows (rows of skipped chunks are zero
+// and stay so)
+template <class T> __device__ inline void yq_shift8(T (&y)[YQ_ROWS], int live) {
+    YQ_ROWS_FROM(0, y[r] = (r + YQ_CH < YQ_ROWS) ? y[r + YQ_CH] : T(0);)
+}
+
+// a column's registers and initial norms (the same arithmetic in both kernels)
+template <class T>
+__device__ inline void yq_load(T (&y)[YQ_ROWS], T &dn, T &en, const T *Y, long ldy, int col,
+                               bool valid, int mY) {
+    const T *src = Y + (long)(valid ? col : 0) * ldy;
+    T sq = 0;
+#pragma unroll
+    for (int r = 0; r < YQ_ROWS; ++r) {
+        y[r] = (valid && r < mY) ? src[r] : T(0);
+        sq += y[r] * y[r];
+    }
+    dn = sqrt(sq);
+    en = dn;
+}
+
+// the reflector (dlarfg) of the wave's column in lane wl over its live rows
+// jj.. (pivot row y[S]), formed in that lane: the norm from xs, the lane's sum
+// of squares of its rows below the pivot row; v[S] = 1, v[r] = sc y[r] below
+template <int S, class T>
+__device__ inline T yq_reflector(const T (&y)[YQ_ROWS], T xs, int wl, int lane, T *dst, int live) {
+    const T alpha = readlane(y[S], wl);
+    const T xnorm2 = readlane(xs, wl);
+    T tau = 0, sc = 0;
+    if (xnorm2 != T(0)) {
+        const T beta = -copysign(sqrt(fma(alpha, alpha, xnorm2)), alpha);
+        tau = (beta - alpha) / beta;
+        sc = T(1) / (alpha - beta);
+    }
+    if (lane == wl) { YQ_ROWS_FROM(S, dst[r] = r == S ? T(1) : sc * y[r];) }
+    return tau;
+}
 
 template <class T, int W>
 __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
     __shared__ T s_wv[W];
     __shared__ int s_wp[W], s_wl[W];
     extern __shared__ unsigned char yq_smem[];
+    // nothing to do when the single-workgroup search already chose the pivots
+    if (a.skip && *a.skip != 0) return;
     const int P = gridDim.x, blk = blockIdx.x;
     T *s_vec = reinterpret_cast<T *>(yq_smem);      // P x YQ_MAXROWS candidate reflectors
     T *s_cv = s_vec + (size_t)P * YQ_MAXROWS;       // P candidate norms
@@ -1269,40 +1392,15 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
     const T tol3z = sqrt(eps_half<T>());
     const int col = (blk * W + wave) * 64 + lane;
     const bool valid = col < nv;
-    // y[r] is row jj + r of the column at step jj: the rows above the current
-    // pivot row are never read again, so the column shifts up by one row per
-    // step and every access has a fixed register index
     T y[YQ_ROWS];
     T dn, en;
-    {
-        const T *src = a.Y + (long)(valid ? col : 0) * a.ldy;
-        T sq = 0;
-#pragma unroll
-        for (int r = 0; r < YQ_ROWS; ++r) {
-            y[r] = (valid && r < mY) ? src[r] : T(0);
-            sq += y[r] * y[r];
-        }
-        dn = sqrt(sq);
-        en = dn;
-    }
+    yq_load(y, dn, en, a.Y, a.ldy, col, valid, mY);
     int pos = valid ? col : NONE;
-    for (int jj = 0; jj < a.nsteps; ++jj) {
+    auto step = [&](auto Sc, int jj, int live) -> bool {
+        constexpr int S = decltype(Sc)::value;
         const int buf = jj & 1;
         SKL_PHASE(1, 0);
-        // sum of squares of the column's rows below the current one (four
-        // partial sums), for the reflector if this column wins
-        T xs;
-        {
-            T q0 = 0, q1 = 0, q2 = 0, q3 = 0;
-#pragma unroll
-            for (int r = 1; r < YQ_ROWS; r += 4) {
-                q0 += y[r] * y[r];
-                if (r + 1 < YQ_ROWS) q1 += y[r + 1] * y[r + 1];
-                if (r + 2 < YQ_ROWS) q2 += y[r + 2] * y[r + 2];
-                if (r + 3 < YQ_ROWS) q3 += y[r + 3] * y[r + 3];
-            }
-            xs = (q0 + q1) + (q2 + q3);
-        }
+        const T xs = yq_tail_sumsq<S>(y, live);
         // best column of this wave (every lane ends with the same answer) ...
         T bv = T(-1);
         int bp = NONE, bl = 0;
@@ -1332,32 +1430,19 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
                 gw = u;
             }
         if (wave == gw) {
-            // the reflector (dlarfg) of the winning column over rows jj..,
-            // formed in the lane that holds it: ||y(1:)|| from xs, the sum of
-            // squares every lane keeps of its column's rows below the first
-            const int wl = s_wl[gw];
-            const T alpha = readlane(y[0], wl);
-            const T xnorm = sqrt(readlane(xs, wl));
-            T tau = 0, sc = 0;
-            if (xnorm != T(0)) {
-                const T beta = -copysign(dlapy2_d(alpha, xnorm), alpha);
-                tau = (beta - alpha) / beta;
-                sc = T(1) / (alpha - beta);
-            }
-            if (lane == wl) {
-                T *dst = a.cvec + ((size_t)buf * P + blk) * YQ_MAXROWS;
-                dst[0] = T(1);
-#pragma unroll
-                for (int r = 1; r < YQ_ROWS; ++r) dst[r] = sc * y[r];
+            T *dst = a.cvec + ((size_t)buf * P + blk) * YQ_MAXROWS;
+            const T tau = yq_reflector<S>(y, xs, s_wl[gw], lane, dst, live);
+            if (lane == 0) {
                 a.cand[buf * P + blk] = gv;
                 a.candp[buf * P + blk] = gp;
                 a.ctau[buf * P + blk] = tau;
             }
         }
         SKL_PHASE(1, 1);
-        if (!grid_barrier(a.bar, bar_target)) return;
+        if (!grid_barrier(a.bar, bar_target)) return false;
         SKL_PHASE(1, 2);
-        // every workgroup's candidate, staged in LDS in one round trip
+        // every workgroup's candidate, staged in LDS in one round trip (rows
+        // at or beyond R are never read)
         for (int e = threadIdx.x; e < P * YQ_MAXROWS; e += 64 * W)
             s_vec[e] = a.cvec[(size_t)buf * P * YQ_MAXROWS + e];
         for (int t = threadIdx.x; t < P; t += 64 * W) {
@@ -1388,8 +1473,6 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
         }
         const int pl = p;
         const T tau = s_tau[u];
-        // lane l holds v[l] and v[l + 64]; rows reach every lane as scalar
-        // operands (v_readlane) rather than as LDS broadcasts
         const T vlo = s_vec[(size_t)u * YQ_MAXROWS + lane];
         const T vhi = s_vec[(size_t)u * YQ_MAXROWS + 64 + lane];
         SKL_PHASE(1, 4);
@@ -1399,45 +1482,224 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
         else if (pos == jj)
             pos = pl;
         const bool act = pos > jj && pos != NONE;
-        // every lane runs the loops (v_readlane reads lanes outside EXEC, so
-        // the values it reads must have been written by all lanes); retired
-        // columns get f = 0
-        {
-            T g = 0;
-#pragma unroll
-            for (int r = 0; r < YQ_ROWS; ++r) g += (r < 64 ? readlane(vlo, r) : readlane(vhi, r - 64)) * y[r];
-            const T f = (act && tau != T(0)) ? -tau * g : T(0);
-            // read the lanes again rather than keeping 2 x 74 scalars live
-            T wlo = vlo, whi = vhi;
-            asm volatile("" : "+v"(wlo), "+v"(whi));
-#pragma unroll
-            for (int r = 0; r < YQ_ROWS; ++r) y[r] += (r < 64 ? readlane(wlo, r) : readlane(whi, r - 64)) * f;
-        }
-        // norm downdating (skint.c:920-990), recomputed in the lane when needed
-        if (act && dn != T(0)) {
-            T temp = fabs(y[0]) / dn;
-            temp = (T(1) + temp) * (T(1) - temp);
-            if (temp < T(0)) temp = T(0);
-            const T t5 = dn / en;
-            const T temp2 = temp * t5 * t5;
-            if (temp2 <= tol3z) {
-                T sq = 0;
-#pragma unroll
-                for (int r = 1; r < YQ_ROWS; ++r) sq += y[r] * y[r];
-                dn = (mY - jj - 1 > 0) ? sqrt(sq) : T(0);
-                en = dn;
-            } else {
-                dn = dn * sqrt(temp);
-            }
-        }
-        // drop row jj
-#pragma unroll
-        for (int r = 0; r + 1 < YQ_ROWS; ++r) y[r] = y[r + 1];
-        y[YQ_ROWS - 1] = T(0);
+        yq_apply<S>(y, vlo, vhi, tau, act, live);
+        yq_downdate<S>(y, dn, en, act, mY, jj, tol3z, live);
         // s_v, s_wv and the staged candidates are rewritten only after the
         // next step's barriers
-    }
+        return true;
+    };
+    if (!yq_for_steps(a.nsteps, step, [&](int live) { yq_shift8(y, live); })) return;
     if (pos != NONE) a.perm_out[pos] = col;
+}
+
+// Column norms of Y (mY rows), eight lanes per column (each row read by one
+// lane of a group of eight, so a column's rows come in as whole cache lines)
+template <class T> __global__ __launch_bounds__(256) void k_ynorms(int nv, int mY, const T *Y,
+                                                                   long ldy, T *nrm) {
+    const int c = blockIdx.x * 32 + (threadIdx.x >> 3), part = threadIdx.x & 7;
+    T sq = 0;
+    if (c < nv) {
+        const T *src = Y + (long)c * ldy;
+        for (int r = part; r < mY; r += 8) sq = fma(src[r], src[r], sq);
+    }
+    sq += __shfl_xor(sq, 1, 64);
+    sq += __shfl_xor(sq, 2, 64);
+    sq += __shfl_xor(sq, 4, 64);
+    if (c < nv && part == 0) nrm[c] = sqrt(sq);
+}
+
+// Pivot search of a block on the columns that can win it, in one workgroup.
+// Partial norms never grow: a downdate multiplies by sqrt((1+t)(1-t)) <= 1,
+// and a recomputation returns the norm of the trailing rows of the column,
+// which the orthogonal reflectors keep at its initial norm up to rounding
+// (relative 64 x 74 ulps).  So a column of initial norm N never has a partial
+// norm above N (1 + 1e-9) in this search.  The kernel keeps the (at most 512)
+// columns of largest initial norm -- those above a threshold chosen from a
+// histogram of log2 of the norms in 1/16-octave bins -- one per lane, runs the
+// pivot steps of k_yqrp_lanes on them, and checks at every step that the
+// winner's partial norm exceeds X (1 + margin), where X is the largest
+// initial norm left out (from k_ynorms, whose summation order moves a norm
+// by a few ulps, far inside the margin of 1e-6): then no left-out column
+// could have won, and the
+// pivots are those of the search over all columns.  Otherwise it stops, and
+// *done = 0 tells k_yqrp_lanes (launched after it) to run the search over all
+// columns; on success *done = 1 and perm_out is written.  Left-out columns
+// move only when a pivot displaces them from position jj; their positions are
+// tracked in LDS (column at each position).
+constexpr int YT_W = 8, YT_MAXNV = 16384;
+template <class T> struct YtArgs {
+    int mY, nv, nsteps;
+    const T *Y;
+    long ldy;
+    const T *nrm0; // initial column norms (k_ynorms)
+    int32_t *perm_out;
+    int32_t *done;
+};
+
+template <class T>
+__global__ __launch_bounds__(64 * YT_W) void k_yqrp_top(YtArgs<T> a) {
+    constexpr int W = YT_W, NT = 64 * W, VR = 80;
+    __shared__ T s_v[2][W][VR];
+    __shared__ T s_bv[2][W], s_tau[2][W];
+    __shared__ int s_bp[2][W];
+    __shared__ int s_hist[256];
+    __shared__ int s_cols[NT];
+    __shared__ int s_cnt, s_cut;
+    __shared__ T s_red[W];
+    extern __shared__ unsigned char yt_smem[];
+    int *s_colat = reinterpret_cast<int *>(yt_smem); // column at each logical position
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int mY = a.mY, nv = a.nv;
+    const int NONE = 0x7fffffff;
+    const T tol3z = sqrt(eps_half<T>());
+    const T margin = sizeof(T) == 8 ? T(1e-6) : T(1e-2);
+    for (int t = tid; t < nv; t += NT) s_colat[t] = t;
+    // choose the columns: all of them when they fit, else the histogram cut
+    T xmax = T(-1); // largest initial norm left out (-1: none)
+    int nsel;
+    if (nv <= NT) {
+        nsel = nv;
+        s_cols[tid] = tid;
+    } else {
+        T mx = T(0);
+        for (int c = tid; c < nv; c += NT) {
+            const T v = a.nrm0[c];
+            if (v == v) mx = fmax(mx, v);
+        }
+        mx = wave_max(mx);
+        if (lane == 0) s_red[wave] = mx;
+        for (int b = tid; b < 256; b += NT) s_hist[b] = 0;
+        if (tid == 0) s_cnt = 0;
+        __syncthreads();
+        mx = s_red[0];
+        for (int q = 1; q < W; ++q) mx = fmax(mx, s_red[q]);
+        // bin 0..254: 16 log2(max / norm); 255: zero, NaN or far below
+        auto bin_of = [&](T v) -> int {
+            if (!(v > T(0))) return 255;
+            const float l = 16.0f * log2f((float)(mx / v));
+            return l < 254.0f ? (int)l : 254;
+        };
+        for (int c = tid; c < nv; c += NT) atomicAdd(&s_hist[bin_of(a.nrm0[c])], 1);
+        __syncthreads();
+        if (wave == 0) {
+            // cut = largest B with sum of bins below B at most NT
+            int h[4], run = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                h[q] = s_hist[4 * lane + q];
+                run += h[q];
+            }
+            int incl = run; // inclusive prefix over lanes of the four-bin sums
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += t;
+            }
+            int before = incl - run, cut = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (before + h[q] <= NT) cut = 4 * lane + q + 1;
+                before += h[q];
+            }
+            // bins are consecutive, so the largest passing cut is the answer
+            cut = wave_max_int(cut);
+            if (lane == 0) s_cut = cut;
+        }
+        __syncthreads();
+        const int cut = s_cut;
+        T lo = T(-1);
+        for (int c = tid; c < nv; c += NT) {
+            const T v = a.nrm0[c];
+            if (bin_of(v) < cut)
+                s_cols[atomicAdd(&s_cnt, 1)] = c;
+            else if (v == v)
+                lo = fmax(lo, v);
+        }
+        lo = wave_max(lo);
+        __syncthreads();
+        if (lane == 0) s_red[wave] = lo;
+        __syncthreads();
+        xmax = s_red[0];
+        for (int q = 1; q < W; ++q) xmax = fmax(xmax, s_red[q]);
+        nsel = s_cnt;
+    }
+    __syncthreads();
+    SKL_PHASE(3, 0);
+    const T lim = xmax < T(0) ? T(-1) : xmax * (T(1) + margin);
+    const bool valid = tid < nsel;
+    const int col = valid ? s_cols[tid] : 0;
+    T y[YQ_ROWS];
+    T dn, en;
+    yq_load(y, dn, en, a.Y, a.ldy, col, valid, mY);
+    int pos = valid ? col : NONE;
+    SKL_PHASE(3, 1);
+    auto step = [&](auto Sc, int jj, int live) -> bool {
+        constexpr int S = decltype(Sc)::value;
+        const int buf = jj & 1;
+        const T xs = yq_tail_sumsq<S>(y, live);
+        SKL_PHASE(3, 2);
+        // every wave publishes its best column as a reflector over rows jj..
+        {
+            const T av = fabs(dn);
+            const T cand = (pos >= jj && pos != NONE && av == av) ? av : T(-1);
+            const int wl = wave_argmax(cand, pos);
+            if (wl >= 0) {
+                const T tau = yq_reflector<S>(y, xs, wl, lane, &s_v[buf][wave][0], live);
+                if (lane == 0) {
+                    s_bv[buf][wave] = readlane(cand, wl);
+                    s_bp[buf][wave] = __builtin_amdgcn_readlane(pos, wl);
+                    s_tau[buf][wave] = tau;
+                }
+            } else if (lane == 0) {
+                s_bv[buf][wave] = T(-1);
+                s_bp[buf][wave] = NONE;
+            }
+        }
+        SKL_PHASE(3, 3);
+        __syncthreads();
+        SKL_PHASE(3, 4);
+        // the winner over the waves (largest norm, first position on ties)
+        T gv = T(-1);
+        int gp = NONE, u = 0;
+        for (int q = 0; q < W; ++q)
+            if (yq_better(s_bv[buf][q], s_bp[buf][q], gv, gp)) {
+                gv = s_bv[buf][q];
+                gp = s_bp[buf][q];
+                u = q;
+            }
+        // a left-out column might have won: leave the block to k_yqrp_lanes
+        if (gp == NONE || !(gv > lim)) return false;
+        const T tau = s_tau[buf][u];
+        const T vlo = s_v[buf][u][lane];
+        const T vhi = (lane < live - 64) ? s_v[buf][u][64 + lane] : T(0);
+        if (tid == 0) {
+            const int t = s_colat[jj];
+            s_colat[jj] = s_colat[gp];
+            s_colat[gp] = t;
+        }
+        if (pos == gp)
+            pos = jj;
+        else if (pos == jj)
+            pos = gp;
+        const bool act = pos > jj && pos != NONE;
+        SKL_PHASE(3, 5);
+        yq_apply<S>(y, vlo, vhi, tau, act, live);
+        SKL_PHASE(3, 6);
+        yq_downdate<S>(y, dn, en, act, mY, jj, tol3z, live);
+        SKL_PHASE(3, 7);
+        // the buffers of this step are rewritten two steps on, after the
+        // next step's barrier
+        return true;
+    };
+    const bool ok = yq_for_steps(a.nsteps, step, [&](int live) { yq_shift8(y, live); });
+    if (!ok) {
+        if (tid == 0) *a.done = 0;
+        return;
+    }
+    __syncthreads();
+    for (int t = tid; t < nv; t += NT) a.perm_out[t] = s_colat[t];
+    if (tid == 0) *a.done = 1;
 }
 
 // changed[0..*nch) = ascending positions t with perm[t] != t (one workgroup)
@@ -1616,6 +1878,13 @@ static bool yqrp_lanes_on() {
     return on;
 }
 
+// the single-workgroup search over the leading columns first, unless
+// SKL_YQRP_TOP=0 (read at every factorization)
+static bool yqrp_top_on() {
+    const char *e = getenv("SKL_YQRP_TOP");
+    return !(e && std::string(e) == "0");
+}
+
 template <class T>
 void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *tau,
                 const double *G_host, Arena &ar) {
@@ -1654,6 +1923,9 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
     T *ctau = ar.alloc_n<T>((size_t)2 * P_max);
     T *cvec = ar.alloc_n<T>((size_t)2 * P_max * YQ_MAXROWS);
     GridSync bar = grid_sync_fresh(ar); // (a fresh counter per pivot launch below)
+    const bool top_on = yqrp_top_on();
+    T *ynrm = ar.alloc_n<T>((size_t)n);
+    int32_t *ydone = ar.alloc_n<int32_t>((size_t)(n / nb + 1));
     Context &c = ctx();
     const bool ahead = lookahead_on() && !ctx().no_alloc;
     if (ahead) c.make_side();
@@ -1682,6 +1954,29 @@ void cpqr_hqrrp(long m, long ncol, long nfac, T *W, long ldw, int32_t *jpvt, T *
             bar = grid_sync_fresh(ar);
             ya.bar = bar;
             if (mY == YQ_ROWS && yqrp_lanes_on()) {
+                // the columns that can win the block in one workgroup first;
+                // the grid-wide search runs only if that one stops
+                if (top_on && nv <= YT_MAXNV) {
+                    k_ynorms<T><<<(unsigned)((nv + 31) / 32), 256, 0, s>>>((int)nv, mY,
+                                                                         Y + j * mY, mY, ynrm);
+                    YtArgs<T> ta;
+                    ta.mY = mY;
+                    ta.nv = (int)nv;
+                    ta.nsteps = b;
+                    ta.Y = Y + j * mY;
+                    ta.ldy = mY;
+                    ta.nrm0 = ynrm;
+                    ta.perm_out = perm;
+                    ta.done = ydone + j / nb;
+                    const size_t sht = (size_t)nv * sizeof(int32_t);
+                    if (sht > 32768)
+                        SKL_HIP(hipFuncSetAttribute((const void *)k_yqrp_top<T>,
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                    (int)sht));
+                    k_yqrp_top<T><<<1, 64 * YT_W, sht, s>>>(ta);
+                    SKL_CHECK_LAUNCH();
+                    ya.skip = ydone + j / nb;
+                }
                 // 512 columns per workgroup, one per lane
                 constexpr int WL = 8;
                 const int P = (int)((nv + 64 * WL - 1) / (64 * WL));

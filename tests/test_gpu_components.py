@@ -108,6 +108,32 @@ def test_cpqr_hqrrp_pivots(skl, O):
     assert rel(np.triu(W[:n]), np.triu(Wo[:n])) < 1e-11
 
 
+@pytest.mark.parametrize("scale", ["logspace(0,-6)", "ones", "logspace(0,-0.7)"])
+def test_hqrrp_leading_column_search_same_bits(skl, O, monkeypatch, scale):
+    """HQRRP's block pivots first come from a one-workgroup search over the
+    columns of largest initial norm, which stops when a left-out column could
+    win; the grid-wide search then runs.  With widely scaled columns it
+    finishes every block, with equal norms it stops at once, and with mildly
+    scaled ones some blocks stop part way.  In each case the factorization is
+    bit for bit the one of the grid-wide search alone (SKL_YQRP_TOP=0), and
+    the pivots are the oracle's."""
+    rng = np.random.default_rng(21)
+    m, n = 1200, 1000
+    sc = {"logspace(0,-6)": np.logspace(0, -6, n), "ones": np.ones(n),
+          "logspace(0,-0.7)": np.logspace(0, -0.7, n)}[scale]
+    A = rng.standard_normal((m, n)) * sc
+    monkeypatch.setenv("SKL_YQRP_TOP", "0")
+    W0, j0, t0 = skl.cpqr(A)
+    monkeypatch.setenv("SKL_YQRP_TOP", "1")
+    W1, j1, t1 = skl.cpqr(A)
+    assert np.array_equal(j0, j1)
+    assert np.array_equal(t0, t1)
+    assert np.array_equal(W0, W1)
+    Wo, jo, to = O.cpqr(A)
+    assert np.array_equal(j1, jo)
+    assert rel(np.triu(W1[:n]), np.triu(Wo[:n])) < 1e-11
+
+
 @pytest.mark.parametrize("r", [1, 31, 32, 33, 200, 1000])
 def test_trinv(skl, r):
     rng = np.random.default_rng(r)
