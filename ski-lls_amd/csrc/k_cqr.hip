@@ -204,12 +204,14 @@ __device__ void tri_inv_upper_lds(const double *Rs, double *X, double *Y) {
 // perm (final position -> panel column) and the sequential swaps; flag = 0 /
 // 1 (failed).  pass 2 (plain): runs only when flag is 0 and sets it on
 // failure.
-// Thread t keeps entries (p, g + 4 m), m < 16, of the Gram matrix in
-// registers, p = t & 63 its row and g = t >> 6; rows and columns stay where
-// they are (p is the panel column; lane p of every wave keeps that column's
-// logical position, partial norms and Schur diagonal, alike in all waves).  A
-// step: every wave picks the pivot, the pivot row's owners publish it in LDS
-// (double buffered), every thread updates its entries -- one barrier.
+// Blocked, 16 steps at a time, with no barrier inside a block: wave 0 runs
+// the steps, lane p for panel column p (rows and columns stay where they are;
+// the lane keeps the column's logical position, partial norms, Schur
+// diagonal and its entries of the block's rows of R).  A step reads column pk
+// of the Schur complement as it was at the block's start (LDS) and subtracts
+// the block's earlier rows of R (the pivot's entries by v_readlane); after
+// the block the four waves update the Schur complement by those 16 rows on
+// the matrix cores.
 template <bool PIV>
 __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double *G, int ldg,
                                                   double *R, double *Z, int32_t *perm,
@@ -217,103 +219,136 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
                                                   int pass) {
     __shared__ double Rb[64 * CQ_LD]; // row k of R by panel column; later X = R^-1
     __shared__ double Rr[64 * CQ_LD]; // R in logical order
-    __shared__ double Ys[64 * CQ_LD];
-    // rowbuf[buf]: the pivot row with zeros in the retired columns and in the
-    // pivot's own, its diagonal at [64]
-    __shared__ double rowbuf[2][65];
+    __shared__ double Ss[64 * CQ_LD]; // Schur complement at the start of the block
     __shared__ int s_perm[64];
+    __shared__ int s_k;               // steps done (the first failed step when short)
     const int tid = threadIdx.x, p = tid & 63, g = tid >> 6;
     if (pass == 2 && *flag != 0) return;
-    double sv[16];
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const int q = g + 4 * m;
-        sv[m] = (p < b && q < b) ? G[p + (long)q * ldg] : 0.0;
+    for (int e = tid; e < 64 * 64; e += CQ_T) {
+        const int i = e & 63, j = e >> 6;
+        Ss[i + j * CQ_LD] = (i < b && j < b) ? G[i + (long)j * ldg] : 0.0;
+        Rb[i + j * CQ_LD] = 0.0;
     }
-    for (int e = tid; e < 64 * CQ_LD; e += CQ_T) Rb[e] = 0.0;
-    // every wave keeps, for column p in lane p, its Gram diagonal, partial
-    // norms, logical position and whether it is still active (the same values
-    // in every wave), so each wave finds the pivot itself: one barrier per step
-    double dg = p < b ? G[p + (long)p * ldg] : 0.0;
-    const double g0 = dg; // the column's squared norm (lane p)
-    double vn1 = sqrt(fmax(dg, 0.0)), vn2 = vn1;
+    // wave 0 runs the steps, lane p for panel column p: its Schur diagonal,
+    // partial norms, logical position, whether it is still active, and its
+    // entries of the block's rows of R
+    double dg = 0.0, g0 = 0.0, vn1 = 0.0, vn2 = 0.0;
     int pos = p < b ? p : 0x7fffffff;
     bool mine = p < b;
-    // active flags of this thread's columns g + 4 m, bit m
-    unsigned amask = 0;
-#pragma unroll
-    for (int m = 0; m < 16; ++m) amask |= (g + 4 * m < b) ? (1u << m) : 0u;
     const double tol3z = sqrt(1.1102230246251565e-16);
+    if (tid == 0) s_k = 0;
+    __syncthreads();
+    if (g == 0 && p < b) {
+        dg = Ss[p + p * CQ_LD];
+        g0 = dg;
+        vn1 = sqrt(fmax(dg, 0.0));
+        vn2 = vn1;
+    }
     int k = 0;
-    for (; k < b; ++k) {
-        const int buf = k & 1;
-        int pk = k;
-        if (PIV) {
-            // largest partial norm among the columns left, first logical
-            // position on ties
-            const double bv = (mine && vn1 == vn1) ? fabs(vn1) : -1.0;
-            int bl = wave_argmax(bv, pos);
-            if (bl < 0) bl = __builtin_ctzll(__ballot(pos == k)); // no finite candidate
-            pk = bl;
-            const int lp = __shfl(pos, pk, 64); // logical position of the pivot
-            const bool atk = pos == k;
-            if (p == pk)
-                pos = k;
-            else if (atk)
-                pos = lp;
-            if (tid == 0 && swp) swp[k] = lp;
-        }
-        // retire the pivot column: its flag in lane pk, its bit in the threads
-        // that hold it
-        const double g0k = readlane(g0, pk);
-        if (p == pk) mine = false;
-        if ((pk & 3) == g) amask &= ~(1u << (pk >> 2));
-        if (p == pk) {
+    for (int k0 = 0; k0 < b; k0 += 16) {
+        if (g == 0) {
+            double rl[16];
 #pragma unroll
-            for (int m = 0; m < 16; ++m) rowbuf[buf][g + 4 * m] = ((amask >> m) & 1u) ? sv[m] : 0.0;
-            if ((pk & 3) == g) {
-                double dk = 0.0;
+            for (int t = 0; t < 16; ++t) rl[t] = 0.0;
+            bool ok = true;
 #pragma unroll
-                for (int m = 0; m < 16; ++m)
-                    if (m == (pk >> 2)) dk = sv[m];
-                rowbuf[buf][64] = dk;
+            for (int t = 0; t < 16; ++t) {
+                const int kk = k0 + t;
+                if (ok && kk < b) {
+                    int pk = kk;
+                    if (PIV) {
+                        // largest partial norm among the columns left, first
+                        // logical position on ties
+                        const double bv = (mine && vn1 == vn1) ? fabs(vn1) : -1.0;
+                        int bl = wave_argmax(bv, pos);
+                        if (bl < 0) bl = __builtin_ctzll(__ballot(pos == kk)); // no finite candidate
+                        pk = bl;
+                        const int lp = __shfl(pos, pk, 64); // logical position of the pivot
+                        const bool atk = pos == kk;
+                        if (p == pk)
+                            pos = kk;
+                        else if (atk)
+                            pos = lp;
+                        if (p == 0 && swp) swp[kk] = lp;
+                    }
+                    const double g0k = readlane(g0, pk);
+                    if (p == pk) mine = false;
+                    // row pk of the current Schur complement: the block's start
+                    // minus this block's earlier rows of R
+                    double sc = Ss[p + pk * CQ_LD];
+#pragma unroll
+                    for (int u = 0; u < t; ++u) sc = fma(-readlane(rl[u], pk), rl[u], sc);
+                    const double d = readlane(sc, pk);
+                    // not positive, or the column's partial norm below 1e-4 of its norm
+                    if (!(d > 0.0) || !(d >= 1e-8 * g0k)) {
+                        ok = false; // the same in every lane
+                    } else {
+                        const double inv = rsqrt_nr(d), rkk = d * inv;
+                        const double rp = sc * inv; // R(kk, column p) for an active p
+                        rl[t] = p == pk ? rkk : (mine ? rp : 0.0);
+                        if (mine) {
+                            dg -= rp * rp;
+                            if (PIV && vn1 != 0.0) {
+                                // partial norm downdating with the new row of R
+                                double temp = fabs(rp) / vn1;
+                                if (hq_norms)
+                                    temp = (1.0 + temp) * (1.0 - temp);
+                                else
+                                    temp = 1.0 - temp * temp;
+                                if (temp < 0.0) temp = 0.0;
+                                const double q5 = vn1 / vn2;
+                                const double temp2 = temp * q5 * q5;
+                                if (temp2 <= tol3z) {
+                                    vn1 = (mp - kk - 1 > 0) ? sqrt(fmax(dg, 0.0)) : 0.0;
+                                    vn2 = vn1;
+                                } else {
+                                    vn1 = vn1 * sqrt(temp);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            // the block's rows of R, by panel column
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                if (k0 + t < b) Rb[(k0 + t) + p * CQ_LD] = rl[t];
+            if (p == 0) {
+                int done = k0;
+#pragma unroll
+                for (int t = 0; t < 16; ++t) done += (k0 + t < b) ? 1 : 0;
+                s_k = ok ? done : -1;
             }
         }
         __syncthreads();
-        const double d = rowbuf[buf][64];
-        const double rowp = rowbuf[buf][p];
-        double rq[16];
+        const int kd = s_k;
+        if (kd < 0) break;
+        k = kd;
+        if (k >= b) break;
+        // Schur complement of the next block: S -= R(k0:k0+16, :)^T R(k0:k0+16, :)
+        // on the matrix cores, 16 x 16 tiles, four per wave
+        {
+            typedef double acc4 __attribute__((ext_vector_type(4)));
+            const int lane = p, li = lane & 15, lk = lane >> 4;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) rq[m] = rowbuf[buf][g + 4 * m];
-        // not positive, or the column's partial norm below 1e-4 of its norm
-        if (!(d > 0.0) || !(d >= 1e-8 * g0k)) break; // the same in every thread
-        const double inv = rsqrt_nr(d), rkk = d * inv;
-        const double rp = rowp * inv; // R(k, column p) for an active p
-        const double c = mine ? rp * inv : 0.0;
+            for (int q = 0; q < 4; ++q) {
+                const int tile = g * 4 + q, ti = tile & 3, tj = tile >> 2;
+                acc4 c;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) sv[m] = fma(-c, rq[m], sv[m]);
-        if (g == 0) Rb[k + p * CQ_LD] = p == pk ? rkk : (mine ? rp : 0.0);
-        if (mine) {
-            dg -= rp * rp;
-            if (PIV && vn1 != 0.0) {
-                // partial norm downdating with the new row k of R
-                double temp = fabs(rp) / vn1;
-                if (hq_norms)
-                    temp = (1.0 + temp) * (1.0 - temp);
-                else
-                    temp = 1.0 - temp * temp;
-                if (temp < 0.0) temp = 0.0;
-                const double q5 = vn1 / vn2;
-                const double temp2 = temp * q5 * q5;
-                if (temp2 <= tol3z) {
-                    vn1 = (mp - k - 1 > 0) ? sqrt(fmax(dg, 0.0)) : 0.0;
-                    vn2 = vn1;
-                } else {
-                    vn1 = vn1 * sqrt(temp);
-                }
+                for (int r = 0; r < 4; ++r) c[r] = Ss[(16 * ti + lk + 4 * r) + (16 * tj + li) * CQ_LD];
+#pragma unroll
+                for (int u0 = 0; u0 < 16; u0 += 4)
+                    c = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        -Rb[(k0 + u0 + lk) + (16 * ti + li) * CQ_LD],
+                        Rb[(k0 + u0 + lk) + (16 * tj + li) * CQ_LD], c, 0, 0, 0);
+                // each tile is read and written by its own wave only
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Ss[(16 * ti + lk + 4 * r) + (16 * tj + li) * CQ_LD] = c[r];
             }
         }
+        __syncthreads();
     }
+    // k: the steps done, b unless a pivot failed (pos is wave 0's)
     const int fail = k < b ? 1 : 0;
     if (tid == 0) {
         if (pass == 1)
@@ -333,7 +368,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
     if (PIV && perm && tid < b) perm[tid] = s_perm[tid];
     __syncthreads();
     // Z = P R^-1: row perm[t] of Z is row t of R^-1
-    tri_inv_upper_lds(Rr, Rb, Ys);
+    tri_inv_upper_lds(Rr, Rb, Ss);
     for (int e = tid; e < b * b; e += CQ_T) {
         const int t = e % b, j = e / b;
         Z[s_perm[t] + (long)j * b] = Rb[t + j * CQ_LD];
