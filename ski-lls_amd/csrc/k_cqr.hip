@@ -425,14 +425,10 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long l
                                                  const int32_t *swp, double *Wp, long ldw,
                                                  double *tau, double *Tout, double *Zc,
                                                  int32_t *jpvt, int32_t *swaps, const int *flag) {
-    // thread t keeps entries (i, g + 4 m) of the top block in registers,
-    // i = t & 63, g = t >> 6; step k publishes row k and column k (double
-    // buffered): one barrier per step
     __shared__ double Ub[64 * CQ_LD];
     __shared__ double Lb[64 * CQ_LD];
     __shared__ double Ys[64 * CQ_LD];
     __shared__ double Xs[64 * CQ_LD];
-    __shared__ double rowbuf[2][64], colbuf[2][64];
     __shared__ double sg[64];
     __shared__ int jp[64];
     if (*flag != 0) return;
@@ -447,54 +443,68 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long l
     __syncthreads();
     mm64_lds(Ys, Lb, [&](int r, int c, double v) { Xs[r + c * CQ_LD] = v; });
     __syncthreads();
-    double a[16];
-#pragma unroll
-    for (int m = 0; m < 16; ++m) a[m] = Xs[i + (g + 4 * m) * CQ_LD];
-    __syncthreads();
     for (int e = tid; e < 64 * CQ_LD; e += CQ_T) {
         Ub[e] = 0.0;
         Lb[e] = 0.0;
     }
-    if (i == 0) {
-#pragma unroll
-        for (int m = 0; m < 16; ++m) rowbuf[0][g + 4 * m] = a[m];
-    }
-    if (g == 0) colbuf[0][i] = a[0];
     __syncthreads();
-    for (int k = 0; k < b; ++k) {
-        const int buf = k & 1, nb = buf ^ 1;
-        const double akk = rowbuf[buf][k];
-        const double sk = akk >= 0.0 ? -1.0 : 1.0;
-        const double piv = akk - sk;
-        const double l = colbuf[buf][i] * rcp_nr(piv);
-        {
-            // branch-free, so that the sixteen LDS reads are issued together
-            double rj[16];
-#pragma unroll
-            for (int m = 0; m < 16; ++m) rj[m] = rowbuf[buf][g + 4 * m];
-            const double li = (i > k && i < b) ? l : 0.0;
-#pragma unroll
-            for (int m = 0; m < 16; ++m) {
-                const int j = g + 4 * m;
-                a[m] -= li * ((j > k && j < b) ? rj[m] : 0.0);
-            }
-        }
+    // LU without pivoting of Xs - S, blocked by 16 steps: wave 0 runs a block
+    // with no barrier, lane p keeping L(p, k) and U(k, p) of the block's steps;
+    // step k reads column k and row k of the block-start matrix (LDS) and
+    // subtracts the block's earlier terms (the other factor's entries by
+    // v_readlane); after the block the four waves update the matrix by
+    // L(:, block) U(block, :) on the matrix cores
+    for (int k0 = 0; k0 < b; k0 += 16) {
         if (g == 0) {
-            if (i > k && i < b) Lb[i + k * CQ_LD] = l;
-            if (i >= k && i < b) Ub[k + i * CQ_LD] = i == k ? piv : rowbuf[buf][i];
-            if (i == 0) sg[k] = sk;
-        }
-        if (k + 1 < b) {
-            if (i == k + 1) {
+            double lc[16], ur[16];
 #pragma unroll
-                for (int m = 0; m < 16; ++m) rowbuf[nb][g + 4 * m] = a[m];
+            for (int t = 0; t < 16; ++t) {
+                lc[t] = 0.0;
+                ur[t] = 0.0;
             }
-            if (g == ((k + 1) & 3)) {
-                double c = 0.0;
 #pragma unroll
-                for (int m = 0; m < 16; ++m)
-                    if (m == ((k + 1) >> 2)) c = a[m];
-                colbuf[nb][i] = c;
+            for (int t = 0; t < 16; ++t) {
+                const int k = k0 + t;
+                if (k < b) {
+                    double cv = Xs[i + k * CQ_LD]; // column k, row i
+                    double rv = Xs[k + i * CQ_LD]; // row k, column i
+#pragma unroll
+                    for (int u = 0; u < t; ++u) {
+                        cv = fma(-lc[u], readlane(ur[u], k), cv);
+                        rv = fma(-readlane(lc[u], k), ur[u], rv);
+                    }
+                    const double akk = readlane(rv, k);
+                    const double sk = akk >= 0.0 ? -1.0 : 1.0;
+                    const double piv = akk - sk;
+                    const double l = cv * rcp_nr(piv);
+                    lc[t] = (i > k && i < b) ? l : 0.0;
+                    ur[t] = (i > k && i < b) ? rv : 0.0;
+                    if (i > k && i < b) Lb[i + k * CQ_LD] = l;
+                    if (i >= k && i < b) Ub[k + i * CQ_LD] = i == k ? piv : rv;
+                    if (i == 0) sg[k] = sk;
+                }
+            }
+        }
+        __syncthreads();
+        if (k0 + 16 >= b) break;
+        // Xs -= L(:, k0:k0+16) U(k0:k0+16, :): 16 x 16 tiles, four per wave,
+        // each read and written by its own wave only
+        {
+            typedef double acc4 __attribute__((ext_vector_type(4)));
+            const int li = i & 15, lk = i >> 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int tile = g * 4 + q, ti = tile & 3, tj = tile >> 2;
+                acc4 c;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) c[r] = Xs[(16 * ti + lk + 4 * r) + (16 * tj + li) * CQ_LD];
+#pragma unroll
+                for (int u0 = 0; u0 < 16; u0 += 4)
+                    c = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        -Lb[(16 * ti + li) + (k0 + u0 + lk) * CQ_LD],
+                        Ub[(k0 + u0 + lk) + (16 * tj + li) * CQ_LD], c, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Xs[(16 * ti + lk + 4 * r) + (16 * tj + li) * CQ_LD] = c[r];
             }
         }
         __syncthreads();
