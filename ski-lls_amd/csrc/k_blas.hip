@@ -180,13 +180,24 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T alpha, cons
 }
 
 // C = alpha sum_z W[z] + beta C over S split-K partials (fixed order)
+// (the partials are loaded eight at a time, then added in order, so the
+// loads of a group are in flight together)
 template <class T>
 __global__ void k_splitk_reduce(int M, int N, int S, const T *W, T alpha, T beta, T *C, long ldc) {
     long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (long)M * N) return;
     int i = (int)(e % M), j = (int)(e / M);
+    const long MN = (long)M * N;
     T s = W[e];
-    for (int z = 1; z < S; ++z) s += W[e + (long)z * M * N];
+    int z = 1;
+    for (; z + 8 <= S; z += 8) {
+        T w[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) w[q] = W[e + (long)(z + q) * MN];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s += w[q];
+    }
+    for (; z < S; ++z) s += W[e + (long)z * MN];
     T *c = C + i + (long)j * ldc;
     *c = (beta == T(0)) ? alpha * s : alpha * s + beta * *c;
 }

@@ -23,6 +23,34 @@ __device__ inline double shfl_xor_d(double v, int mask) {
     return __shfl_xor(v, mask, 64);
 }
 
+// 1/sqrt(d) and 1/x from the hardware estimates with Newton steps (two in
+// FP64, one in FP32; quadratic convergence from the estimate's 2^-22 or
+// better): a chain of a few fused multiply-adds instead of the IEEE
+// square-root and division sequences on a step's critical path.  Within an
+// ulp or two of the correctly rounded values (d > 0, x != 0 finite).
+__device__ inline double rsqrt_nr(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+    y = y * fma(-0.5 * d * y, y, 1.5);
+    y = y * fma(-0.5 * d * y, y, 1.5);
+    return y;
+}
+__device__ inline double rcp_nr(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    y = y * fma(-x, y, 2.0);
+    y = y * fma(-x, y, 2.0);
+    return y;
+}
+__device__ inline float rsqrt_nr(float d) {
+    float y = __builtin_amdgcn_rsqf(d);
+    return y * fmaf(-0.5f * d * y, y, 1.5f);
+}
+__device__ inline float rcp_nr(float x) {
+    float y = __builtin_amdgcn_rcpf(x);
+    return y * fmaf(-x, y, 2.0f);
+}
+// sqrt(x) for x >= 0 as x / sqrt(x) (0 for 0)
+template <class T> __device__ inline T sqrt_nr(T x) { return x > T(0) ? x * rsqrt_nr(x) : T(0); }
+
 template <class T> __device__ inline T wave_sum(T v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -37,24 +37,6 @@ namespace skl {
 constexpr int CQ_T = 256;
 constexpr int CQ_LD = 65;
 
-// 1/sqrt(d) and 1/x from the hardware estimates with two Newton steps each
-// (quadratic convergence from the estimate's 2^-22 or better): a chain of a
-// few fused multiply-adds instead of the IEEE square-root and division
-// sequences on every step's critical path.  Within an ulp or two of the
-// correctly rounded values.
-__device__ inline double rsqrt_nr(double d) {
-    double y = __builtin_amdgcn_rsq(d);
-    y = y * fma(-0.5 * d * y, y, 1.5);
-    y = y * fma(-0.5 * d * y, y, 1.5);
-    return y;
-}
-__device__ inline double rcp_nr(double x) {
-    double y = __builtin_amdgcn_rcp(x);
-    y = y * fma(-x, y, 2.0);
-    y = y * fma(-x, y, 2.0);
-    return y;
-}
-
 // C = A B for 64 x 64 matrices in LDS (ld CQ_LD) on the matrix cores: wave w
 // of the CQ_T threads forms the 32 x 32 quadrant (32 (w & 1), 32 (w >> 1)) as
 // 2 x 2 blocks of v_mfma_f64_16x16x4f64 (A(i, k) from lane i + 16 k, B(k, j)

@@ -1314,23 +1314,25 @@ __device__ inline void yq_apply(T (&y)[YQ_ROWS], T vlo, T vhi, T tau, bool act, 
 }
 
 // norm downdating of skint.c:920-990 (NoFLA_QRP_downdate_partial_norms) with
-// the updated pivot-row entry y[S], recomputed in the lane when needed
+// the updated pivot-row entry y[S], recomputed in the lane when needed (the
+// quotients and square roots from rcp_nr / rsqrt_nr)
 template <int S, class T>
 __device__ inline void yq_downdate(const T (&y)[YQ_ROWS], T &dn, T &en, bool act, int mY, int jj,
                                    T tol3z, int live) {
     if (act && dn != T(0)) {
-        T temp = fabs(y[S]) / dn;
+        const T rd = rcp_nr(dn);
+        T temp = fabs(y[S]) * rd;
         temp = (T(1) + temp) * (T(1) - temp);
         if (temp < T(0)) temp = T(0);
-        const T t5 = dn / en;
+        const T t5 = dn * rcp_nr(en);
         const T temp2 = temp * t5 * t5;
         if (temp2 <= tol3z) {
             T sq = 0;
             YQ_ROWS_FROM(S + 1, sq = fma(y[r], y[r], sq);)
-            dn = (mY - jj - 1 > 0) ? sqrt(sq) : T(0);
+            dn = (mY - jj - 1 > 0) ? sqrt_nr(sq) : T(0);
             en = dn;
         } else {
-            dn = dn * sqrt(temp);
+            dn = dn * sqrt_nr(temp);
         }
     }
 }
@@ -1356,21 +1358,38 @@ __device__ inline void yq_load(T (&y)[YQ_ROWS], T &dn, T &en, const T *Y, long l
     en = dn;
 }
 
-// the reflector (dlarfg) of the wave's column in lane wl over its live rows
-// jj.. (pivot row y[S]), formed in that lane: the norm from xs, the lane's sum
-// of squares of its rows below the pivot row; v[S] = 1, v[r] = sc y[r] below
+// The reflector (dlarfg) of the wave's column in lane wl over its live rows
+// jj.. (pivot row y[S]): tau and the scale sc of the rows below the pivot,
+// v[S] = 1, v[r] = sc y[r], from alpha = y[S] and xs, that lane's sum of
+// squares of its rows below the pivot row (wave-uniform results).  The lane
+// publishes its raw rows S.. in pairs (dst 16-byte aligned); a reader forms
+// v from them with yq_v.
+template <class T> struct alignas(2 * sizeof(T)) yq_pair { T a, b; };
 template <int S, class T>
-__device__ inline T yq_reflector(const T (&y)[YQ_ROWS], T xs, int wl, int lane, T *dst, int live) {
+__device__ inline void yq_publish(const T (&y)[YQ_ROWS], T xs, int wl, int lane, T *dst, int live,
+                                  T &tau, T &sc) {
     const T alpha = readlane(y[S], wl);
     const T xnorm2 = readlane(xs, wl);
-    T tau = 0, sc = 0;
+    tau = 0;
+    sc = 0;
     if (xnorm2 != T(0)) {
-        const T beta = -copysign(sqrt(fma(alpha, alpha, xnorm2)), alpha);
-        tau = (beta - alpha) / beta;
-        sc = T(1) / (alpha - beta);
+        const T beta = -copysign(sqrt_nr(fma(alpha, alpha, xnorm2)), alpha);
+        tau = (beta - alpha) * rcp_nr(beta);
+        sc = rcp_nr(alpha - beta);
     }
-    if (lane == wl) { YQ_ROWS_FROM(S, dst[r] = r == S ? T(1) : sc * y[r];) }
-    return tau;
+    if (lane == wl) {
+        yq_pair<T> *d2 = reinterpret_cast<yq_pair<T> *>(dst);
+#pragma unroll
+        for (int r0 = S & ~1; r0 < YQ_ROWS; r0 += 2)
+            if (YQ_CH * (r0 / YQ_CH) < live) d2[r0 / 2] = {y[r0], r0 + 1 < YQ_ROWS ? y[r0 + 1] : T(0)};
+    }
+}
+
+// the reflector's rows l and l + 64 in lane l from the published raw rows
+template <int S, class T>
+__device__ inline void yq_v(const T *src, T sc, int lane, int live, T &vlo, T &vhi) {
+    vlo = lane == S ? T(1) : sc * src[lane];
+    vhi = (lane < live - 64) ? sc * src[64 + lane] : T(0);
 }
 
 template <class T, int W>
@@ -1430,9 +1449,12 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
                 gw = u;
             }
         if (wave == gw) {
+            // raw rows in cvec, sc after them (entry YQ_MAXROWS - 1)
             T *dst = a.cvec + ((size_t)buf * P + blk) * YQ_MAXROWS;
-            const T tau = yq_reflector<S>(y, xs, s_wl[gw], lane, dst, live);
+            T tau, sc;
+            yq_publish<S>(y, xs, s_wl[gw], lane, dst, live, tau, sc);
             if (lane == 0) {
+                dst[YQ_MAXROWS - 1] = sc;
                 a.cand[buf * P + blk] = gv;
                 a.candp[buf * P + blk] = gp;
                 a.ctau[buf * P + blk] = tau;
@@ -1473,8 +1495,9 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
         }
         const int pl = p;
         const T tau = s_tau[u];
-        const T vlo = s_vec[(size_t)u * YQ_MAXROWS + lane];
-        const T vhi = s_vec[(size_t)u * YQ_MAXROWS + 64 + lane];
+        T vlo, vhi;
+        yq_v<S>(s_vec + (size_t)u * YQ_MAXROWS, s_vec[(size_t)u * YQ_MAXROWS + YQ_MAXROWS - 1],
+                lane, live, vlo, vhi);
         SKL_PHASE(1, 4);
         // logical swap of positions jj and pl, then y -= tau v (v^T y)
         if (pos == pl)
@@ -1539,8 +1562,8 @@ template <class T> struct YtArgs {
 template <class T>
 __global__ __launch_bounds__(64 * YT_W) void k_yqrp_top(YtArgs<T> a) {
     constexpr int W = YT_W, NT = 64 * W, VR = 80;
-    __shared__ T s_v[2][W][VR];
-    __shared__ T s_bv[2][W], s_tau[2][W];
+    __shared__ __align__(16) T s_v[2][W][VR]; // candidates' raw rows
+    __shared__ T s_bv[2][W], s_tau[2][W], s_sc[2][W];
     __shared__ int s_bp[2][W];
     __shared__ int s_hist[256];
     __shared__ int s_cols[NT];
@@ -1600,6 +1623,7 @@ __global__ __launch_bounds__(64 * YT_W) void k_yqrp_top(YtArgs<T> a) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 if (before + h[q] <= NT) cut = 4 * lane + q + 1;
+                s_hist[4 * lane + q] = before; // first slot of the bin
                 before += h[q];
             }
             // bins are consecutive, so the largest passing cut is the answer
@@ -1611,9 +1635,12 @@ __global__ __launch_bounds__(64 * YT_W) void k_yqrp_top(YtArgs<T> a) {
         T lo = T(-1);
         for (int c = tid; c < nv; c += NT) {
             const T v = a.nrm0[c];
-            if (bin_of(v) < cut)
-                s_cols[atomicAdd(&s_cnt, 1)] = c;
-            else if (v == v)
+            const int bn = bin_of(v);
+            if (bn < cut) {
+                // slots in bin order (largest norms in the first waves)
+                s_cols[atomicAdd(&s_hist[bn], 1)] = c;
+                atomicAdd(&s_cnt, 1);
+            } else if (v == v)
                 lo = fmax(lo, v);
         }
         lo = wave_max(lo);
@@ -1639,17 +1666,19 @@ __global__ __launch_bounds__(64 * YT_W) void k_yqrp_top(YtArgs<T> a) {
         const int buf = jj & 1;
         const T xs = yq_tail_sumsq<S>(y, live);
         SKL_PHASE(3, 2);
-        // every wave publishes its best column as a reflector over rows jj..
+        // every wave publishes its best column's raw rows and reflector scalars
         {
             const T av = fabs(dn);
             const T cand = (pos >= jj && pos != NONE && av == av) ? av : T(-1);
             const int wl = wave_argmax(cand, pos);
             if (wl >= 0) {
-                const T tau = yq_reflector<S>(y, xs, wl, lane, &s_v[buf][wave][0], live);
+                T tau, sc;
+                yq_publish<S>(y, xs, wl, lane, &s_v[buf][wave][0], live, tau, sc);
                 if (lane == 0) {
                     s_bv[buf][wave] = readlane(cand, wl);
                     s_bp[buf][wave] = __builtin_amdgcn_readlane(pos, wl);
                     s_tau[buf][wave] = tau;
+                    s_sc[buf][wave] = sc;
                 }
             } else if (lane == 0) {
                 s_bv[buf][wave] = T(-1);
@@ -1671,8 +1700,8 @@ __global__ __launch_bounds__(64 * YT_W) void k_yqrp_top(YtArgs<T> a) {
         // a left-out column might have won: leave the block to k_yqrp_lanes
         if (gp == NONE || !(gv > lim)) return false;
         const T tau = s_tau[buf][u];
-        const T vlo = s_v[buf][u][lane];
-        const T vhi = (lane < live - 64) ? s_v[buf][u][64 + lane] : T(0);
+        T vlo, vhi;
+        yq_v<S>(&s_v[buf][u][0], s_sc[buf][u], lane, live, vlo, vhi);
         if (tid == 0) {
             const int t = s_colat[jj];
             s_colat[jj] = s_colat[gp];

@@ -143,9 +143,14 @@ struct PendingBarrier {
     hipEvent_t ev;
     int tokens;
 };
+struct UnrecordedBarrier {
+    hipStream_t s;
+    int tokens;
+};
 std::mutex g_tok_mu;
-int g_tok_cap = 0, g_tok_free = 0;
+int g_tok_cap = 0, g_tok_free = 0, g_tok_unrec_total = 0;
 std::vector<PendingBarrier> g_tok_pending;
+std::vector<UnrecordedBarrier> g_tok_unrec; // launches with no event after them yet
 std::vector<hipEvent_t> g_tok_events;
 
 // returns the tokens of completed launches (caller holds g_tok_mu)
@@ -162,6 +167,50 @@ void reap_barriers() {
         }
     }
 }
+
+// Records one event after the launches that have none yet (one per stream),
+// from the event pool, refilled first outside the token lock.  The records
+// happen under the lock, so a stream that release_thread_context is about to
+// destroy (it takes its entries out under the same lock) is never recorded on.
+void flush_unrecorded() {
+    size_t want;
+    {
+        std::lock_guard<std::mutex> lk(g_tok_mu);
+        want = g_tok_unrec.size();
+        want = want > g_tok_events.size() ? want - g_tok_events.size() : 0;
+    }
+    std::vector<hipEvent_t> fresh;
+    if (want && !ctx().no_alloc) {
+        AllocLock alk(capture_mutex());
+        for (size_t i = 0; i < want; ++i) {
+            hipEvent_t ev = nullptr;
+            if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) fresh.push_back(ev);
+            else (void)hipGetLastError();
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_tok_mu);
+    g_tok_events.insert(g_tok_events.end(), fresh.begin(), fresh.end());
+    for (size_t i = 0; i < g_tok_unrec.size();) {
+        const UnrecordedBarrier u = g_tok_unrec[i];
+        hipEvent_t ev = nullptr;
+        if (!g_tok_events.empty()) {
+            ev = g_tok_events.back();
+            g_tok_events.pop_back();
+        }
+        if (ev && hipEventRecord(ev, u.s) == hipSuccess) {
+            g_tok_pending.push_back({ev, u.tokens});
+        } else {
+            // no event to wait on: hand the tokens back after the stream drains
+            (void)hipGetLastError();
+            (void)hipStreamSynchronize(u.s);
+            g_tok_free += u.tokens;
+            if (ev) g_tok_events.push_back(ev);
+        }
+        g_tok_unrec_total -= u.tokens;
+        g_tok_unrec[i] = g_tok_unrec.back();
+        g_tok_unrec.pop_back();
+    }
+}
 } // namespace
 
 BarrierLaunch::BarrierLaunch(int grid, bool exclusive) {
@@ -172,40 +221,51 @@ BarrierLaunch::BarrierLaunch(int grid, bool exclusive) {
     while (g_tok_free < need) {
         reap_barriers();
         if (g_tok_free >= need) break;
+        const bool unrec = !g_tok_unrec.empty();
         lk.unlock();
-        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        if (unrec)
+            flush_unrecorded();
+        else
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
         lk.lock();
     }
     g_tok_free -= need;
     tokens_ = need;
 }
 
+// The tokens wait for an event after the launch, but one event serves every
+// launch before it on the stream: the event is recorded only once launches
+// without one hold half the tokens, or when a launch waits for tokens (an
+// event record costs the stream several microseconds, about as much as the
+// short fallback kernels behind these guards).
 BarrierLaunch::~BarrierLaunch() {
-    hipEvent_t ev = nullptr;
+    bool flush;
     {
         std::lock_guard<std::mutex> lk(g_tok_mu);
-        if (!g_tok_events.empty()) {
-            ev = g_tok_events.back();
-            g_tok_events.pop_back();
-        }
+        const hipStream_t s = ctx().stream;
+        if (!g_tok_unrec.empty() && g_tok_unrec.back().s == s)
+            g_tok_unrec.back().tokens += tokens_;
+        else
+            g_tok_unrec.push_back({s, tokens_});
+        g_tok_unrec_total += tokens_;
+        flush = 2 * g_tok_unrec_total >= g_tok_cap;
     }
-    // (no event is created in no-allocation mode: without a pooled one the
-    // tokens come back after the stream drains)
-    bool ok = ev != nullptr;
-    if (!ok && !ctx().no_alloc) {
-        AllocLock alk(capture_mutex());
-        ok = hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
-    }
-    ok = ok && hipEventRecord(ev, ctx().stream) == hipSuccess;
+    if (flush) flush_unrecorded();
+}
+
+// A drained stream's launches hold no tokens any more (release_thread_context,
+// before it destroys the stream).
+void barrier_tokens_stream_drained(hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_tok_mu);
-    if (ok) {
-        g_tok_pending.push_back({ev, tokens_});
-    } else {
-        // no event to wait on: hand the tokens back after the stream drains
-        (void)hipGetLastError();
-        (void)hipStreamSynchronize(ctx().stream);
-        g_tok_free += tokens_;
-        if (ev) g_tok_events.push_back(ev);
+    for (size_t i = 0; i < g_tok_unrec.size();) {
+        if (g_tok_unrec[i].s == s) {
+            g_tok_free += g_tok_unrec[i].tokens;
+            g_tok_unrec_total -= g_tok_unrec[i].tokens;
+            g_tok_unrec[i] = g_tok_unrec.back();
+            g_tok_unrec.pop_back();
+        } else {
+            ++i;
+        }
     }
 }
 
@@ -274,6 +334,9 @@ void release_thread_context() {
     (void)hipStreamSynchronize(t_ctx->stream);
     if (t_ctx->side) (void)hipStreamSynchronize(t_ctx->side);
     if (t_ctx->aux) (void)hipStreamSynchronize(t_ctx->aux);
+    barrier_tokens_stream_drained(t_ctx->stream);
+    if (t_ctx->side) barrier_tokens_stream_drained(t_ctx->side);
+    if (t_ctx->aux) barrier_tokens_stream_drained(t_ctx->aux);
     lsqr_release_graphs();
     t_ctx->arena.release(); // (takes the capture lock itself)
     AllocLock alk(capture_mutex());

@@ -168,18 +168,20 @@ bool has_context();
 void set_last_error(const std::string &msg);
 void ensure_init();
 void release_thread_context();        // frees the calling thread's context
+void barrier_tokens_stream_drained(hipStream_t s);
 
 // Kernels with a grid barrier need every workgroup resident at once, and each
 // of their workgroups fills a CU (registers or LDS), so the workgroups of all
 // such kernels in flight on the device must fit on its CUs together.  A launch
 // reserves one token per workgroup from a process-wide pool of num_cu tokens
 // (an exclusive launch, for grids over half the device, reserves them all) and
-// returns them when an event recorded after it completes; no host thread waits
-// for a launch unless the pool is short.
+// returns them when an event recorded after it completes (one event may serve
+// several launches); no host thread waits for a launch unless the pool is
+// short.
 class BarrierLaunch {
   public:
     explicit BarrierLaunch(int grid, bool exclusive = false);
-    ~BarrierLaunch(); // records the completion event on ctx().stream
+    ~BarrierLaunch(); // the completion event on ctx().stream, now or later
     BarrierLaunch(const BarrierLaunch &) = delete;
     BarrierLaunch &operator=(const BarrierLaunch &) = delete;
 
