@@ -1296,21 +1296,66 @@ template <int S, class T> __device__ inline T yq_tail_sumsq(const T (&y)[YQ_ROWS
     return (q[0] + q[1]) + (q[2] + q[3]);
 }
 
-// y -= tau v (v^T y) for an active column (f = 0 otherwise); lane l holds
-// v[l] and v[l + 64] (rows of the group's frame), and the rows reach every
-// lane as scalar operands (v_readlane).  Every lane runs the loops
-// (v_readlane reads lanes outside EXEC, so the values it reads must have been
-// written by all lanes).
+// Compile-time loop: f(std::integral_constant<int, i>{}) for i in [I0, I1).
+template <int I0, int I1, class F> __device__ inline void yq_static_for(F &&f) {
+    if constexpr (I0 < I1) {
+        f(std::integral_constant<int, I0>{});
+        yq_static_for<I0 + 1, I1>(f);
+    }
+}
+
+// acc += v[R] * x, with v[R] taken from lane R % 16 of each row of sixteen
+// lanes of vb (DPP row_newbcast: lane l of vb holds v[16 c + l % 16], so every
+// lane reads v[R] from its own row) -- one fused instruction per term, no
+// scalar round trip.  A DPP source must not be written by the two VALU
+// instructions before the read: vb is written once per step, and the caller
+// fences it (yq_fence) before the first use.
+template <int R>
+__device__ inline void yq_bcast_fma(double &acc, const double (&vb)[5], double x) {
+    asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                 : "+v"(acc)
+                 : "v"(vb[R >> 4]), "v"(x), "i"(R & 15));
+}
+template <int R> __device__ inline void yq_bcast_fma(float &acc, const float (&vb)[5], float x) {
+    const float v = __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(vb[R >> 4]), 0x150 + (R & 15), 0xF, 0xF, false));
+    acc = fmaf(v, x, acc);
+}
+template <class T> __device__ inline void yq_fence(T (&vb)[5]) {
+    asm volatile("s_nop 1" : "+v"(vb[0]), "+v"(vb[1]), "+v"(vb[2]), "+v"(vb[3]), "+v"(vb[4]));
+}
+
+// rows [S, live) of the column, chunk by chunk (a chunk is skipped when it
+// starts at or beyond live), with compile-time row numbers
+template <int S, class F> __device__ inline void yq_rows_static(int live, F &&f) {
+    yq_static_for<0, YQ_NCH>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        if constexpr (YQ_CH * c + YQ_CH > S) {
+            if (YQ_CH * c < live) {
+                yq_static_for<0, YQ_CH>([&](auto ii) {
+                    constexpr int r = YQ_CH * c + decltype(ii)::value;
+                    if constexpr (r >= S && r < YQ_ROWS) f(std::integral_constant<int, r>{});
+                });
+            }
+        }
+    });
+}
+
+// y -= tau v (v^T y) for an active column (f = 0 otherwise), v broadcast
+// from vb (yq_v) by DPP: the dot in four interleaved partial sums.
 template <int S, class T>
-__device__ inline void yq_apply(T (&y)[YQ_ROWS], T vlo, T vhi, T tau, bool act, int live) {
-    T g[2] = {0, 0};
-    YQ_ROWS_FROM(S, g[(r - S) & 1] = fma(r < 64 ? readlane(vlo, r) : readlane(vhi, r - 64), y[r],
-                                         g[(r - S) & 1]);)
-    const T f = (act && tau != T(0)) ? -tau * (g[0] + g[1]) : T(0);
-    // read the lanes again rather than keeping 2 x 74 scalars live
-    T wlo = vlo, whi = vhi;
-    asm volatile("" : "+v"(wlo), "+v"(whi));
-    YQ_ROWS_FROM(S, y[r] = fma(r < 64 ? readlane(wlo, r) : readlane(whi, r - 64), f, y[r]);)
+__device__ inline void yq_apply(T (&y)[YQ_ROWS], T (&vb)[5], T tau, bool act, int live) {
+    yq_fence(vb);
+    T g[4] = {0, 0, 0, 0};
+    yq_rows_static<S>(live, [&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        yq_bcast_fma<r>(g[(r - S) & 3], vb, y[r]);
+    });
+    T f = (act && tau != T(0)) ? -tau * ((g[0] + g[2]) + (g[1] + g[3])) : T(0);
+    yq_rows_static<S>(live, [&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        yq_bcast_fma<r>(y[r], vb, f);
+    });
 }
 
 // norm downdating of skint.c:920-990 (NoFLA_QRP_downdate_partial_norms) with
@@ -1385,11 +1430,15 @@ __device__ inline void yq_publish(const T (&y)[YQ_ROWS], T xs, int wl, int lane,
     }
 }
 
-// the reflector's rows l and l + 64 in lane l from the published raw rows
-template <int S, class T>
-__device__ inline void yq_v(const T *src, T sc, int lane, int live, T &vlo, T &vhi) {
-    vlo = lane == S ? T(1) : sc * src[lane];
-    vhi = (lane < live - 64) ? sc * src[64 + lane] : T(0);
+// the reflector for yq_apply from the published raw rows: lane l of vb[c]
+// holds v[16 c + l % 16] (v[S] = 1, v[r] = sc y[r] below; rows outside
+// [S, live) are never read)
+template <int S, class T> __device__ inline void yq_v(const T *src, T sc, int lane, T (&vb)[5]) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        const int r = 16 * c + (lane & 15);
+        vb[c] = r == S ? T(1) : sc * src[r];
+    }
 }
 
 template <class T, int W>
@@ -1495,9 +1544,9 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
         }
         const int pl = p;
         const T tau = s_tau[u];
-        T vlo, vhi;
+        T vb[5];
         yq_v<S>(s_vec + (size_t)u * YQ_MAXROWS, s_vec[(size_t)u * YQ_MAXROWS + YQ_MAXROWS - 1],
-                lane, live, vlo, vhi);
+                lane, vb);
         SKL_PHASE(1, 4);
         // logical swap of positions jj and pl, then y -= tau v (v^T y)
         if (pos == pl)
@@ -1505,7 +1554,7 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
         else if (pos == jj)
             pos = pl;
         const bool act = pos > jj && pos != NONE;
-        yq_apply<S>(y, vlo, vhi, tau, act, live);
+        yq_apply<S>(y, vb, tau, act, live);
         yq_downdate<S>(y, dn, en, act, mY, jj, tol3z, live);
         // s_v, s_wv and the staged candidates are rewritten only after the
         // next step's barriers
@@ -1700,8 +1749,8 @@ __global__ __launch_bounds__(64 * YT_W) void k_yqrp_top(YtArgs<T> a) {
         // a left-out column might have won: leave the block to k_yqrp_lanes
         if (gp == NONE || !(gv > lim)) return false;
         const T tau = s_tau[buf][u];
-        T vlo, vhi;
-        yq_v<S>(&s_v[buf][u][0], s_sc[buf][u], lane, live, vlo, vhi);
+        T vb[5];
+        yq_v<S>(&s_v[buf][u][0], s_sc[buf][u], lane, vb);
         if (tid == 0) {
             const int t = s_colat[jj];
             s_colat[jj] = s_colat[gp];
@@ -1713,7 +1762,7 @@ __global__ __launch_bounds__(64 * YT_W) void k_yqrp_top(YtArgs<T> a) {
             pos = gp;
         const bool act = pos > jj && pos != NONE;
         SKL_PHASE(3, 5);
-        yq_apply<S>(y, vlo, vhi, tau, act, live);
+        yq_apply<S>(y, vb, tau, act, live);
         SKL_PHASE(3, 6);
         yq_downdate<S>(y, dn, en, act, mY, jj, tol3z, live);
         SKL_PHASE(3, 7);
