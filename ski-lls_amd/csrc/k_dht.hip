@@ -317,11 +317,14 @@ __global__ __launch_bounds__(NT, 2048 / NT) void k_fft4_pass1(long n, int n1, in
 // [0, n1/2] plus their mirrors n1 - k1): X[k1 + n1 k2] = FFT_{n2}(Y[k1, :])[k2],
 // then H at k and n - k from the primary/mirror rows, written to H (n x ncols).
 // ---------------------------------------------------------------------------
+// pairs_out: H holds the pair's two transforms interleaved (H[2 (pair n + k)
+// + {0, 1}]), for the bucket gather of both columns in one read per member;
+// otherwise column c at H + c ldh.
 template <class T, int NT>
 __global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K1, long ncols,
                                                    FftDims dims, const cplx<T> *__restrict__ tw2,
                                                    const cplx<T> *__restrict__ Y,
-                                                   T *__restrict__ H, long ldh) {
+                                                   T *__restrict__ H, long ldh, int pairs_out) {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const long lb = xcd_local_block(blockIdx.x + (long)gridDim.x * blockIdx.y,
@@ -383,8 +386,12 @@ __global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K
         long k = row + (long)n1 * k2;
         T h1 = T(0.5) * ((a.x + c.x) - (a.y - c.y));
         T h2 = T(0.5) * ((a.y + c.y) + (a.x - c.x));
-        H[k + c0 * ldh] = h1;
-        if (has1) H[k + c1 * ldh] = h2;
+        if (pairs_out) {
+            reinterpret_cast<cplx<T> *>(H)[pair * n + k] = {h1, has1 ? h2 : T(0)};
+        } else {
+            H[k + c0 * ldh] = h1;
+            if (has1) H[k + c1 * ldh] = h2;
+        }
     }
 }
 
@@ -421,6 +428,49 @@ __global__ __launch_bounds__(256) void k_hash_gather(long m, long ncols, long nb
     }
     for (; q < q1; ++q) s = s + bcoef[q] * x[bsrc[q]];
     Y[h + c * ldy] = s;
+}
+
+// The same for column pairs stored interleaved (X2[pair n + i] = (column
+// 2 pair, column 2 pair + 1) of row i): one 2-element read per member serves
+// both columns; each column's sum keeps the source order.
+template <class T>
+__global__ __launch_bounds__(256) void k_hash_gather_pairs(long m, long n, long ncols, long nbh,
+                                                           const int32_t *__restrict__ bptr,
+                                                           const int32_t *__restrict__ bsrc,
+                                                           const T *__restrict__ bcoef,
+                                                           const cplx<T> *__restrict__ X2,
+                                                           T *__restrict__ Y, long ldy) {
+#pragma clang fp contract(off)
+    const long npairs = (ncols + 1) / 2;
+    const long lb = xcd_local_block(blockIdx.x, gridDim.x);
+    const long pr = lb / nbh;
+    const long h = (lb % nbh) * 256 + threadIdx.x;
+    if (h >= m || pr >= npairs) return;
+    const int q0 = bptr[h], q1 = bptr[h + 1];
+    const cplx<T> *x = X2 + pr * n;
+    T s0 = 0, s1 = 0;
+    int q = q0;
+    for (; q + 4 <= q1; q += 4) { // four reads in flight, summed in source order
+        const int i0 = bsrc[q], i1 = bsrc[q + 1], i2 = bsrc[q + 2], i3 = bsrc[q + 3];
+        const T w0 = bcoef[q], w1 = bcoef[q + 1], w2 = bcoef[q + 2], w3 = bcoef[q + 3];
+        const cplx<T> x0 = x[i0], x1 = x[i1], x2 = x[i2], x3 = x[i3];
+        s0 = s0 + w0 * x0.x;
+        s1 = s1 + w0 * x0.y;
+        s0 = s0 + w1 * x1.x;
+        s1 = s1 + w1 * x1.y;
+        s0 = s0 + w2 * x2.x;
+        s1 = s1 + w2 * x2.y;
+        s0 = s0 + w3 * x3.x;
+        s1 = s1 + w3 * x3.y;
+    }
+    for (; q < q1; ++q) {
+        const T w = bcoef[q];
+        const cplx<T> xv = x[bsrc[q]];
+        s0 = s0 + w * xv.x;
+        s1 = s1 + w * xv.y;
+    }
+    Y[h + 2 * pr * ldy] = s0;
+    if (2 * pr + 1 < ncols) Y[h + (2 * pr + 1) * ldy] = s1;
 }
 
 template <class T>
@@ -865,7 +915,8 @@ template <class T> static const FftPlanDev<T> &get_plan(long n, Arena &ar) {
 // four-step into H (n x ncols, T)
 template <class T>
 static void dht_fourstep(const FftPlanDev<T> &P, long ncols, const double *X, long ldx,
-                         const double *D, double scale, T *H, long ldh, Arena &ar) {
+                         const double *D, double scale, T *H, long ldh, Arena &ar,
+                         int pairs_out = 0) {
     constexpr int NT = fft_nt<T>();
     long npairs = (ncols + 1) / 2;
     cplx<T> *Y = ar.alloc_n<cplx<T>>((size_t)npairs * P.n);
@@ -881,7 +932,8 @@ static void dht_fourstep(const FftPlanDev<T> &P, long ncols, const double *X, lo
         int half = P.n1 / 2;
         dim3 g((unsigned)((half + 1 + P.K1 - 1) / P.K1), (unsigned)npairs);
         size_t sh = sizeof(cplx<T>) * (size_t)P.n2 * 2 * P.K1;
-        k_fft4_pass2<T, NT><<<g, NT, sh, s>>>(P.n, P.n1, P.n2, P.K1, ncols, P.d2, P.tw2, Y, H, ldh);
+        k_fft4_pass2<T, NT><<<g, NT, sh, s>>>(P.n, P.n1, P.n2, P.K1, ncols, P.d2, P.tw2, Y, H, ldh,
+                                              pairs_out);
         SKL_CHECK_LAUNCH();
     }
 }
@@ -914,12 +966,21 @@ void dht_sketch(long n, long ncols, const double *X, long ldx, const double *D, 
         for (long c0 = 0; c0 < ncols; c0 += chunk) {
             const long nc = std::min(chunk, ncols - c0);
             const size_t mark = ar.used();
-            T *H = ar.alloc_n<T>((size_t)n * nc);
-            if (P.mode == 2)
-                dht_fourstep<T>(P, nc, X + c0 * ldx, ldx, D, scale, H, n, ar);
-            else
+            T *H = ar.alloc_n<T>((size_t)n * (nc + 1));
+            if (P.mode == 2) {
+                // pairs interleaved, gathered two columns per read
+                dht_fourstep<T>(P, nc, X + c0 * ldx, ldx, D, scale, H, n, ar, 1);
+                const long nbh = (m + 255) / 256, npairs = (nc + 1) / 2;
+                SKL_REQUIRE(nbh * npairs <= 2147483647L, SKL_E_UNSUPPORTED,
+                            "bucket gather: grid too large");
+                k_hash_gather_pairs<T><<<(unsigned)(nbh * npairs), 256, 0, s>>>(
+                    m, n, nc, nbh, bptr, bsrc, bcoef, reinterpret_cast<const cplx<T> *>(H),
+                    SA + c0 * ldsa, ldsa);
+                SKL_CHECK_LAUNCH();
+            } else {
                 dht_bluestein<T>(n, nc, X + c0 * ldx, ldx, D, scale, H, n, ar);
-            hash_gather<T>(m, nc, bptr, bsrc, bcoef, H, n, SA + c0 * ldsa, ldsa);
+                hash_gather<T>(m, nc, bptr, bsrc, bcoef, H, n, SA + c0 * ldsa, ldsa);
+            }
             if (chunk < ncols) ar.rewind(mark);
         }
     }
