@@ -511,64 +511,90 @@ void trinv_upper(long r, const double *R, long ldr, double perturb, double *M, l
 // ---------------------------------------------------------------------------
 // Cholesky factorization G = R^T R (upper R) of a symmetric positive definite
 // d x d matrix, blocked right-looking with 64-column blocks: the diagonal
-// block in registers (one workgroup), the block row by forward substitution
+// block in one workgroup, the block row by forward substitution
 // (one wave per four columns), the trailing matrix by a GEMM on the matrix
 // cores (upper tiles only).
 // ---------------------------------------------------------------------------
 constexpr int CH_B = 64;
 
-// b x b diagonal block (b <= 64, padded with the identity), in registers:
-// thread (i, g) of 64 x 4 holds entries (i, g + 4q), q < 16.  Step k
-// publishes row k to LDS (double-buffered, so one barrier per step); every
-// thread takes the pivot and row k from there; row k becomes R_k. and the
-// rows below subtract R_ki R_k. over all 64 columns, so the lower triangle is
-// unused scratch (never read as data, finite: products of R entries).
-// Writes U (zeros below the diagonal) over A.  A pivot that is not positive
-// and finite sets *fail and is replaced by 1 so the sweep completes.
-constexpr int CH_NT = 256, CH_Q = CH_B * CH_B / CH_NT; // 16 entries per thread
+constexpr int CH_NT = 256;
+// Cholesky of the b x b diagonal block (upper triangle of A, b <= 64) in
+// place: R upper, zeros below; *fail = 1 when a pivot is not positive and
+// finite (the step then goes on with a unit pivot).  Blocked by 16 steps with
+// no barrier inside a block: wave 0 runs the steps, lane p for column p,
+// reading column k of the Schur complement as it was at the block's start
+// (LDS, the full symmetric matrix) minus the block's earlier rows of R (the
+// pivot column's entries by v_readlane), with 1/sqrt of the pivot from the
+// hardware estimate and two Newton steps -- a few dependent operations per
+// step; between blocks the four waves update the Schur complement by the
+// block's 16 rows on the matrix cores.
+constexpr int CH_LD = CH_B + 1;
 __global__ __launch_bounds__(CH_NT) void k_chol_diag(int b, double *A, long lda, int *fail) {
-    constexpr int G = CH_NT / CH_B;
-    __shared__ double rows[2][CH_B];
-    const int i = threadIdx.x % CH_B, g = threadIdx.x / CH_B;
-    double a[CH_Q];
-#pragma unroll
-    for (int q = 0; q < CH_Q; ++q) {
-        const int c = g + G * q;
-        a[q] = (i < b && c < b) ? (i <= c ? A[i + (long)c * lda] : 0.0) : (i == c ? 1.0 : 0.0);
+    static_assert(CH_NT == 256 && CH_B == 64, "four waves over a 64 x 64 block");
+    __shared__ double Ss[CH_B * CH_LD]; // Schur complement at the block's start
+    __shared__ double Rb[CH_B * CH_LD]; // rows of R by column
+    const int tid = threadIdx.x, p = tid & 63, g = tid >> 6;
+    for (int e = tid; e < CH_B * CH_B; e += CH_NT) {
+        const int i = e & 63, c = e >> 6;
+        const double v = (i < b && c < b) ? (i <= c ? A[i + (long)c * lda] : A[c + (long)i * lda])
+                                          : (i == c ? 1.0 : 0.0);
+        Ss[i + c * CH_LD] = v;
+        Rb[i + c * CH_LD] = 0.0;
     }
+    __syncthreads();
     bool bad = false;
-    for (int k = 0; k < b; ++k) {
-        double *rw = rows[k & 1];
-        if (i == k)
+    for (int k0 = 0; k0 < b; k0 += 16) {
+        if (g == 0) {
+            double rl[16];
 #pragma unroll
-            for (int q = 0; q < CH_Q; ++q) rw[g + G * q] = a[q];
+            for (int t = 0; t < 16; ++t) rl[t] = 0.0;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int k = k0 + t;
+                if (k < b) {
+                    double sc = Ss[p + k * CH_LD];
+#pragma unroll
+                    for (int u = 0; u < t; ++u) sc = fma(-readlane(rl[u], k), rl[u], sc);
+                    double d = readlane(sc, k);
+                    const bool pb = !(d > 0.0) || !(d < INFINITY);
+                    bad = bad || pb;
+                    if (pb) d = 1.0;
+                    const double inv = rsqrt_nr(d);
+                    rl[t] = p == k ? d * inv : ((p > k && p < b) ? sc * inv : 0.0);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                if (k0 + t < b) Rb[(k0 + t) + p * CH_LD] = rl[t];
+        }
         __syncthreads();
-        double p = rw[k];
-        const bool pb = !(p > 0.0) || !(p < INFINITY);
-        bad = bad || pb;
-        if (pb) p = 1.0;
-        // A dependent FP64 operation costs tens of cycles on one wave, and
-        // the pivot is on the critical path of every step: 1/sqrt(p) by the
-        // hardware estimate and two Newton steps, then R_kc = a_kc / sqrt(p)
-        // and a_ic -= (a_ki / p) a_kc, one dependent operation each.
-        double y = __builtin_amdgcn_rsq(p);
-        const double h = 0.5 * p;
-        y = y * fma(-h, y * y, 1.5);
-        y = y * fma(-h, y * y, 1.5);
-        const double ip = y * y;     // 1 / p
-        const double wki = rw[i] * ip; // R_ki / sqrt(p) (used by rows i > k)
-        double rc[CH_Q]; // all reads before the arithmetic
+        if (k0 + 16 >= b) break;
+        // S -= R(k0:k0+16, :)^T R(k0:k0+16, :): 16 x 16 tiles, four per wave,
+        // each read and written by its own wave only
+        {
+            typedef double acc4 __attribute__((ext_vector_type(4)));
+            const int li = p & 15, lk = p >> 4;
 #pragma unroll
-        for (int q = 0; q < CH_Q; ++q) rc[q] = rw[g + G * q];
+            for (int q = 0; q < 4; ++q) {
+                const int tile = g * 4 + q, ti = tile & 3, tj = tile >> 2;
+                acc4 c;
 #pragma unroll
-        for (int q = 0; q < CH_Q; ++q)
-            a[q] = i == k ? rc[q] * y : (i > k ? fma(-wki, rc[q], a[q]) : a[q]);
+                for (int r = 0; r < 4; ++r) c[r] = Ss[(16 * ti + lk + 4 * r) + (16 * tj + li) * CH_LD];
+#pragma unroll
+                for (int u0 = 0; u0 < 16; u0 += 4)
+                    c = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        -Rb[(k0 + u0 + lk) + (16 * ti + li) * CH_LD],
+                        Rb[(k0 + u0 + lk) + (16 * tj + li) * CH_LD], c, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Ss[(16 * ti + lk + 4 * r) + (16 * tj + li) * CH_LD] = c[r];
+            }
+        }
+        __syncthreads();
     }
-    if (bad && threadIdx.x == 0) *fail = 1;
-#pragma unroll
-    for (int q = 0; q < CH_Q; ++q) {
-        const int c = g + G * q;
-        if (i < b && c < b) A[i + (long)c * lda] = i <= c ? a[q] : 0.0;
+    if (g == 0 && __ballot(bad) != 0 && p == 0) *fail = 1;
+    for (int e = tid; e < CH_B * CH_B; e += CH_NT) {
+        const int i = e & 63, c = e >> 6;
+        if (i < b && c < b) A[i + (long)c * lda] = i <= c ? Rb[i + c * CH_LD] : 0.0;
     }
 }
 
