@@ -4,6 +4,7 @@
 #include <cmath>
 
 #include "k_common.h"
+#include "k_tri64.h"
 
 namespace skl {
 
@@ -529,10 +530,13 @@ constexpr int CH_NT = 256;
 // step; between blocks the four waves update the Schur complement by the
 // block's 16 rows on the matrix cores.
 constexpr int CH_LD = CH_B + 1;
-__global__ __launch_bounds__(CH_NT) void k_chol_diag(int b, double *A, long lda, int *fail) {
+static_assert(CH_LD == CQ_LD, "the triangular inverse reads the block with ld CQ_LD");
+__global__ __launch_bounds__(CH_NT) void k_chol_diag(int b, double *A, long lda, int *fail,
+                                                   double *Uinv) {
     static_assert(CH_NT == 256 && CH_B == 64, "four waves over a 64 x 64 block");
     __shared__ double Ss[CH_B * CH_LD]; // Schur complement at the block's start
     __shared__ double Rb[CH_B * CH_LD]; // rows of R by column
+    __shared__ double Ys[CH_B * CH_LD]; // scratch of the inverse
     const int tid = threadIdx.x, p = tid & 63, g = tid >> 6;
     for (int e = tid; e < CH_B * CH_B; e += CH_NT) {
         const int i = e & 63, c = e >> 6;
@@ -596,46 +600,45 @@ __global__ __launch_bounds__(CH_NT) void k_chol_diag(int b, double *A, long lda,
         const int i = e & 63, c = e >> 6;
         if (i < b && c < b) A[i + (long)c * lda] = i <= c ? Rb[i + c * CH_LD] : 0.0;
     }
+    if (!Uinv) return;
+    // U^-1 (64 x 64, ld 64; identity past b) for the block row's product
+    if (tid < CH_B && tid >= b) Rb[tid + tid * CH_LD] = 1.0;
+    __syncthreads();
+    tri_inv_upper_lds(Rb, Ss, Ys);
+    for (int e = tid; e < CH_B * CH_B; e += CH_NT) Uinv[e] = Ss[(e & 63) + (e >> 6) * CH_LD];
 }
 
-// X = U^{-T} B for the b x b upper factor U (b <= 64) and B (b x nc), in
-// place, by forward substitution.  One wave per CH_TC columns, lane p = row p:
-// lane p keeps U(0:p, p) in registers, and step i takes x_i = b_i / U_ii from
-// lane i (v_readlane; every lane ran the previous step) and subtracts
-// U_ip x_i in every lane (U_ip = 0 for p <= i).  The dependent chain is a
-// few operations per row, where one thread per column would chain one FMA
-// per entry, and the loads and stores of B are coalesced along the rows.
-constexpr int CH_TC = 4;
-__global__ __launch_bounds__(64) void k_chol_trsm(int b, const double *U, long ldu, double *B,
-                                                  long ldb, long nc) {
+constexpr int CH_TC = 4; // columns of B per wave in the block-row product
+
+// X = U^{-T} B in place for the b x b upper factor U (b <= 64) and B (b x nc)
+// from Uinv = U^{-1} (k_chol_diag; upper, ld 64): x_p = sum_{i <= p} Uinv(i, p) b_i, lane p = row p with column p of
+// Uinv in registers, b staged through LDS into the row-broadcast layout so
+// that every term is one fused DPP FMA (bcast16_fma) -- independent terms
+// instead of a forward substitution's 64-step chain.
+__global__ __launch_bounds__(64) void k_chol_xmul(int b, const double *Uinv, double *B, long ldb,
+                                                  long nc) {
+    __shared__ double sb[CH_TC][CH_B];
     const int p = threadIdx.x;
-    const int pc = p < b ? p : b - 1; // clamped: every load stays inside the block
     double u[CH_B];
 #pragma unroll
-    for (int i = 0; i < CH_B; ++i) {
-        const double v = U[(i < b ? i : b - 1) + (long)pc * ldu];
-        u[i] = (i < p && p < b) ? v : 0.0; // U_ip, strictly upper
-    }
-    const double rdp = 1.0 / U[pc + (long)pc * ldu];
+    for (int i = 0; i < CH_B; ++i) u[i] = (i < b && p < b && i <= p) ? Uinv[i + CH_B * p] : 0.0;
     const long c0 = (long)blockIdx.x * CH_TC;
-    double x[CH_TC];
+#pragma unroll
+    for (int t = 0; t < CH_TC; ++t) sb[t][p] = (p < b && c0 + t < nc) ? B[p + (c0 + t) * ldb] : 0.0;
+    __syncthreads();
 #pragma unroll
     for (int t = 0; t < CH_TC; ++t) {
-        const long c = c0 + t < nc ? c0 + t : nc - 1;
-        const double v = B[pc + c * ldb];
-        x[t] = (p < b && c0 + t < nc) ? v : 0.0;
+        double vb[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) vb[q] = sb[t][16 * q + (p & 15)];
+        dpp_src_fence(vb);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        static_for<0, CH_B>([&](auto rc) {
+            constexpr int R = decltype(rc)::value;
+            bcast16_fma<R>(acc[R & 3], vb[R >> 4], u[R]);
+        });
+        if (p < b && c0 + t < nc) B[p + (c0 + t) * ldb] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     }
-#pragma unroll
-    for (int i = 0; i < CH_B; ++i) {
-        if (i < b) {
-            const double ri = readlane(rdp, i);
-#pragma unroll
-            for (int t = 0; t < CH_TC; ++t) x[t] = fma(-u[i], readlane(x[t], i) * ri, x[t]);
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < CH_TC; ++t)
-        if (p < b && c0 + t < nc) B[p + (c0 + t) * ldb] = x[t] * rdp;
 }
 
 // R (upper, zeros below, ld ldr) from G (overwritten); false when a pivot
@@ -645,6 +648,8 @@ bool cholesky_upper(long d, double *G, long ldg, double *R, long ldr, Arena &ar)
     const size_t mark = ar.used();
     int *fail = ar.alloc_n<int>(1);
     SKL_HIP(hipMemsetAsync(fail, 0, sizeof(int), s));
+    // the diagonal blocks' inverses, one slot per block
+    double *uinv = ar.alloc_n<double>((size_t)((d + CH_B - 1) / CH_B) * CH_B * CH_B);
     // Look-ahead: after block j's row B_j is solved, the main stream updates
     // only the next block row (what block j + 1 factors and solves next); the
     // rest of the trailing update runs on the context's second stream beside
@@ -658,11 +663,11 @@ bool cholesky_upper(long d, double *G, long ldg, double *R, long ldr, Arena &ar)
         const int b = (int)std::min<long>(CH_B, d - j);
         const long nc = d - j - b;
         double *Gjj = G + j + j * ldg;
-        k_chol_diag<<<1, CH_NT, 0, s>>>(b, Gjj, ldg, fail);
+        double *Uinv = nc > 0 ? uinv + (j / CH_B) * CH_B * CH_B : nullptr;
+        k_chol_diag<<<1, CH_NT, 0, s>>>(b, Gjj, ldg, fail, Uinv);
         if (nc > 0) {
             double *Bp = G + j + (j + b) * ldg;
-            k_chol_trsm<<<(unsigned)((nc + CH_TC - 1) / CH_TC), 64, 0, s>>>(b, Gjj, ldg, Bp, ldg,
-                                                                            nc);
+            k_chol_xmul<<<(unsigned)((nc + CH_TC - 1) / CH_TC), 64, 0, s>>>(b, Uinv, Bp, ldg, nc);
             if (pending) SKL_HIP(hipStreamWaitEvent(s, c.ev[1], 0));
             pending = false;
             // next block row: rows j + b .. j + b + b2 of the trailing matrix

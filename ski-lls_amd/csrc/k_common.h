@@ -3,6 +3,8 @@
 
 #include "skl_internal.h"
 
+#include <type_traits>
+
 namespace skl {
 
 template <class T> void fill2d(long m, long n, T *A, long lda, T v);
@@ -50,6 +52,32 @@ __device__ inline float rcp_nr(float x) {
 }
 // sqrt(x) for x >= 0 as x / sqrt(x) (0 for 0)
 template <class T> __device__ inline T sqrt_nr(T x) { return x > T(0) ? x * rsqrt_nr(x) : T(0); }
+
+// Compile-time loop: f(std::integral_constant<int, i>{}) for i in [I0, I1).
+template <int I0, int I1, class F> __device__ inline void static_for(F &&f) {
+    if constexpr (I0 < I1) {
+        f(std::integral_constant<int, I0>{});
+        static_for<I0 + 1, I1>(f);
+    }
+}
+
+// acc += v * x where v is lane R % 16 of the caller's row of sixteen lanes of
+// src (DPP row_newbcast on the FMA's first operand: lane l of src holding
+// entry 16 c + l % 16 of a vector, every lane gets entry R from its own row).
+// A DPP source must not be written by the two VALU instructions before the
+// read: callers fence src (dpp_src_fence) after writing it.
+template <int R> __device__ inline void bcast16_fma(double &acc, double src, double x) {
+    asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                 : "+v"(acc)
+                 : "v"(src), "v"(x), "i"(R & 15));
+}
+template <int N> __device__ inline void dpp_src_fence(double (&v)[N]) {
+    static_assert(N <= 5, "fence up to five sources");
+    if constexpr (N == 4)
+        asm volatile("s_nop 1" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]));
+    else
+        asm volatile("s_nop 1" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]));
+}
 
 template <class T> __device__ inline T wave_sum(T v) {
 #pragma unroll

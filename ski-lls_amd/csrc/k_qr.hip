@@ -1296,25 +1296,12 @@ template <int S, class T> __device__ inline T yq_tail_sumsq(const T (&y)[YQ_ROWS
     return (q[0] + q[1]) + (q[2] + q[3]);
 }
 
-// Compile-time loop: f(std::integral_constant<int, i>{}) for i in [I0, I1).
-template <int I0, int I1, class F> __device__ inline void yq_static_for(F &&f) {
-    if constexpr (I0 < I1) {
-        f(std::integral_constant<int, I0>{});
-        yq_static_for<I0 + 1, I1>(f);
-    }
-}
-
 // acc += v[R] * x, with v[R] taken from lane R % 16 of each row of sixteen
-// lanes of vb (DPP row_newbcast: lane l of vb holds v[16 c + l % 16], so every
-// lane reads v[R] from its own row) -- one fused instruction per term, no
-// scalar round trip.  A DPP source must not be written by the two VALU
-// instructions before the read: vb is written once per step, and the caller
-// fences it (yq_fence) before the first use.
+// lanes of vb (lane l of vb[c] holds v[16 c + l % 16]) by a fused DPP
+// row-broadcast FMA: one instruction per term, no scalar round trip
 template <int R>
 __device__ inline void yq_bcast_fma(double &acc, const double (&vb)[5], double x) {
-    asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
-                 : "+v"(acc)
-                 : "v"(vb[R >> 4]), "v"(x), "i"(R & 15));
+    bcast16_fma<R>(acc, vb[R >> 4], x);
 }
 template <int R> __device__ inline void yq_bcast_fma(float &acc, const float (&vb)[5], float x) {
     const float v = __int_as_float(
@@ -1324,15 +1311,17 @@ template <int R> __device__ inline void yq_bcast_fma(float &acc, const float (&v
 template <class T> __device__ inline void yq_fence(T (&vb)[5]) {
     asm volatile("s_nop 1" : "+v"(vb[0]), "+v"(vb[1]), "+v"(vb[2]), "+v"(vb[3]), "+v"(vb[4]));
 }
+// (float: the broadcast goes through v_mov_dpp, whose hazards the compiler
+// tracks)
 
 // rows [S, live) of the column, chunk by chunk (a chunk is skipped when it
 // starts at or beyond live), with compile-time row numbers
 template <int S, class F> __device__ inline void yq_rows_static(int live, F &&f) {
-    yq_static_for<0, YQ_NCH>([&](auto cc) {
+    static_for<0, YQ_NCH>([&](auto cc) {
         constexpr int c = decltype(cc)::value;
         if constexpr (YQ_CH * c + YQ_CH > S) {
             if (YQ_CH * c < live) {
-                yq_static_for<0, YQ_CH>([&](auto ii) {
+                static_for<0, YQ_CH>([&](auto ii) {
                     constexpr int r = YQ_CH * c + decltype(ii)::value;
                     if constexpr (r >= S && r < YQ_ROWS) f(std::integral_constant<int, r>{});
                 });
