@@ -1289,13 +1289,6 @@ __device__ inline bool yq_for_steps(int nsteps, F &&step, G &&regroup) {
     return true;
 }
 
-// sum of squares of the rows below the pivot row (four partial sums)
-template <int S, class T> __device__ inline T yq_tail_sumsq(const T (&y)[YQ_ROWS], int live) {
-    T q[4] = {0, 0, 0, 0};
-    YQ_ROWS_FROM(S + 1, q[(r - S - 1) & 3] = fma(y[r], y[r], q[(r - S - 1) & 3]);)
-    return (q[0] + q[1]) + (q[2] + q[3]);
-}
-
 // acc += v[R] * x, with v[R] taken from lane R % 16 of each row of sixteen
 // lanes of vb (lane l of vb[c] holds v[16 c + l % 16]) by a fused DPP
 // row-broadcast FMA: one instruction per term, no scalar round trip
@@ -1393,29 +1386,32 @@ __device__ inline void yq_load(T (&y)[YQ_ROWS], T &dn, T &en, const T *Y, long l
 }
 
 // The reflector (dlarfg) of the wave's column in lane wl over its live rows
-// jj.. (pivot row y[S]): tau and the scale sc of the rows below the pivot,
-// v[S] = 1, v[r] = sc y[r], from alpha = y[S] and xs, that lane's sum of
-// squares of its rows below the pivot row (wave-uniform results).  The lane
-// publishes its raw rows S.. in pairs (dst 16-byte aligned); a reader forms
-// v from them with yq_v.
+// jj.. (pivot row y[S]): the lane stores its raw rows S.. in pairs to `rows`
+// (LDS, 16-byte aligned), and the wave reads them back spread over its lanes
+// (a wave's LDS operations complete in order) for alpha = y[S] and the sum of
+// squares of the rows below it, summed by DPP lane moves; then tau and the
+// scale sc of the rows below the pivot (v[S] = 1, v[r] = sc y[r]; a reader
+// forms v with yq_v).  Wave-uniform results.
 template <class T> struct alignas(2 * sizeof(T)) yq_pair { T a, b; };
 template <int S, class T>
-__device__ inline void yq_publish(const T (&y)[YQ_ROWS], T xs, int wl, int lane, T *dst, int live,
+__device__ inline void yq_publish(const T (&y)[YQ_ROWS], int wl, int lane, T *rows, int live,
                                   T &tau, T &sc) {
-    const T alpha = readlane(y[S], wl);
-    const T xnorm2 = readlane(xs, wl);
+    if (lane == wl) {
+        yq_pair<T> *d2 = reinterpret_cast<yq_pair<T> *>(rows);
+#pragma unroll
+        for (int r0 = S & ~1; r0 < YQ_ROWS; r0 += 2)
+            if (YQ_CH * (r0 / YQ_CH) < live) d2[r0 / 2] = {y[r0], r0 + 1 < YQ_ROWS ? y[r0 + 1] : T(0)};
+    }
+    const T x0 = (lane > S && lane < live) ? rows[lane] : T(0);
+    const T x1 = (lane + 64 < live) ? rows[64 + lane] : T(0);
+    const T xnorm2 = wave_sum_bcast(fma(x1, x1, x0 * x0));
+    const T alpha = rows[S];
     tau = 0;
     sc = 0;
     if (xnorm2 != T(0)) {
         const T beta = -copysign(sqrt_nr(fma(alpha, alpha, xnorm2)), alpha);
         tau = (beta - alpha) * rcp_nr(beta);
         sc = rcp_nr(alpha - beta);
-    }
-    if (lane == wl) {
-        yq_pair<T> *d2 = reinterpret_cast<yq_pair<T> *>(dst);
-#pragma unroll
-        for (int r0 = S & ~1; r0 < YQ_ROWS; r0 += 2)
-            if (YQ_CH * (r0 / YQ_CH) < live) d2[r0 / 2] = {y[r0], r0 + 1 < YQ_ROWS ? y[r0 + 1] : T(0)};
     }
 }
 
@@ -1434,6 +1430,7 @@ template <class T, int W>
 __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
     __shared__ T s_wv[W];
     __shared__ int s_wp[W], s_wl[W];
+    __shared__ __align__(16) T s_rows[80]; // the workgroup's candidate's raw rows
     extern __shared__ unsigned char yq_smem[];
     // nothing to do when the single-workgroup search already chose the pivots
     if (a.skip && *a.skip != 0) return;
@@ -1457,7 +1454,6 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
         constexpr int S = decltype(Sc)::value;
         const int buf = jj & 1;
         SKL_PHASE(1, 0);
-        const T xs = yq_tail_sumsq<S>(y, live);
         // best column of this wave (every lane ends with the same answer) ...
         T bv = T(-1);
         int bp = NONE, bl = 0;
@@ -1487,10 +1483,12 @@ __global__ __launch_bounds__(64 * W) void k_yqrp_lanes(YqArgs<T> a) {
                 gw = u;
             }
         if (wave == gw) {
-            // raw rows in cvec, sc after them (entry YQ_MAXROWS - 1)
+            // raw rows to cvec through LDS, sc after them (entry YQ_MAXROWS - 1)
             T *dst = a.cvec + ((size_t)buf * P + blk) * YQ_MAXROWS;
             T tau, sc;
-            yq_publish<S>(y, xs, s_wl[gw], lane, dst, live, tau, sc);
+            yq_publish<S>(y, s_wl[gw], lane, s_rows, live, tau, sc);
+            dst[lane] = s_rows[lane];
+            if (lane < 80 - 64) dst[64 + lane] = s_rows[64 + lane];
             if (lane == 0) {
                 dst[YQ_MAXROWS - 1] = sc;
                 a.cand[buf * P + blk] = gv;
@@ -1702,7 +1700,6 @@ __global__ __launch_bounds__(64 * YT_W) void k_yqrp_top(YtArgs<T> a) {
     auto step = [&](auto Sc, int jj, int live) -> bool {
         constexpr int S = decltype(Sc)::value;
         const int buf = jj & 1;
-        const T xs = yq_tail_sumsq<S>(y, live);
         SKL_PHASE(3, 2);
         // every wave publishes its best column's raw rows and reflector scalars
         {
@@ -1711,7 +1708,7 @@ __global__ __launch_bounds__(64 * YT_W) void k_yqrp_top(YtArgs<T> a) {
             const int wl = wave_argmax(cand, pos);
             if (wl >= 0) {
                 T tau, sc;
-                yq_publish<S>(y, xs, wl, lane, &s_v[buf][wave][0], live, tau, sc);
+                yq_publish<S>(y, wl, lane, &s_v[buf][wave][0], live, tau, sc);
                 if (lane == 0) {
                     s_bv[buf][wave] = readlane(cand, wl);
                     s_bp[buf][wave] = __builtin_amdgcn_readlane(pos, wl);
