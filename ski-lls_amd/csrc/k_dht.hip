@@ -134,8 +134,8 @@ __device__ __forceinline__ void fft_stage(cplx<T> *buf, int L, int S, int len, i
             for (int k = 0; k < R; ++k) {
                 cplx<T> o = y[k];
                 if (k > 0) {
-                    int e = (int)(((long)p * k) % len) * tws;
-                    o = cmul(o, tw[e]);
+                    // p < len / R and k < R, so p k < len: no reduction
+                    o = cmul(o, tw[p * k * tws]);
                 }
                 buf[(q + stride * (R * p + k)) * S + s] = o;
             }
