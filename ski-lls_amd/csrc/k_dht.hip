@@ -293,8 +293,12 @@ __global__ __launch_bounds__(NT, 2048 / NT) void k_fft4_pass1(long n, int n1, in
         const int e = threadIdx.x + u * NT;
         if (e < tot) buf[e] = v[u]; // e = j1 J2 + jj
     }
+    // the stage twiddles in LDS after the data: an LDS read per butterfly
+    // output instead of a cache round trip per stage
+    cplx<T> *twl = buf + tot;
+    for (int e = threadIdx.x; e < n1; e += NT) twl[e] = tw1[e];
     __syncthreads();
-    lds_fft<T, NT>(buf, J2, dims, tw1);
+    lds_fft<T, NT>(buf, J2, dims, twl);
     cplx<T> *Yp = Y + pair * n;
     // twiddles first (all in flight together), then the products
 #pragma unroll
@@ -358,8 +362,10 @@ __global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K
             if (e < tot) buf[e] = v[u]; // e = j2 S + sl
         }
     }
+    cplx<T> *twl = buf + tot; // the stage twiddles in LDS after the data
+    for (int e = threadIdx.x; e < n2; e += NT) twl[e] = tw2[e];
     __syncthreads();
-    lds_fft<T, NT>(buf, S, dims, tw2);
+    lds_fft<T, NT>(buf, S, dims, twl);
     // outputs: for primary row k1 and column k2: k = k1 + n1 k2,
     //   partner n-k = (n1-k1) + n1 (n2-1-k2) (mirror slot) for k1 > 0,
     //               = n1 ((n2-k2) % n2)      (same slot)   for k1 = 0.
@@ -912,6 +918,11 @@ template <class T> static const FftPlanDev<T> &get_plan(long n, Arena &ar) {
     return res.first->second;
 }
 
+// dynamic LDS above 64 KB needs the kernel's attribute raised first
+static void lds_allow(const void *fn, size_t bytes) {
+    if (bytes > 65536) SKL_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
 // four-step into H (n x ncols, T)
 template <class T>
 static void dht_fourstep(const FftPlanDev<T> &P, long ncols, const double *X, long ldx,
@@ -923,7 +934,8 @@ static void dht_fourstep(const FftPlanDev<T> &P, long ncols, const double *X, lo
     hipStream_t s = ctx().stream;
     {
         dim3 g((unsigned)((P.n2 + P.J2 - 1) / P.J2), (unsigned)npairs);
-        size_t sh = sizeof(cplx<T>) * (size_t)P.n1 * P.J2;
+        size_t sh = sizeof(cplx<T>) * ((size_t)P.n1 * P.J2 + P.n1);
+        lds_allow((const void *)k_fft4_pass1<T, NT>, sh);
         k_fft4_pass1<T, NT><<<g, NT, sh, s>>>(P.n, P.n1, P.n2, P.J2, ncols, X, ldx, D, scale, P.d1,
                                               P.tw1, P.twn, Y);
         SKL_CHECK_LAUNCH();
@@ -931,7 +943,8 @@ static void dht_fourstep(const FftPlanDev<T> &P, long ncols, const double *X, lo
     {
         int half = P.n1 / 2;
         dim3 g((unsigned)((half + 1 + P.K1 - 1) / P.K1), (unsigned)npairs);
-        size_t sh = sizeof(cplx<T>) * (size_t)P.n2 * 2 * P.K1;
+        size_t sh = sizeof(cplx<T>) * ((size_t)P.n2 * 2 * P.K1 + P.n2);
+        lds_allow((const void *)k_fft4_pass2<T, NT>, sh);
         k_fft4_pass2<T, NT><<<g, NT, sh, s>>>(P.n, P.n1, P.n2, P.K1, ncols, P.d2, P.tw2, Y, H, ldh,
                                               pairs_out);
         SKL_CHECK_LAUNCH();
