@@ -211,14 +211,73 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
     }
 }
 
+// Out^T Out of a 64-row tile held in LDS (ld CQ_LD, zero past the rows and
+// columns): the transpose goes to T (LDS scratch) and the 64 x 64 product to
+// P (ld 64) on the matrix cores.  All CQ_T threads; T is free after return.
+__device__ inline void gram_tile_store(const double *O, double *T, double *P) {
+    for (int e = threadIdx.x; e < 64 * 64; e += CQ_T) {
+        const int i = e & 63, k = e >> 6;
+        T[i + k * CQ_LD] = O[k + i * CQ_LD]; // T(i, k) = O(k, i)
+    }
+    __syncthreads();
+    mm64_lds(T, O, [&](int r, int c, double v) { P[r + 64 * c] = v; });
+}
+
+// Partial Gram matrices of a tall panel (rows x b, b <= 64), one per 64-row
+// tile: P[t] = A(tile t, :)^T A(tile t, :) (64 x 64, zero past b), summed by
+// k_gram_reduce in tile order.  Replaces the split-K GEMM for the CholeskyQR
+// Gram matrices: a 64-row tile is one 64 x 64 x 64 product on MFMA.
+__global__ __launch_bounds__(CQ_T) void k_gram_tiles(long rows, int b, const double *A, long lda,
+                                                     double *P, const int *flag) {
+    __shared__ double sA[64 * CQ_LD];
+    __shared__ double sT[64 * CQ_LD];
+    if (flag && *flag != 0) return;
+    const long r0 = (long)blockIdx.x * 64;
+    for (int e = threadIdx.x; e < 64 * 64; e += CQ_T) {
+        const int i = e & 63, j = e >> 6;
+        sA[i + j * CQ_LD] = (j < b && r0 + i < rows) ? A[(r0 + i) + (long)j * lda] : 0.0;
+    }
+    __syncthreads();
+    gram_tile_store(sA, sT, P + (size_t)blockIdx.x * 64 * 64);
+}
+
+// G (b x b, ld b) = sum over the nt tiles of P[t] in increasing t: eight
+// threads per entry, each over every eighth tile (loads eight at a time),
+// combined in a fixed order by lane moves.
+__global__ __launch_bounds__(256) void k_gram_reduce(int nt, int b, const double *P, double *G,
+                                                     const int *flag) {
+    if (flag && *flag != 0) return;
+    const int e = blockIdx.x * 32 + (threadIdx.x >> 3), sub = threadIdx.x & 7;
+    double s = 0.0;
+    if (e < 64 * 64) {
+        int t = sub;
+        for (; t + 56 < nt; t += 64) {
+            double w[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) w[q] = P[(size_t)(t + 8 * q) * 4096 + e];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s += w[q];
+        }
+        for (; t < nt; t += 8) s += P[(size_t)t * 4096 + e];
+    }
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    s += __shfl_xor(s, 4, 64);
+    if (e < 64 * 64 && sub == 0) {
+        const int i = e & 63, j = e >> 6;
+        if (i < b && j < b) G[i + (long)j * b] = s;
+    }
+}
+
 // Out(i, :) = In(i, :) X for rows i < rows (X b x b, ld b): 64-row tiles of
 // In in LDS, thread t computes row t & 63 for every fourth column from
 // t >> 6.  Runs only when *flag is 0.
 __global__ __launch_bounds__(CQ_T) void k_cqr_rmul(long rows, int b, const double *In, long ldi,
                                                    const double *X, double *Out, long ldo,
-                                                   const int *flag) {
+                                                   const int *flag, double *Pg) {
     __shared__ double sI[64 * CQ_LD];
     __shared__ double sX[64 * 64];
+    __shared__ double sO[64 * CQ_LD]; // the output tile, for its Gram
     if (*flag != 0) return;
     const int tid = threadIdx.x, lane = tid & 63, grp = tid >> 6;
     const long r0 = (long)blockIdx.x * 64;
@@ -240,7 +299,12 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_rmul(long rows, int b, const doubl
 #pragma unroll
         for (int k = 0; k < 64; ++k) acc += xi[k] * sX[k + j * 64];
         if (row < rows && j < b) Out[row + (long)j * ldo] = acc;
+        sO[lane + j * CQ_LD] = (row < rows && j < b) ? acc : 0.0;
     }
+    if (!Pg) return;
+    // this tile's part of Out^T Out (64 x 64) for k_gram_reduce
+    __syncthreads();
+    gram_tile_store(sO, sI, Pg + (size_t)blockIdx.x * 64 * 64);
 }
 
 // Householder reconstruction of the panel from Q = Q1 Z2 (Q1 mp x b, ld ldq;
@@ -419,19 +483,22 @@ int *cqr_panel(long mp, int b, int pivot, double *Wp, long ldw, double *tau, int
     int32_t *swp = ar.alloc_n<int32_t>(64);
     int *flag = ar.alloc_n<int>(1);
     const unsigned gr = (unsigned)((mp + 63) / 64);
-    gemm<double>('T', 'N', b, b, mp, 1.0, Wp, ldw, Wp, ldw, 0.0, G, b);
+    // the Gram matrices from per-tile partials (one 64-row tile per workgroup)
+    double *Pg = ar.alloc_n<double>((size_t)gr * 64 * 64);
+    k_gram_tiles<<<gr, CQ_T, 0, s>>>(mp, b, Wp, ldw, Pg, nullptr);
+    k_gram_reduce<<<128, 256, 0, s>>>((int)gr, b, Pg, G, nullptr);
     if (pivot)
         k_cqr_chol<true><<<1, CQ_T, 0, s>>>(b, mp, G, b, R1, Z1, perm, swp, flag, hq_norms, 1);
     else
         k_cqr_chol<false><<<1, CQ_T, 0, s>>>(b, mp, G, b, R1, Z1, nullptr, nullptr, flag, 0, 1);
-    k_cqr_rmul<<<gr, CQ_T, 0, s>>>(mp, b, Wp, ldw, Z1, Q1, mp, flag);
-    gemm<double>('T', 'N', b, b, mp, 1.0, Q1, mp, Q1, mp, 0.0, G, b);
+    k_cqr_rmul<<<gr, CQ_T, 0, s>>>(mp, b, Wp, ldw, Z1, Q1, mp, flag, Pg);
+    k_gram_reduce<<<128, 256, 0, s>>>((int)gr, b, Pg, G, flag);
     k_cqr_chol<false><<<1, CQ_T, 0, s>>>(b, mp, G, b, R2, Z2, nullptr, nullptr, flag, 0, 2);
     k_cqr_hr<<<1, CQ_T, 0, s>>>(b, Q1, mp, Z2, R1, R2, perm, swp, Wp, ldw, tau, Tout, Zc,
                                  pivot ? jpvt : nullptr, pivot ? swaps : nullptr, flag);
     if (mp > b) {
         const unsigned g2 = (unsigned)((mp - b + 63) / 64);
-        k_cqr_rmul<<<g2, CQ_T, 0, s>>>(mp - b, b, Q1 + b, mp, Zc, Wp + b, ldw, flag);
+        k_cqr_rmul<<<g2, CQ_T, 0, s>>>(mp - b, b, Q1 + b, mp, Zc, Wp + b, ldw, flag, nullptr);
     }
     SKL_CHECK_LAUNCH();
     return flag;
