@@ -69,6 +69,20 @@ __global__ __launch_bounds__(GN_THREADS) void k_gemv_n_part(
     const double *a = A + i + jb * lda;
     if (i + 1 < n && ((lda & 1) == 0)) {
         long j = jb;
+        // eight columns' loads in flight before their products (the sums stay
+        // in column order)
+        for (; j + 8 <= j1; j += 8) {
+            double2 v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = *(const double2 *)(a + q * lda);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const double zq = zs[j - j0 + q];
+                a0 += v[q].x * zq;
+                a1 += v[q].y * zq;
+            }
+            a += 8 * lda;
+        }
         for (; j + 4 <= j1; j += 4) {
             double2 v0 = *(const double2 *)(a);
             double2 v1 = *(const double2 *)(a + lda);
@@ -156,52 +170,78 @@ __global__ __launch_bounds__(GT_THREADS) void k_gemv_t_part(
     }
 }
 
-// ---- M^T t for upper-triangular M: one wave per (row block, column) item ------
+// ---- M^T t for upper-triangular M: row-block partials ---------------------------
 // P[rb * r + j] = sum over rows i in block rb (GT_RB rows), i <= j, of
-// M[i, j] t[i].  The items are only those with j >= rb * GT_RB, numbered row
-// block by row block, so no workgroup is empty; entries with j < rb * GT_RB
-// are never written (the caller zeroes them once).
-__global__ __launch_bounds__(256) void k_trmv_t_items(long r, const double *__restrict__ M, long ldm,
-                                                      const double *__restrict__ t,
-                                                      double *__restrict__ P, int nrb,
-                                                      const LsqrState *__restrict__ st) {
+// M[i, j] t[i], for the columns j >= rb * GT_RB only; entries with
+// j < rb * GT_RB are never written (the caller zeroes them once).
+// A workgroup per (row block rb, TC_COLS consecutive columns j >= rb GT_RB):
+// the block's t in registers once (lane l holds t[i0 + l + 64 u]), each wave
+// over TC_COLS / 4 columns two at a time (32 loads in flight); per column the
+// products are summed by halving over the lane's values, then over the wave.
+constexpr int TC_COLS = 32;
+__global__ __launch_bounds__(256) void k_trmv_t_chunks(long r, const double *__restrict__ M, long ldm,
+                                                       const double *__restrict__ t,
+                                                       double *__restrict__ P, int nrb,
+                                                       const LsqrState *__restrict__ st) {
     if (st && st->done) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    long item = (long)blockIdx.x * 4 + w;
+    long chunk = blockIdx.x;
     int rb = 0;
     for (; rb < nrb; ++rb) {
-        const long cnt = r - (long)rb * GT_RB;
-        if (item < cnt) break;
-        item -= cnt;
+        const long cnt = (r - (long)rb * GT_RB + TC_COLS - 1) / TC_COLS;
+        if (chunk < cnt) break;
+        chunk -= cnt;
     }
     if (rb >= nrb) return;
-    const long j = (long)rb * GT_RB + item;
-    const long i0 = (long)rb * GT_RB, i1 = min(j + 1, i0 + GT_RB);
-    const double *c = M + j * ldm;
+    const long i0 = (long)rb * GT_RB, iend = min(r, i0 + GT_RB);
     constexpr int U = GT_RB / 64;
-    double s[U];
+    double tv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const long i = i0 + lane + 64 * u;
-        s[u] = i < i1 ? c[i] * t[i] : 0.0;
+        tv[u] = i < iend ? t[i] : 0.0;
     }
+    constexpr int CPW = TC_COLS / 4;
+    const long jb = i0 + chunk * TC_COLS + (long)w * CPW;
+    for (int c = 0; c < CPW; c += 2) {
+        double s0[U], s1[U];
+        const long j0 = jb + c, j1 = jb + c + 1;
+        const double *c0 = M + min(j0, r - 1) * ldm, *c1 = M + min(j1, r - 1) * ldm;
 #pragma unroll
-    for (int h = U / 2; h > 0; h >>= 1)
+        for (int u = 0; u < U; ++u) {
+            const long i = i0 + lane + 64 * u;
+            s0[u] = (j0 < r && i <= j0 && i < iend) ? c0[i] : 0.0;
+            s1[u] = (j1 < r && i <= j1 && i < iend) ? c1[i] : 0.0;
+        }
 #pragma unroll
-        for (int u = 0; u < h; ++u) s[u] += s[u + h];
-    const double a = wave_sum(s[0]);
-    if (lane == 0) P[(long)rb * r + j] = a;
+        for (int u = 0; u < U; ++u) {
+            s0[u] = s0[u] * tv[u];
+            s1[u] = s1[u] * tv[u];
+        }
+#pragma unroll
+        for (int h = U / 2; h > 0; h >>= 1)
+#pragma unroll
+            for (int u = 0; u < h; ++u) {
+                s0[u] += s0[u + h];
+                s1[u] += s1[u + h];
+            }
+        const double a0 = wave_sum(s0[0]), a1 = wave_sum(s1[0]);
+        if (lane == 0) {
+            if (j0 < r) P[(long)rb * r + j0] = a0;
+            if (j1 < r) P[(long)rb * r + j1] = a1;
+        }
+    }
+}
+static long trmv_t_chunks(long r) {
+    long n = 0;
+    for (long i0 = 0; i0 < r; i0 += GT_RB) n += (r - i0 + TC_COLS - 1) / TC_COLS;
+    return n;
 }
 
 __global__ void k_zero_d(long n, double *__restrict__ p) {
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = 0.0;
 }
 
-static long trmv_t_items(long r) {
-    long n = 0;
-    for (long i0 = 0; i0 < r; i0 += GT_RB) n += r - i0;
-    return n;
-}
 
 // ---- one pass over A: u <- A z - coef u and A^T u together ---------------------
 // The pass reads a row-major copy AT of A (rows of ldt doubles, ldt even, the
@@ -1331,8 +1371,8 @@ static void launch_op_t(const LsqrProblem &p, const LsqrPlan &q, const LsqrBufs 
                                            B.hist, B.hist_cap, init);
     } else {
         // Pm = M^T t (row-block partials; those below the diagonal blocks stay zero)
-        const unsigned g = (unsigned)((trmv_t_items(q.r) + 3) / 4);
-        launch_ev(k_trmv_t_items, dim3(g), dim3(256), 0, s, e6, e7, q.r, p.M, p.ldm, B.t, B.Pm,
+        const unsigned g = (unsigned)trmv_t_chunks(q.r);
+        launch_ev(k_trmv_t_chunks, dim3(g), dim3(256), 0, s, e6, e7, q.r, p.M, p.ldm, B.t, B.Pm,
                   q.nrb_m, B.st);
         launch_ev(k_update_v, dim3(nv), dim3(UV_T), 0, s, nullptr, nullptr, q.r, q.nrb_m, B.Pm,
                   B.Sb, nsb, B.v, B.Sv, B.st);
