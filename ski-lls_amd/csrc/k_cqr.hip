@@ -60,11 +60,10 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
     __shared__ int s_k;               // steps done (the first failed step when short)
     const int tid = threadIdx.x, p = tid & 63, g = tid >> 6;
     if (pass == 2 && *flag != 0) return;
-    for (int e = tid; e < 64 * 64; e += CQ_T) {
-        const int i = e & 63, j = e >> 6;
-        Ss[i + j * CQ_LD] = (i < b && j < b) ? G[i + (long)j * ldg] : 0.0;
-        Rb[i + j * CQ_LD] = 0.0;
-    }
+    tile64_fill(
+        Ss, [&](int i, int j) { return G[min(i, b - 1) + (long)min(j, b - 1) * ldg]; },
+        [&](int i, int j) { return i < b && j < b; });
+    for (int e = tid; e < 64 * CQ_LD; e += CQ_T) Rb[e] = 0.0;
     // wave 0 runs the steps, lane p for panel column p: its Schur diagonal,
     // partial norms, logical position, whether it is still active, and its
     // entries of the block's rows of R
@@ -233,10 +232,12 @@ __global__ __launch_bounds__(CQ_T) void k_gram_tiles(long rows, int b, const dou
     __shared__ double sT[64 * CQ_LD];
     if (flag && *flag != 0) return;
     const long r0 = (long)blockIdx.x * 64;
-    for (int e = threadIdx.x; e < 64 * 64; e += CQ_T) {
-        const int i = e & 63, j = e >> 6;
-        sA[i + j * CQ_LD] = (j < b && r0 + i < rows) ? A[(r0 + i) + (long)j * lda] : 0.0;
-    }
+    tile64_fill(
+        sA,
+        [&](int i, int j) {
+            return A[(r0 + i < rows ? r0 + i : rows - 1) + (long)min(j, b - 1) * lda];
+        },
+        [&](int i, int j) { return j < b && r0 + i < rows; });
     __syncthreads();
     gram_tile_store(sA, sT, P + (size_t)blockIdx.x * 64 * 64);
 }
@@ -276,16 +277,18 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_rmul(long rows, int b, const doubl
                                                    const double *X, double *Out, long ldo,
                                                    const int *flag, double *Pg) {
     __shared__ double sI[64 * CQ_LD];
-    __shared__ double sX[64 * 64];
+    __shared__ double sX[64 * CQ_LD];
     __shared__ double sO[64 * CQ_LD]; // the output tile, for its Gram
     if (*flag != 0) return;
     const int tid = threadIdx.x, lane = tid & 63, grp = tid >> 6;
     const long r0 = (long)blockIdx.x * 64;
-    for (int e = tid; e < 64 * 64; e += CQ_T) {
-        const int i = e & 63, j = e >> 6;
-        sX[e] = (i < b && j < b) ? X[i + (long)j * b] : 0.0;
-        sI[i + j * CQ_LD] = (j < b && r0 + i < rows) ? In[(r0 + i) + (long)j * ldi] : 0.0;
-    }
+    tile64_fill2(
+        sX, [&](int i, int j) { return X[min(i, b - 1) + (long)min(j, b - 1) * b]; },
+        [&](int i, int j) { return i < b && j < b; }, sI,
+        [&](int i, int j) {
+            return In[(r0 + i < rows ? r0 + i : rows - 1) + (long)min(j, b - 1) * ldi];
+        },
+        [&](int i, int j) { return j < b && r0 + i < rows; });
     __syncthreads();
     const long row = r0 + lane;
     // the row in registers (zero past b), then 16 columns per thread
@@ -297,7 +300,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_rmul(long rows, int b, const doubl
         const int j = grp + 4 * m;
         double acc = 0.0;
 #pragma unroll
-        for (int k = 0; k < 64; ++k) acc += xi[k] * sX[k + j * 64];
+        for (int k = 0; k < 64; ++k) acc += xi[k] * sX[k + j * CQ_LD];
         if (row < rows && j < b) Out[row + (long)j * ldo] = acc;
         sO[lane + j * CQ_LD] = (row < rows && j < b) ? acc : 0.0;
     }
@@ -335,11 +338,11 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long l
     const int tid = threadIdx.x, i = tid & 63, g = tid >> 6;
     // the top block of Q = Q1(0:64, :) Z2 (Q1's top block in Ys, Z2 in Lb,
     // the product into Xs)
-    for (int e = tid; e < 64 * 64; e += CQ_T) {
-        const int r = e & 63, c = e >> 6;
-        Ys[r + c * CQ_LD] = (r < b && c < b) ? Q1[r + (long)c * ldq] : 0.0;
-        Lb[r + c * CQ_LD] = (r < b && c < b && r <= c) ? Z2[r + (long)c * b] : 0.0;
-    }
+    tile64_fill2(
+        Ys, [&](int r, int c) { return Q1[min(r, b - 1) + (long)min(c, b - 1) * ldq]; },
+        [&](int r, int c) { return r < b && c < b; }, Lb,
+        [&](int r, int c) { return Z2[min(r, b - 1) + (long)min(c, b - 1) * b]; },
+        [&](int r, int c) { return r < b && c < b && r <= c; });
     __syncthreads();
     mm64_lds(Ys, Lb, [&](int r, int c, double v) { Xs[r + c * CQ_LD] = v; });
     __syncthreads();
@@ -412,10 +415,9 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long l
     if (tid < b) tau[tid] = -Ub[tid + tid * CQ_LD] * sg[tid];
     // top block: S R2 R1 on and above the diagonal, L1 below (R2 in Xs, R1
     // in Ys; both upper with zeros elsewhere, ld 64)
-    for (int e = tid; e < 64 * 64; e += CQ_T) {
-        Xs[(e & 63) + (e >> 6) * CQ_LD] = R2[e];
-        Ys[(e & 63) + (e >> 6) * CQ_LD] = R1[e];
-    }
+    tile64_fill2(
+        Xs, [&](int r, int c) { return R2[r + 64 * c]; }, [](int, int) { return true; }, Ys,
+        [&](int r, int c) { return R1[r + 64 * c]; }, [](int, int) { return true; });
     __syncthreads();
     mm64_lds(Xs, Ys, [&](int r, int c, double v) {
         if (r < b && c < b) Wp[r + (long)c * ldw] = r <= c ? sg[r] * v : Lb[r + c * CQ_LD];
@@ -454,10 +456,9 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long l
     }
     __syncthreads();
     tri_inv_upper_lds(Ub, Lb, Ys);
-    for (int e = tid; e < 64 * 64; e += CQ_T) {
-        const int r = e & 63, c = e >> 6;
-        Ys[r + c * CQ_LD] = (r < b && c < b && r <= c) ? Z2[r + (long)c * b] : 0.0;
-    }
+    tile64_fill(
+        Ys, [&](int r, int c) { return Z2[min(r, b - 1) + (long)min(c, b - 1) * b]; },
+        [&](int r, int c) { return r < b && c < b && r <= c; });
     __syncthreads();
     mm64_lds(Ys, Lb, [&](int r, int c, double v) {
         if (r < b && c < b) Zc[r + (long)c * b] = r <= c ? v : 0.0;

@@ -173,6 +173,40 @@ template <> __device__ inline float load_scaled<float>(const double *X, long idx
     double f = D ? scale * D[j] : scale;
     return (float)f * (float)X[idx];
 }
+template <class T> __device__ inline T scaled(double f, double x) { return (T)(f * x); }
+template <> __device__ inline float scaled<float>(double f, double x) { return (float)f * (float)x; }
+
+// The scaled entries (scale D[j]) A(j, c) of a column pair (c1 only when
+// has1; zero otherwise) at the rows row(u) >= 0 of the thread's FFT_MAXE
+// elements (zero where row(u) < 0), the same products as load_scaled.  The
+// raw values go to registers first, each load behind nothing but its row
+// test, and the products follow: a product next to its load would make the
+// compiler wait on each load in turn, one memory latency per element.
+template <class T, class ROW>
+__device__ inline void load_pair_scaled(const double *__restrict__ X, long ldx, long c0, long c1,
+                                        bool has1, const double *__restrict__ D, double scale,
+                                        ROW row, cplx<T> (&v)[FFT_MAXE]) {
+    double x0[FFT_MAXE], x1[FFT_MAXE], dd[FFT_MAXE];
+    const long c1c = has1 ? c1 : c0; // a valid column either way
+#pragma unroll
+    for (int u = 0; u < FFT_MAXE; ++u) {
+        const long j = row(u);
+        x0[u] = 0.0;
+        x1[u] = 0.0;
+        dd[u] = 0.0;
+        if (j >= 0) {
+            x0[u] = X[j + c0 * ldx];
+            x1[u] = X[j + c1c * ldx];
+            if (D) dd[u] = D[j];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < FFT_MAXE; ++u) {
+        const double f = D ? scale * dd[u] : scale;
+        v[u].x = scaled<T>(f, x0[u]);
+        v[u].y = has1 ? scaled<T>(f, x1[u]) : T(0);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Single pass: one workgroup per column pair (c0 = 2 blockIdx.x, c1 = c0 + 1).
@@ -193,15 +227,10 @@ __global__ __launch_bounds__(NT) void k_dht_single(long n, long ncols, const dou
     const bool has1 = c1 < ncols;
     {
         cplx<T> v[FFT_MAXE]; // n <= NT * FFT_MAXE; all loads in flight together
-#pragma unroll
-        for (int u = 0; u < FFT_MAXE; ++u) {
+        load_pair_scaled<T>(X, ldx, c0, c1, has1, D, scale, [&](int u) {
             const long j = threadIdx.x + (long)u * NT;
-            v[u] = {T(0), T(0)};
-            if (j < n) {
-                v[u].x = load_scaled<T>(X, j + c0 * ldx, D, j, scale);
-                v[u].y = has1 ? load_scaled<T>(X, j + c1 * ldx, D, j, scale) : T(0);
-            }
-        }
+            return j < n ? j : -1L;
+        }, v);
 #pragma unroll
         for (int u = 0; u < FFT_MAXE; ++u) {
             const long j = threadIdx.x + (long)u * NT;
@@ -277,17 +306,11 @@ __global__ __launch_bounds__(NT, 2048 / NT) void k_fft4_pass1(long n, int n1, in
     // elements are independent, and a loop that consumed each load before
     // issuing the next would pay the memory latency once per element
     cplx<T> v[FFT_MAXE];
-#pragma unroll
-    for (int u = 0; u < FFT_MAXE; ++u) {
+    load_pair_scaled<T>(X, ldx, c0, c1, has1, D, scale, [&](int u) {
         const int e = threadIdx.x + u * NT;
         const int jj = e % J2, j1 = e / J2;
-        v[u] = {T(0), T(0)};
-        if (e < tot && jj < nJ) {
-            const long j = (long)j1 * n2 + j2b + jj;
-            v[u].x = load_scaled<T>(X, j + c0 * ldx, D, j, scale);
-            v[u].y = has1 ? load_scaled<T>(X, j + c1 * ldx, D, j, scale) : T(0);
-        }
-    }
+        return (e < tot && jj < nJ) ? (long)j1 * n2 + j2b + jj : -1L;
+    }, v);
 #pragma unroll
     for (int u = 0; u < FFT_MAXE; ++u) {
         const int e = threadIdx.x + u * NT;
@@ -306,7 +329,8 @@ __global__ __launch_bounds__(NT, 2048 / NT) void k_fft4_pass1(long n, int n1, in
         const int e = threadIdx.x + u * NT;
         const int jj = e % J2, k1 = e / J2;
         // j2 k1 <= (n2 - 1)(n1 - 1) < n: no reduction needed
-        v[u] = (e < tot && jj < nJ) ? twn[(j2b + jj) * k1] : cplx<T>{T(0), T(0)};
+        v[u] = {T(0), T(0)};
+        if (e < tot && jj < nJ) v[u] = twn[(j2b + jj) * k1];
     }
 #pragma unroll
     for (int u = 0; u < FFT_MAXE; ++u) {

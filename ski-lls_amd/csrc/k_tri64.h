@@ -9,6 +9,43 @@ namespace skl {
 constexpr int CQ_T = 256;
 constexpr int CQ_LD = 65;
 
+// Fills 64 x 64 LDS tiles (ld CQ_LD) from fa(row, col) and fb(row, col),
+// then zeroes the entries where ka(row, col) / kb(row, col) is false.  Each
+// thread's loads of both tiles are issued before the first is waited on, so
+// the fill costs one memory latency rather than one per loop trip (a loop of
+// load-then-store pays the latency sixteen times).  The loaders must read a
+// valid address for every (row, col) -- clamp the indices -- and the masking
+// is a separate LDS pass after a barrier: a select on the loaded value lets
+// the compiler put each load behind a branch of its own and wait on it there.
+// All CQ_T threads call it.
+template <class FA, class KA, class FB, class KB>
+__device__ inline void tile64_fill2(double *da, FA fa, KA ka, double *db, FB fb, KB kb) {
+    constexpr int NQ = 64 * 64 / CQ_T;
+    double va[NQ], vb[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int e = threadIdx.x + q * CQ_T;
+        va[q] = fa(e & 63, e >> 6);
+        if (db) vb[q] = fb(e & 63, e >> 6);
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int e = threadIdx.x + q * CQ_T;
+        da[(e & 63) + (e >> 6) * CQ_LD] = va[q];
+        if (db) db[(e & 63) + (e >> 6) * CQ_LD] = vb[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int e = threadIdx.x + q * CQ_T, r = e & 63, c = e >> 6;
+        if (!ka(r, c)) da[r + c * CQ_LD] = 0.0;
+        if (db && !kb(r, c)) db[r + c * CQ_LD] = 0.0;
+    }
+}
+template <class FA, class KA> __device__ inline void tile64_fill(double *da, FA fa, KA ka) {
+    tile64_fill2(da, fa, ka, (double *)nullptr, fa, ka);
+}
+
 // C = A B for 64 x 64 matrices in LDS (ld CQ_LD) on the matrix cores: wave w
 // of the CQ_T threads forms the 32 x 32 quadrant (32 (w & 1), 32 (w >> 1)) as
 // 2 x 2 blocks of v_mfma_f64_16x16x4f64 (A(i, k) from lane i + 16 k, B(k, j)
