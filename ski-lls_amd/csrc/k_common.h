@@ -53,6 +53,25 @@ __device__ inline float rcp_nr(float x) {
 // sqrt(x) for x >= 0 as x / sqrt(x) (0 for 0)
 template <class T> __device__ inline T sqrt_nr(T x) { return x > T(0) ? x * rsqrt_nr(x) : T(0); }
 
+// Division by a run-time divisor d (1 <= d < 2^16) of dividends x < 2^16 as
+// one multiply-high: q = (x m) >> 32 with m = ceil(2^32 / d).  m d = 2^32 + e
+// with 0 <= e < d, so x m / 2^32 = x / d + x e / (d 2^32), and the second term
+// stays below 1 / d (x d < 2^32): the floor is exact.  m comes from a
+// correctly rounded double quotient, which stays below the next integer when
+// d is not a power of two (2^32 / d is then at least 2^-48 of itself away from
+// it).  Build once per divisor; each div/mod then costs a 32 x 64-bit product
+// instead of the ~20-instruction integer division sequence.
+struct FastDiv {
+    unsigned d;
+    unsigned long long m;
+    __device__ explicit FastDiv(unsigned d_)
+        : d(d_), m((unsigned long long)ceil(4294967296.0 / (double)d_)) {}
+    __device__ unsigned div(unsigned x) const {
+        return (unsigned)(((unsigned long long)x * m) >> 32);
+    }
+    __device__ unsigned mod(unsigned x) const { return x - div(x) * d; }
+};
+
 // Compile-time loop: f(std::integral_constant<int, i>{}) for i in [I0, I1).
 template <int I0, int I1, class F> __device__ inline void static_for(F &&f) {
     if constexpr (I0 < I1) {

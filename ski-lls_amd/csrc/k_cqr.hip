@@ -33,6 +33,10 @@
 #include "k_common.h"
 #include "k_tri64.h"
 
+#ifndef SKL_PHASE
+#define SKL_PHASE(kernel, mark)
+#endif
+
 namespace skl {
 
 // Pivoted (PIV) or plain Cholesky of the b x b Gram matrix G (ld ldg) into
@@ -60,6 +64,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
     __shared__ int s_k;               // steps done (the first failed step when short)
     const int tid = threadIdx.x, p = tid & 63, g = tid >> 6;
     if (pass == 2 && *flag != 0) return;
+    SKL_PHASE(4, 0);
     tile64_fill(
         Ss, [&](int i, int j) { return G[min(i, b - 1) + (long)min(j, b - 1) * ldg]; },
         [&](int i, int j) { return i < b && j < b; });
@@ -73,6 +78,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
     const double tol3z = sqrt(1.1102230246251565e-16);
     if (tid == 0) s_k = 0;
     __syncthreads();
+    SKL_PHASE(4, 1);
     if (g == 0 && p < b) {
         dg = Ss[p + p * CQ_LD];
         g0 = dg;
@@ -82,14 +88,22 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
     int k = 0;
     for (int k0 = 0; k0 < b; k0 += 16) {
         if (g == 0) {
-            double rl[16];
+            double rl[16], pre[16];
 #pragma unroll
-            for (int t = 0; t < 16; ++t) rl[t] = 0.0;
+            for (int t = 0; t < 16; ++t) {
+                rl[t] = 0.0;
+                // unpivoted: the block's columns of the Schur complement, all
+                // read before the first step (the pivoted steps read theirs
+                // once the pivot is known)
+                if (!PIV) pre[t] = Ss[p + min(k0 + t, 63) * CQ_LD];
+            }
             bool ok = true;
+            // the steps run without a branch on ok: after a failed step the
+            // values are garbage, but the block ends and the panel is given up
 #pragma unroll
             for (int t = 0; t < 16; ++t) {
                 const int kk = k0 + t;
-                if (ok && kk < b) {
+                if (kk < b) {
                     int pk = kk;
                     if (PIV) {
                         // largest partial norm among the columns left, first
@@ -110,35 +124,36 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
                     if (p == pk) mine = false;
                     // row pk of the current Schur complement: the block's start
                     // minus this block's earlier rows of R
-                    double sc = Ss[p + pk * CQ_LD];
+                    double sc = PIV ? Ss[p + pk * CQ_LD] : pre[t];
 #pragma unroll
                     for (int u = 0; u < t; ++u) sc = fma(-readlane(rl[u], pk), rl[u], sc);
                     const double d = readlane(sc, pk);
                     // not positive, or the column's partial norm below 1e-4 of its norm
-                    if (!(d > 0.0) || !(d >= 1e-8 * g0k)) {
-                        ok = false; // the same in every lane
-                    } else {
-                        const double inv = rsqrt_nr(d), rkk = d * inv;
-                        const double rp = sc * inv; // R(kk, column p) for an active p
-                        rl[t] = p == pk ? rkk : (mine ? rp : 0.0);
-                        if (mine) {
-                            dg -= rp * rp;
-                            if (PIV && vn1 != 0.0) {
-                                // partial norm downdating with the new row of R
-                                double temp = fabs(rp) / vn1;
-                                if (hq_norms)
-                                    temp = (1.0 + temp) * (1.0 - temp);
-                                else
-                                    temp = 1.0 - temp * temp;
-                                if (temp < 0.0) temp = 0.0;
-                                const double q5 = vn1 / vn2;
-                                const double temp2 = temp * q5 * q5;
-                                if (temp2 <= tol3z) {
-                                    vn1 = (mp - kk - 1 > 0) ? sqrt(fmax(dg, 0.0)) : 0.0;
-                                    vn2 = vn1;
-                                } else {
-                                    vn1 = vn1 * sqrt(temp);
-                                }
+                    ok = ok && d > 0.0 && d >= 1e-8 * g0k; // the same in every lane
+                    const double inv = rsqrt_nr(d), rkk = d * inv;
+                    const double rp = sc * inv; // R(kk, column p) for an active p
+                    rl[t] = p == pk ? rkk : (mine ? rp : 0.0);
+                    if (mine) {
+                        dg -= rp * rp;
+                        if (PIV && vn1 != 0.0) {
+                            // partial norm downdating with the new row of R
+                            // (quotients and square roots from rcp_nr and
+                            // rsqrt_nr; both outcomes formed, one kept)
+                            double temp = fabs(rp) * rcp_nr(vn1);
+                            if (hq_norms)
+                                temp = (1.0 + temp) * (1.0 - temp);
+                            else
+                                temp = 1.0 - temp * temp;
+                            if (temp < 0.0) temp = 0.0;
+                            const double q5 = vn1 * rcp_nr(vn2);
+                            const double temp2 = temp * q5 * q5;
+                            const double rec = (mp - kk - 1 > 0) ? sqrt_nr(fmax(dg, 0.0)) : 0.0;
+                            const double dwn = vn1 * sqrt_nr(temp);
+                            if (temp2 <= tol3z) {
+                                vn1 = rec;
+                                vn2 = rec;
+                            } else {
+                                vn1 = dwn;
                             }
                         }
                     }
@@ -154,6 +169,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
                 for (int t = 0; t < 16; ++t) done += (k0 + t < b) ? 1 : 0;
                 s_k = ok ? done : -1;
             }
+            SKL_PHASE(4, 2);
         }
         __syncthreads();
         const int kd = s_k;
@@ -182,6 +198,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
             }
         }
         __syncthreads();
+        SKL_PHASE(4, 3);
     }
     // k: the steps done, b unless a pivot failed (pos is wave 0's)
     const int fail = k < b ? 1 : 0;
@@ -202,12 +219,15 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
     }
     if (PIV && perm && tid < b) perm[tid] = s_perm[tid];
     __syncthreads();
+    SKL_PHASE(4, 4);
     // Z = P R^-1: row perm[t] of Z is row t of R^-1
     tri_inv_upper_lds(Rr, Rb, Ss);
+    SKL_PHASE(4, 5);
     for (int e = tid; e < b * b; e += CQ_T) {
         const int t = e % b, j = e / b;
         Z[s_perm[t] + (long)j * b] = Rb[t + j * CQ_LD];
     }
+    SKL_PHASE(4, 6);
 }
 
 // Out^T Out of a 64-row tile held in LDS (ld CQ_LD, zero past the rows and
@@ -335,6 +355,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long l
     __shared__ double sg[64];
     __shared__ int jp[64];
     if (*flag != 0) return;
+    SKL_PHASE(5, 0);
     const int tid = threadIdx.x, i = tid & 63, g = tid >> 6;
     // the top block of Q = Q1(0:64, :) Z2 (Q1's top block in Ys, Z2 in Lb,
     // the product into Xs)
@@ -351,6 +372,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long l
         Lb[e] = 0.0;
     }
     __syncthreads();
+    SKL_PHASE(5, 1);
     // LU without pivoting of Xs - S, blocked by 16 steps: wave 0 runs a block
     // with no barrier, lane p keeping L(p, k) and U(k, p) of the block's steps;
     // step k reads column k and row k of the block-start matrix (LDS) and
@@ -359,18 +381,23 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long l
     // L(:, block) U(block, :) on the matrix cores
     for (int k0 = 0; k0 < b; k0 += 16) {
         if (g == 0) {
-            double lc[16], ur[16];
+            double lc[16], ur[16], pc[16], pr[16];
 #pragma unroll
             for (int t = 0; t < 16; ++t) {
                 lc[t] = 0.0;
                 ur[t] = 0.0;
+                // the block's columns and rows of the block-start matrix,
+                // all read before the first step
+                const int k = min(k0 + t, 63);
+                pc[t] = Xs[i + k * CQ_LD]; // column k, row i
+                pr[t] = Xs[k + i * CQ_LD]; // row k, column i
             }
 #pragma unroll
             for (int t = 0; t < 16; ++t) {
                 const int k = k0 + t;
                 if (k < b) {
-                    double cv = Xs[i + k * CQ_LD]; // column k, row i
-                    double rv = Xs[k + i * CQ_LD]; // row k, column i
+                    double cv = pc[t];
+                    double rv = pr[t];
 #pragma unroll
                     for (int u = 0; u < t; ++u) {
                         cv = fma(-lc[u], readlane(ur[u], k), cv);
@@ -387,6 +414,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long l
                     if (i == 0) sg[k] = sk;
                 }
             }
+            SKL_PHASE(5, 2);
         }
         __syncthreads();
         if (k0 + 16 >= b) break;
@@ -411,6 +439,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long l
             }
         }
         __syncthreads();
+        SKL_PHASE(5, 3);
     }
     if (tid < b) tau[tid] = -Ub[tid + tid * CQ_LD] * sg[tid];
     // top block: S R2 R1 on and above the diagonal, L1 below (R2 in Xs, R1
@@ -429,6 +458,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long l
     }
     if (swaps && tid < b) swaps[tid] = swp[tid];
     __syncthreads(); // L1 (Lb) and R1 (Ys) read
+    SKL_PHASE(5, 4);
     if (Tout) {
         // T = -U S X with X = L1^-T: L1^T (unit upper) into Ys, its inverse
         // into Xs (Lb as the inversion's scratch)
@@ -449,6 +479,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long l
         });
         __syncthreads();
     }
+    SKL_PHASE(5, 5);
     // U^-1, then Zc = Z2 U^-1 (Z2 into Ys)
     for (int e = tid; e < 64 * 64; e += CQ_T) {
         const int r = e & 63, c = e >> 6;
@@ -456,6 +487,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long l
     }
     __syncthreads();
     tri_inv_upper_lds(Ub, Lb, Ys);
+    SKL_PHASE(5, 6);
     tile64_fill(
         Ys, [&](int r, int c) { return Z2[min(r, b - 1) + (long)min(c, b - 1) * b]; },
         [&](int r, int c) { return r < b && c < b && r <= c; });
@@ -463,6 +495,7 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_hr(int b, const double *Q1, long l
     mm64_lds(Ys, Lb, [&](int r, int c, double v) {
         if (r < b && c < b) Zc[r + (long)c * b] = r <= c ? v : 0.0;
     });
+    SKL_PHASE(5, 7);
 }
 
 // Queues the fast path for the panel Wp (mp x b, ld ldw): pivot != 0 uses the

@@ -32,6 +32,10 @@
 #include "fft_roots.h"
 #include "k_common.h"
 
+#ifndef SKL_PHASE
+#define SKL_PHASE(kernel, mark)
+#endif
+
 namespace skl {
 
 constexpr int FFT_MAXE = 8;    // complex elements per thread held per stage
@@ -68,20 +72,52 @@ __device__ __forceinline__ void dft4(cplx<T> p0, cplx<T> p1, cplx<T> p2, cplx<T>
     o3 = {t1.x - t3.y, t1.y + t3.x}; // t1 + i t3
 }
 
+// radix-5 DFT with W = exp(-2 pi i / 5): y_k = sum_j x_j W^{jk}, from the
+// sums and differences of the mirrored inputs (t1 = x1 + x4, t2 = x2 + x3,
+// d1 = x1 - x4, d2 = x2 - x3): y_0 = x0 + t1 + t2; y_{1,4} = a -/+ i c and
+// y_{2,3} = b -/+ i e with a = x0 + c1 t1 + c2 t2, b = x0 + c2 t1 + c1 t2,
+// c = s1 d1 + s2 d2, e = s2 d1 - s1 d2 (c_k = cos(2 pi k / 5), s_k =
+// sin(2 pi k / 5)): 12 multiplies instead of the 25 complex products of the
+// direct sum.
+template <class T>
+__device__ __forceinline__ void dft5(const cplx<T> (&x)[5], cplx<T> (&y)[5]) {
+    const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+    const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+    const cplx<T> t1 = cadd(x[1], x[4]), t2 = cadd(x[2], x[3]);
+    const cplx<T> d1 = csub(x[1], x[4]), d2 = csub(x[2], x[3]);
+    y[0] = cadd(x[0], cadd(t1, t2));
+    const cplx<T> a = {x[0].x + c1 * t1.x + c2 * t2.x, x[0].y + c1 * t1.y + c2 * t2.y};
+    const cplx<T> b = {x[0].x + c2 * t1.x + c1 * t2.x, x[0].y + c2 * t1.y + c1 * t2.y};
+    const cplx<T> c = {s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y};
+    const cplx<T> e = {s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y};
+    // -i c = (c.y, -c.x)
+    y[1] = {a.x + c.y, a.y - c.x};
+    y[4] = {a.x - c.y, a.y + c.x};
+    y[2] = {b.x + e.y, b.y - e.x};
+    y[3] = {b.x - e.y, b.y + e.x};
+}
+
+// S and stride enter as FastDiv: the butterfly's (sequence, group) indices
+// come from a multiply-high rather than integer division.
 template <class T, int R, int NT>
-__device__ __forceinline__ void fft_stage(cplx<T> *buf, int L, int S, int len, int stride,
-                                          const cplx<T> *__restrict__ tw) {
+__device__ __forceinline__ void fft_stage(cplx<T> *buf, int L, const FastDiv &fS, int len,
+                                          const FastDiv &fst, const cplx<T> *__restrict__ tw) {
     constexpr int NB = (FFT_MAXE + R - 1) / R;
+    const int S = (int)fS.d, stride = (int)fst.d;
     const int m = len / R;
     const int nbfly = (L / R) * S;
     const int tws = L / len;
     cplx<T> v[NB][R];
+    int bs[NB], bq[NB], bp[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         int id = threadIdx.x + b * NT;
+        const int t = (int)fS.div((unsigned)id), tq = (int)fst.div((unsigned)t);
+        bs[b] = id - t * S;
+        bq[b] = t - tq * stride;
+        bp[b] = tq;
         if (id < nbfly) {
-            int s = id % S, t = id / S;
-            int q = t % stride, p = t / stride;
+            const int s = bs[b], q = bq[b], p = bp[b];
 #pragma unroll
             for (int j = 0; j < R; ++j) v[b][j] = buf[(q + stride * (p + j * m)) * S + s];
         }
@@ -91,14 +127,15 @@ __device__ __forceinline__ void fft_stage(cplx<T> *buf, int L, int S, int len, i
     for (int b = 0; b < NB; ++b) {
         int id = threadIdx.x + b * NT;
         if (id < nbfly) {
-            int s = id % S, t = id / S;
-            int q = t % stride, p = t / stride;
+            const int s = bs[b], q = bq[b], p = bp[b];
             cplx<T> y[R];
             if constexpr (R == 2) {
                 y[0] = cadd(v[b][0], v[b][1]);
                 y[1] = csub(v[b][0], v[b][1]);
             } else if constexpr (R == 4) {
                 dft4(v[b][0], v[b][1], v[b][2], v[b][3], y[0], y[1], y[2], y[3]);
+            } else if constexpr (R == 5) {
+                dft5(v[b], y);
             } else if constexpr (R == 8) {
                 // even/odd split: y_k = a_k + W8^k b_k, y_{k+4} = a_k - W8^k b_k
                 cplx<T> a[4], c[4];
@@ -147,14 +184,16 @@ __device__ __forceinline__ void fft_stage(cplx<T> *buf, int L, int S, int len, i
 template <class T, int NT>
 __device__ void lds_fft(cplx<T> *buf, int S, const FftDims &D, const cplx<T> *__restrict__ tw) {
     int len = D.L, stride = 1;
+    const FastDiv fS((unsigned)S);
     for (int st = 0; st < D.nst; ++st) {
         int r = D.radix[st];
+        const FastDiv fst((unsigned)stride);
         switch (r) {
-        case 2: fft_stage<T, 2, NT>(buf, D.L, S, len, stride, tw); break;
-        case 3: fft_stage<T, 3, NT>(buf, D.L, S, len, stride, tw); break;
-        case 4: fft_stage<T, 4, NT>(buf, D.L, S, len, stride, tw); break;
-        case 5: fft_stage<T, 5, NT>(buf, D.L, S, len, stride, tw); break;
-        case 8: fft_stage<T, 8, NT>(buf, D.L, S, len, stride, tw); break;
+        case 2: fft_stage<T, 2, NT>(buf, D.L, fS, len, fst, tw); break;
+        case 3: fft_stage<T, 3, NT>(buf, D.L, fS, len, fst, tw); break;
+        case 4: fft_stage<T, 4, NT>(buf, D.L, fS, len, fst, tw); break;
+        case 5: fft_stage<T, 5, NT>(buf, D.L, fS, len, fst, tw); break;
+        case 8: fft_stage<T, 8, NT>(buf, D.L, fS, len, fst, tw); break;
         default: break;
         }
         len /= r;
@@ -302,13 +341,15 @@ __global__ __launch_bounds__(NT, 2048 / NT) void k_fft4_pass1(long n, int n1, in
     const int j2b = (int)(lb % gridDim.x) * J2;
     const int nJ = min(J2, n2 - j2b);
     const int tot = n1 * J2; // <= NT * FFT_MAXE
+    const FastDiv fJ((unsigned)J2);
+    SKL_PHASE(6, 0);
     // every load of the thread is issued before the first is used: the
     // elements are independent, and a loop that consumed each load before
     // issuing the next would pay the memory latency once per element
     cplx<T> v[FFT_MAXE];
     load_pair_scaled<T>(X, ldx, c0, c1, has1, D, scale, [&](int u) {
         const int e = threadIdx.x + u * NT;
-        const int jj = e % J2, j1 = e / J2;
+        const int j1 = (int)fJ.div((unsigned)e), jj = e - j1 * J2;
         return (e < tot && jj < nJ) ? (long)j1 * n2 + j2b + jj : -1L;
     }, v);
 #pragma unroll
@@ -321,13 +362,15 @@ __global__ __launch_bounds__(NT, 2048 / NT) void k_fft4_pass1(long n, int n1, in
     cplx<T> *twl = buf + tot;
     for (int e = threadIdx.x; e < n1; e += NT) twl[e] = tw1[e];
     __syncthreads();
+    SKL_PHASE(6, 1);
     lds_fft<T, NT>(buf, J2, dims, twl);
+    SKL_PHASE(6, 2);
     cplx<T> *Yp = Y + pair * n;
     // twiddles first (all in flight together), then the products
 #pragma unroll
     for (int u = 0; u < FFT_MAXE; ++u) {
         const int e = threadIdx.x + u * NT;
-        const int jj = e % J2, k1 = e / J2;
+        const int k1 = (int)fJ.div((unsigned)e), jj = e - k1 * J2;
         // j2 k1 <= (n2 - 1)(n1 - 1) < n: no reduction needed
         v[u] = {T(0), T(0)};
         if (e < tot && jj < nJ) v[u] = twn[(j2b + jj) * k1];
@@ -335,9 +378,10 @@ __global__ __launch_bounds__(NT, 2048 / NT) void k_fft4_pass1(long n, int n1, in
 #pragma unroll
     for (int u = 0; u < FFT_MAXE; ++u) {
         const int e = threadIdx.x + u * NT;
-        const int jj = e % J2, k1 = e / J2;
+        const int k1 = (int)fJ.div((unsigned)e), jj = e - k1 * J2;
         if (e < tot && jj < nJ) Yp[(long)k1 * n2 + j2b + jj] = cmul(buf[k1 * J2 + jj], v[u]);
     }
+    SKL_PHASE(6, 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -366,17 +410,19 @@ __global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K
     const int S = 2 * K1;                    // sequence slots: [0,K1) primary, [K1,2K1) mirror
     const cplx<T> *Yp = Y + pair * n;
     const int tot = n2 * S; // <= NT * FFT_MAXE
+    const FastDiv fS((unsigned)S);
+    SKL_PHASE(7, 0);
     {
         cplx<T> v[FFT_MAXE]; // all loads in flight before the first use
 #pragma unroll
         for (int u = 0; u < FFT_MAXE; ++u) {
             const int e = threadIdx.x + u * NT;
-            const int sl = e % S, j2 = e / S;
-            const int kk = sl % K1;
+            const int j2 = (int)fS.div((unsigned)e), sl = e - j2 * S;
+            const int kk = sl < K1 ? sl : sl - K1;
             v[u] = {T(0), T(0)};
             if (e < tot && kk < nK) {
                 const int k1 = kb + kk;
-                const int row = (sl < K1) ? k1 : (n1 - k1) % n1;
+                const int row = (sl < K1 || k1 == 0) ? k1 : n1 - k1;
                 v[u] = Yp[(long)row * n2 + j2];
             }
         }
@@ -389,15 +435,17 @@ __global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K
     cplx<T> *twl = buf + tot; // the stage twiddles in LDS after the data
     for (int e = threadIdx.x; e < n2; e += NT) twl[e] = tw2[e];
     __syncthreads();
+    SKL_PHASE(7, 1);
     lds_fft<T, NT>(buf, S, dims, twl);
+    SKL_PHASE(7, 2);
     // outputs: for primary row k1 and column k2: k = k1 + n1 k2,
     //   partner n-k = (n1-k1) + n1 (n2-1-k2) (mirror slot) for k1 > 0,
     //               = n1 ((n2-k2) % n2)      (same slot)   for k1 = 0.
     // mirror row k1' = n1 - k1 (k1 > 0, k1' != k1): k' = k1' + n1 k2,
     //   partner = k1 + n1 (n2-1-k2) (primary slot).
     for (int e = threadIdx.x; e < tot; e += NT) {
-        int sl = e % S, k2 = e / S;
-        int kk = sl % K1;
+        const int k2 = (int)fS.div((unsigned)e), sl = e - k2 * S;
+        const int kk = sl < K1 ? sl : sl - K1;
         if (kk >= nK) continue;
         int k1 = kb + kk;
         bool mirror = sl >= K1;
@@ -406,7 +454,7 @@ __global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K
         cplx<T> a = buf[k2 * S + sl];
         cplx<T> c;
         if (k1 == 0) {
-            c = buf[((n2 - k2) % n2) * S + sl];
+            c = buf[(k2 == 0 ? 0 : n2 - k2) * S + sl];
         } else if (2 * k1 == n1) {
             c = buf[(n2 - 1 - k2) * S + sl];
         } else {
@@ -423,6 +471,7 @@ __global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K
             if (has1) H[k + c1 * ldh] = h2;
         }
     }
+    SKL_PHASE(7, 3);
 }
 
 // ---------------------------------------------------------------------------
