@@ -56,6 +56,7 @@ template <class T> struct FftPlanDev {
     cplx<T> *tw2 = nullptr; // W_{n2}^e
     cplx<T> *twn = nullptr; // W_n^e (four-step twiddles)
     int J2 = 0, K1 = 0;     // sequences per workgroup in the two passes
+    int G = 1;              // column pairs per pass-2 workgroup (and per H group)
 };
 
 // ---------------------------------------------------------------------------
@@ -385,32 +386,34 @@ __global__ __launch_bounds__(NT, 2048 / NT) void k_fft4_pass1(long n, int n1, in
 }
 
 // ---------------------------------------------------------------------------
-// Four-step, pass 2: workgroup (pair, block of K1 primary rows k1 in
-// [0, n1/2] plus their mirrors n1 - k1): X[k1 + n1 k2] = FFT_{n2}(Y[k1, :])[k2],
-// then H at k and n - k from the primary/mirror rows, written to H (n x ncols).
+// Four-step, pass 2: workgroup (group of G pairs, block of K1 primary rows k1
+// in [0, n1/2] plus their mirrors n1 - k1, of each pair):
+// X[k1 + n1 k2] = FFT_{n2}(Y[k1, :])[k2], then H at k and n - k from the
+// primary/mirror rows.
 // ---------------------------------------------------------------------------
-// pairs_out: H holds the pair's two transforms interleaved (H[2 (pair n + k)
-// + {0, 1}]), for the bucket gather of both columns in one read per member;
+// pairs_out: H holds the group's transforms interleaved by pair, each pair's
+// two columns side by side (H[2 ((grp n + k) G + g) + {0, 1}] for pair
+// G grp + g), for the bucket gather of 2 G columns in one line per member;
 // otherwise column c at H + c ldh.
 template <class T, int NT>
-__global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K1, long ncols,
-                                                   FftDims dims, const cplx<T> *__restrict__ tw2,
+__global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K1, int G,
+                                                   long ncols, FftDims dims,
+                                                   const cplx<T> *__restrict__ tw2,
                                                    const cplx<T> *__restrict__ Y,
                                                    T *__restrict__ H, long ldh, int pairs_out) {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const long lb = xcd_local_block(blockIdx.x + (long)gridDim.x * blockIdx.y,
                                     (long)gridDim.x * gridDim.y);
-    const long pair = lb / gridDim.x;
-    const long c0 = 2 * pair, c1 = c0 + 1;
-    const bool has1 = c1 < ncols;
+    const long grp = lb / gridDim.x;
+    const long npairs = (ncols + 1) / 2;
     const int half = n1 / 2;                 // primary rows 0..half
     const int kb = (int)(lb % gridDim.x) * K1;
     const int nK = min(K1, half + 1 - kb);   // primary rows in this block
-    const int S = 2 * K1;                    // sequence slots: [0,K1) primary, [K1,2K1) mirror
-    const cplx<T> *Yp = Y + pair * n;
+    const int S2 = 2 * K1;                   // a pair's slots: [0,K1) primary, [K1,2K1) mirror
+    const int S = S2 * G;                    // sequence slots: pair g at [g S2, (g + 1) S2)
     const int tot = n2 * S; // <= NT * FFT_MAXE
-    const FastDiv fS((unsigned)S);
+    const FastDiv fS((unsigned)S), fS2((unsigned)S2);
     SKL_PHASE(7, 0);
     {
         cplx<T> v[FFT_MAXE]; // all loads in flight before the first use
@@ -418,12 +421,14 @@ __global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K
         for (int u = 0; u < FFT_MAXE; ++u) {
             const int e = threadIdx.x + u * NT;
             const int j2 = (int)fS.div((unsigned)e), sl = e - j2 * S;
-            const int kk = sl < K1 ? sl : sl - K1;
+            const int g = (int)fS2.div((unsigned)sl), w = sl - g * S2;
+            const int kk = w < K1 ? w : w - K1;
+            const long pair = grp * G + g;
             v[u] = {T(0), T(0)};
-            if (e < tot && kk < nK) {
+            if (e < tot && kk < nK && pair < npairs) {
                 const int k1 = kb + kk;
-                const int row = (sl < K1 || k1 == 0) ? k1 : n1 - k1;
-                v[u] = Yp[(long)row * n2 + j2];
+                const int row = (w < K1 || k1 == 0) ? k1 : n1 - k1;
+                v[u] = Y[pair * n + (long)row * n2 + j2];
             }
         }
 #pragma unroll
@@ -445,10 +450,12 @@ __global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K
     //   partner = k1 + n1 (n2-1-k2) (primary slot).
     for (int e = threadIdx.x; e < tot; e += NT) {
         const int k2 = (int)fS.div((unsigned)e), sl = e - k2 * S;
-        const int kk = sl < K1 ? sl : sl - K1;
-        if (kk >= nK) continue;
+        const int g = (int)fS2.div((unsigned)sl), w = sl - g * S2;
+        const int kk = w < K1 ? w : w - K1;
+        const long pair = grp * G + g;
+        if (kk >= nK || pair >= npairs) continue;
         int k1 = kb + kk;
-        bool mirror = sl >= K1;
+        bool mirror = w >= K1;
         if (mirror && (k1 == 0 || 2 * k1 == n1)) continue; // self-mirrored rows done once
         int row = mirror ? n1 - k1 : k1;
         cplx<T> a = buf[k2 * S + sl];
@@ -458,14 +465,16 @@ __global__ __launch_bounds__(NT) void k_fft4_pass2(long n, int n1, int n2, int K
         } else if (2 * k1 == n1) {
             c = buf[(n2 - 1 - k2) * S + sl];
         } else {
-            int osl = mirror ? kk : kk + K1;
+            int osl = g * S2 + (mirror ? kk : kk + K1);
             c = buf[(n2 - 1 - k2) * S + osl];
         }
         long k = row + (long)n1 * k2;
         T h1 = T(0.5) * ((a.x + c.x) - (a.y - c.y));
         T h2 = T(0.5) * ((a.y + c.y) + (a.x - c.x));
+        const long c0 = 2 * pair, c1 = c0 + 1;
+        const bool has1 = c1 < ncols;
         if (pairs_out) {
-            reinterpret_cast<cplx<T> *>(H)[pair * n + k] = {h1, has1 ? h2 : T(0)};
+            reinterpret_cast<cplx<T> *>(H)[(grp * n + k) * G + g] = {h1, has1 ? h2 : T(0)};
         } else {
             H[k + c0 * ldh] = h1;
             if (has1) H[k + c1 * ldh] = h2;
@@ -509,47 +518,66 @@ __global__ __launch_bounds__(256) void k_hash_gather(long m, long ncols, long nb
     Y[h + c * ldy] = s;
 }
 
-// The same for column pairs stored interleaved (X2[pair n + i] = (column
-// 2 pair, column 2 pair + 1) of row i): one 2-element read per member serves
-// both columns; each column's sum keeps the source order.
-template <class T>
-__global__ __launch_bounds__(256) void k_hash_gather_pairs(long m, long n, long ncols, long nbh,
-                                                           const int32_t *__restrict__ bptr,
-                                                           const int32_t *__restrict__ bsrc,
-                                                           const T *__restrict__ bcoef,
-                                                           const cplx<T> *__restrict__ X2,
-                                                           T *__restrict__ Y, long ldy) {
+// The same for the grouped pair layout of pass 2 (X2[(grp n + i) G + g] =
+// (column 2 p, column 2 p + 1) of row i for pair p = G grp + g): one bucket
+// member's row of a group is 16 G contiguous bytes, one line read for 2 G
+// columns (a lone 16-byte read per member and pair moved a whole line from
+// L2 for 16 bytes, and the gather ran at the rate of those lines).  NTH
+// buckets per workgroup; the workgroups of a group are consecutive on one
+// XCD.  A bucket's members go in batches -- the indices and coefficients
+// first (the last member repeated past the end), then the rows -- and each
+// column's sum keeps the source order.
+constexpr int GATHER_NTH = 64, GATHER_GMAX = 10;
+template <class T, int G>
+__global__ __launch_bounds__(GATHER_NTH) void k_hash_gather_groups(
+    long m, long n, long ncols, long nbh, const int32_t *__restrict__ bptr,
+    const int32_t *__restrict__ bsrc, const T *__restrict__ bcoef,
+    const cplx<T> *__restrict__ X2, T *__restrict__ Y, long ldy) {
 #pragma clang fp contract(off)
-    const long npairs = (ncols + 1) / 2;
+    const long ngrp = ((ncols + 1) / 2 + G - 1) / G;
     const long lb = xcd_local_block(blockIdx.x, gridDim.x);
-    const long pr = lb / nbh;
-    const long h = (lb % nbh) * 256 + threadIdx.x;
-    if (h >= m || pr >= npairs) return;
+    const long gr = lb / nbh;
+    const long h = (lb % nbh) * GATHER_NTH + threadIdx.x;
+    if (h >= m || gr >= ngrp) return;
     const int q0 = bptr[h], q1 = bptr[h + 1];
-    const cplx<T> *x = X2 + pr * n;
-    T s0 = 0, s1 = 0;
-    int q = q0;
-    for (; q + 4 <= q1; q += 4) { // four reads in flight, summed in source order
-        const int i0 = bsrc[q], i1 = bsrc[q + 1], i2 = bsrc[q + 2], i3 = bsrc[q + 3];
-        const T w0 = bcoef[q], w1 = bcoef[q + 1], w2 = bcoef[q + 2], w3 = bcoef[q + 3];
-        const cplx<T> x0 = x[i0], x1 = x[i1], x2 = x[i2], x3 = x[i3];
-        s0 = s0 + w0 * x0.x;
-        s1 = s1 + w0 * x0.y;
-        s0 = s0 + w1 * x1.x;
-        s1 = s1 + w1 * x1.y;
-        s0 = s0 + w2 * x2.x;
-        s1 = s1 + w2 * x2.y;
-        s0 = s0 + w3 * x3.x;
-        s1 = s1 + w3 * x3.y;
+    const cplx<T> *x = X2 + gr * n * G;
+    T s0[G], s1[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        s0[g] = 0;
+        s1[g] = 0;
     }
-    for (; q < q1; ++q) {
-        const T w = bcoef[q];
-        const cplx<T> xv = x[bsrc[q]];
-        s0 = s0 + w * xv.x;
-        s1 = s1 + w * xv.y;
+    constexpr int MB = G <= 5 ? 4 : 2; // members per batch (registers: MB G rows)
+    for (int q = q0; q < q1; q += MB) {
+        int ix[MB];
+        T w[MB];
+#pragma unroll
+        for (int k = 0; k < MB; ++k) {
+            const int qk = min(q + k, q1 - 1);
+            ix[k] = bsrc[qk];
+            w[k] = bcoef[qk];
+        }
+        cplx<T> xv[MB][G];
+#pragma unroll
+        for (int k = 0; k < MB; ++k)
+#pragma unroll
+            for (int g = 0; g < G; ++g) xv[k][g] = x[(long)ix[k] * G + g];
+#pragma unroll
+        for (int k = 0; k < MB; ++k)
+            if (q + k < q1) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    s0[g] = s0[g] + w[k] * xv[k][g].x;
+                    s1[g] = s1[g] + w[k] * xv[k][g].y;
+                }
+            }
     }
-    Y[h + 2 * pr * ldy] = s0;
-    if (2 * pr + 1 < ncols) Y[h + (2 * pr + 1) * ldy] = s1;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const long c = 2 * (gr * G + g);
+        if (c < ncols) Y[h + c * ldy] = s0[g];
+        if (c + 1 < ncols) Y[h + (c + 1) * ldy] = s1[g];
+    }
 }
 
 template <class T>
@@ -980,7 +1008,13 @@ template <class T> static const FftPlanDev<T> &get_plan(long n, Arena &ar) {
         p.tw2 = tables(p.n2);
         p.twn = tables(n);
         p.J2 = (int)std::min<long>(32, cap / p.n1);
-        p.K1 = (int)std::min<long>(16, cap / (2L * p.n2));
+        // pass 2 takes G pairs at once (K1 rows of each) and writes H in
+        // groups of G pairs: a row of a group is 16 G contiguous bytes, so
+        // the bucket gather reads whole lines (n2 <= cap / 8, so G >= 4;
+        // the workgroup's 2 K1 G sequences fill the LDS tile best with
+        // K1 = 1 and G as large as the gather's registers allow)
+        p.G = (int)std::min<long>(GATHER_GMAX, cap / (2L * p.n2));
+        p.K1 = (int)std::max<long>(1, std::min<long>(16, cap / (2L * p.n2 * p.G)));
     }
     if (scratch) {
         thread_local FftPlanDev<T> tmp;
@@ -1015,11 +1049,12 @@ static void dht_fourstep(const FftPlanDev<T> &P, long ncols, const double *X, lo
     }
     {
         int half = P.n1 / 2;
-        dim3 g((unsigned)((half + 1 + P.K1 - 1) / P.K1), (unsigned)npairs);
-        size_t sh = sizeof(cplx<T>) * ((size_t)P.n2 * 2 * P.K1 + P.n2);
+        const long ngrp = (npairs + P.G - 1) / P.G;
+        dim3 g((unsigned)((half + 1 + P.K1 - 1) / P.K1), (unsigned)ngrp);
+        size_t sh = sizeof(cplx<T>) * ((size_t)P.n2 * 2 * P.K1 * P.G + P.n2);
         lds_allow((const void *)k_fft4_pass2<T, NT>, sh);
-        k_fft4_pass2<T, NT><<<g, NT, sh, s>>>(P.n, P.n1, P.n2, P.K1, ncols, P.d2, P.tw2, Y, H, ldh,
-                                              pairs_out);
+        k_fft4_pass2<T, NT><<<g, NT, sh, s>>>(P.n, P.n1, P.n2, P.K1, P.G, ncols, P.d2, P.tw2, Y, H,
+                                              ldh, pairs_out);
         SKL_CHECK_LAUNCH();
     }
 }
@@ -1039,29 +1074,48 @@ void dht_sketch(long n, long ncols, const double *X, long ldx, const double *D, 
                                                              ldsa);
         SKL_CHECK_LAUNCH();
     } else {
-        // the transformed columns pass through H (n x cols) and the four-step
-        // pass's complex pairs; in no-allocation mode the columns go in
-        // chunks of an even count (the pairs stay as in one pass, so the
-        // result is the same) sized to a quarter of the scratch left
+        // the transformed columns pass through H (n x cols, in groups of G
+        // pairs) and the four-step pass's complex pairs; in no-allocation
+        // mode the columns go in chunks of an even count, a multiple of 2 G
+        // where that fits (the pairs stay as in one pass, so the result is
+        // the same), sized to a quarter of the scratch left
+        const long cg = 2L * P.G; // columns per group (mode 2)
         long chunk = ncols;
         if (ctx().no_alloc) {
             const size_t per_col = 2 * sizeof(T) * (size_t)n + 64;
             const size_t room = (ar.capacity() - std::min(ar.used(), ar.capacity())) / 4;
-            chunk = std::max<long>(2, (long)(room / per_col) & ~1L);
+            const long fit = (long)(room / per_col);
+            chunk = fit >= cg ? fit / cg * cg : std::max<long>(2, fit & ~1L);
         }
         for (long c0 = 0; c0 < ncols; c0 += chunk) {
             const long nc = std::min(chunk, ncols - c0);
             const size_t mark = ar.used();
-            T *H = ar.alloc_n<T>((size_t)n * (nc + 1));
+            const long ngrp = ((nc + 1) / 2 + P.G - 1) / P.G;
+            T *H = ar.alloc_n<T>(std::max((size_t)n * (nc + 1), (size_t)n * ngrp * cg));
             if (P.mode == 2) {
-                // pairs interleaved, gathered two columns per read
+                // groups of G pairs interleaved, gathered 2 G columns per line
                 dht_fourstep<T>(P, nc, X + c0 * ldx, ldx, D, scale, H, n, ar, 1);
-                const long nbh = (m + 255) / 256, npairs = (nc + 1) / 2;
-                SKL_REQUIRE(nbh * npairs <= 2147483647L, SKL_E_UNSUPPORTED,
+                const long nbh = (m + GATHER_NTH - 1) / GATHER_NTH;
+                SKL_REQUIRE(nbh * ngrp <= 2147483647L, SKL_E_UNSUPPORTED,
                             "bucket gather: grid too large");
-                k_hash_gather_pairs<T><<<(unsigned)(nbh * npairs), 256, 0, s>>>(
-                    m, n, nc, nbh, bptr, bsrc, bcoef, reinterpret_cast<const cplx<T> *>(H),
-                    SA + c0 * ldsa, ldsa);
+                const unsigned gg = (unsigned)(nbh * ngrp);
+                const cplx<T> *H2 = reinterpret_cast<const cplx<T> *>(H);
+                auto gather = [&](auto Gc) {
+                    constexpr int G = decltype(Gc)::value;
+                    k_hash_gather_groups<T, G><<<gg, GATHER_NTH, 0, s>>>(
+                        m, n, nc, nbh, bptr, bsrc, bcoef, H2, SA + c0 * ldsa, ldsa);
+                };
+                static_assert(GATHER_GMAX == 10, "the cases below");
+                switch (P.G) {
+                case 4: gather(std::integral_constant<int, 4>{}); break;
+                case 5: gather(std::integral_constant<int, 5>{}); break;
+                case 6: gather(std::integral_constant<int, 6>{}); break;
+                case 7: gather(std::integral_constant<int, 7>{}); break;
+                case 8: gather(std::integral_constant<int, 8>{}); break;
+                case 9: gather(std::integral_constant<int, 9>{}); break;
+                case 10: gather(std::integral_constant<int, 10>{}); break;
+                default: SKL_REQUIRE(false, SKL_E_UNSUPPORTED, "bucket gather: group size");
+                }
                 SKL_CHECK_LAUNCH();
             } else {
                 dht_bluestein<T>(n, nc, X + c0 * ldx, ldx, D, scale, H, n, ar);

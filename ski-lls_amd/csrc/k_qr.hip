@@ -1386,22 +1386,26 @@ __device__ inline void yq_load(T (&y)[YQ_ROWS], T &dn, T &en, const T *Y, long l
 }
 
 // The reflector (dlarfg) of the wave's column in lane wl over its live rows
-// jj.. (pivot row y[S]): the lane stores its raw rows S.. in pairs to `rows`
-// (LDS, 16-byte aligned), and the wave reads them back spread over its lanes
-// (a wave's LDS operations complete in order) for alpha = y[S] and the sum of
-// squares of the rows below it, summed by DPP lane moves; then tau and the
-// scale sc of the rows below the pivot (v[S] = 1, v[r] = sc y[r]; a reader
-// forms v with yq_v).  Wave-uniform results.
+// jj.. (pivot row y[S]): yq_store_rows has the lane store its raw rows S.. in
+// pairs to `rows` (LDS, 16-byte aligned); yq_reflector has a wave read them
+// back spread over its lanes (after the stores: the same wave, whose LDS
+// operations complete in order, or any wave after a barrier) for alpha = y[S]
+// and the sum of squares of the rows below it, summed by DPP lane moves; then
+// tau and the scale sc of the rows below the pivot (v[S] = 1, v[r] = sc y[r];
+// a reader forms v with yq_v).  Wave-uniform results, the same bits in every
+// wave that reads the same rows.
 template <class T> struct alignas(2 * sizeof(T)) yq_pair { T a, b; };
 template <int S, class T>
-__device__ inline void yq_publish(const T (&y)[YQ_ROWS], int wl, int lane, T *rows, int live,
-                                  T &tau, T &sc) {
+__device__ inline void yq_store_rows(const T (&y)[YQ_ROWS], int wl, int lane, T *rows, int live) {
     if (lane == wl) {
         yq_pair<T> *d2 = reinterpret_cast<yq_pair<T> *>(rows);
 #pragma unroll
         for (int r0 = S & ~1; r0 < YQ_ROWS; r0 += 2)
             if (YQ_CH * (r0 / YQ_CH) < live) d2[r0 / 2] = {y[r0], r0 + 1 < YQ_ROWS ? y[r0 + 1] : T(0)};
     }
+}
+template <int S, class T>
+__device__ inline void yq_reflector(const T *rows, int lane, int live, T &tau, T &sc) {
     const T x0 = (lane > S && lane < live) ? rows[lane] : T(0);
     const T x1 = (lane + 64 < live) ? rows[64 + lane] : T(0);
     const T xnorm2 = wave_sum_bcast(fma(x1, x1, x0 * x0));
@@ -1413,6 +1417,12 @@ __device__ inline void yq_publish(const T (&y)[YQ_ROWS], int wl, int lane, T *ro
         tau = (beta - alpha) * rcp_nr(beta);
         sc = rcp_nr(alpha - beta);
     }
+}
+template <int S, class T>
+__device__ inline void yq_publish(const T (&y)[YQ_ROWS], int wl, int lane, T *rows, int live,
+                                  T &tau, T &sc) {
+    yq_store_rows<S>(y, wl, lane, rows, live);
+    yq_reflector<S>(rows, lane, live, tau, sc);
 }
 
 // the reflector for yq_apply from the published raw rows: lane l of vb[c]
@@ -1598,9 +1608,9 @@ template <class T> struct YtArgs {
 template <class T>
 __global__ __launch_bounds__(64 * YT_W) void k_yqrp_top(YtArgs<T> a) {
     constexpr int W = YT_W, NT = 64 * W, VR = 80;
-    __shared__ __align__(16) T s_v[2][W][VR]; // candidates' raw rows
-    __shared__ T s_bv[2][W], s_tau[2][W], s_sc[2][W];
-    __shared__ int s_bp[2][W];
+    __shared__ __align__(16) T s_v[2][VR]; // the winner's raw rows
+    __shared__ T s_bv[2][W], s_tau[2], s_sc[2];
+    __shared__ int s_bp[2][W], s_bl[2][W];
     __shared__ int s_hist[256];
     __shared__ int s_cols[NT];
     __shared__ int s_cnt, s_cut;
@@ -1701,23 +1711,20 @@ __global__ __launch_bounds__(64 * YT_W) void k_yqrp_top(YtArgs<T> a) {
         constexpr int S = decltype(Sc)::value;
         const int buf = jj & 1;
         SKL_PHASE(3, 2);
-        // every wave publishes its best column's raw rows and reflector scalars
+        // every wave offers its best column (value, position, lane); the
+        // winner over the waves alone stores its rows and forms its
+        // reflector, which the others read after a second barrier (eight
+        // waves storing and reducing their candidates at once, one barrier a
+        // step, measured slower: their LDS stores queue in the CU's one LDS
+        // pipeline)
         {
             const T av = fabs(dn);
             const T cand = (pos >= jj && pos != NONE && av == av) ? av : T(-1);
-            const int wl = wave_argmax(cand, pos);
-            if (wl >= 0) {
-                T tau, sc;
-                yq_publish<S>(y, wl, lane, &s_v[buf][wave][0], live, tau, sc);
-                if (lane == 0) {
-                    s_bv[buf][wave] = readlane(cand, wl);
-                    s_bp[buf][wave] = __builtin_amdgcn_readlane(pos, wl);
-                    s_tau[buf][wave] = tau;
-                    s_sc[buf][wave] = sc;
-                }
-            } else if (lane == 0) {
-                s_bv[buf][wave] = T(-1);
-                s_bp[buf][wave] = NONE;
+            const int l = wave_argmax(cand, pos);
+            if (lane == 0) {
+                s_bv[buf][wave] = l >= 0 ? readlane(cand, l) : T(-1);
+                s_bp[buf][wave] = l >= 0 ? __builtin_amdgcn_readlane(pos, l) : NONE;
+                s_bl[buf][wave] = l;
             }
         }
         SKL_PHASE(3, 3);
@@ -1734,9 +1741,18 @@ __global__ __launch_bounds__(64 * YT_W) void k_yqrp_top(YtArgs<T> a) {
             }
         // a left-out column might have won: leave the block to k_yqrp_lanes
         if (gp == NONE || !(gv > lim)) return false;
-        const T tau = s_tau[buf][u];
+        if (wave == u) {
+            T tau, sc;
+            yq_publish<S>(y, s_bl[buf][u], lane, &s_v[buf][0], live, tau, sc);
+            if (lane == 0) {
+                s_tau[buf] = tau;
+                s_sc[buf] = sc;
+            }
+        }
+        __syncthreads();
+        const T tau = s_tau[buf];
         T vb[5];
-        yq_v<S>(&s_v[buf][u][0], s_sc[buf][u], lane, vb);
+        yq_v<S>(&s_v[buf][0], s_sc[buf], lane, vb);
         if (tid == 0) {
             const int t = s_colat[jj];
             s_colat[jj] = s_colat[gp];
