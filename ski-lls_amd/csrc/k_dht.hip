@@ -321,6 +321,8 @@ __global__ __launch_bounds__(NT) void k_dht_single(long n, long ncols, const dou
     }
 }
 
+constexpr int TW4_LO = 256; // low part of the split four-step twiddle exponent
+
 // ---------------------------------------------------------------------------
 // Four-step, pass 1: workgroup (pair, block of J2 consecutive j2):
 // Y[k1*n2 + j2] = W_n^{j2 k1} * FFT_{n1}( x[j1*n2 + j2] )[k1]
@@ -359,28 +361,31 @@ __global__ __launch_bounds__(NT, 2048 / NT) void k_fft4_pass1(long n, int n1, in
         if (e < tot) buf[e] = v[u]; // e = j1 J2 + jj
     }
     // the stage twiddles in LDS after the data: an LDS read per butterfly
-    // output instead of a cache round trip per stage
+    // output instead of a cache round trip per stage; then the two halves of
+    // the four-step twiddles W_n^a = W_n^{256 (a >> 8)} W_n^{a & 255}: the
+    // elements' exponents are scattered over all of W_n, and a read of the
+    // full table per element moved a cache line from L2 for 16 bytes
     cplx<T> *twl = buf + tot;
+    cplx<T> *twlo = twl + n1, *twhi = twlo + TW4_LO;
+    const int nhi = (int)((n + TW4_LO - 1) / TW4_LO);
     for (int e = threadIdx.x; e < n1; e += NT) twl[e] = tw1[e];
+    for (int e = threadIdx.x; e < TW4_LO; e += NT) twlo[e] = twn[e];
+    for (int e = threadIdx.x; e < nhi; e += NT) twhi[e] = twn[(long)TW4_LO * e];
     __syncthreads();
     SKL_PHASE(6, 1);
     lds_fft<T, NT>(buf, J2, dims, twl);
     SKL_PHASE(6, 2);
     cplx<T> *Yp = Y + pair * n;
-    // twiddles first (all in flight together), then the products
 #pragma unroll
     for (int u = 0; u < FFT_MAXE; ++u) {
         const int e = threadIdx.x + u * NT;
         const int k1 = (int)fJ.div((unsigned)e), jj = e - k1 * J2;
-        // j2 k1 <= (n2 - 1)(n1 - 1) < n: no reduction needed
-        v[u] = {T(0), T(0)};
-        if (e < tot && jj < nJ) v[u] = twn[(j2b + jj) * k1];
-    }
-#pragma unroll
-    for (int u = 0; u < FFT_MAXE; ++u) {
-        const int e = threadIdx.x + u * NT;
-        const int k1 = (int)fJ.div((unsigned)e), jj = e - k1 * J2;
-        if (e < tot && jj < nJ) Yp[(long)k1 * n2 + j2b + jj] = cmul(buf[k1 * J2 + jj], v[u]);
+        if (e < tot && jj < nJ) {
+            // j2 k1 <= (n2 - 1)(n1 - 1) < n: no reduction needed
+            const int a = (j2b + jj) * k1;
+            const cplx<T> w = cmul(twhi[a / TW4_LO], twlo[a % TW4_LO]);
+            Yp[(long)k1 * n2 + j2b + jj] = cmul(buf[k1 * J2 + jj], w);
+        }
     }
     SKL_PHASE(6, 3);
 }
@@ -1041,7 +1046,8 @@ static void dht_fourstep(const FftPlanDev<T> &P, long ncols, const double *X, lo
     hipStream_t s = ctx().stream;
     {
         dim3 g((unsigned)((P.n2 + P.J2 - 1) / P.J2), (unsigned)npairs);
-        size_t sh = sizeof(cplx<T>) * ((size_t)P.n1 * P.J2 + P.n1);
+        const size_t nhi = (size_t)((P.n + TW4_LO - 1) / TW4_LO);
+        size_t sh = sizeof(cplx<T>) * ((size_t)P.n1 * P.J2 + P.n1 + TW4_LO + nhi);
         lds_allow((const void *)k_fft4_pass1<T, NT>, sh);
         k_fft4_pass1<T, NT><<<g, NT, sh, s>>>(P.n, P.n1, P.n2, P.J2, ncols, X, ldx, D, scale, P.d1,
                                               P.tw1, P.twn, Y);
