@@ -301,7 +301,23 @@ void Context::make_side() {
     if (side) return;
     if (no_alloc) throw Error(SKL_E_UNSUPPORTED, "second stream in no-allocation mode");
     AllocLock lk(capture_mutex());
-    SKL_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    // the second stream runs bulk GEMMs beside the critical path's
+    // single-workgroup kernels (panel factorizations, the pivot search),
+    // which otherwise share CUs with the GEMM's workgroups: the side stream
+    // is kept off the first num_cu / 8 CUs of the mask (an XCD's worth; the
+    // first 32 bits are one XCD on MI355X), SKL_SIDE_CU_RESERVE = R sets R
+    // (0: no mask).  Measured: the sparse 120000 x 5000 FP64 solve's QR
+    // 28.9 -> 26.5 ms; the dense 40000 x 4000 solve unchanged (its side GEMM
+    // overlaps the pivot search by a third)
+    const char *rs = getenv("SKL_SIDE_CU_RESERVE");
+    const int reserve = rs ? std::max(0, atoi(rs)) : (num_cu >= 64 ? num_cu / 8 : 0);
+    if (reserve > 0 && reserve < num_cu) {
+        std::vector<uint32_t> mask((size_t)(num_cu + 31) / 32, 0u);
+        for (int c = reserve; c < num_cu; ++c) mask[(size_t)c / 32] |= 1u << (c % 32);
+        SKL_HIP(hipExtStreamCreateWithCUMask(&side, (uint32_t)mask.size(), mask.data()));
+    } else {
+        SKL_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    }
     for (auto &e : ev) SKL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
