@@ -1867,15 +1867,6 @@ __global__ __launch_bounds__(256) void k_apply_swaps(int b, const int32_t *swaps
     __shared__ int src[64];
     __shared__ T tile[64 * 65];
     if (grid_failed(err)) return;
-    if (threadIdx.x == 0) {
-        for (int s = 0; s < b; ++s) src[s] = s;
-        for (int s = 0; s < b; ++s) {
-            int p = swaps[s];
-            int t = src[s];
-            src[s] = src[p];
-            src[p] = t;
-        }
-    }
     const long nb1 = (r1 + 63) / 64;
     long blk = blockIdx.x;
     T *X = X1;
@@ -1888,10 +1879,34 @@ __global__ __launch_bounds__(256) void k_apply_swaps(int b, const int32_t *swaps
     }
     const long i0 = blk * 64;
     const int ni = (int)min(64L, rows - i0);
-    __syncthreads();
-    for (int e = threadIdx.x; e < 64 * b; e += 256) {
-        int i = e & 63, t = e >> 6;
-        if (i < ni) tile[i + t * 65] = X[(i0 + i) + (long)t * ld];
+    // the block's rows of the b columns, every load in flight before the
+    // first is used (clamped indices; rows past ni are never written back)
+    constexpr int NQ = 64 * 64 / 256;
+    T v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int e = threadIdx.x + q * 256, i = e & 63, t = e >> 6;
+        v[q] = X[(i0 + min(i, ni - 1)) + (long)min(t, b - 1) * ld];
+    }
+    // wave 0: the column order from the sequential swaps, lane t holding
+    // the source of column t (each swap exchanges two lanes' values)
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        int sv = lane;
+        const int sw = lane < b ? swaps[lane] : lane;
+        for (int s = 0; s < b; ++s) {
+            const int p = __builtin_amdgcn_readlane(sw, s);
+            if (p != s) {
+                const int a = __builtin_amdgcn_readlane(sv, s), c = __builtin_amdgcn_readlane(sv, p);
+                sv = lane == s ? c : (lane == p ? a : sv);
+            }
+        }
+        src[lane] = sv;
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int e = threadIdx.x + q * 256, i = e & 63, t = e >> 6;
+        tile[i + t * 65] = v[q];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < 64 * b; e += 256) {
@@ -1911,20 +1926,43 @@ __global__ __launch_bounds__(256) void k_yd_small(int mY, int b, const T *B0, co
                                                   const T *G1, long ldg, const T *Vf, long mp,
                                                   T *BT, T *B) {
     // (row, col) of the 16 x 64 row blocks at row + 16 col; sS: the slice of
-    // Tm (q + 64 c: Tm(q, 16 s + c)) or of U11 (c + 16 q: U11(16 s + c, q))
+    // Tm (q + 64 c: Tm(q, 16 s + c)) or of U11 (c + 16 q: U11(16 s + c, q)).
+    // Every global read is unconditional (clamped indices) and issued in a
+    // batch before its first use: entries outside the triangles, the panel
+    // or the rows are never read by the sums (q <= r < b) or stored (i <
+    // rows), so they need no zeroing.
     __shared__ T sB0[16 * 64], sBT[16 * 64], sS[64 * 16];
     const int tid = threadIdx.x, i = tid & 15, c = tid >> 4;
     const int r0 = blockIdx.x * 16;
     const int rows = min(16, mY - r0);
-    for (int e = tid; e < 16 * 64; e += 256) {
-        const int ii = e & 15, q = e >> 4;
-        sB0[ii + q * 16] = (ii < rows && q < b) ? B0[(r0 + ii) + (long)q * mY] : T(0);
+    const int ic = min(i, rows - 1);
+    T g1[4]; // G1(r0 + i, 16 s + c) for the four slices
+    {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u, ii = e & 15, q = e >> 4;
+            v[u] = B0[(r0 + min(ii, rows - 1)) + (long)min(q, b - 1) * mY];
+            g1[u] = G1[(r0 + ic) + (long)min(16 * u + c, b - 1) * ldg];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u, ii = e & 15, q = e >> 4;
+            sB0[ii + q * 16] = v[u];
+        }
     }
     for (int s0 = 0; s0 < b; s0 += 16) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u, q = e & 63, cc = e >> 6;
+            v[u] = Tm[q + (long)min(s0 + cc, b - 1) * ldt];
+        }
         __syncthreads(); // sB0 stored / previous slice consumed
-        for (int e = tid; e < 64 * 16; e += 256) {
-            const int q = e & 63, cc = e >> 6, r = s0 + cc;
-            sS[q + 64 * cc] = (r < b && q <= r) ? Tm[q + (long)r * ldt] : T(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u, q = e & 63, cc = e >> 6;
+            sS[q + 64 * cc] = v[u];
         }
         __syncthreads();
         const int r = s0 + c;
@@ -1934,17 +1972,25 @@ __global__ __launch_bounds__(256) void k_yd_small(int mY, int b, const T *B0, co
         if (i < rows && r < b) BT[(r0 + i) + (long)r * mY] = acc;
     }
     for (int s0 = 0; s0 < b; s0 += 16) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u, cc = e & 15, q = e >> 4;
+            v[u] = Vf[min(s0 + cc, b - 1) + (long)q * mp];
+        }
         __syncthreads(); // sBT complete / previous slice consumed
-        for (int e = tid; e < 64 * 16; e += 256) {
-            const int cc = e & 15, q = e >> 4, r = s0 + cc;
-            sS[cc + 16 * q] = (r < b && q <= r) ? (q == r ? T(1) : Vf[r + (long)q * mp]) : T(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u, cc = e & 15, q = e >> 4;
+            sS[cc + 16 * q] = v[u];
         }
         __syncthreads();
-        // (BT U11^T)(i, r) = sum_{q <= r} BT(i, q) U11(r, q)
+        // (BT U11^T)(i, r) = sum_{q <= r} BT(i, q) U11(r, q), U11(r, r) = 1
         const int r = s0 + c;
         T acc = 0;
-        for (int q = 0; q <= r && q < b; ++q) acc += sBT[i + q * 16] * sS[c + 16 * q];
-        if (i < rows && r < b) B[(r0 + i) + (long)r * mY] = G1[(r0 + i) + (long)r * ldg] - acc;
+        for (int q = 0; q < r && q < b; ++q) acc += sBT[i + q * 16] * sS[c + 16 * q];
+        if (r < b) acc += sBT[i + r * 16];
+        if (i < rows && r < b) B[(r0 + i) + (long)r * mY] = g1[s0 / 16] - acc;
     }
 }
 
