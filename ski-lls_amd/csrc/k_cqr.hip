@@ -112,7 +112,8 @@ __global__ __launch_bounds__(CQ_T) void k_cqr_chol(int b, long mp, const double 
                         int bl = wave_argmax(bv, pos);
                         if (bl < 0) bl = __builtin_ctzll(__ballot(pos == kk)); // no finite candidate
                         pk = bl;
-                        const int lp = __shfl(pos, pk, 64); // logical position of the pivot
+                        // logical position of the pivot (pk is wave-uniform)
+                        const int lp = __builtin_amdgcn_readlane(pos, pk);
                         const bool atk = pos == kk;
                         if (p == pk)
                             pos = kk;
