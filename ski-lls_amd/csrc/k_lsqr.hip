@@ -361,6 +361,27 @@ __global__ __launch_bounds__(256) void k_to_rows(long n, long d, const double *_
 }
 
 // ---- reductions / scatter ----------------------------------------------------
+// sum over q = q0, q0 + step, ... < qn of p[q * stride], added in increasing q
+// (the order of a plain loop), with eight loads in flight at a time: a loop
+// that adds each load before issuing the next waits the memory latency per
+// term.  Past qn the loads repeat entry q0 and are multiplied by 0 (an exact
+// no-op on the finite partial sums; a product by 0 is not folded into a
+// select, which would let the compiler put the load behind a branch).
+__device__ inline double ordered_sum(const double *__restrict__ p, long stride, int q0, int qn,
+                                     int step) {
+    double s = 0.0;
+    for (int q = q0; q < qn; q += 8 * step) {
+        double w[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int qk = q + k * step;
+            w[k] = p[(long)(qk < qn ? qk : q0) * stride];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += w[k] * (q + k * step < qn ? 1.0 : 0.0);
+    }
+    return s;
+}
 // Sums of partial products: 64 outputs per 256-thread block, each summed by
 // four threads over every fourth partial, then combined in a fixed order.
 constexpr int RED_OUT = 64, RED_GRP = 4;
@@ -379,7 +400,7 @@ __global__ __launch_bounds__(256) void k_z_reduce(long d, long r, int nck,
     if (j < d) {
         const long i = Einv ? (long)Einv[j] : j;
         if (i >= 0 && i < r)
-            for (int c = g; c < nck; c += RED_GRP) s += Pz[(long)c * r + i];
+            s = ordered_sum(Pz + i, r, g, nck, RED_GRP);
     }
     sh[g][e] = s;
     __syncthreads();
@@ -398,7 +419,7 @@ __global__ __launch_bounds__(256) void k_reduce_n(long n, int nck, const double 
         long i = (long)blockIdx.x * RN_PER + q * 256 + threadIdx.x;
         if (i < n) {
             double s = 0.0;
-            for (int c = 0; c < nck; ++c) s += Pn[(long)c * n + i];
+            s = ordered_sum(Pn + i, n, 0, nck, 1);
             double v = s - coef * ut[i];
             ut[i] = v;
             sq += v * v;
@@ -447,7 +468,7 @@ __global__ __launch_bounds__(256) void k_reduce_t(long d, long r, int nrb,
     double s = 0.0;
     if (i < r) {
         const long j = E ? (long)E[i] : i;
-        for (int q = g; q < nrb; q += RED_GRP) s += Pt[(long)q * d + j];
+        s = ordered_sum(Pt + j, d, g, nrb, RED_GRP);
     }
     sh[g][e] = s;
     __syncthreads();
@@ -471,7 +492,7 @@ __global__ __launch_bounds__(256) void k_reduce_t_wide(long d, long r, int np,
     double s = 0.0;
     if (i < r) {
         const long j = E ? (long)E[i] : i;
-        for (int q = g; q < np; q += RW_GRP) s += P[(long)q * d + j];
+        s = ordered_sum(P + j, d, g, np, RW_GRP);
     }
     sh[g][e] = s;
     // beta after the partial sums (their loads in flight meanwhile)
@@ -503,7 +524,7 @@ __global__ __launch_bounds__(UPD_THREADS) void k_update(
         double sq = 0.0;
         for (long j = threadIdx.x; j < r; j += UPD_THREADS) {
             double s = 0.0;
-            for (int q = 0; q < nrbm; ++q) s += Pm[(long)q * r + j];
+            s = ordered_sum(Pm + j, r, 0, nrbm, 1);
             double vv = init ? s : ((-beta) * v[j] + s);
             v[j] = vv;
             sq += vv * vv;
@@ -610,7 +631,7 @@ __global__ __launch_bounds__(UV_T) void k_update_v(long r, int nrbm, const doubl
     double sq = 0.0;
     if (beta > 0 && j < r) {
         double s = 0.0;
-        for (int q = 0; q < nrbm; ++q) s += Pm[(long)q * r + j];
+        s = ordered_sum(Pm + j, r, 0, nrbm, 1);
         const double vv = (-beta) * v[j] + s;
         v[j] = vv;
         sq = vv * vv;
