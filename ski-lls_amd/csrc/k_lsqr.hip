@@ -1074,16 +1074,29 @@ __global__ __launch_bounds__(256) void k_spmv_n_lds(long n, long d, int nrb,
             long i = (long)rb * SP_ROWS + q * RPI + grp;
             double s = 0.0;
             if (i < n) {
+                // the lane's entries k = 0, 1, ... (p0 + sub + k LPR) eight at
+                // a time, all loads issued first: even k into s, odd k into
+                // s1, each in increasing k; past the row's end the entry
+                // repeats and is multiplied by 0
                 const int p0 = ptr[i], p1 = ptr[i + 1];
                 double s1 = 0.0;
-                int p = p0 + sub;
-                for (; p + LPR < p1; p += 2 * LPR) {
-                    const int c0 = col[p], c1 = col[p + LPR];
-                    const double v0 = val[p], v1 = val[p + LPR];
-                    s = fma(v0, zs[c0], s);
-                    s1 = fma(v1, zs[c1], s1);
+                for (int p = p0 + sub; p < p1; p += 8 * LPR) {
+                    int cc[8];
+                    double vv[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const int pk = p + k * LPR, pc = pk < p1 ? pk : p;
+                        cc[k] = col[pc];
+                        vv[k] = val[pc];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; k += 2) {
+                        const double m0 = p + k * LPR < p1 ? 1.0 : 0.0;
+                        const double m1 = p + (k + 1) * LPR < p1 ? 1.0 : 0.0;
+                        s = fma(vv[k] * m0, zs[cc[k]], s);
+                        s1 = fma(vv[k + 1] * m1, zs[cc[k + 1]], s1);
+                    }
                 }
-                if (p < p1) s = fma(val[p], zs[col[p]], s);
                 s += s1;
             }
 #pragma unroll
@@ -1112,7 +1125,8 @@ __global__ __launch_bounds__(256) void k_csc_t(long d, const int32_t *__restrict
     if (j >= d) return;
     const int p0 = ptr[j], p1 = ptr[j + 1];
     // one running sum (four independent ones measured slower: 32 vs 27 us at
-    // 120000 x 5000, density 1%)
+    // 120000 x 5000, density 1%; so was this loop with four entries' loads
+    // issued together, 33.5 us)
     double s = 0.0;
     for (int p = p0 + lane; p < p1; p += 64) s += val[p] * ut[row[p]];
     s = wave_sum(s);
