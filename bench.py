@@ -784,8 +784,8 @@ def main():
     global SKIP_REFERENCE
     SKIP_REFERENCE = args.no_reference
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the solve shards "
-              "as described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the solver has no multi-GPU path (the sketch "
+              "would shard by columns and LSQR by rows); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
